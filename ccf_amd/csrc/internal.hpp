@@ -1,0 +1,161 @@
+// Internal state of a ccf_amd context: host-side derived parameters and the
+// device buffers of one star field. Not part of the public interface.
+#pragma once
+
+#include "../../include/ccf_amd.h"
+#include "cplx.hpp"
+#include "imf.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <sstream>
+#include <string>
+#include <vector>
+
+namespace ccf
+{
+
+constexpr int MAX_NUM_STARS_DIRECT = 32; // tree_node.cuh:14
+constexpr int MAX_EXPANSION_ORDER = 31;  // tree_node.cuh:20
+constexpr int MAX_NEIGHBORS = 8;         // tree_node.cuh:23
+constexpr int MAX_SAME_LEVEL = 27;       // tree_node.cuh:24
+constexpr int MAX_DIFF_LEVEL = 5;        // tree_node.cuh:25
+constexpr int MAX_NEAR_RANGES = MAX_NEIGHBORS + 1;
+constexpr int MAX_TAYLOR_SMOOTH = 101;   // ccf.cuh:79
+
+// Quadtree over the stars, structure of arrays indexed by node id. Nodes of
+// a level are contiguous ([level_off[l], level_off[l] + level_cnt[l])) and
+// the four children of a node are consecutive ids starting at child[n].
+struct DeviceTree
+{
+	int nnodes = 0;
+	int capacity = 0;
+	int levels = 0; // deepest level index (the reference's tree_levels)
+	int order = 0;  // expansion order p (0: no far field)
+	std::vector<int> level_off, level_cnt;
+
+	double2* center = nullptr;
+	double* hl = nullptr;
+	int* parent = nullptr;
+	int* child = nullptr;  // first child id, -1 for a leaf
+	int* sstart = nullptr; // first star (tree order)
+	int* scount = nullptr;
+	int* nbr = nullptr;    // [capacity][MAX_NEIGHBORS]
+	int* nnbr = nullptr;
+	int* sl = nullptr;     // same-level interaction list [capacity][27]
+	int* nsl = nullptr;
+	int* dl = nullptr;     // coarser-level interaction list [capacity][5]
+	int* ndl = nullptr;
+	int* split = nullptr;  // per-level scratch: split flags / ranks
+	int* rank = nullptr;
+	int2* nf = nullptr;    // near-field star ranges: self then neighbours [capacity][9]
+	int* nnf = nullptr;
+	double2* mp = nullptr;  // multipole coefficients [nnodes][order + 1]
+	double2* loc = nullptr; // local coefficients [nnodes][order + 1]
+	float2* loc32 = nullptr;
+	double* binom = nullptr;
+};
+
+// Kernel-side view of the lens model and tree (passed by value).
+struct LensView
+{
+	double kappa, gamma, theta, theta2, kappastar;
+	int rectangular, approx, taylor_smooth;
+	double corner_re, corner_im, corner_abs2, corner_arg;
+	int order;
+	int nstars;
+	int nts;                   // number of precomputed smooth-matter series terms
+	const double2* ts_a;       // (1 - e^{2 i k arg c}) / k for k = ts-1, ts-3, ..., 2
+	const double2* ts_b;       // (1 - e^{2 i k arg c})
+	const double2* spos;       // star positions, tree order
+	const double* smass;
+	const double2* center;
+	const double* hl;
+	const int* child;
+	const int2* nf;
+	const int* nnf;
+	const double2* loc;
+	const float2* loc32;
+};
+
+struct Timer
+{
+	hipEvent_t a = nullptr, b = nullptr;
+};
+
+} // namespace ccf
+
+struct ccf_ctx
+{
+	ccf_params P{};
+	ccf_info info{};
+	std::string last_error;
+
+	// derived parameters (ccf.cuh:96-126)
+	ccf::Imf imf;
+	double mean_mass = 0, mean_mass2 = 0, mean_mass2_ln_mass = 0;
+	double kappa_star_actual = 0, m_lower_actual = 0, m_upper_actual = 0;
+	double mean_mass_actual = 0, mean_mass2_actual = 0, mean_mass2_ln_mass_actual = 0;
+	double mu_ave = 0, alpha_error = 0, root_half_length = 0, max_error = 0;
+	ccf::cd corner;
+	int taylor_smooth = 1, num_roots = 0, num_stars = 0, expansion_order = 0;
+	int rectangular = 0;
+	double theta = 1, kappa_star = 0;
+	int random_seed = 0;
+	bool from_file = false;
+
+	// stage completion
+	bool prepared = false, have_roots = false, have_ccs = false, have_caustics = false, have_mu = false;
+
+	// host copies
+	std::vector<double> stars_in;   // {x1, x2, m} in generation/file order
+	std::vector<double> seeds;      // initial roots (re, im) interleaved
+
+	// device state
+	hipStream_t stream = nullptr;
+	ccf::DeviceTree tree;
+	double2* spos = nullptr;      // tree order
+	double* smass = nullptr;
+	double2* spos_tmp = nullptr;
+	double* smass_tmp = nullptr;
+	float2* spos32 = nullptr;
+	double2* ts_a = nullptr;
+	double2* ts_b = nullptr;
+	int nts = 0;
+	double2* ccs_init = nullptr;  // [(np + nb) rows][R], row-major by phi
+	double2* ccs = nullptr;       // transposed, file order
+	double2* caustics = nullptr;
+	double* mu_scales = nullptr;
+	// Aberth work buffers
+	int max_rows = 0;
+	int* act = nullptr;           // [2][max_rows][R] active root indices
+	int* act_cnt = nullptr;       // [2][max_rows]
+	double2* nv = nullptr;        // [max_rows][R] updated values (compacted)
+	unsigned char* conv = nullptr;// [max_rows][R]
+	int* rows_dev = nullptr;      // [max_rows] phi-row index of each solved row
+	double* phis_dev = nullptr;   // [max_rows] phi of each solved row
+	unsigned long long* counters = nullptr; // device statistics, see solve.hip
+	std::vector<ccf::Timer> timers;
+
+	ccf::LensView lens() const;
+	void fail(const std::string& msg);
+	bool hip_ok(hipError_t e, const char* what, const char* file, int line);
+	void free_device();
+};
+
+#define CCF_HIP(ctx, call) ((ctx)->hip_ok((call), #call, __FILE__, __LINE__))
+#define CCF_LAUNCH(ctx, name) ((ctx)->hip_ok(hipGetLastError(), name, __FILE__, __LINE__))
+
+namespace ccf
+{
+// tree.hip
+bool build_tree(ccf_ctx* ctx, int verbose);
+// solve.hip
+bool alloc_solver(ccf_ctx* ctx);
+bool find_initial_roots_dev(ccf_ctx* ctx, int verbose);
+bool find_ccs_dev(ccf_ctx* ctx, int verbose);
+bool find_caustics_dev(ccf_ctx* ctx, bool caustics, bool mu);
+bool generate_stars_dev(ccf_ctx* ctx);
+bool eval_points_dev(ccf_ctx* ctx, const double* z, int n, double phi, double* out);
+bool aberth_sweep_dev(ccf_ctx* ctx, const double* roots, int n, double phi, double* out);
+} // namespace ccf

@@ -1,0 +1,254 @@
+// Device-side lens model: star, far-field and smooth-matter deflections and
+// Witt's parametric critical-curve function (weisluke/ccf
+// include/alpha_star.cuh, alpha_local.cuh, alpha_smooth.cuh and
+// ccf_functions.cuh:33-166).
+//
+// The reference evaluates each derivative in its own loop with a Smith
+// division per star; here one pass over the near-field stars forms
+// t = 1/(z - s) once and accumulates m t, m t^2 and m t^3 together, and one
+// Horner pass over the local coefficients produces the first three
+// derivative sums of the far-field potential.
+#pragma once
+
+#include "internal.hpp"
+
+namespace ccf
+{
+
+// tree_node.cuh:529-561
+__device__ inline int nearest_node(const LensView& L, cd z)
+{
+	int n = 0;
+	while (L.child[n] >= 0)
+	{
+		const double2 c = L.center[n];
+		const double h = L.hl[n];
+		if (fabs(z.re - c.x) > h || fabs(z.im - c.y) > h) return n;
+		int q;
+		if (z.re >= c.x && z.im >= c.y) q = 0;
+		else if (z.re < c.x && z.im >= c.y) q = 1;
+		else if (z.re < c.x && z.im < c.y) q = 2;
+		else q = 3;
+		n = L.child[n] + q;
+	}
+	return n;
+}
+
+// near-field sums over a node and its neighbours:
+// s1 = sum m/(z-s), s2 = sum m/(z-s)^2, s3 = sum m/(z-s)^3
+struct Near
+{
+	cd s1, s2, s3;
+	int count = 0;
+};
+
+__device__ inline Near p2p(const LensView& L, int node, cd z)
+{
+	Near r;
+	const int nr = L.nnf[node];
+	for (int k = 0; k < nr; k++)
+	{
+		const int2 rg = L.nf[node * MAX_NEAR_RANGES + k];
+		for (int s = rg.x; s < rg.x + rg.y; s++)
+		{
+			const double2 p = L.spos[s];
+			const cd t = cd(z.re - p.x, z.im - p.y).inv();
+			const cd t1 = t * L.smass[s];
+			r.s1 += t1;
+			const cd t2 = t1 * t;
+			r.s2 += t2;
+			r.s3 += t2 * t;
+		}
+		r.count += rg.y;
+	}
+	return r;
+}
+
+// local-expansion sums with dz = (z - c)/hl (alpha_local.cuh:18-102):
+// h1 = sum i c_i dz^(i-1), h2 = sum i(i-1) c_i dz^(i-2), h3 = sum i(i-1)(i-2) c_i dz^(i-3)
+struct Far
+{
+	cd h1, h2, h3;
+	double hl = 1;
+};
+
+__device__ inline Far l2p(const LensView& L, int node, cd z)
+{
+	Far f;
+	f.hl = L.hl[node];
+	if (L.order == 0) return f;
+	const double2 c = L.center[node];
+	const cd dz = cd(z.re - c.x, z.im - c.y) / f.hl;
+	const double2* a = L.loc + (size_t)node * (L.order + 1);
+	for (int i = L.order; i >= 1; i--)
+	{
+		const double2 ci = a[i];
+		const cd c1(ci.x * i, ci.y * i);
+		f.h1 = f.h1 * dz + c1;
+		if (i >= 2) f.h2 = f.h2 * dz + c1 * (double)(i - 1);
+		if (i >= 3) f.h3 = f.h3 * dz + c1 * (double)((i - 1) * (i - 2));
+	}
+	return f;
+}
+
+__device__ inline double boxcar(cd z, double cre, double cim)
+{
+	return (-cre <= z.re && z.re <= cre && -cim <= z.im && z.im <= cim) ? 1.0 : 0.0;
+}
+
+// alpha_smooth.cuh:141-170
+__device__ inline double smooth_dz(const LensView& L, cd z)
+{
+	if (L.rectangular) return L.approx ? -L.kappastar : -L.kappastar * boxcar(z, L.corner_re, L.corner_im);
+	return L.approx ? -L.kappastar : -L.kappastar * (L.corner_abs2 >= z.norm2() ? 1.0 : 0.0);
+}
+
+// d alpha_smooth / d zbar and d^2 alpha_smooth / d zbar^2 (alpha_smooth.cuh:185-301)
+__device__ inline void smooth_dzbar(const LensView& L, cd z, cd& d1, cd& d2)
+{
+	const double ks = L.kappastar;
+	const cd zb = z.conj();
+	d1 = cd();
+	d2 = cd();
+	if (L.rectangular)
+	{
+		const cd c(L.corner_re, L.corner_im);
+		if (L.approx)
+		{
+			// Taylor series in r^2, r = zbar / corner; the z-independent
+			// coefficients (1 - e^{2 i k arg c})/k and (1 - e^{2 i k arg c}) are precomputed
+			const cd r = zb / c;
+			const cd r2 = r * r;
+			for (int k = 0; k < L.nts; k++)
+			{
+				const double2 a = L.ts_a[k], b = L.ts_b[k];
+				d1 = (d1 + cd(a.x, a.y)) * r2;
+				d2 = (d2 + cd(b.x, b.y)) * r2;
+			}
+			const cd m(0, -ks / PI);
+			d1 = d1 * 2.0 * m + cd(ks - 4 * ks * L.corner_arg / PI);
+			d2 = d2 * zb.inv() * 2.0 * m;
+		}
+		else
+		{
+			const cd c1 = c.conj() - zb, c2 = c - zb, c3 = -c - zb, c4 = -c.conj() - zb;
+			const cd m(0, -ks / PI);
+			d1 = (c1.log() - c2.log() - c3.log() + c4.log()) * m - cd(ks * boxcar(z, L.corner_re, L.corner_im));
+			d2 = (-(c1.inv()) + c2.inv() + c3.inv() - c4.inv()) * m;
+		}
+	}
+	else if (!L.approx)
+	{
+		const double a2 = ks * L.corner_abs2;
+		const cd izb = zb.inv();
+		d1 = a2 * izb * izb;
+		d2 = -2 * a2 * izb * izb * izb;
+	}
+}
+
+// alpha_smooth.cuh:62-127
+__device__ inline cd smooth_alpha(const LensView& L, cd z)
+{
+	const double ks = L.kappastar;
+	const cd zb = z.conj();
+	if (L.rectangular)
+	{
+		const cd c(L.corner_re, L.corner_im);
+		if (L.approx)
+		{
+			cd s1, s2, s3, s4;
+			const cd r1 = zb / c, r2 = zb / c.conj(), r3 = zb / (-c), r4 = zb / (-c.conj());
+			for (int i = L.taylor_smooth; i >= 1; i--)
+			{
+				const double q = 1.0 / i;
+				s1 = (s1 + q) * r1;
+				s2 = (s2 + q) * r2;
+				s3 = (s3 + q) * r3;
+				s4 = (s4 + q) * r4;
+			}
+			cd a = (c - zb) * (c.log() - s1) - (c.conj() - zb) * (c.conj().log() - s2)
+				+ (-c - zb) * ((-c).log() - s3) - (-c.conj() - zb) * ((-c).conj().log() - s4);
+			return a * cd(0, -ks / PI) - cd(ks * 2 * (c.re + z.re));
+		}
+		const cd c1 = c - zb, c2 = c.conj() - zb, c3 = -c - zb, c4 = -c.conj() - zb;
+		cd a = (c1 * c1.log() - c2 * c2.log() + c3 * c3.log() - c4 * c4.log()) * cd(0, -ks / PI);
+		a -= cd(ks * 2 * (c.re + z.re) * boxcar(z, c.re, c.im));
+		const double hv = (c.im + z.im >= 0 ? 1.0 : 0.0) * (c.im - z.im >= 0 ? 1.0 : 0.0) * (z.re - c.re >= 0 ? 1.0 : 0.0);
+		a -= cd(ks * 4 * c.re * hv);
+		return a;
+	}
+	if (L.approx || z.norm2() <= L.corner_abs2) return -ks * z;
+	return -ks * L.corner_abs2 * zb.inv();
+}
+
+// Witt's function f0 and the pieces the Aberth update and the 1/mu bound
+// need (ccf_functions.cuh:69-86, 193-211)
+struct Witt
+{
+	cd f0;
+	double a = 0;   // 1 - kappa - d alpha_smooth / dz
+	bool early = false;
+	int node = 0;
+	Near nr;
+	Far fr;
+};
+
+__device__ inline Witt witt_f0(const LensView& L, cd z, double phi)
+{
+	Witt w;
+	w.node = nearest_node(L, z);
+	w.nr = p2p(L, w.node, z);
+	w.fr = l2p(L, w.node, z);
+	cd d1, d2;
+	smooth_dzbar(L, z, d1, d2);
+	w.a = 1 - L.kappa - smooth_dz(L, z);
+	double sp, cp;
+	sincos(phi, &sp, &cp);
+	const double ih2 = 1.0 / (w.fr.hl * w.fr.hl);
+	w.f0 = cd(L.gamma) + w.nr.s2 * L.theta2 - w.fr.h2 * (L.theta2 * ih2) - d1.conj() - w.a * cd(cp, -sp);
+	const double a0 = w.f0.abs();
+	w.early = fabs(a0 * (a0 + 2 * w.a)) < 1e-9 && fabs(a0 * (a0 - 2 * w.a)) < 1e-9;
+	return w;
+}
+
+// f1 = d f0 / dz from the sums already formed for f0 (ccf_functions.cuh:108-121)
+__device__ inline cd witt_f1(const LensView& L, cd z, const Witt& w)
+{
+	cd d1, d2;
+	smooth_dzbar(L, z, d1, d2);
+	const double ih3 = 1.0 / (w.fr.hl * w.fr.hl * w.fr.hl);
+	return w.nr.s3 * (-2 * L.theta2) - w.fr.h3 * (L.theta2 * ih3) - d2.conj();
+}
+
+// 1/mu error bound at a root (ccf_functions.cuh:457-463)
+__device__ inline double err_bound(const Witt& w)
+{
+	const double a0 = w.f0.abs();
+	return fmax(fabs(a0 * (a0 + 2 * w.a)), fabs(a0 * (a0 - 2 * w.a)));
+}
+
+// lens map and magnification length scale (ccf_functions.cuh:33-45, 144-166)
+__device__ inline void lens_map(const LensView& L, cd z, cd& wsrc, double& mu_scale)
+{
+	const int node = nearest_node(L, z);
+	const Near nr = p2p(L, node, z);
+	const Far fr = l2p(L, node, z);
+	const double t2 = L.theta2, h = fr.hl;
+	const cd a_star = (nr.s1 * t2).conj();
+	const cd da_star = (nr.s2 * (-t2)).conj();
+	const cd d2a_star = (nr.s3 * (2 * t2)).conj();
+	const cd a_loc = (fr.h1 * (t2 / h)).conj();
+	const cd da_loc = (fr.h2 * (t2 / (h * h))).conj();
+	const cd d2a_loc = (fr.h3 * (t2 / (h * h * h))).conj();
+	cd d1, d2;
+	smooth_dzbar(L, z, d1, d2);
+	wsrc = (1 - L.kappa) * z + L.gamma * z.conj() - a_star - a_loc - smooth_alpha(L, z);
+	const double dwdz = 1 - L.kappa - smooth_dz(L, z);
+	const cd dwdzb = cd(L.gamma) - da_star - da_loc - d1;
+	const cd d2wdzb2 = -d2a_star - d2a_loc - d2;
+	const cd cct = cd(0, -2) * dwdzb.conj() * d2wdzb2;
+	const cd ct = dwdz * cct + dwdzb * cct.conj();
+	mu_scale = 1 / (2 * ct.abs());
+}
+
+} // namespace ccf

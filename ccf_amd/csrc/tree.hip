@@ -1,0 +1,622 @@
+// Quadtree and 2-D complex FMM build on gfx950.
+//
+// Same tree as weisluke/ccf (tree_node.cuh:183-267, 321-519; driver
+// ccf.cuh:732-904): split every node whose own plus neighbour stars exceed
+// 32, children in quadrant order (+,+), (-,+), (-,-), (+,-), neighbours
+// closer than 3 (same level) or 3.5 (coarser leaf) half-lengths, the rest
+// in the interaction lists. The construction differs in how, not what:
+//  - nodes live in structure-of-arrays with integer ids, children of a node
+//    are 4 consecutive ids given by an exclusive prefix sum over the split
+//    flags (deterministic, where the reference hands out slots by atomics);
+//  - each split node's stars are stably partitioned by one workgroup using
+//    wavefront ballots, so the tree order of the stars is reproducible.
+// The FMM passes (fmm.cuh) run one thread per (node, coefficient).
+#include "internal.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <iostream>
+
+namespace ccf
+{
+namespace
+{
+
+constexpr int BLOCK = 256;
+
+__device__ inline int quadrant(double2 z, double2 c)
+{
+	if (z.y >= c.y) return z.x >= c.x ? 0 : 1;
+	return z.x < c.x ? 2 : 3;
+}
+
+__device__ inline cd ld(const double2* p, size_t i) { double2 v = p[i]; return cd(v.x, v.y); }
+__device__ inline void st(double2* p, size_t i, cd v) { p[i] = make_double2(v.re, v.im); }
+
+// scal[0] = max stars in node + neighbours over the level, scal[2] = list overflow flag
+__global__ void k_level_count(int off, int cnt, const int* __restrict__ scount, const int* __restrict__ nbr,
+	const int* __restrict__ nnbr, int* __restrict__ split, int* __restrict__ scal)
+{
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	int tot = 0;
+	if (i < cnt)
+	{
+		int n = off + i;
+		tot = scount[n];
+		int k = nnbr[n];
+		for (int j = 0; j < k; j++) tot += scount[nbr[n * MAX_NEIGHBORS + j]];
+		split[i] = tot > MAX_NUM_STARS_DIRECT ? 1 : 0;
+	}
+	for (int d = 32; d >= 1; d >>= 1) tot = max(tot, __shfl_xor(tot, d, 64));
+	if ((threadIdx.x & 63) == 0) atomicMax(&scal[0], tot);
+}
+
+// exclusive prefix sum of flags[0..n) into rank, total into *total (one block)
+__global__ void k_scan(const int* __restrict__ flags, int n, int* __restrict__ rank, int* __restrict__ total)
+{
+	__shared__ int wsum[16];
+	__shared__ int carry;
+	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+	if (threadIdx.x == 0) carry = 0;
+	__syncthreads();
+	for (int base = 0; base < n; base += blockDim.x)
+	{
+		int i = base + threadIdx.x;
+		int v = i < n ? flags[i] : 0;
+		int x = v;
+		for (int d = 1; d < 64; d <<= 1)
+		{
+			int y = __shfl_up(x, d, 64);
+			if (lane >= d) x += y;
+		}
+		if (lane == 63) wsum[w] = x;
+		__syncthreads();
+		int woff = 0;
+		for (int k = 0; k < w; k++) woff += wsum[k];
+		if (i < n) rank[i] = carry + woff + x - v;
+		int blocktot = 0;
+		for (int k = 0; k < nw; k++) blocktot += wsum[k];
+		__syncthreads();
+		if (threadIdx.x == 0) carry += blocktot;
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ void k_make_children(int off, int cnt, int new_off, const int* __restrict__ split, const int* __restrict__ rank,
+	double2* center, double* hl, int* parent, int* child, int* sstart, int* scount, int* nnbr, int* nsl, int* ndl)
+{
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= cnt || !split[i]) return;
+	int n = off + i;
+	int c0 = new_off + 4 * rank[i];
+	child[n] = c0;
+	const double h = hl[n] / 2;
+	const double2 c = center[n];
+	const double ox[4] = {1, -1, -1, 1}, oy[4] = {1, 1, -1, -1};
+	for (int q = 0; q < 4; q++)
+	{
+		int k = c0 + q;
+		hl[k] = h;
+		center[k] = make_double2(c.x + h * ox[q], c.y + h * oy[q]);
+		parent[k] = n;
+		child[k] = -1;
+		sstart[k] = sstart[n];
+		scount[k] = 0;
+		nnbr[k] = 0;
+		nsl[k] = 0;
+		ndl[k] = 0;
+	}
+}
+
+// one workgroup per node of the level: stable 4-way partition of the stars
+// of every split node into its children (tree_node.cuh:392-501)
+__global__ __launch_bounds__(BLOCK) void k_partition(int off, const int* __restrict__ split, const int* __restrict__ child,
+	const double2* __restrict__ center, int* sstart, int* scount,
+	double2* spos, double* smass, double2* tpos, double* tmass)
+{
+	const int i = blockIdx.x;
+	if (!split[i]) return;
+	const int n = off + i;
+	const int c0 = child[n];
+	const int start = sstart[n], count = scount[n];
+	if (count == 0)
+	{
+		if (threadIdx.x < 4)
+		{
+			sstart[c0 + threadIdx.x] = start;
+			scount[c0 + threadIdx.x] = 0;
+		}
+		return;
+	}
+	const double2 c = center[n];
+	__shared__ int qcnt[4];
+	__shared__ int run[4];
+	__shared__ int wtot[BLOCK / 64][4];
+	if (threadIdx.x < 4) { qcnt[threadIdx.x] = 0; run[threadIdx.x] = 0; }
+	__syncthreads();
+	int lc[4] = {0, 0, 0, 0};
+	for (int s = threadIdx.x; s < count; s += BLOCK) lc[quadrant(spos[start + s], c)]++;
+	for (int q = 0; q < 4; q++) atomicAdd(&qcnt[q], lc[q]);
+	__syncthreads();
+	const int base[4] = {0, qcnt[0], qcnt[0] + qcnt[1], qcnt[0] + qcnt[1] + qcnt[2]};
+	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+	for (int chunk = 0; chunk < count; chunk += BLOCK)
+	{
+		const int s = chunk + threadIdx.x;
+		const int q = s < count ? quadrant(spos[start + s], c) : -1;
+		int myrank = 0;
+		for (int k = 0; k < 4; k++)
+		{
+			unsigned long long m = __ballot(q == k);
+			if (q == k) myrank = __popcll(m & below);
+			if (lane == 0) wtot[w][k] = __popcll(m);
+		}
+		__syncthreads();
+		if (q >= 0)
+		{
+			int pos = base[q] + run[q] + myrank;
+			for (int k = 0; k < w; k++) pos += wtot[k][q];
+			tpos[start + pos] = spos[start + s];
+			tmass[start + pos] = smass[start + s];
+		}
+		__syncthreads();
+		if (threadIdx.x < 4)
+		{
+			int t = 0;
+			for (int k = 0; k < BLOCK / 64; k++) t += wtot[k][threadIdx.x];
+			run[threadIdx.x] += t;
+		}
+		__syncthreads();
+	}
+	for (int s = threadIdx.x; s < count; s += BLOCK)
+	{
+		spos[start + s] = tpos[start + s];
+		smass[start + s] = tmass[start + s];
+	}
+	if (threadIdx.x < 4)
+	{
+		sstart[c0 + threadIdx.x] = start + base[threadIdx.x];
+		scount[c0 + threadIdx.x] = qcnt[threadIdx.x];
+	}
+}
+
+// tree_node.cuh:183-267 for every node of a new level
+__global__ void k_set_neighbors(int off, int cnt, const double2* __restrict__ center, const double* __restrict__ hl,
+	const int* __restrict__ parent, const int* __restrict__ child, const int* __restrict__ scount,
+	int* nbr, int* nnbr, int* sl, int* nsl, int* dl, int* ndl, int* scal)
+{
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= cnt) return;
+	const int n = off + i;
+	const int p = parent[n];
+	const double h = hl[n];
+	const double2 c = center[n];
+	int a = 0, b = 0, d = 0;
+	bool overflow = false;
+	for (int q = 0; q < 4; q++)
+	{
+		int s = child[p] + q;
+		if (s == n || scount[s] == 0) continue;
+		if (a < MAX_NEIGHBORS) nbr[n * MAX_NEIGHBORS + a++] = s; else overflow = true;
+	}
+	const int pn = nnbr[p];
+	for (int k = 0; k < pn; k++)
+	{
+		const int m = nbr[p * MAX_NEIGHBORS + k];
+		if (scount[m] == 0) continue;
+		if (child[m] < 0)
+		{
+			const double2 cm = center[m];
+			if (fabs(cm.x - c.x) < 3.5 * h && fabs(cm.y - c.y) < 3.5 * h)
+			{
+				if (a < MAX_NEIGHBORS) nbr[n * MAX_NEIGHBORS + a++] = m; else overflow = true;
+			}
+			else
+			{
+				if (d < MAX_DIFF_LEVEL) dl[n * MAX_DIFF_LEVEL + d++] = m; else overflow = true;
+			}
+		}
+		else
+		{
+			for (int q = 0; q < 4; q++)
+			{
+				const int s = child[m] + q;
+				if (scount[s] == 0) continue;
+				const double2 cs = center[s];
+				if (fabs(cs.x - c.x) < 3 * h && fabs(cs.y - c.y) < 3 * h)
+				{
+					if (a < MAX_NEIGHBORS) nbr[n * MAX_NEIGHBORS + a++] = s; else overflow = true;
+				}
+				else
+				{
+					if (b < MAX_SAME_LEVEL) sl[n * MAX_SAME_LEVEL + b++] = s; else overflow = true;
+				}
+			}
+		}
+	}
+	nnbr[n] = a;
+	nsl[n] = b;
+	ndl[n] = d;
+	if (overflow) atomicOr(&scal[2], 1);
+}
+
+// star ranges summed directly at a node: its own stars then each neighbour's
+__global__ void k_near_field(int nnodes, const int* __restrict__ sstart, const int* __restrict__ scount,
+	const int* __restrict__ nbr, const int* __restrict__ nnbr, int2* nf, int* nnf)
+{
+	int n = blockIdx.x * blockDim.x + threadIdx.x;
+	if (n >= nnodes) return;
+	int k = 0;
+	nf[n * MAX_NEAR_RANGES + k++] = make_int2(sstart[n], scount[n]);
+	for (int j = 0; j < nnbr[n]; j++)
+	{
+		int m = nbr[n * MAX_NEIGHBORS + j];
+		nf[n * MAX_NEAR_RANGES + k++] = make_int2(sstart[m], scount[m]);
+	}
+	nnf[n] = k;
+}
+
+__device__ inline double bin(const double* __restrict__ B, int n, int k) { return n < k ? 0.0 : B[n * (n + 1) / 2 + k]; }
+__device__ inline double pow2_divisor(int l) { return (double)(int)(1u << l); } // (1 << l) as the reference's int
+
+// upward pass for one level: P2M at leaves with stars (fmm.cuh:21-49),
+// M2M from the four children elsewhere (fmm.cuh:109-139)
+__global__ void k_upward(int off, int cnt, int p, const double2* __restrict__ center, const double* __restrict__ hl,
+	const int* __restrict__ child, const int* __restrict__ sstart, const int* __restrict__ scount,
+	const double2* __restrict__ spos, const double* __restrict__ smass, const double* __restrict__ B, double2* mp)
+{
+	const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+	const int i = idx / (p + 1), l = idx % (p + 1);
+	if (i >= cnt) return;
+	const int n = off + i;
+	const int P1 = p + 1;
+	cd r;
+	if (child[n] < 0)
+	{
+		const int s0 = sstart[n], ns = scount[n];
+		if (ns == 0) return;
+		const cd c = ld(center, n);
+		const double h = hl[n];
+		if (l == 0)
+		{
+			for (int s = 0; s < ns; s++) r += cd(smass[s0 + s]);
+		}
+		else
+		{
+			for (int s = 0; s < ns; s++)
+			{
+				cd t = (ld(spos, s0 + s) - c) / h;
+				r -= t.pow(l) * smass[s0 + s];
+			}
+			r /= (double)l;
+		}
+	}
+	else
+	{
+		for (int q = 0; q < 4; q++)
+		{
+			const int ch = child[n] + q;
+			const double2* a = mp + (size_t)ch * P1;
+			if (l == 0)
+			{
+				r += ld(a, 0);
+				continue;
+			}
+			const cd dz = (ld(center, ch) - ld(center, n)) / hl[ch];
+			const cd idz = dz.inv();
+			cd t;
+			for (int k = l; k >= 1; k--)
+			{
+				t += ld(a, k) * bin(B, l - 1, k - 1);
+				t *= idz;
+			}
+			t -= ld(a, 0) / (double)l;
+			t *= dz.pow(l);
+			t /= pow2_divisor(l);
+			r += t;
+		}
+	}
+	st(mp, (size_t)n * P1 + l, r);
+}
+
+// downward pass for one level >= 2: L2L from the parent (fmm.cuh:204-223),
+// M2L from the same-level list (fmm.cuh:281-322), P2L from the coarser
+// leaves (fmm.cuh:387-413)
+__global__ void k_downward(int off, int cnt, int p, const double2* __restrict__ center, const double* __restrict__ hl,
+	const int* __restrict__ parent, const int* __restrict__ sl, const int* __restrict__ nsl,
+	const int* __restrict__ dl, const int* __restrict__ ndl, const int* __restrict__ sstart, const int* __restrict__ scount,
+	const double2* __restrict__ spos, const double* __restrict__ smass, const double* __restrict__ B,
+	const double2* __restrict__ mp, double2* loc)
+{
+	const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+	const int i = idx / (p + 1), l = idx % (p + 1);
+	if (i >= cnt) return;
+	const int n = off + i;
+	const int P1 = p + 1;
+	const cd cn = ld(center, n);
+	const double hn = hl[n];
+	cd r;
+	{
+		const int pa = parent[n];
+		const double2* a = loc + (size_t)pa * P1;
+		const cd dz = (cn - ld(center, pa)) / hl[pa];
+		for (int k = p; k >= 1; k--)
+		{
+			r += ld(a, k) * bin(B, k, l);
+			r *= dz;
+		}
+		r *= dz.pow(l).inv();
+		r /= pow2_divisor(l);
+	}
+	for (int j = 0; j < nsl[n]; j++)
+	{
+		const int m = sl[n * MAX_SAME_LEVEL + j];
+		const double2* b = mp + (size_t)m * P1;
+		const cd dz = (ld(center, m) - cn) / hl[m];
+		const cd imdz = (-dz).inv();
+		cd t;
+		if (l == 0)
+		{
+			for (int k = p; k >= 1; k--)
+			{
+				t += ld(b, k);
+				t *= imdz;
+			}
+			t += ld(b, 0) * (-dz).log();
+			t += ld(b, 0) * log(hl[m]);
+		}
+		else
+		{
+			for (int k = p; k >= 1; k--)
+			{
+				t += ld(b, k) * bin(B, l + k - 1, k - 1);
+				t *= imdz;
+			}
+			t -= ld(b, 0) / (double)l;
+			t *= dz.pow(l).inv();
+		}
+		r += t;
+	}
+	for (int j = 0; j < ndl[n]; j++)
+	{
+		const int m = dl[n * MAX_DIFF_LEVEL + j];
+		const int s0 = sstart[m], ns = scount[m];
+		for (int s = 0; s < ns; s++)
+		{
+			const cd dz = (ld(spos, s0 + s) - cn) / hn;
+			const double ms = smass[s0 + s];
+			if (l == 0)
+			{
+				r += ms * (-dz).log();
+				r += cd(ms * log(hn));
+			}
+			else
+			{
+				r -= (ms / (double)l) * dz.pow(l).inv();
+			}
+		}
+	}
+	st(loc, (size_t)n * P1 + l, r);
+}
+
+__global__ void k_to_float(const double2* __restrict__ a, float2* b, size_t n)
+{
+	size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) b[i] = make_float2((float)a[i].x, (float)a[i].y);
+}
+
+inline int grid(long long n, int b = BLOCK) { return (int)((n + b - 1) / b); }
+
+template <typename V>
+bool grow(ccf_ctx* ctx, V*& ptr, int old_n, int new_n, int width)
+{
+	V* p = nullptr;
+	if (!CCF_HIP(ctx, hipMalloc(&p, sizeof(V) * (size_t)new_n * width))) return false;
+	if (ptr)
+	{
+		if (old_n > 0 && !CCF_HIP(ctx, hipMemcpyAsync(p, ptr, sizeof(V) * (size_t)old_n * width, hipMemcpyDeviceToDevice, ctx->stream))) return false;
+		if (!CCF_HIP(ctx, hipStreamSynchronize(ctx->stream))) return false;
+		hipFree(ptr);
+	}
+	ptr = p;
+	return true;
+}
+
+bool ensure_capacity(ccf_ctx* ctx, int need)
+{
+	DeviceTree& T = ctx->tree;
+	if (need <= T.capacity) return true;
+	int cap = std::max(need, 2 * T.capacity);
+	const int n = T.nnodes;
+	bool ok = grow(ctx, T.center, n, cap, 1) && grow(ctx, T.hl, n, cap, 1) && grow(ctx, T.parent, n, cap, 1) &&
+		grow(ctx, T.child, n, cap, 1) && grow(ctx, T.sstart, n, cap, 1) && grow(ctx, T.scount, n, cap, 1) &&
+		grow(ctx, T.nbr, n, cap, MAX_NEIGHBORS) && grow(ctx, T.nnbr, n, cap, 1) &&
+		grow(ctx, T.sl, n, cap, MAX_SAME_LEVEL) && grow(ctx, T.nsl, n, cap, 1) &&
+		grow(ctx, T.dl, n, cap, MAX_DIFF_LEVEL) && grow(ctx, T.ndl, n, cap, 1) &&
+		// split flags and ranks of the level being split are still live
+		grow(ctx, T.split, T.capacity, cap, 1) && grow(ctx, T.rank, T.capacity, cap, 1);
+	if (ok) T.capacity = cap;
+	return ok;
+}
+
+// Pascal rows 0..n as the reference tabulates them (binomial_coefficients.cuh:11-44):
+// int rows that wrap past row 33, or exact values
+std::vector<double> binomial_table(int n, bool exact)
+{
+	std::vector<double> t((size_t)(n + 1) * (n + 2) / 2);
+	std::vector<uint32_t> w(t.size());
+	std::vector<long double> e(t.size());
+	for (int r = 0; r <= n; r++)
+	{
+		const int s = r * (r + 1) / 2, ps = (r - 1) * r / 2;
+		for (int k = 0; k <= r; k++)
+		{
+			if (k == 0 || k == r) { w[s + k] = 1; e[s + k] = 1; }
+			else { w[s + k] = w[ps + k - 1] + w[ps + k]; e[s + k] = e[ps + k - 1] + e[ps + k]; }
+			t[s + k] = exact ? (double)e[s + k] : (double)(int32_t)w[s + k];
+		}
+	}
+	return t;
+}
+
+} // namespace
+
+bool build_tree(ccf_ctx* ctx, int verbose)
+{
+	DeviceTree& T = ctx->tree;
+	const ccf_params& P = ctx->P;
+	const int N = ctx->num_stars;
+	hipStream_t s = ctx->stream;
+	auto t0 = std::chrono::steady_clock::now();
+
+	ctx->root_half_length = (ctx->rectangular ? std::max(ctx->corner.re, ctx->corner.im) : ctx->corner.abs()) * 1.1;
+	if (verbose >= 2) std::cout << "root_half_length set to: " << ctx->root_half_length << "\n\n";
+	if (verbose >= 1) std::cout << "Creating children and sorting stars...\n";
+
+	T.nnodes = 0;
+	T.capacity = 0;
+	if (!ensure_capacity(ctx, 4096 + N / 2)) return false;
+	int* scal = nullptr;
+	if (!CCF_HIP(ctx, hipMalloc(&scal, 4 * sizeof(int)))) return false;
+
+	// root node
+	{
+		double2 c = make_double2(0, 0);
+		double h = ctx->root_half_length;
+		int m1 = -1, z = 0;
+		bool ok = CCF_HIP(ctx, hipMemcpyAsync(T.center, &c, sizeof(c), hipMemcpyHostToDevice, s)) &&
+			CCF_HIP(ctx, hipMemcpyAsync(T.hl, &h, sizeof(h), hipMemcpyHostToDevice, s)) &&
+			CCF_HIP(ctx, hipMemcpyAsync(T.parent, &m1, sizeof(int), hipMemcpyHostToDevice, s)) &&
+			CCF_HIP(ctx, hipMemcpyAsync(T.child, &m1, sizeof(int), hipMemcpyHostToDevice, s)) &&
+			CCF_HIP(ctx, hipMemcpyAsync(T.sstart, &z, sizeof(int), hipMemcpyHostToDevice, s)) &&
+			CCF_HIP(ctx, hipMemcpyAsync(T.scount, &N, sizeof(int), hipMemcpyHostToDevice, s)) &&
+			CCF_HIP(ctx, hipMemcpyAsync(T.nnbr, &z, sizeof(int), hipMemcpyHostToDevice, s)) &&
+			CCF_HIP(ctx, hipMemcpyAsync(T.nsl, &z, sizeof(int), hipMemcpyHostToDevice, s)) &&
+			CCF_HIP(ctx, hipMemcpyAsync(T.ndl, &z, sizeof(int), hipMemcpyHostToDevice, s)) &&
+			CCF_HIP(ctx, hipStreamSynchronize(s));
+		if (!ok) { hipFree(scal); return false; }
+	}
+	T.nnodes = 1;
+	T.level_off = {0};
+	T.level_cnt = {1};
+	T.levels = 0;
+
+	bool ok = true;
+	while (!P.direct)
+	{
+		const int off = T.level_off[T.levels], cnt = T.level_cnt[T.levels];
+		int h_scal[4] = {0, 0, 0, 0};
+		ok = CCF_HIP(ctx, hipMemsetAsync(scal, 0, 4 * sizeof(int), s));
+		if (!ok) break;
+		k_level_count<<<grid(cnt), BLOCK, 0, s>>>(off, cnt, T.scount, T.nbr, T.nnbr, T.split, scal);
+		k_scan<<<1, 1024, 0, s>>>(T.split, cnt, T.rank, scal + 1);
+		ok = CCF_LAUNCH(ctx, "tree level count") && CCF_HIP(ctx, hipMemcpyAsync(h_scal, scal, 4 * sizeof(int), hipMemcpyDeviceToHost, s)) &&
+			CCF_HIP(ctx, hipStreamSynchronize(s));
+		if (!ok) break;
+		if (verbose >= 3)
+		{
+			std::cout << "\nProcessing level " << T.levels << "\n";
+			std::cout << "Maximum number of stars in a node and its neighbors is " << h_scal[0] << "\n";
+		}
+		if (h_scal[0] <= MAX_NUM_STARS_DIRECT) break;
+		const int nsplit = h_scal[1];
+		const int new_off = off + cnt, new_cnt = 4 * nsplit;
+		if (verbose >= 3) std::cout << "Number of non-empty children: " << new_cnt << "\n";
+		if (!(ok = ensure_capacity(ctx, new_off + new_cnt))) break;
+		k_make_children<<<grid(cnt), BLOCK, 0, s>>>(off, cnt, new_off, T.split, T.rank, T.center, T.hl, T.parent, T.child,
+			T.sstart, T.scount, T.nnbr, T.nsl, T.ndl);
+		k_partition<<<cnt, BLOCK, 0, s>>>(off, T.split, T.child, T.center, T.sstart, T.scount, ctx->spos, ctx->smass,
+			ctx->spos_tmp, ctx->smass_tmp);
+		k_set_neighbors<<<grid(new_cnt), BLOCK, 0, s>>>(new_off, new_cnt, T.center, T.hl, T.parent, T.child, T.scount,
+			T.nbr, T.nnbr, T.sl, T.nsl, T.dl, T.ndl, scal);
+		ok = CCF_LAUNCH(ctx, "tree split level") && CCF_HIP(ctx, hipMemcpyAsync(h_scal, scal, 4 * sizeof(int), hipMemcpyDeviceToHost, s)) &&
+			CCF_HIP(ctx, hipStreamSynchronize(s));
+		if (!ok) break;
+		if (h_scal[2])
+		{
+			ctx->fail("Error. A node's neighbour or interaction list exceeded its maximum size.\n");
+			ok = false;
+			break;
+		}
+		T.nnodes = new_off + new_cnt;
+		T.level_off.push_back(new_off);
+		T.level_cnt.push_back(new_cnt);
+		T.levels++;
+	}
+	hipFree(scal);
+	if (!ok) return false;
+	if (verbose >= 2) std::cout << "tree_levels set to: " << T.levels << "\n\n";
+
+	auto t1 = std::chrono::steady_clock::now();
+	if (verbose >= 1)
+		std::cout << "Done creating children and sorting stars. Elapsed time: " << std::chrono::duration<double>(t1 - t0).count() << " seconds.\n\n";
+
+	// near-field ranges of every node
+	if (!CCF_HIP(ctx, hipMalloc(&T.nf, sizeof(int2) * (size_t)T.nnodes * MAX_NEAR_RANGES))) return false;
+	if (!CCF_HIP(ctx, hipMalloc(&T.nnf, sizeof(int) * (size_t)T.nnodes))) return false;
+	k_near_field<<<grid(T.nnodes), BLOCK, 0, s>>>(T.nnodes, T.sstart, T.scount, T.nbr, T.nnbr, T.nf, T.nnf);
+	if (!CCF_LAUNCH(ctx, "near field ranges")) return false;
+
+	// expansion order (ccf.cuh:840-852)
+	int p;
+	if (P.fmm_order > 0) p = P.fmm_order;
+	else p = (int)std::ceil(2 * std::log2(ctx->theta) + T.levels - std::log2(ctx->root_half_length) - std::log2(ctx->alpha_error));
+	if (verbose >= 2) std::cout << "expansion_order set to: " << p << "\n\n";
+	if (!P.direct)
+	{
+		if (p < 3) { ctx->fail("Error. Expansion order needs to be >= 3\n"); return false; }
+		if (p > MAX_EXPANSION_ORDER)
+		{
+			ctx->fail("Error. Maximum allowed expansion order is " + std::to_string(MAX_EXPANSION_ORDER) + "\n");
+			return false;
+		}
+	}
+	ctx->expansion_order = p;
+	T.order = P.direct ? 0 : p;
+
+	// FMM: upward from the deepest level, downward from level 2 (ccf.cuh:866-893)
+	if (verbose >= 1) std::cout << "Calculating multipole and local coefficients...\n";
+	const int P1 = T.order + 1;
+	if (!CCF_HIP(ctx, hipMalloc(&T.mp, sizeof(double2) * (size_t)T.nnodes * P1))) return false;
+	if (!CCF_HIP(ctx, hipMalloc(&T.loc, sizeof(double2) * (size_t)T.nnodes * P1))) return false;
+	if (!CCF_HIP(ctx, hipMalloc(&T.loc32, sizeof(float2) * (size_t)T.nnodes * P1))) return false;
+	if (!CCF_HIP(ctx, hipMemsetAsync(T.mp, 0, sizeof(double2) * (size_t)T.nnodes * P1, s))) return false;
+	if (!CCF_HIP(ctx, hipMemsetAsync(T.loc, 0, sizeof(double2) * (size_t)T.nnodes * P1, s))) return false;
+	if (T.order > 0)
+	{
+		std::vector<double> B = binomial_table(2 * p, P.binomial_exact != 0);
+		if (!CCF_HIP(ctx, hipMalloc(&T.binom, sizeof(double) * B.size()))) return false;
+		if (!CCF_HIP(ctx, hipMemcpyAsync(T.binom, B.data(), sizeof(double) * B.size(), hipMemcpyHostToDevice, s))) return false;
+		for (int l = T.levels; l >= 0; l--)
+		{
+			const long long work = (long long)T.level_cnt[l] * P1;
+			k_upward<<<grid(work), BLOCK, 0, s>>>(T.level_off[l], T.level_cnt[l], p, T.center, T.hl, T.child, T.sstart, T.scount,
+				ctx->spos, ctx->smass, T.binom, T.mp);
+		}
+		for (int l = 2; l <= T.levels; l++)
+		{
+			const long long work = (long long)T.level_cnt[l] * P1;
+			k_downward<<<grid(work), BLOCK, 0, s>>>(T.level_off[l], T.level_cnt[l], p, T.center, T.hl, T.parent, T.sl, T.nsl,
+				T.dl, T.ndl, T.sstart, T.scount, ctx->spos, ctx->smass, T.binom, T.mp, T.loc);
+		}
+		if (!CCF_LAUNCH(ctx, "calculate_coeffs_kernels")) return false;
+	}
+	k_to_float<<<grid((long long)T.nnodes * P1), BLOCK, 0, s>>>(T.loc, T.loc32, (size_t)T.nnodes * P1);
+	if (!CCF_LAUNCH(ctx, "local coefficients to float")) return false;
+	if (!CCF_HIP(ctx, hipStreamSynchronize(s))) return false;
+	auto t2 = std::chrono::steady_clock::now();
+	if (verbose >= 1)
+		std::cout << "Done calculating multipole and local coefficients. Elapsed time: " << std::chrono::duration<double>(t2 - t1).count() << " seconds.\n\n";
+	ctx->info.t_tree = std::chrono::duration<double>(t2 - t0).count();
+	ctx->info.tree_levels = T.levels;
+	ctx->info.num_nodes = T.nnodes;
+	ctx->info.expansion_order = T.order;
+	return true;
+}
+
+} // namespace ccf

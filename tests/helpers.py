@@ -1,0 +1,69 @@
+"""Shared helpers of the test suite: synthetic fields, oracle runs and
+set matching of root columns."""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+from scipy.optimize import linear_sum_assignment
+
+import ccf_amd
+import oracle
+from ccf_amd import io
+
+# a few small fields covering the four smooth-matter variants
+FIELDS = {
+    # 100 equal masses, kappa = kappa_star = 0.3, shear 0.1
+    "cfg1": dict(num_stars=100, kappa_tot=0.3, kappa_star=0.3, shear=0.1, mass_function="equal", seed=0),
+    "salpeter300": dict(num_stars=300, kappa_tot=0.45, kappa_star=0.45, shear=0.35, mass_function="salpeter",
+                        m_lower=0.08, m_upper=1.0, seed=1),
+    "rect200": dict(num_stars=200, kappa_tot=0.7, kappa_star=0.35, shear=0.25, mass_function="kroupa",
+                    m_lower=0.08, m_upper=1.0, rectangular=1, seed=3),
+    "smooth150": dict(num_stars=150, kappa_tot=0.6, kappa_star=0.4, shear=0.2, mass_function="uniform",
+                      m_lower=0.1, m_upper=1.0, seed=5),
+}
+
+
+def make_field(dirpath, name: str) -> str:
+    spec = dict(FIELDS[name])
+    seed = spec.pop("seed")
+    path = os.path.join(str(dirpath), f"{name}.bin")
+    if not os.path.exists(path):
+        ccf_amd.write_synthetic_stars(path, seed, **spec)
+    return path
+
+
+def oracle_run(starfile: str, prefix: str, **kw) -> dict:
+    p = oracle.params(starfile=starfile, outfile_prefix=prefix, **kw)
+    return oracle.run(p)
+
+
+def match_column(a: np.ndarray, b: np.ndarray):
+    """Match root set a to root set b; returns |a_i - b_pi(i)| (index match
+    first, Hungarian assignment on the entries that do not agree by index)."""
+    d = np.abs(a - b)
+    scale = np.maximum(np.abs(b), 1.0)
+    bad = np.nonzero(d > 1e-9 * scale)[0]
+    if bad.size:
+        aa, bb = a[bad], b[bad]
+        cost = np.abs(aa[:, None] - bb[None, :])
+        r, c = linear_sum_assignment(cost)
+        d = d.copy()
+        d[bad[r]] = cost[r, c]
+    return d
+
+
+def compare_arrays(x: np.ndarray, y: np.ndarray, num_roots: int, num_branches: int):
+    """Max |dz| / max(|z|, 1) over all phi columns after set matching, and the
+    number of points whose relative difference exceeds 1e-9."""
+    cx = io.columns_by_phi(x, num_roots, num_branches)
+    cy = io.columns_by_phi(y, num_roots, num_branches)
+    worst = 0.0
+    nbad = 0
+    for c in range(cx.shape[0]):
+        for k in range(cx.shape[1]):
+            d = match_column(cx[c, k], cy[c, k])
+            rel = d / np.maximum(np.abs(cy[c, k]), 1.0)
+            worst = max(worst, float(rel.max()))
+            nbad += int((rel > 1e-9).sum())
+    return worst, nbad

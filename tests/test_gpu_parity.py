@@ -1,0 +1,120 @@
+"""Parity of the HIP path with the FP64 oracle on identical star files.
+
+Every test calls the product through its C-ABI (libccf_amd.so) and compares
+with the oracle (oracle/libccf_oracle.so) run on the same inputs:
+  - field evaluation (f0, f1, lens map, length scale, 1/mu bound) at random
+    points, tree and direct (relative 1e-11);
+  - one Jacobi Aberth sweep (relative 1e-11);
+  - whole runs: critical curves and caustics per phi column as sets,
+    |dz| <= 1e-9 max(|z|, theta_E) (the parity bar), plus the parameter
+    file, the star file and the 1/mu error.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import ccf_amd
+import oracle
+from ccf_amd import io
+from helpers import FIELDS, compare_arrays, make_field, oracle_run
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx(starfile, prefix, **kw):
+    return ccf_amd.Context(starfile=starfile, outfile_prefix=prefix, **kw)
+
+
+@pytest.mark.parametrize("name,kw", [
+    ("cfg1", dict(kappa_tot=0.3, shear=0.1)),
+    ("salpeter300", dict(kappa_tot=0.45, shear=0.35)),
+    ("rect200", dict(kappa_tot=0.7, shear=0.25)),
+    ("rect200", dict(kappa_tot=0.7, shear=0.25, approx=0)),
+    ("smooth150", dict(kappa_tot=0.6, shear=0.2, approx=0)),
+    ("smooth150", dict(kappa_tot=0.6, shear=0.2, direct=1)),
+])
+def test_eval_matches_oracle(workdir, name, kw):
+    sf = make_field(workdir, name)
+    rng = np.random.default_rng(7)
+    n, rect, corner, theta, st = io.read_stars(sf)
+    scale = max(abs(corner.real), abs(corner.imag), abs(corner))
+    # points near stars, inside and outside the field
+    z = st[rng.integers(0, n, 300), 0] + 1j * st[rng.integers(0, n, 300), 1]
+    z = np.concatenate([z + 0.3 * (rng.standard_normal(300) + 1j * rng.standard_normal(300)),
+                        1.3 * scale * (rng.random(50) - 0.5 + 1j * (rng.random(50) - 0.5))])
+    phi = 1.234
+    with _ctx(sf, str(workdir) + "/", num_phi=10, **kw) as c:
+        c.prepare()
+        g = c.eval_points(z, phi)
+    o, _ = oracle.evaluate(oracle.params(starfile=sf, num_phi=10, **kw), z, phi)
+    for col, what in [(0, "f0"), (2, "f1"), (4, "w")]:
+        a = g[:, col] + 1j * g[:, col + 1]
+        b = o[:, col] + 1j * o[:, col + 1]
+        rel = np.abs(a - b) / np.maximum(np.abs(b), 1.0)
+        assert rel.max() < 1e-11, (what, rel.max())
+    rel = np.abs(g[:, 6] - o[:, 6]) / np.abs(o[:, 6])
+    assert rel.max() < 1e-9
+    assert np.allclose(g[:, 7], o[:, 7], rtol=1e-9, atol=1e-15)
+
+
+def test_aberth_sweep_matches_oracle(workdir):
+    sf = make_field(workdir, "salpeter300")
+    n, rect, corner, theta, st = io.read_stars(sf)
+    roots = np.concatenate([st[:, 0] + 1 + 1j * st[:, 1], st[:, 0] - 1 + 1j * st[:, 1]])
+    phi = np.pi
+    kw = dict(kappa_tot=0.45, shear=0.35, num_phi=10)
+    with _ctx(sf, str(workdir) + "/", **kw) as c:
+        c.prepare()
+        g = c.aberth_sweep(roots, phi)
+    o = oracle.aberth_sweep(oracle.params(starfile=sf, **kw), roots, phi)
+    rel = np.abs(g - o) / np.maximum(np.abs(o), 1.0)
+    assert rel.max() < 1e-11
+
+
+RUNS = [
+    ("cfg1", dict(kappa_tot=0.3, shear=0.1, num_phi=100, num_branches=1, direct=1)),
+    ("cfg1", dict(kappa_tot=0.3, shear=0.1, num_phi=100, num_branches=1)),
+    ("salpeter300", dict(kappa_tot=0.45, shear=0.35, num_phi=120, num_branches=3, fmm_order=20)),
+    ("rect200", dict(kappa_tot=0.7, shear=0.25, num_phi=80, num_branches=2)),
+    ("smooth150", dict(kappa_tot=0.6, shear=0.2, num_phi=60, num_branches=1, approx=0)),
+]
+
+
+@pytest.mark.parametrize("name,kw", RUNS)
+def test_run_matches_oracle(workdir, name, kw):
+    sf = make_field(workdir, name)
+    tag = f"{name}_{abs(hash(tuple(sorted(kw.items())))) % 100000}"
+    gp = os.path.join(str(workdir), f"g_{tag}_")
+    op = os.path.join(str(workdir), f"o_{tag}_")
+    with _ctx(sf, gp, write_mu_length_scales=1, **kw) as c:
+        assert c.run(0), c.last_error()
+        assert c.save(0), c.last_error()
+        ginfo = c.info()
+    ostats = oracle_run(sf, op, write_mu_length_scales=1, **kw)
+    R, nb = ostats["num_roots"], kw["num_branches"]
+    assert ginfo["num_roots"] == R
+    assert ginfo["tree_levels"] == ostats["tree_levels"]
+    assert ginfo["expansion_order"] == ostats["expansion_order"]
+    gccs = io.read_array(gp + "ccf_ccs.bin")
+    occs = io.read_array(op + "ccf_ccs.bin")
+    assert gccs.shape == occs.shape == (R * nb, kw["num_phi"] // nb + 1)
+    worst, nbad = compare_arrays(gccs, occs, R, nb)
+    assert nbad == 0, (worst, nbad)
+    gc = io.read_array(gp + "ccf_caustics.bin")
+    oc = io.read_array(op + "ccf_caustics.bin")
+    worst_c, nbad_c = compare_arrays(gc, oc, R, nb)
+    assert nbad_c == 0, (worst_c, nbad_c)
+    gm = io.read_array(gp + "ccf_mu_length_scales.bin", complex_values=False)
+    assert gm.shape == gccs.shape and np.all(np.isfinite(gm)) and np.all(gm > 0)
+    # parameter files agree except for the timings
+    a = io.read_params(gp + "ccf_parameter_info.txt")
+    b = io.read_params(op + "ccf_parameter_info.txt")
+    for k in ("kappa_tot", "shear", "mu_ave", "kappa_star", "num_stars", "num_roots", "num_phi", "num_branches"):
+        assert a[k] == b[k], k
+    assert a["max_error_1/mu"] < 1e-8 and b["max_error_1/mu"] < 1e-8
+    # both write the same star set (tree order may differ)
+    _, _, _, _, gs = io.read_stars(gp + "ccf_stars.bin")
+    _, _, _, _, os_ = io.read_stars(op + "ccf_stars.bin")
+    assert np.array_equal(np.sort(gs.view("f8,f8,f8"), axis=0), np.sort(os_.view("f8,f8,f8"), axis=0))
+    assert ginfo["points_converged"] == ostats["points_converged"] == R * (kw["num_phi"] + nb)
