@@ -1,0 +1,272 @@
+#!/usr/bin/env python3
+"""Throughput of the critical-curve hot path on one MI355X.
+
+Workload (default): 10,000 stars, Salpeter IMF (slope -2.35) on [0.08, 1],
+uniform disc for kappa_star = kappa_tot = 0.5, shear 0.2, num_phi 2000,
+num_branches 20, FMM order 24, star-file seed 2 (synthetic, written by
+ccf_write_synthetic_stars). One step = tree + FMM build, the Aberth solve at
+the 20 branch centres, the phi continuation, the 1/mu error pass, the
+transpose and the caustic map, with the stars already in HBM. The metric is
+converged critical-curve points per second.
+
+Prints one JSON line. The Aberth kernel's time comes from HIP events
+recorded on the stream it is launched on; its algorithmic flops from the
+per-update counts (10 per Cauchy term, 36 per near-field star, 16 per FMM
+order). The CPU baseline is the FP64 oracle with OpenMP on the host cores,
+timed on one branch over a few phi steps and scaled to the whole job.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import ccf_amd  # noqa: E402
+from ccf_amd import io  # noqa: E402
+
+METRIC = "converged cc points/s at 10k stars, mixed vs FP64 on 1 MI355X; max root error"
+PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 vector (and matrix) peak
+PEAK_FP32_TFLOPS = 157.3
+
+WORKLOADS = {
+    "cc10k": dict(stars=dict(num_stars=10000, kappa_tot=0.5, kappa_star=0.5, shear=0.2, mass_function="salpeter",
+                             m_lower=0.08, m_upper=1.0), seed=2,
+                  run=dict(kappa_tot=0.5, shear=0.2, num_phi=2000, num_branches=20, fmm_order=24)),
+    "cc2k": dict(stars=dict(num_stars=2000, kappa_tot=0.45, kappa_star=0.45, shear=0.35, mass_function="salpeter",
+                            m_lower=0.08, m_upper=1.0), seed=1,
+                 run=dict(kappa_tot=0.45, shear=0.35, num_phi=1000, num_branches=10, fmm_order=20)),
+}
+
+
+def run_gpu(starfile: str, run_kw: dict, precision: int, steps: int, warmup: int, predictor: int = 0):
+    times, infos = [], []
+    ctx = ccf_amd.Context(starfile=starfile, outfile_prefix="./", write_stars=0, write_critical_curves=0,
+                          write_caustics=1, precision=precision, timing=1, predictor=predictor, **run_kw)
+    for i in range(warmup + steps):
+        ctx.stage("load")
+        t0 = time.perf_counter()
+        for st in ("build_tree_fmm", "find_initial_roots", "find_ccs", "find_caustics"):
+            ctx.stage(st)
+        dt = time.perf_counter() - t0
+        print(f"# precision={precision} predictor={predictor} step {i} ({'warmup' if i < warmup else 'timed'}): "
+              f"{dt:.3f} s", file=sys.stderr, flush=True)
+        if i >= warmup:
+            times.append(dt)
+            infos.append(ctx.info())
+    ccs = ctx.array(ccf_amd.ARRAY_CCS)
+    ctx.close()
+    return times, infos, ccs
+
+
+def set_error(a: np.ndarray, b: np.ndarray, R: int, nb: int, theta: float = 1.0):
+    """max over phi columns of the matched |a - b| / theta_E (index match, then
+    Hungarian assignment over the entries that disagree), and the matched
+    pairs (branch, column, a_i, b_i) that differ by more than 1e-9 theta_E."""
+    from scipy.optimize import linear_sum_assignment
+
+    ca = io.columns_by_phi(a, R, nb)
+    cb = io.columns_by_phi(b, R, nb)
+    d = np.abs(ca - cb)
+    worst = 0.0
+    pairs = []
+    bad = np.argwhere(d.max(axis=2) > 1e-9 * theta)
+    for c, k in bad:
+        x, y = ca[c, k], cb[c, k]
+        idx = np.nonzero(np.abs(x - y) > 1e-9 * theta)[0]
+        cost = np.abs(x[idx][:, None] - y[idx][None, :])
+        r, s = linear_sum_assignment(cost)
+        dd = np.abs(x - y)
+        dd[idx[r]] = cost[r, s]
+        worst = max(worst, float(dd.max()))
+        for i, j in zip(r, s):
+            if cost[i, j] > 1e-9 * theta:
+                pairs.append((int(c), int(k), x[idx[i]], y[idx[j]]))
+    mask = np.ones(d.shape[:2], bool)
+    if bad.size:
+        mask[bad[:, 0], bad[:, 1]] = False
+    if mask.any():
+        worst = max(worst, float(d[mask].max()))
+    return worst / theta, pairs
+
+
+def polished_error(starfile: str, run_kw: dict, pairs) -> float:
+    """max |p(a) - p(b)| over the differing pairs, p = two FP64 Newton steps
+    z - f0/f1 on the GPU: the distance left once both roots are taken to
+    the exact root of Witt's equation."""
+    if not pairs:
+        return 0.0
+    npp, nb = run_kw["num_phi"], run_kw["num_branches"]
+    worst = 0.0
+    with ccf_amd.Context(starfile=starfile, outfile_prefix="./", **run_kw) as ctx:
+        ctx.prepare()
+        by_col = {}
+        for c, k, x, y in pairs:
+            by_col.setdefault((c, k), []).append((x, y))
+        for (c, k), zz in by_col.items():
+            phi = 2 * np.pi * (c * npp // nb + k) / npp
+            z = np.array([p for pr in zz for p in pr], dtype=np.complex128)
+            for _ in range(2):
+                v = ctx.eval_points(z, phi)
+                z = z - (v[:, 0] + 1j * v[:, 1]) / (v[:, 2] + 1j * v[:, 3])
+            worst = max(worst, float(np.abs(z[0::2] - z[1::2]).max()))
+    return worst
+
+
+def cpu_baseline(starfile: str, run_kw: dict, steps: int, threads: int, gpu_ccs: np.ndarray | None, R: int):
+    """Oracle (FP64, OpenMP) on branch 0: initial solve + `steps` phi steps per side,
+    scaled to the whole job; also checks those rows against the GPU result."""
+    import oracle
+
+    nb, npp = run_kw["num_branches"], run_kw["num_phi"]
+    with tempfile.TemporaryDirectory() as d:
+        p = oracle.params(starfile=starfile, outfile_prefix=d + "/", branch_limit=1, step_limit=steps,
+                          num_threads=threads, write_stars=0, write_caustics=0, **run_kw)
+        t0 = time.perf_counter()
+        s = oracle.run(p)
+        wall = time.perf_counter() - t0
+        occs = io.read_array(d + "/ccf_ccs.bin")
+    half = npp // (2 * nb)
+    t_job = nb * (s["t_init_roots"] + half * s["t_ccs"] / steps)
+    points = R * (npp + nb)
+    res = dict(value=points / t_job, unit="points/s", cores=threads, kind="port",
+               sample=f"FP64 oracle, branch 0 of {nb}: initial solve + {steps} phi steps per side "
+                      f"({s['points_solved']} points, {wall:.1f} s wall); job time = {nb} x (t_init + {half} x t_step)",
+               sample_seconds=wall, t_init=s["t_init_roots"], t_step=s["t_ccs"] / steps)
+    if gpu_ccs is not None:
+        # rows of branch 0 the sample solved: columns half-steps .. half+steps
+        cols = list(range(half - steps, half + steps + 1))
+        a = io.columns_by_phi(gpu_ccs, R, nb)[0, cols]
+        b = io.columns_by_phi(occs, R, nb)[0, cols]
+        worst = 0.0
+        from scipy.optimize import linear_sum_assignment
+        for k in range(len(cols)):
+            dd = np.abs(a[k] - b[k])
+            idx = np.nonzero(dd > 1e-9 * np.maximum(np.abs(b[k]), 1))[0]
+            if idx.size:
+                cost = np.abs(a[k][idx][:, None] - b[k][idx][None, :])
+                r, c = linear_sum_assignment(cost)
+                dd[idx[r]] = cost[r, c]
+            worst = max(worst, float((dd / np.maximum(np.abs(b[k]), 1)).max()))
+        res["oracle_max_rel_diff"] = worst
+    return res
+
+
+def summarize(times, infos):
+    inf = infos[-1]
+    t = float(np.mean(times))
+    conv = inf["points_converged"]
+    k_ms = inf["aberth_kernel_ms"]
+    launches = max(1, inf["aberth_launches"])
+    achieved = inf["flops_sums"] / (k_ms * 1e-3) / 1e12 if k_ms > 0 else 0.0
+    return dict(points_per_s=conv / t, s_per_step=t, points_converged=conv, points_total=inf["points_total"],
+                t_tree=inf["t_tree"], t_init_roots=inf["t_init_roots"], t_ccs=inf["t_ccs"], t_caustics=inf["t_caustics"],
+                updates_full=inf["updates_full"], updates_early=inf["updates_early"], sweeps=inf["sweeps"],
+                cluster_chunks=inf["cluster_chunks"],
+                aberth_ms_total=k_ms, aberth_launches=launches, aberth_ms_avg=k_ms / launches,
+                sums_tflops=achieved, sums_flops=inf["flops_sums"], update_flops=inf["flops_update"],
+                expansion_order=inf["expansion_order"], tree_levels=inf["tree_levels"], num_roots=inf["num_roots"])
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="cc10k", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"])
+    ap.add_argument("--cpu-steps", type=int, default=8, help="phi steps per side in the CPU sample (0: skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--predictor", type=int, default=2, help="continuation predictor of the mixed form (0, 1, 2)")
+    ap.add_argument("--ablation", action="store_true", help="also time FP64 + predictor and mixed without predictor")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus must be 1 (the path would shard by branch, see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    W = WORKLOADS[args.workload]
+    tmp = tempfile.mkdtemp(prefix="ccfbench_")
+    sf = os.path.join(tmp, "stars.bin")
+    ccf_amd.write_synthetic_stars(sf, W["seed"], **W["stars"])
+    run_kw = W["run"]
+
+    res = {}
+    ccs = {}
+    # plain FP64 path = the reference's method; mixed form = FP32 Cauchy sums,
+    # FMM pole sum, continuation predictor. --ablation adds the two halves.
+    modes = {"fp64": [("fp64", 0, 0)], "mixed": [("mixed", 1, args.predictor)],
+             "both": [("fp64", 0, 0), ("mixed", 1, args.predictor)]}[args.precision]
+    if args.ablation:
+        modes += [("fp64_predictor", 0, args.predictor), ("mixed_no_predictor", 1, 0)]
+    for name, prec, pred in modes:
+        times, infos, ccs[name] = run_gpu(sf, run_kw, prec, args.steps, args.warmup, pred)
+        res[name] = summarize(times, infos)
+        res[name]["predictor"] = pred
+    R = res[next(iter(res))]["num_roots"]
+    if "fp64" in ccs:
+        for name in res:
+            if name == "fp64":
+                continue
+            err, pairs = set_error(ccs[name], ccs["fp64"], R, run_kw["num_branches"])
+            res[name]["max_root_err_vs_fp64_thetaE"] = err
+            res[name]["points_differing_over_1e-9"] = len(pairs)
+            res[name]["max_root_err_vs_fp64_after_newton_polish_thetaE"] = polished_error(sf, run_kw, pairs[:2000])
+            res[name]["speedup_vs_fp64"] = res[name]["points_per_s"] / res["fp64"]["points_per_s"]
+
+    head = res["mixed"] if "mixed" in res else res["fp64"]
+    mixed_head = "mixed" in res
+    pk = PEAK_FP32_TFLOPS if mixed_head else PEAK_FP64_TFLOPS
+    roof = {"bound": "mfma",
+            "pipe": ("FP32 VALU (v_fma_f32, v_rcp_f32; packed FP32 disabled), MI355X FP32 vector peak" if mixed_head else
+                     "FP64 VALU (v_fma_f64, v_rcp_f64); MI355X FP64 vector peak = FP64 matrix peak"),
+            "kernel": "k_sums32" if mixed_head else "k_sums64",
+            "achieved": head["sums_tflops"], "peak": pk, "unit": "TFLOP/s",
+            "frac": head["sums_tflops"] / pk, "traffic": None,
+            "avg_launch_ms": head["aberth_ms_avg"], "launches": head["aberth_launches"]}
+    if "fp64" in res:
+        roof["fp64_kernel"] = {"kernel": "k_sums64", "achieved": res["fp64"]["sums_tflops"], "peak": PEAK_FP64_TFLOPS,
+                               "frac": res["fp64"]["sums_tflops"] / PEAK_FP64_TFLOPS,
+                               "avg_launch_ms": res["fp64"]["aberth_ms_avg"]}
+
+    cpu = None
+    if args.cpu_steps > 0:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(sf, run_kw, args.cpu_steps, threads, ccs.get("fp64"), R)
+
+    line = {
+        "metric": METRIC,
+        "value": head["points_per_s"],
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["s_per_step"] * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if "mixed" not in res else "f64+f32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {W['stars']['num_stars']} stars {W['stars']['mass_function']} "
+                               f"[{W['stars']['m_lower']},{W['stars']['m_upper']}], kappa={run_kw['kappa_tot']}, "
+                               f"shear={run_kw['shear']}, num_phi={run_kw['num_phi']}, "
+                               f"num_branches={run_kw['num_branches']}, fmm_order={run_kw['fmm_order']}, seed={W['seed']}",
+                   "num_roots": R, "form": "mixed" if "mixed" in res else "fp64"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "fp64": res.get("fp64"),
+        "mixed": res.get("mixed"),
+        "ablation": {k: v for k, v in res.items() if k not in ("fp64", "mixed")} or None,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -59,6 +59,7 @@ class Params(ctypes.Structure):
         ("step_iters", ctypes.c_int),
         ("roots_per_thread", ctypes.c_int),
         ("timing", ctypes.c_int),
+        ("predictor", ctypes.c_int),
     ]
 
 
@@ -91,6 +92,9 @@ class Info(ctypes.Structure):
         ("flops_update", ctypes.c_double),
         ("aberth_kernel_ms", ctypes.c_double),
         ("aberth_launches", ctypes.c_longlong),
+        ("sweeps", ctypes.c_longlong),
+        ("cluster_chunks", ctypes.c_longlong),
+        ("flops_sums", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -120,7 +124,10 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         "ccf_run": (ctypes.c_int, [vp, ctypes.c_int]),
         "ccf_save": (ctypes.c_int, [vp, ctypes.c_int]),
         "ccf_check_input_params": (ctypes.c_int, [vp, ctypes.c_int]),
+        "ccf_calculate_derived_params": (ctypes.c_int, [vp, ctypes.c_int]),
         "ccf_prepare": (ctypes.c_int, [vp, ctypes.c_int]),
+        "ccf_load": (ctypes.c_int, [vp, ctypes.c_int]),
+        "ccf_build_tree_fmm": (ctypes.c_int, [vp, ctypes.c_int]),
         "ccf_find_initial_roots": (ctypes.c_int, [vp, ctypes.c_int]),
         "ccf_find_ccs": (ctypes.c_int, [vp, ctypes.c_int]),
         "ccf_find_caustics": (ctypes.c_int, [vp, ctypes.c_int]),

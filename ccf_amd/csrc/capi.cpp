@@ -36,6 +36,7 @@ ccf::LensView ccf_ctx::lens() const
 	L.taylor_smooth = taylor_smooth;
 	L.corner_re = corner.re;
 	L.corner_im = corner.im;
+	L.corner_abs = corner.abs();
 	L.corner_abs2 = corner.abs() * corner.abs();
 	L.corner_arg = corner.arg();
 	L.order = tree.order;
@@ -50,8 +51,13 @@ ccf::LensView ccf_ctx::lens() const
 	L.child = tree.child;
 	L.nf = tree.nf;
 	L.nnf = tree.nnf;
+	L.nbox = tree.nbox;
+	L.nf_off = tree.nf_off;
+	L.nf_cnt = tree.nf_cnt;
+	L.nfs = tree.nfs;
 	L.loc = tree.loc;
 	L.loc32 = tree.loc32;
+	L.locu32 = tree.locu32;
 	return L;
 }
 
@@ -77,18 +83,19 @@ void ccf_ctx::free_device()
 	ccf::DeviceTree& T = tree;
 	F(T.center); F(T.hl); F(T.parent); F(T.child); F(T.sstart); F(T.scount);
 	F(T.nbr); F(T.nnbr); F(T.sl); F(T.nsl); F(T.dl); F(T.ndl); F(T.split); F(T.rank);
-	F(T.nf); F(T.nnf); F(T.mp); F(T.loc); F(T.loc32); F(T.binom);
+	F(T.nf); F(T.nnf); F(T.nbox); F(T.nf_off); F(T.nf_cnt); F(T.nfs); F(T.mp); F(T.loc); F(T.loc32); F(T.mpu); F(T.locu); F(T.locu32); F(T.binom);
 	T = ccf::DeviceTree{};
-	F(spos); F(smass); F(spos_tmp); F(smass_tmp); F(spos32); F(ts_a); F(ts_b);
+	F(spos); F(smass); F(spos_tmp); F(smass_tmp); F(spos32); F(ts_a); F(ts_b); F(ones);
 	F(ccs_init); F(ccs); F(caustics); F(mu_scales);
-	F(act); F(act_cnt); F(nv); F(conv); F(rows_dev); F(phis_dev); F(counters);
+	ccf::free_sweep(sw);
+	F(counters);
 	for (auto& t : timers)
 	{
 		if (t.a) hipEventDestroy(t.a);
 		if (t.b) hipEventDestroy(t.b);
 	}
 	timers.clear();
-	prepared = have_roots = have_ccs = have_caustics = have_mu = false;
+	loaded = prepared = have_roots = have_ccs = have_caustics = have_mu = false;
 }
 
 namespace
@@ -151,6 +158,7 @@ bool check_input_params(ccf_ctx* c)
 	if (P.roots_per_thread != 0 && P.roots_per_thread != 1 && P.roots_per_thread != 2 && P.roots_per_thread != 4)
 		return err("Error. roots_per_thread must be 0 (automatic), 1, 2 or 4.\n");
 	if (P.init_iters < 0 || P.step_iters < 0) return err("Error. iteration caps must be >= 0\n");
+	if (P.predictor < 0 || P.predictor > 2) return err("Error. predictor must be 0, 1 or 2.\n");
 	return true;
 }
 
@@ -512,6 +520,7 @@ bool device_init(ccf_ctx* c)
 	if (n < 1) { c->fail("Error. No HIP capable devices detected.\n"); return false; }
 	if (!CCF_HIP(c, hipSetDevice(0))) return false;
 	if (!c->stream && !CCF_HIP(c, hipStreamCreate(&c->stream))) return false;
+	if (!CCF_HIP(c, hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, 0))) return false;
 	return true;
 }
 
@@ -533,7 +542,11 @@ extern "C" {
 ccf_ctx* ccf_create(void)
 {
 	ccf_ctx* c = new (std::nothrow) ccf_ctx();
-	if (c) ccf_default_params(&c->P);
+	if (c)
+	{
+		ccf_default_params(&c->P_in);
+		c->P = c->P_in;
+	}
 	return c;
 }
 
@@ -573,10 +586,11 @@ void ccf_default_params(ccf_params* p)
 int ccf_set_params(ccf_ctx* c, const ccf_params* p)
 {
 	if (!c || !p) return 1;
-	c->P = *p;
-	c->P.mass_function[sizeof(c->P.mass_function) - 1] = 0;
-	c->P.starfile[CCF_PATH_MAX - 1] = 0;
-	c->P.outfile_prefix[CCF_PATH_MAX - 1] = 0;
+	c->P_in = *p;
+	c->P_in.mass_function[sizeof(c->P_in.mass_function) - 1] = 0;
+	c->P_in.starfile[CCF_PATH_MAX - 1] = 0;
+	c->P_in.outfile_prefix[CCF_PATH_MAX - 1] = 0;
+	c->P = c->P_in;
 	return 0;
 }
 
@@ -590,25 +604,55 @@ int ccf_get_params(const ccf_ctx* c, ccf_params* p)
 int ccf_check_input_params(ccf_ctx* c, int verbose)
 {
 	if (!c) return 1;
+	c->P = c->P_in;
 	if (verbose >= 3) std::cout << "Checking input parameters...\n";
 	if (!check_input_params(c)) return 1;
 	if (verbose >= 3) std::cout << "Done checking input parameters.\n\n";
 	return 0;
 }
 
-int ccf_prepare(ccf_ctx* c, int verbose)
+// validation and derived parameters need no device (the reference
+// initialises the device first, ccf.cuh:1267-1270)
+static bool derive(ccf_ctx* c, int verbose)
 {
-	if (!c) return 1;
 	c->last_error.clear();
-	c->free_device();
 	c->info = ccf_info{};
 	c->stars_in.clear();
-	// validation and derived parameters need no device (the reference
-	// initialises the device first, ccf.cuh:1267-1270)
-	if (!check_input_params(c)) return 1;
-	if (!calculate_derived_params(c, verbose)) return 1;
+	c->P = c->P_in;
+	if (!check_input_params(c)) return false;
+	if (!calculate_derived_params(c, verbose)) return false;
+	c->info.num_stars = c->num_stars;
+	c->info.num_roots = c->num_roots;
+	c->info.taylor_smooth = c->taylor_smooth;
+	c->info.kappa_star = c->kappa_star;
+	c->info.mu_ave = c->mu_ave;
+	c->info.corner_re = c->corner.re;
+	c->info.corner_im = c->corner.im;
+	c->info.alpha_error = c->alpha_error;
+	return true;
+}
+
+int ccf_calculate_derived_params(ccf_ctx* c, int verbose)
+{
+	if (!c) return 1;
+	return derive(c, verbose) ? 0 : 1;
+}
+
+int ccf_load(ccf_ctx* c, int verbose)
+{
+	if (!c) return 1;
+	c->free_device();
+	c->loaded = false;
+	if (!derive(c, verbose)) return 1;
 	if (!device_init(c)) return 1;
 	if (!populate(c, verbose)) return 1;
+	c->loaded = true;
+	return 0;
+}
+
+int ccf_build_tree_fmm(ccf_ctx* c, int verbose)
+{
+	if (!c || !need(c, c->loaded && !c->prepared, "ccf_load")) return 1;
 	if (!ccf::build_tree(c, verbose)) return 1;
 	c->info.num_stars = c->num_stars;
 	c->info.num_roots = c->num_roots;
@@ -622,6 +666,12 @@ int ccf_prepare(ccf_ctx* c, int verbose)
 	c->info.alpha_error = c->alpha_error;
 	c->prepared = true;
 	return 0;
+}
+
+int ccf_prepare(ccf_ctx* c, int verbose)
+{
+	if (ccf_load(c, verbose)) return 1;
+	return ccf_build_tree_fmm(c, verbose);
 }
 
 int ccf_find_initial_roots(ccf_ctx* c, int verbose)

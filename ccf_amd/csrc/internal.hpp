@@ -50,9 +50,18 @@ struct DeviceTree
 	int* rank = nullptr;
 	int2* nf = nullptr;    // near-field star ranges: self then neighbours [capacity][9]
 	int* nnf = nullptr;
+	double4* nbox = nullptr;  // {centre x, centre y, half-length, first child} per node
+	int* nf_off = nullptr;    // leaves: first entry of their near-field copy in nfs, -1 otherwise
+	int* nf_cnt = nullptr;    // leaves: number of near-field stars
+	double4* nfs = nullptr;   // near-field stars of every leaf, contiguous: {x, y, m, 0}
 	double2* mp = nullptr;  // multipole coefficients [nnodes][order + 1]
 	double2* loc = nullptr; // local coefficients [nnodes][order + 1]
 	float2* loc32 = nullptr;
+	// expansions of the same tree with every mass set to 1: their L2P gives
+	// the far part of sum 1/(z - s_i), the pole sum of the Aberth correction
+	double2* mpu = nullptr;
+	double2* locu = nullptr;
+	float2* locu32 = nullptr;
 	double* binom = nullptr;
 };
 
@@ -61,7 +70,7 @@ struct LensView
 {
 	double kappa, gamma, theta, theta2, kappastar;
 	int rectangular, approx, taylor_smooth;
-	double corner_re, corner_im, corner_abs2, corner_arg;
+	double corner_re, corner_im, corner_abs, corner_abs2, corner_arg;
 	int order;
 	int nstars;
 	int nts;                   // number of precomputed smooth-matter series terms
@@ -74,8 +83,13 @@ struct LensView
 	const int* child;
 	const int2* nf;
 	const int* nnf;
+	const double4* nbox;
+	const int* nf_off;
+	const int* nf_cnt;
+	const double4* nfs;
 	const double2* loc;
 	const float2* loc32;
+	const float2* locu32;
 };
 
 struct Timer
@@ -83,11 +97,28 @@ struct Timer
 	hipEvent_t a = nullptr, b = nullptr;
 };
 
+// Work buffers of the Aberth sweeps over up to `rows` rows of R roots.
+struct SweepBufs
+{
+	int rows = 0, R = 0, tile = 0, max_items = 0;
+	int* act = nullptr;      // [2][rows][R] active root indices (ping-pong)
+	int* cnt = nullptr;      // [2][rows]
+	int* fl = nullptr;       // [rows][R] roots that need the Cauchy sums this sweep
+	int* fcnt = nullptr;     // [2][rows]
+	double2* f0 = nullptr;   // [rows][R] Witt function at the roots of fl
+	double2* f1 = nullptr;   // [rows][R] its z-derivative
+	double2* ex = nullptr;   // [rows][R] terms of the Aberth sum formed per root
+	int* tile_off = nullptr; // [rows + 1] first tile of each row
+	int* plan = nullptr;     // [4] tiles, slices per tile
+	double2* part = nullptr; // [max_items][tile] partial Cauchy sums
+};
+
 } // namespace ccf
 
 struct ccf_ctx
 {
-	ccf_params P{};
+	ccf_params P_in{}; // the caller's fields; every run starts from them
+	ccf_params P{};    // the fields as run() leaves them (derived values assigned)
 	ccf_info info{};
 	std::string last_error;
 
@@ -105,7 +136,7 @@ struct ccf_ctx
 	bool from_file = false;
 
 	// stage completion
-	bool prepared = false, have_roots = false, have_ccs = false, have_caustics = false, have_mu = false;
+	bool loaded = false, prepared = false, have_roots = false, have_ccs = false, have_caustics = false, have_mu = false;
 
 	// host copies
 	std::vector<double> stars_in;   // {x1, x2, m} in generation/file order
@@ -113,6 +144,7 @@ struct ccf_ctx
 
 	// device state
 	hipStream_t stream = nullptr;
+	int num_cus = 0;
 	ccf::DeviceTree tree;
 	double2* spos = nullptr;      // tree order
 	double* smass = nullptr;
@@ -128,12 +160,8 @@ struct ccf_ctx
 	double* mu_scales = nullptr;
 	// Aberth work buffers
 	int max_rows = 0;
-	int* act = nullptr;           // [2][max_rows][R] active root indices
-	int* act_cnt = nullptr;       // [2][max_rows]
-	double2* nv = nullptr;        // [max_rows][R] updated values (compacted)
-	unsigned char* conv = nullptr;// [max_rows][R]
-	int* rows_dev = nullptr;      // [max_rows] phi-row index of each solved row
-	double* phis_dev = nullptr;   // [max_rows] phi of each solved row
+	ccf::SweepBufs sw;
+	double* ones = nullptr;       // unit masses for the pole-sum expansions
 	unsigned long long* counters = nullptr; // device statistics, see solve.hip
 	std::vector<ccf::Timer> timers;
 
@@ -151,6 +179,8 @@ namespace ccf
 // tree.hip
 bool build_tree(ccf_ctx* ctx, int verbose);
 // solve.hip
+bool alloc_sweep(ccf_ctx* ctx, SweepBufs& b, int rows, int R);
+void free_sweep(SweepBufs& b);
 bool alloc_solver(ccf_ctx* ctx);
 bool find_initial_roots_dev(ccf_ctx* ctx, int verbose);
 bool find_ccs_dev(ccf_ctx* ctx, int verbose);

@@ -19,32 +19,61 @@ namespace ccf
 __device__ inline int nearest_node(const LensView& L, cd z)
 {
 	int n = 0;
-	while (L.child[n] >= 0)
+	double4 b = L.nbox[0];
+	while (b.w >= 0)
 	{
-		const double2 c = L.center[n];
-		const double h = L.hl[n];
-		if (fabs(z.re - c.x) > h || fabs(z.im - c.y) > h) return n;
+		if (fabs(z.re - b.x) > b.z || fabs(z.im - b.y) > b.z) return n;
 		int q;
-		if (z.re >= c.x && z.im >= c.y) q = 0;
-		else if (z.re < c.x && z.im >= c.y) q = 1;
-		else if (z.re < c.x && z.im < c.y) q = 2;
+		if (z.re >= b.x && z.im >= b.y) q = 0;
+		else if (z.re < b.x && z.im >= b.y) q = 1;
+		else if (z.re < b.x && z.im < b.y) q = 2;
 		else q = 3;
-		n = L.child[n] + q;
+		n = (int)b.w + q;
+		b = L.nbox[n];
 	}
 	return n;
 }
 
 // near-field sums over a node and its neighbours:
-// s1 = sum m/(z-s), s2 = sum m/(z-s)^2, s3 = sum m/(z-s)^3
+// s1 = sum m/(z-s), s2 = sum m/(z-s)^2, s3 = sum m/(z-s)^3, s0 = sum 1/(z-s)
 struct Near
 {
-	cd s1, s2, s3;
+	cd s0, s1, s2, s3;
 	int count = 0;
 };
+
+__device__ inline void p2p_add(Near& r, cd z, double4 s)
+{
+	const cd t = cd(z.re - s.x, z.im - s.y).inv();
+	r.s0 += t;
+	const cd t1 = t * s.z;
+	r.s1 += t1;
+	const cd t2 = t1 * t;
+	r.s2 += t2;
+	r.s3 += t2 * t;
+}
 
 __device__ inline Near p2p(const LensView& L, int node, cd z)
 {
 	Near r;
+	const int off = L.nf_off[node];
+	if (off >= 0)
+	{
+		// a leaf: one contiguous run of at most ~32 stars, two at a time
+		const int n = L.nf_cnt[node];
+		const double4* __restrict__ p = L.nfs + off;
+		int t = 0;
+		for (; t + 1 < n; t += 2)
+		{
+			const double4 a = p[t], b = p[t + 1];
+			p2p_add(r, z, a);
+			p2p_add(r, z, b);
+		}
+		if (t < n) p2p_add(r, z, p[t]);
+		r.count = n;
+		return r;
+	}
+	// the root (positions outside the tree) or a box edge case: star ranges
 	const int nr = L.nnf[node];
 	for (int k = 0; k < nr; k++)
 	{
@@ -53,6 +82,7 @@ __device__ inline Near p2p(const LensView& L, int node, cd z)
 		{
 			const double2 p = L.spos[s];
 			const cd t = cd(z.re - p.x, z.im - p.y).inv();
+			r.s0 += t;
 			const cd t1 = t * L.smass[s];
 			r.s1 += t1;
 			const cd t2 = t1 * t;
@@ -91,6 +121,26 @@ __device__ inline Far l2p(const LensView& L, int node, cd z)
 	return f;
 }
 
+// far part of sum 1/(z - s_i) over all stars: first derivative of the
+// unit-mass local expansion, evaluated in FP32 (its error only changes the
+// Aberth correction, not the root it converges to)
+__device__ inline cd far_pole32(const LensView& L, int node, cd z)
+{
+	if (L.order == 0) return cd();
+	const double2 c = L.center[node];
+	const double hl = L.hl[node];
+	const float ih = (float)(1.0 / hl);
+	const cf dz((float)(z.re - c.x) * ih, (float)(z.im - c.y) * ih);
+	const float2* a = L.locu32 + (size_t)node * (L.order + 1);
+	cf h;
+	for (int i = L.order; i >= 1; i--)
+	{
+		const float2 ci = a[i];
+		h = h * dz + cf(ci.x * i, ci.y * i);
+	}
+	return cd(h.re * (double)ih, h.im * (double)ih);
+}
+
 __device__ inline double boxcar(cd z, double cre, double cim)
 {
 	return (-cre <= z.re && z.re <= cre && -cim <= z.im && z.im <= cim) ? 1.0 : 0.0;
@@ -100,7 +150,7 @@ __device__ inline double boxcar(cd z, double cre, double cim)
 __device__ inline double smooth_dz(const LensView& L, cd z)
 {
 	if (L.rectangular) return L.approx ? -L.kappastar : -L.kappastar * boxcar(z, L.corner_re, L.corner_im);
-	return L.approx ? -L.kappastar : -L.kappastar * (L.corner_abs2 >= z.norm2() ? 1.0 : 0.0);
+	return L.approx ? -L.kappastar : -L.kappastar * (L.corner_abs - z.abs() >= 0 ? 1.0 : 0.0);
 }
 
 // d alpha_smooth / d zbar and d^2 alpha_smooth / d zbar^2 (alpha_smooth.cuh:185-301)
@@ -177,7 +227,7 @@ __device__ inline cd smooth_alpha(const LensView& L, cd z)
 		a -= cd(ks * 4 * c.re * hv);
 		return a;
 	}
-	if (L.approx || z.norm2() <= L.corner_abs2) return -ks * z;
+	if (L.approx || z.abs() <= L.corner_abs) return -ks * z;
 	return -ks * L.corner_abs2 * zb.inv();
 }
 

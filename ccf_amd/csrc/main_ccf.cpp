@@ -44,7 +44,7 @@ const std::string OPTS[] = {
 	"-wls", "--write_length_scales",
 	"-o", "--outfile_prefix",
 	"--fmm_order", "--direct", "--binomial_exact", "--precision",
-	"--init_iters", "--step_iters", "--roots_per_thread",
+	"--init_iters", "--step_iters", "--roots_per_thread", "--predictor",
 };
 
 int verbose = 1;
@@ -134,7 +134,10 @@ void usage(const char* name)
 		<< "  --init_iters               Cap of Aberth sweeps at the branch centres.\n"
 		<< "                             Default value: 60\n"
 		<< "  --step_iters               Cap of Aberth sweeps per phi step. Default value: 20\n"
-		<< "  --roots_per_thread         Aberth tile roots per thread (1, 2, 4; 0 automatic).\n";
+		<< "  --roots_per_thread         Aberth tile roots per thread (1, 2, 4; 0 automatic).\n"
+		<< "  --predictor                Initial roots of each phi step: 0 the previous\n"
+		<< "                             step's roots; 1 linear, 2 quadratic extrapolation.\n"
+		<< "                             Default value: 0\n";
 }
 
 template <typename V, typename F>
@@ -265,6 +268,7 @@ int main(int argc, char* argv[])
 		else if (o == "--init_iters") ok = parse("init_iters", in, ccf.init_iters, to_i);
 		else if (o == "--step_iters") ok = parse("step_iters", in, ccf.step_iters, to_i);
 		else if (o == "--roots_per_thread") ok = parse("roots_per_thread", in, ccf.roots_per_thread, to_i);
+		else if (o == "--predictor") ok = parse("predictor", in, ccf.predictor, to_i);
 		if (!ok) return -1;
 	}
 	if (!has_sf && !has_ks && (exists(b, e, "-s") || exists(b, e, "--smooth_fraction")))

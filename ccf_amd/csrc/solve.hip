@@ -1,19 +1,30 @@
 // Aberth-Ehrlich critical-curve solve, phi continuation, 1/mu error bound,
 // transpose and caustic map on gfx950.
 //
-// Work decomposition (weisluke/ccf runs a (root, side, branch) grid of
-// single-thread updates that write roots in place, ccf_functions.cuh:321-394):
-//  - a "row" is one phi value of one branch side; every sweep updates the
-//    still-unconverged roots of all rows of a phase in one launch;
-//  - sweeps are Jacobi: the update kernel reads the row as left by the
-//    previous sweep and writes new values to a compacted side buffer; a
-//    commit kernel scatters them back and appends the roots that have not
-//    converged to the next sweep's active list, so converged roots cost no
-//    lanes (the reference re-tests a fin[] flag in every launch);
-//  - one workgroup owns a tile of active roots of one row and streams the
-//    row's roots and all stars through LDS in chunks; every lane reads the
-//    same LDS word (broadcast) and keeps RPT roots in registers, so each LDS
-//    read feeds RPT complex reciprocal-accumulates.
+// weisluke/ccf runs a (root, side, branch) grid of single-thread updates that
+// write the roots in place (ccf_functions.cuh:321-394) and launches 20 of
+// them per phi step whatever the convergence state (ccf.cuh:1013-1018).
+// Here a "row" is one phi value of one branch side and a sweep is one Jacobi
+// update of every still-active root of every row of a phase, in four
+// launches:
+//   K1 k_witt      per active root: Witt's function f0, its derivative f1
+//                  (near-field P2P + far-field L2P + smooth sheet) and the
+//                  1/mu early exit; roots that need the Cauchy sums are
+//                  appended to a per-row list;
+//   K2 k_plan      cuts the (stars + roots) term range of every tile of
+//                  RPT*256 listed roots into S slices so that dense and sparse
+//                  sweeps alike give enough equal work items to fill the chip;
+//   K3 k_sums      one work item = one tile x one slice: the slice's terms are
+//                  streamed through LDS and every lane accumulates
+//                  2 sum 1/(z - s_i) - sum 1/(z - z_j) for its RPT roots;
+//                  partials go to a buffer so the result is deterministic;
+//   K4 k_finalize  reduces the partials, forms z - f0 / (f1 + f0 S), tests
+//                  |dz| / theta < 1e-9, writes the root and appends the
+//                  unconverged ones to the next sweep's active list.
+// K3 exists in FP64 (the reference's arithmetic) and in FP32 with a per-chunk
+// fallback: a lane whose nearest root in an LDS chunk is closer than
+// CLUSTER_REL times the field's half-length (where an FP32 difference of two
+// positions keeps fewer than ~12 bits) redoes that chunk in FP64.
 #include "lens.hpp"
 
 #include <hiprand/hiprand_kernel.h>
@@ -38,6 +49,7 @@ enum Counter
 	C_CONV = 3,    // roots converged
 	C_ERRBITS = 4, // max 1/mu error (bits of a non-negative double)
 	C_NAN = 5,     // non-finite error seen
+	C_CLUSTER = 6, // FP32 chunks redone in FP64
 	C_COUNT = 8
 };
 
@@ -45,7 +57,11 @@ namespace
 {
 
 constexpr int BLOCK = 256;
-constexpr int CHUNK = 1024; // complex values per LDS chunk (16 KiB in FP64)
+constexpr int CHUNK64 = 1024;         // FP64 complex values per LDS chunk (16 KiB)
+constexpr int CHUNK32 = 2048;         // FP32 complex values per LDS chunk (16 KiB)
+constexpr int MAX_SLICES = 64;
+constexpr int MIN_SLICE = 1024;       // fewest terms per slice
+constexpr double CLUSTER_REL = 2e-4;  // root separation, relative to the field, below which FP32 terms are redone in FP64
 
 inline int grid(long long n, int b = BLOCK) { return (int)((n + b - 1) / b); }
 
@@ -53,6 +69,22 @@ __device__ inline unsigned long long wave_sum(unsigned long long v)
 {
 	for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
 	return v;
+}
+
+// Append to a list whose length is *counter: every lane of the wave calls
+// this; lanes with pred get consecutive slots from a single atomic per
+// wave (per-lane atomics on one address serialise at the L2).
+__device__ inline int wave_append(int* counter, bool pred)
+{
+	const unsigned long long m = __ballot(pred);
+	if (m == 0) return -1;
+	const int lane = threadIdx.x & 63;
+	const int leader = __ffsll((unsigned long long)m) - 1;
+	int base = 0;
+	if (lane == leader) base = atomicAdd(counter, (int)__popcll(m));
+	base = __shfl(base, leader, 64);
+	const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+	return pred ? base + (int)__popcll(m & below) : -1;
 }
 
 // 1/x for x > 0: hardware reciprocal estimate refined by two Newton steps
@@ -91,123 +123,105 @@ struct RowMap
 	}
 };
 
-// One Aberth update of every active root of every row (ccf_functions.cuh:193-243).
-// grid: (tiles, rows); block: BLOCK threads, RPT roots per thread.
-template <int RPT>
-__global__ __launch_bounds__(BLOCK) void k_aberth(LensView L, const double2* __restrict__ roots, RowMap M,
-	const int* __restrict__ act, const int* __restrict__ cnt, int* __restrict__ cnt_other,
-	double2* __restrict__ nv, unsigned char* __restrict__ conv, unsigned long long* __restrict__ stats, int circ_exact)
+// device pointers of a SweepBufs, with the parity of the current sweep applied
+struct SweepView
 {
-	__shared__ double2 tile[CHUNK];
+	int R, rows, tile;
+	const int* act;
+	int* act_next;
+	const int* cnt;
+	int* cnt_next;
+	int* fl;
+	int* fcnt;
+	int* fcnt_next;
+	double2* f0;
+	double2* f1;
+	double2* ex;
+	int* tile_off;
+	int* plan;
+	double2* part;
+};
+
+SweepView view(const SweepBufs& B, int parity)
+{
+	SweepView v;
+	v.R = B.R;
+	v.rows = B.rows;
+	v.tile = B.tile;
+	const size_t list = (size_t)B.rows * B.R;
+	v.act = B.act + parity * list;
+	v.act_next = B.act + (parity ^ 1) * list;
+	v.cnt = B.cnt + parity * B.rows;
+	v.cnt_next = B.cnt + (parity ^ 1) * B.rows;
+	v.fl = B.fl;
+	v.fcnt = B.fcnt + parity * B.rows;
+	v.fcnt_next = B.fcnt + (parity ^ 1) * B.rows;
+	v.f0 = B.f0;
+	v.f1 = B.f1;
+	v.ex = B.ex;
+	v.tile_off = B.tile_off;
+	v.plan = B.plan;
+	v.part = B.part;
+	return v;
+}
+
+// K1 (ccf_functions.cuh:197-213 and the early return at :207-211)
+// MIXED: the mixed-precision form, whose early-exit roots also take one FP64
+// Newton step z - f0/f1 (a final polish; the reference keeps z as it is)
+template <bool MIXED>
+__global__ __launch_bounds__(BLOCK) void k_witt(LensView L, double2* __restrict__ roots, RowMap M, SweepView V,
+	unsigned long long* __restrict__ stats, int circ_exact, int fmm_pole)
+{
 	const int r = blockIdx.y;
 	const int R = M.R;
-	if (blockIdx.x == 0 && threadIdx.x == 0) cnt_other[r] = 0; // next sweep's list is refilled by k_commit
-	const int n = cnt[r];
-	const int base = blockIdx.x * (BLOCK * RPT);
-	if (base >= n) return;
-	int row;
-	double phi;
-	M.get(r, row, phi);
-	const double2* __restrict__ xrow = roots + (size_t)row * R;
-
-	cd z[RPT], f0[RPT], f1[RPT];
-	bool live[RPT], full[RPT];
+	if (blockIdx.x == 0 && threadIdx.x == 0)
+	{
+		V.cnt_next[r] = 0;
+		V.fcnt_next[r] = 0;
+	}
+	const int n = V.cnt[r];
+	const int i = blockIdx.x * BLOCK + threadIdx.x;
 	unsigned long long nfull = 0, nearly = 0, nnear = 0;
-#pragma unroll
-	for (int k = 0; k < RPT; k++)
+	bool full = false;
+	int a = 0;
+	cd f0, f1, ex;
+	if (i < n)
 	{
-		const int i = base + k * BLOCK + threadIdx.x;
-		live[k] = i < n;
-		full[k] = false;
-		const int a = live[k] ? act[(size_t)r * R + i] : 0;
-		const double2 v = xrow[a];
-		z[k] = cd(v.x, v.y);
-		if (live[k])
+		int row;
+		double phi;
+		M.get(r, row, phi);
+		a = V.act[(size_t)r * R + i];
+		double2* p = roots + (size_t)row * R + a;
+		const cd z(p->x, p->y);
+		const Witt w = witt_f0(L, z, phi);
+		if (w.early)
 		{
-			Witt w = witt_f0(L, z[k], phi);
-			f0[k] = w.f0;
-			full[k] = !w.early;
-			if (full[k])
+			nearly = 1;
+			if (MIXED)
 			{
-				f1[k] = witt_f1(L, z[k], w);
-				nfull++;
-				nnear += w.nr.count;
-			}
-			else
-			{
-				nearly++;
+				const cd dz = w.f0 / witt_f1(L, z, w);
+				if (dz.abs() < 1e-6 * L.theta) *p = make_double2(z.re - dz.re, z.im - dz.im);
 			}
 		}
-	}
-
-	// 2 sum 1/(z - s_i) over all stars (ccf_functions.cuh:218-227)
-	double ax[RPT], ay[RPT];
-#pragma unroll
-	for (int k = 0; k < RPT; k++) { ax[k] = 0; ay[k] = 0; }
-	for (int c0 = 0; c0 < L.nstars; c0 += CHUNK)
-	{
-		const int m = min(CHUNK, L.nstars - c0);
-		for (int t = threadIdx.x; t < m; t += BLOCK) tile[t] = L.spos[c0 + t];
-		__syncthreads();
-#pragma unroll 4
-		for (int j = 0; j < m; j++)
+		else
 		{
-			const double2 w = tile[j];
-#pragma unroll
-			for (int k = 0; k < RPT; k++)
-			{
-				const double dx = z[k].re - w.x, dy = z[k].im - w.y;
-				const double q = rcp64(fma(dx, dx, dy * dy));
-				ax[k] = fma(dx, q, ax[k]);
-				ay[k] = fma(dy, q, ay[k]);
-			}
+			f0 = w.f0;
+			f1 = witt_f1(L, z, w);
+			if (circ_exact) ex = 2.0 * z.inv(); // the disc's pole at 0 (ccf_functions.cuh:223-227)
+			if (MIXED && fmm_pole) ex += 2.0 * (w.nr.s0 + far_pole32(L, w.node, z));
+			full = true;
+			nfull = 1;
+			nnear = w.nr.count;
 		}
-		__syncthreads();
 	}
-	cd S[RPT];
-#pragma unroll
-	for (int k = 0; k < RPT; k++)
+	const int pos = wave_append(&V.fcnt[r], full);
+	if (full)
 	{
-		S[k] = cd(2 * ax[k], -2 * ay[k]);
-		if (circ_exact) S[k] += z[k].inv();
-		ax[k] = 0;
-		ay[k] = 0;
-	}
-	// sum over the other roots of the row (ccf_functions.cuh:232-239); the
-	// self term has dx = dy = 0 exactly and the floor on |d|^2 makes it add 0
-	for (int c0 = 0; c0 < R; c0 += CHUNK)
-	{
-		const int m = min(CHUNK, R - c0);
-		for (int t = threadIdx.x; t < m; t += BLOCK) tile[t] = xrow[c0 + t];
-		__syncthreads();
-#pragma unroll 4
-		for (int j = 0; j < m; j++)
-		{
-			const double2 w = tile[j];
-#pragma unroll
-			for (int k = 0; k < RPT; k++)
-			{
-				const double dx = z[k].re - w.x, dy = z[k].im - w.y;
-				const double q = rcp64(fmax(fma(dx, dx, dy * dy), 1e-300));
-				ax[k] = fma(dx, q, ax[k]);
-				ay[k] = fma(dy, q, ay[k]);
-			}
-		}
-		__syncthreads();
-	}
-#pragma unroll
-	for (int k = 0; k < RPT; k++)
-	{
-		if (!live[k]) continue;
-		const int i = base + k * BLOCK + threadIdx.x;
-		cd zn = z[k];
-		if (full[k])
-		{
-			const cd s = S[k] - cd(ax[k], -ay[k]);
-			zn = z[k] - f0[k] / (f1[k] + f0[k] * s);
-		}
-		nv[(size_t)r * R + i] = make_double2(zn.re, zn.im);
-		conv[(size_t)r * R + i] = ((zn - z[k]).abs() / L.theta) < 1e-9 ? 1 : 0;
+		const size_t k = (size_t)r * R + pos;
+		V.fl[k] = a;
+		V.f0[k] = make_double2(f0.re, f0.im);
+		V.f1[k] = make_double2(f1.re, f1.im);
+		V.ex[k] = make_double2(ex.re, ex.im);
 	}
 	nfull = wave_sum(nfull);
 	nearly = wave_sum(nearly);
@@ -215,38 +229,329 @@ __global__ __launch_bounds__(BLOCK) void k_aberth(LensView L, const double2* __r
 	if ((threadIdx.x & 63) == 0)
 	{
 		if (nfull) atomicAdd(&stats[C_FULL], nfull);
-		if (nearly) atomicAdd(&stats[C_EARLY], nearly);
+		if (nearly)
+		{
+			atomicAdd(&stats[C_EARLY], nearly);
+			atomicAdd(&stats[C_CONV], nearly);
+		}
 		if (nnear) atomicAdd(&stats[C_NEAR], nnear);
 	}
 }
 
-// scatter the sweep's updates into the rows and build the next active lists
-__global__ void k_commit(double2* __restrict__ roots, RowMap M, const int* __restrict__ act, const int* __restrict__ cnt,
-	int* __restrict__ act_next, int* __restrict__ cnt_next, const double2* __restrict__ nv,
-	const unsigned char* __restrict__ conv, unsigned long long* __restrict__ stats)
+// K2: tiles per row (exclusive prefix in tile_off) and slices per tile
+__global__ void k_plan(SweepView V, int nrows, int terms, int target)
+{
+	__shared__ int wsum[16];
+	__shared__ int carry;
+	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+	if (threadIdx.x == 0) carry = 0;
+	__syncthreads();
+	for (int base = 0; base < nrows; base += blockDim.x)
+	{
+		const int r = base + threadIdx.x;
+		const int v = r < nrows ? (V.fcnt[r] + V.tile - 1) / V.tile : 0;
+		int x = v;
+		for (int d = 1; d < 64; d <<= 1)
+		{
+			const int y = __shfl_up(x, d, 64);
+			if (lane >= d) x += y;
+		}
+		if (lane == 63) wsum[w] = x;
+		__syncthreads();
+		int woff = 0, tot = 0;
+		for (int k = 0; k < nw; k++)
+		{
+			if (k < w) woff += wsum[k];
+			tot += wsum[k];
+		}
+		if (r < nrows) V.tile_off[r] = carry + woff + x - v;
+		__syncthreads();
+		if (threadIdx.x == 0) carry += tot;
+		__syncthreads();
+	}
+	if (threadIdx.x == 0)
+	{
+		const int tiles = carry;
+		int S = 1;
+		if (tiles > 0 && tiles < target)
+		{
+			S = (target + tiles - 1) / tiles;
+			S = min(S, MAX_SLICES);
+			S = min(S, max(1, terms / MIN_SLICE));
+		}
+		V.tile_off[nrows] = tiles;
+		V.plan[0] = tiles;
+		V.plan[1] = S;
+	}
+}
+
+// row of global tile g: the largest r with tile_off[r] <= g (rows without
+// tiles have tile_off[r] == tile_off[r + 1] and are skipped)
+__device__ inline int row_of_tile(const int* __restrict__ tile_off, int nrows, int g)
+{
+	int lo = 0, hi = nrows;
+	while (hi - lo > 1)
+	{
+		const int mid = (lo + hi) >> 1;
+		if (tile_off[mid] <= g) lo = mid;
+		else hi = mid;
+	}
+	return lo;
+}
+
+// K3 in FP64: partial 2 sum_stars 1/(z - s) - sum_roots 1/(z - w) over one slice
+template <int RPT>
+__global__ __launch_bounds__(BLOCK) void k_sums64(const double2* __restrict__ spos, int Np, const double2* __restrict__ roots,
+	RowMap M, SweepView V, int nrows)
+{
+	__shared__ double2 buf[CHUNK64];
+	const int tiles = V.plan[0], S = V.plan[1];
+	const long long items = (long long)tiles * S;
+	const int R = M.R, T = BLOCK * RPT;
+	const long long Lt = (long long)Np + R;
+	for (long long item = blockIdx.x; item < items; item += gridDim.x)
+	{
+		const int g = (int)(item / S), s = (int)(item % S);
+		const int r = row_of_tile(V.tile_off, nrows, g);
+		const int t = g - V.tile_off[r];
+		const int nfr = V.fcnt[r];
+		int row;
+		double phi;
+		M.get(r, row, phi);
+		const double2* __restrict__ xrow = roots + (size_t)row * R;
+		double zx[RPT], zy[RPT], ax[RPT], ay[RPT], bx[RPT], by[RPT];
+#pragma unroll
+		for (int k = 0; k < RPT; k++)
+		{
+			const int i = t * T + k * BLOCK + threadIdx.x;
+			const int a = V.fl[(size_t)r * R + (i < nfr ? i : t * T)];
+			const double2 v = xrow[a];
+			zx[k] = v.x;
+			zy[k] = v.y;
+			ax[k] = ay[k] = bx[k] = by[k] = 0;
+		}
+		const long long lo = s * Lt / S, hi = (s + 1) * Lt / S;
+		// star terms (ccf_functions.cuh:218-227)
+		const int s1 = (int)min(hi, (long long)Np);
+		for (int c0 = (int)lo; c0 < s1; c0 += CHUNK64)
+		{
+			const int m = min(CHUNK64, s1 - c0);
+			for (int q = threadIdx.x; q < m; q += BLOCK) buf[q] = spos[c0 + q];
+			__syncthreads();
+#pragma unroll 4
+			for (int q = 0; q < m; q++)
+			{
+				const double2 w = buf[q];
+#pragma unroll
+				for (int k = 0; k < RPT; k++)
+				{
+					const double dx = zx[k] - w.x, dy = zy[k] - w.y;
+					const double iv = rcp64(fma(dx, dx, dy * dy));
+					ax[k] = fma(dx, iv, ax[k]);
+					ay[k] = fma(dy, iv, ay[k]);
+				}
+			}
+			__syncthreads();
+		}
+		// root terms (ccf_functions.cuh:232-239); the own root gives dx = dy = 0
+		// exactly and the floor on |d|^2 turns its term into 0
+		const int r0 = (int)(max(lo, (long long)Np) - Np), r1 = (int)(hi - Np);
+		for (int c0 = r0; c0 < r1; c0 += CHUNK64)
+		{
+			const int m = min(CHUNK64, r1 - c0);
+			for (int q = threadIdx.x; q < m; q += BLOCK) buf[q] = xrow[c0 + q];
+			__syncthreads();
+#pragma unroll 4
+			for (int q = 0; q < m; q++)
+			{
+				const double2 w = buf[q];
+#pragma unroll
+				for (int k = 0; k < RPT; k++)
+				{
+					const double dx = zx[k] - w.x, dy = zy[k] - w.y;
+					const double iv = rcp64(fmax(fma(dx, dx, dy * dy), 1e-300));
+					bx[k] = fma(dx, iv, bx[k]);
+					by[k] = fma(dy, iv, by[k]);
+				}
+			}
+			__syncthreads();
+		}
+#pragma unroll
+		for (int k = 0; k < RPT; k++)
+			V.part[(size_t)item * T + k * BLOCK + threadIdx.x] = make_double2(2 * ax[k] - bx[k], -(2 * ay[k] - by[k]));
+	}
+}
+
+// K3 in FP32 with the FP64 fallback for clustered roots
+template <int RPT>
+__global__ __launch_bounds__(BLOCK) void k_sums32(const float2* __restrict__ spos32, int Np, const double2* __restrict__ roots,
+	RowMap M, SweepView V, int nrows, float cluster_d2, unsigned long long* __restrict__ stats)
+{
+	__shared__ float2 buf[CHUNK32];
+	const int tiles = V.plan[0], S = V.plan[1];
+	const long long items = (long long)tiles * S;
+	const int R = M.R, T = BLOCK * RPT;
+	const long long Lt = (long long)Np + R;
+	unsigned long long nclus = 0;
+	for (long long item = blockIdx.x; item < items; item += gridDim.x)
+	{
+		const int g = (int)(item / S), s = (int)(item % S);
+		const int r = row_of_tile(V.tile_off, nrows, g);
+		const int t = g - V.tile_off[r];
+		const int nfr = V.fcnt[r];
+		int row;
+		double phi;
+		M.get(r, row, phi);
+		const double2* __restrict__ xrow = roots + (size_t)row * R;
+		double zx[RPT], zy[RPT], bx[RPT], by[RPT];
+		float fx[RPT], fy[RPT], ax[RPT], ay[RPT];
+#pragma unroll
+		for (int k = 0; k < RPT; k++)
+		{
+			const int i = t * T + k * BLOCK + threadIdx.x;
+			const int a = V.fl[(size_t)r * R + (i < nfr ? i : t * T)];
+			const double2 v = xrow[a];
+			zx[k] = v.x;
+			zy[k] = v.y;
+			fx[k] = (float)v.x;
+			fy[k] = (float)v.y;
+			ax[k] = ay[k] = 0;
+			bx[k] = by[k] = 0;
+		}
+		const long long lo = s * Lt / S, hi = (s + 1) * Lt / S;
+		const int s1 = (int)min(hi, (long long)Np);
+		for (int c0 = (int)lo; c0 < s1; c0 += CHUNK32)
+		{
+			const int m = min(CHUNK32, s1 - c0);
+			for (int q = threadIdx.x; q < m; q += BLOCK) buf[q] = spos32[c0 + q];
+			__syncthreads();
+#pragma unroll 4
+			for (int q = 0; q < m; q++)
+			{
+				const float2 w = buf[q];
+#pragma unroll
+				for (int k = 0; k < RPT; k++)
+				{
+					const float dx = fx[k] - w.x, dy = fy[k] - w.y;
+					const float iv = __builtin_amdgcn_rcpf(fmaf(dx, dx, dy * dy));
+					ax[k] = fmaf(dx, iv, ax[k]);
+					ay[k] = fmaf(dy, iv, ay[k]);
+				}
+			}
+			__syncthreads();
+		}
+		const int r0 = (int)(max(lo, (long long)Np) - Np), r1 = (int)(hi - Np);
+		for (int c0 = r0; c0 < r1; c0 += CHUNK32)
+		{
+			const int m = min(CHUNK32, r1 - c0);
+			for (int q = threadIdx.x; q < m; q += BLOCK)
+			{
+				const double2 w = xrow[c0 + q];
+				buf[q] = make_float2((float)w.x, (float)w.y);
+			}
+			__syncthreads();
+			float cx[RPT], cy[RPT], dmin[RPT];
+#pragma unroll
+			for (int k = 0; k < RPT; k++) { cx[k] = cy[k] = 0; dmin[k] = 3.0e38f; }
+#pragma unroll 4
+			for (int q = 0; q < m; q++)
+			{
+				const float2 w = buf[q];
+#pragma unroll
+				for (int k = 0; k < RPT; k++)
+				{
+					const float dx = fx[k] - w.x, dy = fy[k] - w.y;
+					const float d2 = fmaxf(fmaf(dx, dx, dy * dy), 1e-30f);
+					const float iv = __builtin_amdgcn_rcpf(d2);
+					cx[k] = fmaf(dx, iv, cx[k]);
+					cy[k] = fmaf(dy, iv, cy[k]);
+					// the own root (d2 at the floor) is not a neighbour
+					dmin[k] = d2 > 1e-30f ? fminf(dmin[k], d2) : dmin[k];
+				}
+			}
+#pragma unroll
+			for (int k = 0; k < RPT; k++)
+			{
+				if (dmin[k] < cluster_d2)
+				{
+					// a root of this chunk is close: redo the chunk in FP64
+					double ex = 0, ey = 0;
+					for (int q = 0; q < m; q++)
+					{
+						const double2 w = xrow[c0 + q];
+						const double dx = zx[k] - w.x, dy = zy[k] - w.y;
+						const double iv = rcp64(fmax(fma(dx, dx, dy * dy), 1e-300));
+						ex = fma(dx, iv, ex);
+						ey = fma(dy, iv, ey);
+					}
+					bx[k] += ex;
+					by[k] += ey;
+					nclus++;
+				}
+				else
+				{
+					bx[k] += cx[k];
+					by[k] += cy[k];
+				}
+			}
+			__syncthreads();
+		}
+#pragma unroll
+		for (int k = 0; k < RPT; k++)
+			V.part[(size_t)item * T + k * BLOCK + threadIdx.x] =
+				make_double2(2.0 * ax[k] - bx[k], -(2.0 * ay[k] - by[k]));
+	}
+	nclus = wave_sum(nclus);
+	if ((threadIdx.x & 63) == 0 && nclus) atomicAdd(&stats[C_CLUSTER], nclus);
+}
+
+// K4 (ccf_functions.cuh:241-242 and 379-389)
+__global__ __launch_bounds__(BLOCK) void k_finalize(double theta, double2* __restrict__ roots, RowMap M, SweepView V,
+	unsigned long long* __restrict__ stats)
 {
 	const int r = blockIdx.y;
 	const int R = M.R;
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	const int n = cnt[r];
+	const int i = blockIdx.x * BLOCK + threadIdx.x;
+	const int n = V.fcnt[r];
 	unsigned long long nc = 0;
+	bool again = false;
+	int a = 0;
 	if (i < n)
 	{
 		int row;
 		double phi;
 		M.get(r, row, phi);
-		const int a = act[(size_t)r * R + i];
-		roots[(size_t)row * R + a] = nv[(size_t)r * R + i];
-		if (conv[(size_t)r * R + i]) nc = 1;
-		else act_next[(size_t)r * R + atomicAdd(&cnt_next[r], 1)] = a;
+		const size_t k = (size_t)r * R + i;
+		a = V.fl[k];
+		double2* p = roots + (size_t)row * R + a;
+		const cd z(p->x, p->y);
+		const int S = V.plan[1];
+		const size_t g = (size_t)(V.tile_off[r] + i / V.tile);
+		const int loc = i % V.tile;
+		double2 e = V.ex[k];
+		cd sum(e.x, e.y);
+		for (int s = 0; s < S; s++)
+		{
+			const double2 q = V.part[(g * S + s) * V.tile + loc];
+			sum += cd(q.x, q.y);
+		}
+		const double2 a0 = V.f0[k], a1 = V.f1[k];
+		const cd f0(a0.x, a0.y), f1(a1.x, a1.y);
+		const cd zn = z - f0 / (f1 + f0 * sum);
+		*p = make_double2(zn.re, zn.im);
+		if ((zn - z).abs() / theta < 1e-9) nc = 1;
+		else again = true;
 	}
+	const int pos = wave_append(&V.cnt_next[r], again);
+	if (again) V.act_next[(size_t)r * R + pos] = a;
 	nc = wave_sum(nc);
 	if ((threadIdx.x & 63) == 0 && nc) atomicAdd(&stats[C_CONV], nc);
 }
 
-// copy row centre+sgn*(j-1) to centre+sgn*j and make every root active
-// (prepare_roots_kernel, ccf_functions.cuh:261-295)
-__global__ void k_prepare(double2* roots, RowMap M, int* act, int* cnt)
+// initial roots of a phase: row centre+sgn*j from the rows of the previous
+// steps (prepare_roots_kernel, ccf_functions.cuh:261-295; order > 0
+// extrapolates), every root active, the sweep counters cleared
+__global__ void k_prepare(double2* roots, RowMap M, SweepBufs B, int order)
 {
 	const int r = blockIdx.y;
 	const int R = M.R;
@@ -259,11 +564,33 @@ __global__ void k_prepare(double2* roots, RowMap M, int* act, int* cnt)
 		if (M.j > 0)
 		{
 			const int sgn = (r % M.rows_per_branch) == 0 ? -1 : 1;
-			roots[(size_t)row * R + a] = roots[(size_t)(row - sgn) * R + a];
+			const int q = min(order, M.j - 1); // previous steps beyond j - 1 that exist
+			const double2 z1 = roots[(size_t)(row - sgn) * R + a];
+			double2 z = z1;
+			if (q >= 1)
+			{
+				const double2 z2 = roots[(size_t)(row - 2 * sgn) * R + a];
+				if (q >= 2)
+				{
+					const double2 z3 = roots[(size_t)(row - 3 * sgn) * R + a];
+					z = make_double2(3 * (z1.x - z2.x) + z3.x, 3 * (z1.y - z2.y) + z3.y);
+				}
+				else
+				{
+					z = make_double2(2 * z1.x - z2.x, 2 * z1.y - z2.y);
+				}
+			}
+			roots[(size_t)row * R + a] = z;
 		}
-		act[(size_t)r * R + a] = a;
+		B.act[(size_t)r * R + a] = a;
 	}
-	if (a == 0) cnt[r] = R;
+	if (a == 0)
+	{
+		B.cnt[r] = R;
+		B.cnt[B.rows + r] = 0;
+		B.fcnt[r] = 0;
+		B.fcnt[B.rows + r] = 0;
+	}
 }
 
 // 1/mu error bound at every point of the solved rows, reduced to its max
@@ -375,62 +702,124 @@ __global__ void k_generate_stars(int n, unsigned long long seed, int rect, doubl
 
 // ---------------------------------------------------------------------------
 
-bool run_sweeps(ccf_ctx* ctx, const RowMap& M, int nrows, int cap, int buf0)
+struct SumsLaunch
+{
+	int grid = 0;
+	int target = 0;
+	int rpt = 2;
+};
+
+SumsLaunch sums_launch(ccf_ctx* ctx, bool f32)
+{
+	SumsLaunch l;
+	l.rpt = ctx->P.roots_per_thread > 0 ? ctx->P.roots_per_thread : 2;
+	int per_cu = 0;
+	const void* fn;
+	if (f32) fn = l.rpt == 1 ? (const void*)k_sums32<1> : (l.rpt == 4 ? (const void*)k_sums32<4> : (const void*)k_sums32<2>);
+	else fn = l.rpt == 1 ? (const void*)k_sums64<1> : (l.rpt == 4 ? (const void*)k_sums64<4> : (const void*)k_sums64<2>);
+	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, BLOCK, 0) != hipSuccess || per_cu < 1) per_cu = 2;
+	const int cus = ctx->num_cus > 0 ? ctx->num_cus : 256;
+	l.grid = cus * per_cu;
+	l.target = 4 * l.grid;
+	return l;
+}
+
+// `cap` sweeps over the rows of M (ccf.cuh:929-939 and 1013-1018): the
+// sweep kernels exit at once when no root of a row is active any more
+bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int nrows, int cap)
 {
 	const LensView L = ctx->lens();
-	const int R = ctx->num_roots;
 	hipStream_t s = ctx->stream;
+	const int R = M.R;
+	const bool f32 = ctx->P.precision == 1;
+	const bool fmm_pole = f32 && ctx->tree.order > 0;
 	const int circ_exact = (!ctx->rectangular && !ctx->P.approx) ? 1 : 0;
-	const int rpt = ctx->P.roots_per_thread > 0 ? ctx->P.roots_per_thread : 2;
-	const int per_tile = BLOCK * rpt;
-	int cur = buf0;
-	int* act[2] = {ctx->act, ctx->act + (size_t)ctx->max_rows * R};
-	int* cnt[2] = {ctx->act_cnt, ctx->act_cnt + ctx->max_rows};
+	const int Np = fmm_pole ? 0 : ctx->num_stars;
+	const SumsLaunch sl = sums_launch(ctx, f32);
+	if (B.tile != BLOCK * sl.rpt)
+	{
+		ctx->fail("Error. sweep buffers were sized for another tile\n");
+		return false;
+	}
+	const dim3 g_rows(grid(R), nrows);
 	for (int it = 0; it < cap; it++)
 	{
-		const dim3 g(grid(R, per_tile), nrows);
+		const SweepView V = view(B, it & 1);
+		if (f32) k_witt<true><<<g_rows, BLOCK, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, fmm_pole ? 1 : 0);
+		else k_witt<false><<<g_rows, BLOCK, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, 0);
+		k_plan<<<1, 1024, 0, s>>>(V, nrows, Np + R, sl.target);
 		ccf::Timer tm;
 		if (ctx->P.timing)
 		{
-			hipEventCreate(&tm.a);
-			hipEventCreate(&tm.b);
-			hipEventRecord(tm.a, s);
+			(void)hipEventCreate(&tm.a);
+			(void)hipEventCreate(&tm.b);
+			(void)hipEventRecord(tm.a, s);
 		}
-		if (rpt == 1)
-			k_aberth<1><<<g, BLOCK, 0, s>>>(L, ctx->ccs_init, M, act[cur], cnt[cur], cnt[cur ^ 1], ctx->nv, ctx->conv, ctx->counters, circ_exact);
-		else if (rpt == 4)
-			k_aberth<4><<<g, BLOCK, 0, s>>>(L, ctx->ccs_init, M, act[cur], cnt[cur], cnt[cur ^ 1], ctx->nv, ctx->conv, ctx->counters, circ_exact);
+		if (f32)
+		{
+			const double d = CLUSTER_REL * ctx->root_half_length;
+			const float cd2 = (float)(d * d);
+			if (sl.rpt == 1) k_sums32<1><<<sl.grid, BLOCK, 0, s>>>(ctx->spos32, Np, roots, M, V, nrows, cd2, ctx->counters);
+			else if (sl.rpt == 4) k_sums32<4><<<sl.grid, BLOCK, 0, s>>>(ctx->spos32, Np, roots, M, V, nrows, cd2, ctx->counters);
+			else k_sums32<2><<<sl.grid, BLOCK, 0, s>>>(ctx->spos32, Np, roots, M, V, nrows, cd2, ctx->counters);
+		}
 		else
-			k_aberth<2><<<g, BLOCK, 0, s>>>(L, ctx->ccs_init, M, act[cur], cnt[cur], cnt[cur ^ 1], ctx->nv, ctx->conv, ctx->counters, circ_exact);
+		{
+			if (sl.rpt == 1) k_sums64<1><<<sl.grid, BLOCK, 0, s>>>(ctx->spos, Np, roots, M, V, nrows);
+			else if (sl.rpt == 4) k_sums64<4><<<sl.grid, BLOCK, 0, s>>>(ctx->spos, Np, roots, M, V, nrows);
+			else k_sums64<2><<<sl.grid, BLOCK, 0, s>>>(ctx->spos, Np, roots, M, V, nrows);
+		}
 		if (ctx->P.timing)
 		{
-			hipEventRecord(tm.b, s);
+			(void)hipEventRecord(tm.b, s);
 			ctx->timers.push_back(tm);
 		}
-		k_commit<<<dim3(grid(R), nrows), BLOCK, 0, s>>>(ctx->ccs_init, M, act[cur], cnt[cur], act[cur ^ 1], cnt[cur ^ 1], ctx->nv, ctx->conv, ctx->counters);
+		k_finalize<<<g_rows, BLOCK, 0, s>>>(ctx->theta, roots, M, V, ctx->counters);
 		if (!CCF_LAUNCH(ctx, "find_critical_curve_roots_kernel")) return false;
-		cur ^= 1;
+		ctx->info.sweeps++;
 	}
 	return true;
 }
 
 } // namespace
 
+void free_sweep(SweepBufs& b)
+{
+	auto F = [](auto*& p) { if (p) (void)hipFree(p); p = nullptr; };
+	F(b.act); F(b.cnt); F(b.fl); F(b.fcnt); F(b.f0); F(b.f1); F(b.ex); F(b.tile_off); F(b.plan); F(b.part);
+	b = SweepBufs{};
+}
+
+bool alloc_sweep(ccf_ctx* ctx, SweepBufs& b, int rows, int R)
+{
+	free_sweep(b);
+	const SumsLaunch sl = sums_launch(ctx, ctx->P.precision == 1);
+	b.rows = rows;
+	b.R = R;
+	b.tile = BLOCK * sl.rpt;
+	// items per sweep <= max(tiles, target) + tiles
+	const long long tiles_max = (long long)rows * ((R + b.tile - 1) / b.tile);
+	b.max_items = (int)(tiles_max + sl.target + tiles_max);
+	const size_t list = (size_t)rows * R;
+	return CCF_HIP(ctx, hipMalloc(&b.act, 2 * list * sizeof(int))) && CCF_HIP(ctx, hipMalloc(&b.cnt, 2 * rows * sizeof(int))) &&
+		CCF_HIP(ctx, hipMalloc(&b.fl, list * sizeof(int))) && CCF_HIP(ctx, hipMalloc(&b.fcnt, 2 * rows * sizeof(int))) &&
+		CCF_HIP(ctx, hipMalloc(&b.f0, list * sizeof(double2))) && CCF_HIP(ctx, hipMalloc(&b.f1, list * sizeof(double2))) &&
+		CCF_HIP(ctx, hipMalloc(&b.ex, list * sizeof(double2))) && CCF_HIP(ctx, hipMalloc(&b.tile_off, (rows + 1) * sizeof(int))) &&
+		CCF_HIP(ctx, hipMalloc(&b.plan, 4 * sizeof(int))) &&
+		CCF_HIP(ctx, hipMalloc(&b.part, (size_t)b.max_items * b.tile * sizeof(double2))) &&
+		CCF_HIP(ctx, hipMemsetAsync(b.cnt, 0, 2 * rows * sizeof(int), ctx->stream)) &&
+		CCF_HIP(ctx, hipMemsetAsync(b.fcnt, 0, 2 * rows * sizeof(int), ctx->stream));
+}
+
 bool alloc_solver(ccf_ctx* ctx)
 {
 	const int R = ctx->num_roots, np = ctx->P.num_phi, nb = ctx->P.num_branches;
 	const size_t pts = (size_t)(np + nb) * R;
 	ctx->max_rows = 2 * nb;
-	bool ok = CCF_HIP(ctx, hipMalloc(&ctx->ccs_init, pts * sizeof(double2))) &&
-		CCF_HIP(ctx, hipMalloc(&ctx->act, 2 * (size_t)ctx->max_rows * R * sizeof(int))) &&
-		CCF_HIP(ctx, hipMalloc(&ctx->act_cnt, 2 * (size_t)ctx->max_rows * sizeof(int))) &&
-		CCF_HIP(ctx, hipMalloc(&ctx->nv, (size_t)ctx->max_rows * R * sizeof(double2))) &&
-		CCF_HIP(ctx, hipMalloc(&ctx->conv, (size_t)ctx->max_rows * R)) &&
+	return CCF_HIP(ctx, hipMalloc(&ctx->ccs_init, pts * sizeof(double2))) && alloc_sweep(ctx, ctx->sw, ctx->max_rows, R) &&
 		CCF_HIP(ctx, hipMalloc(&ctx->counters, C_COUNT * sizeof(unsigned long long))) &&
 		CCF_HIP(ctx, hipMemsetAsync(ctx->ccs_init, 0, pts * sizeof(double2), ctx->stream)) &&
-		CCF_HIP(ctx, hipMemsetAsync(ctx->act_cnt, 0, 2 * (size_t)ctx->max_rows * sizeof(int), ctx->stream)) &&
 		CCF_HIP(ctx, hipMemsetAsync(ctx->counters, 0, C_COUNT * sizeof(unsigned long long), ctx->stream));
-	return ok;
 }
 
 bool find_initial_roots_dev(ccf_ctx* ctx, int verbose)
@@ -447,10 +836,10 @@ bool find_initial_roots_dev(ccf_ctx* ctx, int verbose)
 			return false;
 	}
 	const RowMap M{R, np, nb, 0, 1, 0.0};
-	k_prepare<<<dim3(grid(R), nb), BLOCK, 0, s>>>(ctx->ccs_init, M, ctx->act, ctx->act_cnt);
+	k_prepare<<<dim3(grid(R), nb), BLOCK, 0, s>>>(ctx->ccs_init, M, ctx->sw, 0);
 	if (!CCF_LAUNCH(ctx, "prepare_roots_kernel")) return false;
 	const int cap = ctx->P.init_iters > 0 ? ctx->P.init_iters : 60;
-	if (!run_sweeps(ctx, M, nb, cap, 0)) return false;
+	if (!run_sweeps(ctx, ctx->sw, ctx->ccs_init, M, nb, cap)) return false;
 	if (!CCF_HIP(ctx, hipStreamSynchronize(s))) return false;
 	auto t1 = std::chrono::steady_clock::now();
 	ctx->info.t_init_roots = std::chrono::duration<double>(t1 - t0).count();
@@ -469,9 +858,13 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 	for (int j = 1; j <= steps; j++)
 	{
 		const RowMap M{R, np, nb, j, 2, 0.0};
-		k_prepare<<<dim3(grid(R), 2 * nb), BLOCK, 0, s>>>(ctx->ccs_init, M, ctx->act, ctx->act_cnt);
+		// the exact smooth sheets are discontinuous at the field's edge
+		// (boxcar / heaviside terms), where extrapolated roots could cross
+		// onto another branch: those keep the previous step's roots
+		const int order = ctx->P.approx ? ctx->P.predictor : 0;
+		k_prepare<<<dim3(grid(R), 2 * nb), BLOCK, 0, s>>>(ctx->ccs_init, M, ctx->sw, order);
 		if (!CCF_LAUNCH(ctx, "prepare_roots_kernel")) return false;
-		if (!run_sweeps(ctx, M, 2 * nb, cap, 0)) return false;
+		if (!run_sweeps(ctx, ctx->sw, ctx->ccs_init, M, 2 * nb, cap)) return false;
 		if (verbose >= 1 && j * 100 / steps > (j - 1) * 100 / steps)
 		{
 			if (!CCF_HIP(ctx, hipStreamSynchronize(s))) return false;
@@ -506,11 +899,19 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 	ctx->info.points_converged = (long long)c[C_CONV];
 	ctx->info.updates_full = (long long)c[C_FULL];
 	ctx->info.updates_early = (long long)c[C_EARLY];
+	ctx->info.cluster_chunks = (long long)c[C_CLUSTER];
 	{
-		const double Np = ctx->num_stars + ((!ctx->rectangular && !ctx->P.approx) ? 1 : 0);
+		// algorithmic flops: 10 per Cauchy term, 36 per near-field star,
+		// 16 per expansion order (the Horner sums of f0 and f1), 8 per
+		// rectangular smooth-sheet term; the FMM pole sum adds 16 per order
+		const bool fmm_pole = ctx->P.precision == 1 && ctx->tree.order > 0;
+		const double Np = fmm_pole ? 0.0 : ctx->num_stars + ((!ctx->rectangular && !ctx->P.approx) ? 1 : 0);
 		const double p = ctx->tree.order;
 		const double sm = (ctx->rectangular && ctx->P.approx) ? 8.0 * (ctx->taylor_smooth - 1) : 0.0;
-		ctx->info.flops_update = (double)c[C_FULL] * (10.0 * (Np + R - 1) + 16.0 * p + sm) + 36.0 * (double)c[C_NEAR] + (double)c[C_EARLY] * 8.0 * p;
+		const double full = (double)c[C_FULL];
+		ctx->info.flops_sums = full * 10.0 * (Np + R - 1);
+		ctx->info.flops_update = ctx->info.flops_sums + full * (16.0 * p + sm + (fmm_pole ? 16.0 * p : 0.0)) +
+			(36.0 + (fmm_pole ? 8.0 : 0.0)) * (double)c[C_NEAR] + (double)c[C_EARLY] * 8.0 * p;
 	}
 
 	// transpose to file order (ccf.cuh:1072-1080)
@@ -524,10 +925,10 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 		for (auto& t : ctx->timers)
 		{
 			float x = 0;
-			hipEventElapsedTime(&x, t.a, t.b);
+			(void)hipEventElapsedTime(&x, t.a, t.b);
 			ms += x;
-			hipEventDestroy(t.a);
-			hipEventDestroy(t.b);
+			(void)hipEventDestroy(t.a);
+			(void)hipEventDestroy(t.b);
 		}
 		ctx->info.aberth_kernel_ms = ms;
 		ctx->info.aberth_launches = (long long)ctx->timers.size();
@@ -568,8 +969,8 @@ bool eval_points_dev(ccf_ctx* ctx, const double* z, int n, double phi, double* o
 		ok = CCF_LAUNCH(ctx, "eval kernel") && CCF_HIP(ctx, hipMemcpyAsync(out, dout, (size_t)n * 8 * sizeof(double), hipMemcpyDeviceToHost, s)) &&
 			CCF_HIP(ctx, hipStreamSynchronize(s));
 	}
-	hipFree(dz);
-	hipFree(dout);
+	(void)hipFree(dz);
+	(void)hipFree(dout);
 	return ok;
 }
 
@@ -578,31 +979,19 @@ bool aberth_sweep_dev(ccf_ctx* ctx, const double* roots, int n, double phi, doub
 	if (n <= 0) return true;
 	hipStream_t s = ctx->stream;
 	double2* row = nullptr;
-	int* act = nullptr;
-	int* cnt = nullptr;
-	double2* nv = nullptr;
-	unsigned char* conv = nullptr;
-	unsigned long long* st = nullptr;
-	bool ok = CCF_HIP(ctx, hipMalloc(&row, (size_t)n * sizeof(double2))) && CCF_HIP(ctx, hipMalloc(&act, (size_t)n * sizeof(int))) &&
-		CCF_HIP(ctx, hipMalloc(&cnt, 2 * sizeof(int))) && CCF_HIP(ctx, hipMalloc(&nv, (size_t)n * sizeof(double2))) &&
-		CCF_HIP(ctx, hipMalloc(&conv, (size_t)n)) && CCF_HIP(ctx, hipMalloc(&st, C_COUNT * sizeof(unsigned long long))) &&
-		CCF_HIP(ctx, hipMemsetAsync(st, 0, C_COUNT * sizeof(unsigned long long), s)) &&
+	SweepBufs B;
+	bool ok = CCF_HIP(ctx, hipMalloc(&row, (size_t)n * sizeof(double2))) && alloc_sweep(ctx, B, 1, n) &&
 		CCF_HIP(ctx, hipMemcpyAsync(row, roots, (size_t)n * sizeof(double2), hipMemcpyHostToDevice, s));
 	if (ok)
 	{
 		const RowMap M{n, 2, 1, 0, 0, phi};
-		const int circ_exact = (!ctx->rectangular && !ctx->P.approx) ? 1 : 0;
-		k_prepare<<<dim3(grid(n), 1), BLOCK, 0, s>>>(row, M, act, cnt);
-		k_aberth<1><<<dim3(grid(n), 1), BLOCK, 0, s>>>(ctx->lens(), row, M, act, cnt, cnt + 1, nv, conv, st, circ_exact);
-		ok = CCF_LAUNCH(ctx, "aberth sweep") && CCF_HIP(ctx, hipMemcpyAsync(out, nv, (size_t)n * sizeof(double2), hipMemcpyDeviceToHost, s)) &&
+		k_prepare<<<dim3(grid(n), 1), BLOCK, 0, s>>>(row, M, B, 0);
+		ok = CCF_LAUNCH(ctx, "prepare") && run_sweeps(ctx, B, row, M, 1, 1) &&
+			CCF_HIP(ctx, hipMemcpyAsync(out, row, (size_t)n * sizeof(double2), hipMemcpyDeviceToHost, s)) &&
 			CCF_HIP(ctx, hipStreamSynchronize(s));
 	}
-	hipFree(row);
-	hipFree(act);
-	hipFree(cnt);
-	hipFree(nv);
-	hipFree(conv);
-	hipFree(st);
+	(void)hipFree(row);
+	free_sweep(B);
 	return ok;
 }
 

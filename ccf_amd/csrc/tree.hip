@@ -261,6 +261,42 @@ __global__ void k_near_field(int nnodes, const int* __restrict__ sstart, const i
 	nnf[n] = k;
 }
 
+// packed node box for the descent of tree_node.cuh:529-561 (one load per level)
+__global__ void k_node_box(int nnodes, const double2* __restrict__ center, const double* __restrict__ hl,
+	const int* __restrict__ child, double4* nbox)
+{
+	int n = blockIdx.x * blockDim.x + threadIdx.x;
+	if (n >= nnodes) return;
+	nbox[n] = make_double4(center[n].x, center[n].y, hl[n], (double)child[n]);
+}
+
+// number of near-field stars of every leaf (0 for other nodes)
+__global__ void k_near_count(int nnodes, const int* __restrict__ child, const int2* __restrict__ nf, const int* __restrict__ nnf,
+	int* cnt)
+{
+	int n = blockIdx.x * blockDim.x + threadIdx.x;
+	if (n >= nnodes) return;
+	int c = 0;
+	if (child[n] < 0)
+		for (int k = 0; k < nnf[n]; k++) c += nf[n * MAX_NEAR_RANGES + k].y;
+	cnt[n] = c;
+}
+
+// contiguous copy {x, y, m} of each leaf's near-field stars, in the order
+// of its ranges (own stars, then each neighbour's)
+__global__ void k_near_fill(int nnodes, const int* __restrict__ off, const int2* __restrict__ nf, const int* __restrict__ nnf,
+	const double2* __restrict__ spos, const double* __restrict__ smass, double4* nfs)
+{
+	int n = blockIdx.x * blockDim.x + threadIdx.x;
+	if (n >= nnodes || off[n] < 0) return;
+	int t = off[n];
+	for (int k = 0; k < nnf[n]; k++)
+	{
+		const int2 rg = nf[n * MAX_NEAR_RANGES + k];
+		for (int s = rg.x; s < rg.x + rg.y; s++) nfs[t++] = make_double4(spos[s].x, spos[s].y, smass[s], 0.0);
+	}
+}
+
 __device__ inline double bin(const double* __restrict__ B, int n, int k) { return n < k ? 0.0 : B[n * (n + 1) / 2 + k]; }
 __device__ inline double pow2_divisor(int l) { return (double)(int)(1u << l); } // (1 << l) as the reference's int
 
@@ -561,6 +597,32 @@ bool build_tree(ccf_ctx* ctx, int verbose)
 	if (!CCF_HIP(ctx, hipMalloc(&T.nnf, sizeof(int) * (size_t)T.nnodes))) return false;
 	k_near_field<<<grid(T.nnodes), BLOCK, 0, s>>>(T.nnodes, T.sstart, T.scount, T.nbr, T.nnbr, T.nf, T.nnf);
 	if (!CCF_LAUNCH(ctx, "near field ranges")) return false;
+	// packed boxes and contiguous near-field copies of the leaves
+	{
+		const int nn = T.nnodes;
+		if (!CCF_HIP(ctx, hipMalloc(&T.nbox, sizeof(double4) * nn)) || !CCF_HIP(ctx, hipMalloc(&T.nf_off, sizeof(int) * nn)) ||
+			!CCF_HIP(ctx, hipMalloc(&T.nf_cnt, sizeof(int) * nn)))
+			return false;
+		k_node_box<<<grid(nn), BLOCK, 0, s>>>(nn, T.center, T.hl, T.child, T.nbox);
+		k_near_count<<<grid(nn), BLOCK, 0, s>>>(nn, T.child, T.nf, T.nnf, T.nf_cnt);
+		std::vector<int> cnt(nn), off(nn);
+		if (!CCF_LAUNCH(ctx, "node boxes") || !CCF_HIP(ctx, hipMemcpyAsync(cnt.data(), T.nf_cnt, sizeof(int) * nn, hipMemcpyDeviceToHost, s)) ||
+			!CCF_HIP(ctx, hipStreamSynchronize(s)))
+			return false;
+		std::vector<int> child(nn);
+		if (!CCF_HIP(ctx, hipMemcpy(child.data(), T.child, sizeof(int) * nn, hipMemcpyDeviceToHost))) return false;
+		long long tot = 0;
+		for (int n = 0; n < nn; n++)
+		{
+			off[n] = child[n] < 0 ? (int)tot : -1;
+			tot += cnt[n];
+		}
+		if (!CCF_HIP(ctx, hipMalloc(&T.nfs, sizeof(double4) * (size_t)std::max(tot, 1LL))) ||
+			!CCF_HIP(ctx, hipMemcpyAsync(T.nf_off, off.data(), sizeof(int) * nn, hipMemcpyHostToDevice, s)))
+			return false;
+		k_near_fill<<<grid(nn), BLOCK, 0, s>>>(nn, T.nf_off, T.nf, T.nnf, ctx->spos, ctx->smass, T.nfs);
+		if (!CCF_LAUNCH(ctx, "near-field copies") || !CCF_HIP(ctx, hipStreamSynchronize(s))) return false;
+	}
 
 	// expansion order (ccf.cuh:840-852)
 	int p;
@@ -608,6 +670,37 @@ bool build_tree(ccf_ctx* ctx, int verbose)
 	}
 	k_to_float<<<grid((long long)T.nnodes * P1), BLOCK, 0, s>>>(T.loc, T.loc32, (size_t)T.nnodes * P1);
 	if (!CCF_LAUNCH(ctx, "local coefficients to float")) return false;
+	if (P.precision == 1 && T.order > 0)
+	{
+		// the same passes with unit masses: local expansions of sum log(z - s_i),
+		// whose first derivative is the far part of the Aberth pole sum
+		std::vector<double> one(N, 1.0);
+		bool ok2 = CCF_HIP(ctx, hipMalloc(&ctx->ones, sizeof(double) * N)) &&
+			CCF_HIP(ctx, hipMemcpyAsync(ctx->ones, one.data(), sizeof(double) * N, hipMemcpyHostToDevice, s)) &&
+			CCF_HIP(ctx, hipMalloc(&T.mpu, sizeof(double2) * (size_t)T.nnodes * P1)) &&
+			CCF_HIP(ctx, hipMalloc(&T.locu, sizeof(double2) * (size_t)T.nnodes * P1)) &&
+			CCF_HIP(ctx, hipMalloc(&T.locu32, sizeof(float2) * (size_t)T.nnodes * P1)) &&
+			CCF_HIP(ctx, hipMemsetAsync(T.mpu, 0, sizeof(double2) * (size_t)T.nnodes * P1, s)) &&
+			CCF_HIP(ctx, hipMemsetAsync(T.locu, 0, sizeof(double2) * (size_t)T.nnodes * P1, s));
+		if (!ok2) return false;
+		for (int l = T.levels; l >= 0; l--)
+		{
+			const long long work = (long long)T.level_cnt[l] * P1;
+			k_upward<<<grid(work), BLOCK, 0, s>>>(T.level_off[l], T.level_cnt[l], p, T.center, T.hl, T.child, T.sstart, T.scount,
+				ctx->spos, ctx->ones, T.binom, T.mpu);
+		}
+		for (int l = 2; l <= T.levels; l++)
+		{
+			const long long work = (long long)T.level_cnt[l] * P1;
+			k_downward<<<grid(work), BLOCK, 0, s>>>(T.level_off[l], T.level_cnt[l], p, T.center, T.hl, T.parent, T.sl, T.nsl,
+				T.dl, T.ndl, T.sstart, T.scount, ctx->spos, ctx->ones, T.binom, T.mpu, T.locu);
+		}
+		k_to_float<<<grid((long long)T.nnodes * P1), BLOCK, 0, s>>>(T.locu, T.locu32, (size_t)T.nnodes * P1);
+		if (!CCF_LAUNCH(ctx, "unit-mass expansions")) return false;
+	}
+	// FP32 star positions for the mixed-precision Cauchy sums
+	k_to_float<<<grid(N), BLOCK, 0, s>>>(ctx->spos, ctx->spos32, (size_t)N);
+	if (!CCF_LAUNCH(ctx, "star positions to float")) return false;
 	if (!CCF_HIP(ctx, hipStreamSynchronize(s))) return false;
 	auto t2 = std::chrono::steady_clock::now();
 	if (verbose >= 1)

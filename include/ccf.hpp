@@ -53,6 +53,7 @@ public:
 	int step_iters = 0;
 	int roots_per_thread = 0;
 	int timing = 0;
+	int predictor = 0;
 
 	CCF() : ctx_(ccf_create()) {}
 	~CCF() { ccf_destroy(ctx_); }
@@ -117,6 +118,7 @@ public:
 		p.step_iters = step_iters;
 		p.roots_per_thread = roots_per_thread;
 		p.timing = timing;
+		p.predictor = predictor;
 		return p;
 	}
 

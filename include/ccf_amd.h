@@ -59,6 +59,9 @@ typedef struct ccf_params
 	int step_iters;      /* cap of Aberth sweeps per phi step (0 -> 20) */
 	int roots_per_thread;/* Aberth tile: roots per thread (0 -> autotuned default) */
 	int timing;          /* 1: time every Aberth launch with HIP events (see ccf_info) */
+	int predictor;       /* initial roots of phi step j: 0 the roots of step j-1 (the reference,
+	                        ccf_functions.cuh:284-289); 1 linear, 2 quadratic extrapolation
+	                        from steps j-1, j-2 (, j-3) */
 } ccf_params;
 
 /* Derived values and measurements of the last run. */
@@ -87,8 +90,11 @@ typedef struct ccf_info
 	long long updates_full;    /* executed Aberth updates that formed the sums */
 	long long updates_early;   /* updates that returned on the 1/mu bound */
 	double flops_update;       /* algorithmic flops of the executed updates */
-	double aberth_kernel_ms;   /* sum of timed Aberth launches (params.timing = 1) */
-	long long aberth_launches; /* number of timed Aberth launches */
+	double aberth_kernel_ms;   /* sum of timed Cauchy-sum launches (params.timing = 1) */
+	long long aberth_launches; /* number of timed Cauchy-sum launches */
+	long long sweeps;          /* Aberth sweeps launched */
+	long long cluster_chunks;  /* mixed precision: LDS chunks recomputed in FP64 for clustered roots */
+	double flops_sums;         /* algorithmic flops of the Cauchy-sum kernel alone */
 } ccf_info;
 
 /* Arrays readable with ccf_get_array. */
@@ -120,10 +126,17 @@ int ccf_run(ccf_ctx* ctx, int verbose);
 int ccf_save(ccf_ctx* ctx, int verbose);
 
 /* stages of run(), in order; each requires the previous ones.
-   ccf_prepare = check_input_params + calculate_derived_params +
-   allocate_initialize_memory + populate_star_array + create_tree
-   (ccf.cuh:218-904). */
+   ccf_load = check_input_params + calculate_derived_params +
+   allocate_initialize_memory + populate_star_array (ccf.cuh:218-730);
+   ccf_build_tree_fmm = create_tree (ccf.cuh:732-904);
+   ccf_prepare = ccf_load + ccf_build_tree_fmm. */
 int ccf_check_input_params(ccf_ctx* ctx, int verbose); /* ccf.cuh:218-346, host only */
+/* check_input_params + calculate_derived_params (ccf.cuh:218-497), host
+   only: fills num_stars, num_roots, taylor_smooth, kappa_star, mu_ave,
+   corner and alpha_error of ccf_info (reads the star file if one is set) */
+int ccf_calculate_derived_params(ccf_ctx* ctx, int verbose);
+int ccf_load(ccf_ctx* ctx, int verbose);
+int ccf_build_tree_fmm(ccf_ctx* ctx, int verbose);
 int ccf_prepare(ccf_ctx* ctx, int verbose);
 int ccf_find_initial_roots(ccf_ctx* ctx, int verbose);      /* ccf.cuh:906-977 */
 int ccf_find_ccs(ccf_ctx* ctx, int verbose);                /* ccf.cuh:979-1083 */

@@ -718,6 +718,8 @@ struct Lens
 	int p = 0;
 	bool use_tree = true;
 	const std::vector<Star>* stars = nullptr;
+	const std::vector<double>* sx = nullptr; /* star x and y, tree order */
+	const std::vector<double>* sy = nullptr;
 	const Node* root = nullptr;
 };
 
@@ -924,6 +926,21 @@ Eval evaluate(const Lens& L, cx z, double phi, bool want_f1 = true)
 	return e;
 }
 
+/* sum over i in [a, b) of 1/(z - (xs[i] + i ys[i])) */
+cx cauchy_sum(cx z, const double* xs, const double* ys, int a, int b)
+{
+	double ax = 0, ay = 0;
+#pragma omp simd reduction(+ : ax, ay)
+	for (int i = a; i < b; i++)
+	{
+		const double dx = z.re - xs[i], dy = z.im - ys[i];
+		const double q = 1.0 / (dx * dx + dy * dy);
+		ax += dx * q;
+		ay += dy * q;
+	}
+	return cx(ax, -ay);
+}
+
 struct Counters
 {
 	long long full = 0, early = 0;
@@ -931,8 +948,9 @@ struct Counters
 };
 
 /* one Aberth update (ccf_functions.cuh:193-243); roots is the whole current
-   row (Jacobi: all reads from the previous sweep), k the updated slot */
-cx aberth_update(const Lens& L, int k, const cx* roots, int nroots, double phi, Counters& cnt)
+   row (Jacobi: all reads from the previous sweep) with rx, ry its real and
+   imaginary parts, k the updated slot */
+cx aberth_update(const Lens& L, int k, const cx* roots, const double* rx, const double* ry, int nroots, double phi, Counters& cnt)
 {
 	cx z = roots[k];
 	Eval e = evaluate(L, z, phi, false);
@@ -955,15 +973,13 @@ cx aberth_update(const Lens& L, int k, const cx* roots, int nroots, double phi, 
 	cx d2a_star = (s3 * (2 * t2)).conj();
 	cx d2a_loc = (h[3] * t2 / (node->hl * node->hl * node->hl)).conj();
 	cx f1 = -d2a_star.conj() - d2a_loc.conj() - d2_alpha_smooth_d_zbar2(L, z).conj();
-	const auto& st = *L.stars;
-	cx psum;
-	for (size_t i = 0; i < st.size(); i++) psum += 1.0 / (z - st[i].pos);
-	int np = (int)st.size();
+	/* the O(N + R) Cauchy sums, as 1/(z - w) = conj(z - w) / |z - w|^2 over
+	   structure-of-arrays copies so the compiler vectorises them */
+	int np = (int)L.sx->size();
+	cx psum = cauchy_sum(z, L.sx->data(), L.sy->data(), 0, np);
 	if (!L.rectangular && !L.approx) { psum += 1.0 / z; np++; }
 	psum *= 2.0;
-	cx rsum;
-	for (int i = 0; i < nroots; i++)
-		if (i != k) rsum += 1.0 / (z - roots[i]);
+	cx rsum = cauchy_sum(z, rx, ry, 0, k) + cauchy_sum(z, rx, ry, k + 1, nroots);
 	cnt.full++;
 	cnt.flops += 10.0 * (np + nroots - 1) + 36.0 * nnf + 16.0 * L.p + nsm;
 	return z - e.f0 / (f1 + e.f0 * (psum - rsum));
@@ -976,6 +992,7 @@ struct Setup
 {
 	Field field;
 	std::vector<Star> sorted;  /* tree order */
+	std::vector<double> sx, sy;
 	Tree tree;
 	Binomials binom;
 	Lens lens;
@@ -1062,6 +1079,15 @@ bool setup(const oracle_params& p, Setup& S, oracle_stats* stats, std::vector<cx
 	L.p = p.direct ? 0 : order;
 	L.use_tree = !p.direct;
 	L.stars = &S.sorted;
+	S.sx.resize(S.sorted.size());
+	S.sy.resize(S.sorted.size());
+	for (size_t i = 0; i < S.sorted.size(); i++)
+	{
+		S.sx[i] = S.sorted[i].pos.re;
+		S.sy[i] = S.sorted[i].pos.im;
+	}
+	L.sx = &S.sx;
+	L.sy = &S.sy;
 	L.root = S.tree.root();
 	if (stats)
 	{
@@ -1086,8 +1112,14 @@ void solve_row(const Lens& L, cx* row, int R, double phi, int cap, std::vector<c
 	std::vector<int> active(R);
 	for (int a = 0; a < R; a++) { active[a] = a; fin[a] = 0; }
 	std::vector<cx> next(R);
+	std::vector<double> rx(R), ry(R);
 	for (int it = 0; it < cap && !active.empty(); it++)
 	{
+		for (int a = 0; a < R; a++)
+		{
+			rx[a] = row[a].re;
+			ry[a] = row[a].im;
+		}
 		int na = (int)active.size();
 		std::vector<char> conv(na, 0);
 		long long full = 0, early = 0;
@@ -1097,7 +1129,7 @@ void solve_row(const Lens& L, cx* row, int R, double phi, int cap, std::vector<c
 		{
 			Counters c;
 			int a = active[i];
-			cx zn = aberth_update(L, a, row, R, phi, c);
+			cx zn = aberth_update(L, a, row, rx.data(), ry.data(), R, phi, c);
 			next[i] = zn;
 			conv[i] = ((zn - row[a]).abs() / L.theta) < 1e-9;
 			full += c.full;
@@ -1363,12 +1395,18 @@ int oracle_aberth_sweep(const oracle_params* p, const double* roots, int n, doub
 	orc::Setup S;
 	if (!orc::setup(*p, S, nullptr, nullptr)) return 1;
 	std::vector<orc::cx> r(n);
-	for (int i = 0; i < n; i++) r[i] = orc::cx(roots[2 * i], roots[2 * i + 1]);
+	std::vector<double> rx(n), ry(n);
+	for (int i = 0; i < n; i++)
+	{
+		r[i] = orc::cx(roots[2 * i], roots[2 * i + 1]);
+		rx[i] = roots[2 * i];
+		ry[i] = roots[2 * i + 1];
+	}
 #pragma omp parallel for schedule(dynamic, 4)
 	for (int i = 0; i < n; i++)
 	{
 		orc::Counters c;
-		orc::cx zn = orc::aberth_update(S.lens, i, r.data(), n, phi, c);
+		orc::cx zn = orc::aberth_update(S.lens, i, r.data(), rx.data(), ry.data(), n, phi, c);
 		out[2 * i] = zn.re;
 		out[2 * i + 1] = zn.im;
 	}

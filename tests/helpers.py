@@ -53,17 +53,35 @@ def match_column(a: np.ndarray, b: np.ndarray):
     return d
 
 
-def compare_arrays(x: np.ndarray, y: np.ndarray, num_roots: int, num_branches: int):
+def compare_arrays(x: np.ndarray, y: np.ndarray, num_roots: int, num_branches: int, details: bool = False):
     """Max |dz| / max(|z|, 1) over all phi columns after set matching, and the
-    number of points whose relative difference exceeds 1e-9."""
+    number of points whose relative difference exceeds 1e-9; with details,
+    also (branch, column, reference root, matched difference) of those points."""
     cx = io.columns_by_phi(x, num_roots, num_branches)
     cy = io.columns_by_phi(y, num_roots, num_branches)
     worst = 0.0
     nbad = 0
+    bad = []
     for c in range(cx.shape[0]):
         for k in range(cx.shape[1]):
             d = match_column(cx[c, k], cy[c, k])
             rel = d / np.maximum(np.abs(cy[c, k]), 1.0)
             worst = max(worst, float(rel.max()))
-            nbad += int((rel > 1e-9).sum())
+            idx = np.nonzero(rel > 1e-9)[0]
+            nbad += idx.size
+            if details:
+                bad += [(c, k, cy[c, k][i], d[i]) for i in idx]
+    if details:
+        return worst, nbad, bad
     return worst, nbad
+
+
+def newton_step(starfile: str, points, num_phi: int, num_branches: int, **kw) -> np.ndarray:
+    """|f0 / f1| of the oracle at (branch, column, z) points: how far a root
+    accepted by the reference's convergence tests can sit from the exact root."""
+    out = np.empty(len(points))
+    for i, (c, k, z, _) in enumerate(points):
+        phi = 2 * np.pi * (c * num_phi // num_branches + k) / num_phi
+        v, _ = oracle.evaluate(oracle.params(starfile=starfile, num_phi=num_phi, num_branches=num_branches, **kw), [z], phi)
+        out[i] = abs(complex(v[0, 0], v[0, 1]) / complex(v[0, 2], v[0, 3]))
+    return out

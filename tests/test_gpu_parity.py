@@ -17,7 +17,7 @@ import pytest
 import ccf_amd
 import oracle
 from ccf_amd import io
-from helpers import FIELDS, compare_arrays, make_field, oracle_run
+from helpers import FIELDS, compare_arrays, make_field, newton_step, oracle_run
 
 pytestmark = pytest.mark.gpu
 
@@ -58,12 +58,22 @@ def test_eval_matches_oracle(workdir, name, kw):
     assert np.allclose(g[:, 7], o[:, 7], rtol=1e-9, atol=1e-15)
 
 
-def test_aberth_sweep_matches_oracle(workdir):
-    sf = make_field(workdir, "salpeter300")
+@pytest.mark.parametrize("name,kw", [
+    ("salpeter300", dict(kappa_tot=0.45, shear=0.35)),
+    ("rect200", dict(kappa_tot=0.7, shear=0.25)),
+    ("rect200", dict(kappa_tot=0.7, shear=0.25, approx=0)),
+    ("smooth150", dict(kappa_tot=0.6, shear=0.2, approx=0)),
+    ("cfg1", dict(kappa_tot=0.3, shear=0.1, direct=1)),
+])
+def test_aberth_sweep_matches_oracle(workdir, name, kw):
+    sf = make_field(workdir, name)
     n, rect, corner, theta, st = io.read_stars(sf)
-    roots = np.concatenate([st[:, 0] + 1 + 1j * st[:, 1], st[:, 0] - 1 + 1j * st[:, 1]])
-    phi = np.pi
-    kw = dict(kappa_tot=0.45, shear=0.35, num_phi=10)
+    # extra roots off the disc boundary |z| = |corner|, where the exact
+    # circular smooth term is discontinuous
+    roots = np.concatenate([st[:, 0] + 1 + 1j * st[:, 1], st[:, 0] - 1 + 1j * st[:, 1],
+                            1.05 * abs(corner) * np.exp(2j * np.pi * (np.arange(7) + 0.5) / 7)])
+    phi = np.pi / 3
+    kw = dict(kw, num_phi=10)
     with _ctx(sf, str(workdir) + "/", **kw) as c:
         c.prepare()
         g = c.aberth_sweep(roots, phi)
@@ -79,15 +89,25 @@ RUNS = [
     ("rect200", dict(kappa_tot=0.7, shear=0.25, num_phi=80, num_branches=2)),
     ("smooth150", dict(kappa_tot=0.6, shear=0.2, num_phi=60, num_branches=1, approx=0)),
 ]
+# the mixed-precision form and the continuation predictors change only the
+# path to each root: the same oracle run is the reference for them
+MIXED = [
+    ("cfg1", dict(kappa_tot=0.3, shear=0.1, num_phi=100, num_branches=1, direct=1), dict(precision=1)),
+    ("salpeter300", dict(kappa_tot=0.45, shear=0.35, num_phi=120, num_branches=3, fmm_order=20), dict(precision=1)),
+    ("salpeter300", dict(kappa_tot=0.45, shear=0.35, num_phi=120, num_branches=3, fmm_order=20), dict(precision=1, predictor=1)),
+    ("salpeter300", dict(kappa_tot=0.45, shear=0.35, num_phi=120, num_branches=3, fmm_order=20), dict(predictor=2)),
+    ("rect200", dict(kappa_tot=0.7, shear=0.25, num_phi=80, num_branches=2), dict(precision=1, predictor=2)),
+    ("smooth150", dict(kappa_tot=0.6, shear=0.2, num_phi=60, num_branches=1, approx=0), dict(precision=1, predictor=1)),
+]
 
 
-@pytest.mark.parametrize("name,kw", RUNS)
-def test_run_matches_oracle(workdir, name, kw):
+@pytest.mark.parametrize("name,kw,gkw", [(n, k, {}) for n, k in RUNS] + MIXED)
+def test_run_matches_oracle(workdir, name, kw, gkw):
     sf = make_field(workdir, name)
-    tag = f"{name}_{abs(hash(tuple(sorted(kw.items())))) % 100000}"
+    tag = f"{name}_{abs(hash(tuple(sorted(kw.items())) + tuple(sorted(gkw.items())))) % 100000}"
     gp = os.path.join(str(workdir), f"g_{tag}_")
     op = os.path.join(str(workdir), f"o_{tag}_")
-    with _ctx(sf, gp, write_mu_length_scales=1, **kw) as c:
+    with _ctx(sf, gp, write_mu_length_scales=1, **kw, **gkw) as c:
         assert c.run(0), c.last_error()
         assert c.save(0), c.last_error()
         ginfo = c.info()
@@ -99,12 +119,25 @@ def test_run_matches_oracle(workdir, name, kw):
     gccs = io.read_array(gp + "ccf_ccs.bin")
     occs = io.read_array(op + "ccf_ccs.bin")
     assert gccs.shape == occs.shape == (R * nb, kw["num_phi"] // nb + 1)
-    worst, nbad = compare_arrays(gccs, occs, R, nb)
-    assert nbad == 0, (worst, nbad)
+    worst, nbad, bad = compare_arrays(gccs, occs, R, nb, details=True)
+    if not gkw:
+        # the FP64 path follows the oracle's iterates: 1e-9 relative everywhere
+        assert nbad == 0, (worst, nbad)
+    else:
+        # other paths may stop elsewhere inside the reference's convergence
+        # band: within twice the oracle's own Newton step |f0/f1| of its root
+        allowed = 2 * newton_step(sf, bad, kw["num_phi"], nb, **{k: v for k, v in kw.items()
+                                                                  if k not in ("num_phi", "num_branches")})
+        over = [(b, a) for b, a in zip(bad, allowed) if b[3] > 1e-9 * max(abs(b[2]), 1) + a]
+        assert not over, over[:5]
+        assert nbad <= max(5, 1e-3 * R * (kw["num_phi"] + nb)), nbad
     gc = io.read_array(gp + "ccf_caustics.bin")
     oc = io.read_array(op + "ccf_caustics.bin")
     worst_c, nbad_c = compare_arrays(gc, oc, R, nb)
-    assert nbad_c == 0, (worst_c, nbad_c)
+    if not gkw:
+        assert nbad_c == 0, (worst_c, nbad_c)
+    else:
+        assert nbad_c <= nbad and worst_c < 1e-6, (worst_c, nbad_c)
     gm = io.read_array(gp + "ccf_mu_length_scales.bin", complex_values=False)
     assert gm.shape == gccs.shape and np.all(np.isfinite(gm)) and np.all(gm > 0)
     # parameter files agree except for the timings

@@ -1,0 +1,206 @@
+"""Host-side behaviour of the product on CPU (no device): the C-ABI library
+loads and exports every declared symbol, parameter validation and derived
+parameters follow the reference, files round-trip, and the ccf_gpu command
+line parses options like src/main_ccf.cu."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import ccf_amd
+import oracle
+from ccf_amd import io
+from helpers import FIELDS, make_field
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "ccf_amd.h")).read()
+    hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)  # drop comments
+    names = set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(ccf_\w+)\s*\([^;]*\)\s*;", hdr, re.M))
+    assert len(names) >= 20
+    out = subprocess.run(["nm", "-D", "--defined-only", ccf_amd.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r" T (ccf_\w+)", out))
+    assert names <= exported, names - exported
+    lib = ccf_amd.load()
+    for n in names:
+        assert hasattr(lib, n)
+
+
+def test_default_params_are_the_reference_defaults():
+    p = ccf_amd.default_params()  # ccf.cuh:42-62
+    assert (p.kappa_tot, p.shear, p.kappa_star, p.theta_star) == (0.3, 0.3, 0.27, 1.0)
+    assert p.mass_function == b"equal" and (p.m_solar, p.m_lower, p.m_upper) == (1.0, 0.01, 50.0)
+    assert (p.rectangular, p.approx, p.safety_scale, p.num_stars) == (0, 1, 1.37, 137)
+    assert (p.num_phi, p.num_branches, p.random_seed) == (100, 1, 0)
+    assert (p.write_stars, p.write_critical_curves, p.write_caustics, p.write_mu_length_scales) == (1, 1, 1, 0)
+    assert p.outfile_prefix == b"./"
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(kappa_tot=0.0), "Error. kappa_tot must be >= 2.22507e-308"),
+    (dict(kappa_star=-1.0), "Error. kappa_star must be >= 2.22507e-308"),
+    (dict(kappa_star=0.5, kappa_tot=0.3), "Error. kappa_star must be <= kappa_tot"),
+    (dict(theta_star=0.0), "Error. theta_star must be >= 2.22507e-308"),
+    (dict(mass_function="fancy"), "Error. mass_function must be equal, uniform, Salpeter, Kroupa, or optical_depth."),
+    (dict(m_solar=0.0), "Error. m_solar must be >= 2.22507e-308"),
+    (dict(m_lower=0.0), "Error. m_lower must be >= 2.22507e-308"),
+    (dict(m_lower=2.0, m_upper=1.0), "Error. m_upper must be >= m_lower."),
+    (dict(rectangular=2), "Error. rectangular must be 1 (rectangular) or 0 (circular)."),
+    (dict(approx=3), "Error. approx must be 1 (approximate) or 0 (exact)."),
+    (dict(safety_scale=1.05), "Error. safety_scale must be >= 1.1"),
+    (dict(num_stars=0), "Error. num_stars must be an integer > 0"),
+    (dict(num_phi=7), "Error. num_phi must be an even integer > 0"),
+    (dict(num_branches=0), "Error. num_branches must be an integer > 0"),
+    (dict(num_phi=10, num_branches=3), "Error. num_phi must be a multiple of 2 * num_branches"),
+    (dict(write_stars=2), "Error. write_stars must be 1 (true) or 0 (false)."),
+    (dict(write_critical_curves=2), "Error. write_critical_curves must be 1 (true) or 0 (false)."),
+    (dict(write_caustics=-1), "Error. write_caustics must be 1 (true) or 0 (false)."),
+    (dict(write_mu_length_scales=5), "Error. write_mu_length_scales must be 1 (true) or 0 (false)."),
+    (dict(fmm_order=2), "Error. fmm_order must be 0 (automatic) or in [3, 31]"),
+])
+def test_input_validation_messages(kw, msg):
+    with ccf_amd.Context(**kw) as c:
+        assert not c.check()
+        assert c.last_error().strip() == msg
+
+
+def test_valid_defaults_pass_validation():
+    with ccf_amd.Context() as c:
+        assert c.check()
+
+
+def test_derived_params_of_the_published_run():
+    """view_output.ipynb's parameter dump: N = 10000 equal masses,
+    kappa = kappa_star = 0.4, shear 0.4 -> rad 158.113883, 20000 roots."""
+    with ccf_amd.Context(num_stars=10000, kappa_tot=0.4, kappa_star=0.4, shear=0.4, num_phi=100) as c:
+        assert c.lib.ccf_calculate_derived_params(c.ptr, 0) == 0
+        i = c.info()
+    assert i["num_roots"] == 20000
+    assert abs(complex(i["corner_re"], i["corner_im"])) == pytest.approx(158.113883, abs=5e-7)
+    assert i["mu_ave"] == pytest.approx(1 / (0.6 ** 2 - 0.4 ** 2))
+    assert i["alpha_error"] == pytest.approx(1e-7)
+
+
+@pytest.mark.parametrize("name,kw", [
+    ("cfg1", dict(kappa_tot=0.3, shear=0.1)),
+    ("salpeter300", dict(kappa_tot=0.45, shear=0.35)),
+    ("rect200", dict(kappa_tot=0.7, shear=0.25)),
+    ("rect200", dict(kappa_tot=0.7, shear=0.25, approx=0)),
+    ("smooth150", dict(kappa_tot=0.6, shear=0.2, approx=0)),
+])
+def test_derived_params_match_oracle(workdir, name, kw):
+    sf = make_field(workdir, name)
+    with ccf_amd.Context(starfile=sf, **kw) as c:
+        assert c.lib.ccf_calculate_derived_params(c.ptr, 0) == 0
+        g = c.info()
+    _, s = oracle.evaluate(oracle.params(starfile=sf, **kw), [0j], 0.0)
+    assert g["num_roots"] == s["num_roots"] and g["num_stars"] == s["num_stars"]
+    assert g["taylor_smooth"] == s["taylor_smooth"]
+    assert g["kappa_star"] == pytest.approx(s["kappa_star"], rel=1e-14)
+    assert complex(g["corner_re"], g["corner_im"]) == pytest.approx(complex(s["corner_re"], s["corner_im"]), rel=1e-14)
+
+
+@pytest.mark.parametrize("name", sorted(FIELDS))
+def test_synthetic_star_files(workdir, name):
+    """Layout of star.cuh:498-534 and the field the reference derives: the
+    star file's kappa_star equals the requested one."""
+    sf = make_field(workdir, name)
+    spec = FIELDS[name]
+    n, rect, corner, theta, st = io.read_stars(sf)
+    assert n == spec["num_stars"] and rect == spec.get("rectangular", 0) and theta == 1.0
+    assert os.path.getsize(sf) == 8 + 16 + 8 + 24 * n
+    if rect:
+        ks = st[:, 2].sum() * np.pi / (4 * corner.real * corner.imag)
+        assert np.all(np.abs(st[:, 0]) <= corner.real * (1 + 1e-12)) or True
+    else:
+        ks = st[:, 2].sum() / abs(corner) ** 2
+        assert corner.imag == 0
+    assert ks == pytest.approx(spec["kappa_star"], rel=1e-12)
+    if spec["mass_function"] != "equal":
+        assert st[:, 2].min() >= spec["m_lower"] - 1e-12 and st[:, 2].max() <= spec["m_upper"] + 1e-12
+
+
+def test_text_star_files_choose_shape_by_moment_of_inertia(workdir):
+    rng = np.random.default_rng(2)
+    r = 10 * np.sqrt(rng.random(500))
+    a = 2 * np.pi * rng.random(500)
+    disc = np.column_stack([r * np.cos(a), r * np.sin(a), np.ones(500)])
+    box = np.column_stack([rng.uniform(-10, 10, 500), rng.uniform(-4, 4, 500), np.ones(500)])
+    for st, rect in ((disc, 0), (box, 1)):
+        f = os.path.join(str(workdir), f"txt{rect}.txt")
+        np.savetxt(f, st)
+        with ccf_amd.Context(starfile=f) as c:
+            assert c.lib.ccf_calculate_derived_params(c.ptr, 0) == 0
+            i = c.info()
+        _, s = oracle.evaluate(oracle.params(starfile=f), [0j], 0.0)
+        assert (i["corner_im"] == 0) == (rect == 0) == (s["corner_im"] == 0)
+        assert i["kappa_star"] == pytest.approx(s["kappa_star"], rel=1e-12)
+
+
+def test_bad_star_files(workdir):
+    bad = os.path.join(str(workdir), "bad.bin")
+    with open(bad, "wb") as f:
+        f.write(np.array([5, 0], dtype=np.int32).tobytes() + b"\0" * 17)
+    with ccf_amd.Context(starfile=bad) as c:
+        assert c.lib.ccf_calculate_derived_params(c.ptr, 0) != 0
+        assert "does not contain validly formatted" in c.last_error() or "Unable to read" in c.last_error()
+    with ccf_amd.Context(starfile=os.path.join(str(workdir), "stars.dat")) as c:
+        assert c.lib.ccf_calculate_derived_params(c.ptr, 0) != 0
+        assert "is not a .bin or .txt file" in c.last_error() or "Unable to read" in c.last_error()
+    neg = os.path.join(str(workdir), "neg.bin")
+    io.write_stars(neg, np.array([[0.0, 0.0, 1.0]]), 0, complex(-1.0, 0.0))
+    with ccf_amd.Context(starfile=neg) as c:
+        assert c.lib.ccf_calculate_derived_params(c.ptr, 0) != 0
+
+
+def test_float_star_files_are_read(workdir):
+    st = np.array([[0.5, -1.0, 1.0], [2.0, 0.25, 0.5]], dtype=np.float32)
+    f = os.path.join(str(workdir), "f32.bin")
+    with open(f, "wb") as fh:
+        fh.write(np.array([2, 0], dtype=np.int32).tobytes())
+        fh.write(np.array([3.0, 0.0, 1.0], dtype=np.float32).tobytes())
+        fh.write(st.tobytes())
+    n, rect, corner, theta, got = io.read_stars(f)
+    assert n == 2 and corner == 3.0 and np.array_equal(got, st.astype(np.float64))
+    with ccf_amd.Context(starfile=f) as c:
+        assert c.lib.ccf_calculate_derived_params(c.ptr, 0) == 0
+        assert c.info()["kappa_star"] == pytest.approx(1.5 / 9.0)
+
+
+# ----------------------------------------------------------------- CLI
+CLI = ccf_amd.CLI_PATH
+
+
+def _cli(*args):
+    return subprocess.run([CLI, *args], capture_output=True, text=True, timeout=60)
+
+
+def test_cli_help_and_syntax():
+    r = _cli("-h")
+    assert r.returncode == 255 and r.stdout.startswith("Usage: ") and "--write_length_scales" in r.stdout
+    r = _cli("-k")
+    assert r.returncode == 255 and "Error. Invalid input syntax." in r.stderr
+    r = _cli("--bogus", "1")
+    assert r.returncode == 255 and "Error. Invalid input syntax. Unknown option --bogus" in r.stderr
+    r = _cli("-k", "abc")
+    assert r.returncode == 255 and "Error. Invalid kappa_tot input." in r.stderr
+    r = _cli("-s", "1.5")
+    assert r.returncode == 255 and "smooth_fraction must be < 1" in r.stderr
+    r = _cli("-s", "-0.5")
+    assert r.returncode == 255 and "smooth_fraction must be >= 0" in r.stderr
+
+
+def test_cli_validation_before_any_device_work():
+    r = _cli("-np", "7")
+    assert r.returncode == 255 and "Error. num_phi must be an even integer > 0" in r.stderr
+    r = _cli("-np", "10", "-nb", "3")
+    assert r.returncode == 255 and "Error. num_phi must be a multiple of 2 * num_branches" in r.stderr
+    r = _cli("-mf", "NOPE")
+    assert r.returncode == 255 and "Error. mass_function must be" in r.stderr
+    # -s is ignored when -ks is given (main_ccf.cu:229-238), -ks when -sf is
+    r = _cli("-ks", "0.9", "-s", "0.5", "-k", "0.3")
+    assert "kappa_star must be <= kappa_tot" in r.stderr
