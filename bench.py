@@ -159,6 +159,21 @@ def cpu_baseline(starfile: str, run_kw: dict, steps: int, threads: int, gpu_ccs:
     return res
 
 
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (two rocprofv3 --pmc passes, FETCH_SIZE and WRITE_SIZE, scaled by a copy
+    of known size); None when there is none for this workload."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_sums_hbm_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return json.loads(line.split(":", 1)[1]).get(kernel)
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 def summarize(times, infos):
     inf = infos[-1]
     t = float(np.mean(times))
@@ -186,11 +201,23 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--predictor", type=int, default=2, help="continuation predictor of the mixed form (0, 1, 2)")
     ap.add_argument("--ablation", action="store_true", help="also time FP64 + predictor and mixed without predictor")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device to device: the known-size kernel that scales the HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus must be 1 (the path would shard by branch, see DESIGN.md)",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.calibrate_copy > 0:
+        import torch
+
+        a = torch.ones(args.calibrate_copy << 18, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+        print(f"# copied {args.calibrate_copy} MiB device to device", file=sys.stderr, flush=True)
 
     W = WORKLOADS[args.workload]
     tmp = tempfile.mkdtemp(prefix="ccfbench_")
@@ -229,7 +256,9 @@ def main():
                      "FP64 VALU (v_fma_f64, v_rcp_f64); MI355X FP64 vector peak = FP64 matrix peak"),
             "kernel": "k_sums32" if mixed_head else "k_sums64",
             "achieved": head["sums_tflops"], "peak": pk, "unit": "TFLOP/s",
-            "frac": head["sums_tflops"] / pk, "traffic": None,
+            "frac": head["sums_tflops"] / pk,
+            "traffic": measured_traffic(args.workload, "k_sums32" if mixed_head else "k_sums64"),
+            "traffic_source": f"profiles/{args.workload}_sums_hbm_traffic.md",
             "avg_launch_ms": head["aberth_ms_avg"], "launches": head["aberth_launches"]}
     if "fp64" in res:
         roof["fp64_kernel"] = {"kernel": "k_sums64", "achieved": res["fp64"]["sums_tflops"], "peak": PEAK_FP64_TFLOPS,

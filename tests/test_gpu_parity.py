@@ -137,7 +137,16 @@ def test_run_matches_oracle(workdir, name, kw, gkw):
     if not gkw:
         assert nbad_c == 0, (worst_c, nbad_c)
     else:
-        assert nbad_c <= nbad and worst_c < 1e-6, (worst_c, nbad_c)
+        # the lens map amplifies a within-band root difference (|dw| reaches
+        # about 2|1 - kappa| |dz| on a critical curve), so the caustics are
+        # pinned pointwise instead: each one is the oracle's lens map of the
+        # product's own critical-curve point
+        okw = {k: v for k, v in kw.items() if k not in ("num_phi", "num_branches")}
+        o, _ = oracle.evaluate(oracle.params(starfile=sf, **okw), gccs.ravel(), 0.0)
+        w = o[:, 4] + 1j * o[:, 5]
+        rel = np.abs(gc.ravel() - w) / np.maximum(np.abs(w), 1.0)
+        assert rel.max() < 1e-11, rel.max()
+        assert nbad_c <= max(5, 1e-3 * R * (kw["num_phi"] + nb)) and worst_c < 1e-6, (worst_c, nbad_c)
     gm = io.read_array(gp + "ccf_mu_length_scales.bin", complex_values=False)
     assert gm.shape == gccs.shape and np.all(np.isfinite(gm)) and np.all(gm > 0)
     # parameter files agree except for the timings
