@@ -46,17 +46,19 @@ WORKLOADS = {
 }
 
 
-def run_gpu(starfile: str, run_kw: dict, precision: int, steps: int, warmup: int, predictor: int = 0):
+def run_gpu(starfile: str, run_kw: dict, precision: int, steps: int, warmup: int, predictor: int = 0,
+            root_fmm: int = 0, fmm_kw: dict | None = None):
     times, infos = [], []
     ctx = ccf_amd.Context(starfile=starfile, outfile_prefix="./", write_stars=0, write_critical_curves=0,
-                          write_caustics=1, precision=precision, timing=1, predictor=predictor, **run_kw)
+                          write_caustics=1, precision=precision, timing=1, predictor=predictor, root_fmm=root_fmm,
+                          **(fmm_kw or {}), **run_kw)
     for i in range(warmup + steps):
         ctx.stage("load")
         t0 = time.perf_counter()
         for st in ("build_tree_fmm", "find_initial_roots", "find_ccs", "find_caustics"):
             ctx.stage(st)
         dt = time.perf_counter() - t0
-        print(f"# precision={precision} predictor={predictor} step {i} ({'warmup' if i < warmup else 'timed'}): "
+        print(f"# precision={precision} predictor={predictor} root_fmm={root_fmm} step {i} ({'warmup' if i < warmup else 'timed'}): "
               f"{dt:.3f} s", file=sys.stderr, flush=True)
         if i >= warmup:
             times.append(dt)
@@ -184,6 +186,8 @@ def summarize(times, infos):
     return dict(points_per_s=conv / t, s_per_step=t, points_converged=conv, points_total=inf["points_total"],
                 t_tree=inf["t_tree"], t_init_roots=inf["t_init_roots"], t_ccs=inf["t_ccs"], t_caustics=inf["t_caustics"],
                 updates_full=inf["updates_full"], updates_early=inf["updates_early"], sweeps=inf["sweeps"],
+                root_fmm_updates=inf["root_fmm_updates"], root_fmm_rows=inf["root_fmm_rows"],
+                root_fmm_order=inf["root_fmm_order"], root_fmm_leaf_level=inf["root_fmm_leaf_level"],
                 cluster_chunks=inf["cluster_chunks"],
                 aberth_ms_total=k_ms, aberth_launches=launches, aberth_ms_avg=k_ms / launches,
                 sums_tflops=achieved, sums_flops=inf["flops_sums"], update_flops=inf["flops_update"],
@@ -200,7 +204,11 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8, help="phi steps per side in the CPU sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--predictor", type=int, default=2, help="continuation predictor of the mixed form (0, 1, 2)")
-    ap.add_argument("--ablation", action="store_true", help="also time FP64 + predictor and mixed without predictor")
+    ap.add_argument("--root-fmm", type=int, default=1, help="root sums of dense sweeps by the FMM in the mixed form")
+    ap.add_argument("--root-fmm-order", type=int, default=0, help="its expansion order (0: the library default)")
+    ap.add_argument("--root-fmm-leaf", type=int, default=0, help="its mean roots per leaf cell (0: the library default)")
+    ap.add_argument("--ablation", action="store_true",
+                    help="also time FP64 + predictor, mixed without predictor, mixed with direct root sums")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device: the known-size kernel that scales the HBM byte counters")
     args = ap.parse_args()
@@ -227,16 +235,21 @@ def main():
 
     res = {}
     ccs = {}
-    # plain FP64 path = the reference's method; mixed form = FP32 Cauchy sums,
-    # FMM pole sum, continuation predictor. --ablation adds the two halves.
-    modes = {"fp64": [("fp64", 0, 0)], "mixed": [("mixed", 1, args.predictor)],
-             "both": [("fp64", 0, 0), ("mixed", 1, args.predictor)]}[args.precision]
+    # plain FP64 path = the reference's method; mixed form = FMM root sums of
+    # the dense sweeps (FP64 near field), FP32 direct sums otherwise, FMM
+    # pole sum, continuation predictor. --ablation takes the parts away.
+    mixed = ("mixed", 1, args.predictor, args.root_fmm)
+    modes = {"fp64": [("fp64", 0, 0, 0)], "mixed": [mixed],
+             "both": [("fp64", 0, 0, 0), mixed]}[args.precision]
     if args.ablation:
-        modes += [("fp64_predictor", 0, args.predictor), ("mixed_no_predictor", 1, 0)]
-    for name, prec, pred in modes:
-        times, infos, ccs[name] = run_gpu(sf, run_kw, prec, args.steps, args.warmup, pred)
+        modes += [("fp64_predictor", 0, args.predictor, 0), ("mixed_no_predictor", 1, 0, args.root_fmm),
+                  ("mixed_direct_root_sums", 1, args.predictor, 0)]
+    for name, prec, pred, rfmm in modes:
+        fmm_kw = dict(root_fmm_order=args.root_fmm_order, root_fmm_leaf=args.root_fmm_leaf)
+        times, infos, ccs[name] = run_gpu(sf, run_kw, prec, args.steps, args.warmup, pred, rfmm, fmm_kw)
         res[name] = summarize(times, infos)
         res[name]["predictor"] = pred
+        res[name]["root_fmm"] = rfmm
     R = res[next(iter(res))]["num_roots"]
     if "fp64" in ccs:
         for name in res:

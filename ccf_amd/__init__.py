@@ -60,6 +60,9 @@ class Params(ctypes.Structure):
         ("roots_per_thread", ctypes.c_int),
         ("timing", ctypes.c_int),
         ("predictor", ctypes.c_int),
+        ("root_fmm", ctypes.c_int),
+        ("root_fmm_order", ctypes.c_int),
+        ("root_fmm_leaf", ctypes.c_int),
     ]
 
 
@@ -95,6 +98,10 @@ class Info(ctypes.Structure):
         ("sweeps", ctypes.c_longlong),
         ("cluster_chunks", ctypes.c_longlong),
         ("flops_sums", ctypes.c_double),
+        ("root_fmm_updates", ctypes.c_longlong),
+        ("root_fmm_rows", ctypes.c_longlong),
+        ("root_fmm_order", ctypes.c_int),
+        ("root_fmm_leaf_level", ctypes.c_int),
     ]
 
     def as_dict(self):
@@ -138,6 +145,7 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         "ccf_last_error": (ctypes.c_char_p, [vp]),
         "ccf_eval_points": (ctypes.c_int, [vp, dp, ctypes.c_int, ctypes.c_double, dp]),
         "ccf_aberth_sweep": (ctypes.c_int, [vp, dp, ctypes.c_int, ctypes.c_double, dp]),
+        "ccf_root_sums": (ctypes.c_int, [vp, dp, ctypes.c_int, dp]),
         "ccf_write_synthetic_stars": (ctypes.c_int, [P, ctypes.c_ulonglong, ctypes.c_char_p]),
         "ccf_mass_function": (ctypes.c_double, [ctypes.c_char_p, ctypes.c_int, ctypes.c_double,
                                                 ctypes.c_double, ctypes.c_double, ctypes.c_double]),
@@ -272,6 +280,14 @@ class Context:
         out = np.empty_like(r)
         self._check(self.lib.ccf_aberth_sweep(self.ptr, _dptr(r.view(np.float64)), r.size, phi,
                                               _dptr(out.view(np.float64))), "ccf_aberth_sweep")
+        return out
+
+    def root_sums(self, roots) -> np.ndarray:
+        """sum_{j != k} 1/(z_k - z_j) for every root, by the root-sum FMM."""
+        r = np.ascontiguousarray(np.asarray(roots, dtype=np.complex128).ravel())
+        out = np.empty_like(r)
+        self._check(self.lib.ccf_root_sums(self.ptr, _dptr(r.view(np.float64)), r.size, _dptr(out.view(np.float64))),
+                    "ccf_root_sums")
         return out
 
 

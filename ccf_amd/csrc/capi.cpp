@@ -159,6 +159,10 @@ bool check_input_params(ccf_ctx* c)
 		return err("Error. roots_per_thread must be 0 (automatic), 1, 2 or 4.\n");
 	if (P.init_iters < 0 || P.step_iters < 0) return err("Error. iteration caps must be >= 0\n");
 	if (P.predictor < 0 || P.predictor > 2) return err("Error. predictor must be 0, 1 or 2.\n");
+	if (P.root_fmm != 0 && P.root_fmm != 1) return err("Error. root_fmm must be 1 (true) or 0 (false).\n");
+	if (P.root_fmm_order < 0 || P.root_fmm_order > ccf::MAX_ROOT_FMM_ORDER || (P.root_fmm_order > 0 && P.root_fmm_order < 3))
+		return err("Error. root_fmm_order must be 0 (automatic) or in [3, " + std::to_string(ccf::MAX_ROOT_FMM_ORDER) + "]\n");
+	if (P.root_fmm_leaf < 0) return err("Error. root_fmm_leaf must be >= 0\n");
 	return true;
 }
 
@@ -889,6 +893,12 @@ int ccf_aberth_sweep(ccf_ctx* c, const double* roots, int n, double phi, double*
 {
 	if (!c || !need(c, c->prepared, "ccf_prepare")) return 1;
 	return ccf::aberth_sweep_dev(c, roots, n, phi, out) ? 0 : 1;
+}
+
+int ccf_root_sums(ccf_ctx* c, const double* roots, int n, double* out)
+{
+	if (!c || !need(c, c->prepared, "ccf_prepare")) return 1;
+	return ccf::root_sums_dev(c, roots, n, out) ? 0 : 1;
 }
 
 double ccf_mass_function(const char* name, int which, double p, double m_lower, double m_upper, double m_solar)

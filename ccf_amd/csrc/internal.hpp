@@ -22,6 +22,8 @@ constexpr int MAX_SAME_LEVEL = 27;       // tree_node.cuh:24
 constexpr int MAX_DIFF_LEVEL = 5;        // tree_node.cuh:25
 constexpr int MAX_NEAR_RANGES = MAX_NEIGHBORS + 1;
 constexpr int MAX_TAYLOR_SMOOTH = 101;   // ccf.cuh:79
+constexpr int MAX_ROOT_FMM_ORDER = 24;
+constexpr int MAX_ROOT_GRID_LEVELS = 10;
 
 // Quadtree over the stars, structure of arrays indexed by node id. Nodes of
 // a level are contiguous ([level_off[l], level_off[l] + level_cnt[l])) and
@@ -97,6 +99,20 @@ struct Timer
 	hipEvent_t a = nullptr, b = nullptr;
 };
 
+// Uniform quadtree over the field for the root-sum FMM. Level l is a grid of
+// (nx << l) x (ny << l) square cells of side s0 / 2^l whose lower-left corner
+// is (x0, y0); its cells are numbered row-major from off[l].
+struct RootGrid
+{
+	int nx = 0, ny = 0;  // top-level cells
+	int leaf = 0;        // leaf level
+	int order = 0;       // expansion order (0: no FMM)
+	int ncells = 0;      // cells of all levels
+	int nleaf = 0;       // leaf cells
+	double x0 = 0, y0 = 0, s0 = 0;
+	int off[MAX_ROOT_GRID_LEVELS + 1] = {};
+};
+
 // Work buffers of the Aberth sweeps over up to `rows` rows of R roots.
 struct SweepBufs
 {
@@ -105,12 +121,29 @@ struct SweepBufs
 	int* cnt = nullptr;      // [2][rows]
 	int* fl = nullptr;       // [rows][R] roots that need the Cauchy sums this sweep
 	int* fcnt = nullptr;     // [2][rows]
+	int* dl = nullptr;       // [rows][R] roots outside the tree, deferred to the cooperative Witt kernel
+	int* dcnt = nullptr;     // [2][rows]
 	double2* f0 = nullptr;   // [rows][R] Witt function at the roots of fl
 	double2* f1 = nullptr;   // [rows][R] its z-derivative
 	double2* ex = nullptr;   // [rows][R] terms of the Aberth sum formed per root
 	int* tile_off = nullptr; // [rows + 1] first tile of each row
 	int* plan = nullptr;     // [4] tiles, slices per tile
 	double2* part = nullptr; // [max_items][tile] partial Cauchy sums
+	// root-sum FMM (grid.order > 0 only)
+	RootGrid grid;
+	int* slot = nullptr;     // [rows][R] position of root a in fl this sweep, -1 otherwise
+	int* rmode = nullptr;    // [rows] 1: the row's root sums of this sweep come from the FMM
+	double2* rsum = nullptr; // [rows][R] FMM root sums, indexed like fl
+	int* ccnt = nullptr;     // [rows][ncells] roots in each cell's subtree
+	int* cstart = nullptr;   // [rows][nleaf + 1] first sorted position of each leaf cell
+	int* ccur = nullptr;     // [rows][nleaf] counts, then scatter cursors
+	int* cell = nullptr;     // [rows][R] leaf cell of each root, -1 outside the grid
+	int* sidx = nullptr;     // [rows][R] root indices sorted by leaf cell
+	double2* spos = nullptr; // [rows][R] their positions
+	int* ocnt = nullptr;     // [rows] roots outside the grid
+	int* oidx = nullptr;     // [rows][R] their indices
+	double2* mp = nullptr;   // [rows][ncells][order + 1] multipole coefficients
+	double2* lc = nullptr;   // [rows][ncells][order + 1] local coefficients
 };
 
 } // namespace ccf
@@ -188,4 +221,5 @@ bool find_caustics_dev(ccf_ctx* ctx, bool caustics, bool mu);
 bool generate_stars_dev(ccf_ctx* ctx);
 bool eval_points_dev(ccf_ctx* ctx, const double* z, int n, double phi, double* out);
 bool aberth_sweep_dev(ccf_ctx* ctx, const double* roots, int n, double phi, double* out);
+bool root_sums_dev(ccf_ctx* ctx, const double* roots, int n, double* out);
 } // namespace ccf

@@ -243,11 +243,12 @@ struct Witt
 	Far fr;
 };
 
-__device__ inline Witt witt_f0(const LensView& L, cd z, double phi)
+// the rest of witt_f0 once the near-field sums of z's node are known
+__device__ inline Witt witt_from(const LensView& L, cd z, double phi, int node, const Near& nr)
 {
 	Witt w;
-	w.node = nearest_node(L, z);
-	w.nr = p2p(L, w.node, z);
+	w.node = node;
+	w.nr = nr;
 	w.fr = l2p(L, w.node, z);
 	cd d1, d2;
 	smooth_dzbar(L, z, d1, d2);
@@ -259,6 +260,12 @@ __device__ inline Witt witt_f0(const LensView& L, cd z, double phi)
 	const double a0 = w.f0.abs();
 	w.early = fabs(a0 * (a0 + 2 * w.a)) < 1e-9 && fabs(a0 * (a0 - 2 * w.a)) < 1e-9;
 	return w;
+}
+
+__device__ inline Witt witt_f0(const LensView& L, cd z, double phi)
+{
+	const int node = nearest_node(L, z);
+	return witt_from(L, z, phi, node, p2p(L, node, z));
 }
 
 // f1 = d f0 / dz from the sums already formed for f0 (ccf_functions.cuh:108-121)

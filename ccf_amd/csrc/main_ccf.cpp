@@ -45,6 +45,7 @@ const std::string OPTS[] = {
 	"-o", "--outfile_prefix",
 	"--fmm_order", "--direct", "--binomial_exact", "--precision",
 	"--init_iters", "--step_iters", "--roots_per_thread", "--predictor",
+	"--root_fmm", "--root_fmm_order", "--root_fmm_leaf",
 };
 
 int verbose = 1;
@@ -137,7 +138,11 @@ void usage(const char* name)
 		<< "  --roots_per_thread         Aberth tile roots per thread (1, 2, 4; 0 automatic).\n"
 		<< "  --predictor                Initial roots of each phi step: 0 the previous\n"
 		<< "                             step's roots; 1 linear, 2 quadratic extrapolation.\n"
-		<< "                             Default value: 0\n";
+		<< "                             Default value: 0\n"
+		<< "  --root_fmm                 Root sums of dense sweeps by a fast multipole\n"
+		<< "                             method (1) or directly (0). Default value: 0\n"
+		<< "  --root_fmm_order           Expansion order of the root-sum FMM (0 automatic).\n"
+		<< "  --root_fmm_leaf            Mean roots per leaf cell of its grid (0 automatic).\n";
 }
 
 template <typename V, typename F>
@@ -269,6 +274,9 @@ int main(int argc, char* argv[])
 		else if (o == "--step_iters") ok = parse("step_iters", in, ccf.step_iters, to_i);
 		else if (o == "--roots_per_thread") ok = parse("roots_per_thread", in, ccf.roots_per_thread, to_i);
 		else if (o == "--predictor") ok = parse("predictor", in, ccf.predictor, to_i);
+		else if (o == "--root_fmm") ok = parse("root_fmm", in, ccf.root_fmm, to_i);
+		else if (o == "--root_fmm_order") ok = parse("root_fmm_order", in, ccf.root_fmm_order, to_i);
+		else if (o == "--root_fmm_leaf") ok = parse("root_fmm_leaf", in, ccf.root_fmm_leaf, to_i);
 		if (!ok) return -1;
 	}
 	if (!has_sf && !has_ks && (exists(b, e, "-s") || exists(b, e, "--smooth_fraction")))

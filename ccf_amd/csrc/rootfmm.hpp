@@ -1,0 +1,595 @@
+// Root sums of an Aberth sweep by a fast multipole method.
+//
+// Every full Aberth update needs S_k = sum_{j != k} 1/(z_k - z_j) over the R
+// roots of its row (ccf_functions.cuh:232-239), O(R^2) per row and sweep.
+// For the rows of a sweep in which many roots are active this file forms
+// the sums in O(R p^2 / leaf + R leaf) instead:
+//
+//  * The field is covered by a uniform quadtree: the top level is a grid of
+//    nx x ny square cells of side s0 (one square for a disc, a strip of
+//    squares for an elongated rectangle) and level l splits every top cell
+//    into 2^l x 2^l. Its geometry is fixed per field; the roots of each row
+//    are binned into the leaf cells at every sweep (counting sort, then a
+//    sort by root index inside each cell so that sums are deterministic).
+//  * Expansions are of the Cauchy kernel 1/(z - w), scaled by the half side
+//    h of the cell's level (Greengard & Rokhlin 1987, written for the
+//    complex kernel rather than for log|z - w|):
+//      multipole   A_n = sum_j ((z_j - c)/h)^n,
+//                  sum_j 1/(z - z_j) = (1/h) sum_n A_n / w^(n+1),  w = (z - c)/h
+//      local       sum_far 1/(z - z_j) = sum_m L_m w^m
+//      M2M         A_n(parent) = sum_k C(n,k) 2^-k A_k(child) e^(n-k),  e = (c_child - c_parent)/h_parent
+//      M2L         L_m += (-1)^m / h sum_n C(n+m,m) A_n d^-(n+m+1),    d = (c_target - c_source)/h
+//      L2L         L_k(child) = 2^-k sum_{m>=k} C(m,k) L_m(parent) e^(m-k)
+//    The interaction list of a cell at level >= 1 is the set of children of
+//    its parent's neighbours that are not its own neighbours; at level 0 it
+//    is every top cell that is not a neighbour. A leaf's near field is its
+//    3 x 3 block of leaf cells, summed directly in FP64 (the own root gives
+//    a zero difference and, as in the direct kernels, a zero term).
+//  * Roots outside the grid are summed directly, as sources and as targets.
+//
+// The Aberth map z - f0 / (f1 + f0 S) has the roots of f0 as fixed points
+// whatever S is; the truncation error of the far field (at most about
+// 0.71^(p+1) of the far part, for the closest well-separated cells, far less
+// on average) steers the iteration, it does not move the roots it reaches.
+
+constexpr int RB = 2 * MAX_ROOT_FMM_ORDER; // binomial rows C(n, k), n <= 2p
+constexpr int ROOT_FMM_MIN_ROOTS = 64;     // smaller rows are always summed directly
+// sweeps of a phase that may use the FMM: the dense ones (the initial solve
+// converges over a few dozen sweeps; a continuation step mostly in one or two)
+constexpr int ROOT_FMM_INIT_SWEEPS = 24;
+constexpr int ROOT_FMM_STEP_SWEEPS = 2;
+
+// listed roots from which a row's sums are cheaper by the FMM (its build
+// costs about as much as R/32 direct updates)
+inline int root_fmm_min(int R) { return std::max(32, R / 32); }
+__constant__ double c_rbinom[(RB + 1) * (RB + 1)];
+
+__device__ inline double rbinom(int n, int k) { return c_rbinom[n * (RB + 1) + k]; }
+
+struct RootView
+{
+	RootGrid g;
+	int* slot;
+	int* rmode;
+	double2* rsum;
+	int* ccnt;
+	int* cstart;
+	int* ccur;
+	int* cell;
+	int* sidx;
+	double2* spos;
+	int* ocnt;
+	int* oidx;
+	double2* mp;
+	double2* lc;
+};
+
+RootView root_view(const SweepBufs& B)
+{
+	return RootView{B.grid, B.slot, B.rmode, B.rsum, B.ccnt, B.cstart, B.ccur, B.cell, B.sidx, B.spos, B.ocnt, B.oidx, B.mp, B.lc};
+}
+
+__device__ inline cd cell_center(const RootGrid& g, int l, int ix, int iy)
+{
+	const double s = ldexp(g.s0, -l);
+	return cd(g.x0 + (ix + 0.5) * s, g.y0 + (iy + 0.5) * s);
+}
+
+// counts and the outlier count of the rows of this sweep
+__global__ void k_rzero(RootView V)
+{
+	const int r = blockIdx.y;
+	if (!V.rmode[r]) return;
+	const int c = blockIdx.x * blockDim.x + threadIdx.x;
+	if (c < V.g.nleaf) V.ccur[(size_t)r * V.g.nleaf + c] = 0;
+	if (c == 0) V.ocnt[r] = 0;
+}
+
+// leaf cell of every root of the row; roots outside the grid (or not finite)
+// go to the row's outlier list
+__global__ __launch_bounds__(BLOCK) void k_rbin(RootView V, const double2* __restrict__ roots, RowMap M)
+{
+	const int r = blockIdx.y;
+	if (!V.rmode[r]) return;
+	const int R = M.R;
+	const int j = blockIdx.x * BLOCK + threadIdx.x;
+	bool out = false;
+	if (j < R)
+	{
+		int row;
+		double phi;
+		M.get(r, row, phi);
+		const double2 z = roots[(size_t)row * R + j];
+		const double sl = ldexp(V.g.s0, -V.g.leaf);
+		const double fx = (z.x - V.g.x0) / sl, fy = (z.y - V.g.y0) / sl;
+		const int nxl = V.g.nx << V.g.leaf, nyl = V.g.ny << V.g.leaf;
+		if (fx >= 0 && fx < nxl && fy >= 0 && fy < nyl)
+		{
+			const int c = (int)fy * nxl + (int)fx;
+			V.cell[(size_t)r * R + j] = c;
+			atomicAdd(&V.ccur[(size_t)r * V.g.nleaf + c], 1);
+		}
+		else
+		{
+			V.cell[(size_t)r * R + j] = -1;
+			out = true;
+		}
+	}
+	const int pos = wave_append(&V.ocnt[r], out);
+	if (out) V.oidx[(size_t)r * R + pos] = j;
+}
+
+// exclusive prefix of the leaf counts: first sorted position of each cell
+__global__ __launch_bounds__(1024) void k_rscan(RootView V, unsigned long long* __restrict__ stats)
+{
+	const int r = blockIdx.x;
+	if (!V.rmode[r]) return;
+	if (threadIdx.x == 0) atomicAdd(&stats[C_RROWS], 1ull);
+	__shared__ int wsum[16];
+	__shared__ int carry;
+	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+	const int nl = V.g.nleaf;
+	int* cnt = V.ccur + (size_t)r * nl;
+	int* st = V.cstart + (size_t)r * (nl + 1);
+	if (threadIdx.x == 0) carry = 0;
+	__syncthreads();
+	for (int base = 0; base < nl; base += blockDim.x)
+	{
+		const int c = base + threadIdx.x;
+		const int v = c < nl ? cnt[c] : 0;
+		int x = v;
+		for (int d = 1; d < 64; d <<= 1)
+		{
+			const int y = __shfl_up(x, d, 64);
+			if (lane >= d) x += y;
+		}
+		if (lane == 63) wsum[w] = x;
+		__syncthreads();
+		int woff = 0, tot = 0;
+		for (int k = 0; k < nw; k++)
+		{
+			if (k < w) woff += wsum[k];
+			tot += wsum[k];
+		}
+		if (c < nl)
+		{
+			const int e = carry + woff + x - v;
+			st[c] = e;
+			cnt[c] = e;
+		}
+		__syncthreads();
+		if (threadIdx.x == 0) carry += tot;
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) st[nl] = carry;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_rscatter(RootView V, int R)
+{
+	const int r = blockIdx.y;
+	if (!V.rmode[r]) return;
+	const int j = blockIdx.x * BLOCK + threadIdx.x;
+	if (j >= R) return;
+	const int c = V.cell[(size_t)r * R + j];
+	if (c >= 0)
+	{
+		const int pos = atomicAdd(&V.ccur[(size_t)r * V.g.nleaf + c], 1);
+		V.sidx[(size_t)r * R + pos] = j;
+	}
+}
+
+// per leaf cell: the cell's roots in index order, their positions copied
+// contiguously, and the multipole expansion (P2M)
+template <int P>
+__global__ __launch_bounds__(BLOCK) void k_rleaf(RootView V, const double2* __restrict__ roots, RowMap M)
+{
+	const int r = blockIdx.y;
+	if (!V.rmode[r]) return;
+	const int c = blockIdx.x * BLOCK + threadIdx.x;
+	if (c >= V.g.nleaf) return;
+	const int R = M.R;
+	int row;
+	double phi;
+	M.get(r, row, phi);
+	const int* st = V.cstart + (size_t)r * (V.g.nleaf + 1);
+	const int b = st[c], e = st[c + 1], n = e - b;
+	const int L = V.g.leaf;
+	const size_t cid = (size_t)r * V.g.ncells + V.g.off[L] + c;
+	V.ccnt[cid] = n;
+	if (n == 0) return;
+	int* ix = V.sidx + (size_t)r * R;
+	if (n <= 128)
+	{
+		for (int q = b + 1; q < e; q++)
+		{
+			const int v = ix[q];
+			int t = q - 1;
+			while (t >= b && ix[t] > v)
+			{
+				ix[t + 1] = ix[t];
+				t--;
+			}
+			ix[t + 1] = v;
+		}
+	}
+	const int nxl = V.g.nx << L;
+	const cd cc = cell_center(V.g, L, c % nxl, c / nxl);
+	const double ih = 2.0 / ldexp(V.g.s0, -L);
+	cd A[P + 1];
+#pragma unroll
+	for (int k = 0; k <= P; k++) A[k] = cd(0, 0);
+	for (int q = b; q < e; q++)
+	{
+		const double2 z = roots[(size_t)row * R + ix[q]];
+		V.spos[(size_t)r * R + q] = z;
+		const cd w = (cd(z.x, z.y) - cc) * ih;
+		cd t(1, 0);
+#pragma unroll
+		for (int k = 0; k <= P; k++)
+		{
+			A[k] += t;
+			t = t * w;
+		}
+	}
+	double2* out = V.mp + cid * (P + 1);
+#pragma unroll
+	for (int k = 0; k <= P; k++) out[k] = make_double2(A[k].re, A[k].im);
+}
+
+// M2M: level l from its children at level l + 1
+template <int P>
+__global__ __launch_bounds__(BLOCK) void k_rm2m(RootView V, int l)
+{
+	const int r = blockIdx.y;
+	if (!V.rmode[r]) return;
+	const int nxl = V.g.nx << l, nyl = V.g.ny << l;
+	const int c = blockIdx.x * BLOCK + threadIdx.x;
+	if (c >= nxl * nyl) return;
+	const int ix = c % nxl, iy = c / nxl, nxc = 2 * nxl;
+	const size_t base = (size_t)r * V.g.ncells;
+	int tot = 0;
+	cd A[P + 1];
+#pragma unroll
+	for (int k = 0; k <= P; k++) A[k] = cd(0, 0);
+	for (int q = 0; q < 4; q++)
+	{
+		const int a = q & 1, b = q >> 1;
+		const size_t ch = base + V.g.off[l + 1] + (size_t)(2 * iy + b) * nxc + 2 * ix + a;
+		const int n = V.ccnt[ch];
+		if (n == 0) continue;
+		tot += n;
+		const double2* src = V.mp + ch * (P + 1);
+		cd ak[P + 1], ep[P + 1];
+		const cd e(a - 0.5, b - 0.5);
+		double sc = 1;
+		ep[0] = cd(1, 0);
+#pragma unroll
+		for (int k = 0; k <= P; k++)
+		{
+			ak[k] = cd(src[k].x * sc, src[k].y * sc);
+			sc *= 0.5;
+			if (k > 0) ep[k] = ep[k - 1] * e;
+		}
+#pragma unroll
+		for (int m = 0; m <= P; m++)
+		{
+#pragma unroll
+			for (int k = 0; k <= m; k++) A[m] += rbinom(m, k) * (ak[k] * ep[m - k]);
+		}
+	}
+	const size_t cid = base + V.g.off[l] + c;
+	V.ccnt[cid] = tot;
+	if (tot == 0) return;
+	double2* out = V.mp + cid * (P + 1);
+#pragma unroll
+	for (int k = 0; k <= P; k++) out[k] = make_double2(A[k].re, A[k].im);
+}
+
+// L2L from the parent and M2L from the interaction list, level l
+template <int P>
+__global__ __launch_bounds__(BLOCK) void k_rdown(RootView V, int l, unsigned long long* __restrict__ stats)
+{
+	const int r = blockIdx.y;
+	if (!V.rmode[r]) return;
+	const int nxl = V.g.nx << l, nyl = V.g.ny << l;
+	const int c = blockIdx.x * BLOCK + threadIdx.x;
+	unsigned long long nt = 0;
+	const size_t base = (size_t)r * V.g.ncells;
+	const size_t cid = base + V.g.off[l] + c;
+	if (c < nxl * nyl && V.ccnt[cid] > 0)
+	{
+		const int ix = c % nxl, iy = c / nxl;
+		cd Lc[P + 1];
+#pragma unroll
+		for (int k = 0; k <= P; k++) Lc[k] = cd(0, 0);
+		if (l >= 1)
+		{
+			const double2* lp = V.lc + (base + V.g.off[l - 1] + (size_t)(iy >> 1) * (nxl >> 1) + (ix >> 1)) * (P + 1);
+			cd Lp[P + 1], ep[P + 1];
+			const cd e((ix & 1) - 0.5, (iy & 1) - 0.5);
+			ep[0] = cd(1, 0);
+#pragma unroll
+			for (int m = 0; m <= P; m++)
+			{
+				Lp[m] = cd(lp[m].x, lp[m].y);
+				if (m > 0) ep[m] = ep[m - 1] * e;
+			}
+			double sc = 1;
+#pragma unroll
+			for (int k = 0; k <= P; k++)
+			{
+				cd s(0, 0);
+#pragma unroll
+				for (int m = k; m <= P; m++) s += rbinom(m, k) * (Lp[m] * ep[m - k]);
+				Lc[k] = s * sc;
+				sc *= 0.5;
+			}
+		}
+		const double ih = 2.0 / ldexp(V.g.s0, -l);
+		auto m2l = [&](int jx, int jy) {
+			const size_t sid = base + V.g.off[l] + (size_t)jy * nxl + jx;
+			if (V.ccnt[sid] == 0) return;
+			nt++;
+			const double2* src = V.mp + sid * (P + 1);
+			const cd rho = cd(2.0 * (ix - jx), 2.0 * (iy - jy)).inv();
+			cd bq[P + 1];
+			cd t(1, 0);
+#pragma unroll
+			for (int n = 0; n <= P; n++)
+			{
+				bq[n] = cd(src[n].x, src[n].y) * t;
+				t = t * rho;
+			}
+			cd rp = rho * ih;
+#pragma unroll
+			for (int m = 0; m <= P; m++)
+			{
+				double sx = 0, sy = 0;
+#pragma unroll
+				for (int n = 0; n <= P; n++)
+				{
+					const double bc = rbinom(n + m, m);
+					sx = fma(bc, bq[n].re, sx);
+					sy = fma(bc, bq[n].im, sy);
+				}
+				const cd v = rp * cd(sx, sy);
+				if (m & 1) Lc[m] -= v;
+				else Lc[m] += v;
+				rp = rp * rho;
+			}
+		};
+		if (l == 0)
+		{
+			for (int jy = 0; jy < nyl; jy++)
+				for (int jx = 0; jx < nxl; jx++)
+					if (abs(jx - ix) >= 2 || abs(jy - iy) >= 2) m2l(jx, jy);
+		}
+		else
+		{
+			const int px = ix >> 1, py = iy >> 1;
+			for (int qy = max(py - 1, 0); qy <= min(py + 1, (nyl >> 1) - 1); qy++)
+				for (int qx = max(px - 1, 0); qx <= min(px + 1, (nxl >> 1) - 1); qx++)
+					for (int q = 0; q < 4; q++)
+					{
+						const int jx = 2 * qx + (q & 1), jy = 2 * qy + (q >> 1);
+						if (abs(jx - ix) >= 2 || abs(jy - iy) >= 2) m2l(jx, jy);
+					}
+		}
+		double2* out = V.lc + cid * (P + 1);
+#pragma unroll
+		for (int k = 0; k <= P; k++) out[k] = make_double2(Lc[k].re, Lc[k].im);
+	}
+	nt = wave_sum(nt);
+	if ((threadIdx.x & 63) == 0 && nt) atomicAdd(&stats[C_RM2L], nt);
+}
+
+// per listed root of a row in FMM mode, in cell order: far field by L2P,
+// near field and outliers directly; rsum = -S (the root part of the
+// Aberth correction, the sign that the direct kernels' partials carry)
+template <int P>
+__global__ __launch_bounds__(BLOCK) void k_rsum(RootView V, const double2* __restrict__ roots, RowMap M,
+	unsigned long long* __restrict__ stats)
+{
+	const int r = blockIdx.y;
+	if (!V.rmode[r]) return;
+	const int R = M.R;
+	const int k = blockIdx.x * BLOCK + threadIdx.x;
+	const int* st = V.cstart + (size_t)r * (V.g.nleaf + 1);
+	unsigned long long nn = 0, nt = 0;
+	if (k < st[V.g.nleaf])
+	{
+		const int j = V.sidx[(size_t)r * R + k];
+		const int s = V.slot[(size_t)r * R + j];
+		if (s >= 0)
+		{
+			const double2 zz = V.spos[(size_t)r * R + k];
+			const cd z(zz.x, zz.y);
+			const int L = V.g.leaf, nxl = V.g.nx << L, nyl = V.g.ny << L;
+			const int c = V.cell[(size_t)r * R + j];
+			const int ix = c % nxl, iy = c / nxl;
+			const cd w = (z - cell_center(V.g, L, ix, iy)) * (2.0 / ldexp(V.g.s0, -L));
+			const double2* lp = V.lc + ((size_t)r * V.g.ncells + V.g.off[L] + c) * (P + 1);
+			cd S(lp[P].x, lp[P].y);
+#pragma unroll
+			for (int m = P - 1; m >= 0; m--) S = S * w + cd(lp[m].x, lp[m].y);
+			double ax = 0, ay = 0;
+			const double2* sp = V.spos + (size_t)r * R;
+			for (int y = max(iy - 1, 0); y <= min(iy + 1, nyl - 1); y++)
+			{
+				const int q0 = st[y * nxl + max(ix - 1, 0)], q1 = st[y * nxl + min(ix + 1, nxl - 1) + 1];
+				nn += q1 - q0;
+				for (int q = q0; q < q1; q++)
+				{
+					const double2 u = sp[q];
+					const double dx = z.re - u.x, dy = z.im - u.y;
+					const double iv = rcp64(fmax(fma(dx, dx, dy * dy), 1e-300));
+					ax = fma(dx, iv, ax);
+					ay = fma(dy, iv, ay);
+				}
+			}
+			const int no = V.ocnt[r];
+			if (no > 0)
+			{
+				int row;
+				double phi;
+				M.get(r, row, phi);
+				for (int o = 0; o < no; o++)
+				{
+					const double2 u = roots[(size_t)row * R + V.oidx[(size_t)r * R + o]];
+					const double dx = z.re - u.x, dy = z.im - u.y;
+					const double iv = rcp64(fmax(fma(dx, dx, dy * dy), 1e-300));
+					ax = fma(dx, iv, ax);
+					ay = fma(dy, iv, ay);
+				}
+			}
+			S += cd(ax, -ay);
+			V.rsum[(size_t)r * R + s] = make_double2(-S.re, -S.im);
+			nt = 1;
+		}
+	}
+	nn = wave_sum(nn);
+	nt = wave_sum(nt);
+	if ((threadIdx.x & 63) == 0)
+	{
+		if (nn) atomicAdd(&stats[C_RNEAR], nn);
+		if (nt) atomicAdd(&stats[C_RTGT], nt);
+	}
+}
+
+// listed roots outside the grid: direct sums over the whole row, one block
+// per row, one root at a time
+__global__ __launch_bounds__(BLOCK) void k_rsum_out(RootView V, const double2* __restrict__ roots, RowMap M,
+	unsigned long long* __restrict__ stats)
+{
+	const int r = blockIdx.x;
+	if (!V.rmode[r]) return;
+	const int no = V.ocnt[r];
+	if (no == 0) return;
+	const int R = M.R;
+	int row;
+	double phi;
+	M.get(r, row, phi);
+	const double2* zr = roots + (size_t)row * R;
+	__shared__ double red[2][BLOCK / 64];
+	for (int o = 0; o < no; o++)
+	{
+		const int j = V.oidx[(size_t)r * R + o];
+		const int s = V.slot[(size_t)r * R + j];
+		if (s < 0) continue;
+		const double2 z = zr[j];
+		double ax = 0, ay = 0;
+		for (int q = threadIdx.x; q < R; q += BLOCK)
+		{
+			if (q == j) continue;
+			const double dx = z.x - zr[q].x, dy = z.y - zr[q].y;
+			const double iv = rcp64(fmax(fma(dx, dx, dy * dy), 1e-300));
+			ax = fma(dx, iv, ax);
+			ay = fma(dy, iv, ay);
+		}
+		for (int d = 32; d >= 1; d >>= 1)
+		{
+			ax += __shfl_xor(ax, d, 64);
+			ay += __shfl_xor(ay, d, 64);
+		}
+		if ((threadIdx.x & 63) == 0)
+		{
+			red[0][threadIdx.x >> 6] = ax;
+			red[1][threadIdx.x >> 6] = ay;
+		}
+		__syncthreads();
+		if (threadIdx.x == 0)
+		{
+			double sx = 0, sy = 0;
+			for (int w = 0; w < BLOCK / 64; w++)
+			{
+				sx += red[0][w];
+				sy += red[1][w];
+			}
+			V.rsum[(size_t)r * R + s] = make_double2(-sx, sy);
+			atomicAdd(&stats[C_RTGT], 1ull);
+		}
+		__syncthreads();
+	}
+}
+
+// the FMM pass of one sweep over nrows rows (rows not in FMM mode exit at once)
+template <int P>
+void root_fmm_pass_p(const SweepBufs& B, const double2* roots, const RowMap& M, int nrows, unsigned long long* stats, hipStream_t s)
+{
+	const RootView V = root_view(B);
+	const RootGrid& g = B.grid;
+	const int R = M.R;
+	k_rzero<<<dim3(grid(g.nleaf), nrows), BLOCK, 0, s>>>(V);
+	k_rbin<<<dim3(grid(R), nrows), BLOCK, 0, s>>>(V, roots, M);
+	k_rscan<<<nrows, 1024, 0, s>>>(V, stats);
+	k_rscatter<<<dim3(grid(R), nrows), BLOCK, 0, s>>>(V, R);
+	k_rleaf<P><<<dim3(grid(g.nleaf), nrows), BLOCK, 0, s>>>(V, roots, M);
+	for (int l = g.leaf - 1; l >= 0; l--)
+		k_rm2m<P><<<dim3(grid((g.nx << l) * (g.ny << l)), nrows), BLOCK, 0, s>>>(V, l);
+	for (int l = 0; l <= g.leaf; l++)
+		k_rdown<P><<<dim3(grid((g.nx << l) * (g.ny << l)), nrows), BLOCK, 0, s>>>(V, l, stats);
+	k_rsum<P><<<dim3(grid(R), nrows), BLOCK, 0, s>>>(V, roots, M, stats);
+	k_rsum_out<<<nrows, BLOCK, 0, s>>>(V, roots, M, stats);
+}
+
+// compiled expansion orders; a requested order runs at the next one up
+constexpr int ROOT_ORDERS[] = {6, 8, 10, 12, 14, 16, 20, 24};
+
+int root_order_compiled(int p)
+{
+	for (int q : ROOT_ORDERS)
+		if (q >= p) return q;
+	return MAX_ROOT_FMM_ORDER;
+}
+
+void root_fmm_pass(const SweepBufs& B, const double2* roots, const RowMap& M, int nrows, unsigned long long* stats, hipStream_t s)
+{
+	switch (B.grid.order)
+	{
+	case 6: root_fmm_pass_p<6>(B, roots, M, nrows, stats, s); break;
+	case 8: root_fmm_pass_p<8>(B, roots, M, nrows, stats, s); break;
+	case 10: root_fmm_pass_p<10>(B, roots, M, nrows, stats, s); break;
+	case 12: root_fmm_pass_p<12>(B, roots, M, nrows, stats, s); break;
+	case 14: root_fmm_pass_p<14>(B, roots, M, nrows, stats, s); break;
+	case 16: root_fmm_pass_p<16>(B, roots, M, nrows, stats, s); break;
+	case 20: root_fmm_pass_p<20>(B, roots, M, nrows, stats, s); break;
+	default: root_fmm_pass_p<24>(B, roots, M, nrows, stats, s); break;
+	}
+}
+
+// grid geometry for a field: top cells square, the depth chosen so that the
+// mean number of roots per leaf cell is closest (in ratio) to `leaf_target`
+RootGrid plan_root_grid(double hx, double hy, int R, int order, double leaf_target)
+{
+	RootGrid g;
+	const double s0 = 2.0 * std::min(hx, hy);
+	g.nx = std::max(1, (int)std::ceil(2.0 * hx / s0 - 1e-9));
+	g.ny = std::max(1, (int)std::ceil(2.0 * hy / s0 - 1e-9));
+	g.s0 = s0;
+	g.x0 = -0.5 * g.nx * s0;
+	g.y0 = -0.5 * g.ny * s0;
+	int best = 0;
+	double bd = 1e300;
+	for (int l = 0; l <= MAX_ROOT_GRID_LEVELS; l++)
+	{
+		const double cells = (double)g.nx * g.ny * std::ldexp(1.0, 2 * l);
+		if (cells > 4.0 * R + 64) break;
+		const double d = std::fabs(std::log((double)R / cells / leaf_target));
+		if (d < bd)
+		{
+			bd = d;
+			best = l;
+		}
+	}
+	g.leaf = best;
+	g.order = order;
+	int tot = 0;
+	for (int l = 0; l <= g.leaf; l++)
+	{
+		g.off[l] = tot;
+		tot += (g.nx << l) * (g.ny << l);
+	}
+	g.ncells = tot;
+	g.nleaf = (g.nx << g.leaf) * (g.ny << g.leaf);
+	return g;
+}

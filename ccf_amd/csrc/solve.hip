@@ -50,7 +50,11 @@ enum Counter
 	C_ERRBITS = 4, // max 1/mu error (bits of a non-negative double)
 	C_NAN = 5,     // non-finite error seen
 	C_CLUSTER = 6, // FP32 chunks redone in FP64
-	C_COUNT = 8
+	C_RNEAR = 7,   // root-sum FMM: near-field terms
+	C_RM2L = 8,    // root-sum FMM: M2L translations
+	C_RROWS = 9,   // root-sum FMM: (row, sweep) pairs summed by the FMM
+	C_RTGT = 10,   // root-sum FMM: updates whose root sums came from the FMM
+	C_COUNT = 16
 };
 
 namespace
@@ -134,12 +138,18 @@ struct SweepView
 	int* fl;
 	int* fcnt;
 	int* fcnt_next;
+	int* dl;
+	int* dcnt;
+	int* dcnt_next;
 	double2* f0;
 	double2* f1;
 	double2* ex;
 	int* tile_off;
 	int* plan;
 	double2* part;
+	int* slot;     // root-sum FMM only (else null)
+	int* rmode;
+	double2* rsum;
 };
 
 SweepView view(const SweepBufs& B, int parity)
@@ -156,18 +166,49 @@ SweepView view(const SweepBufs& B, int parity)
 	v.fl = B.fl;
 	v.fcnt = B.fcnt + parity * B.rows;
 	v.fcnt_next = B.fcnt + (parity ^ 1) * B.rows;
+	v.dl = B.dl;
+	v.dcnt = B.dcnt + parity * B.rows;
+	v.dcnt_next = B.dcnt + (parity ^ 1) * B.rows;
 	v.f0 = B.f0;
 	v.f1 = B.f1;
 	v.ex = B.ex;
 	v.tile_off = B.tile_off;
 	v.plan = B.plan;
 	v.part = B.part;
+	v.slot = B.slot;
+	v.rmode = B.rmode;
+	v.rsum = B.rsum;
 	return v;
 }
 
-// K1 (ccf_functions.cuh:197-213 and the early return at :207-211)
+// what K1 does with a root once its Witt sums are known (ccf_functions.cuh:
+// 197-227): the 1/mu early exit, or f1 and the Aberth terms formed per root.
 // MIXED: the mixed-precision form, whose early-exit roots also take one FP64
-// Newton step z - f0/f1 (a final polish; the reference keeps z as it is)
+// Newton step z - f0/f1 (a final polish; the reference keeps z as it is) and
+// whose pole sum is the near-field sum plus the FP32 far field.
+template <bool MIXED>
+__device__ inline bool witt_root(const LensView& L, double2* p, cd z, const Witt& w, int circ_exact, int fmm_pole, cd& f1,
+	cd& ex)
+{
+	if (w.early)
+	{
+		if (MIXED)
+		{
+			const cd dz = w.f0 / witt_f1(L, z, w);
+			if (dz.abs() < 1e-6 * L.theta) *p = make_double2(z.re - dz.re, z.im - dz.im);
+		}
+		return false;
+	}
+	f1 = witt_f1(L, z, w);
+	ex = cd();
+	if (circ_exact) ex = 2.0 * z.inv(); // the disc's pole at 0 (ccf_functions.cuh:223-227)
+	if (MIXED && fmm_pole) ex += 2.0 * (w.nr.s0 + far_pole32(L, w.node, z));
+	return true;
+}
+
+// K1 (ccf_functions.cuh:197-213 and the early return at :207-211). A root
+// outside the tree (whose near field is every star, tree_node.cuh:529-561)
+// is deferred to k_witt_far, which sums those stars with a whole workgroup.
 template <bool MIXED>
 __global__ __launch_bounds__(BLOCK) void k_witt(LensView L, double2* __restrict__ roots, RowMap M, SweepView V,
 	unsigned long long* __restrict__ stats, int circ_exact, int fmm_pole)
@@ -178,11 +219,12 @@ __global__ __launch_bounds__(BLOCK) void k_witt(LensView L, double2* __restrict_
 	{
 		V.cnt_next[r] = 0;
 		V.fcnt_next[r] = 0;
+		V.dcnt_next[r] = 0;
 	}
 	const int n = V.cnt[r];
 	const int i = blockIdx.x * BLOCK + threadIdx.x;
 	unsigned long long nfull = 0, nearly = 0, nnear = 0;
-	bool full = false;
+	bool full = false, deferred = false;
 	int a = 0;
 	cd f0, f1, ex;
 	if (i < n)
@@ -193,32 +235,35 @@ __global__ __launch_bounds__(BLOCK) void k_witt(LensView L, double2* __restrict_
 		a = V.act[(size_t)r * R + i];
 		double2* p = roots + (size_t)row * R + a;
 		const cd z(p->x, p->y);
-		const Witt w = witt_f0(L, z, phi);
-		if (w.early)
+		const int node = nearest_node(L, z);
+		if (L.nf_off[node] < 0)
 		{
-			nearly = 1;
-			if (MIXED)
-			{
-				const cd dz = w.f0 / witt_f1(L, z, w);
-				if (dz.abs() < 1e-6 * L.theta) *p = make_double2(z.re - dz.re, z.im - dz.im);
-			}
+			deferred = true;
 		}
 		else
 		{
-			f0 = w.f0;
-			f1 = witt_f1(L, z, w);
-			if (circ_exact) ex = 2.0 * z.inv(); // the disc's pole at 0 (ccf_functions.cuh:223-227)
-			if (MIXED && fmm_pole) ex += 2.0 * (w.nr.s0 + far_pole32(L, w.node, z));
-			full = true;
-			nfull = 1;
-			nnear = w.nr.count;
+			const Witt w = witt_from(L, z, phi, node, p2p(L, node, z));
+			full = witt_root<MIXED>(L, p, z, w, circ_exact, fmm_pole, f1, ex);
+			if (full)
+			{
+				f0 = w.f0;
+				nfull = 1;
+				nnear = w.nr.count;
+			}
+			else
+			{
+				nearly = 1;
+			}
 		}
 	}
+	const int dpos = wave_append(&V.dcnt[r], deferred);
+	if (deferred) V.dl[(size_t)r * R + dpos] = a;
 	const int pos = wave_append(&V.fcnt[r], full);
 	if (full)
 	{
 		const size_t k = (size_t)r * R + pos;
 		V.fl[k] = a;
+		if (V.slot) V.slot[(size_t)r * R + a] = pos;
 		V.f0[k] = make_double2(f0.re, f0.im);
 		V.f1[k] = make_double2(f1.re, f1.im);
 		V.ex[k] = make_double2(ex.re, ex.im);
@@ -238,8 +283,92 @@ __global__ __launch_bounds__(BLOCK) void k_witt(LensView L, double2* __restrict_
 	}
 }
 
-// K2: tiles per row (exclusive prefix in tile_off) and slices per tile
-__global__ void k_plan(SweepView V, int nrows, int terms, int target)
+// K1 for the deferred roots: one workgroup per root sums the star ranges of
+// its node (the same terms as p2p's range loop, split over the lanes and
+// reduced in a fixed order), then finishes the root as k_witt does
+constexpr int WITT_FAR_BLOCKS = 64;
+
+template <bool MIXED>
+__global__ __launch_bounds__(BLOCK) void k_witt_far(LensView L, double2* __restrict__ roots, RowMap M, SweepView V,
+	unsigned long long* __restrict__ stats, int circ_exact, int fmm_pole)
+{
+	const int r = blockIdx.y;
+	const int R = M.R;
+	const int nd = V.dcnt[r];
+	__shared__ double red[BLOCK / 64][8];
+	for (int d = blockIdx.x; d < nd; d += gridDim.x)
+	{
+		int row;
+		double phi;
+		M.get(r, row, phi);
+		const int a = V.dl[(size_t)r * R + d];
+		double2* p = roots + (size_t)row * R + a;
+		const cd z(p->x, p->y);
+		const int node = nearest_node(L, z);
+		Near nr;
+		const int ranges = L.nnf[node];
+		for (int k = 0; k < ranges; k++)
+		{
+			const int2 rg = L.nf[node * MAX_NEAR_RANGES + k];
+			for (int s = rg.x + threadIdx.x; s < rg.x + rg.y; s += BLOCK)
+			{
+				const double2 q = L.spos[s];
+				const cd t = cd(z.re - q.x, z.im - q.y).inv();
+				nr.s0 += t;
+				const cd t1 = t * L.smass[s];
+				nr.s1 += t1;
+				const cd t2 = t1 * t;
+				nr.s2 += t2;
+				nr.s3 += t2 * t;
+			}
+			nr.count += rg.y;
+		}
+		double v[8] = {nr.s0.re, nr.s0.im, nr.s1.re, nr.s1.im, nr.s2.re, nr.s2.im, nr.s3.re, nr.s3.im};
+#pragma unroll
+		for (int q = 0; q < 8; q++)
+			for (int o = 32; o >= 1; o >>= 1) v[q] += __shfl_xor(v[q], o, 64);
+		if ((threadIdx.x & 63) == 0)
+			for (int q = 0; q < 8; q++) red[threadIdx.x >> 6][q] = v[q];
+		__syncthreads();
+		if (threadIdx.x == 0)
+		{
+			double t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+			for (int w = 0; w < BLOCK / 64; w++)
+				for (int q = 0; q < 8; q++) t[q] += red[w][q];
+			Near sum;
+			sum.s0 = cd(t[0], t[1]);
+			sum.s1 = cd(t[2], t[3]);
+			sum.s2 = cd(t[4], t[5]);
+			sum.s3 = cd(t[6], t[7]);
+			sum.count = nr.count;
+			const Witt w = witt_from(L, z, phi, node, sum);
+			cd f1, ex;
+			if (witt_root<MIXED>(L, p, z, w, circ_exact, fmm_pole, f1, ex))
+			{
+				const int pos = atomicAdd(&V.fcnt[r], 1);
+				const size_t k = (size_t)r * R + pos;
+				V.fl[k] = a;
+				if (V.slot) V.slot[(size_t)r * R + a] = pos;
+				V.f0[k] = make_double2(w.f0.re, w.f0.im);
+				V.f1[k] = make_double2(f1.re, f1.im);
+				V.ex[k] = make_double2(ex.re, ex.im);
+				atomicAdd(&stats[C_FULL], 1ull);
+				atomicAdd(&stats[C_NEAR], (unsigned long long)sum.count);
+			}
+			else
+			{
+				atomicAdd(&stats[C_EARLY], 1ull);
+				atomicAdd(&stats[C_CONV], 1ull);
+			}
+		}
+		__syncthreads();
+	}
+}
+
+// K2: tiles per row (exclusive prefix in tile_off) and slices per tile; a
+// row with at least fmm_min listed roots (fmm_min > 0) takes its root sums
+// from the FMM pass instead and gets no tiles
+__global__ void k_plan(SweepView V, int nrows, int terms, int target, int fmm_min)
 {
 	__shared__ int wsum[16];
 	__shared__ int carry;
@@ -249,7 +378,9 @@ __global__ void k_plan(SweepView V, int nrows, int terms, int target)
 	for (int base = 0; base < nrows; base += blockDim.x)
 	{
 		const int r = base + threadIdx.x;
-		const int v = r < nrows ? (V.fcnt[r] + V.tile - 1) / V.tile : 0;
+		const bool fmm = r < nrows && fmm_min > 0 && V.fcnt[r] >= fmm_min;
+		if (r < nrows && V.rmode) V.rmode[r] = fmm ? 1 : 0;
+		const int v = (r < nrows && !fmm) ? (V.fcnt[r] + V.tile - 1) / V.tile : 0;
 		int x = v;
 		for (int d = 1; d < 64; d <<= 1)
 		{
@@ -298,6 +429,8 @@ __device__ inline int row_of_tile(const int* __restrict__ tile_off, int nrows, i
 	}
 	return lo;
 }
+
+#include "rootfmm.hpp"
 
 // K3 in FP64: partial 2 sum_stars 1/(z - s) - sum_roots 1/(z - w) over one slice
 template <int RPT>
@@ -530,11 +663,20 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(double theta, double2* __res
 		const int loc = i % V.tile;
 		double2 e = V.ex[k];
 		cd sum(e.x, e.y);
-		for (int s = 0; s < S; s++)
+		if (V.rmode && V.rmode[r])
 		{
-			const double2 q = V.part[(g * S + s) * V.tile + loc];
+			const double2 q = V.rsum[k];
 			sum += cd(q.x, q.y);
 		}
+		else
+		{
+			for (int s = 0; s < S; s++)
+			{
+				const double2 q = V.part[(g * S + s) * V.tile + loc];
+				sum += cd(q.x, q.y);
+			}
+		}
+		if (V.slot) V.slot[(size_t)r * R + a] = -1;
 		const double2 a0 = V.f0[k], a1 = V.f1[k];
 		const cd f0(a0.x, a0.y), f1(a1.x, a1.y);
 		const cd zn = z - f0 / (f1 + f0 * sum);
@@ -590,6 +732,8 @@ __global__ void k_prepare(double2* roots, RowMap M, SweepBufs B, int order)
 		B.cnt[B.rows + r] = 0;
 		B.fcnt[r] = 0;
 		B.fcnt[B.rows + r] = 0;
+		B.dcnt[r] = 0;
+		B.dcnt[B.rows + r] = 0;
 	}
 }
 
@@ -726,7 +870,9 @@ SumsLaunch sums_launch(ccf_ctx* ctx, bool f32)
 
 // `cap` sweeps over the rows of M (ccf.cuh:929-939 and 1013-1018): the
 // sweep kernels exit at once when no root of a row is active any more
-bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int nrows, int cap)
+// The first fmm_sweeps sweeps may take the root sums of their dense rows
+// from the FMM pass (when the buffers carry a root grid).
+bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int nrows, int cap, int fmm_sweeps)
 {
 	const LensView L = ctx->lens();
 	hipStream_t s = ctx->stream;
@@ -745,9 +891,19 @@ bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int
 	for (int it = 0; it < cap; it++)
 	{
 		const SweepView V = view(B, it & 1);
-		if (f32) k_witt<true><<<g_rows, BLOCK, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, fmm_pole ? 1 : 0);
-		else k_witt<false><<<g_rows, BLOCK, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, 0);
-		k_plan<<<1, 1024, 0, s>>>(V, nrows, Np + R, sl.target);
+		const dim3 g_far(WITT_FAR_BLOCKS, nrows);
+		if (f32)
+		{
+			k_witt<true><<<g_rows, BLOCK, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, fmm_pole ? 1 : 0);
+			k_witt_far<true><<<g_far, BLOCK, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, fmm_pole ? 1 : 0);
+		}
+		else
+		{
+			k_witt<false><<<g_rows, BLOCK, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, 0);
+			k_witt_far<false><<<g_far, BLOCK, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, 0);
+		}
+		const bool fmm = B.grid.order > 0 && it < fmm_sweeps;
+		k_plan<<<1, 1024, 0, s>>>(V, nrows, Np + R, sl.target, fmm ? root_fmm_min(R) : 0);
 		ccf::Timer tm;
 		if (ctx->P.timing)
 		{
@@ -755,6 +911,7 @@ bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int
 			(void)hipEventCreate(&tm.b);
 			(void)hipEventRecord(tm.a, s);
 		}
+		if (fmm) root_fmm_pass(B, roots, M, nrows, ctx->counters, s);
 		if (f32)
 		{
 			const double d = CLUSTER_REL * ctx->root_half_length;
@@ -781,12 +938,56 @@ bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int
 	return true;
 }
 
+bool alloc_root_fmm(ccf_ctx* ctx, SweepBufs& b)
+{
+	const int rows = b.rows, R = b.R;
+	const size_t list = (size_t)rows * R;
+	// root-sum FMM: grid over the field (with a tenth of margin, as the
+	// star tree's root node, ccf.cuh:738-746)
+	double hx, hy;
+	if (ctx->rectangular)
+	{
+		hx = 1.1 * std::fabs(ctx->corner.re);
+		hy = 1.1 * std::fabs(ctx->corner.im);
+	}
+	else
+	{
+		hx = hy = 1.1 * ctx->corner.abs();
+	}
+	const int order = root_order_compiled(ctx->P.root_fmm_order > 0 ? ctx->P.root_fmm_order : 8);
+	const double leaf = ctx->P.root_fmm_leaf > 0 ? ctx->P.root_fmm_leaf : 6.0;
+	b.grid = plan_root_grid(hx, hy, R, order, leaf);
+	const RootGrid& g = b.grid;
+	std::vector<double> binom((RB + 1) * (RB + 1), 0.0);
+	for (int n = 0; n <= RB; n++)
+	{
+		binom[n * (RB + 1)] = 1;
+		for (int k = 1; k <= n; k++) binom[n * (RB + 1) + k] = binom[(n - 1) * (RB + 1) + k - 1] + (k < n ? binom[(n - 1) * (RB + 1) + k] : 0.0);
+	}
+	const size_t ce = (size_t)rows * g.ncells * (g.order + 1);
+	return CCF_HIP(ctx, hipMalloc(&b.slot, list * sizeof(int))) && CCF_HIP(ctx, hipMalloc(&b.rmode, rows * sizeof(int))) &&
+		CCF_HIP(ctx, hipMalloc(&b.rsum, list * sizeof(double2))) &&
+		CCF_HIP(ctx, hipMalloc(&b.ccnt, (size_t)rows * g.ncells * sizeof(int))) &&
+		CCF_HIP(ctx, hipMalloc(&b.cstart, (size_t)rows * (g.nleaf + 1) * sizeof(int))) &&
+		CCF_HIP(ctx, hipMalloc(&b.ccur, (size_t)rows * g.nleaf * sizeof(int))) && CCF_HIP(ctx, hipMalloc(&b.cell, list * sizeof(int))) &&
+		CCF_HIP(ctx, hipMalloc(&b.sidx, list * sizeof(int))) && CCF_HIP(ctx, hipMalloc(&b.spos, list * sizeof(double2))) &&
+		CCF_HIP(ctx, hipMalloc(&b.ocnt, rows * sizeof(int))) && CCF_HIP(ctx, hipMalloc(&b.oidx, list * sizeof(int))) &&
+		CCF_HIP(ctx, hipMalloc(&b.mp, ce * sizeof(double2))) && CCF_HIP(ctx, hipMalloc(&b.lc, ce * sizeof(double2))) &&
+		CCF_HIP(ctx, hipMemsetAsync(b.slot, 0xff, list * sizeof(int), ctx->stream)) &&
+		CCF_HIP(ctx, hipMemsetAsync(b.rmode, 0, rows * sizeof(int), ctx->stream)) &&
+		CCF_HIP(ctx, hipMemcpyToSymbolAsync(HIP_SYMBOL(c_rbinom), binom.data(), binom.size() * sizeof(double), 0,
+			hipMemcpyHostToDevice, ctx->stream)) &&
+		CCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+}
+
 } // namespace
 
 void free_sweep(SweepBufs& b)
 {
 	auto F = [](auto*& p) { if (p) (void)hipFree(p); p = nullptr; };
-	F(b.act); F(b.cnt); F(b.fl); F(b.fcnt); F(b.f0); F(b.f1); F(b.ex); F(b.tile_off); F(b.plan); F(b.part);
+	F(b.act); F(b.cnt); F(b.fl); F(b.fcnt); F(b.dl); F(b.dcnt); F(b.f0); F(b.f1); F(b.ex); F(b.tile_off); F(b.plan); F(b.part);
+	F(b.slot); F(b.rmode); F(b.rsum); F(b.ccnt); F(b.cstart); F(b.ccur); F(b.cell); F(b.sidx); F(b.spos); F(b.ocnt);
+	F(b.oidx); F(b.mp); F(b.lc);
 	b = SweepBufs{};
 }
 
@@ -801,14 +1002,19 @@ bool alloc_sweep(ccf_ctx* ctx, SweepBufs& b, int rows, int R)
 	const long long tiles_max = (long long)rows * ((R + b.tile - 1) / b.tile);
 	b.max_items = (int)(tiles_max + sl.target + tiles_max);
 	const size_t list = (size_t)rows * R;
-	return CCF_HIP(ctx, hipMalloc(&b.act, 2 * list * sizeof(int))) && CCF_HIP(ctx, hipMalloc(&b.cnt, 2 * rows * sizeof(int))) &&
+	const bool ok = CCF_HIP(ctx, hipMalloc(&b.act, 2 * list * sizeof(int))) && CCF_HIP(ctx, hipMalloc(&b.cnt, 2 * rows * sizeof(int))) &&
 		CCF_HIP(ctx, hipMalloc(&b.fl, list * sizeof(int))) && CCF_HIP(ctx, hipMalloc(&b.fcnt, 2 * rows * sizeof(int))) &&
+		CCF_HIP(ctx, hipMalloc(&b.dl, list * sizeof(int))) && CCF_HIP(ctx, hipMalloc(&b.dcnt, 2 * rows * sizeof(int))) &&
+		CCF_HIP(ctx, hipMemsetAsync(b.dcnt, 0, 2 * rows * sizeof(int), ctx->stream)) &&
 		CCF_HIP(ctx, hipMalloc(&b.f0, list * sizeof(double2))) && CCF_HIP(ctx, hipMalloc(&b.f1, list * sizeof(double2))) &&
 		CCF_HIP(ctx, hipMalloc(&b.ex, list * sizeof(double2))) && CCF_HIP(ctx, hipMalloc(&b.tile_off, (rows + 1) * sizeof(int))) &&
 		CCF_HIP(ctx, hipMalloc(&b.plan, 4 * sizeof(int))) &&
 		CCF_HIP(ctx, hipMalloc(&b.part, (size_t)b.max_items * b.tile * sizeof(double2))) &&
 		CCF_HIP(ctx, hipMemsetAsync(b.cnt, 0, 2 * rows * sizeof(int), ctx->stream)) &&
 		CCF_HIP(ctx, hipMemsetAsync(b.fcnt, 0, 2 * rows * sizeof(int), ctx->stream));
+	if (!ok) return false;
+	if (ctx->P.root_fmm != 1 || R < ROOT_FMM_MIN_ROOTS) return true;
+	return alloc_root_fmm(ctx, b);
 }
 
 bool alloc_solver(ccf_ctx* ctx)
@@ -839,7 +1045,7 @@ bool find_initial_roots_dev(ccf_ctx* ctx, int verbose)
 	k_prepare<<<dim3(grid(R), nb), BLOCK, 0, s>>>(ctx->ccs_init, M, ctx->sw, 0);
 	if (!CCF_LAUNCH(ctx, "prepare_roots_kernel")) return false;
 	const int cap = ctx->P.init_iters > 0 ? ctx->P.init_iters : 60;
-	if (!run_sweeps(ctx, ctx->sw, ctx->ccs_init, M, nb, cap)) return false;
+	if (!run_sweeps(ctx, ctx->sw, ctx->ccs_init, M, nb, cap, ROOT_FMM_INIT_SWEEPS)) return false;
 	if (!CCF_HIP(ctx, hipStreamSynchronize(s))) return false;
 	auto t1 = std::chrono::steady_clock::now();
 	ctx->info.t_init_roots = std::chrono::duration<double>(t1 - t0).count();
@@ -864,7 +1070,7 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 		const int order = ctx->P.approx ? ctx->P.predictor : 0;
 		k_prepare<<<dim3(grid(R), 2 * nb), BLOCK, 0, s>>>(ctx->ccs_init, M, ctx->sw, order);
 		if (!CCF_LAUNCH(ctx, "prepare_roots_kernel")) return false;
-		if (!run_sweeps(ctx, ctx->sw, ctx->ccs_init, M, 2 * nb, cap)) return false;
+		if (!run_sweeps(ctx, ctx->sw, ctx->ccs_init, M, 2 * nb, cap, ROOT_FMM_STEP_SWEEPS)) return false;
 		if (verbose >= 1 && j * 100 / steps > (j - 1) * 100 / steps)
 		{
 			if (!CCF_HIP(ctx, hipStreamSynchronize(s))) return false;
@@ -909,7 +1115,18 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 		const double p = ctx->tree.order;
 		const double sm = (ctx->rectangular && ctx->P.approx) ? 8.0 * (ctx->taylor_smooth - 1) : 0.0;
 		const double full = (double)c[C_FULL];
-		ctx->info.flops_sums = full * 10.0 * (Np + R - 1);
+		// root-sum FMM: 10 per near-field term, 4 (p+1)^2 + 16 (p+1) per M2L
+		// (real binomials times complex terms), 8 p per root for P2M and for
+		// L2P, about 20 (p+1)^2 per cell for M2M and L2L
+		const RootGrid& g = ctx->sw.grid;
+		const double pr = g.order, rt = (double)c[C_RTGT], rows = (double)c[C_RROWS];
+		const double fmm = 10.0 * (double)c[C_RNEAR] + (double)c[C_RM2L] * (4.0 * (pr + 1) * (pr + 1) + 16.0 * (pr + 1)) +
+			rows * R * 8.0 * pr + rt * 8.0 * pr + rows * g.ncells * 20.0 * (pr + 1) * (pr + 1);
+		ctx->info.flops_sums = (full - rt) * 10.0 * (Np + R - 1) + (g.order > 0 ? fmm : 0.0);
+		ctx->info.root_fmm_updates = (long long)c[C_RTGT];
+		ctx->info.root_fmm_rows = (long long)c[C_RROWS];
+		ctx->info.root_fmm_order = g.order;
+		ctx->info.root_fmm_leaf_level = g.leaf;
 		ctx->info.flops_update = ctx->info.flops_sums + full * (16.0 * p + sm + (fmm_pole ? 16.0 * p : 0.0)) +
 			(36.0 + (fmm_pole ? 8.0 : 0.0)) * (double)c[C_NEAR] + (double)c[C_EARLY] * 8.0 * p;
 	}
@@ -986,11 +1203,48 @@ bool aberth_sweep_dev(ccf_ctx* ctx, const double* roots, int n, double phi, doub
 	{
 		const RowMap M{n, 2, 1, 0, 0, phi};
 		k_prepare<<<dim3(grid(n), 1), BLOCK, 0, s>>>(row, M, B, 0);
-		ok = CCF_LAUNCH(ctx, "prepare") && run_sweeps(ctx, B, row, M, 1, 1) &&
+		ok = CCF_LAUNCH(ctx, "prepare") && run_sweeps(ctx, B, row, M, 1, 1, 1) &&
 			CCF_HIP(ctx, hipMemcpyAsync(out, row, (size_t)n * sizeof(double2), hipMemcpyDeviceToHost, s)) &&
 			CCF_HIP(ctx, hipStreamSynchronize(s));
 	}
 	(void)hipFree(row);
+	free_sweep(B);
+	return ok;
+}
+
+bool root_sums_dev(ccf_ctx* ctx, const double* roots, int n, double* out)
+{
+	if (n <= 0) return true;
+	hipStream_t s = ctx->stream;
+	double2* row = nullptr;
+	unsigned long long* stats = nullptr;
+	SweepBufs B;
+	B.rows = 1;
+	B.R = n;
+	std::vector<int> iota(n), one{1};
+	for (int i = 0; i < n; i++) iota[i] = i;
+	std::vector<double2> res(n);
+	bool ok = CCF_HIP(ctx, hipMalloc(&row, (size_t)n * sizeof(double2))) && alloc_root_fmm(ctx, B) &&
+		CCF_HIP(ctx, hipMalloc(&stats, C_COUNT * sizeof(unsigned long long))) &&
+		CCF_HIP(ctx, hipMemcpyAsync(row, roots, (size_t)n * sizeof(double2), hipMemcpyHostToDevice, s)) &&
+		CCF_HIP(ctx, hipMemcpyAsync(B.slot, iota.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, s)) &&
+		CCF_HIP(ctx, hipMemcpyAsync(B.rmode, one.data(), sizeof(int), hipMemcpyHostToDevice, s));
+	if (ok)
+	{
+		const RowMap M{n, 2, 1, 0, 0, 0.0};
+		root_fmm_pass(B, row, M, 1, stats, s);
+		ok = CCF_LAUNCH(ctx, "root-sum FMM") &&
+			CCF_HIP(ctx, hipMemcpyAsync(res.data(), B.rsum, (size_t)n * sizeof(double2), hipMemcpyDeviceToHost, s)) &&
+			CCF_HIP(ctx, hipStreamSynchronize(s));
+	}
+	if (ok)
+		for (int i = 0; i < n; i++)
+		{
+			out[2 * i] = -res[i].x;
+			out[2 * i + 1] = -res[i].y;
+		}
+	(void)hipFree(row);
+	(void)hipFree(stats);
 	free_sweep(B);
 	return ok;
 }

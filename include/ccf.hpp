@@ -54,6 +54,9 @@ public:
 	int roots_per_thread = 0;
 	int timing = 0;
 	int predictor = 0;
+	int root_fmm = 0;
+	int root_fmm_order = 0;
+	int root_fmm_leaf = 0;
 
 	CCF() : ctx_(ccf_create()) {}
 	~CCF() { ccf_destroy(ctx_); }
@@ -119,6 +122,9 @@ public:
 		p.roots_per_thread = roots_per_thread;
 		p.timing = timing;
 		p.predictor = predictor;
+		p.root_fmm = root_fmm;
+		p.root_fmm_order = root_fmm_order;
+		p.root_fmm_leaf = root_fmm_leaf;
 		return p;
 	}
 

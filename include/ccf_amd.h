@@ -62,6 +62,12 @@ typedef struct ccf_params
 	int predictor;       /* initial roots of phi step j: 0 the roots of step j-1 (the reference,
 	                        ccf_functions.cuh:284-289); 1 linear, 2 quadratic extrapolation
 	                        from steps j-1, j-2 (, j-3) */
+	int root_fmm;        /* 1: the root sum sum_{j!=k} 1/(z_k - z_j) of sweeps in which many roots
+	                        are active by a fast multipole method over a uniform grid of the field
+	                        (exact FP64 near field, far field to order root_fmm_order); 0: direct
+	                        sums as ccf_functions.cuh:232-239 */
+	int root_fmm_order;  /* expansion order of the root-sum FMM (0 -> 8; at most 24) */
+	int root_fmm_leaf;   /* mean roots per leaf cell the grid depth aims at (0 -> 6) */
 } ccf_params;
 
 /* Derived values and measurements of the last run. */
@@ -94,7 +100,11 @@ typedef struct ccf_info
 	long long aberth_launches; /* number of timed Cauchy-sum launches */
 	long long sweeps;          /* Aberth sweeps launched */
 	long long cluster_chunks;  /* mixed precision: LDS chunks recomputed in FP64 for clustered roots */
-	double flops_sums;         /* algorithmic flops of the Cauchy-sum kernel alone */
+	double flops_sums;         /* algorithmic flops of the Cauchy-sum kernels (direct and FMM) */
+	long long root_fmm_updates;/* updates whose root sums came from the root-sum FMM */
+	long long root_fmm_rows;   /* (row, sweep) pairs summed by the root-sum FMM */
+	int root_fmm_order;        /* its expansion order (0: not used) */
+	int root_fmm_leaf_level;   /* depth of its grid */
 } ccf_info;
 
 /* Arrays readable with ccf_get_array. */
@@ -160,6 +170,13 @@ int ccf_eval_points(ccf_ctx* ctx, const double* z, int n, double phi, double* ou
 /* One Jacobi Aberth sweep (ccf_functions.cuh:193-243) of n host roots on
    the device: out[i] = update of roots[i] given all of roots[0..n). */
 int ccf_aberth_sweep(ccf_ctx* ctx, const double* roots, int n, double phi, double* out);
+
+/* The root sums S_k = sum_{j != k} 1/(z_k - z_j) of n host roots (the root
+   part of ccf_functions.cuh:232-239) by the root-sum FMM of a prepared
+   context, with its root_fmm_order and root_fmm_leaf, whether root_fmm is
+   set or not: out[2k], out[2k+1] = S_k. No counterpart in the reference;
+   for checking the FMM against direct sums. */
+int ccf_root_sums(ccf_ctx* ctx, const double* roots, int n, double* out);
 
 /* Write a seeded synthetic star file (layout of star.cuh:498-534) drawn
    on the host with std::mt19937_64: positions uniform in the disc or

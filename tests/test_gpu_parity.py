@@ -82,6 +82,50 @@ def test_aberth_sweep_matches_oracle(workdir, name, kw):
     assert rel.max() < 1e-11
 
 
+def _direct_root_sums(z):
+    """sum_{j != k} 1/(z_k - z_j) in FP64, blockwise, and the scale of each
+    sum, sum_{j != k} 1/|z_k - z_j| (the far-field error bound's unit)."""
+    S = np.empty_like(z)
+    scale = np.empty(z.size)
+    for a in range(0, z.size, 2048):
+        d = z[a:a + 2048, None] - z[None, :]
+        d[np.arange(d.shape[0]), a + np.arange(d.shape[0])] = np.inf
+        S[a:a + 2048] = (1.0 / d).sum(axis=1)
+        scale[a:a + 2048] = (1.0 / np.abs(d)).sum(axis=1)
+    return S, scale
+
+
+@pytest.mark.parametrize("name,kw,n,tol", [
+    ("salpeter300", dict(kappa_tot=0.45, shear=0.35), 6000, 5e-5),
+    ("salpeter300", dict(kappa_tot=0.45, shear=0.35, root_fmm_order=14), 6000, 2e-7),
+    ("salpeter300", dict(kappa_tot=0.45, shear=0.35, root_fmm_order=6, root_fmm_leaf=3), 6000, 5e-4),
+    ("rect200", dict(kappa_tot=0.7, shear=0.25), 5000, 5e-5),
+    ("cfg1", dict(kappa_tot=0.3, shear=0.1, root_fmm_order=24, root_fmm_leaf=12), 3000, 1e-9),
+])
+def test_root_sums_fmm_matches_direct(workdir, name, kw, n, tol):
+    """The root-sum FMM against direct FP64 sums: roots clustered at the stars
+    and spread over the field, a few outside the grid, a close pair."""
+    sf = make_field(workdir, name)
+    rng = np.random.default_rng(11)
+    _, rect, corner, _, st = io.read_stars(sf)
+    ext = np.array([abs(corner.real), abs(corner.imag)]) if rect else np.array([abs(corner), abs(corner)])
+    s = st[rng.integers(0, st.shape[0], n // 2), :2]
+    z = np.concatenate([s[:, 0] + 1j * s[:, 1] + 0.05 * (rng.standard_normal(n // 2) + 1j * rng.standard_normal(n // 2)),
+                        ext[0] * rng.uniform(-1, 1, n - n // 2) + 1j * ext[1] * rng.uniform(-1, 1, n - n // 2)])
+    z[:4] = [2.0 * ext[0], -1.5j * ext[1], 0.3 + 0.2j, 0.3 + 0.2j + 1e-7]
+    with _ctx(sf, str(workdir) + "/", num_phi=10, **kw) as c:
+        c.prepare()
+        S = c.root_sums(z)
+    D, scale = _direct_root_sums(z)
+    # truncation error of the far field, in units of sum 1/|z_k - z_j|
+    # (Greengard-Rokhlin bound ~ 0.71^(p+1) for the closest well-separated
+    # cells; measured on uniform sets: 3e-8 at p = 14, 8e-6 at p = 8, 5e-5 at p = 6, 1e-11 at p = 24;
+    # the default order is 8)
+    rel = np.abs(S - D) / scale
+    assert np.all(np.isfinite(S))
+    assert rel.max() <= tol, rel.max()
+
+
 RUNS = [
     ("cfg1", dict(kappa_tot=0.3, shear=0.1, num_phi=100, num_branches=1, direct=1)),
     ("cfg1", dict(kappa_tot=0.3, shear=0.1, num_phi=100, num_branches=1)),
@@ -98,6 +142,12 @@ MIXED = [
     ("salpeter300", dict(kappa_tot=0.45, shear=0.35, num_phi=120, num_branches=3, fmm_order=20), dict(predictor=2)),
     ("rect200", dict(kappa_tot=0.7, shear=0.25, num_phi=80, num_branches=2), dict(precision=1, predictor=2)),
     ("smooth150", dict(kappa_tot=0.6, shear=0.2, num_phi=60, num_branches=1, approx=0), dict(precision=1, predictor=1)),
+    # root sums of the dense sweeps by the FMM
+    ("cfg1", dict(kappa_tot=0.3, shear=0.1, num_phi=100, num_branches=1, direct=1), dict(root_fmm=1)),
+    ("salpeter300", dict(kappa_tot=0.45, shear=0.35, num_phi=120, num_branches=3, fmm_order=20),
+     dict(precision=1, predictor=2, root_fmm=1)),
+    ("rect200", dict(kappa_tot=0.7, shear=0.25, num_phi=80, num_branches=2), dict(precision=1, predictor=2, root_fmm=1)),
+    ("smooth150", dict(kappa_tot=0.6, shear=0.2, num_phi=60, num_branches=1, approx=0), dict(precision=1, root_fmm=1)),
 ]
 
 
@@ -160,3 +210,5 @@ def test_run_matches_oracle(workdir, name, kw, gkw):
     _, _, _, _, os_ = io.read_stars(op + "ccf_stars.bin")
     assert np.array_equal(np.sort(gs.view("f8,f8,f8"), axis=0), np.sort(os_.view("f8,f8,f8"), axis=0))
     assert ginfo["points_converged"] == ostats["points_converged"] == R * (kw["num_phi"] + nb)
+    if gkw.get("root_fmm"):
+        assert ginfo["root_fmm_updates"] > 0 and ginfo["root_fmm_rows"] > 0
