@@ -58,6 +58,10 @@ ccf::LensView ccf_ctx::lens() const
 	L.loc = tree.loc;
 	L.loc32 = tree.loc32;
 	L.locu32 = tree.locu32;
+	L.lookup = tree.lookup;
+	L.lookup_n = 1 << tree.lookup_level;
+	L.lookup_x0 = tree.lookup_x0;
+	L.lookup_inv = tree.lookup_inv;
 	return L;
 }
 
@@ -83,12 +87,14 @@ void ccf_ctx::free_device()
 	ccf::DeviceTree& T = tree;
 	F(T.center); F(T.hl); F(T.parent); F(T.child); F(T.sstart); F(T.scount);
 	F(T.nbr); F(T.nnbr); F(T.sl); F(T.nsl); F(T.dl); F(T.ndl); F(T.split); F(T.rank);
-	F(T.nf); F(T.nnf); F(T.nbox); F(T.nf_off); F(T.nf_cnt); F(T.nfs); F(T.mp); F(T.loc); F(T.loc32); F(T.mpu); F(T.locu); F(T.locu32); F(T.binom);
+	F(T.nf); F(T.nnf); F(T.nbox); F(T.nf_off); F(T.nf_cnt); F(T.nfs); F(T.mp); F(T.loc); F(T.loc32); F(T.mpu); F(T.locu); F(T.locu32); F(T.binom); F(T.lookup);
 	T = ccf::DeviceTree{};
 	F(spos); F(smass); F(spos_tmp); F(smass_tmp); F(spos32); F(ts_a); F(ts_b); F(ones);
-	F(ccs_init); F(ccs); F(caustics); F(mu_scales);
+	F(ccs_init); F(ccs); F(caustics); F(mu_scales); F(phase);
 	ccf::free_sweep(sw);
 	F(counters);
+	if (host_counts) (void)hipHostFree(host_counts);
+	host_counts = nullptr;
 	for (auto& t : timers)
 	{
 		if (t.a) hipEventDestroy(t.a);

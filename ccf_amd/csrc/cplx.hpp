@@ -45,7 +45,27 @@ struct cplx
 	__host__ __device__ cplx& operator/=(T s) { re /= s; im /= s; return *this; }
 	__host__ __device__ cplx inv() const
 	{
-		T d = T(1) / (re * re + im * im);
+		const T n = re * re + im * im;
+#if defined(__HIP_DEVICE_COMPILE__)
+		// device FP64: the reciprocal estimate refined by two Newton steps
+		// (each squares the ~2^-23 relative error) instead of the IEEE
+		// division sequence
+		T d;
+		if constexpr (sizeof(T) == 8)
+		{
+			d = __builtin_amdgcn_rcp(n);
+			T e = __builtin_fma(-n, d, T(1));
+			d = __builtin_fma(d, e, d);
+			e = __builtin_fma(-n, d, T(1));
+			d = __builtin_fma(d, e, d);
+		}
+		else
+		{
+			d = T(1) / n;
+		}
+#else
+		const T d = T(1) / n;
+#endif
 		return cplx(re * d, -im * d);
 	}
 	__host__ __device__ cplx& operator/=(cplx o) { return *this *= o.inv(); }

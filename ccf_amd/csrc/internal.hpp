@@ -65,6 +65,9 @@ struct DeviceTree
 	double2* locu = nullptr;
 	float2* locu32 = nullptr;
 	double* binom = nullptr;
+	int* lookup = nullptr;  // [4^lookup_level]: node reached at that level by each cell of a uniform grid
+	int lookup_level = 0;
+	double lookup_x0 = 0, lookup_inv = 0;
 };
 
 // Kernel-side view of the lens model and tree (passed by value).
@@ -92,6 +95,9 @@ struct LensView
 	const double2* loc;
 	const float2* loc32;
 	const float2* locu32;
+	const int* lookup;       // point-location table (null: descend from the root)
+	int lookup_n;            // its cells per side
+	double lookup_x0, lookup_inv; // lower-left corner of the root box, cells per unit length
 };
 
 struct Timer
@@ -197,6 +203,8 @@ struct ccf_ctx
 	double* ones = nullptr;       // unit masses for the pole-sum expansions
 	unsigned long long* counters = nullptr; // device statistics, see solve.hip
 	std::vector<ccf::Timer> timers;
+	int* host_counts = nullptr;   // pinned: active roots per row, read back between sweeps
+	double2* phase = nullptr;     // (cos phi, sin phi) of every phi row
 
 	ccf::LensView lens() const;
 	void fail(const std::string& msg);

@@ -15,11 +15,9 @@
 namespace ccf
 {
 
-// tree_node.cuh:529-561
-__device__ inline int nearest_node(const LensView& L, cd z)
+// the descent of tree_node.cuh:529-561 from node n (box b)
+__device__ inline int descend(const LensView& L, cd z, int n, double4 b)
 {
-	int n = 0;
-	double4 b = L.nbox[0];
 	while (b.w >= 0)
 	{
 		if (fabs(z.re - b.x) > b.z || fabs(z.im - b.y) > b.z) return n;
@@ -32,6 +30,26 @@ __device__ inline int nearest_node(const LensView& L, cd z)
 		b = L.nbox[n];
 	}
 	return n;
+}
+
+// tree_node.cuh:529-561. The point-location table gives the node the
+// descent reaches at the table's level in one load; it is used only when z
+// lies inside that node's box by a margin far above the rounding of the
+// stored centres, where the descent from the root takes the same path.
+__device__ inline int nearest_node(const LensView& L, cd z)
+{
+	if (L.lookup)
+	{
+		const double fx = (z.re - L.lookup_x0) * L.lookup_inv, fy = (z.im - L.lookup_x0) * L.lookup_inv;
+		if (fx >= 0 && fx < L.lookup_n && fy >= 0 && fy < L.lookup_n)
+		{
+			const int n = L.lookup[(int)fy * L.lookup_n + (int)fx];
+			const double4 b = L.nbox[n];
+			const double m = b.z * (1 - 1e-9);
+			if (fabs(z.re - b.x) < m && fabs(z.im - b.y) < m) return descend(L, z, n, b);
+		}
+	}
+	return descend(L, z, 0, L.nbox[0]);
 }
 
 // near-field sums over a node and its neighbours:
@@ -60,16 +78,18 @@ __device__ inline Near p2p(const LensView& L, int node, cd z)
 	if (off >= 0)
 	{
 		// a leaf: one contiguous run of at most ~32 stars, two at a time
+		// four stars per batch of loads, summed in order
 		const int n = L.nf_cnt[node];
 		const double4* __restrict__ p = L.nfs + off;
-		int t = 0;
-		for (; t + 1 < n; t += 2)
+		for (int t = 0; t < n; t += 4)
 		{
-			const double4 a = p[t], b = p[t + 1];
-			p2p_add(r, z, a);
-			p2p_add(r, z, b);
+			double4 s[4];
+#pragma unroll
+			for (int k = 0; k < 4; k++) s[k] = p[min(t + k, n - 1)];
+#pragma unroll
+			for (int k = 0; k < 4; k++)
+				if (t + k < n) p2p_add(r, z, s[k]);
 		}
-		if (t < n) p2p_add(r, z, p[t]);
 		r.count = n;
 		return r;
 	}
@@ -110,13 +130,22 @@ __device__ inline Far l2p(const LensView& L, int node, cd z)
 	const double2 c = L.center[node];
 	const cd dz = cd(z.re - c.x, z.im - c.y) / f.hl;
 	const double2* a = L.loc + (size_t)node * (L.order + 1);
-	for (int i = L.order; i >= 1; i--)
+	// Horner from the top, eight coefficients per batch of loads
+	for (int i0 = L.order; i0 >= 1; i0 -= 8)
 	{
-		const double2 ci = a[i];
-		const cd c1(ci.x * i, ci.y * i);
-		f.h1 = f.h1 * dz + c1;
-		if (i >= 2) f.h2 = f.h2 * dz + c1 * (double)(i - 1);
-		if (i >= 3) f.h3 = f.h3 * dz + c1 * (double)((i - 1) * (i - 2));
+		double2 cb[8];
+#pragma unroll
+		for (int k = 0; k < 8; k++) cb[k] = a[max(i0 - k, 0)];
+#pragma unroll
+		for (int k = 0; k < 8; k++)
+		{
+			const int i = i0 - k;
+			if (i < 1) break;
+			const cd c1(cb[k].x * i, cb[k].y * i);
+			f.h1 = f.h1 * dz + c1;
+			if (i >= 2) f.h2 = f.h2 * dz + c1 * (double)(i - 1);
+			if (i >= 3) f.h3 = f.h3 * dz + c1 * (double)((i - 1) * (i - 2));
+		}
 	}
 	return f;
 }
@@ -133,10 +162,18 @@ __device__ inline cd far_pole32(const LensView& L, int node, cd z)
 	const cf dz((float)(z.re - c.x) * ih, (float)(z.im - c.y) * ih);
 	const float2* a = L.locu32 + (size_t)node * (L.order + 1);
 	cf h;
-	for (int i = L.order; i >= 1; i--)
+	for (int i0 = L.order; i0 >= 1; i0 -= 8)
 	{
-		const float2 ci = a[i];
-		h = h * dz + cf(ci.x * i, ci.y * i);
+		float2 cb[8];
+#pragma unroll
+		for (int k = 0; k < 8; k++) cb[k] = a[max(i0 - k, 0)];
+#pragma unroll
+		for (int k = 0; k < 8; k++)
+		{
+			const int i = i0 - k;
+			if (i < 1) break;
+			h = h * dz + cf(cb[k].x * i, cb[k].y * i);
+		}
 	}
 	return cd(h.re * (double)ih, h.im * (double)ih);
 }
@@ -244,7 +281,7 @@ struct Witt
 };
 
 // the rest of witt_f0 once the near-field sums of z's node are known
-__device__ inline Witt witt_from(const LensView& L, cd z, double phi, int node, const Near& nr)
+__device__ inline Witt witt_from(const LensView& L, cd z, double cp, double sp, int node, const Near& nr)
 {
 	Witt w;
 	w.node = node;
@@ -253,8 +290,6 @@ __device__ inline Witt witt_from(const LensView& L, cd z, double phi, int node, 
 	cd d1, d2;
 	smooth_dzbar(L, z, d1, d2);
 	w.a = 1 - L.kappa - smooth_dz(L, z);
-	double sp, cp;
-	sincos(phi, &sp, &cp);
 	const double ih2 = 1.0 / (w.fr.hl * w.fr.hl);
 	w.f0 = cd(L.gamma) + w.nr.s2 * L.theta2 - w.fr.h2 * (L.theta2 * ih2) - d1.conj() - w.a * cd(cp, -sp);
 	const double a0 = w.f0.abs();
@@ -265,7 +300,9 @@ __device__ inline Witt witt_from(const LensView& L, cd z, double phi, int node, 
 __device__ inline Witt witt_f0(const LensView& L, cd z, double phi)
 {
 	const int node = nearest_node(L, z);
-	return witt_from(L, z, phi, node, p2p(L, node, z));
+	double sp, cp;
+	sincos(phi, &sp, &cp);
+	return witt_from(L, z, cp, sp, node, p2p(L, node, z));
 }
 
 // f1 = d f0 / dz from the sums already formed for f0 (ccf_functions.cuh:108-121)

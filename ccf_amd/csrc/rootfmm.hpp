@@ -124,7 +124,7 @@ __global__ __launch_bounds__(1024) void k_rscan(RootView V, unsigned long long* 
 {
 	const int r = blockIdx.x;
 	if (!V.rmode[r]) return;
-	if (threadIdx.x == 0) atomicAdd(&stats[C_RROWS], 1ull);
+	if (threadIdx.x == 0) atomicAdd(stat_slot(stats, C_RROWS), 1ull);
 	__shared__ int wsum[16];
 	__shared__ int carry;
 	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -379,8 +379,9 @@ __global__ __launch_bounds__(BLOCK) void k_rdown(RootView V, int l, unsigned lon
 #pragma unroll
 		for (int k = 0; k <= P; k++) out[k] = make_double2(Lc[k].re, Lc[k].im);
 	}
-	nt = wave_sum(nt);
-	if ((threadIdx.x & 63) == 0 && nt) atomicAdd(&stats[C_RM2L], nt);
+	const int which[1] = {C_RM2L};
+	unsigned long long v[1] = {nt};
+	block_stats(stats, which, v);
 }
 
 // per listed root of a row in FMM mode, in cell order: far field by L2P,
@@ -447,13 +448,9 @@ __global__ __launch_bounds__(BLOCK) void k_rsum(RootView V, const double2* __res
 			nt = 1;
 		}
 	}
-	nn = wave_sum(nn);
-	nt = wave_sum(nt);
-	if ((threadIdx.x & 63) == 0)
-	{
-		if (nn) atomicAdd(&stats[C_RNEAR], nn);
-		if (nt) atomicAdd(&stats[C_RTGT], nt);
-	}
+	const int which[2] = {C_RNEAR, C_RTGT};
+	unsigned long long v[2] = {nn, nt};
+	block_stats(stats, which, v);
 }
 
 // listed roots outside the grid: direct sums over the whole row, one block
@@ -506,7 +503,7 @@ __global__ __launch_bounds__(BLOCK) void k_rsum_out(RootView V, const double2* _
 				sy += red[1][w];
 			}
 			V.rsum[(size_t)r * R + s] = make_double2(-sx, sy);
-			atomicAdd(&stats[C_RTGT], 1ull);
+			atomicAdd(stat_slot(stats, C_RTGT), 1ull);
 		}
 		__syncthreads();
 	}

@@ -91,6 +91,70 @@ __device__ inline int wave_append(int* counter, bool pred)
 	return pred ? base + (int)__popcll(m & below) : -1;
 }
 
+// Device statistics are spread over STAT_SLOTS copies of each counter, one
+// per 128-byte line, and added once per workgroup: atomics from thousands of
+// workgroups on one address serialise at the L2. The host folds the copies.
+constexpr int STAT_SLOTS = 64;
+constexpr int STAT_STRIDE = 16; // unsigned long longs per slot (128 bytes)
+constexpr size_t STAT_WORDS = (size_t)C_COUNT * STAT_SLOTS * STAT_STRIDE;
+
+__device__ inline unsigned long long* stat_slot(unsigned long long* stats, int c)
+{
+	return stats + ((size_t)c * STAT_SLOTS + ((blockIdx.x + 31u * blockIdx.y) & (STAT_SLOTS - 1))) * STAT_STRIDE;
+}
+
+// adds v[k] of every thread of the block to counter which[k]; every thread
+// of a BLOCK-sized workgroup calls this
+template <int NC>
+__device__ inline void block_stats(unsigned long long* stats, const int (&which)[NC], unsigned long long (&v)[NC])
+{
+	__shared__ unsigned long long red[NC][BLOCK / 64];
+	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+	for (int k = 0; k < NC; k++)
+	{
+		const unsigned long long s = wave_sum(v[k]);
+		if (lane == 0) red[k][w] = s;
+	}
+	__syncthreads();
+	if (threadIdx.x < NC)
+	{
+		unsigned long long s = 0;
+		for (int q = 0; q < BLOCK / 64; q++) s += red[threadIdx.x][q];
+		if (s) atomicAdd(stat_slot(stats, which[threadIdx.x]), s);
+	}
+	__syncthreads();
+}
+
+// Append to a list whose length is *counter with one atomic per workgroup;
+// every thread of a BLOCK-sized workgroup calls this. Returns the slot of a
+// thread with pred, -1 otherwise.
+__device__ inline int block_append(int* counter, bool pred)
+{
+	__shared__ int wcnt[BLOCK / 64];
+	__shared__ int base;
+	const unsigned long long m = __ballot(pred);
+	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	if (lane == 0) wcnt[w] = (int)__popcll(m);
+	__syncthreads();
+	if (threadIdx.x == 0)
+	{
+		int t = 0;
+		for (int q = 0; q < BLOCK / 64; q++)
+		{
+			const int c = wcnt[q];
+			wcnt[q] = t;
+			t += c;
+		}
+		base = t ? atomicAdd(counter, t) : 0;
+	}
+	__syncthreads();
+	const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+	const int pos = pred ? base + wcnt[w] + (int)__popcll(m & below) : -1;
+	__syncthreads();
+	return pos;
+}
+
 // 1/x for x > 0: hardware reciprocal estimate refined by two Newton steps
 // (each squares the relative error of the ~2^-23 estimate)
 __device__ inline double rcp64(double x)
@@ -105,10 +169,19 @@ __device__ inline double rcp64(double x)
 // phi-row index and angle of solved row r of a phase (ccf_functions.cuh:338-374).
 // rows_per_branch 1: the branch centres (j = 0); 2: both sides of step j;
 // 0: row r of a caller-supplied array, all at angle phi_fixed.
+// angle of phi step j on side sgn of branch c (ccf_functions.cuh:338-374)
+__device__ inline double branch_phi(int c, int sgn, int j, int np, int nb)
+{
+	const double phi0 = PI / nb + c * 2 * PI / nb;
+	const double dphi = 2 * PI / np * j;
+	return phi0 + sgn * dphi;
+}
+
 struct RowMap
 {
 	int R, np, nb, j, rows_per_branch;
 	double phi_fixed;
+	const double2* cs = nullptr; // (cos phi, sin phi) of every phi row, or null
 	__device__ inline void get(int r, int& row, double& phi) const
 	{
 		if (rows_per_branch == 0)
@@ -121,11 +194,35 @@ struct RowMap
 		const int sgn = rows_per_branch == 1 ? 0 : ((r % rows_per_branch) == 0 ? -1 : 1);
 		const int center = np / (2 * nb) + c * (np / nb + 1);
 		row = center + sgn * j;
-		const double phi0 = PI / nb + c * 2 * PI / nb;
-		const double dphi = 2 * PI / np * j;
-		phi = phi0 + sgn * dphi;
+		phi = branch_phi(c, sgn, j, np, nb);
+	}
+	__device__ inline void phase(int row, double phi, double& cp, double& sp) const
+	{
+		if (cs && rows_per_branch != 0)
+		{
+			const double2 t = cs[row];
+			cp = t.x;
+			sp = t.y;
+		}
+		else
+		{
+			sincos(phi, &sp, &cp);
+		}
 	}
 };
+
+// (cos phi, sin phi) of every phi row of the output, row = c (np/nb + 1) + k
+__global__ void k_phase(int np, int nb, double2* cs)
+{
+	const int row = blockIdx.x * blockDim.x + threadIdx.x;
+	if (row >= np + nb) return;
+	const int per = np / nb + 1;
+	const int c = row / per, k = row % per - np / (2 * nb);
+	const int sgn = k < 0 ? -1 : 1;
+	double sp, cp;
+	sincos(branch_phi(c, sgn, sgn * k, np, nb), &sp, &cp);
+	cs[row] = make_double2(cp, sp);
+}
 
 // device pointers of a SweepBufs, with the parity of the current sweep applied
 struct SweepView
@@ -242,7 +339,9 @@ __global__ __launch_bounds__(BLOCK) void k_witt(LensView L, double2* __restrict_
 		}
 		else
 		{
-			const Witt w = witt_from(L, z, phi, node, p2p(L, node, z));
+			double cp, sp;
+			M.phase(row, phi, cp, sp);
+			const Witt w = witt_from(L, z, cp, sp, node, p2p(L, node, z));
 			full = witt_root<MIXED>(L, p, z, w, circ_exact, fmm_pole, f1, ex);
 			if (full)
 			{
@@ -256,9 +355,9 @@ __global__ __launch_bounds__(BLOCK) void k_witt(LensView L, double2* __restrict_
 			}
 		}
 	}
-	const int dpos = wave_append(&V.dcnt[r], deferred);
+	const int dpos = block_append(&V.dcnt[r], deferred);
 	if (deferred) V.dl[(size_t)r * R + dpos] = a;
-	const int pos = wave_append(&V.fcnt[r], full);
+	const int pos = block_append(&V.fcnt[r], full);
 	if (full)
 	{
 		const size_t k = (size_t)r * R + pos;
@@ -268,19 +367,9 @@ __global__ __launch_bounds__(BLOCK) void k_witt(LensView L, double2* __restrict_
 		V.f1[k] = make_double2(f1.re, f1.im);
 		V.ex[k] = make_double2(ex.re, ex.im);
 	}
-	nfull = wave_sum(nfull);
-	nearly = wave_sum(nearly);
-	nnear = wave_sum(nnear);
-	if ((threadIdx.x & 63) == 0)
-	{
-		if (nfull) atomicAdd(&stats[C_FULL], nfull);
-		if (nearly)
-		{
-			atomicAdd(&stats[C_EARLY], nearly);
-			atomicAdd(&stats[C_CONV], nearly);
-		}
-		if (nnear) atomicAdd(&stats[C_NEAR], nnear);
-	}
+	const int which[4] = {C_FULL, C_EARLY, C_CONV, C_NEAR};
+	unsigned long long v[4] = {nfull, nearly, nearly, nnear};
+	block_stats(stats, which, v);
 }
 
 // K1 for the deferred roots: one workgroup per root sums the star ranges of
@@ -341,7 +430,9 @@ __global__ __launch_bounds__(BLOCK) void k_witt_far(LensView L, double2* __restr
 			sum.s2 = cd(t[4], t[5]);
 			sum.s3 = cd(t[6], t[7]);
 			sum.count = nr.count;
-			const Witt w = witt_from(L, z, phi, node, sum);
+			double cp, sp;
+			M.phase(row, phi, cp, sp);
+			const Witt w = witt_from(L, z, cp, sp, node, sum);
 			cd f1, ex;
 			if (witt_root<MIXED>(L, p, z, w, circ_exact, fmm_pole, f1, ex))
 			{
@@ -352,13 +443,13 @@ __global__ __launch_bounds__(BLOCK) void k_witt_far(LensView L, double2* __restr
 				V.f0[k] = make_double2(w.f0.re, w.f0.im);
 				V.f1[k] = make_double2(f1.re, f1.im);
 				V.ex[k] = make_double2(ex.re, ex.im);
-				atomicAdd(&stats[C_FULL], 1ull);
-				atomicAdd(&stats[C_NEAR], (unsigned long long)sum.count);
+				atomicAdd(stat_slot(stats, C_FULL), 1ull);
+				atomicAdd(stat_slot(stats, C_NEAR), (unsigned long long)sum.count);
 			}
 			else
 			{
-				atomicAdd(&stats[C_EARLY], 1ull);
-				atomicAdd(&stats[C_CONV], 1ull);
+				atomicAdd(stat_slot(stats, C_EARLY), 1ull);
+				atomicAdd(stat_slot(stats, C_CONV), 1ull);
 			}
 		}
 		__syncthreads();
@@ -635,7 +726,7 @@ __global__ __launch_bounds__(BLOCK) void k_sums32(const float2* __restrict__ spo
 				make_double2(2.0 * ax[k] - bx[k], -(2.0 * ay[k] - by[k]));
 	}
 	nclus = wave_sum(nclus);
-	if ((threadIdx.x & 63) == 0 && nclus) atomicAdd(&stats[C_CLUSTER], nclus);
+	if ((threadIdx.x & 63) == 0 && nclus) atomicAdd(stat_slot(stats, C_CLUSTER), nclus);
 }
 
 // K4 (ccf_functions.cuh:241-242 and 379-389)
@@ -684,10 +775,11 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(double theta, double2* __res
 		if ((zn - z).abs() / theta < 1e-9) nc = 1;
 		else again = true;
 	}
-	const int pos = wave_append(&V.cnt_next[r], again);
+	const int pos = block_append(&V.cnt_next[r], again);
 	if (again) V.act_next[(size_t)r * R + pos] = a;
-	nc = wave_sum(nc);
-	if ((threadIdx.x & 63) == 0 && nc) atomicAdd(&stats[C_CONV], nc);
+	const int which[1] = {C_CONV};
+	unsigned long long v[1] = {nc};
+	block_stats(stats, which, v);
 }
 
 // initial roots of a phase: row centre+sgn*j from the rows of the previous
@@ -740,7 +832,8 @@ __global__ void k_prepare(double2* roots, RowMap M, SweepBufs B, int order)
 // 1/mu error bound at every point of the solved rows, reduced to its max
 // with a non-finite flag (find_errors_kernel + has_nan_err_kernel +
 // thrust::max_element, ccf_functions.cuh:419-489)
-__global__ void k_errors(LensView L, const double2* __restrict__ roots, int R, int np, int nb, unsigned long long* stats)
+__global__ void k_errors(LensView L, const double2* __restrict__ roots, int R, int np, int nb, const double2* __restrict__ cs,
+	unsigned long long* stats)
 {
 	const long long total = (long long)(np + nb) * R;
 	const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -749,13 +842,11 @@ __global__ void k_errors(LensView L, const double2* __restrict__ roots, int R, i
 	if (idx < total)
 	{
 		const int row = (int)(idx / R);
-		const int per = np / nb + 1;
-		const int c = row / per;
-		const int k = row % per - np / (2 * nb);
-		const int sgn = k < 0 ? -1 : 1;
-		const double phi = PI / nb + c * 2 * PI / nb + sgn * (2 * PI / np * (sgn * k));
+		const double2 t = cs[row];
 		const double2 v = roots[idx];
-		Witt w = witt_f0(L, cd(v.x, v.y), phi);
+		const cd z(v.x, v.y);
+		const int node = nearest_node(L, z);
+		Witt w = witt_from(L, z, t.x, t.y, node, p2p(L, node, z));
 		e = err_bound(w);
 		if (!isfinite(e)) { bad = 1; e = 0; }
 	}
@@ -763,8 +854,8 @@ __global__ void k_errors(LensView L, const double2* __restrict__ roots, int R, i
 	bad = wave_sum(bad);
 	if ((threadIdx.x & 63) == 0)
 	{
-		atomicMax(&stats[C_ERRBITS], (unsigned long long)__double_as_longlong(e));
-		if (bad) atomicOr(&stats[C_NAN], 1ull);
+		atomicMax(stat_slot(stats, C_ERRBITS), (unsigned long long)__double_as_longlong(e));
+		if (bad) atomicOr(stat_slot(stats, C_NAN), 1ull);
 	}
 }
 
@@ -870,6 +961,15 @@ SumsLaunch sums_launch(ccf_ctx* ctx, bool f32)
 
 // `cap` sweeps over the rows of M (ccf.cuh:929-939 and 1013-1018): the
 // sweep kernels exit at once when no root of a row is active any more
+// whether to read the active counts back after `done` of `cap` sweeps: after
+// sweeps 2 and 3 (a phi step with a continuation predictor mostly ends
+// there), then every 4th
+inline bool sweep_check(int done, int cap)
+{
+	if (done >= cap) return false;
+	return done == 2 || done == 3 || done % 4 == 0;
+}
+
 // The first fmm_sweeps sweeps may take the root sums of their dense rows
 // from the FMM pass (when the buffers carry a root grid).
 bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int nrows, int cap, int fmm_sweeps)
@@ -934,6 +1034,18 @@ bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int
 		k_finalize<<<g_rows, BLOCK, 0, s>>>(ctx->theta, roots, M, V, ctx->counters);
 		if (!CCF_LAUNCH(ctx, "find_critical_curve_roots_kernel")) return false;
 		ctx->info.sweeps++;
+		// stop once no root of any row is active: the remaining sweeps up to
+		// the cap would do nothing (checked at a few sweeps only, each check
+		// waits for the device)
+		if (ctx->host_counts && nrows <= ctx->max_rows && sweep_check(it + 1, cap))
+		{
+			if (!CCF_HIP(ctx, hipMemcpyAsync(ctx->host_counts, V.cnt_next, nrows * sizeof(int), hipMemcpyDeviceToHost, s)) ||
+				!CCF_HIP(ctx, hipStreamSynchronize(s)))
+				return false;
+			long long active = 0;
+			for (int r = 0; r < nrows; r++) active += ctx->host_counts[r];
+			if (active == 0) break;
+		}
 	}
 	return true;
 }
@@ -1017,15 +1129,24 @@ bool alloc_sweep(ccf_ctx* ctx, SweepBufs& b, int rows, int R)
 	return alloc_root_fmm(ctx, b);
 }
 
+bool launch_phase(ccf_ctx* ctx)
+{
+	const int rows = ctx->P.num_phi + ctx->P.num_branches;
+	k_phase<<<grid(rows), BLOCK, 0, ctx->stream>>>(ctx->P.num_phi, ctx->P.num_branches, ctx->phase);
+	return CCF_LAUNCH(ctx, "phase table");
+}
+
 bool alloc_solver(ccf_ctx* ctx)
 {
 	const int R = ctx->num_roots, np = ctx->P.num_phi, nb = ctx->P.num_branches;
 	const size_t pts = (size_t)(np + nb) * R;
 	ctx->max_rows = 2 * nb;
 	return CCF_HIP(ctx, hipMalloc(&ctx->ccs_init, pts * sizeof(double2))) && alloc_sweep(ctx, ctx->sw, ctx->max_rows, R) &&
-		CCF_HIP(ctx, hipMalloc(&ctx->counters, C_COUNT * sizeof(unsigned long long))) &&
+		CCF_HIP(ctx, hipHostMalloc((void**)&ctx->host_counts, ctx->max_rows * sizeof(int), hipHostMallocDefault)) &&
+		CCF_HIP(ctx, hipMalloc(&ctx->phase, (size_t)(np + nb) * sizeof(double2))) && launch_phase(ctx) &&
+		CCF_HIP(ctx, hipMalloc(&ctx->counters, STAT_WORDS * sizeof(unsigned long long))) &&
 		CCF_HIP(ctx, hipMemsetAsync(ctx->ccs_init, 0, pts * sizeof(double2), ctx->stream)) &&
-		CCF_HIP(ctx, hipMemsetAsync(ctx->counters, 0, C_COUNT * sizeof(unsigned long long), ctx->stream));
+		CCF_HIP(ctx, hipMemsetAsync(ctx->counters, 0, STAT_WORDS * sizeof(unsigned long long), ctx->stream));
 }
 
 bool find_initial_roots_dev(ccf_ctx* ctx, int verbose)
@@ -1041,7 +1162,7 @@ bool find_initial_roots_dev(ccf_ctx* ctx, int verbose)
 		if (!CCF_HIP(ctx, hipMemcpyAsync(ctx->ccs_init + (size_t)center * R, ctx->seeds.data(), (size_t)R * sizeof(double2), hipMemcpyHostToDevice, s)))
 			return false;
 	}
-	const RowMap M{R, np, nb, 0, 1, 0.0};
+	const RowMap M{R, np, nb, 0, 1, 0.0, ctx->phase};
 	k_prepare<<<dim3(grid(R), nb), BLOCK, 0, s>>>(ctx->ccs_init, M, ctx->sw, 0);
 	if (!CCF_LAUNCH(ctx, "prepare_roots_kernel")) return false;
 	const int cap = ctx->P.init_iters > 0 ? ctx->P.init_iters : 60;
@@ -1063,7 +1184,7 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 	auto t0 = std::chrono::steady_clock::now();
 	for (int j = 1; j <= steps; j++)
 	{
-		const RowMap M{R, np, nb, j, 2, 0.0};
+		const RowMap M{R, np, nb, j, 2, 0.0, ctx->phase};
 		// the exact smooth sheets are discontinuous at the field's edge
 		// (boxcar / heaviside terms), where extrapolated roots could cross
 		// onto another branch: those keep the previous step's roots
@@ -1087,11 +1208,25 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 	// max 1/mu error and the NaN abort (ccf.cuh:1039-1069)
 	const long long total = (long long)(np + nb) * R;
 	const LensView L = ctx->lens();
-	k_errors<<<grid(total), BLOCK, 0, s>>>(L, ctx->ccs_init, R, np, nb, ctx->counters);
+	k_errors<<<grid(total), BLOCK, 0, s>>>(L, ctx->ccs_init, R, np, nb, ctx->phase, ctx->counters);
 	if (!CCF_LAUNCH(ctx, "find_errors_kernel")) return false;
-	unsigned long long c[C_COUNT];
-	if (!CCF_HIP(ctx, hipMemcpyAsync(c, ctx->counters, sizeof(c), hipMemcpyDeviceToHost, s)) || !CCF_HIP(ctx, hipStreamSynchronize(s)))
-		return false;
+	unsigned long long c[C_COUNT] = {};
+	{
+		// fold the per-slot copies: max for the error bits, or for the NaN
+		// flag, sums otherwise
+		std::vector<unsigned long long> raw(STAT_WORDS);
+		if (!CCF_HIP(ctx, hipMemcpyAsync(raw.data(), ctx->counters, STAT_WORDS * sizeof(unsigned long long), hipMemcpyDeviceToHost, s)) ||
+			!CCF_HIP(ctx, hipStreamSynchronize(s)))
+			return false;
+		for (int k = 0; k < C_COUNT; k++)
+			for (int q = 0; q < STAT_SLOTS; q++)
+			{
+				const unsigned long long v = raw[((size_t)k * STAT_SLOTS + q) * STAT_STRIDE];
+				if (k == C_ERRBITS) c[k] = std::max(c[k], v);
+				else if (k == C_NAN) c[k] |= v;
+				else c[k] += v;
+			}
+	}
 	if (c[C_NAN])
 	{
 		ctx->fail("Error. Errors in 1/mu contain values which are not positive real numbers.\n");
@@ -1225,7 +1360,7 @@ bool root_sums_dev(ccf_ctx* ctx, const double* roots, int n, double* out)
 	for (int i = 0; i < n; i++) iota[i] = i;
 	std::vector<double2> res(n);
 	bool ok = CCF_HIP(ctx, hipMalloc(&row, (size_t)n * sizeof(double2))) && alloc_root_fmm(ctx, B) &&
-		CCF_HIP(ctx, hipMalloc(&stats, C_COUNT * sizeof(unsigned long long))) &&
+		CCF_HIP(ctx, hipMalloc(&stats, STAT_WORDS * sizeof(unsigned long long))) &&
 		CCF_HIP(ctx, hipMemcpyAsync(row, roots, (size_t)n * sizeof(double2), hipMemcpyHostToDevice, s)) &&
 		CCF_HIP(ctx, hipMemcpyAsync(B.slot, iota.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, s)) &&
 		CCF_HIP(ctx, hipMemcpyAsync(B.rmode, one.data(), sizeof(int), hipMemcpyHostToDevice, s));

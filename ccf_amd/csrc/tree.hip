@@ -270,6 +270,32 @@ __global__ void k_node_box(int nnodes, const double2* __restrict__ center, const
 	nbox[n] = make_double4(center[n].x, center[n].y, hl[n], (double)child[n]);
 }
 
+// point-location table: cell (ix, iy) of the 2^lv x 2^lv grid over the root
+// box gets the node the descent (tree_node.cuh:529-561) reaches from the
+// cell's centre after at most lv levels
+constexpr int MAX_LOOKUP_LEVEL = 10;
+
+__global__ void k_lookup(int lv, double x0, double cell, const double4* __restrict__ nbox, int* lookup)
+{
+	const int n = 1 << lv;
+	const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (c >= (long long)n * n) return;
+	const double px = x0 + ((int)(c % n) + 0.5) * cell, py = x0 + ((int)(c / n) + 0.5) * cell;
+	int node = 0;
+	for (int l = 0; l < lv; l++)
+	{
+		const double4 b = nbox[node];
+		if (b.w < 0) break;
+		int q;
+		if (px >= b.x && py >= b.y) q = 0;
+		else if (px < b.x && py >= b.y) q = 1;
+		else if (px < b.x && py < b.y) q = 2;
+		else q = 3;
+		node = (int)b.w + q;
+	}
+	lookup[c] = node;
+}
+
 // number of near-field stars of every leaf (0 for other nodes)
 __global__ void k_near_count(int nnodes, const int* __restrict__ child, const int2* __restrict__ nf, const int* __restrict__ nnf,
 	int* cnt)
@@ -604,6 +630,20 @@ bool build_tree(ccf_ctx* ctx, int verbose)
 			!CCF_HIP(ctx, hipMalloc(&T.nf_cnt, sizeof(int) * nn)))
 			return false;
 		k_node_box<<<grid(nn), BLOCK, 0, s>>>(nn, T.center, T.hl, T.child, T.nbox);
+		// point-location table over the root box (centre 0, half-length root_half_length)
+		if (T.lookup) (void)hipFree(T.lookup);
+		T.lookup = nullptr;
+		T.lookup_level = 0;
+		if (T.levels >= 2)
+		{
+			T.lookup_level = std::min(T.levels, MAX_LOOKUP_LEVEL);
+			const int n = 1 << T.lookup_level;
+			const double H = ctx->root_half_length;
+			T.lookup_x0 = -H;
+			T.lookup_inv = n / (2 * H);
+			if (!CCF_HIP(ctx, hipMalloc(&T.lookup, sizeof(int) * (size_t)n * n))) return false;
+			k_lookup<<<grid((long long)n * n), BLOCK, 0, s>>>(T.lookup_level, -H, 2 * H / n, T.nbox, T.lookup);
+		}
 		k_near_count<<<grid(nn), BLOCK, 0, s>>>(nn, T.child, T.nf, T.nnf, T.nf_cnt);
 		std::vector<int> cnt(nn), off(nn);
 		if (!CCF_LAUNCH(ctx, "node boxes") || !CCF_HIP(ctx, hipMemcpyAsync(cnt.data(), T.nf_cnt, sizeof(int) * nn, hipMemcpyDeviceToHost, s)) ||
