@@ -285,103 +285,132 @@ __global__ __launch_bounds__(BLOCK) void k_rm2m(RootView V, int l)
 	for (int k = 0; k <= P; k++) out[k] = make_double2(A[k].re, A[k].im);
 }
 
-// L2L from the parent and M2L from the interaction list, level l
+// L2L and M2L of level l with 32 lanes per target cell: lane k translates
+// the k-th (and k+32-th) source of the target's interaction list into a
+// partial local expansion, the partials are added through LDS in lane
+// order, and lane m adds coefficient m of the parent's shifted expansion.
 template <int P>
-__global__ __launch_bounds__(BLOCK) void k_rdown(RootView V, int l, unsigned long long* __restrict__ stats)
+constexpr int down_targets() { return (P + 1) <= 14 ? 8 : 4; } // 32 (P+1) double2 of LDS per target
+
+template <int P>
+__global__ __launch_bounds__(32 * down_targets<P>()) void k_rdown32(RootView V, int l, unsigned long long* __restrict__ stats)
 {
+	constexpr int TPB = down_targets<P>();
+	__shared__ double2 part[TPB][32][P + 1];
+	__shared__ unsigned long long nsum[TPB * 32 / 64];
 	const int r = blockIdx.y;
 	if (!V.rmode[r]) return;
 	const int nxl = V.g.nx << l, nyl = V.g.ny << l;
-	const int c = blockIdx.x * BLOCK + threadIdx.x;
-	unsigned long long nt = 0;
+	const int tl = threadIdx.x >> 5, lane = threadIdx.x & 31;
+	const int c = blockIdx.x * TPB + tl;
 	const size_t base = (size_t)r * V.g.ncells;
 	const size_t cid = base + V.g.off[l] + c;
-	if (c < nxl * nyl && V.ccnt[cid] > 0)
-	{
-		const int ix = c % nxl, iy = c / nxl;
-		cd Lc[P + 1];
+	const bool live = c < nxl * nyl && V.ccnt[cid] > 0;
+	const int ix = live ? c % nxl : 0, iy = live ? c / nxl : 0;
+	const double ih = 2.0 / ldexp(V.g.s0, -l);
+	cd Lc[P + 1];
 #pragma unroll
-		for (int k = 0; k <= P; k++) Lc[k] = cd(0, 0);
-		if (l >= 1)
+	for (int k = 0; k <= P; k++) Lc[k] = cd(0, 0);
+	unsigned long long nt = 0;
+	auto m2l = [&](int jx, int jy) {
+		const size_t sid = base + V.g.off[l] + (size_t)jy * nxl + jx;
+		if (V.ccnt[sid] == 0) return;
+		nt++;
+		const double2* src = V.mp + sid * (P + 1);
+		double2 am[P + 1];
+#pragma unroll
+		for (int n = 0; n <= P; n++) am[n] = src[n];
+		const cd rho = cd(2.0 * (ix - jx), 2.0 * (iy - jy)).inv();
+		cd bq[P + 1];
+		cd t(1, 0);
+#pragma unroll
+		for (int n = 0; n <= P; n++)
 		{
-			const double2* lp = V.lc + (base + V.g.off[l - 1] + (size_t)(iy >> 1) * (nxl >> 1) + (ix >> 1)) * (P + 1);
-			cd Lp[P + 1], ep[P + 1];
-			const cd e((ix & 1) - 0.5, (iy & 1) - 0.5);
-			ep[0] = cd(1, 0);
-#pragma unroll
-			for (int m = 0; m <= P; m++)
-			{
-				Lp[m] = cd(lp[m].x, lp[m].y);
-				if (m > 0) ep[m] = ep[m - 1] * e;
-			}
-			double sc = 1;
-#pragma unroll
-			for (int k = 0; k <= P; k++)
-			{
-				cd s(0, 0);
-#pragma unroll
-				for (int m = k; m <= P; m++) s += rbinom(m, k) * (Lp[m] * ep[m - k]);
-				Lc[k] = s * sc;
-				sc *= 0.5;
-			}
+			bq[n] = cd(am[n].x, am[n].y) * t;
+			t = t * rho;
 		}
-		const double ih = 2.0 / ldexp(V.g.s0, -l);
-		auto m2l = [&](int jx, int jy) {
-			const size_t sid = base + V.g.off[l] + (size_t)jy * nxl + jx;
-			if (V.ccnt[sid] == 0) return;
-			nt++;
-			const double2* src = V.mp + sid * (P + 1);
-			const cd rho = cd(2.0 * (ix - jx), 2.0 * (iy - jy)).inv();
-			cd bq[P + 1];
-			cd t(1, 0);
+		cd rp = rho * ih;
+#pragma unroll
+		for (int m = 0; m <= P; m++)
+		{
+			double sx = 0, sy = 0;
 #pragma unroll
 			for (int n = 0; n <= P; n++)
 			{
-				bq[n] = cd(src[n].x, src[n].y) * t;
-				t = t * rho;
+				const double bc = rbinom(n + m, m);
+				sx = fma(bc, bq[n].re, sx);
+				sy = fma(bc, bq[n].im, sy);
 			}
-			cd rp = rho * ih;
-#pragma unroll
-			for (int m = 0; m <= P; m++)
-			{
-				double sx = 0, sy = 0;
-#pragma unroll
-				for (int n = 0; n <= P; n++)
-				{
-					const double bc = rbinom(n + m, m);
-					sx = fma(bc, bq[n].re, sx);
-					sy = fma(bc, bq[n].im, sy);
-				}
-				const cd v = rp * cd(sx, sy);
-				if (m & 1) Lc[m] -= v;
-				else Lc[m] += v;
-				rp = rp * rho;
-			}
-		};
+			const cd v = rp * cd(sx, sy);
+			if (m & 1) Lc[m] -= v;
+			else Lc[m] += v;
+			rp = rp * rho;
+		}
+	};
+	if (live)
+	{
 		if (l == 0)
 		{
-			for (int jy = 0; jy < nyl; jy++)
-				for (int jx = 0; jx < nxl; jx++)
-					if (abs(jx - ix) >= 2 || abs(jy - iy) >= 2) m2l(jx, jy);
+			for (int k = lane; k < nxl * nyl; k += 32)
+			{
+				const int jx = k % nxl, jy = k / nxl;
+				if (abs(jx - ix) >= 2 || abs(jy - iy) >= 2) m2l(jx, jy);
+			}
 		}
 		else
 		{
-			const int px = ix >> 1, py = iy >> 1;
-			for (int qy = max(py - 1, 0); qy <= min(py + 1, (nyl >> 1) - 1); qy++)
-				for (int qx = max(px - 1, 0); qx <= min(px + 1, (nxl >> 1) - 1); qx++)
-					for (int q = 0; q < 4; q++)
-					{
-						const int jx = 2 * qx + (q & 1), jy = 2 * qy + (q >> 1);
-						if (abs(jx - ix) >= 2 || abs(jy - iy) >= 2) m2l(jx, jy);
-					}
+			// the 36 children of the parent's 3 x 3 block
+			const int px = ix >> 1, py = iy >> 1, npx = nxl >> 1, npy = nyl >> 1;
+			for (int k = lane; k < 36; k += 32)
+			{
+				const int b = k >> 2, q = k & 3;
+				const int qx = px - 1 + b % 3, qy = py - 1 + b / 3;
+				if (qx < 0 || qy < 0 || qx >= npx || qy >= npy) continue;
+				const int jx = 2 * qx + (q & 1), jy = 2 * qy + (q >> 1);
+				if (abs(jx - ix) >= 2 || abs(jy - iy) >= 2) m2l(jx, jy);
+			}
 		}
-		double2* out = V.lc + cid * (P + 1);
-#pragma unroll
-		for (int k = 0; k <= P; k++) out[k] = make_double2(Lc[k].re, Lc[k].im);
 	}
-	const int which[1] = {C_RM2L};
-	unsigned long long v[1] = {nt};
-	block_stats(stats, which, v);
+#pragma unroll
+	for (int k = 0; k <= P; k++) part[tl][lane][k] = make_double2(Lc[k].re, Lc[k].im);
+	{
+		const unsigned long long s = wave_sum(nt);
+		if ((threadIdx.x & 63) == 0) nsum[threadIdx.x >> 6] = s;
+	}
+	__syncthreads();
+	if (live && lane <= P)
+	{
+		const int m = lane;
+		double sx = 0, sy = 0;
+		for (int k = 0; k < 32; k++)
+		{
+			const double2 v = part[tl][k][m];
+			sx += v.x;
+			sy += v.y;
+		}
+		if (l >= 1)
+		{
+			// L2L: 2^-m sum_{n >= m} C(n, m) L_n(parent) e^(n - m)
+			const double2* lp = V.lc + (base + V.g.off[l - 1] + (size_t)(iy >> 1) * (nxl >> 1) + (ix >> 1)) * (P + 1);
+			const cd e((ix & 1) - 0.5, (iy & 1) - 0.5);
+			cd s(0, 0), ep(1, 0);
+			for (int n = m; n <= P; n++)
+			{
+				s += rbinom(n, m) * (cd(lp[n].x, lp[n].y) * ep);
+				ep = ep * e;
+			}
+			const double sc = ldexp(1.0, -m);
+			sx += s.re * sc;
+			sy += s.im * sc;
+		}
+		V.lc[cid * (P + 1) + m] = make_double2(sx, sy);
+	}
+	if (threadIdx.x == 0)
+	{
+		unsigned long long s = 0;
+		for (int w = 0; w < TPB * 32 / 64; w++) s += nsum[w];
+		if (s) atomicAdd(stat_slot(stats, C_RM2L), s);
+	}
 }
 
 // per listed root of a row in FMM mode, in cell order: far field by L2P,
@@ -523,8 +552,9 @@ void root_fmm_pass_p(const SweepBufs& B, const double2* roots, const RowMap& M, 
 	k_rleaf<P><<<dim3(grid(g.nleaf), nrows), BLOCK, 0, s>>>(V, roots, M);
 	for (int l = g.leaf - 1; l >= 0; l--)
 		k_rm2m<P><<<dim3(grid((g.nx << l) * (g.ny << l)), nrows), BLOCK, 0, s>>>(V, l);
+	constexpr int TPB = down_targets<P>();
 	for (int l = 0; l <= g.leaf; l++)
-		k_rdown<P><<<dim3(grid((g.nx << l) * (g.ny << l)), nrows), BLOCK, 0, s>>>(V, l, stats);
+		k_rdown32<P><<<dim3(grid((g.nx << l) * (g.ny << l), TPB), nrows), 32 * TPB, 0, s>>>(V, l, stats);
 	k_rsum<P><<<dim3(grid(R), nrows), BLOCK, 0, s>>>(V, roots, M, stats);
 	k_rsum_out<<<nrows, BLOCK, 0, s>>>(V, roots, M, stats);
 }
