@@ -146,6 +146,20 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         "ccf_eval_points": (ctypes.c_int, [vp, dp, ctypes.c_int, ctypes.c_double, dp]),
         "ccf_aberth_sweep": (ctypes.c_int, [vp, dp, ctypes.c_int, ctypes.c_double, dp]),
         "ccf_root_sums": (ctypes.c_int, [vp, dp, ctypes.c_int, dp]),
+        "ccf_batch_create": (vp, []),
+        "ccf_batch_destroy": (None, [vp]),
+        "ccf_batch_add": (ctypes.c_int, [vp, P]),
+        "ccf_batch_size": (ctypes.c_int, [vp]),
+        "ccf_batch_load": (ctypes.c_int, [vp, ctypes.c_int]),
+        "ccf_batch_build": (ctypes.c_int, [vp, ctypes.c_int]),
+        "ccf_batch_solve": (ctypes.c_int, [vp, ctypes.c_int]),
+        "ccf_batch_run": (ctypes.c_int, [vp, ctypes.c_int]),
+        "ccf_batch_save": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
+        "ccf_batch_get_info": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(Info)]),
+        "ccf_batch_get_array": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_longlong,
+                                               ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+        "ccf_batch_workgroup": (ctypes.c_int, [vp]),
+        "ccf_batch_last_error": (ctypes.c_char_p, [vp]),
         "ccf_write_synthetic_stars": (ctypes.c_int, [P, ctypes.c_ulonglong, ctypes.c_char_p]),
         "ccf_mass_function": (ctypes.c_double, [ctypes.c_char_p, ctypes.c_int, ctypes.c_double,
                                                 ctypes.c_double, ctypes.c_double, ctypes.c_double]),
@@ -289,6 +303,86 @@ class Context:
         self._check(self.lib.ccf_root_sums(self.ptr, _dptr(r.view(np.float64)), r.size, _dptr(out.view(np.float64))),
                     "ccf_root_sums")
         return out
+
+
+class Batch:
+    """Several star fields solved together (ccf_batch_* of include/ccf_amd.h)."""
+
+    def __init__(self):
+        self.lib = load()
+        self.ptr = self.lib.ccf_batch_create()
+        if not self.ptr:
+            raise MemoryError("ccf_batch_create failed")
+        self.params = []
+
+    def close(self):
+        if self.ptr:
+            self.lib.ccf_batch_destroy(self.ptr)
+            self.ptr = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        self.close()
+
+    def add(self, params: Params | None = None, **kw) -> int:
+        p = params if params is not None else make_params(**kw)
+        i = self.lib.ccf_batch_add(self.ptr, ctypes.byref(p))
+        if i < 0:
+            raise RuntimeError("ccf_batch_add failed")
+        self.params.append(p)
+        return i
+
+    def last_error(self) -> str:
+        return self.lib.ccf_batch_last_error(self.ptr).decode()
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what}: {self.last_error().strip()}")
+
+    def load(self, verbose: int = 0):
+        self._check(self.lib.ccf_batch_load(self.ptr, verbose), "ccf_batch_load")
+
+    def build(self, verbose: int = 0):
+        self._check(self.lib.ccf_batch_build(self.ptr, verbose), "ccf_batch_build")
+
+    def solve(self, verbose: int = 0):
+        self._check(self.lib.ccf_batch_solve(self.ptr, verbose), "ccf_batch_solve")
+
+    def run(self, verbose: int = 0) -> bool:
+        return self.lib.ccf_batch_run(self.ptr, verbose) == 0
+
+    def save(self, field: int, verbose: int = 0) -> bool:
+        return self.lib.ccf_batch_save(self.ptr, field, verbose) == 0
+
+    def info(self, field: int = -1) -> dict:
+        i = Info()
+        self._check(self.lib.ccf_batch_get_info(self.ptr, field, ctypes.byref(i)), "ccf_batch_get_info")
+        return i.as_dict()
+
+    def workgroup(self) -> int:
+        return self.lib.ccf_batch_workgroup(self.ptr)
+
+    def array(self, field: int, which: int) -> np.ndarray:
+        inf = self.info(field)
+        p = self.params[field]
+        R = inf["num_roots"]
+        npb = p.num_phi + p.num_branches
+        if which == ARRAY_STARS:
+            out = np.empty((inf["num_stars"], 3), dtype=np.float64)
+        elif which == ARRAY_MU_LENGTH_SCALES:
+            out = np.empty(R * npb, dtype=np.float64)
+        else:
+            out = np.empty(R * npb, dtype=np.complex128)
+        nr = ctypes.c_int()
+        nc = ctypes.c_int()
+        self._check(self.lib.ccf_batch_get_array(self.ptr, field, which, out.ctypes.data_as(ctypes.c_void_p), out.nbytes,
+                                                 ctypes.byref(nr), ctypes.byref(nc)), "ccf_batch_get_array")
+        return out.reshape(nr.value, nc.value)
 
 
 class CCF:

@@ -1,0 +1,788 @@
+// Many independent star fields solved together: the batched form.
+//
+// The reference solves one field per process (main_ccf.cu:514-515). Here a
+// batch of fields of uneven size shares every launch:
+//  * each field keeps its own context (parameters, stars, seeds, output rows,
+//    the files ccf_save writes), loaded as a single field would be;
+//  * their trees and FMM coefficients are built as one forest
+//    (build_forest), level by level for all fields at once;
+//  * the Aberth solves are two persistent launches over a task queue:
+//      phase 0, one task per (field, branch): the branch centre row from the
+//               seeds (ccf.cuh:906-977);
+//      phase 1, one task per (field, branch, side): the chain of phi steps of
+//               that side toward 0 or 2 pi (ccf.cuh:979-1083), each step
+//               predicted from the previous ones as in the single-field path.
+//    One workgroup runs a task. The R roots of its row live in LDS with the
+//    field's stars; a Jacobi sweep is three barrier-separated phases over a
+//    compacted list of the active roots: the Witt function and the 1/mu
+//    early exit (ccf_functions.cuh:197-213), the Cauchy sums over stars and
+//    roots and the Aberth update kept in registers (ccf_functions.cuh:
+//    214-242), then the commit and the |dz| / theta < 1e-9 test
+//    (ccf_functions.cuh:379-389). Workgroups take tasks from the queue,
+//    largest rows first, until it is empty.
+//  * the 1/mu error pass, the transpose and the caustic map run once over
+//    all fields (one grid dimension per field);
+//  * the launches of a solve are captured in a HIP graph and replayed.
+// An autotuner picks the workgroup size (roots per thread follow from the
+// largest row) by timing the phase-0 launch of the batch itself.
+#include "lens.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <iostream>
+#include <numeric>
+#include <vector>
+
+namespace ccf
+{
+namespace
+{
+
+constexpr int BLOCK = 256;
+constexpr int MAX_BATCH_RPT = 8;
+
+inline int grid(long long n, int b = BLOCK) { return (int)((n + b - 1) / b); }
+
+__device__ inline double rcp64(double x)
+{
+	double r = __builtin_amdgcn_rcp(x);
+	double e = fma(-x, r, 1.0);
+	r = fma(r, e, r);
+	e = fma(-x, r, 1.0);
+	return fma(r, e, r);
+}
+
+__device__ inline unsigned long long wave_sum(unsigned long long v)
+{
+	for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+	return v;
+}
+
+// one field of the batch as the kernels see it
+struct BField
+{
+	LensView L;              // lens model and tree (L.root = the field's root node)
+	double2* roots;          // [(np + nb)][R] phi rows, row-major
+	const double2* seeds;    // [R] initial roots
+	const double2* cs;       // (cos phi, sin phi) per phi row
+	const double2* stars;    // the field's star positions, tree order
+	double2* ccs;            // [R nb][np/nb + 1] output (file order)
+	double2* caustics;       // same layout
+	double* mu;              // same layout
+	unsigned long long* stats; // [BSTAT]
+	int N, R, np, nb, circ_exact, predictor, cap_init, cap_step;
+};
+
+// per-field counters
+enum BStat
+{
+	B_FULL = 0,
+	B_EARLY = 1,
+	B_CONV = 2,
+	B_NEAR = 3,
+	B_SWEEPS = 4,
+	B_ERRBITS = 5,
+	B_NAN = 6,
+	BSTAT = 8
+};
+
+struct BTask
+{
+	int field, branch, side, cost;
+};
+
+__device__ inline int centre_row(const BField& F, int c) { return F.np / (2 * F.nb) + c * (F.np / F.nb + 1); }
+
+// All sweeps of one row (at most cap): zc holds the row's roots, la/lb are
+// the ping-pong active lists. Returns after the first sweep that leaves no
+// root active.
+template <int BS, int RPT, bool MIXED>
+__device__ void solve_row(const BField& F, int row, int cap, double2* zc, const double2* st, int* la, int* lb, int* s_n,
+	unsigned long long (&cnt)[5])
+{
+	const LensView& L = F.L;
+	const int R = F.R;
+	const double2 e = F.cs[row];
+	const bool pole_fmm = MIXED && L.order > 0;
+	for (int q = threadIdx.x; q < R; q += BS) la[q] = q;
+	if (threadIdx.x == 0)
+	{
+		s_n[0] = R;
+		s_n[1] = 0;
+	}
+	__syncthreads();
+	int* cur = la;
+	int* nxt = lb;
+	for (int it = 0; it < cap; it++)
+	{
+		const int n = s_n[0];
+		if (n == 0) break;
+		cnt[4] += threadIdx.x == 0 ? 1 : 0;
+		// Witt function and 1/mu early exit for the listed roots
+		int idx[RPT];
+		bool full[RPT];
+		cd f0[RPT], f1[RPT], ex[RPT];
+#pragma unroll
+		for (int k = 0; k < RPT; k++)
+		{
+			const int q = threadIdx.x + k * BS;
+			idx[k] = -1;
+			full[k] = false;
+			if (q >= n) continue;
+			const int i = cur[q];
+			idx[k] = i;
+			const cd z(zc[i].x, zc[i].y);
+			const int node = nearest_node(L, z);
+			const Witt w = witt_from(L, z, e.x, e.y, node, p2p(L, node, z));
+			if (w.early)
+			{
+				cnt[1]++;
+				cnt[2]++;
+				if (MIXED)
+				{
+					const cd dz = w.f0 / witt_f1(L, z, w);
+					if (dz.abs() < 1e-6 * L.theta) zc[i] = make_double2(z.re - dz.re, z.im - dz.im);
+				}
+			}
+			else
+			{
+				full[k] = true;
+				f0[k] = w.f0;
+				f1[k] = witt_f1(L, z, w);
+				ex[k] = cd();
+				if (F.circ_exact) ex[k] = 2.0 * z.inv(); // ccf_functions.cuh:223-227
+				if (pole_fmm) ex[k] += 2.0 * (w.nr.s0 + far_pole32(L, node, z));
+				cnt[0]++;
+				cnt[3] += w.nr.count;
+			}
+		}
+		__syncthreads();
+		// Cauchy sums 2 sum 1/(z - s_i) - sum_{j != k} 1/(z - z_j) and the update
+		double zx[RPT], zy[RPT], ax[RPT], ay[RPT], bx[RPT], by[RPT];
+		bool any = false;
+#pragma unroll
+		for (int k = 0; k < RPT; k++)
+		{
+			any |= full[k];
+			const double2 v = full[k] ? zc[idx[k]] : make_double2(0, 0);
+			zx[k] = v.x;
+			zy[k] = v.y;
+			ax[k] = ay[k] = bx[k] = by[k] = 0;
+		}
+		if (any)
+		{
+			if (!pole_fmm)
+				for (int s = 0; s < F.N; s++)
+				{
+					const double2 w = st[s];
+#pragma unroll
+					for (int k = 0; k < RPT; k++)
+					{
+						const double dx = zx[k] - w.x, dy = zy[k] - w.y;
+						const double iv = rcp64(fma(dx, dx, dy * dy));
+						ax[k] = fma(dx, iv, ax[k]);
+						ay[k] = fma(dy, iv, ay[k]);
+					}
+				}
+			for (int j = 0; j < R; j++)
+			{
+				const double2 w = zc[j];
+#pragma unroll
+				for (int k = 0; k < RPT; k++)
+				{
+					// the own root gives dx = dy = 0 and, floored, a zero term
+					const double dx = zx[k] - w.x, dy = zy[k] - w.y;
+					const double iv = rcp64(fmax(fma(dx, dx, dy * dy), 1e-300));
+					bx[k] = fma(dx, iv, bx[k]);
+					by[k] = fma(dy, iv, by[k]);
+				}
+			}
+		}
+		cd zn[RPT];
+		bool moved[RPT];
+#pragma unroll
+		for (int k = 0; k < RPT; k++)
+		{
+			moved[k] = false;
+			if (!full[k]) continue;
+			const cd z(zx[k], zy[k]);
+			const cd sum = cd(2 * ax[k] - bx[k], -(2 * ay[k] - by[k])) + ex[k];
+			zn[k] = z - f0[k] / (f1[k] + f0[k] * sum);
+			moved[k] = !((zn[k] - z).abs() / L.theta < 1e-9); // a non-finite step stays active
+		}
+		__syncthreads();
+		// commit, and list the roots that moved by 1e-9 theta or more
+#pragma unroll
+		for (int k = 0; k < RPT; k++)
+		{
+			if (!full[k]) continue;
+			zc[idx[k]] = make_double2(zn[k].re, zn[k].im);
+			if (moved[k]) nxt[atomicAdd(&s_n[1], 1)] = idx[k];
+			else cnt[2]++;
+		}
+		__syncthreads();
+		if (threadIdx.x == 0)
+		{
+			s_n[0] = s_n[1];
+			s_n[1] = 0;
+		}
+		int* t = cur;
+		cur = nxt;
+		nxt = t;
+		__syncthreads();
+	}
+}
+
+// The persistent solve of one phase (0: branch centres, 1: phi chains).
+// Dynamic LDS: rmax roots, nlds stars (fields with more stars read them from
+// global memory), two lists of rmax indices.
+template <int BS, int RPT, bool MIXED>
+__global__ __launch_bounds__(BS) void k_batch(const BField* __restrict__ fields, const BTask* __restrict__ tasks, int ntasks,
+	int* __restrict__ next, int phase, int rmax, int nlds)
+{
+	extern __shared__ double2 smem[];
+	double2* zc = smem;
+	double2* sl = smem + rmax;
+	int* la = (int*)(sl + nlds);
+	int* lb = la + rmax;
+	__shared__ int s_task;
+	__shared__ int s_n[2];
+	__shared__ unsigned long long s_red[5][BS / 64];
+	for (;;)
+	{
+		__syncthreads();
+		if (threadIdx.x == 0) s_task = atomicAdd(next, 1);
+		__syncthreads();
+		const int t = __builtin_amdgcn_readfirstlane(s_task);
+		if (t >= ntasks) break;
+		const BTask tk = tasks[t];
+		const BField& F = fields[tk.field];
+		const int R = F.R;
+		const double2* st = F.stars;
+		if (F.N <= nlds)
+		{
+			for (int s = threadIdx.x; s < F.N; s += BS) sl[s] = F.stars[s];
+			st = sl;
+		}
+		unsigned long long cnt[5] = {0, 0, 0, 0, 0};
+		if (phase == 0)
+		{
+			const int row = centre_row(F, tk.branch);
+			for (int i = threadIdx.x; i < R; i += BS) zc[i] = F.seeds[i];
+			solve_row<BS, RPT, MIXED>(F, row, F.cap_init, zc, st, la, lb, s_n, cnt);
+			for (int i = threadIdx.x; i < R; i += BS) F.roots[(size_t)row * R + i] = zc[i];
+		}
+		else
+		{
+			const int c0 = centre_row(F, tk.branch);
+			const int sgn = tk.side ? 1 : -1;
+			const int steps = F.np / (2 * F.nb);
+			for (int j = 1; j <= steps; j++)
+			{
+				const int row = c0 + sgn * j;
+				// prepare_roots_kernel (ccf_functions.cuh:261-295); a predictor
+				// extrapolates from the rows this thread itself wrote
+				const int q = min(F.predictor, j - 1);
+				for (int i = threadIdx.x; i < R; i += BS)
+				{
+					const double2 z1 = F.roots[(size_t)(row - sgn) * R + i];
+					double2 z = z1;
+					if (q >= 1)
+					{
+						const double2 z2 = F.roots[(size_t)(row - 2 * sgn) * R + i];
+						if (q >= 2)
+						{
+							const double2 z3 = F.roots[(size_t)(row - 3 * sgn) * R + i];
+							z = make_double2(3 * (z1.x - z2.x) + z3.x, 3 * (z1.y - z2.y) + z3.y);
+						}
+						else
+						{
+							z = make_double2(2 * z1.x - z2.x, 2 * z1.y - z2.y);
+						}
+					}
+					zc[i] = z;
+				}
+				solve_row<BS, RPT, MIXED>(F, row, F.cap_step, zc, st, la, lb, s_n, cnt);
+				for (int i = threadIdx.x; i < R; i += BS) F.roots[(size_t)row * R + i] = zc[i];
+			}
+		}
+		// per-field counters, one atomic per counter and task
+#pragma unroll
+		for (int k = 0; k < 5; k++)
+		{
+			const unsigned long long s = wave_sum(cnt[k]);
+			if ((threadIdx.x & 63) == 0) s_red[k][threadIdx.x >> 6] = s;
+		}
+		__syncthreads();
+		if (threadIdx.x < 5)
+		{
+			unsigned long long s = 0;
+			for (int w = 0; w < BS / 64; w++) s += s_red[threadIdx.x][w];
+			const int which[5] = {B_FULL, B_EARLY, B_CONV, B_NEAR, B_SWEEPS};
+			if (s) atomicAdd(&F.stats[which[threadIdx.x]], s);
+		}
+	}
+}
+
+__global__ void k_zero_int(int* p, int n)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) p[i] = 0;
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) p[i] = 0;
+}
+
+// 1/mu error bound and non-finite flag over every point of every field
+// (find_errors_kernel + has_nan_err_kernel, ccf_functions.cuh:419-489)
+__global__ __launch_bounds__(BLOCK) void k_batch_errors(const BField* __restrict__ fields)
+{
+	const BField& F = fields[blockIdx.y];
+	const long long total = (long long)(F.np + F.nb) * F.R;
+	double e = 0;
+	unsigned long long bad = 0;
+	for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (long long)gridDim.x * blockDim.x)
+	{
+		const int row = (int)(idx / F.R);
+		const double2 t = F.cs[row];
+		const double2 v = F.roots[idx];
+		const cd z(v.x, v.y);
+		const int node = nearest_node(F.L, z);
+		const double b = err_bound(witt_from(F.L, z, t.x, t.y, node, p2p(F.L, node, z)));
+		if (!isfinite(b)) bad = 1;
+		else e = fmax(e, b);
+	}
+	for (int d = 32; d >= 1; d >>= 1) e = fmax(e, __shfl_xor(e, d, 64));
+	bad = wave_sum(bad);
+	if ((threadIdx.x & 63) == 0)
+	{
+		atomicMax(&F.stats[B_ERRBITS], (unsigned long long)__double_as_longlong(e));
+		if (bad) atomicOr(&F.stats[B_NAN], 1ull);
+	}
+}
+
+// [rows][R] -> [R][rows] per field (transpose_array_kernel), 32 x 32 tiles
+__global__ __launch_bounds__(BLOCK) void k_batch_transpose(const BField* __restrict__ fields)
+{
+	const BField& F = fields[blockIdx.z];
+	const int rows = F.np + F.nb, R = F.R;
+	__shared__ double2 t[32][33];
+	const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+	for (int a0 = blockIdx.x * 32; a0 < R; a0 += gridDim.x * 32)
+		for (int r0 = blockIdx.y * 32; r0 < rows; r0 += gridDim.y * 32)
+		{
+			for (int y = ty; y < 32; y += 8)
+			{
+				const int r = r0 + y, a = a0 + tx;
+				if (r < rows && a < R) t[y][tx] = F.roots[(size_t)r * R + a];
+			}
+			__syncthreads();
+			for (int y = ty; y < 32; y += 8)
+			{
+				const int a = a0 + y, r = r0 + tx;
+				if (r < rows && a < R) F.ccs[(size_t)a * rows + r] = t[tx][y];
+			}
+			__syncthreads();
+		}
+}
+
+// lens map and length scale at every point of every field
+// (find_caustics_kernel, find_mu_length_scales_kernel)
+__global__ __launch_bounds__(BLOCK) void k_batch_caustics(const BField* __restrict__ fields)
+{
+	const BField& F = fields[blockIdx.y];
+	const long long total = (long long)(F.np + F.nb) * F.R;
+	for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long long)gridDim.x * blockDim.x)
+	{
+		cd w;
+		double m;
+		lens_map(F.L, cd(F.ccs[i].x, F.ccs[i].y), w, m);
+		if (F.caustics) F.caustics[i] = make_double2(w.re, w.im);
+		if (F.mu) F.mu[i] = m;
+	}
+}
+
+using BatchKernel = void (*)(const BField*, const BTask*, int, int*, int, int, int);
+
+template <int BS, bool MIXED>
+BatchKernel pick_rpt(int rpt)
+{
+	switch (rpt)
+	{
+	case 1: return k_batch<BS, 1, MIXED>;
+	case 2: return k_batch<BS, 2, MIXED>;
+	case 4: return k_batch<BS, 4, MIXED>;
+	default: return k_batch<BS, 8, MIXED>;
+	}
+}
+
+template <bool MIXED>
+BatchKernel pick_kernel(int bs, int rpt)
+{
+	switch (bs)
+	{
+	case 256: return pick_rpt<256, MIXED>(rpt);
+	case 512: return pick_rpt<512, MIXED>(rpt);
+	default: return pick_rpt<1024, MIXED>(rpt);
+	}
+}
+
+int rpt_for(int rmax, int bs)
+{
+	const int need = (rmax + bs - 1) / bs;
+	return need <= 1 ? 1 : need <= 2 ? 2 : need <= 4 ? 4 : 8;
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+
+struct BatchDevice
+{
+	BField* fields = nullptr;
+	BTask* tasks0 = nullptr;
+	BTask* tasks1 = nullptr;
+	int* next = nullptr;
+	unsigned long long* stats = nullptr;
+	int n0 = 0, n1 = 0, nf = 0;
+	int rmax = 0, nlds = 0, maxrows = 0;
+	long long maxpts = 0;
+};
+
+namespace
+{
+
+void free_bdev(BatchDevice& d)
+{
+	auto F = [](auto*& p) { if (p) (void)hipFree(p); p = nullptr; };
+	F(d.fields); F(d.tasks0); F(d.tasks1); F(d.next); F(d.stats);
+	d = BatchDevice{};
+}
+
+// LDS bytes of a workgroup: roots, staged stars, two index lists
+size_t batch_lds(int rmax, int nlds) { return (size_t)(rmax + nlds) * sizeof(double2) + 2 * (size_t)rmax * sizeof(int); }
+
+// persistent grid: as many workgroups as fit the device at once, at most
+// one per task (queried outside any graph capture)
+int batch_grid(ccf_batch* b, int phase, int bs)
+{
+	ccf_ctx* m = b->master;
+	BatchDevice& d = *b->dev;
+	const bool mixed = m->P.precision == 1;
+	const BatchKernel fn = mixed ? pick_kernel<true>(bs, rpt_for(d.rmax, bs)) : pick_kernel<false>(bs, rpt_for(d.rmax, bs));
+	int per_cu = 0;
+	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)fn, bs, batch_lds(d.rmax, d.nlds)) != hipSuccess || per_cu < 1)
+		per_cu = 1;
+	const int cus = m->num_cus > 0 ? m->num_cus : 256;
+	return std::max(1, std::min(phase == 0 ? d.n0 : d.n1, cus * per_cu));
+}
+
+bool launch_batch_phase(ccf_batch* b, int phase, int bs, int g, hipStream_t s)
+{
+	ccf_ctx* m = b->master;
+	BatchDevice& d = *b->dev;
+	const bool mixed = m->P.precision == 1;
+	const int rpt = rpt_for(d.rmax, bs);
+	const BatchKernel fn = mixed ? pick_kernel<true>(bs, rpt) : pick_kernel<false>(bs, rpt);
+	const size_t lds = batch_lds(d.rmax, d.nlds);
+	const int ntasks = phase == 0 ? d.n0 : d.n1;
+	k_zero_int<<<1, 64, 0, s>>>(d.next, 1);
+	hipLaunchKernelGGL(fn, dim3(g), dim3(bs), lds, s, (const BField*)d.fields, (const BTask*)(phase == 0 ? d.tasks0 : d.tasks1),
+		ntasks, d.next, phase, d.rmax, d.nlds);
+	return CCF_LAUNCH(m, "batched Aberth solve");
+}
+
+bool launch_post(ccf_batch* b, hipStream_t s)
+{
+	ccf_ctx* m = b->master;
+	BatchDevice& d = *b->dev;
+	const int blocks = std::max(1, std::min(grid(d.maxpts), 1024));
+	k_batch_errors<<<dim3(blocks, d.nf), BLOCK, 0, s>>>(d.fields);
+	k_batch_transpose<<<dim3(std::min(grid(d.rmax, 32), 64), std::min(grid(d.maxrows, 32), 64), d.nf), BLOCK, 0, s>>>(d.fields);
+	k_batch_caustics<<<dim3(blocks, d.nf), BLOCK, 0, s>>>(d.fields);
+	return CCF_LAUNCH(m, "batched errors, transpose, caustics");
+}
+
+// the whole solve of a loaded and built batch, on stream s
+bool enqueue_solve(ccf_batch* b, hipStream_t s)
+{
+	BatchDevice& d = *b->dev;
+	k_zero_u64<<<grid((long long)d.nf * BSTAT), BLOCK, 0, s>>>(d.stats, d.nf * BSTAT);
+	return launch_batch_phase(b, 0, b->block, b->grid0, s) && launch_batch_phase(b, 1, b->block, b->grid1, s) && launch_post(b, s);
+}
+
+} // namespace
+
+void free_batch_device(ccf_batch* b)
+{
+	if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+	if (b->graph) (void)hipGraphDestroy(b->graph);
+	b->graph_exec = nullptr;
+	b->graph = nullptr;
+	if (b->dev)
+	{
+		free_bdev(*b->dev);
+		delete b->dev;
+		b->dev = nullptr;
+	}
+}
+
+int batch_lds_limit(ccf_ctx* m);
+
+// forest of all fields' trees, then the device tables of the batch
+bool batch_build(ccf_batch* b, int verbose)
+{
+	ccf_ctx* m = b->master;
+	hipStream_t s = m->stream;
+	const int nf = (int)b->fields.size();
+	free_batch_device(b);
+	b->dev = new BatchDevice();
+	BatchDevice& d = *b->dev;
+	d.nf = nf;
+	// the fields' stars back to back
+	ForestSpec spec;
+	int N = 0;
+	for (ccf_ctx* f : b->fields)
+	{
+		spec.soff.push_back(N);
+		spec.nstars.push_back(f->num_stars);
+		f->root_half_length = (f->rectangular ? std::max(f->corner.re, f->corner.im) : f->corner.abs()) * 1.1;
+		spec.root_hl.push_back(f->root_half_length);
+		spec.theta.push_back(f->theta);
+		spec.alpha_error.push_back(f->alpha_error);
+		N += f->num_stars;
+	}
+	m->free_device();
+	m->num_stars = N;
+	if (!CCF_HIP(m, hipMalloc(&m->spos, (size_t)N * sizeof(double2))) || !CCF_HIP(m, hipMalloc(&m->smass, (size_t)N * sizeof(double))) ||
+		!CCF_HIP(m, hipMalloc(&m->spos_tmp, (size_t)N * sizeof(double2))) || !CCF_HIP(m, hipMalloc(&m->smass_tmp, (size_t)N * sizeof(double))) ||
+		!CCF_HIP(m, hipMalloc(&m->spos32, (size_t)N * sizeof(float2))))
+		return false;
+	for (int f = 0; f < nf; f++)
+	{
+		ccf_ctx* c = b->fields[f];
+		if (!CCF_HIP(m, hipMemcpyAsync(m->spos + spec.soff[f], c->spos, (size_t)c->num_stars * sizeof(double2), hipMemcpyDeviceToDevice, s)) ||
+			!CCF_HIP(m, hipMemcpyAsync(m->smass + spec.soff[f], c->smass, (size_t)c->num_stars * sizeof(double), hipMemcpyDeviceToDevice, s)))
+			return false;
+	}
+	if (!build_forest(m, spec, verbose)) return false;
+	// tree-order stars back into each field (ccf_stars.bin is written in tree order)
+	for (int f = 0; f < nf; f++)
+	{
+		ccf_ctx* c = b->fields[f];
+		if (!CCF_HIP(m, hipMemcpyAsync(c->spos, m->spos + spec.soff[f], (size_t)c->num_stars * sizeof(double2), hipMemcpyDeviceToDevice, s)) ||
+			!CCF_HIP(m, hipMemcpyAsync(c->smass, m->smass + spec.soff[f], (size_t)c->num_stars * sizeof(double), hipMemcpyDeviceToDevice, s)))
+			return false;
+		c->expansion_order = spec.order;
+		c->info.tree_levels = spec.levels[f];
+		c->info.expansion_order = m->tree.order;
+		c->info.num_nodes = m->tree.nnodes;
+		c->info.t_tree = m->info.t_tree;
+		c->prepared = true;
+	}
+	// device tables
+	std::vector<BField> hf(nf);
+	std::vector<BTask> t0, t1;
+	for (int f = 0; f < nf; f++)
+	{
+		ccf_ctx* c = b->fields[f];
+		LensView L = m->lens();
+		const LensView Lc = c->lens();
+		L.kappa = Lc.kappa;
+		L.gamma = Lc.gamma;
+		L.theta = Lc.theta;
+		L.theta2 = Lc.theta2;
+		L.kappastar = Lc.kappastar;
+		L.rectangular = Lc.rectangular;
+		L.approx = Lc.approx;
+		L.taylor_smooth = Lc.taylor_smooth;
+		L.corner_re = Lc.corner_re;
+		L.corner_im = Lc.corner_im;
+		L.corner_abs = Lc.corner_abs;
+		L.corner_abs2 = Lc.corner_abs2;
+		L.corner_arg = Lc.corner_arg;
+		L.nstars = c->num_stars;
+		L.nts = Lc.nts;
+		L.ts_a = Lc.ts_a;
+		L.ts_b = Lc.ts_b;
+		L.root = f;
+		L.lookup = nullptr;
+		const int R = c->num_roots, np = c->P.num_phi, nb = c->P.num_branches;
+		const size_t pts = (size_t)(np + nb) * R;
+		if (!c->ccs && !CCF_HIP(m, hipMalloc(&c->ccs, pts * sizeof(double2)))) return false;
+		if (c->P.write_caustics && !c->caustics && !CCF_HIP(m, hipMalloc(&c->caustics, pts * sizeof(double2)))) return false;
+		if (c->P.write_mu_length_scales && !c->mu_scales && !CCF_HIP(m, hipMalloc(&c->mu_scales, pts * sizeof(double)))) return false;
+		BField& F = hf[f];
+		F.L = L;
+		F.roots = c->ccs_init;
+		F.seeds = c->dseeds;
+		F.cs = c->phase;
+		F.stars = m->spos + spec.soff[f];
+		F.ccs = c->ccs;
+		F.caustics = c->P.write_caustics ? c->caustics : nullptr;
+		F.mu = c->P.write_mu_length_scales ? c->mu_scales : nullptr;
+		F.N = c->num_stars;
+		F.R = R;
+		F.np = np;
+		F.nb = nb;
+		F.circ_exact = (!c->rectangular && !c->P.approx) ? 1 : 0;
+		F.predictor = c->P.approx ? c->P.predictor : 0;
+		F.cap_init = c->P.init_iters > 0 ? c->P.init_iters : 60;
+		F.cap_step = c->P.step_iters > 0 ? c->P.step_iters : 20;
+		d.rmax = std::max(d.rmax, R);
+		d.maxrows = std::max(d.maxrows, np + nb);
+		d.maxpts = std::max(d.maxpts, (long long)pts);
+		const int cost = (int)std::min<long long>((long long)R * (R + c->num_stars) / 64, 1 << 30);
+		for (int br = 0; br < nb; br++)
+		{
+			t0.push_back(BTask{f, br, 0, cost});
+			t1.push_back(BTask{f, br, 0, cost});
+			t1.push_back(BTask{f, br, 1, cost});
+		}
+	}
+	// LDS of a workgroup: the row, its two active lists, then as many stars as fit
+	const long long lim = batch_lds_limit(m);
+	if ((long long)batch_lds(d.rmax, 0) > lim || d.rmax > MAX_BATCH_RPT * 1024)
+	{
+		m->fail("Error. A batched field has more roots than fit one workgroup (" + std::to_string(d.rmax) + ")\n");
+		return false;
+	}
+	// largest rows first (longest-processing-time order of the queue)
+	auto by_cost = [](const BTask& a, const BTask& b2) { return a.cost > b2.cost; };
+	std::stable_sort(t0.begin(), t0.end(), by_cost);
+	std::stable_sort(t1.begin(), t1.end(), by_cost);
+	d.n0 = (int)t0.size();
+	d.n1 = (int)t1.size();
+	const long long room = lim - (long long)batch_lds(d.rmax, 0);
+	int nmax = 0;
+	for (ccf_ctx* c : b->fields) nmax = std::max(nmax, c->num_stars);
+	d.nlds = (int)std::max(0LL, std::min<long long>(nmax, room / (long long)sizeof(double2)));
+	if (!CCF_HIP(m, hipMalloc(&d.stats, sizeof(unsigned long long) * nf * BSTAT))) return false;
+	for (int f = 0; f < nf; f++) hf[f].stats = d.stats + (size_t)f * BSTAT;
+	return CCF_HIP(m, hipMalloc(&d.fields, sizeof(BField) * nf)) && CCF_HIP(m, hipMalloc(&d.tasks0, sizeof(BTask) * d.n0)) &&
+		CCF_HIP(m, hipMalloc(&d.tasks1, sizeof(BTask) * d.n1)) && CCF_HIP(m, hipMalloc(&d.next, sizeof(int))) &&
+		CCF_HIP(m, hipMemcpyAsync(d.fields, hf.data(), sizeof(BField) * nf, hipMemcpyHostToDevice, s)) &&
+		CCF_HIP(m, hipMemcpyAsync(d.tasks0, t0.data(), sizeof(BTask) * d.n0, hipMemcpyHostToDevice, s)) &&
+		CCF_HIP(m, hipMemcpyAsync(d.tasks1, t1.data(), sizeof(BTask) * d.n1, hipMemcpyHostToDevice, s)) &&
+		CCF_HIP(m, hipStreamSynchronize(s));
+}
+
+// workgroup size by timing the phase-0 launch of this batch (idempotent: it
+// recomputes the centre rows from the seeds)
+bool batch_autotune(ccf_batch* b, int verbose)
+{
+	ccf_ctx* m = b->master;
+	hipStream_t s = m->stream;
+	const int cands[3] = {256, 512, 1024};
+	double best = 1e300;
+	int pick = 0;
+	hipEvent_t e0, e1;
+	if (!CCF_HIP(m, hipEventCreate(&e0)) || !CCF_HIP(m, hipEventCreate(&e1))) return false;
+	for (int bs : cands)
+	{
+		if (rpt_for(b->dev->rmax, bs) > MAX_BATCH_RPT) continue;
+		float ms = 0;
+		const int g = batch_grid(b, 0, bs);
+		if (!launch_batch_phase(b, 0, bs, g, s) || !CCF_HIP(m, hipEventRecord(e0, s)) || !launch_batch_phase(b, 0, bs, g, s) ||
+			!CCF_HIP(m, hipEventRecord(e1, s)) || !CCF_HIP(m, hipEventSynchronize(e1)) || !CCF_HIP(m, hipEventElapsedTime(&ms, e0, e1)))
+			return false;
+		if (verbose >= 2) std::cout << "batch autotune: workgroup " << bs << " roots/thread " << rpt_for(b->dev->rmax, bs) << ": " << ms << " ms\n";
+		b->tuned_ms.push_back(ms);
+		if (ms < best)
+		{
+			best = ms;
+			pick = bs;
+		}
+	}
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
+	if (!pick)
+	{
+		m->fail("Error. no workgroup size fits the batch\n");
+		return false;
+	}
+	b->block = pick;
+	return true;
+}
+
+int batch_lds_limit(ccf_ctx* m)
+{
+	int v = 0;
+	if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, 0) != hipSuccess || v <= 0) v = 65536;
+	return std::min(v, 65536);
+}
+
+// solve, errors, transpose and caustics of every field: captured once into a
+// HIP graph per build, replayed afterwards
+bool batch_solve(ccf_batch* b, int verbose)
+{
+	ccf_ctx* m = b->master;
+	hipStream_t s = m->stream;
+	BatchDevice& d = *b->dev;
+	if (b->block == 0 && !batch_autotune(b, verbose)) return false;
+	b->grid0 = batch_grid(b, 0, b->block);
+	b->grid1 = batch_grid(b, 1, b->block);
+	auto t0 = std::chrono::steady_clock::now();
+	if (!b->graph_exec)
+	{
+		if (!CCF_HIP(m, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal))) return false;
+		const bool ok = enqueue_solve(b, s);
+		hipGraph_t g = nullptr;
+		const hipError_t e = hipStreamEndCapture(s, &g);
+		if (!ok || !CCF_HIP(m, e)) return false;
+		b->graph = g;
+		if (!CCF_HIP(m, hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0))) return false;
+	}
+	if (!CCF_HIP(m, hipGraphLaunch(b->graph_exec, s)) || !CCF_HIP(m, hipStreamSynchronize(s))) return false;
+	auto t1 = std::chrono::steady_clock::now();
+	const double dt = std::chrono::duration<double>(t1 - t0).count();
+	std::vector<unsigned long long> st((size_t)d.nf * BSTAT);
+	if (!CCF_HIP(m, hipMemcpy(st.data(), d.stats, st.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost))) return false;
+	bool nan = false;
+	b->info = ccf_info{};
+	for (int f = 0; f < d.nf; f++)
+	{
+		ccf_ctx* c = b->fields[f];
+		const unsigned long long* x = st.data() + (size_t)f * BSTAT;
+		const long long pts = (long long)(c->P.num_phi + c->P.num_branches) * c->num_roots;
+		double e;
+		std::memcpy(&e, &x[B_ERRBITS], sizeof(e));
+		c->max_error = e;
+		c->info.max_error = e;
+		c->info.points_total = pts;
+		c->info.points_converged = (long long)x[B_CONV];
+		c->info.updates_full = (long long)x[B_FULL];
+		c->info.updates_early = (long long)x[B_EARLY];
+		c->info.sweeps = (long long)x[B_SWEEPS];
+		c->info.t_init_roots = 0;
+		c->info.t_ccs = dt;
+		c->have_roots = c->have_ccs = true;
+		c->have_caustics = c->P.write_caustics != 0;
+		c->have_mu = c->P.write_mu_length_scales != 0;
+		if (x[B_NAN]) nan = true;
+		b->info.points_total += pts;
+		b->info.points_converged += (long long)x[B_CONV];
+		b->info.updates_full += (long long)x[B_FULL];
+		b->info.updates_early += (long long)x[B_EARLY];
+		b->info.sweeps += (long long)x[B_SWEEPS];
+		b->info.max_error = std::max(b->info.max_error, e);
+		const double Np = c->num_stars + ((!c->rectangular && !c->P.approx) ? 1 : 0);
+		b->info.flops_sums += (double)x[B_FULL] * 10.0 * (Np + c->num_roots - 1);
+	}
+	b->info.t_ccs = dt;
+	b->info.t_tree = m->info.t_tree;
+	b->info.num_stars = m->num_stars;
+	if (nan)
+	{
+		m->fail("Error. Errors in 1/mu contain values which are not positive real numbers.\n");
+		return false;
+	}
+	return true;
+}
+
+} // namespace ccf

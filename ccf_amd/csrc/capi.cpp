@@ -58,6 +58,7 @@ ccf::LensView ccf_ctx::lens() const
 	L.loc = tree.loc;
 	L.loc32 = tree.loc32;
 	L.locu32 = tree.locu32;
+	L.root = 0;
 	L.lookup = tree.lookup;
 	L.lookup_n = 1 << tree.lookup_level;
 	L.lookup_x0 = tree.lookup_x0;
@@ -87,10 +88,10 @@ void ccf_ctx::free_device()
 	ccf::DeviceTree& T = tree;
 	F(T.center); F(T.hl); F(T.parent); F(T.child); F(T.sstart); F(T.scount);
 	F(T.nbr); F(T.nnbr); F(T.sl); F(T.nsl); F(T.dl); F(T.ndl); F(T.split); F(T.rank);
-	F(T.nf); F(T.nnf); F(T.nbox); F(T.nf_off); F(T.nf_cnt); F(T.nfs); F(T.mp); F(T.loc); F(T.loc32); F(T.mpu); F(T.locu); F(T.locu32); F(T.binom); F(T.lookup);
+	F(T.field); F(T.nf); F(T.nnf); F(T.nbox); F(T.nf_off); F(T.nf_cnt); F(T.nfs); F(T.mp); F(T.loc); F(T.loc32); F(T.mpu); F(T.locu); F(T.locu32); F(T.binom); F(T.lookup);
 	T = ccf::DeviceTree{};
 	F(spos); F(smass); F(spos_tmp); F(smass_tmp); F(spos32); F(ts_a); F(ts_b); F(ones);
-	F(ccs_init); F(ccs); F(caustics); F(mu_scales); F(phase);
+	F(ccs_init); F(ccs); F(caustics); F(mu_scales); F(phase); F(dseeds);
 	ccf::free_sweep(sw);
 	F(counters);
 	if (host_counts) (void)hipHostFree(host_counts);
@@ -564,7 +565,7 @@ void ccf_destroy(ccf_ctx* c)
 {
 	if (!c) return;
 	c->free_device();
-	if (c->stream) hipStreamDestroy(c->stream);
+	if (c->stream && c->own_stream) hipStreamDestroy(c->stream);
 	delete c;
 }
 
@@ -914,6 +915,146 @@ double ccf_mass_function(const char* name, int which, double p, double m_lower, 
 	if (which == 0) return ccf::imf_mass(f, p);
 	return ccf::imf_moment(f, which);
 }
+
+// ---------------------------------------------------------------------------
+// batched fields
+
+ccf_batch* ccf_batch_create(void)
+{
+	ccf_batch* b = new (std::nothrow) ccf_batch();
+	if (b && !(b->master = ccf_create()))
+	{
+		delete b;
+		return nullptr;
+	}
+	return b;
+}
+
+void ccf_batch_destroy(ccf_batch* b)
+{
+	if (!b) return;
+	ccf::free_batch_device(b);
+	for (ccf_ctx* c : b->fields) ccf_destroy(c);
+	ccf_destroy(b->master);
+	delete b;
+}
+
+int ccf_batch_add(ccf_batch* b, const ccf_params* p)
+{
+	if (!b || !p) return -1;
+	ccf_ctx* c = ccf_create();
+	if (!c) return -1;
+	c->P_in = *p;
+	c->P = *p;
+	b->fields.push_back(c);
+	b->loaded = b->built = b->solved = false;
+	return (int)b->fields.size() - 1;
+}
+
+int ccf_batch_size(const ccf_batch* b) { return b ? (int)b->fields.size() : 0; }
+
+int ccf_batch_load(ccf_batch* b, int verbose)
+{
+	if (!b) return 1;
+	ccf_ctx* m = b->master;
+	m->last_error.clear();
+	b->loaded = b->built = b->solved = false;
+	ccf::free_batch_device(b);
+	if (b->fields.empty())
+	{
+		m->fail("Error. The batch has no fields.\n");
+		return 1;
+	}
+	const ccf_params& p0 = b->fields[0]->P_in;
+	for (ccf_ctx* c : b->fields)
+	{
+		const ccf_params& p = c->P_in;
+		if (p.fmm_order != p0.fmm_order || p.direct != p0.direct || p.binomial_exact != p0.binomial_exact || p.precision != p0.precision)
+		{
+			m->fail("Error. The fields of a batch must share fmm_order, direct, binomial_exact and precision.\n");
+			return 1;
+		}
+	}
+	m->P_in = p0;
+	m->P = p0;
+	if (!device_init(m)) return 1;
+	for (size_t i = 0; i < b->fields.size(); i++)
+	{
+		ccf_ctx* c = b->fields[i];
+		c->batch_member = true;
+		if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
+		c->stream = m->stream;
+		c->own_stream = false;
+		if (ccf_load(c, verbose))
+		{
+			m->last_error = "Error. Field " + std::to_string(i) + ": " + c->last_error;
+			return 1;
+		}
+	}
+	b->loaded = true;
+	return 0;
+}
+
+int ccf_batch_build(ccf_batch* b, int verbose)
+{
+	if (!b) return 1;
+	if (!b->loaded)
+	{
+		b->master->fail("Error. ccf_batch_load must run first.\n");
+		return 1;
+	}
+	b->built = b->solved = false;
+	if (!ccf::batch_build(b, verbose)) return 1;
+	b->built = true;
+	return 0;
+}
+
+int ccf_batch_solve(ccf_batch* b, int verbose)
+{
+	if (!b) return 1;
+	if (!b->built)
+	{
+		b->master->fail("Error. ccf_batch_build must run first.\n");
+		return 1;
+	}
+	if (!ccf::batch_solve(b, verbose)) return 1;
+	b->solved = true;
+	return 0;
+}
+
+int ccf_batch_run(ccf_batch* b, int verbose)
+{
+	if (ccf_batch_load(b, verbose) || ccf_batch_build(b, verbose)) return 1;
+	return ccf_batch_solve(b, verbose);
+}
+
+int ccf_batch_save(ccf_batch* b, int field, int verbose)
+{
+	if (!b || field < 0 || field >= (int)b->fields.size()) return 1;
+	if (!b->solved)
+	{
+		b->master->fail("Error. ccf_batch_solve must run first.\n");
+		return 1;
+	}
+	return ccf_save(b->fields[field], verbose);
+}
+
+int ccf_batch_get_info(const ccf_batch* b, int field, ccf_info* info)
+{
+	if (!b || !info || field < -1 || field >= (int)b->fields.size()) return 1;
+	*info = field < 0 ? b->info : b->fields[field]->info;
+	return 0;
+}
+
+int ccf_batch_get_array(ccf_batch* b, int field, int which, void* dst, long long capacity, int* nrows, int* ncols)
+{
+	if (!b || field < 0 || field >= (int)b->fields.size()) return 1;
+	return ccf_get_array(b->fields[field], which, dst, capacity, nrows, ncols);
+}
+
+int ccf_batch_workgroup(const ccf_batch* b) { return b ? b->block : 0; }
+
+const char* ccf_batch_last_error(const ccf_batch* b) { return b && b->master ? b->master->last_error.c_str() : ""; }
 
 int ccf_write_synthetic_stars(const ccf_params* p, unsigned long long seed, const char* path)
 {

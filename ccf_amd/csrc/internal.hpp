@@ -48,6 +48,7 @@ struct DeviceTree
 	int* nsl = nullptr;
 	int* dl = nullptr;     // coarser-level interaction list [capacity][5]
 	int* ndl = nullptr;
+	int* field = nullptr;  // field of each node (a forest of several fields' trees)
 	int* split = nullptr;  // per-level scratch: split flags / ranks
 	int* rank = nullptr;
 	int2* nf = nullptr;    // near-field star ranges: self then neighbours [capacity][9]
@@ -95,6 +96,7 @@ struct LensView
 	const double2* loc;
 	const float2* loc32;
 	const float2* locu32;
+	int root;                // root node of the field (several fields share one forest)
 	const int* lookup;       // point-location table (null: descend from the root)
 	int lookup_n;            // its cells per side
 	double lookup_x0, lookup_inv; // lower-left corner of the root box, cells per unit length
@@ -205,11 +207,34 @@ struct ccf_ctx
 	std::vector<ccf::Timer> timers;
 	int* host_counts = nullptr;   // pinned: active roots per row, read back between sweeps
 	double2* phase = nullptr;     // (cos phi, sin phi) of every phi row
+	bool batch_member = false;    // a field of a ccf_batch: no sweep buffers, seeds on the device
+	bool own_stream = true;       // ccf_destroy destroys the stream
+	double2* dseeds = nullptr;    // [R] initial roots (batch members)
 
 	ccf::LensView lens() const;
 	void fail(const std::string& msg);
 	bool hip_ok(hipError_t e, const char* what, const char* file, int line);
 	void free_device();
+};
+
+namespace ccf
+{
+struct BatchDevice;
+}
+
+// several fields solved together (batch.hip)
+struct ccf_batch
+{
+	std::vector<ccf_ctx*> fields;
+	ccf_ctx* master = nullptr;      // stream, forest, concatenated stars, errors
+	ccf::BatchDevice* dev = nullptr;
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t graph_exec = nullptr;
+	int block = 0;                  // autotuned workgroup size
+	int grid0 = 0, grid1 = 0;       // persistent grids of the two phases
+	std::vector<float> tuned_ms;
+	ccf_info info{};
+	bool loaded = false, built = false, solved = false;
 };
 
 #define CCF_HIP(ctx, call) ((ctx)->hip_ok((call), #call, __FILE__, __LINE__))
@@ -219,6 +244,15 @@ namespace ccf
 {
 // tree.hip
 bool build_tree(ccf_ctx* ctx, int verbose);
+// trees of several fields at once (ctx->spos holds their stars back to back)
+struct ForestSpec
+{
+	std::vector<int> soff, nstars;
+	std::vector<double> root_hl, theta, alpha_error;
+	std::vector<int> levels; // out: depth of each field's tree
+	int order = 0;           // out: the expansion order used
+};
+bool build_forest(ccf_ctx* ctx, ForestSpec& spec, int verbose);
 // solve.hip
 bool alloc_sweep(ccf_ctx* ctx, SweepBufs& b, int rows, int R);
 void free_sweep(SweepBufs& b);
@@ -230,4 +264,8 @@ bool generate_stars_dev(ccf_ctx* ctx);
 bool eval_points_dev(ccf_ctx* ctx, const double* z, int n, double phi, double* out);
 bool aberth_sweep_dev(ccf_ctx* ctx, const double* roots, int n, double phi, double* out);
 bool root_sums_dev(ccf_ctx* ctx, const double* roots, int n, double* out);
+// batch.hip
+bool batch_build(ccf_batch* b, int verbose);
+bool batch_solve(ccf_batch* b, int verbose);
+void free_batch_device(ccf_batch* b);
 } // namespace ccf

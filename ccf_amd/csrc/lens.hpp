@@ -49,7 +49,7 @@ __device__ inline int nearest_node(const LensView& L, cd z)
 			if (fabs(z.re - b.x) < m && fabs(z.im - b.y) < m) return descend(L, z, n, b);
 		}
 	}
-	return descend(L, z, 0, L.nbox[0]);
+	return descend(L, z, L.root, L.nbox[L.root]);
 }
 
 // near-field sums over a node and its neighbours:

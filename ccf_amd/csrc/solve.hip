@@ -1141,6 +1141,13 @@ bool alloc_solver(ccf_ctx* ctx)
 	const int R = ctx->num_roots, np = ctx->P.num_phi, nb = ctx->P.num_branches;
 	const size_t pts = (size_t)(np + nb) * R;
 	ctx->max_rows = 2 * nb;
+	if (ctx->batch_member)
+		return CCF_HIP(ctx, hipMalloc(&ctx->ccs_init, pts * sizeof(double2))) &&
+			CCF_HIP(ctx, hipMalloc(&ctx->phase, (size_t)(np + nb) * sizeof(double2))) && launch_phase(ctx) &&
+			CCF_HIP(ctx, hipMalloc(&ctx->dseeds, (size_t)R * sizeof(double2))) &&
+			CCF_HIP(ctx, hipMemcpyAsync(ctx->dseeds, ctx->seeds.data(), (size_t)R * sizeof(double2), hipMemcpyHostToDevice, ctx->stream)) &&
+			CCF_HIP(ctx, hipMemsetAsync(ctx->ccs_init, 0, pts * sizeof(double2), ctx->stream)) &&
+			CCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
 	return CCF_HIP(ctx, hipMalloc(&ctx->ccs_init, pts * sizeof(double2))) && alloc_sweep(ctx, ctx->sw, ctx->max_rows, R) &&
 		CCF_HIP(ctx, hipHostMalloc((void**)&ctx->host_counts, ctx->max_rows * sizeof(int), hipHostMallocDefault)) &&
 		CCF_HIP(ctx, hipMalloc(&ctx->phase, (size_t)(np + nb) * sizeof(double2))) && launch_phase(ctx) &&

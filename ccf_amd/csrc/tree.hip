@@ -87,7 +87,7 @@ __global__ void k_scan(const int* __restrict__ flags, int n, int* __restrict__ r
 }
 
 __global__ void k_make_children(int off, int cnt, int new_off, const int* __restrict__ split, const int* __restrict__ rank,
-	double2* center, double* hl, int* parent, int* child, int* sstart, int* scount, int* nnbr, int* nsl, int* ndl)
+	double2* center, double* hl, int* parent, int* child, int* sstart, int* scount, int* nnbr, int* nsl, int* ndl, int* field)
 {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= cnt || !split[i]) return;
@@ -109,6 +109,7 @@ __global__ void k_make_children(int off, int cnt, int new_off, const int* __rest
 		nnbr[k] = 0;
 		nsl[k] = 0;
 		ndl[k] = 0;
+		field[k] = field[n];
 	}
 }
 
@@ -499,7 +500,7 @@ bool ensure_capacity(ccf_ctx* ctx, int need)
 		grow(ctx, T.child, n, cap, 1) && grow(ctx, T.sstart, n, cap, 1) && grow(ctx, T.scount, n, cap, 1) &&
 		grow(ctx, T.nbr, n, cap, MAX_NEIGHBORS) && grow(ctx, T.nnbr, n, cap, 1) &&
 		grow(ctx, T.sl, n, cap, MAX_SAME_LEVEL) && grow(ctx, T.nsl, n, cap, 1) &&
-		grow(ctx, T.dl, n, cap, MAX_DIFF_LEVEL) && grow(ctx, T.ndl, n, cap, 1) &&
+		grow(ctx, T.dl, n, cap, MAX_DIFF_LEVEL) && grow(ctx, T.ndl, n, cap, 1) && grow(ctx, T.field, n, cap, 1) &&
 		// split flags and ranks of the level being split are still live
 		grow(ctx, T.split, T.capacity, cap, 1) && grow(ctx, T.rank, T.capacity, cap, 1);
 	if (ok) T.capacity = cap;
@@ -530,42 +531,67 @@ std::vector<double> binomial_table(int n, bool exact)
 
 bool build_tree(ccf_ctx* ctx, int verbose)
 {
-	DeviceTree& T = ctx->tree;
-	const ccf_params& P = ctx->P;
-	const int N = ctx->num_stars;
-	hipStream_t s = ctx->stream;
-	auto t0 = std::chrono::steady_clock::now();
-
 	ctx->root_half_length = (ctx->rectangular ? std::max(ctx->corner.re, ctx->corner.im) : ctx->corner.abs()) * 1.1;
 	if (verbose >= 2) std::cout << "root_half_length set to: " << ctx->root_half_length << "\n\n";
+	ForestSpec f;
+	f.soff = {0};
+	f.nstars = {ctx->num_stars};
+	f.root_hl = {ctx->root_half_length};
+	f.theta = {ctx->theta};
+	f.alpha_error = {ctx->alpha_error};
+	if (!build_forest(ctx, f, verbose)) return false;
+	ctx->expansion_order = f.order;
+	ctx->info.tree_levels = ctx->tree.levels;
+	ctx->info.num_nodes = ctx->tree.nnodes;
+	ctx->info.expansion_order = ctx->tree.order;
+	return true;
+}
+
+// The trees of spec.nstars.size() independent fields built together: level 0
+// holds one root per field (centre 0, half-length root_hl[f], the field's
+// stars [soff[f], soff[f] + nstars[f]) of ctx->spos), every level splits the
+// nodes that the reference's rule splits, and since siblings and neighbours
+// never leave a parent's subtree the fields stay apart. One expansion order
+// serves all fields: fmm_order if set, else the largest the reference's rule
+// gives over the fields (the rule of ccf.cuh:840-852 itself for one field).
+bool build_forest(ccf_ctx* ctx, ForestSpec& spec, int verbose)
+{
+	DeviceTree& T = ctx->tree;
+	const ccf_params& P = ctx->P;
+	const int F = (int)spec.nstars.size();
+	int N = 0;
+	for (int f = 0; f < F; f++) N = std::max(N, spec.soff[f] + spec.nstars[f]);
+	hipStream_t s = ctx->stream;
+	auto t0 = std::chrono::steady_clock::now();
 	if (verbose >= 1) std::cout << "Creating children and sorting stars...\n";
 
 	T.nnodes = 0;
 	T.capacity = 0;
-	if (!ensure_capacity(ctx, 4096 + N / 2)) return false;
+	if (!ensure_capacity(ctx, 4096 + N / 2 + 4 * F)) return false;
 	int* scal = nullptr;
 	if (!CCF_HIP(ctx, hipMalloc(&scal, 4 * sizeof(int)))) return false;
 
-	// root node
+	// root nodes
 	{
-		double2 c = make_double2(0, 0);
-		double h = ctx->root_half_length;
-		int m1 = -1, z = 0;
-		bool ok = CCF_HIP(ctx, hipMemcpyAsync(T.center, &c, sizeof(c), hipMemcpyHostToDevice, s)) &&
-			CCF_HIP(ctx, hipMemcpyAsync(T.hl, &h, sizeof(h), hipMemcpyHostToDevice, s)) &&
-			CCF_HIP(ctx, hipMemcpyAsync(T.parent, &m1, sizeof(int), hipMemcpyHostToDevice, s)) &&
-			CCF_HIP(ctx, hipMemcpyAsync(T.child, &m1, sizeof(int), hipMemcpyHostToDevice, s)) &&
-			CCF_HIP(ctx, hipMemcpyAsync(T.sstart, &z, sizeof(int), hipMemcpyHostToDevice, s)) &&
-			CCF_HIP(ctx, hipMemcpyAsync(T.scount, &N, sizeof(int), hipMemcpyHostToDevice, s)) &&
-			CCF_HIP(ctx, hipMemcpyAsync(T.nnbr, &z, sizeof(int), hipMemcpyHostToDevice, s)) &&
-			CCF_HIP(ctx, hipMemcpyAsync(T.nsl, &z, sizeof(int), hipMemcpyHostToDevice, s)) &&
-			CCF_HIP(ctx, hipMemcpyAsync(T.ndl, &z, sizeof(int), hipMemcpyHostToDevice, s)) &&
+		std::vector<double2> c(F, make_double2(0, 0));
+		std::vector<int> m1(F, -1), z(F, 0), fid(F);
+		for (int f = 0; f < F; f++) fid[f] = f;
+		bool ok = CCF_HIP(ctx, hipMemcpyAsync(T.center, c.data(), F * sizeof(double2), hipMemcpyHostToDevice, s)) &&
+			CCF_HIP(ctx, hipMemcpyAsync(T.hl, spec.root_hl.data(), F * sizeof(double), hipMemcpyHostToDevice, s)) &&
+			CCF_HIP(ctx, hipMemcpyAsync(T.parent, m1.data(), F * sizeof(int), hipMemcpyHostToDevice, s)) &&
+			CCF_HIP(ctx, hipMemcpyAsync(T.child, m1.data(), F * sizeof(int), hipMemcpyHostToDevice, s)) &&
+			CCF_HIP(ctx, hipMemcpyAsync(T.sstart, spec.soff.data(), F * sizeof(int), hipMemcpyHostToDevice, s)) &&
+			CCF_HIP(ctx, hipMemcpyAsync(T.scount, spec.nstars.data(), F * sizeof(int), hipMemcpyHostToDevice, s)) &&
+			CCF_HIP(ctx, hipMemcpyAsync(T.nnbr, z.data(), F * sizeof(int), hipMemcpyHostToDevice, s)) &&
+			CCF_HIP(ctx, hipMemcpyAsync(T.nsl, z.data(), F * sizeof(int), hipMemcpyHostToDevice, s)) &&
+			CCF_HIP(ctx, hipMemcpyAsync(T.ndl, z.data(), F * sizeof(int), hipMemcpyHostToDevice, s)) &&
+			CCF_HIP(ctx, hipMemcpyAsync(T.field, fid.data(), F * sizeof(int), hipMemcpyHostToDevice, s)) &&
 			CCF_HIP(ctx, hipStreamSynchronize(s));
 		if (!ok) { hipFree(scal); return false; }
 	}
-	T.nnodes = 1;
+	T.nnodes = F;
 	T.level_off = {0};
-	T.level_cnt = {1};
+	T.level_cnt = {F};
 	T.levels = 0;
 
 	bool ok = true;
@@ -591,7 +617,7 @@ bool build_tree(ccf_ctx* ctx, int verbose)
 		if (verbose >= 3) std::cout << "Number of non-empty children: " << new_cnt << "\n";
 		if (!(ok = ensure_capacity(ctx, new_off + new_cnt))) break;
 		k_make_children<<<grid(cnt), BLOCK, 0, s>>>(off, cnt, new_off, T.split, T.rank, T.center, T.hl, T.parent, T.child,
-			T.sstart, T.scount, T.nnbr, T.nsl, T.ndl);
+			T.sstart, T.scount, T.nnbr, T.nsl, T.ndl, T.field);
 		k_partition<<<cnt, BLOCK, 0, s>>>(off, T.split, T.child, T.center, T.sstart, T.scount, ctx->spos, ctx->smass,
 			ctx->spos_tmp, ctx->smass_tmp);
 		k_set_neighbors<<<grid(new_cnt), BLOCK, 0, s>>>(new_off, new_cnt, T.center, T.hl, T.parent, T.child, T.scount,
@@ -634,11 +660,11 @@ bool build_tree(ccf_ctx* ctx, int verbose)
 		if (T.lookup) (void)hipFree(T.lookup);
 		T.lookup = nullptr;
 		T.lookup_level = 0;
-		if (T.levels >= 2)
+		if (F == 1 && T.levels >= 2)
 		{
 			T.lookup_level = std::min(T.levels, MAX_LOOKUP_LEVEL);
 			const int n = 1 << T.lookup_level;
-			const double H = ctx->root_half_length;
+			const double H = spec.root_hl[0];
 			T.lookup_x0 = -H;
 			T.lookup_inv = n / (2 * H);
 			if (!CCF_HIP(ctx, hipMalloc(&T.lookup, sizeof(int) * (size_t)n * n))) return false;
@@ -664,10 +690,21 @@ bool build_tree(ccf_ctx* ctx, int verbose)
 		if (!CCF_LAUNCH(ctx, "near-field copies") || !CCF_HIP(ctx, hipStreamSynchronize(s))) return false;
 	}
 
+	// depth of every field's tree
+	spec.levels.assign(F, 0);
+	{
+		std::vector<int> fid(T.nnodes);
+		if (!CCF_HIP(ctx, hipMemcpy(fid.data(), T.field, sizeof(int) * T.nnodes, hipMemcpyDeviceToHost))) return false;
+		for (int l = 0; l <= T.levels; l++)
+			for (int n = T.level_off[l]; n < T.level_off[l] + T.level_cnt[l]; n++) spec.levels[fid[n]] = l;
+	}
 	// expansion order (ccf.cuh:840-852)
-	int p;
+	int p = 0;
 	if (P.fmm_order > 0) p = P.fmm_order;
-	else p = (int)std::ceil(2 * std::log2(ctx->theta) + T.levels - std::log2(ctx->root_half_length) - std::log2(ctx->alpha_error));
+	else
+		for (int f = 0; f < F; f++)
+			p = std::max(p, (int)std::ceil(2 * std::log2(spec.theta[f]) + spec.levels[f] - std::log2(spec.root_hl[f]) -
+				std::log2(spec.alpha_error[f])));
 	if (verbose >= 2) std::cout << "expansion_order set to: " << p << "\n\n";
 	if (!P.direct)
 	{
@@ -678,7 +715,7 @@ bool build_tree(ccf_ctx* ctx, int verbose)
 			return false;
 		}
 	}
-	ctx->expansion_order = p;
+	spec.order = p;
 	T.order = P.direct ? 0 : p;
 
 	// FMM: upward from the deepest level, downward from level 2 (ccf.cuh:866-893)
@@ -746,9 +783,6 @@ bool build_tree(ccf_ctx* ctx, int verbose)
 	if (verbose >= 1)
 		std::cout << "Done calculating multipole and local coefficients. Elapsed time: " << std::chrono::duration<double>(t2 - t1).count() << " seconds.\n\n";
 	ctx->info.t_tree = std::chrono::duration<double>(t2 - t0).count();
-	ctx->info.tree_levels = T.levels;
-	ctx->info.num_nodes = T.nnodes;
-	ctx->info.expansion_order = T.order;
 	return true;
 }
 

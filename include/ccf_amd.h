@@ -178,6 +178,38 @@ int ccf_aberth_sweep(ccf_ctx* ctx, const double* roots, int n, double phi, doubl
    for checking the FMM against direct sums. */
 int ccf_root_sums(ccf_ctx* ctx, const double* roots, int n, double* out);
 
+/* ---- batched fields ----
+   No counterpart in the reference, which solves one field per process
+   (main_ccf.cu:514-515). Fields of uneven size are added with their own
+   parameters (as for ccf_set_params) and solved together: their trees are
+   built as one forest and the Aberth solves of all fields run in shared
+   persistent launches. Every field then has the reference's outputs and
+   files (ccf_batch_save). The fields must share fmm_order, direct,
+   binomial_exact and precision; the expansion order is the largest the
+   reference's rule gives over the fields unless fmm_order is set. */
+typedef struct ccf_batch ccf_batch;
+ccf_batch* ccf_batch_create(void);
+void ccf_batch_destroy(ccf_batch* b);
+/* adds a field; returns its index, or -1 */
+int ccf_batch_add(ccf_batch* b, const ccf_params* p);
+int ccf_batch_size(const ccf_batch* b);
+/* every field: validation, derived parameters, stars, seeds (device) */
+int ccf_batch_load(ccf_batch* b, int verbose);
+/* the forest of all fields' trees with its FMM coefficients */
+int ccf_batch_build(ccf_batch* b, int verbose);
+/* initial roots, critical curves, 1/mu errors, caustics and length scales of every field */
+int ccf_batch_solve(ccf_batch* b, int verbose);
+/* load + build + solve */
+int ccf_batch_run(ccf_batch* b, int verbose);
+/* the reference's files of one field (its outfile_prefix) */
+int ccf_batch_save(ccf_batch* b, int field, int verbose);
+/* info of one field, or of the whole batch for field = -1 (totals; t_ccs = solve time) */
+int ccf_batch_get_info(const ccf_batch* b, int field, ccf_info* info);
+int ccf_batch_get_array(ccf_batch* b, int field, int which, void* dst, long long capacity, int* nrows, int* ncols);
+/* workgroup size the autotuner picked (0 before the first solve) */
+int ccf_batch_workgroup(const ccf_batch* b);
+const char* ccf_batch_last_error(const ccf_batch* b);
+
 /* Write a seeded synthetic star file (layout of star.cuh:498-534) drawn
    on the host with std::mt19937_64: positions uniform in the disc or
    rectangle the reference derives for these parameters (ccf.cuh:420-436),

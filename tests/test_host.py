@@ -171,6 +171,19 @@ def test_float_star_files_are_read(workdir):
         assert c.info()["kappa_star"] == pytest.approx(1.5 / 9.0)
 
 
+def test_batch_rejects_fields_with_different_fmm_settings(workdir):
+    """checked before any device work"""
+    a, b_ = make_field(workdir, "cfg1"), make_field(workdir, "salpeter300")
+    with ccf_amd.Batch() as b:
+        b.add(starfile=a, kappa_tot=0.3, shear=0.1, num_phi=10)
+        b.add(starfile=b_, kappa_tot=0.45, shear=0.35, num_phi=10, fmm_order=12)
+        assert not b.run(0)
+        assert "must share fmm_order, direct, binomial_exact and precision" in b.last_error()
+    with ccf_amd.Batch() as b:
+        assert not b.run(0)
+        assert "no fields" in b.last_error()
+
+
 # ----------------------------------------------------------------- CLI
 CLI = ccf_amd.CLI_PATH
 
