@@ -194,12 +194,121 @@ def summarize(times, infos):
                 expansion_order=inf["expansion_order"], tree_levels=inf["tree_levels"], num_roots=inf["num_roots"])
 
 
+# 256 independent fields: N uniform in [50, 1000], equal masses, kappa_star =
+# kappa_tot uniform in [0.1, 0.6], shear uniform in [0, 0.5], num_phi 504
+# (500 breaks the rule num_phi % (2 num_branches) == 0), num_branches 4,
+# field k drawn and written with seed 100 + k
+BATCH_FIELDS, BATCH_RUN = 256, dict(num_phi=504, num_branches=4)
+
+
+def batch_fields(tmp: str):
+    fields = []
+    for k in range(BATCH_FIELDS):
+        seed = 100 + k
+        r = np.random.default_rng(seed)
+        n = int(r.integers(50, 1001))
+        ks = float(r.uniform(0.1, 0.6))
+        g = float(r.uniform(0.0, 0.5))
+        path = os.path.join(tmp, f"field{k}.bin")
+        ccf_amd.write_synthetic_stars(path, seed, num_stars=n, kappa_tot=ks, kappa_star=ks, shear=g, mass_function="equal")
+        fields.append((path, dict(kappa_tot=ks, shear=g), n))
+    return fields
+
+
+def run_batch(fields, precision: int, predictor: int, steps: int, warmup: int):
+    b = ccf_amd.Batch()
+    for path, kw, _ in fields:
+        b.add(starfile=path, outfile_prefix="./", write_stars=0, write_critical_curves=0, write_caustics=1,
+              precision=precision, predictor=predictor, **kw, **BATCH_RUN)
+    times = []
+    for i in range(warmup + steps):
+        b.load()
+        t0 = time.perf_counter()
+        b.build()
+        b.solve()
+        dt = time.perf_counter() - t0
+        print(f"# batch precision={precision} predictor={predictor} step {i} ({'warmup' if i < warmup else 'timed'}): "
+              f"{dt:.3f} s (workgroup {b.workgroup()})", file=sys.stderr, flush=True)
+        if i >= warmup:
+            times.append(dt)
+    inf = b.info(-1)
+    orders = sorted({b.info(f)["expansion_order"] for f in range(len(fields))})
+    ccs = [b.array(f, ccf_amd.ARRAY_CCS) for f in range(len(fields))]
+    wg = b.workgroup()
+    b.close()
+    t = float(np.mean(times))
+    return dict(points_per_s=inf["points_converged"] / t, s_per_step=t, points_converged=inf["points_converged"],
+                points_total=inf["points_total"], sweeps=inf["sweeps"], updates_full=inf["updates_full"],
+                updates_early=inf["updates_early"], max_error_1_over_mu=inf["max_error"], workgroup=wg,
+                expansion_orders=orders, t_tree=inf["t_tree"], t_solve=inf["t_ccs"]), ccs
+
+
+def run_sequential(fields, nfields: int):
+    """the plain FP64 single-field path, one field after another (stars in
+    HBM before each timed field, as for the batch)"""
+    tot = conv = pts = 0.0
+    for path, kw, _ in fields[:nfields]:
+        with ccf_amd.Context(starfile=path, outfile_prefix="./", write_stars=0, write_critical_curves=0, write_caustics=1,
+                             **kw, **BATCH_RUN) as c:
+            c.stage("load")
+            t0 = time.perf_counter()
+            for st in ("build_tree_fmm", "find_initial_roots", "find_ccs", "find_caustics"):
+                c.stage(st)
+            tot += time.perf_counter() - t0
+            inf = c.info()
+            conv += inf["points_converged"]
+            pts += inf["points_total"]
+    return dict(points_per_s=conv / tot, seconds=tot, fields=nfields, points_converged=conv, points_total=pts)
+
+
+def bench_batch(args):
+    tmp = tempfile.mkdtemp(prefix="ccfbatch_")
+    fields = batch_fields(tmp)
+    fp64, ccs64 = run_batch(fields, 0, 0, args.steps, args.warmup)
+    mixed, ccsm = run_batch(fields, 1, args.predictor, args.steps, args.warmup)
+    # the mixed batch against the FP64 batch, field by field
+    worst = 0.0
+    ndiff = 0
+    for (path, kw, n), a, b in zip(fields, ccsm, ccs64):
+        err, pairs = set_error(a, b, 2 * n, BATCH_RUN["num_branches"])
+        worst = max(worst, err)
+        ndiff += len(pairs)
+    mixed["max_root_err_vs_fp64_thetaE"] = worst
+    mixed["points_differing_over_1e-9"] = ndiff
+    seq = run_sequential(fields, min(args.seq_fields, len(fields)))
+    line = {
+        "metric": "converged cc points/s over 256 uneven fields (batched form) on 1 MI355X",
+        "value": mixed["points_per_s"],
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": mixed["s_per_step"] * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64+f32",
+        "data": "synthetic",
+        "config": {"workload": "batch256: 256 fields, N ~ U{50..1000} equal masses, kappa_star = kappa_tot ~ U(0.1, 0.6), "
+                               "shear ~ U(0, 0.5), num_phi 504, num_branches 4, seeds 100..355",
+                   "points": mixed["points_total"]},
+        "batched_mixed": mixed,
+        "batched_fp64": fp64,
+        "single_field_fp64_sequential": seq,
+        "speedup_batched_fp64_vs_sequential": fp64["points_per_s"] / seq["points_per_s"],
+        "speedup_batched_mixed_vs_sequential": mixed["points_per_s"] / seq["points_per_s"],
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="cc10k", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="cc10k", choices=sorted(WORKLOADS) + ["batch256"])
+    ap.add_argument("--seq-fields", type=int, default=64,
+                    help="batch256: fields of the sample timed one by one on the single-field FP64 path")
     ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"])
     ap.add_argument("--cpu-steps", type=int, default=8, help="phi steps per side in the CPU sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -213,8 +322,8 @@ def main():
                     help="first copy MIB MiB device to device: the known-size kernel that scales the HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus must be 1 (the path would shard by branch, see DESIGN.md)",
-              file=sys.stderr)
+        print("bench.py: this build runs on one GPU; --gpus must be 1 (the path would shard by branch across GPUs, "
+              "which is not part of this build)", file=sys.stderr)
         sys.exit(2)
 
     if args.calibrate_copy > 0:
@@ -226,6 +335,10 @@ def main():
         torch.cuda.synchronize()
         del a, b
         print(f"# copied {args.calibrate_copy} MiB device to device", file=sys.stderr, flush=True)
+
+    if args.workload == "batch256":
+        bench_batch(args)
+        return
 
     W = WORKLOADS[args.workload]
     tmp = tempfile.mkdtemp(prefix="ccfbench_")
