@@ -102,6 +102,10 @@ class Info(ctypes.Structure):
         ("root_fmm_rows", ctypes.c_longlong),
         ("root_fmm_order", ctypes.c_int),
         ("root_fmm_leaf_level", ctypes.c_int),
+        ("max_witt_residual", ctypes.c_double),
+        ("witt_kernel_ms", ctypes.c_double),
+        ("witt_launches", ctypes.c_longlong),
+        ("flops_witt", ctypes.c_double),
     ]
 
     def as_dict(self):

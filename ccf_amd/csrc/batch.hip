@@ -85,6 +85,7 @@ enum BStat
 	B_SWEEPS = 4,
 	B_ERRBITS = 5,
 	B_NAN = 6,
+	B_RESID = 7,
 	BSTAT = 8
 };
 
@@ -344,7 +345,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_errors(const BField* __restrict
 {
 	const BField& F = fields[blockIdx.y];
 	const long long total = (long long)(F.np + F.nb) * F.R;
-	double e = 0;
+	double e = 0, res = 0;
 	unsigned long long bad = 0;
 	for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (long long)gridDim.x * blockDim.x)
 	{
@@ -353,15 +354,25 @@ __global__ __launch_bounds__(BLOCK) void k_batch_errors(const BField* __restrict
 		const double2 v = F.roots[idx];
 		const cd z(v.x, v.y);
 		const int node = nearest_node(F.L, z);
-		const double b = err_bound(witt_from(F.L, z, t.x, t.y, node, p2p(F.L, node, z)));
+		const Witt w = witt_from(F.L, z, t.x, t.y, node, p2p(F.L, node, z));
+		const double b = err_bound(w);
 		if (!isfinite(b)) bad = 1;
-		else e = fmax(e, b);
+		else
+		{
+			e = fmax(e, b);
+			res = fmax(res, w.f0.abs());
+		}
 	}
-	for (int d = 32; d >= 1; d >>= 1) e = fmax(e, __shfl_xor(e, d, 64));
+	for (int d = 32; d >= 1; d >>= 1)
+	{
+		e = fmax(e, __shfl_xor(e, d, 64));
+		res = fmax(res, __shfl_xor(res, d, 64));
+	}
 	bad = wave_sum(bad);
 	if ((threadIdx.x & 63) == 0)
 	{
 		atomicMax(&F.stats[B_ERRBITS], (unsigned long long)__double_as_longlong(e));
+		atomicMax(&F.stats[B_RESID], (unsigned long long)__double_as_longlong(res));
 		if (bad) atomicOr(&F.stats[B_NAN], 1ull);
 	}
 }
@@ -754,6 +765,8 @@ bool batch_solve(ccf_batch* b, int verbose)
 		std::memcpy(&e, &x[B_ERRBITS], sizeof(e));
 		c->max_error = e;
 		c->info.max_error = e;
+		std::memcpy(&c->info.max_witt_residual, &x[B_RESID], sizeof(double));
+		b->info.max_witt_residual = std::max(b->info.max_witt_residual, c->info.max_witt_residual);
 		c->info.points_total = pts;
 		c->info.points_converged = (long long)x[B_CONV];
 		c->info.updates_full = (long long)x[B_FULL];

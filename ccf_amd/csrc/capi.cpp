@@ -96,12 +96,15 @@ void ccf_ctx::free_device()
 	F(counters);
 	if (host_counts) (void)hipHostFree(host_counts);
 	host_counts = nullptr;
-	for (auto& t : timers)
+	for (auto* v : {&timers, &witt_timers})
 	{
-		if (t.a) hipEventDestroy(t.a);
-		if (t.b) hipEventDestroy(t.b);
+		for (auto& t : *v)
+		{
+			if (t.a) hipEventDestroy(t.a);
+			if (t.b) hipEventDestroy(t.b);
+		}
+		v->clear();
 	}
-	timers.clear();
 	loaded = prepared = have_roots = have_ccs = have_caustics = have_mu = false;
 }
 

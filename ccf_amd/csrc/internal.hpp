@@ -205,6 +205,7 @@ struct ccf_ctx
 	double* ones = nullptr;       // unit masses for the pole-sum expansions
 	unsigned long long* counters = nullptr; // device statistics, see solve.hip
 	std::vector<ccf::Timer> timers;
+	std::vector<ccf::Timer> witt_timers;
 	int* host_counts = nullptr;   // pinned: active roots per row, read back between sweeps
 	double2* phase = nullptr;     // (cos phi, sin phi) of every phi row
 	bool batch_member = false;    // a field of a ccf_batch: no sweep buffers, seeds on the device

@@ -54,6 +54,7 @@ enum Counter
 	C_RM2L = 8,    // root-sum FMM: M2L translations
 	C_RROWS = 9,   // root-sum FMM: (row, sweep) pairs summed by the FMM
 	C_RTGT = 10,   // root-sum FMM: updates whose root sums came from the FMM
+	C_RESID = 11,  // max |f0| over the solved points (bits of a non-negative double)
 	C_COUNT = 16
 };
 
@@ -837,7 +838,7 @@ __global__ void k_errors(LensView L, const double2* __restrict__ roots, int R, i
 {
 	const long long total = (long long)(np + nb) * R;
 	const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-	double e = 0;
+	double e = 0, res = 0;
 	unsigned long long bad = 0;
 	if (idx < total)
 	{
@@ -848,13 +849,19 @@ __global__ void k_errors(LensView L, const double2* __restrict__ roots, int R, i
 		const int node = nearest_node(L, z);
 		Witt w = witt_from(L, z, t.x, t.y, node, p2p(L, node, z));
 		e = err_bound(w);
-		if (!isfinite(e)) { bad = 1; e = 0; }
+		res = w.f0.abs();
+		if (!isfinite(e)) { bad = 1; e = 0; res = 0; }
 	}
-	for (int d = 32; d >= 1; d >>= 1) e = fmax(e, __shfl_xor(e, d, 64));
+	for (int d = 32; d >= 1; d >>= 1)
+	{
+		e = fmax(e, __shfl_xor(e, d, 64));
+		res = fmax(res, __shfl_xor(res, d, 64));
+	}
 	bad = wave_sum(bad);
 	if ((threadIdx.x & 63) == 0)
 	{
 		atomicMax(stat_slot(stats, C_ERRBITS), (unsigned long long)__double_as_longlong(e));
+		atomicMax(stat_slot(stats, C_RESID), (unsigned long long)__double_as_longlong(res));
 		if (bad) atomicOr(stat_slot(stats, C_NAN), 1ull);
 	}
 }
@@ -992,6 +999,13 @@ bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int
 	{
 		const SweepView V = view(B, it & 1);
 		const dim3 g_far(WITT_FAR_BLOCKS, nrows);
+		ccf::Timer tw;
+		if (ctx->P.timing)
+		{
+			(void)hipEventCreate(&tw.a);
+			(void)hipEventCreate(&tw.b);
+			(void)hipEventRecord(tw.a, s);
+		}
 		if (f32)
 		{
 			k_witt<true><<<g_rows, BLOCK, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, fmm_pole ? 1 : 0);
@@ -1001,6 +1015,11 @@ bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int
 		{
 			k_witt<false><<<g_rows, BLOCK, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, 0);
 			k_witt_far<false><<<g_far, BLOCK, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, 0);
+		}
+		if (ctx->P.timing)
+		{
+			(void)hipEventRecord(tw.b, s);
+			ctx->witt_timers.push_back(tw);
 		}
 		const bool fmm = B.grid.order > 0 && it < fmm_sweeps;
 		k_plan<<<1, 1024, 0, s>>>(V, nrows, Np + R, sl.target, fmm ? root_fmm_min(R) : 0);
@@ -1229,7 +1248,7 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 			for (int q = 0; q < STAT_SLOTS; q++)
 			{
 				const unsigned long long v = raw[((size_t)k * STAT_SLOTS + q) * STAT_STRIDE];
-				if (k == C_ERRBITS) c[k] = std::max(c[k], v);
+				if (k == C_ERRBITS || k == C_RESID) c[k] = std::max(c[k], v);
 				else if (k == C_NAN) c[k] |= v;
 				else c[k] += v;
 			}
@@ -1242,6 +1261,7 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 	double e;
 	std::memcpy(&e, &c[C_ERRBITS], sizeof(e));
 	ctx->max_error = e;
+	std::memcpy(&ctx->info.max_witt_residual, &c[C_RESID], sizeof(double));
 	if (verbose >= 1) std::cout << "Maximum error in 1/mu: " << e << "\n\n";
 	ctx->info.points_total = total;
 	ctx->info.points_converged = (long long)c[C_CONV];
@@ -1292,7 +1312,20 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 		ctx->info.aberth_kernel_ms = ms;
 		ctx->info.aberth_launches = (long long)ctx->timers.size();
 		ctx->timers.clear();
+		double mw = 0;
+		for (auto& t : ctx->witt_timers)
+		{
+			float x = 0;
+			(void)hipEventElapsedTime(&x, t.a, t.b);
+			mw += x;
+			(void)hipEventDestroy(t.a);
+			(void)hipEventDestroy(t.b);
+		}
+		ctx->info.witt_kernel_ms = mw;
+		ctx->info.witt_launches = (long long)ctx->witt_timers.size();
+		ctx->witt_timers.clear();
 	}
+	ctx->info.flops_witt = ctx->info.flops_update - ctx->info.flops_sums;
 	return true;
 }
 

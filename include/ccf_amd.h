@@ -105,6 +105,10 @@ typedef struct ccf_info
 	long long root_fmm_rows;   /* (row, sweep) pairs summed by the root-sum FMM */
 	int root_fmm_order;        /* its expansion order (0: not used) */
 	int root_fmm_leaf_level;   /* depth of its grid */
+	double max_witt_residual;  /* max |f0| (Witt's function) over the solved points */
+	double witt_kernel_ms;     /* sum of timed Witt-evaluation launches (params.timing = 1) */
+	long long witt_launches;
+	double flops_witt;         /* algorithmic flops of the Witt evaluations (flops_update - flops_sums) */
 } ccf_info;
 
 /* Arrays readable with ccf_get_array. */

@@ -212,3 +212,48 @@ def test_run_matches_oracle(workdir, name, kw, gkw):
     assert ginfo["points_converged"] == ostats["points_converged"] == R * (kw["num_phi"] + nb)
     if gkw.get("root_fmm"):
         assert ginfo["root_fmm_updates"] > 0 and ginfo["root_fmm_rows"] > 0
+
+
+def test_cli_run_matches_oracle(workdir):
+    """ccf_gpu end to end (main_ccf.cu's flags, -sf star file): the files it
+    writes hold the oracle's critical curves and caustics"""
+    import subprocess
+
+    sf = make_field(workdir, "salpeter300")
+    gp = os.path.join(str(workdir), "cli_")
+    op = os.path.join(str(workdir), "cli_o_")
+    r = subprocess.run([ccf_amd.CLI_PATH, "-k", "0.45", "-y", "0.35", "-sf", sf, "-np", "60", "-nb", "3", "-ws", "1",
+                        "-wcc", "1", "-wc", "1", "-wls", "1", "-o", gp, "-v", "1"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "Done writing critical curve positions" in r.stdout
+    kw = dict(kappa_tot=0.45, shear=0.35, num_phi=60, num_branches=3)
+    ost = oracle_run(sf, op, write_mu_length_scales=1, **kw)
+    R = ost["num_roots"]
+    for f in ("ccf_ccs.bin", "ccf_caustics.bin"):
+        worst, nbad = compare_arrays(io.read_array(gp + f), io.read_array(op + f), R, 3)
+        assert nbad == 0, (f, worst, nbad)
+    a = io.read_params(gp + "ccf_parameter_info.txt")
+    assert a["num_roots"] == R and a["num_phi"] == 60 and a["max_error_1/mu"] < 1e-8
+
+
+def test_config2_full_size_matches_oracle(workdir):
+    """BASELINE config 2 at its own size: 2,000 Salpeter stars, kappa 0.45,
+    shear 0.35, num_phi 1000, num_branches 10, FMM order 20; every one of
+    the 4,040,000 critical-curve points against the oracle's full run"""
+    sf = os.path.join(str(workdir), "cfg2.bin")
+    ccf_amd.write_synthetic_stars(sf, 1, num_stars=2000, kappa_tot=0.45, kappa_star=0.45, shear=0.35,
+                                  mass_function="salpeter", m_lower=0.08, m_upper=1.0)
+    kw = dict(kappa_tot=0.45, shear=0.35, num_phi=1000, num_branches=10, fmm_order=20)
+    gp = os.path.join(str(workdir), "c2g_")
+    op = os.path.join(str(workdir), "c2o_")
+    with _ctx(sf, gp, **kw) as c:
+        assert c.run(0), c.last_error()
+        assert c.save(0), c.last_error()
+        gi = c.info()
+    ost = oracle_run(sf, op, num_threads=0, **kw)
+    R = ost["num_roots"]
+    assert gi["num_roots"] == R == 4000 and gi["points_converged"] == ost["points_converged"]
+    for f in ("ccf_ccs.bin", "ccf_caustics.bin"):
+        worst, nbad = compare_arrays(io.read_array(gp + f), io.read_array(op + f), R, 10)
+        assert nbad == 0, (f, worst, nbad)
+    assert gi["max_error"] < 1e-8 and np.isfinite(gi["max_witt_residual"])
