@@ -68,6 +68,7 @@ struct BField
 	const double2* seeds;    // [R] initial roots
 	const double2* cs;       // (cos phi, sin phi) per phi row
 	const double2* stars;    // the field's star positions, tree order
+	const int* order;        // [R] processing order of the roots
 	double2* ccs;            // [R nb][np/nb + 1] output (file order)
 	double2* caustics;       // same layout
 	double* mu;              // same layout
@@ -107,7 +108,7 @@ __device__ void solve_row(const BField& F, int row, int cap, double2* zc, const 
 	const int R = F.R;
 	const double2 e = F.cs[row];
 	const bool pole_fmm = MIXED && L.order > 0;
-	for (int q = threadIdx.x; q < R; q += BS) la[q] = q;
+	for (int q = threadIdx.x; q < R; q += BS) la[q] = F.order[q];
 	if (threadIdx.x == 0)
 	{
 		s_n[0] = R;
@@ -634,6 +635,7 @@ bool batch_build(ccf_batch* b, int verbose)
 		F.seeds = c->dseeds;
 		F.cs = c->phase;
 		F.stars = m->spos + spec.soff[f];
+		F.order = c->dorder;
 		F.ccs = c->ccs;
 		F.caustics = c->P.write_caustics ? c->caustics : nullptr;
 		F.mu = c->P.write_mu_length_scales ? c->mu_scales : nullptr;

@@ -91,7 +91,7 @@ void ccf_ctx::free_device()
 	F(T.field); F(T.nf); F(T.nnf); F(T.nbox); F(T.nf_off); F(T.nf_cnt); F(T.nfs); F(T.mp); F(T.loc); F(T.loc32); F(T.mpu); F(T.locu); F(T.locu32); F(T.binom); F(T.lookup);
 	T = ccf::DeviceTree{};
 	F(spos); F(smass); F(spos_tmp); F(smass_tmp); F(spos32); F(ts_a); F(ts_b); F(ones);
-	F(ccs_init); F(ccs); F(caustics); F(mu_scales); F(phase); F(dseeds);
+	F(ccs_init); F(ccs); F(caustics); F(mu_scales); F(phase); F(dseeds); F(dorder);
 	ccf::free_sweep(sw);
 	F(counters);
 	if (host_counts) (void)hipHostFree(host_counts);
@@ -498,6 +498,36 @@ bool populate(ccf_ctx* c, int verbose)
 		const double a = 2 * ccf::PI / nx * i;
 		c->seeds[2 * (2 * N + i)] = c->corner.abs() * std::cos(a);
 		c->seeds[2 * (2 * N + i) + 1] = c->corner.abs() * std::sin(a);
+	}
+	// order in which the roots are processed (not their indices, which stay
+	// the reference's): by the Morton key of their seeds, so that the lanes
+	// of a wave evaluate roots in neighbouring tree leaves
+	{
+		double x0 = 1e300, x1 = -1e300, y0 = 1e300, y1 = -1e300;
+		for (int i = 0; i < R; i++)
+		{
+			x0 = std::min(x0, c->seeds[2 * i]);
+			x1 = std::max(x1, c->seeds[2 * i]);
+			y0 = std::min(y0, c->seeds[2 * i + 1]);
+			y1 = std::max(y1, c->seeds[2 * i + 1]);
+		}
+		const double span = std::max(std::max(x1 - x0, y1 - y0), 1e-300);
+		auto spread = [](uint32_t v) {
+			v &= 0xffff;
+			v = (v | (v << 8)) & 0x00ff00ff;
+			v = (v | (v << 4)) & 0x0f0f0f0f;
+			v = (v | (v << 2)) & 0x33333333;
+			return (v | (v << 1)) & 0x55555555;
+		};
+		std::vector<uint32_t> key(R);
+		for (int i = 0; i < R; i++)
+		{
+			const double fx = std::clamp((c->seeds[2 * i] - x0) / span, 0.0, 1.0), fy = std::clamp((c->seeds[2 * i + 1] - y0) / span, 0.0, 1.0);
+			key[i] = spread((uint32_t)(fx * 65535.0)) | (spread((uint32_t)(fy * 65535.0)) << 1);
+		}
+		c->order.resize(R);
+		for (int i = 0; i < R; i++) c->order[i] = i;
+		std::stable_sort(c->order.begin(), c->order.end(), [&](int a2, int b2) { return key[a2] < key[b2]; });
 	}
 	// z-independent terms of the rectangular smooth-sheet series (alpha_smooth.cuh:201-204, 267-270)
 	if (c->rectangular && P.approx)

@@ -182,6 +182,7 @@ struct ccf_ctx
 	// host copies
 	std::vector<double> stars_in;   // {x1, x2, m} in generation/file order
 	std::vector<double> seeds;      // initial roots (re, im) interleaved
+	std::vector<int> order;         // processing order of the roots (Morton order of the seeds)
 
 	// device state
 	hipStream_t stream = nullptr;
@@ -211,6 +212,7 @@ struct ccf_ctx
 	bool batch_member = false;    // a field of a ccf_batch: no sweep buffers, seeds on the device
 	bool own_stream = true;       // ccf_destroy destroys the stream
 	double2* dseeds = nullptr;    // [R] initial roots (batch members)
+	int* dorder = nullptr;        // [R] processing order of the roots
 
 	ccf::LensView lens() const;
 	void fail(const std::string& msg);

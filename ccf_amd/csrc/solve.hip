@@ -786,7 +786,7 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(double theta, double2* __res
 // initial roots of a phase: row centre+sgn*j from the rows of the previous
 // steps (prepare_roots_kernel, ccf_functions.cuh:261-295; order > 0
 // extrapolates), every root active, the sweep counters cleared
-__global__ void k_prepare(double2* roots, RowMap M, SweepBufs B, int order)
+__global__ void k_prepare(double2* roots, RowMap M, SweepBufs B, int order, const int* __restrict__ perm)
 {
 	const int r = blockIdx.y;
 	const int R = M.R;
@@ -817,7 +817,7 @@ __global__ void k_prepare(double2* roots, RowMap M, SweepBufs B, int order)
 			}
 			roots[(size_t)row * R + a] = z;
 		}
-		B.act[(size_t)r * R + a] = a;
+		B.act[(size_t)r * R + a] = perm ? perm[a] : a;
 	}
 	if (a == 0)
 	{
@@ -834,10 +834,12 @@ __global__ void k_prepare(double2* roots, RowMap M, SweepBufs B, int order)
 // with a non-finite flag (find_errors_kernel + has_nan_err_kernel +
 // thrust::max_element, ccf_functions.cuh:419-489)
 __global__ void k_errors(LensView L, const double2* __restrict__ roots, int R, int np, int nb, const double2* __restrict__ cs,
-	unsigned long long* stats)
+	const int* __restrict__ perm, unsigned long long* stats)
 {
 	const long long total = (long long)(np + nb) * R;
-	const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	// points of a row in the roots' processing order
+	const long long idx = t < total ? (t / R) * R + perm[t % R] : t;
 	double e = 0, res = 0;
 	unsigned long long bad = 0;
 	if (idx < total)
@@ -1160,6 +1162,9 @@ bool alloc_solver(ccf_ctx* ctx)
 	const int R = ctx->num_roots, np = ctx->P.num_phi, nb = ctx->P.num_branches;
 	const size_t pts = (size_t)(np + nb) * R;
 	ctx->max_rows = 2 * nb;
+	if (!CCF_HIP(ctx, hipMalloc(&ctx->dorder, (size_t)R * sizeof(int))) ||
+		!CCF_HIP(ctx, hipMemcpyAsync(ctx->dorder, ctx->order.data(), (size_t)R * sizeof(int), hipMemcpyHostToDevice, ctx->stream)))
+		return false;
 	if (ctx->batch_member)
 		return CCF_HIP(ctx, hipMalloc(&ctx->ccs_init, pts * sizeof(double2))) &&
 			CCF_HIP(ctx, hipMalloc(&ctx->phase, (size_t)(np + nb) * sizeof(double2))) && launch_phase(ctx) &&
@@ -1189,7 +1194,7 @@ bool find_initial_roots_dev(ccf_ctx* ctx, int verbose)
 			return false;
 	}
 	const RowMap M{R, np, nb, 0, 1, 0.0, ctx->phase};
-	k_prepare<<<dim3(grid(R), nb), BLOCK, 0, s>>>(ctx->ccs_init, M, ctx->sw, 0);
+	k_prepare<<<dim3(grid(R), nb), BLOCK, 0, s>>>(ctx->ccs_init, M, ctx->sw, 0, ctx->dorder);
 	if (!CCF_LAUNCH(ctx, "prepare_roots_kernel")) return false;
 	const int cap = ctx->P.init_iters > 0 ? ctx->P.init_iters : 60;
 	if (!run_sweeps(ctx, ctx->sw, ctx->ccs_init, M, nb, cap, ROOT_FMM_INIT_SWEEPS)) return false;
@@ -1215,7 +1220,7 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 		// (boxcar / heaviside terms), where extrapolated roots could cross
 		// onto another branch: those keep the previous step's roots
 		const int order = ctx->P.approx ? ctx->P.predictor : 0;
-		k_prepare<<<dim3(grid(R), 2 * nb), BLOCK, 0, s>>>(ctx->ccs_init, M, ctx->sw, order);
+		k_prepare<<<dim3(grid(R), 2 * nb), BLOCK, 0, s>>>(ctx->ccs_init, M, ctx->sw, order, ctx->dorder);
 		if (!CCF_LAUNCH(ctx, "prepare_roots_kernel")) return false;
 		if (!run_sweeps(ctx, ctx->sw, ctx->ccs_init, M, 2 * nb, cap, ROOT_FMM_STEP_SWEEPS)) return false;
 		if (verbose >= 1 && j * 100 / steps > (j - 1) * 100 / steps)
@@ -1234,7 +1239,7 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 	// max 1/mu error and the NaN abort (ccf.cuh:1039-1069)
 	const long long total = (long long)(np + nb) * R;
 	const LensView L = ctx->lens();
-	k_errors<<<grid(total), BLOCK, 0, s>>>(L, ctx->ccs_init, R, np, nb, ctx->phase, ctx->counters);
+	k_errors<<<grid(total), BLOCK, 0, s>>>(L, ctx->ccs_init, R, np, nb, ctx->phase, ctx->dorder, ctx->counters);
 	if (!CCF_LAUNCH(ctx, "find_errors_kernel")) return false;
 	unsigned long long c[C_COUNT] = {};
 	{
@@ -1377,7 +1382,7 @@ bool aberth_sweep_dev(ccf_ctx* ctx, const double* roots, int n, double phi, doub
 	if (ok)
 	{
 		const RowMap M{n, 2, 1, 0, 0, phi};
-		k_prepare<<<dim3(grid(n), 1), BLOCK, 0, s>>>(row, M, B, 0);
+		k_prepare<<<dim3(grid(n), 1), BLOCK, 0, s>>>(row, M, B, 0, nullptr);
 		ok = CCF_LAUNCH(ctx, "prepare") && run_sweeps(ctx, B, row, M, 1, 1, 1) &&
 			CCF_HIP(ctx, hipMemcpyAsync(out, row, (size_t)n * sizeof(double2), hipMemcpyDeviceToHost, s)) &&
 			CCF_HIP(ctx, hipStreamSynchronize(s));
