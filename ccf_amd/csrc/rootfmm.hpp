@@ -20,6 +20,11 @@
 //      M2M         A_n(parent) = sum_k C(n,k) 2^-k A_k(child) e^(n-k),  e = (c_child - c_parent)/h_parent
 //      M2L         L_m += (-1)^m / h sum_n C(n+m,m) A_n d^-(n+m+1),    d = (c_target - c_source)/h
 //      L2L         L_k(child) = 2^-k sum_{m>=k} C(m,k) L_m(parent) e^(m-k)
+//    each formed without a table of binomials (whose constants would
+//    crowd the scalar registers): M2M as the in-place binomial transform
+//    a_n += e a_(n-1), L2L as a Taylor shift by repeated synthetic
+//    division, and M2L's sum_n C(n+m,m) b_n as element 0 after m + 1
+//    suffix-sum passes over b_n = A_n d^-n.
 //    The interaction list of a cell at level >= 1 is the set of children of
 //    its parent's neighbours that are not its own neighbours; at level 0 it
 //    is every top cell that is not a neighbour. A leaf's near field is its
@@ -32,7 +37,6 @@
 // 0.71^(p+1) of the far part, for the closest well-separated cells, far less
 // on average) steers the iteration, it does not move the roots it reaches.
 
-constexpr int RB = 2 * MAX_ROOT_FMM_ORDER; // binomial rows C(n, k), n <= 2p
 constexpr int ROOT_FMM_MIN_ROOTS = 64;     // smaller rows are always summed directly
 // sweeps of a phase that may use the FMM: the dense ones (the initial solve
 // converges over a few dozen sweeps; a continuation step mostly in one or two)
@@ -42,9 +46,6 @@ constexpr int ROOT_FMM_STEP_SWEEPS = 2;
 // listed roots from which a row's sums are cheaper by the FMM (its build
 // costs about as much as R/32 direct updates)
 inline int root_fmm_min(int R) { return std::max(32, R / 32); }
-__constant__ double c_rbinom[(RB + 1) * (RB + 1)];
-
-__device__ inline double rbinom(int n, int k) { return c_rbinom[n * (RB + 1) + k]; }
 
 struct RootView
 {
@@ -259,23 +260,22 @@ __global__ __launch_bounds__(BLOCK) void k_rm2m(RootView V, int l)
 		if (n == 0) continue;
 		tot += n;
 		const double2* src = V.mp + ch * (P + 1);
-		cd ak[P + 1], ep[P + 1];
+		cd ak[P + 1];
 		const cd e(a - 0.5, b - 0.5);
 		double sc = 1;
-		ep[0] = cd(1, 0);
 #pragma unroll
 		for (int k = 0; k <= P; k++)
 		{
 			ak[k] = cd(src[k].x * sc, src[k].y * sc);
 			sc *= 0.5;
-			if (k > 0) ep[k] = ep[k - 1] * e;
 		}
+		// binomial transform in place: a_n <- sum_k C(n, k) a_k e^(n-k)
 #pragma unroll
-		for (int m = 0; m <= P; m++)
-		{
+		for (int i = 1; i <= P; i++)
 #pragma unroll
-			for (int k = 0; k <= m; k++) A[m] += rbinom(m, k) * (ak[k] * ep[m - k]);
-		}
+			for (int n = P; n >= i; n--) ak[n] += e * ak[n - 1];
+#pragma unroll
+		for (int m = 0; m <= P; m++) A[m] += ak[m];
 	}
 	const size_t cid = base + V.g.off[l] + c;
 	V.ccnt[cid] = tot;
@@ -329,19 +329,14 @@ __global__ __launch_bounds__(32 * down_targets<P>()) void k_rdown32(RootView V, 
 			bq[n] = cd(am[n].x, am[n].y) * t;
 			t = t * rho;
 		}
+		// sum_n C(n+m, m) b_n is element 0 after the (m+1)-th suffix-sum pass
 		cd rp = rho * ih;
 #pragma unroll
 		for (int m = 0; m <= P; m++)
 		{
-			double sx = 0, sy = 0;
 #pragma unroll
-			for (int n = 0; n <= P; n++)
-			{
-				const double bc = rbinom(n + m, m);
-				sx = fma(bc, bq[n].re, sx);
-				sy = fma(bc, bq[n].im, sy);
-			}
-			const cd v = rp * cd(sx, sy);
+			for (int k = P - 1; k >= 0; k--) bq[k] += bq[k + 1];
+			const cd v = rp * bq[0];
 			if (m & 1) Lc[m] -= v;
 			else Lc[m] += v;
 			rp = rp * rho;
@@ -390,15 +385,21 @@ __global__ __launch_bounds__(32 * down_targets<P>()) void k_rdown32(RootView V, 
 		}
 		if (l >= 1)
 		{
-			// L2L: 2^-m sum_{n >= m} C(n, m) L_n(parent) e^(n - m)
+			// L2L: 2^-m sum_{n >= m} C(n, m) L_n(parent) e^(n - m), the
+			// parent's polynomial shifted by e (repeated synthetic division)
 			const double2* lp = V.lc + (base + V.g.off[l - 1] + (size_t)(iy >> 1) * (nxl >> 1) + (ix >> 1)) * (P + 1);
 			const cd e((ix & 1) - 0.5, (iy & 1) - 0.5);
-			cd s(0, 0), ep(1, 0);
-			for (int n = m; n <= P; n++)
-			{
-				s += rbinom(n, m) * (cd(lp[n].x, lp[n].y) * ep);
-				ep = ep * e;
-			}
+			cd c[P + 1];
+#pragma unroll
+			for (int n = 0; n <= P; n++) c[n] = cd(lp[n].x, lp[n].y);
+#pragma unroll
+			for (int i = 0; i < P; i++)
+#pragma unroll
+				for (int j = P - 1; j >= i; j--) c[j] += e * c[j + 1];
+			cd s(0, 0);
+#pragma unroll
+			for (int n = 0; n <= P; n++)
+				if (n == m) s = c[n];
 			const double sc = ldexp(1.0, -m);
 			sx += s.re * sc;
 			sy += s.im * sc;

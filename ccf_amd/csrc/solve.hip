@@ -1091,12 +1091,6 @@ bool alloc_root_fmm(ccf_ctx* ctx, SweepBufs& b)
 	const double leaf = ctx->P.root_fmm_leaf > 0 ? ctx->P.root_fmm_leaf : 6.0;
 	b.grid = plan_root_grid(hx, hy, R, order, leaf);
 	const RootGrid& g = b.grid;
-	std::vector<double> binom((RB + 1) * (RB + 1), 0.0);
-	for (int n = 0; n <= RB; n++)
-	{
-		binom[n * (RB + 1)] = 1;
-		for (int k = 1; k <= n; k++) binom[n * (RB + 1) + k] = binom[(n - 1) * (RB + 1) + k - 1] + (k < n ? binom[(n - 1) * (RB + 1) + k] : 0.0);
-	}
 	const size_t ce = (size_t)rows * g.ncells * (g.order + 1);
 	return CCF_HIP(ctx, hipMalloc(&b.slot, list * sizeof(int))) && CCF_HIP(ctx, hipMalloc(&b.rmode, rows * sizeof(int))) &&
 		CCF_HIP(ctx, hipMalloc(&b.rsum, list * sizeof(double2))) &&
@@ -1108,8 +1102,6 @@ bool alloc_root_fmm(ccf_ctx* ctx, SweepBufs& b)
 		CCF_HIP(ctx, hipMalloc(&b.mp, ce * sizeof(double2))) && CCF_HIP(ctx, hipMalloc(&b.lc, ce * sizeof(double2))) &&
 		CCF_HIP(ctx, hipMemsetAsync(b.slot, 0xff, list * sizeof(int), ctx->stream)) &&
 		CCF_HIP(ctx, hipMemsetAsync(b.rmode, 0, rows * sizeof(int), ctx->stream)) &&
-		CCF_HIP(ctx, hipMemcpyToSymbolAsync(HIP_SYMBOL(c_rbinom), binom.data(), binom.size() * sizeof(double), 0,
-			hipMemcpyHostToDevice, ctx->stream)) &&
 		CCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
 }
 
@@ -1282,12 +1274,13 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 		const double p = ctx->tree.order;
 		const double sm = (ctx->rectangular && ctx->P.approx) ? 8.0 * (ctx->taylor_smooth - 1) : 0.0;
 		const double full = (double)c[C_FULL];
-		// root-sum FMM: 10 per near-field term, 4 (p+1)^2 + 16 (p+1) per M2L
-		// (real binomials times complex terms), 8 p per root for P2M and for
-		// L2P, about 20 (p+1)^2 per cell for M2M and L2L
+		// root-sum FMM: 10 per near-field term, 8 (p+1)^2 per M2L (the
+		// textbook complex matrix-vector product; the suffix-sum form takes
+		// about as many), 8 p per root for P2M and for L2P, about 20 (p+1)^2
+		// per cell for M2M and L2L
 		const RootGrid& g = ctx->sw.grid;
 		const double pr = g.order, rt = (double)c[C_RTGT], rows = (double)c[C_RROWS];
-		const double fmm = 10.0 * (double)c[C_RNEAR] + (double)c[C_RM2L] * (4.0 * (pr + 1) * (pr + 1) + 16.0 * (pr + 1)) +
+		const double fmm = 10.0 * (double)c[C_RNEAR] + (double)c[C_RM2L] * 8.0 * (pr + 1) * (pr + 1) +
 			rows * R * 8.0 * pr + rt * 8.0 * pr + rows * g.ncells * 20.0 * (pr + 1) * (pr + 1);
 		ctx->info.flops_sums = (full - rt) * 10.0 * (Np + R - 1) + (g.order > 0 ? fmm : 0.0);
 		ctx->info.root_fmm_updates = (long long)c[C_RTGT];
