@@ -115,14 +115,16 @@ def set_error(a: np.ndarray, b: np.ndarray, R: int, nb: int, theta: float = 1.0)
     return worst / theta, pairs
 
 
-def polished_error(starfile: str, run_kw: dict, pairs) -> float:
-    """max |p(a) - p(b)| over the differing pairs, p = two FP64 Newton steps
-    z - f0/f1 on the GPU: the distance left once both roots are taken to
-    the exact root of Witt's equation."""
+def polished_error(starfile: str, run_kw: dict, pairs):
+    """Over the differing pairs (a, b), with p = two FP64 Newton steps
+    z - f0/f1 on the GPU (the exact root of Witt's equation to rounding):
+    max |p(a) - p(b)| (the two forms converge to the same root), max |a - p(a)|
+    and max |b - p(b)| (how far each form stopped from that root; the
+    reference's 1/mu early exit bounds |f0|, not the distance)."""
     if not pairs:
-        return 0.0
+        return 0.0, 0.0, 0.0
     npp, nb = run_kw["num_phi"], run_kw["num_branches"]
-    worst = 0.0
+    worst = da = db = 0.0
     with ccf_amd.Context(starfile=starfile, outfile_prefix="./", **run_kw) as ctx:
         ctx.prepare()
         by_col = {}
@@ -130,12 +132,15 @@ def polished_error(starfile: str, run_kw: dict, pairs) -> float:
             by_col.setdefault((c, k), []).append((x, y))
         for (c, k), zz in by_col.items():
             phi = 2 * np.pi * (c * npp // nb + k) / npp
-            z = np.array([p for pr in zz for p in pr], dtype=np.complex128)
+            z0 = np.array([p for pr in zz for p in pr], dtype=np.complex128)
+            z = z0
             for _ in range(2):
                 v = ctx.eval_points(z, phi)
                 z = z - (v[:, 0] + 1j * v[:, 1]) / (v[:, 2] + 1j * v[:, 3])
             worst = max(worst, float(np.abs(z[0::2] - z[1::2]).max()))
-    return worst
+            da = max(da, float(np.abs(z0[0::2] - z[0::2]).max()))
+            db = max(db, float(np.abs(z0[1::2] - z[1::2]).max()))
+    return worst, da, db
 
 
 def cpu_baseline(starfile: str, run_kw: dict, branches: int, threads: int, gpu_ccs: np.ndarray | None, R: int):
@@ -203,7 +208,8 @@ def summarize(times, infos):
     w_launches = max(1, inf["witt_launches"])
     return dict(points_per_s=conv / t, s_per_step=t, points_converged=conv, points_total=inf["points_total"],
                 t_tree=inf["t_tree"], t_init_roots=inf["t_init_roots"], t_ccs=inf["t_ccs"], t_caustics=inf["t_caustics"],
-                updates_full=inf["updates_full"], updates_early=inf["updates_early"], sweeps=inf["sweeps"],
+                updates_full=inf["updates_full"], updates_early=inf["updates_early"], updates_newton=inf["updates_newton"],
+                sweeps=inf["sweeps"],
                 root_fmm_updates=inf["root_fmm_updates"], root_fmm_rows=inf["root_fmm_rows"],
                 root_fmm_order=inf["root_fmm_order"], root_fmm_leaf_level=inf["root_fmm_leaf_level"],
                 cluster_chunks=inf["cluster_chunks"],
@@ -260,7 +266,7 @@ def run_batch(fields, precision: int, predictor: int, steps: int, warmup: int):
     t = float(np.mean(times))
     return dict(points_per_s=inf["points_converged"] / t, s_per_step=t, points_converged=inf["points_converged"],
                 points_total=inf["points_total"], sweeps=inf["sweeps"], updates_full=inf["updates_full"],
-                updates_early=inf["updates_early"], max_error_1_over_mu=inf["max_error"],
+                updates_early=inf["updates_early"], updates_newton=inf["updates_newton"], max_error_1_over_mu=inf["max_error"],
                 max_witt_residual=inf["max_witt_residual"], workgroup=wg,
                 expansion_orders=orders, t_tree=inf["t_tree"], t_solve=inf["t_ccs"]), ccs
 
@@ -335,7 +341,7 @@ def main():
     ap.add_argument("--cpu-branches", type=int, default=6, help="branches in the CPU sample, all phi steps (0: skip)")
     ap.add_argument("--cpu-steps", type=int, default=None, help="0: skip the CPU sample (same as --cpu-branches 0)")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--predictor", type=int, default=2, help="continuation predictor of the mixed form (0, 1, 2)")
+    ap.add_argument("--predictor", type=int, default=4, help="continuation predictor of the mixed form (0..4)")
     ap.add_argument("--root-fmm", type=int, default=1, help="root sums of dense sweeps by the FMM in the mixed form")
     ap.add_argument("--root-fmm-order", type=int, default=0, help="its expansion order (0: the library default)")
     ap.add_argument("--root-fmm-leaf", type=int, default=0, help="its mean roots per leaf cell (0: the library default)")
@@ -396,7 +402,11 @@ def main():
             err, pairs = set_error(ccs[name], ccs["fp64"], R, run_kw["num_branches"])
             res[name]["max_root_err_vs_fp64_thetaE"] = err
             res[name]["points_differing_over_1e-9"] = len(pairs)
-            res[name]["max_root_err_vs_fp64_after_newton_polish_thetaE"] = polished_error(sf, run_kw, pairs[:2000])
+            pol, da, db = polished_error(sf, run_kw, pairs[:2000])
+            res[name]["max_root_err_vs_fp64_after_newton_polish_thetaE"] = pol
+            # over the same pairs: each form's distance to the exact root
+            res[name]["differing_points_max_dist_to_exact_root_thetaE"] = da
+            res[name]["differing_points_fp64_max_dist_to_exact_root_thetaE"] = db
             res[name]["speedup_vs_fp64"] = res[name]["points_per_s"] / res["fp64"]["points_per_s"]
 
     head = res["mixed"] if "mixed" in res else res["fp64"]

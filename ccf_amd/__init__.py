@@ -106,6 +106,7 @@ class Info(ctypes.Structure):
         ("witt_kernel_ms", ctypes.c_double),
         ("witt_launches", ctypes.c_longlong),
         ("flops_witt", ctypes.c_double),
+        ("updates_newton", ctypes.c_longlong),
     ]
 
     def as_dict(self):

@@ -87,7 +87,8 @@ enum BStat
 	B_ERRBITS = 5,
 	B_NAN = 6,
 	B_RESID = 7,
-	BSTAT = 8
+	B_NEWTON = 8,
+	BSTAT = 9
 };
 
 struct BTask
@@ -99,10 +100,14 @@ __device__ inline int centre_row(const BField& F, int c) { return F.np / (2 * F.
 
 // All sweeps of one row (at most cap): zc holds the row's roots, la/lb are
 // the ping-pong active lists. Returns after the first sweep that leaves no
-// root active.
+// root active. newton (MIXED, continuation steps): a root whose FP64 Newton
+// step is already below the convergence threshold takes it and ends without
+// the sums, as in the single-field path.
+constexpr int BCNT = 6; // full, early, converged, near-field stars, sweeps, Newton
+
 template <int BS, int RPT, bool MIXED>
-__device__ void solve_row(const BField& F, int row, int cap, double2* zc, const double2* st, int* la, int* lb, int* s_n,
-	unsigned long long (&cnt)[5])
+__device__ void solve_row(const BField& F, int row, int cap, bool newton, double2* zc, const double2* st, int* la, int* lb,
+	int* s_n, unsigned long long (&cnt)[BCNT])
 {
 	const LensView& L = F.L;
 	const int R = F.R;
@@ -150,9 +155,21 @@ __device__ void solve_row(const BField& F, int row, int cap, double2* zc, const 
 			}
 			else
 			{
-				full[k] = true;
 				f0[k] = w.f0;
 				f1[k] = witt_f1(L, z, w);
+				if (MIXED && newton)
+				{
+					const cd dz = f0[k] / f1[k];
+					if (dz.abs() < CONV_TOL * L.theta) // false for a non-finite step
+					{
+						zc[i] = make_double2(z.re - dz.re, z.im - dz.im);
+						cnt[2]++;
+						cnt[3] += w.nr.count;
+						cnt[5]++;
+						continue;
+					}
+				}
+				full[k] = true;
 				ex[k] = cd();
 				if (F.circ_exact) ex[k] = 2.0 * z.inv(); // ccf_functions.cuh:223-227
 				if (pole_fmm) ex[k] += 2.0 * (w.nr.s0 + far_pole32(L, node, z));
@@ -212,7 +229,7 @@ __device__ void solve_row(const BField& F, int row, int cap, double2* zc, const 
 			const cd z(zx[k], zy[k]);
 			const cd sum = cd(2 * ax[k] - bx[k], -(2 * ay[k] - by[k])) + ex[k];
 			zn[k] = z - f0[k] / (f1[k] + f0[k] * sum);
-			moved[k] = !((zn[k] - z).abs() / L.theta < 1e-9); // a non-finite step stays active
+			moved[k] = !((zn[k] - z).abs() / L.theta < CONV_TOL); // a non-finite step stays active
 		}
 		__syncthreads();
 		// commit, and list the roots that moved by 1e-9 theta or more
@@ -251,7 +268,7 @@ __global__ __launch_bounds__(BS) void k_batch(const BField* __restrict__ fields,
 	int* lb = la + rmax;
 	__shared__ int s_task;
 	__shared__ int s_n[2];
-	__shared__ unsigned long long s_red[5][BS / 64];
+	__shared__ unsigned long long s_red[BCNT][BS / 64];
 	for (;;)
 	{
 		__syncthreads();
@@ -268,12 +285,12 @@ __global__ __launch_bounds__(BS) void k_batch(const BField* __restrict__ fields,
 			for (int s = threadIdx.x; s < F.N; s += BS) sl[s] = F.stars[s];
 			st = sl;
 		}
-		unsigned long long cnt[5] = {0, 0, 0, 0, 0};
+		unsigned long long cnt[BCNT] = {0, 0, 0, 0, 0, 0};
 		if (phase == 0)
 		{
 			const int row = centre_row(F, tk.branch);
 			for (int i = threadIdx.x; i < R; i += BS) zc[i] = F.seeds[i];
-			solve_row<BS, RPT, MIXED>(F, row, F.cap_init, zc, st, la, lb, s_n, cnt);
+			solve_row<BS, RPT, MIXED>(F, row, F.cap_init, false, zc, st, la, lb, s_n, cnt);
 			for (int i = threadIdx.x; i < R; i += BS) F.roots[(size_t)row * R + i] = zc[i];
 		}
 		else
@@ -288,41 +305,24 @@ __global__ __launch_bounds__(BS) void k_batch(const BField* __restrict__ fields,
 				// extrapolates from the rows this thread itself wrote
 				const int q = min(F.predictor, j - 1);
 				for (int i = threadIdx.x; i < R; i += BS)
-				{
-					const double2 z1 = F.roots[(size_t)(row - sgn) * R + i];
-					double2 z = z1;
-					if (q >= 1)
-					{
-						const double2 z2 = F.roots[(size_t)(row - 2 * sgn) * R + i];
-						if (q >= 2)
-						{
-							const double2 z3 = F.roots[(size_t)(row - 3 * sgn) * R + i];
-							z = make_double2(3 * (z1.x - z2.x) + z3.x, 3 * (z1.y - z2.y) + z3.y);
-						}
-						else
-						{
-							z = make_double2(2 * z1.x - z2.x, 2 * z1.y - z2.y);
-						}
-					}
-					zc[i] = z;
-				}
-				solve_row<BS, RPT, MIXED>(F, row, F.cap_step, zc, st, la, lb, s_n, cnt);
+					zc[i] = predict_root(F.roots + (size_t)(row - sgn) * R + i, (long long)sgn * R, q);
+				solve_row<BS, RPT, MIXED>(F, row, F.cap_step, true, zc, st, la, lb, s_n, cnt);
 				for (int i = threadIdx.x; i < R; i += BS) F.roots[(size_t)row * R + i] = zc[i];
 			}
 		}
 		// per-field counters, one atomic per counter and task
 #pragma unroll
-		for (int k = 0; k < 5; k++)
+		for (int k = 0; k < BCNT; k++)
 		{
 			const unsigned long long s = wave_sum(cnt[k]);
 			if ((threadIdx.x & 63) == 0) s_red[k][threadIdx.x >> 6] = s;
 		}
 		__syncthreads();
-		if (threadIdx.x < 5)
+		if (threadIdx.x < BCNT)
 		{
 			unsigned long long s = 0;
 			for (int w = 0; w < BS / 64; w++) s += s_red[threadIdx.x][w];
-			const int which[5] = {B_FULL, B_EARLY, B_CONV, B_NEAR, B_SWEEPS};
+			const int which[BCNT] = {B_FULL, B_EARLY, B_CONV, B_NEAR, B_SWEEPS, B_NEWTON};
 			if (s) atomicAdd(&F.stats[which[threadIdx.x]], s);
 		}
 	}
@@ -773,6 +773,7 @@ bool batch_solve(ccf_batch* b, int verbose)
 		c->info.points_converged = (long long)x[B_CONV];
 		c->info.updates_full = (long long)x[B_FULL];
 		c->info.updates_early = (long long)x[B_EARLY];
+		c->info.updates_newton = (long long)x[B_NEWTON];
 		c->info.sweeps = (long long)x[B_SWEEPS];
 		c->info.t_init_roots = 0;
 		c->info.t_ccs = dt;
@@ -784,6 +785,7 @@ bool batch_solve(ccf_batch* b, int verbose)
 		b->info.points_converged += (long long)x[B_CONV];
 		b->info.updates_full += (long long)x[B_FULL];
 		b->info.updates_early += (long long)x[B_EARLY];
+		b->info.updates_newton += (long long)x[B_NEWTON];
 		b->info.sweeps += (long long)x[B_SWEEPS];
 		b->info.max_error = std::max(b->info.max_error, e);
 		const double Np = c->num_stars + ((!c->rectangular && !c->P.approx) ? 1 : 0);

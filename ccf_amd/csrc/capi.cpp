@@ -96,15 +96,10 @@ void ccf_ctx::free_device()
 	F(counters);
 	if (host_counts) (void)hipHostFree(host_counts);
 	host_counts = nullptr;
-	for (auto* v : {&timers, &witt_timers})
-	{
-		for (auto& t : *v)
-		{
-			if (t.a) hipEventDestroy(t.a);
-			if (t.b) hipEventDestroy(t.b);
-		}
-		v->clear();
-	}
+	// the events themselves live until ccf_destroy
+	timers.clear();
+	witt_timers.clear();
+	events_used = 0;
 	loaded = prepared = have_roots = have_ccs = have_caustics = have_mu = false;
 }
 
@@ -168,7 +163,7 @@ bool check_input_params(ccf_ctx* c)
 	if (P.roots_per_thread != 0 && P.roots_per_thread != 1 && P.roots_per_thread != 2 && P.roots_per_thread != 4)
 		return err("Error. roots_per_thread must be 0 (automatic), 1, 2 or 4.\n");
 	if (P.init_iters < 0 || P.step_iters < 0) return err("Error. iteration caps must be >= 0\n");
-	if (P.predictor < 0 || P.predictor > 2) return err("Error. predictor must be 0, 1 or 2.\n");
+	if (P.predictor < 0 || P.predictor > ccf::MAX_PREDICTOR) return err("Error. predictor must be 0, 1, 2, 3 or 4.\n");
 	if (P.root_fmm != 0 && P.root_fmm != 1) return err("Error. root_fmm must be 1 (true) or 0 (false).\n");
 	if (P.root_fmm_order < 0 || P.root_fmm_order > ccf::MAX_ROOT_FMM_ORDER || (P.root_fmm_order > 0 && P.root_fmm_order < 3))
 		return err("Error. root_fmm_order must be 0 (automatic) or in [3, " + std::to_string(ccf::MAX_ROOT_FMM_ORDER) + "]\n");
@@ -598,6 +593,7 @@ void ccf_destroy(ccf_ctx* c)
 {
 	if (!c) return;
 	c->free_device();
+	for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
 	if (c->stream && c->own_stream) hipStreamDestroy(c->stream);
 	delete c;
 }

@@ -23,6 +23,8 @@ constexpr int MAX_DIFF_LEVEL = 5;        // tree_node.cuh:25
 constexpr int MAX_NEAR_RANGES = MAX_NEIGHBORS + 1;
 constexpr int MAX_TAYLOR_SMOOTH = 101;   // ccf.cuh:79
 constexpr int MAX_ROOT_FMM_ORDER = 24;
+// highest degree of the continuation predictor
+constexpr int MAX_PREDICTOR = 4;
 constexpr int MAX_ROOT_GRID_LEVELS = 10;
 
 // Quadtree over the stars, structure of arrays indexed by node id. Nodes of
@@ -207,6 +209,19 @@ struct ccf_ctx
 	unsigned long long* counters = nullptr; // device statistics, see solve.hip
 	std::vector<ccf::Timer> timers;
 	std::vector<ccf::Timer> witt_timers;
+	// the timers' events, created once and reused once the timers are read
+	std::vector<hipEvent_t> events;
+	size_t events_used = 0;
+	hipEvent_t next_event()
+	{
+		if (events_used == events.size())
+		{
+			hipEvent_t e = nullptr;
+			if (hipEventCreate(&e) != hipSuccess) return nullptr;
+			events.push_back(e);
+		}
+		return events[events_used++];
+	}
 	int* host_counts = nullptr;   // pinned: active roots per row, read back between sweeps
 	double2* phase = nullptr;     // (cos phi, sin phi) of every phi row
 	bool batch_member = false;    // a field of a ccf_batch: no sweep buffers, seeds on the device

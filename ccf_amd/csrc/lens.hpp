@@ -314,6 +314,55 @@ __device__ inline cd witt_f1(const LensView& L, cd z, const Witt& w)
 	return w.nr.s3 * (-2 * L.theta2) - w.fr.h3 * (L.theta2 * ih3) - d2.conj();
 }
 
+// |dz| / theta below which an Aberth update ends a root (ccf_functions.cuh:384)
+constexpr double CONV_TOL = 1e-9;
+
+// what the Witt evaluation of a listed root decided
+enum RootState
+{
+	ROOT_EARLY = 0,  // the 1/mu early exit (ccf_functions.cuh:207-211)
+	ROOT_FULL = 1,   // the root needs the Cauchy sums and an Aberth update
+	ROOT_NEWTON = 2, // mixed form: ended by a Newton step below CONV_TOL
+};
+
+// Initial root of a phi step (prepare_roots_kernel, ccf_functions.cuh:284-289
+// takes q = 0): the polynomial of degree q through the roots of the last
+// q + 1 steps, evaluated one step on. p points at the root of the previous
+// step and p[-i * back] at the one i steps earlier. The weight of that root
+// is (-1)^i C(q+1, i+1) (exact in the recurrence below); the weights sum to
+// one, so the extrapolation is formed from differences to the previous
+// root, whose rounding scales with the roots' motion rather than with |z|.
+// Above degree 2 the quadratic's value is the guard: where the two differ
+// by more than PREDICT_GUARD times the root's last step, the higher degree
+// is overshooting (a root turning sharply) and the quadratic is taken.
+constexpr double PREDICT_GUARD = 0.1;
+
+__device__ inline double2 predict_root(const double2* p, long long back, int q)
+{
+	const double2 z1 = p[0];
+	double dx = 0, dy = 0;
+	double w = q + 1; // (-1)^i C(q+1, i+1), from i = 0
+	for (int i = 1; i <= q; i++)
+	{
+		w = -w * (q + 1 - i) / (i + 1);
+		const double2 zi = p[-i * back];
+		dx += w * (zi.x - z1.x);
+		dy += w * (zi.y - z1.y);
+	}
+	if (q > 2)
+	{
+		const double2 z2 = p[-back], z3 = p[-2 * back];
+		const double qx = -3 * (z2.x - z1.x) + (z3.x - z1.x), qy = -3 * (z2.y - z1.y) + (z3.y - z1.y);
+		const double ex = dx - qx, ey = dy - qy, mx = z1.x - z2.x, my = z1.y - z2.y;
+		if (ex * ex + ey * ey > PREDICT_GUARD * PREDICT_GUARD * (mx * mx + my * my))
+		{
+			dx = qx;
+			dy = qy;
+		}
+	}
+	return make_double2(z1.x + dx, z1.y + dy);
+}
+
 // 1/mu error bound at a root (ccf_functions.cuh:457-463)
 __device__ inline double err_bound(const Witt& w)
 {

@@ -137,7 +137,8 @@ void usage(const char* name)
 		<< "  --step_iters               Cap of Aberth sweeps per phi step. Default value: 20\n"
 		<< "  --roots_per_thread         Aberth tile roots per thread (1, 2, 4; 0 automatic).\n"
 		<< "  --predictor                Initial roots of each phi step: 0 the previous\n"
-		<< "                             step's roots; 1 linear, 2 quadratic extrapolation.\n"
+		<< "                             step's roots; 1..4 polynomial extrapolation of\n"
+		<< "                             that degree.\n"
 		<< "                             Default value: 0\n"
 		<< "  --root_fmm                 Root sums of dense sweeps by a fast multipole\n"
 		<< "                             method (1) or directly (0). Default value: 0\n"

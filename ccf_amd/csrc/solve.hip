@@ -45,7 +45,7 @@ enum Counter
 {
 	C_FULL = 0,    // updates that formed the Cauchy sums
 	C_EARLY = 1,   // updates that returned on the 1/mu bound
-	C_NEAR = 2,    // near-field stars summed by full updates
+	C_NEAR = 2,    // near-field stars summed by updates that formed f1 (full and Newton)
 	C_CONV = 3,    // roots converged
 	C_ERRBITS = 4, // max 1/mu error (bits of a non-negative double)
 	C_NAN = 5,     // non-finite error seen
@@ -55,6 +55,8 @@ enum Counter
 	C_RROWS = 9,   // root-sum FMM: (row, sweep) pairs summed by the FMM
 	C_RTGT = 10,   // root-sum FMM: updates whose root sums came from the FMM
 	C_RESID = 11,  // max |f0| over the solved points (bits of a non-negative double)
+	C_NEWTON = 12, // mixed form: roots ended by a Newton step below the convergence threshold
+	C_NEAR_EARLY = 13, // near-field stars summed by updates that returned on the 1/mu bound
 	C_COUNT = 16
 };
 
@@ -283,10 +285,12 @@ SweepView view(const SweepBufs& B, int parity)
 // 197-227): the 1/mu early exit, or f1 and the Aberth terms formed per root.
 // MIXED: the mixed-precision form, whose early-exit roots also take one FP64
 // Newton step z - f0/f1 (a final polish; the reference keeps z as it is) and
-// whose pole sum is the near-field sum plus the FP32 far field.
+// whose pole sum is the near-field sum plus the FP32 far field. With newton
+// set (the continuation steps), a root whose Newton step is already below
+// the convergence threshold takes that step and ends without the sums.
 template <bool MIXED>
-__device__ inline bool witt_root(const LensView& L, double2* p, cd z, const Witt& w, int circ_exact, int fmm_pole, cd& f1,
-	cd& ex)
+__device__ inline int witt_root(const LensView& L, double2* p, cd z, const Witt& w, int circ_exact, int fmm_pole, int newton,
+	cd& f1, cd& ex)
 {
 	if (w.early)
 	{
@@ -295,13 +299,22 @@ __device__ inline bool witt_root(const LensView& L, double2* p, cd z, const Witt
 			const cd dz = w.f0 / witt_f1(L, z, w);
 			if (dz.abs() < 1e-6 * L.theta) *p = make_double2(z.re - dz.re, z.im - dz.im);
 		}
-		return false;
+		return ROOT_EARLY;
 	}
 	f1 = witt_f1(L, z, w);
+	if (MIXED && newton)
+	{
+		const cd dz = w.f0 / f1;
+		if (dz.abs() < CONV_TOL * L.theta) // false for a non-finite step
+		{
+			*p = make_double2(z.re - dz.re, z.im - dz.im);
+			return ROOT_NEWTON;
+		}
+	}
 	ex = cd();
 	if (circ_exact) ex = 2.0 * z.inv(); // the disc's pole at 0 (ccf_functions.cuh:223-227)
 	if (MIXED && fmm_pole) ex += 2.0 * (w.nr.s0 + far_pole32(L, w.node, z));
-	return true;
+	return ROOT_FULL;
 }
 
 // K1 (ccf_functions.cuh:197-213 and the early return at :207-211). A root
@@ -321,7 +334,7 @@ __global__ __launch_bounds__(BLOCK) void k_witt(LensView L, double2* __restrict_
 	}
 	const int n = V.cnt[r];
 	const int i = blockIdx.x * BLOCK + threadIdx.x;
-	unsigned long long nfull = 0, nearly = 0, nnear = 0;
+	unsigned long long nfull = 0, nearly = 0, nnewton = 0, nnear = 0, nnear_early = 0;
 	bool full = false, deferred = false;
 	int a = 0;
 	cd f0, f1, ex;
@@ -343,17 +356,14 @@ __global__ __launch_bounds__(BLOCK) void k_witt(LensView L, double2* __restrict_
 			double cp, sp;
 			M.phase(row, phi, cp, sp);
 			const Witt w = witt_from(L, z, cp, sp, node, p2p(L, node, z));
-			full = witt_root<MIXED>(L, p, z, w, circ_exact, fmm_pole, f1, ex);
-			if (full)
-			{
-				f0 = w.f0;
-				nfull = 1;
-				nnear = w.nr.count;
-			}
-			else
-			{
-				nearly = 1;
-			}
+			const int st = witt_root<MIXED>(L, p, z, w, circ_exact, fmm_pole, M.j > 0, f1, ex);
+			full = st == ROOT_FULL;
+			if (full) f0 = w.f0;
+			nfull = full;
+			nearly = st == ROOT_EARLY;
+			nnewton = st == ROOT_NEWTON;
+			if (st != ROOT_EARLY) nnear = w.nr.count;
+			else nnear_early = w.nr.count;
 		}
 	}
 	const int dpos = block_append(&V.dcnt[r], deferred);
@@ -368,8 +378,8 @@ __global__ __launch_bounds__(BLOCK) void k_witt(LensView L, double2* __restrict_
 		V.f1[k] = make_double2(f1.re, f1.im);
 		V.ex[k] = make_double2(ex.re, ex.im);
 	}
-	const int which[4] = {C_FULL, C_EARLY, C_CONV, C_NEAR};
-	unsigned long long v[4] = {nfull, nearly, nearly, nnear};
+	const int which[6] = {C_FULL, C_EARLY, C_CONV, C_NEAR, C_NEWTON, C_NEAR_EARLY};
+	unsigned long long v[6] = {nfull, nearly, nearly + nnewton, nnear, nnewton, nnear_early};
 	block_stats(stats, which, v);
 }
 
@@ -435,7 +445,8 @@ __global__ __launch_bounds__(BLOCK) void k_witt_far(LensView L, double2* __restr
 			M.phase(row, phi, cp, sp);
 			const Witt w = witt_from(L, z, cp, sp, node, sum);
 			cd f1, ex;
-			if (witt_root<MIXED>(L, p, z, w, circ_exact, fmm_pole, f1, ex))
+			const int st = witt_root<MIXED>(L, p, z, w, circ_exact, fmm_pole, M.j > 0, f1, ex);
+			if (st == ROOT_FULL)
 			{
 				const int pos = atomicAdd(&V.fcnt[r], 1);
 				const size_t k = (size_t)r * R + pos;
@@ -449,8 +460,9 @@ __global__ __launch_bounds__(BLOCK) void k_witt_far(LensView L, double2* __restr
 			}
 			else
 			{
-				atomicAdd(stat_slot(stats, C_EARLY), 1ull);
+				atomicAdd(stat_slot(stats, st == ROOT_NEWTON ? C_NEWTON : C_EARLY), 1ull);
 				atomicAdd(stat_slot(stats, C_CONV), 1ull);
+				atomicAdd(stat_slot(stats, st == ROOT_NEWTON ? C_NEAR : C_NEAR_EARLY), (unsigned long long)sum.count);
 			}
 		}
 		__syncthreads();
@@ -608,6 +620,114 @@ __global__ __launch_bounds__(BLOCK) void k_sums64(const double2* __restrict__ sp
 }
 
 // K3 in FP32 with the FP64 fallback for clustered roots
+// The items of tiles with few listed roots (at most SPARSE_TARGETS), in a
+// kernel of their own (k_sums32 skips them) so that neither path's registers
+// limit the other's occupancy: the lanes run over the slice's sources and
+// groups of SPARSE_GROUP targets are held uniform across the workgroup, so
+// no lane idles on an empty target slot. Each target's partial sum is
+// reduced over the lanes in a fixed order and lands where the tile's own
+// lane would have written it (part[i] for target i of the tile). A root
+// within the cluster distance has its one term formed in FP64; the target's
+// own index is skipped by index, not by distance.
+constexpr int SPARSE_TARGETS = 32;
+constexpr int SPARSE_GROUP = 4;
+
+__device__ inline void sums32_sparse(const float2* __restrict__ spos32, int Np, const double2* __restrict__ xrow,
+	const int* __restrict__ fl, int nt, long long lo, long long hi, float cluster_d2, double2* __restrict__ part,
+	unsigned long long& nclus)
+{
+	__shared__ double red[BLOCK / 64][SPARSE_GROUP][2];
+	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+	const int s1 = (int)min(hi, (long long)Np);
+	const int r0 = (int)(max(lo, (long long)Np) - Np), r1 = (int)(hi - Np);
+	for (int g0 = 0; g0 < nt; g0 += SPARSE_GROUP)
+	{
+		int ia[SPARSE_GROUP];
+		double cx[SPARSE_GROUP], cy[SPARSE_GROUP];
+		float fx[SPARSE_GROUP], fy[SPARSE_GROUP], ax[SPARSE_GROUP], ay[SPARSE_GROUP], bx[SPARSE_GROUP], by[SPARSE_GROUP];
+#pragma unroll
+		for (int g = 0; g < SPARSE_GROUP; g++)
+		{
+			// slots past the last target repeat it; their sums are not written
+			ia[g] = fl[min(g0 + g, nt - 1)];
+			const double2 v = xrow[ia[g]];
+			fx[g] = (float)v.x;
+			fy[g] = (float)v.y;
+			ax[g] = ay[g] = bx[g] = by[g] = 0;
+			cx[g] = cy[g] = 0;
+		}
+#pragma unroll 1
+		for (int q = (int)lo + threadIdx.x; q < s1; q += BLOCK)
+		{
+			const float2 w = spos32[q];
+#pragma unroll
+			for (int g = 0; g < SPARSE_GROUP; g++)
+			{
+				const float dx = fx[g] - w.x, dy = fy[g] - w.y;
+				const float iv = __builtin_amdgcn_rcpf(fmaf(dx, dx, dy * dy));
+				ax[g] = fmaf(dx, iv, ax[g]);
+				ay[g] = fmaf(dy, iv, ay[g]);
+			}
+		}
+#pragma unroll 1
+		for (int q = r0 + threadIdx.x; q < r1; q += BLOCK)
+		{
+			const double2 wd = xrow[q];
+			const float wx = (float)wd.x, wy = (float)wd.y;
+#pragma unroll
+			for (int g = 0; g < SPARSE_GROUP; g++)
+			{
+				const float dx = fx[g] - wx, dy = fy[g] - wy;
+				const float d2 = fmaf(dx, dx, dy * dy);
+				if (d2 < cluster_d2 && q != ia[g])
+				{
+					const double2 z = xrow[ia[g]]; // rare: the target in FP64 again
+					const double ex = z.x - wd.x, ey = z.y - wd.y;
+					const double iv = rcp64(fmax(fma(ex, ex, ey * ey), 1e-300));
+					cx[g] = fma(ex, iv, cx[g]);
+					cy[g] = fma(ey, iv, cy[g]);
+					nclus++;
+				}
+				else
+				{
+					// the own root gives dx = dy = 0 and, floored, a zero term
+					const float iv = __builtin_amdgcn_rcpf(fmaxf(d2, 1e-30f));
+					bx[g] = fmaf(dx, iv, bx[g]);
+					by[g] = fmaf(dy, iv, by[g]);
+				}
+			}
+		}
+#pragma unroll
+		for (int g = 0; g < SPARSE_GROUP; g++)
+		{
+			double vx = 2.0 * (double)ax[g] - ((double)bx[g] + cx[g]);
+			double vy = 2.0 * (double)ay[g] - ((double)by[g] + cy[g]);
+			for (int o = 32; o >= 1; o >>= 1)
+			{
+				vx += __shfl_xor(vx, o, 64);
+				vy += __shfl_xor(vy, o, 64);
+			}
+			if (lane == 0)
+			{
+				red[wv][g][0] = vx;
+				red[wv][g][1] = vy;
+			}
+		}
+		__syncthreads();
+		if (threadIdx.x < SPARSE_GROUP && g0 + (int)threadIdx.x < nt)
+		{
+			double vx = 0, vy = 0;
+			for (int w = 0; w < BLOCK / 64; w++)
+			{
+				vx += red[w][threadIdx.x][0];
+				vy += red[w][threadIdx.x][1];
+			}
+			part[g0 + threadIdx.x] = make_double2(vx, -vy);
+		}
+		__syncthreads();
+	}
+}
+
 template <int RPT>
 __global__ __launch_bounds__(BLOCK) void k_sums32(const float2* __restrict__ spos32, int Np, const double2* __restrict__ roots,
 	RowMap M, SweepView V, int nrows, float cluster_d2, unsigned long long* __restrict__ stats)
@@ -628,13 +748,16 @@ __global__ __launch_bounds__(BLOCK) void k_sums32(const float2* __restrict__ spo
 		double phi;
 		M.get(r, row, phi);
 		const double2* __restrict__ xrow = roots + (size_t)row * R;
+		if (nfr - t * T <= SPARSE_TARGETS) continue; // k_sums32_sparse's item
 		double zx[RPT], zy[RPT], bx[RPT], by[RPT];
 		float fx[RPT], fy[RPT], ax[RPT], ay[RPT];
+		int ia[RPT];
 #pragma unroll
 		for (int k = 0; k < RPT; k++)
 		{
 			const int i = t * T + k * BLOCK + threadIdx.x;
 			const int a = V.fl[(size_t)r * R + (i < nfr ? i : t * T)];
+			ia[k] = a;
 			const double2 v = xrow[a];
 			zx[k] = v.x;
 			zy[k] = v.y;
@@ -690,8 +813,9 @@ __global__ __launch_bounds__(BLOCK) void k_sums32(const float2* __restrict__ spo
 					const float iv = __builtin_amdgcn_rcpf(d2);
 					cx[k] = fmaf(dx, iv, cx[k]);
 					cy[k] = fmaf(dy, iv, cy[k]);
-					// the own root (d2 at the floor) is not a neighbour
-					dmin[k] = d2 > 1e-30f ? fminf(dmin[k], d2) : dmin[k];
+					// every root but the own one counts as a neighbour, also
+					// one at the same FP32 position
+					dmin[k] = c0 + q != ia[k] ? fminf(dmin[k], d2) : dmin[k];
 				}
 			}
 #pragma unroll
@@ -725,6 +849,32 @@ __global__ __launch_bounds__(BLOCK) void k_sums32(const float2* __restrict__ spo
 		for (int k = 0; k < RPT; k++)
 			V.part[(size_t)item * T + k * BLOCK + threadIdx.x] =
 				make_double2(2.0 * ax[k] - bx[k], -(2.0 * ay[k] - by[k]));
+	}
+	nclus = wave_sum(nclus);
+	if ((threadIdx.x & 63) == 0 && nclus) atomicAdd(stat_slot(stats, C_CLUSTER), nclus);
+}
+
+template <int RPT>
+__global__ __launch_bounds__(BLOCK) void k_sums32_sparse(const float2* __restrict__ spos32, int Np, const double2* __restrict__ roots,
+	RowMap M, SweepView V, int nrows, float cluster_d2, unsigned long long* __restrict__ stats)
+{
+	const int tiles = V.plan[0], S = V.plan[1];
+	const long long items = (long long)tiles * S;
+	const int R = M.R, T = BLOCK * RPT;
+	const long long Lt = (long long)Np + R;
+	unsigned long long nclus = 0;
+	for (long long item = blockIdx.x; item < items; item += gridDim.x)
+	{
+		const int g = (int)(item / S), s = (int)(item % S);
+		const int r = row_of_tile(V.tile_off, nrows, g);
+		const int t = g - V.tile_off[r];
+		const int nt = V.fcnt[r] - t * T;
+		if (nt > SPARSE_TARGETS) continue; // k_sums32's item
+		int row;
+		double phi;
+		M.get(r, row, phi);
+		sums32_sparse(spos32, Np, roots + (size_t)row * R, V.fl + (size_t)r * R + (size_t)t * T, nt, s * Lt / S,
+			(s + 1) * Lt / S, cluster_d2, V.part + (size_t)item * T, nclus);
 	}
 	nclus = wave_sum(nclus);
 	if ((threadIdx.x & 63) == 0 && nclus) atomicAdd(stat_slot(stats, C_CLUSTER), nclus);
@@ -773,7 +923,7 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(double theta, double2* __res
 		const cd f0(a0.x, a0.y), f1(a1.x, a1.y);
 		const cd zn = z - f0 / (f1 + f0 * sum);
 		*p = make_double2(zn.re, zn.im);
-		if ((zn - z).abs() / theta < 1e-9) nc = 1;
+		if ((zn - z).abs() / theta < CONV_TOL) nc = 1;
 		else again = true;
 	}
 	const int pos = block_append(&V.cnt_next[r], again);
@@ -800,22 +950,7 @@ __global__ void k_prepare(double2* roots, RowMap M, SweepBufs B, int order, cons
 		{
 			const int sgn = (r % M.rows_per_branch) == 0 ? -1 : 1;
 			const int q = min(order, M.j - 1); // previous steps beyond j - 1 that exist
-			const double2 z1 = roots[(size_t)(row - sgn) * R + a];
-			double2 z = z1;
-			if (q >= 1)
-			{
-				const double2 z2 = roots[(size_t)(row - 2 * sgn) * R + a];
-				if (q >= 2)
-				{
-					const double2 z3 = roots[(size_t)(row - 3 * sgn) * R + a];
-					z = make_double2(3 * (z1.x - z2.x) + z3.x, 3 * (z1.y - z2.y) + z3.y);
-				}
-				else
-				{
-					z = make_double2(2 * z1.x - z2.x, 2 * z1.y - z2.y);
-				}
-			}
-			roots[(size_t)row * R + a] = z;
+			roots[(size_t)row * R + a] = predict_root(roots + (size_t)(row - sgn) * R + a, (long long)sgn * R, q);
 		}
 		B.act[(size_t)r * R + a] = perm ? perm[a] : a;
 	}
@@ -971,12 +1106,13 @@ SumsLaunch sums_launch(ccf_ctx* ctx, bool f32)
 // `cap` sweeps over the rows of M (ccf.cuh:929-939 and 1013-1018): the
 // sweep kernels exit at once when no root of a row is active any more
 // whether to read the active counts back after `done` of `cap` sweeps: after
-// sweeps 2 and 3 (a phi step with a continuation predictor mostly ends
-// there), then every 4th
+// each of sweeps 3 to 6 (a phi step ends there: after three with a
+// continuation predictor, five or six without), then every 4th. A read-back
+// waits for the device, about as long as a few sweeps of stragglers take.
 inline bool sweep_check(int done, int cap)
 {
 	if (done >= cap) return false;
-	return done == 2 || done == 3 || done % 4 == 0;
+	return (done >= 3 && done <= 6) || done % 4 == 0;
 }
 
 // The first fmm_sweeps sweeps may take the root sums of their dense rows
@@ -1004,9 +1140,9 @@ bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int
 		ccf::Timer tw;
 		if (ctx->P.timing)
 		{
-			(void)hipEventCreate(&tw.a);
-			(void)hipEventCreate(&tw.b);
-			(void)hipEventRecord(tw.a, s);
+			tw.a = ctx->next_event();
+			tw.b = ctx->next_event();
+			if (tw.a) (void)hipEventRecord(tw.a, s);
 		}
 		if (f32)
 		{
@@ -1018,7 +1154,7 @@ bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int
 			k_witt<false><<<g_rows, BLOCK, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, 0);
 			k_witt_far<false><<<g_far, BLOCK, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, 0);
 		}
-		if (ctx->P.timing)
+		if (ctx->P.timing && tw.a && tw.b)
 		{
 			(void)hipEventRecord(tw.b, s);
 			ctx->witt_timers.push_back(tw);
@@ -1028,9 +1164,9 @@ bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int
 		ccf::Timer tm;
 		if (ctx->P.timing)
 		{
-			(void)hipEventCreate(&tm.a);
-			(void)hipEventCreate(&tm.b);
-			(void)hipEventRecord(tm.a, s);
+			tm.a = ctx->next_event();
+			tm.b = ctx->next_event();
+			if (tm.a) (void)hipEventRecord(tm.a, s);
 		}
 		if (fmm) root_fmm_pass(B, roots, M, nrows, ctx->counters, s);
 		if (f32)
@@ -1040,6 +1176,9 @@ bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int
 			if (sl.rpt == 1) k_sums32<1><<<sl.grid, BLOCK, 0, s>>>(ctx->spos32, Np, roots, M, V, nrows, cd2, ctx->counters);
 			else if (sl.rpt == 4) k_sums32<4><<<sl.grid, BLOCK, 0, s>>>(ctx->spos32, Np, roots, M, V, nrows, cd2, ctx->counters);
 			else k_sums32<2><<<sl.grid, BLOCK, 0, s>>>(ctx->spos32, Np, roots, M, V, nrows, cd2, ctx->counters);
+			if (sl.rpt == 1) k_sums32_sparse<1><<<sl.grid, BLOCK, 0, s>>>(ctx->spos32, Np, roots, M, V, nrows, cd2, ctx->counters);
+			else if (sl.rpt == 4) k_sums32_sparse<4><<<sl.grid, BLOCK, 0, s>>>(ctx->spos32, Np, roots, M, V, nrows, cd2, ctx->counters);
+			else k_sums32_sparse<2><<<sl.grid, BLOCK, 0, s>>>(ctx->spos32, Np, roots, M, V, nrows, cd2, ctx->counters);
 		}
 		else
 		{
@@ -1047,7 +1186,7 @@ bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int
 			else if (sl.rpt == 4) k_sums64<4><<<sl.grid, BLOCK, 0, s>>>(ctx->spos, Np, roots, M, V, nrows);
 			else k_sums64<2><<<sl.grid, BLOCK, 0, s>>>(ctx->spos, Np, roots, M, V, nrows);
 		}
-		if (ctx->P.timing)
+		if (ctx->P.timing && tm.a && tm.b)
 		{
 			(void)hipEventRecord(tm.b, s);
 			ctx->timers.push_back(tm);
@@ -1264,11 +1403,15 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 	ctx->info.points_converged = (long long)c[C_CONV];
 	ctx->info.updates_full = (long long)c[C_FULL];
 	ctx->info.updates_early = (long long)c[C_EARLY];
+	ctx->info.updates_newton = (long long)c[C_NEWTON];
 	ctx->info.cluster_chunks = (long long)c[C_CLUSTER];
 	{
-		// algorithmic flops: 10 per Cauchy term, 36 per near-field star,
-		// 16 per expansion order (the Horner sums of f0 and f1), 8 per
-		// rectangular smooth-sheet term; the FMM pole sum adds 16 per order
+		// algorithmic flops: 10 per Cauchy term of the sums; per Witt
+		// evaluation 36 per near-field star and 16 per expansion order when
+		// f0 and f1 are formed (full updates, Newton-ended roots and, in the
+		// mixed form, the polished early exits), 18 and 8 for f0 alone (the
+		// FP64 form's early exits), 8 per rectangular smooth-sheet term; the
+		// FMM pole sum of a full update adds 8 per order
 		const bool fmm_pole = ctx->P.precision == 1 && ctx->tree.order > 0;
 		const double Np = fmm_pole ? 0.0 : ctx->num_stars + ((!ctx->rectangular && !ctx->P.approx) ? 1 : 0);
 		const double p = ctx->tree.order;
@@ -1287,8 +1430,10 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 		ctx->info.root_fmm_rows = (long long)c[C_RROWS];
 		ctx->info.root_fmm_order = g.order;
 		ctx->info.root_fmm_leaf_level = g.leaf;
-		ctx->info.flops_update = ctx->info.flops_sums + full * (16.0 * p + sm + (fmm_pole ? 16.0 * p : 0.0)) +
-			(36.0 + (fmm_pole ? 8.0 : 0.0)) * (double)c[C_NEAR] + (double)c[C_EARLY] * 8.0 * p;
+		const double ef = ctx->P.precision == 1 ? 1.0 : 0.5; // share of f0 + f1 in an early exit
+		const double early = (double)c[C_EARLY];
+		ctx->info.flops_update = ctx->info.flops_sums + 36.0 * (double)c[C_NEAR] + (full + (double)c[C_NEWTON]) * (16.0 * p + sm) +
+			ef * (36.0 * (double)c[C_NEAR_EARLY] + early * 16.0 * p) + early * sm + (fmm_pole ? full * 8.0 * p : 0.0);
 	}
 
 	// transpose to file order (ccf.cuh:1072-1080)
@@ -1304,8 +1449,6 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 			float x = 0;
 			(void)hipEventElapsedTime(&x, t.a, t.b);
 			ms += x;
-			(void)hipEventDestroy(t.a);
-			(void)hipEventDestroy(t.b);
 		}
 		ctx->info.aberth_kernel_ms = ms;
 		ctx->info.aberth_launches = (long long)ctx->timers.size();
@@ -1316,12 +1459,11 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 			float x = 0;
 			(void)hipEventElapsedTime(&x, t.a, t.b);
 			mw += x;
-			(void)hipEventDestroy(t.a);
-			(void)hipEventDestroy(t.b);
 		}
 		ctx->info.witt_kernel_ms = mw;
 		ctx->info.witt_launches = (long long)ctx->witt_timers.size();
 		ctx->witt_timers.clear();
+		ctx->events_used = 0;
 	}
 	ctx->info.flops_witt = ctx->info.flops_update - ctx->info.flops_sums;
 	return true;

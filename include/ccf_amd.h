@@ -60,8 +60,8 @@ typedef struct ccf_params
 	int roots_per_thread;/* Aberth tile: roots per thread (0 -> autotuned default) */
 	int timing;          /* 1: time every Aberth launch with HIP events (see ccf_info) */
 	int predictor;       /* initial roots of phi step j: 0 the roots of step j-1 (the reference,
-	                        ccf_functions.cuh:284-289); 1 linear, 2 quadratic extrapolation
-	                        from steps j-1, j-2 (, j-3) */
+	                        ccf_functions.cuh:284-289); q = 1..4 the polynomial of degree q
+	                        through steps j-1 .. j-1-q (fewer near the branch centre) */
 	int root_fmm;        /* 1: the root sum sum_{j!=k} 1/(z_k - z_j) of sweeps in which many roots
 	                        are active by a fast multipole method over a uniform grid of the field
 	                        (exact FP64 near field, far field to order root_fmm_order); 0: direct
@@ -109,6 +109,8 @@ typedef struct ccf_info
 	double witt_kernel_ms;     /* sum of timed Witt-evaluation launches (params.timing = 1) */
 	long long witt_launches;
 	double flops_witt;         /* algorithmic flops of the Witt evaluations (flops_update - flops_sums) */
+	long long updates_newton;  /* precision 1, continuation steps: roots whose FP64 Newton step z - f0/f1
+	                              was already below 1e-9 theta took it and ended without the sums */
 } ccf_info;
 
 /* Arrays readable with ccf_get_array. */

@@ -105,14 +105,15 @@ def test_batch_matches_single_field_path(workdir):
         assert nbad == 0, (i, worst, nbad)
 
 
-def test_mixed_batch_with_predictor_within_band(workdir):
-    """precision 1 and the quadratic predictor change the path to each root:
+@pytest.mark.parametrize("predictor", [2, 4])
+def test_mixed_batch_with_predictor_within_band(workdir, predictor):
+    """precision 1 and a continuation predictor change the path to each root:
     roots sit within the oracle's convergence band, caustics are the lens
     map of the product's own critical points"""
-    b, files = _run_batch(workdir, "bm", precision=1, predictor=2)
+    b, files = _run_batch(workdir, f"bm{predictor}", precision=1, predictor=predictor)
     for i, (sf, spec) in enumerate(files):
         gi = b.info(i)
-        op = os.path.join(str(workdir), f"bmo{i}_")
+        op = os.path.join(str(workdir), f"bmo{predictor}_{i}_")
         ost = oracle_run(sf, op, fmm_order=gi["expansion_order"], **_field_kw(spec), **RUN)
         R, nb = ost["num_roots"], RUN["num_branches"]
         g = b.array(i, ccf_amd.ARRAY_CCS)

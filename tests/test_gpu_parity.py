@@ -82,6 +82,31 @@ def test_aberth_sweep_matches_oracle(workdir, name, kw):
     assert rel.max() < 1e-11
 
 
+@pytest.mark.parametrize("count", [20, None])
+def test_mixed_sweep_step_matches_oracle(workdir, count):
+    """precision 1, direct star sums: the Aberth step of one sweep with the
+    Cauchy sums in FP32 (the sparse kernel for a row of at most 32 listed
+    roots, the tiled one above) against the oracle's FP64 step. The sums only
+    scale the step, so the two steps agree to FP32 rounding of S relative to
+    the step's length; a wrong term, sign or index changes it by order one."""
+    sf = make_field(workdir, "cfg1")
+    n, rect, corner, theta, st = io.read_stars(sf)
+    roots = np.concatenate([st[:, 0] + 1 + 1j * st[:, 1], st[:, 0] - 1 + 1j * st[:, 1],
+                            1.05 * abs(corner) * np.exp(2j * np.pi * (np.arange(7) + 0.5) / 7)])
+    if count:
+        roots = roots[np.random.default_rng(7).choice(roots.size, count, replace=False)]
+    phi = np.pi / 3
+    kw = dict(kappa_tot=0.3, shear=0.1, direct=1, num_phi=10)
+    with _ctx(sf, str(workdir) + "/", precision=1, **kw) as c:
+        c.prepare()
+        g = c.aberth_sweep(roots, phi)
+    o = oracle.aberth_sweep(oracle.params(starfile=sf, **kw), roots, phi)
+    step = np.abs(o - roots)
+    assert step.min() > 1e-6  # every root took a full update (none at the 1/mu exit)
+    assert np.all(np.abs(g - o) <= 1e-3 * step + 1e-12 * np.maximum(np.abs(o), 1.0)), \
+        np.max(np.abs(g - o) / step)
+
+
 def _direct_root_sums(z):
     """sum_{j != k} 1/(z_k - z_j) in FP64, blockwise, and the scale of each
     sum, sum_{j != k} 1/|z_k - z_j| (the far-field error bound's unit)."""
@@ -147,6 +172,10 @@ MIXED = [
     ("salpeter300", dict(kappa_tot=0.45, shear=0.35, num_phi=120, num_branches=3, fmm_order=20),
      dict(precision=1, predictor=2, root_fmm=1)),
     ("rect200", dict(kappa_tot=0.7, shear=0.25, num_phi=80, num_branches=2), dict(precision=1, predictor=2, root_fmm=1)),
+    # higher-degree predictors (guarded by the quadratic) with Newton-ended roots
+    ("salpeter300", dict(kappa_tot=0.45, shear=0.35, num_phi=120, num_branches=3, fmm_order=20),
+     dict(precision=1, predictor=4, root_fmm=1)),
+    ("rect200", dict(kappa_tot=0.7, shear=0.25, num_phi=80, num_branches=2), dict(precision=1, predictor=3)),
     ("smooth150", dict(kappa_tot=0.6, shear=0.2, num_phi=60, num_branches=1, approx=0), dict(precision=1, root_fmm=1)),
 ]
 
