@@ -332,7 +332,7 @@ def bench_batch(args):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="cc10k", choices=sorted(WORKLOADS) + ["batch256"])
     ap.add_argument("--seq-fields", type=int, default=64,
@@ -416,8 +416,8 @@ def main():
         # the two event-timed regions of a sweep: the root sums (FMM pass and
         # direct sums) and the Witt evaluation (k_witt + k_witt_far)
         if which == "sums":
-            name = ("root sums: FMM pass (k_r*) + k_sums32" if r.get("root_fmm") else
-                    ("k_sums32" if r is res.get("mixed") else "k_sums64"))
+            name = ("root sums: FMM pass (k_r*) + k_sums32 + k_sums32_sparse" if r.get("root_fmm") else
+                    ("k_sums32 + k_sums32_sparse" if r is res.get("mixed") else "k_sums64"))
             return dict(kernel=name, achieved=r["sums_tflops"], total_ms=r["aberth_ms_total"],
                         avg_launch_ms=r["aberth_ms_avg"], launches=r["aberth_launches"])
         return dict(kernel="k_witt + k_witt_far", achieved=r["witt_tflops"], total_ms=r["witt_ms_total"],

@@ -137,7 +137,7 @@ struct SweepBufs
 	double2* f1 = nullptr;   // [rows][R] its z-derivative
 	double2* ex = nullptr;   // [rows][R] terms of the Aberth sum formed per root
 	int* tile_off = nullptr; // [rows + 1] first tile of each row
-	int* plan = nullptr;     // [4] tiles, slices per tile
+	int* plan = nullptr;     // [4] tiles, slices per tile; rows summed by the root-sum FMM in a phase's first and second sweep
 	double2* part = nullptr; // [max_items][tile] partial Cauchy sums
 	// root-sum FMM (grid.order > 0 only)
 	RootGrid grid;
@@ -222,7 +222,8 @@ struct ccf_ctx
 		}
 		return events[events_used++];
 	}
-	int* host_counts = nullptr;   // pinned: active roots per row, read back between sweeps
+	int* host_counts = nullptr;   // pinned: active roots per row, then the two FMM row counts, read back between sweeps
+	int fmm_rows_sweep[2] = {-1, -1}; // root-sum FMM rows of the last phase's first and second sweep (-1: unknown)
 	double2* phase = nullptr;     // (cos phi, sin phi) of every phi row
 	bool batch_member = false;    // a field of a ccf_batch: no sweep buffers, seeds on the device
 	bool own_stream = true;       // ccf_destroy destroys the stream

@@ -314,6 +314,39 @@ __device__ inline cd witt_f1(const LensView& L, cd z, const Witt& w)
 	return w.nr.s3 * (-2 * L.theta2) - w.fr.h3 * (L.theta2 * ih3) - d2.conj();
 }
 
+// The device hands a launch's workgroups to its XCDs round-robin by linear
+// id (y * gridDim.x + x), and every XCD has an L2 of its own. Kernels that
+// walk a row's roots in Morton order along x place their blocks with
+// xcd_slot / xcd_x: of the `used` blocks of row y that have work, XCD k
+// takes the k-th eighth, so that one L2 holds the tree data of about an
+// eighth of the field instead of all of it, and a row with few active roots
+// still spreads over every XCD. Only locality depends on the dispatch order;
+// each of the used blocks is taken exactly once. gridDim.x must be a
+// multiple of NUM_XCD and at least the largest `used`.
+constexpr int NUM_XCD = 8;
+
+inline int xcd_round(int nx) { return (nx + NUM_XCD - 1) / NUM_XCD * NUM_XCD; }
+
+// .x: the block's slot among its XCD's blocks of the row, .y: the row
+__device__ inline int2 xcd_slot()
+{
+	const int per = gridDim.x / NUM_XCD;
+	const int i = (blockIdx.y * gridDim.x + blockIdx.x) / NUM_XCD;
+	return make_int2(i % per, i / per);
+}
+
+// whether this block is the row's first on XCD 0 (one block per row)
+__device__ inline bool xcd_row_leader(int2 slot) { return slot.x == 0 && (blockIdx.y * gridDim.x + blockIdx.x) % NUM_XCD == 0; }
+
+// the x block of slot j when the row has `used` blocks with work; `used`
+// (no work) for the slots beyond them
+__device__ inline int xcd_x(int j, int used)
+{
+	const int pu = (used + NUM_XCD - 1) / NUM_XCD;
+	const int x = (int)((blockIdx.y * gridDim.x + blockIdx.x) % NUM_XCD) * pu + j;
+	return j < pu && x < used ? x : used;
+}
+
 // |dz| / theta below which an Aberth update ends a root (ccf_functions.cuh:384)
 constexpr double CONV_TOL = 1e-9;
 

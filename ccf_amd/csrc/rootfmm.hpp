@@ -42,6 +42,7 @@ constexpr int ROOT_FMM_MIN_ROOTS = 64;     // smaller rows are always summed dir
 // converges over a few dozen sweeps; a continuation step mostly in one or two)
 constexpr int ROOT_FMM_INIT_SWEEPS = 24;
 constexpr int ROOT_FMM_STEP_SWEEPS = 2;
+constexpr int ROOT_FMM_PROBE = 8;
 
 // listed roots from which a row's sums are cheaper by the FMM (its build
 // costs about as much as R/32 direct updates)

@@ -324,16 +324,18 @@ template <bool MIXED>
 __global__ __launch_bounds__(BLOCK) void k_witt(LensView L, double2* __restrict__ roots, RowMap M, SweepView V,
 	unsigned long long* __restrict__ stats, int circ_exact, int fmm_pole)
 {
-	const int r = blockIdx.y;
+	const int2 sl = xcd_slot();
+	const int r = sl.y;
 	const int R = M.R;
-	if (blockIdx.x == 0 && threadIdx.x == 0)
+	if (xcd_row_leader(sl) && threadIdx.x == 0)
 	{
 		V.cnt_next[r] = 0;
 		V.fcnt_next[r] = 0;
 		V.dcnt_next[r] = 0;
 	}
 	const int n = V.cnt[r];
-	const int i = blockIdx.x * BLOCK + threadIdx.x;
+	const int i = xcd_x(sl.x, (n + BLOCK - 1) / BLOCK) * BLOCK + threadIdx.x;
+	if (i - (int)threadIdx.x >= n) return; // a block without listed roots (uniform)
 	unsigned long long nfull = 0, nearly = 0, nnewton = 0, nnear = 0, nnear_early = 0;
 	bool full = false, deferred = false;
 	int a = 0;
@@ -472,18 +474,19 @@ __global__ __launch_bounds__(BLOCK) void k_witt_far(LensView L, double2* __restr
 // K2: tiles per row (exclusive prefix in tile_off) and slices per tile; a
 // row with at least fmm_min listed roots (fmm_min > 0) takes its root sums
 // from the FMM pass instead and gets no tiles
-__global__ void k_plan(SweepView V, int nrows, int terms, int target, int fmm_min)
+__global__ void k_plan(SweepView V, int nrows, int terms, int target, int fmm_min, int sweep)
 {
 	__shared__ int wsum[16];
-	__shared__ int carry;
+	__shared__ int carry, nfmm;
 	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-	if (threadIdx.x == 0) carry = 0;
+	if (threadIdx.x == 0) carry = nfmm = 0;
 	__syncthreads();
 	for (int base = 0; base < nrows; base += blockDim.x)
 	{
 		const int r = base + threadIdx.x;
 		const bool fmm = r < nrows && fmm_min > 0 && V.fcnt[r] >= fmm_min;
 		if (r < nrows && V.rmode) V.rmode[r] = fmm ? 1 : 0;
+		if (fmm) atomicAdd(&nfmm, 1);
 		const int v = (r < nrows && !fmm) ? (V.fcnt[r] + V.tile - 1) / V.tile : 0;
 		int x = v;
 		for (int d = 1; d < 64; d <<= 1)
@@ -517,6 +520,8 @@ __global__ void k_plan(SweepView V, int nrows, int terms, int target, int fmm_mi
 		V.tile_off[nrows] = tiles;
 		V.plan[0] = tiles;
 		V.plan[1] = S;
+		// FMM rows of the phase's first and second sweep (cleared by k_prepare)
+		if (sweep < 2 && nfmm) V.plan[2 + sweep] += nfmm;
 	}
 }
 
@@ -888,6 +893,7 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(double theta, double2* __res
 	const int R = M.R;
 	const int i = blockIdx.x * BLOCK + threadIdx.x;
 	const int n = V.fcnt[r];
+	if (blockIdx.x * BLOCK >= n) return; // a block without listed roots (uniform)
 	unsigned long long nc = 0;
 	bool again = false;
 	int a = 0;
@@ -962,24 +968,30 @@ __global__ void k_prepare(double2* roots, RowMap M, SweepBufs B, int order, cons
 		B.fcnt[B.rows + r] = 0;
 		B.dcnt[r] = 0;
 		B.dcnt[B.rows + r] = 0;
+		if (r == 0)
+		{
+			B.plan[2] = 0;
+			B.plan[3] = 0;
+		}
 	}
 }
 
 // 1/mu error bound at every point of the solved rows, reduced to its max
 // with a non-finite flag (find_errors_kernel + has_nan_err_kernel +
 // thrust::max_element, ccf_functions.cuh:419-489)
-__global__ void k_errors(LensView L, const double2* __restrict__ roots, int R, int np, int nb, const double2* __restrict__ cs,
+__global__ void k_errors(LensView L, const double2* __restrict__ roots, int R, const double2* __restrict__ cs,
 	const int* __restrict__ perm, unsigned long long* stats)
 {
-	const long long total = (long long)(np + nb) * R;
-	const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-	// points of a row in the roots' processing order
-	const long long idx = t < total ? (t / R) * R + perm[t % R] : t;
+	// grid: (xcd_round(blocks per row), rows); a row's points in the roots'
+	// processing order
+	const int2 sl = xcd_slot();
+	const int row = sl.y;
+	const int k = xcd_x(sl.x, (R + BLOCK - 1) / BLOCK) * BLOCK + threadIdx.x;
+	const long long idx = (long long)row * R + (k < R ? perm[k] : 0);
 	double e = 0, res = 0;
 	unsigned long long bad = 0;
-	if (idx < total)
+	if (k < R)
 	{
-		const int row = (int)(idx / R);
 		const double2 t = cs[row];
 		const double2 v = roots[idx];
 		const cd z(v.x, v.y);
@@ -1026,7 +1038,11 @@ __global__ void k_transpose(const double2* __restrict__ in, int rows, int R, dou
 // find_mu_length_scales_kernel; ccf_functions.cuh:510-563)
 __global__ void k_caustics(LensView L, const double2* __restrict__ z, long long n, double2* w, double* mu)
 {
-	const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	// file order is root-major and roots are seeded in the stars' tree order:
+	// XCD k takes the k-th eighth of the blocks (gridDim.x a multiple of NUM_XCD)
+	const long long per = gridDim.x / NUM_XCD;
+	const long long blk = (long long)(blockIdx.x % NUM_XCD) * per + blockIdx.x / NUM_XCD;
+	const long long i = blk * blockDim.x + threadIdx.x;
 	if (i >= n) return;
 	cd ws;
 	double m;
@@ -1117,7 +1133,7 @@ inline bool sweep_check(int done, int cap)
 
 // The first fmm_sweeps sweeps may take the root sums of their dense rows
 // from the FMM pass (when the buffers carry a root grid).
-bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int nrows, int cap, int fmm_sweeps)
+bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int nrows, int cap, unsigned fmm_mask)
 {
 	const LensView L = ctx->lens();
 	hipStream_t s = ctx->stream;
@@ -1127,12 +1143,14 @@ bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int
 	const int circ_exact = (!ctx->rectangular && !ctx->P.approx) ? 1 : 0;
 	const int Np = fmm_pole ? 0 : ctx->num_stars;
 	const SumsLaunch sl = sums_launch(ctx, f32);
+	ctx->fmm_rows_sweep[0] = ctx->fmm_rows_sweep[1] = -1; // unknown until a read-back
 	if (B.tile != BLOCK * sl.rpt)
 	{
 		ctx->fail("Error. sweep buffers were sized for another tile\n");
 		return false;
 	}
 	const dim3 g_rows(grid(R), nrows);
+	const dim3 g_witt(xcd_round(grid(R)), nrows);
 	for (int it = 0; it < cap; it++)
 	{
 		const SweepView V = view(B, it & 1);
@@ -1146,12 +1164,12 @@ bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int
 		}
 		if (f32)
 		{
-			k_witt<true><<<g_rows, BLOCK, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, fmm_pole ? 1 : 0);
+			k_witt<true><<<g_witt, BLOCK, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, fmm_pole ? 1 : 0);
 			k_witt_far<true><<<g_far, BLOCK, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, fmm_pole ? 1 : 0);
 		}
 		else
 		{
-			k_witt<false><<<g_rows, BLOCK, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, 0);
+			k_witt<false><<<g_witt, BLOCK, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, 0);
 			k_witt_far<false><<<g_far, BLOCK, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, 0);
 		}
 		if (ctx->P.timing && tw.a && tw.b)
@@ -1159,8 +1177,8 @@ bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int
 			(void)hipEventRecord(tw.b, s);
 			ctx->witt_timers.push_back(tw);
 		}
-		const bool fmm = B.grid.order > 0 && it < fmm_sweeps;
-		k_plan<<<1, 1024, 0, s>>>(V, nrows, Np + R, sl.target, fmm ? root_fmm_min(R) : 0);
+		const bool fmm = B.grid.order > 0 && it < 32 && ((fmm_mask >> it) & 1u);
+		k_plan<<<1, 1024, 0, s>>>(V, nrows, Np + R, sl.target, fmm ? root_fmm_min(R) : 0, it);
 		ccf::Timer tm;
 		if (ctx->P.timing)
 		{
@@ -1200,10 +1218,13 @@ bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int
 		if (ctx->host_counts && nrows <= ctx->max_rows && sweep_check(it + 1, cap))
 		{
 			if (!CCF_HIP(ctx, hipMemcpyAsync(ctx->host_counts, V.cnt_next, nrows * sizeof(int), hipMemcpyDeviceToHost, s)) ||
+				!CCF_HIP(ctx, hipMemcpyAsync(ctx->host_counts + ctx->max_rows, B.plan + 2, 2 * sizeof(int), hipMemcpyDeviceToHost, s)) ||
 				!CCF_HIP(ctx, hipStreamSynchronize(s)))
 				return false;
 			long long active = 0;
 			for (int r = 0; r < nrows; r++) active += ctx->host_counts[r];
+			ctx->fmm_rows_sweep[0] = ctx->host_counts[ctx->max_rows];
+			ctx->fmm_rows_sweep[1] = ctx->host_counts[ctx->max_rows + 1];
 			if (active == 0) break;
 		}
 	}
@@ -1304,7 +1325,7 @@ bool alloc_solver(ccf_ctx* ctx)
 			CCF_HIP(ctx, hipMemsetAsync(ctx->ccs_init, 0, pts * sizeof(double2), ctx->stream)) &&
 			CCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
 	return CCF_HIP(ctx, hipMalloc(&ctx->ccs_init, pts * sizeof(double2))) && alloc_sweep(ctx, ctx->sw, ctx->max_rows, R) &&
-		CCF_HIP(ctx, hipHostMalloc((void**)&ctx->host_counts, ctx->max_rows * sizeof(int), hipHostMallocDefault)) &&
+		CCF_HIP(ctx, hipHostMalloc((void**)&ctx->host_counts, (ctx->max_rows + 2) * sizeof(int), hipHostMallocDefault)) &&
 		CCF_HIP(ctx, hipMalloc(&ctx->phase, (size_t)(np + nb) * sizeof(double2))) && launch_phase(ctx) &&
 		CCF_HIP(ctx, hipMalloc(&ctx->counters, STAT_WORDS * sizeof(unsigned long long))) &&
 		CCF_HIP(ctx, hipMemsetAsync(ctx->ccs_init, 0, pts * sizeof(double2), ctx->stream)) &&
@@ -1328,7 +1349,7 @@ bool find_initial_roots_dev(ccf_ctx* ctx, int verbose)
 	k_prepare<<<dim3(grid(R), nb), BLOCK, 0, s>>>(ctx->ccs_init, M, ctx->sw, 0, ctx->dorder);
 	if (!CCF_LAUNCH(ctx, "prepare_roots_kernel")) return false;
 	const int cap = ctx->P.init_iters > 0 ? ctx->P.init_iters : 60;
-	if (!run_sweeps(ctx, ctx->sw, ctx->ccs_init, M, nb, cap, ROOT_FMM_INIT_SWEEPS)) return false;
+	if (!run_sweeps(ctx, ctx->sw, ctx->ccs_init, M, nb, cap, (1u << ROOT_FMM_INIT_SWEEPS) - 1u)) return false;
 	if (!CCF_HIP(ctx, hipStreamSynchronize(s))) return false;
 	auto t1 = std::chrono::steady_clock::now();
 	ctx->info.t_init_roots = std::chrono::duration<double>(t1 - t0).count();
@@ -1344,16 +1365,26 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 	const int cap = ctx->P.step_iters > 0 ? ctx->P.step_iters : 20;
 	if (verbose >= 1) std::cout << "Finding critical curve positions...\n";
 	auto t0 = std::chrono::steady_clock::now();
+	// rows the root-sum FMM took in the previous step's first and second
+	// sweep (-1: not read back); a sweep whose counterpart took none skips
+	// the pass, whose launches cost more than its empty rows, and every
+	// ROOT_FMM_PROBE-th step tries again
+	int prev[2] = {-1, -1};
 	for (int j = 1; j <= steps; j++)
 	{
 		const RowMap M{R, np, nb, j, 2, 0.0, ctx->phase};
+		unsigned mask = 0;
+		for (int q = 0; q < ROOT_FMM_STEP_SWEEPS; q++)
+			if (prev[q] != 0 || j % ROOT_FMM_PROBE == 1) mask |= 1u << q;
 		// the exact smooth sheets are discontinuous at the field's edge
 		// (boxcar / heaviside terms), where extrapolated roots could cross
 		// onto another branch: those keep the previous step's roots
 		const int order = ctx->P.approx ? ctx->P.predictor : 0;
 		k_prepare<<<dim3(grid(R), 2 * nb), BLOCK, 0, s>>>(ctx->ccs_init, M, ctx->sw, order, ctx->dorder);
 		if (!CCF_LAUNCH(ctx, "prepare_roots_kernel")) return false;
-		if (!run_sweeps(ctx, ctx->sw, ctx->ccs_init, M, 2 * nb, cap, ROOT_FMM_STEP_SWEEPS)) return false;
+		if (!run_sweeps(ctx, ctx->sw, ctx->ccs_init, M, 2 * nb, cap, mask)) return false;
+		prev[0] = ctx->fmm_rows_sweep[0];
+		prev[1] = ctx->fmm_rows_sweep[1];
 		if (verbose >= 1 && j * 100 / steps > (j - 1) * 100 / steps)
 		{
 			if (!CCF_HIP(ctx, hipStreamSynchronize(s))) return false;
@@ -1370,7 +1401,7 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 	// max 1/mu error and the NaN abort (ccf.cuh:1039-1069)
 	const long long total = (long long)(np + nb) * R;
 	const LensView L = ctx->lens();
-	k_errors<<<grid(total), BLOCK, 0, s>>>(L, ctx->ccs_init, R, np, nb, ctx->phase, ctx->dorder, ctx->counters);
+	k_errors<<<dim3(xcd_round(grid(R)), np + nb), BLOCK, 0, s>>>(L, ctx->ccs_init, R, ctx->phase, ctx->dorder, ctx->counters);
 	if (!CCF_LAUNCH(ctx, "find_errors_kernel")) return false;
 	unsigned long long c[C_COUNT] = {};
 	{
@@ -1475,7 +1506,7 @@ bool find_caustics_dev(ccf_ctx* ctx, bool caustics, bool mu)
 	hipStream_t s = ctx->stream;
 	if (caustics && !ctx->caustics && !CCF_HIP(ctx, hipMalloc(&ctx->caustics, (size_t)n * sizeof(double2)))) return false;
 	if (mu && !ctx->mu_scales && !CCF_HIP(ctx, hipMalloc(&ctx->mu_scales, (size_t)n * sizeof(double)))) return false;
-	k_caustics<<<grid(n), BLOCK, 0, s>>>(ctx->lens(), ctx->ccs, n, caustics ? ctx->caustics : nullptr, mu ? ctx->mu_scales : nullptr);
+	k_caustics<<<xcd_round(grid(n)), BLOCK, 0, s>>>(ctx->lens(), ctx->ccs, n, caustics ? ctx->caustics : nullptr, mu ? ctx->mu_scales : nullptr);
 	return CCF_LAUNCH(ctx, "find_caustics_kernel") && CCF_HIP(ctx, hipStreamSynchronize(s));
 }
 
