@@ -101,8 +101,9 @@ __device__ inline int centre_row(const BField& F, int c) { return F.np / (2 * F.
 // All sweeps of one row (at most cap): zc holds the row's roots, la/lb are
 // the ping-pong active lists. Returns after the first sweep that leaves no
 // root active. newton (MIXED, continuation steps): a root whose FP64 Newton
-// step is already below the convergence threshold takes it and ends without
-// the sums, as in the single-field path.
+// step is below NEWTON_STEP_TOL takes it instead of an Aberth update, and
+// ends there when the step is below the convergence threshold, as in the
+// single-field path.
 constexpr int BCNT = 6; // full, early, converged, near-field stars, sweeps, Newton
 
 template <int BS, int RPT, bool MIXED>
@@ -160,10 +161,12 @@ __device__ void solve_row(const BField& F, int row, int cap, bool newton, double
 				if (MIXED && newton)
 				{
 					const cd dz = f0[k] / f1[k];
-					if (dz.abs() < CONV_TOL * L.theta) // false for a non-finite step
+					const double adz = dz.abs();
+					if (adz < NEWTON_STEP_TOL * L.theta) // false for a non-finite step
 					{
 						zc[i] = make_double2(z.re - dz.re, z.im - dz.im);
-						cnt[2]++;
+						if (adz < CONV_TOL * L.theta) cnt[2]++;
+						else nxt[atomicAdd(&s_n[1], 1)] = i; // stepped: checked again next sweep
 						cnt[3] += w.nr.count;
 						cnt[5]++;
 						continue;

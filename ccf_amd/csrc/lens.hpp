@@ -349,13 +349,20 @@ __device__ inline int xcd_x(int j, int used)
 
 // |dz| / theta below which an Aberth update ends a root (ccf_functions.cuh:384)
 constexpr double CONV_TOL = 1e-9;
+// mixed form, continuation steps: |f0/f1| / theta below which a root takes
+// a plain FP64 Newton step instead of an Aberth update. The root is then
+// within a millionth of theta_E of a root of Witt's equation, where Newton
+// converges quadratically (the next sweep's step is about 1e-12) and the
+// Aberth sums, which only keep far-off iterates apart, change nothing.
+constexpr double NEWTON_STEP_TOL = 1e-6;
 
 // what the Witt evaluation of a listed root decided
 enum RootState
 {
-	ROOT_EARLY = 0,  // the 1/mu early exit (ccf_functions.cuh:207-211)
-	ROOT_FULL = 1,   // the root needs the Cauchy sums and an Aberth update
-	ROOT_NEWTON = 2, // mixed form: ended by a Newton step below CONV_TOL
+	ROOT_EARLY = 0,   // the 1/mu early exit (ccf_functions.cuh:207-211)
+	ROOT_FULL = 1,    // the root needs the Cauchy sums and an Aberth update
+	ROOT_NEWTON = 2,  // mixed form: ended by a Newton step below CONV_TOL
+	ROOT_STEPPED = 3, // mixed form: took a Newton step below NEWTON_STEP_TOL, still active
 };
 
 // Initial root of a phi step (prepare_roots_kernel, ccf_functions.cuh:284-289

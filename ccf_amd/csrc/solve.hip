@@ -233,7 +233,7 @@ struct SweepView
 	int R, rows, tile;
 	const int* act;
 	int* act_next;
-	const int* cnt;
+	int* cnt;      // cleared by k_plan once read
 	int* cnt_next;
 	int* fl;
 	int* fcnt;
@@ -287,7 +287,8 @@ SweepView view(const SweepBufs& B, int parity)
 // Newton step z - f0/f1 (a final polish; the reference keeps z as it is) and
 // whose pole sum is the near-field sum plus the FP32 far field. With newton
 // set (the continuation steps), a root whose Newton step is already below
-// the convergence threshold takes that step and ends without the sums.
+// the convergence threshold takes that step and ends without the sums; one
+// whose step is below NEWTON_STEP_TOL takes it and stays active.
 template <bool MIXED>
 __device__ inline int witt_root(const LensView& L, double2* p, cd z, const Witt& w, int circ_exact, int fmm_pole, int newton,
 	cd& f1, cd& ex)
@@ -305,10 +306,11 @@ __device__ inline int witt_root(const LensView& L, double2* p, cd z, const Witt&
 	if (MIXED && newton)
 	{
 		const cd dz = w.f0 / f1;
-		if (dz.abs() < CONV_TOL * L.theta) // false for a non-finite step
+		const double adz = dz.abs();
+		if (adz < NEWTON_STEP_TOL * L.theta) // false for a non-finite step
 		{
 			*p = make_double2(z.re - dz.re, z.im - dz.im);
-			return ROOT_NEWTON;
+			return adz < CONV_TOL * L.theta ? ROOT_NEWTON : ROOT_STEPPED;
 		}
 	}
 	ex = cd();
@@ -327,17 +329,18 @@ __global__ __launch_bounds__(BLOCK) void k_witt(LensView L, double2* __restrict_
 	const int2 sl = xcd_slot();
 	const int r = sl.y;
 	const int R = M.R;
+	// (cnt_next was cleared by the previous sweep's k_plan or by k_prepare:
+	// stepped roots are appended to it below)
 	if (xcd_row_leader(sl) && threadIdx.x == 0)
 	{
-		V.cnt_next[r] = 0;
 		V.fcnt_next[r] = 0;
 		V.dcnt_next[r] = 0;
 	}
 	const int n = V.cnt[r];
 	const int i = xcd_x(sl.x, (n + BLOCK - 1) / BLOCK) * BLOCK + threadIdx.x;
 	if (i - (int)threadIdx.x >= n) return; // a block without listed roots (uniform)
-	unsigned long long nfull = 0, nearly = 0, nnewton = 0, nnear = 0, nnear_early = 0;
-	bool full = false, deferred = false;
+	unsigned long long nfull = 0, nearly = 0, nnewton = 0, nconv = 0, nnear = 0, nnear_early = 0;
+	bool full = false, deferred = false, stepped = false;
 	int a = 0;
 	cd f0, f1, ex;
 	if (i < n)
@@ -361,15 +364,19 @@ __global__ __launch_bounds__(BLOCK) void k_witt(LensView L, double2* __restrict_
 			const int st = witt_root<MIXED>(L, p, z, w, circ_exact, fmm_pole, M.j > 0, f1, ex);
 			full = st == ROOT_FULL;
 			if (full) f0 = w.f0;
+			stepped = st == ROOT_STEPPED;
 			nfull = full;
 			nearly = st == ROOT_EARLY;
-			nnewton = st == ROOT_NEWTON;
+			nnewton = st == ROOT_NEWTON || stepped;
+			nconv = st == ROOT_EARLY || st == ROOT_NEWTON;
 			if (st != ROOT_EARLY) nnear = w.nr.count;
 			else nnear_early = w.nr.count;
 		}
 	}
 	const int dpos = block_append(&V.dcnt[r], deferred);
 	if (deferred) V.dl[(size_t)r * R + dpos] = a;
+	const int spos = block_append(&V.cnt_next[r], stepped);
+	if (stepped) V.act_next[(size_t)r * R + spos] = a;
 	const int pos = block_append(&V.fcnt[r], full);
 	if (full)
 	{
@@ -381,7 +388,7 @@ __global__ __launch_bounds__(BLOCK) void k_witt(LensView L, double2* __restrict_
 		V.ex[k] = make_double2(ex.re, ex.im);
 	}
 	const int which[6] = {C_FULL, C_EARLY, C_CONV, C_NEAR, C_NEWTON, C_NEAR_EARLY};
-	unsigned long long v[6] = {nfull, nearly, nearly + nnewton, nnear, nnewton, nnear_early};
+	unsigned long long v[6] = {nfull, nearly, nconv, nnear, nnewton, nnear_early};
 	block_stats(stats, which, v);
 }
 
@@ -460,6 +467,12 @@ __global__ __launch_bounds__(BLOCK) void k_witt_far(LensView L, double2* __restr
 				atomicAdd(stat_slot(stats, C_FULL), 1ull);
 				atomicAdd(stat_slot(stats, C_NEAR), (unsigned long long)sum.count);
 			}
+			else if (st == ROOT_STEPPED)
+			{
+				V.act_next[(size_t)r * R + atomicAdd(&V.cnt_next[r], 1)] = a;
+				atomicAdd(stat_slot(stats, C_NEWTON), 1ull);
+				atomicAdd(stat_slot(stats, C_NEAR), (unsigned long long)sum.count);
+			}
 			else
 			{
 				atomicAdd(stat_slot(stats, st == ROOT_NEWTON ? C_NEWTON : C_EARLY), 1ull);
@@ -487,6 +500,9 @@ __global__ void k_plan(SweepView V, int nrows, int terms, int target, int fmm_mi
 		const bool fmm = r < nrows && fmm_min > 0 && V.fcnt[r] >= fmm_min;
 		if (r < nrows && V.rmode) V.rmode[r] = fmm ? 1 : 0;
 		if (fmm) atomicAdd(&nfmm, 1);
+		// this sweep's active count is consumed (k_witt read it): cleared
+		// here, it is the list the next sweep's k_witt appends stepped roots to
+		if (r < nrows) V.cnt[r] = 0;
 		const int v = (r < nrows && !fmm) ? (V.fcnt[r] + V.tile - 1) / V.tile : 0;
 		int x = v;
 		for (int d = 1; d < 64; d <<= 1)
@@ -1222,7 +1238,15 @@ bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int
 				!CCF_HIP(ctx, hipStreamSynchronize(s)))
 				return false;
 			long long active = 0;
-			for (int r = 0; r < nrows; r++) active += ctx->host_counts[r];
+			int amax = 0;
+			for (int r = 0; r < nrows; r++)
+			{
+				active += ctx->host_counts[r];
+				amax = std::max(amax, ctx->host_counts[r]);
+			}
+			// a row's listed roots are among its active ones: once no row has
+			// as many as the FMM takes, the remaining sweeps skip its pass
+			if (B.grid.order > 0 && amax < root_fmm_min(R)) fmm_mask = 0;
 			ctx->fmm_rows_sweep[0] = ctx->host_counts[ctx->max_rows];
 			ctx->fmm_rows_sweep[1] = ctx->host_counts[ctx->max_rows + 1];
 			if (active == 0) break;

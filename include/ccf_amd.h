@@ -109,8 +109,9 @@ typedef struct ccf_info
 	double witt_kernel_ms;     /* sum of timed Witt-evaluation launches (params.timing = 1) */
 	long long witt_launches;
 	double flops_witt;         /* algorithmic flops of the Witt evaluations (flops_update - flops_sums) */
-	long long updates_newton;  /* precision 1, continuation steps: roots whose FP64 Newton step z - f0/f1
-	                              was already below 1e-9 theta took it and ended without the sums */
+	long long updates_newton;  /* precision 1, continuation steps: FP64 Newton steps z - f0/f1 taken
+	                              instead of an Aberth update (no Cauchy sums) where |f0/f1| < 1e-6 theta;
+	                              below 1e-9 theta the root ends there */
 } ccf_info;
 
 /* Arrays readable with ccf_get_array. */
