@@ -101,6 +101,7 @@ void ccf_ctx::free_device()
 	witt_timers.clear();
 	events_used = 0;
 	loaded = prepared = have_roots = have_ccs = have_caustics = have_mu = false;
+	caustics_done = mu_done = false;
 }
 
 namespace

@@ -69,6 +69,19 @@ struct cplx
 		return cplx(re * d, -im * d);
 	}
 	__host__ __device__ cplx& operator/=(cplx o) { return *this *= o.inv(); }
+	// z^n (n >= 0) by repeated squaring: 2 log2(n) products instead of n,
+	// rounding that differs from pow's at the level of a few ulps per
+	// product (the FMM translations of the tree use it)
+	__host__ __device__ cplx ipow(int n) const
+	{
+		cplx r(1, 0), b = *this;
+		for (; n > 0; n >>= 1)
+		{
+			if (n & 1) r *= b;
+			b *= b;
+		}
+		return r;
+	}
 	__host__ __device__ cplx pow(int n) const
 	{
 		cplx r(1, 0);

@@ -224,6 +224,9 @@ struct ccf_ctx
 	}
 	int* host_counts = nullptr;   // pinned: active roots per row, then the two FMM row counts, read back between sweeps
 	int fmm_rows_sweep[2] = {-1, -1}; // root-sum FMM rows of the last phase's first and second sweep (-1: unknown)
+	int phase_sweeps = 0;         // sweeps the last phase ran (its cap when no read-back saw it end)
+	bool caustics_done = false;   // caustics / length scales formed by the error pass of find_ccs
+	bool mu_done = false;
 	double2* phase = nullptr;     // (cos phi, sin phi) of every phi row
 	bool batch_member = false;    // a field of a ccf_batch: no sweep buffers, seeds on the device
 	bool own_stream = true;       // ccf_destroy destroys the stream

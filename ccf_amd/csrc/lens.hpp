@@ -411,11 +411,9 @@ __device__ inline double err_bound(const Witt& w)
 }
 
 // lens map and magnification length scale (ccf_functions.cuh:33-45, 144-166)
-__device__ inline void lens_map(const LensView& L, cd z, cd& wsrc, double& mu_scale)
+// from the near- and far-field sums of z's node
+__device__ inline void lens_map_from(const LensView& L, cd z, const Near& nr, const Far& fr, cd& wsrc, double& mu_scale)
 {
-	const int node = nearest_node(L, z);
-	const Near nr = p2p(L, node, z);
-	const Far fr = l2p(L, node, z);
 	const double t2 = L.theta2, h = fr.hl;
 	const cd a_star = (nr.s1 * t2).conj();
 	const cd da_star = (nr.s2 * (-t2)).conj();
@@ -432,6 +430,12 @@ __device__ inline void lens_map(const LensView& L, cd z, cd& wsrc, double& mu_sc
 	const cd cct = cd(0, -2) * dwdzb.conj() * d2wdzb2;
 	const cd ct = dwdz * cct + dwdzb * cct.conj();
 	mu_scale = 1 / (2 * ct.abs());
+}
+
+__device__ inline void lens_map(const LensView& L, cd z, cd& wsrc, double& mu_scale)
+{
+	const int node = nearest_node(L, z);
+	lens_map_from(L, z, p2p(L, node, z), l2p(L, node, z), wsrc, mu_scale);
 }
 
 } // namespace ccf

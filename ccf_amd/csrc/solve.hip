@@ -994,9 +994,12 @@ __global__ void k_prepare(double2* roots, RowMap M, SweepBufs B, int order, cons
 
 // 1/mu error bound at every point of the solved rows, reduced to its max
 // with a non-finite flag (find_errors_kernel + has_nan_err_kernel +
-// thrust::max_element, ccf_functions.cuh:419-489)
+// thrust::max_element, ccf_functions.cuh:419-489). With caus / mu set, the
+// same evaluation also gives the caustic point and the length scale
+// (find_caustics_kernel, find_mu_length_scales_kernel), written in file
+// order: root a, phi row r at a * rows + r.
 __global__ void k_errors(LensView L, const double2* __restrict__ roots, int R, const double2* __restrict__ cs,
-	const int* __restrict__ perm, unsigned long long* stats)
+	const int* __restrict__ perm, unsigned long long* stats, double2* __restrict__ caus, double* __restrict__ mu)
 {
 	// grid: (xcd_round(blocks per row), rows); a row's points in the roots'
 	// processing order
@@ -1016,6 +1019,15 @@ __global__ void k_errors(LensView L, const double2* __restrict__ roots, int R, c
 		e = err_bound(w);
 		res = w.f0.abs();
 		if (!isfinite(e)) { bad = 1; e = 0; res = 0; }
+		if (caus || mu)
+		{
+			cd ws;
+			double m;
+			lens_map_from(L, z, w.nr, w.fr, ws, m);
+			const size_t o = (size_t)(idx - (long long)row * R) * gridDim.y + row;
+			if (caus) caus[o] = make_double2(ws.re, ws.im);
+			if (mu) mu[o] = m;
+		}
 	}
 	for (int d = 32; d >= 1; d >>= 1)
 	{
@@ -1137,20 +1149,22 @@ SumsLaunch sums_launch(ccf_ctx* ctx, bool f32)
 
 // `cap` sweeps over the rows of M (ccf.cuh:929-939 and 1013-1018): the
 // sweep kernels exit at once when no root of a row is active any more
-// whether to read the active counts back after `done` of `cap` sweeps: after
-// each of sweeps 3 to 6 (a phi step ends there: after three with a
-// continuation predictor, five or six without), then every 4th. A read-back
-// waits for the device, about as long as a few sweeps of stragglers take.
-inline bool sweep_check(int done, int cap)
+// whether to read the active counts back after `done` of `cap` sweeps: from
+// sweep `first` (the sweeps the previous phi step needed, 3 at first) after
+// each of three sweeps, then every 4th. A read-back waits for the device,
+// about as long as a few sweeps of stragglers take.
+inline bool sweep_check(int done, int cap, int first)
 {
-	if (done >= cap) return false;
-	return (done >= 3 && done <= 6) || done % 4 == 0;
+	if (done >= cap || done < first) return false;
+	return done < first + 3 || done % 4 == 0;
 }
 
 // The first fmm_sweeps sweeps may take the root sums of their dense rows
 // from the FMM pass (when the buffers carry a root grid).
-bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int nrows, int cap, unsigned fmm_mask)
+bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int nrows, int cap, unsigned fmm_mask,
+	int first_check = 3)
 {
+	ctx->phase_sweeps = cap;
 	const LensView L = ctx->lens();
 	hipStream_t s = ctx->stream;
 	const int R = M.R;
@@ -1231,7 +1245,7 @@ bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int
 		// stop once no root of any row is active: the remaining sweeps up to
 		// the cap would do nothing (checked at a few sweeps only, each check
 		// waits for the device)
-		if (ctx->host_counts && nrows <= ctx->max_rows && sweep_check(it + 1, cap))
+		if (ctx->host_counts && nrows <= ctx->max_rows && sweep_check(it + 1, cap, first_check))
 		{
 			if (!CCF_HIP(ctx, hipMemcpyAsync(ctx->host_counts, V.cnt_next, nrows * sizeof(int), hipMemcpyDeviceToHost, s)) ||
 				!CCF_HIP(ctx, hipMemcpyAsync(ctx->host_counts + ctx->max_rows, B.plan + 2, 2 * sizeof(int), hipMemcpyDeviceToHost, s)) ||
@@ -1249,7 +1263,11 @@ bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int
 			if (B.grid.order > 0 && amax < root_fmm_min(R)) fmm_mask = 0;
 			ctx->fmm_rows_sweep[0] = ctx->host_counts[ctx->max_rows];
 			ctx->fmm_rows_sweep[1] = ctx->host_counts[ctx->max_rows + 1];
-			if (active == 0) break;
+			if (active == 0)
+			{
+				ctx->phase_sweeps = it + 1;
+				break;
+			}
 		}
 	}
 	return true;
@@ -1394,6 +1412,7 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 	// the pass, whose launches cost more than its empty rows, and every
 	// ROOT_FMM_PROBE-th step tries again
 	int prev[2] = {-1, -1};
+	int first = 3; // sweeps the previous step needed
 	for (int j = 1; j <= steps; j++)
 	{
 		const RowMap M{R, np, nb, j, 2, 0.0, ctx->phase};
@@ -1406,9 +1425,12 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 		const int order = ctx->P.approx ? ctx->P.predictor : 0;
 		k_prepare<<<dim3(grid(R), 2 * nb), BLOCK, 0, s>>>(ctx->ccs_init, M, ctx->sw, order, ctx->dorder);
 		if (!CCF_LAUNCH(ctx, "prepare_roots_kernel")) return false;
-		if (!run_sweeps(ctx, ctx->sw, ctx->ccs_init, M, 2 * nb, cap, mask)) return false;
+		if (!run_sweeps(ctx, ctx->sw, ctx->ccs_init, M, 2 * nb, cap, mask, first)) return false;
 		prev[0] = ctx->fmm_rows_sweep[0];
 		prev[1] = ctx->fmm_rows_sweep[1];
+		// a step checked first at k can end no earlier than k: every
+		// ROOT_FMM_PROBE-th step starts from 3 again
+		first = (j + 1) % ROOT_FMM_PROBE == 1 ? 3 : std::max(2, std::min(ctx->phase_sweeps, 6));
 		if (verbose >= 1 && j * 100 / steps > (j - 1) * 100 / steps)
 		{
 			if (!CCF_HIP(ctx, hipStreamSynchronize(s))) return false;
@@ -1425,7 +1447,14 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 	// max 1/mu error and the NaN abort (ccf.cuh:1039-1069)
 	const long long total = (long long)(np + nb) * R;
 	const LensView L = ctx->lens();
-	k_errors<<<dim3(xcd_round(grid(R)), np + nb), BLOCK, 0, s>>>(L, ctx->ccs_init, R, ctx->phase, ctx->dorder, ctx->counters);
+	// the caustic map and length scales the run will write come from the
+	// same evaluation of the field; the caustics stage then has them
+	const bool wc = ctx->P.write_caustics != 0, wm = ctx->P.write_mu_length_scales != 0;
+	ctx->caustics_done = ctx->mu_done = false;
+	if (wc && !ctx->caustics && !CCF_HIP(ctx, hipMalloc(&ctx->caustics, (size_t)total * sizeof(double2)))) return false;
+	if (wm && !ctx->mu_scales && !CCF_HIP(ctx, hipMalloc(&ctx->mu_scales, (size_t)total * sizeof(double)))) return false;
+	k_errors<<<dim3(xcd_round(grid(R)), np + nb), BLOCK, 0, s>>>(L, ctx->ccs_init, R, ctx->phase, ctx->dorder, ctx->counters,
+		wc ? ctx->caustics : nullptr, wm ? ctx->mu_scales : nullptr);
 	if (!CCF_LAUNCH(ctx, "find_errors_kernel")) return false;
 	unsigned long long c[C_COUNT] = {};
 	{
@@ -1449,6 +1478,8 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 		ctx->fail("Error. Errors in 1/mu contain values which are not positive real numbers.\n");
 		return false;
 	}
+	ctx->caustics_done = wc;
+	ctx->mu_done = wm;
 	double e;
 	std::memcpy(&e, &c[C_ERRBITS], sizeof(e));
 	ctx->max_error = e;
@@ -1528,6 +1559,10 @@ bool find_caustics_dev(ccf_ctx* ctx, bool caustics, bool mu)
 {
 	const long long n = (long long)(ctx->P.num_phi + ctx->P.num_branches) * ctx->num_roots;
 	hipStream_t s = ctx->stream;
+	// what the error pass of find_ccs already formed
+	if (ctx->caustics_done) caustics = false;
+	if (ctx->mu_done) mu = false;
+	if (!caustics && !mu) return true;
 	if (caustics && !ctx->caustics && !CCF_HIP(ctx, hipMalloc(&ctx->caustics, (size_t)n * sizeof(double2)))) return false;
 	if (mu && !ctx->mu_scales && !CCF_HIP(ctx, hipMalloc(&ctx->mu_scales, (size_t)n * sizeof(double)))) return false;
 	k_caustics<<<xcd_round(grid(n)), BLOCK, 0, s>>>(ctx->lens(), ctx->ccs, n, caustics ? ctx->caustics : nullptr, mu ? ctx->mu_scales : nullptr);

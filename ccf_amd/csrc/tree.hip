@@ -354,7 +354,7 @@ __global__ void k_upward(int off, int cnt, int p, const double2* __restrict__ ce
 			for (int s = 0; s < ns; s++)
 			{
 				cd t = (ld(spos, s0 + s) - c) / h;
-				r -= t.pow(l) * smass[s0 + s];
+				r -= t.ipow(l) * smass[s0 + s];
 			}
 			r /= (double)l;
 		}
@@ -379,7 +379,7 @@ __global__ void k_upward(int off, int cnt, int p, const double2* __restrict__ ce
 				t *= idz;
 			}
 			t -= ld(a, 0) / (double)l;
-			t *= dz.pow(l);
+			t *= dz.ipow(l);
 			t /= pow2_divisor(l);
 			r += t;
 		}
@@ -396,6 +396,13 @@ __global__ void k_downward(int off, int cnt, int p, const double2* __restrict__ 
 	const double2* __restrict__ spos, const double* __restrict__ smass, const double* __restrict__ B,
 	const double2* __restrict__ mp, double2* loc)
 {
+	// the binomial rows 0..2p in LDS (dynamic, launched with their size);
+	// the Horner loops below take their coefficients and binomials four at a
+	// time ahead of the dependent products, in the same order of operations
+	extern __shared__ double sB[];
+	const int nB = (2 * p + 1) * (2 * p + 2) / 2;
+	for (int q = threadIdx.x; q < nB; q += blockDim.x) sB[q] = B[q];
+	__syncthreads();
 	const int idx = blockIdx.x * blockDim.x + threadIdx.x;
 	const int i = idx / (p + 1), l = idx % (p + 1);
 	if (i >= cnt) return;
@@ -408,12 +415,26 @@ __global__ void k_downward(int off, int cnt, int p, const double2* __restrict__ 
 		const int pa = parent[n];
 		const double2* a = loc + (size_t)pa * P1;
 		const cd dz = (cn - ld(center, pa)) / hl[pa];
-		for (int k = p; k >= 1; k--)
+		for (int k0 = p; k0 >= 1; k0 -= 4)
 		{
-			r += ld(a, k) * bin(B, k, l);
-			r *= dz;
+			cd av[4];
+			double bv[4];
+#pragma unroll
+			for (int u = 0; u < 4; u++)
+			{
+				const int k = max(k0 - u, 1);
+				av[u] = ld(a, k);
+				bv[u] = bin(sB, k, l);
+			}
+#pragma unroll
+			for (int u = 0; u < 4; u++)
+				if (k0 - u >= 1)
+				{
+					r += av[u] * bv[u];
+					r *= dz;
+				}
 		}
-		r *= dz.pow(l).inv();
+		r *= dz.inv().ipow(l);
 		r /= pow2_divisor(l);
 	}
 	for (int j = 0; j < nsl[n]; j++)
@@ -423,25 +444,34 @@ __global__ void k_downward(int off, int cnt, int p, const double2* __restrict__ 
 		const cd dz = (ld(center, m) - cn) / hl[m];
 		const cd imdz = (-dz).inv();
 		cd t;
+		for (int k0 = p; k0 >= 1; k0 -= 4)
+		{
+			cd bv[4];
+			double cv[4];
+#pragma unroll
+			for (int u = 0; u < 4; u++)
+			{
+				const int k = max(k0 - u, 1);
+				bv[u] = ld(b, k);
+				cv[u] = l == 0 ? 1.0 : bin(sB, l + k - 1, k - 1);
+			}
+#pragma unroll
+			for (int u = 0; u < 4; u++)
+				if (k0 - u >= 1)
+				{
+					t += l == 0 ? bv[u] : bv[u] * cv[u];
+					t *= imdz;
+				}
+		}
 		if (l == 0)
 		{
-			for (int k = p; k >= 1; k--)
-			{
-				t += ld(b, k);
-				t *= imdz;
-			}
 			t += ld(b, 0) * (-dz).log();
 			t += ld(b, 0) * log(hl[m]);
 		}
 		else
 		{
-			for (int k = p; k >= 1; k--)
-			{
-				t += ld(b, k) * bin(B, l + k - 1, k - 1);
-				t *= imdz;
-			}
 			t -= ld(b, 0) / (double)l;
-			t *= dz.pow(l).inv();
+			t *= dz.inv().ipow(l);
 		}
 		r += t;
 	}
@@ -460,7 +490,7 @@ __global__ void k_downward(int off, int cnt, int p, const double2* __restrict__ 
 			}
 			else
 			{
-				r -= (ms / (double)l) * dz.pow(l).inv();
+				r -= (ms / (double)l) * dz.inv().ipow(l);
 			}
 		}
 	}
@@ -509,6 +539,9 @@ bool ensure_capacity(ccf_ctx* ctx, int need)
 
 // Pascal rows 0..n as the reference tabulates them (binomial_coefficients.cuh:11-44):
 // int rows that wrap past row 33, or exact values
+// LDS bytes of k_downward: the binomial rows 0..2p
+inline size_t binom_lds(int p) { return (size_t)(2 * p + 1) * (2 * p + 2) / 2 * sizeof(double); }
+
 std::vector<double> binomial_table(int n, bool exact)
 {
 	std::vector<double> t((size_t)(n + 1) * (n + 2) / 2);
@@ -740,8 +773,8 @@ bool build_forest(ccf_ctx* ctx, ForestSpec& spec, int verbose)
 		for (int l = 2; l <= T.levels; l++)
 		{
 			const long long work = (long long)T.level_cnt[l] * P1;
-			k_downward<<<grid(work), BLOCK, 0, s>>>(T.level_off[l], T.level_cnt[l], p, T.center, T.hl, T.parent, T.sl, T.nsl,
-				T.dl, T.ndl, T.sstart, T.scount, ctx->spos, ctx->smass, T.binom, T.mp, T.loc);
+			k_downward<<<grid(work), BLOCK, binom_lds(p), s>>>(T.level_off[l], T.level_cnt[l], p, T.center, T.hl,
+				T.parent, T.sl, T.nsl, T.dl, T.ndl, T.sstart, T.scount, ctx->spos, ctx->smass, T.binom, T.mp, T.loc);
 		}
 		if (!CCF_LAUNCH(ctx, "calculate_coeffs_kernels")) return false;
 	}
@@ -769,8 +802,8 @@ bool build_forest(ccf_ctx* ctx, ForestSpec& spec, int verbose)
 		for (int l = 2; l <= T.levels; l++)
 		{
 			const long long work = (long long)T.level_cnt[l] * P1;
-			k_downward<<<grid(work), BLOCK, 0, s>>>(T.level_off[l], T.level_cnt[l], p, T.center, T.hl, T.parent, T.sl, T.nsl,
-				T.dl, T.ndl, T.sstart, T.scount, ctx->spos, ctx->ones, T.binom, T.mpu, T.locu);
+			k_downward<<<grid(work), BLOCK, binom_lds(p), s>>>(T.level_off[l], T.level_cnt[l], p, T.center, T.hl, T.parent, T.sl,
+				T.nsl, T.dl, T.ndl, T.sstart, T.scount, ctx->spos, ctx->ones, T.binom, T.mpu, T.locu);
 		}
 		k_to_float<<<grid((long long)T.nnodes * P1), BLOCK, 0, s>>>(T.locu, T.locu32, (size_t)T.nnodes * P1);
 		if (!CCF_LAUNCH(ctx, "unit-mass expansions")) return false;
