@@ -20,8 +20,9 @@
 //    214-242), then the commit and the |dz| / theta < 1e-9 test
 //    (ccf_functions.cuh:379-389). Workgroups take tasks from the queue,
 //    largest rows first, until it is empty.
-//  * the 1/mu error pass, the transpose and the caustic map run once over
-//    all fields (one grid dimension per field);
+//  * the 1/mu error pass (which also forms the caustic map from the same
+//    field evaluation) and the transpose run once over all fields (one grid
+//    dimension per field);
 //  * the launches of a solve are captured in a HIP graph and replayed.
 // An autotuner picks the workgroup size (roots per thread follow from the
 // largest row) by timing the phase-0 launch of the batch itself.
@@ -344,7 +345,9 @@ __global__ void k_zero_u64(unsigned long long* p, int n)
 }
 
 // 1/mu error bound and non-finite flag over every point of every field
-// (find_errors_kernel + has_nan_err_kernel, ccf_functions.cuh:419-489)
+// (find_errors_kernel + has_nan_err_kernel, ccf_functions.cuh:419-489), and
+// the caustic map and length scales of the fields that write them
+// (find_caustics_kernel, find_mu_length_scales_kernel)
 __global__ __launch_bounds__(BLOCK) void k_batch_errors(const BField* __restrict__ fields)
 {
 	const BField& F = fields[blockIdx.y];
@@ -365,6 +368,17 @@ __global__ __launch_bounds__(BLOCK) void k_batch_errors(const BField* __restrict
 		{
 			e = fmax(e, b);
 			res = fmax(res, w.f0.abs());
+		}
+		// the caustic point and length scale from the same sums, in file
+		// order (root a, phi row r at a * rows + r)
+		if (F.caustics || F.mu)
+		{
+			cd ws;
+			double m;
+			lens_map_from(F.L, z, w.nr, w.fr, ws, m);
+			const size_t o = (size_t)(idx - (long long)row * F.R) * (F.np + F.nb) + row;
+			if (F.caustics) F.caustics[o] = make_double2(ws.re, ws.im);
+			if (F.mu) F.mu[o] = m;
 		}
 	}
 	for (int d = 32; d >= 1; d >>= 1)
@@ -404,22 +418,6 @@ __global__ __launch_bounds__(BLOCK) void k_batch_transpose(const BField* __restr
 			}
 			__syncthreads();
 		}
-}
-
-// lens map and length scale at every point of every field
-// (find_caustics_kernel, find_mu_length_scales_kernel)
-__global__ __launch_bounds__(BLOCK) void k_batch_caustics(const BField* __restrict__ fields)
-{
-	const BField& F = fields[blockIdx.y];
-	const long long total = (long long)(F.np + F.nb) * F.R;
-	for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long long)gridDim.x * blockDim.x)
-	{
-		cd w;
-		double m;
-		lens_map(F.L, cd(F.ccs[i].x, F.ccs[i].y), w, m);
-		if (F.caustics) F.caustics[i] = make_double2(w.re, w.im);
-		if (F.mu) F.mu[i] = m;
-	}
 }
 
 using BatchKernel = void (*)(const BField*, const BTask*, int, int*, int, int, int);
@@ -519,7 +517,6 @@ bool launch_post(ccf_batch* b, hipStream_t s)
 	const int blocks = std::max(1, std::min(grid(d.maxpts), 1024));
 	k_batch_errors<<<dim3(blocks, d.nf), BLOCK, 0, s>>>(d.fields);
 	k_batch_transpose<<<dim3(std::min(grid(d.rmax, 32), 64), std::min(grid(d.maxrows, 32), 64), d.nf), BLOCK, 0, s>>>(d.fields);
-	k_batch_caustics<<<dim3(blocks, d.nf), BLOCK, 0, s>>>(d.fields);
 	return CCF_LAUNCH(m, "batched errors, transpose, caustics");
 }
 
@@ -528,7 +525,7 @@ bool enqueue_solve(ccf_batch* b, hipStream_t s)
 {
 	BatchDevice& d = *b->dev;
 	k_zero_u64<<<grid((long long)d.nf * BSTAT), BLOCK, 0, s>>>(d.stats, d.nf * BSTAT);
-	return launch_batch_phase(b, 0, b->block, b->grid0, s) && launch_batch_phase(b, 1, b->block, b->grid1, s) && launch_post(b, s);
+	return launch_batch_phase(b, 0, b->block, b->grid0, s) && launch_batch_phase(b, 1, b->block1, b->grid1, s) && launch_post(b, s);
 }
 
 } // namespace
@@ -627,6 +624,14 @@ bool batch_build(ccf_batch* b, int verbose)
 		L.ts_b = Lc.ts_b;
 		L.root = f;
 		L.lookup = nullptr;
+		if (spec.lookup_level[f] > 0)
+		{
+			const int n = 1 << spec.lookup_level[f];
+			L.lookup = m->tree.lookup + spec.lookup_off[f];
+			L.lookup_n = n;
+			L.lookup_x0 = -spec.root_hl[f];
+			L.lookup_inv = n / (2 * spec.root_hl[f]);
+		}
 		const int R = c->num_roots, np = c->P.num_phi, nb = c->P.num_branches;
 		const size_t pts = (size_t)(np + nb) * R;
 		if (!c->ccs && !CCF_HIP(m, hipMalloc(&c->ccs, pts * sizeof(double2)))) return false;
@@ -678,6 +683,9 @@ bool batch_build(ccf_batch* b, int verbose)
 	int nmax = 0;
 	for (ccf_ctx* c : b->fields) nmax = std::max(nmax, c->num_stars);
 	d.nlds = (int)std::max(0LL, std::min<long long>(nmax, room / (long long)sizeof(double2)));
+	// the mixed form takes the pole sum from the star FMM: no star is read
+	// from LDS, which then holds only the row (more workgroups per CU)
+	if (m->P.precision == 1 && m->tree.order > 0) d.nlds = 0;
 	if (!CCF_HIP(m, hipMalloc(&d.stats, sizeof(unsigned long long) * nf * BSTAT))) return false;
 	for (int f = 0; f < nf; f++) hf[f].stats = d.stats + (size_t)f * BSTAT;
 	return CCF_HIP(m, hipMalloc(&d.fields, sizeof(BField) * nf)) && CCF_HIP(m, hipMalloc(&d.tasks0, sizeof(BTask) * d.n0)) &&
@@ -688,41 +696,52 @@ bool batch_build(ccf_batch* b, int verbose)
 		CCF_HIP(m, hipStreamSynchronize(s));
 }
 
-// workgroup size by timing the phase-0 launch of this batch (idempotent: it
-// recomputes the centre rows from the seeds)
+// workgroup size of each phase by timing its launch on this batch at every
+// candidate size (idempotent: phase 0 recomputes the centre rows from the
+// seeds, phase 1 the chains from those rows)
 bool batch_autotune(ccf_batch* b, int verbose)
 {
 	ccf_ctx* m = b->master;
 	hipStream_t s = m->stream;
 	const int cands[3] = {256, 512, 1024};
-	double best = 1e300;
-	int pick = 0;
-	hipEvent_t e0, e1;
-	if (!CCF_HIP(m, hipEventCreate(&e0)) || !CCF_HIP(m, hipEventCreate(&e1))) return false;
+	double best[2] = {1e300, 1e300};
+	int pick[2] = {0, 0};
+	hipEvent_t e0, e1, e2;
+	if (!CCF_HIP(m, hipEventCreate(&e0)) || !CCF_HIP(m, hipEventCreate(&e1)) || !CCF_HIP(m, hipEventCreate(&e2))) return false;
+	b->tuned_ms.clear();
 	for (int bs : cands)
 	{
 		if (rpt_for(b->dev->rmax, bs) > MAX_BATCH_RPT) continue;
-		float ms = 0;
-		const int g = batch_grid(b, 0, bs);
-		if (!launch_batch_phase(b, 0, bs, g, s) || !CCF_HIP(m, hipEventRecord(e0, s)) || !launch_batch_phase(b, 0, bs, g, s) ||
-			!CCF_HIP(m, hipEventRecord(e1, s)) || !CCF_HIP(m, hipEventSynchronize(e1)) || !CCF_HIP(m, hipEventElapsedTime(&ms, e0, e1)))
+		float ms[2] = {0, 0};
+		const int g0 = batch_grid(b, 0, bs), g1 = batch_grid(b, 1, bs);
+		if (!launch_batch_phase(b, 0, bs, g0, s) || !CCF_HIP(m, hipEventRecord(e0, s)) || !launch_batch_phase(b, 0, bs, g0, s) ||
+			!CCF_HIP(m, hipEventRecord(e1, s)) || !launch_batch_phase(b, 1, bs, g1, s) || !CCF_HIP(m, hipEventRecord(e2, s)) ||
+			!CCF_HIP(m, hipEventSynchronize(e2)) || !CCF_HIP(m, hipEventElapsedTime(&ms[0], e0, e1)) ||
+			!CCF_HIP(m, hipEventElapsedTime(&ms[1], e1, e2)))
 			return false;
-		if (verbose >= 2) std::cout << "batch autotune: workgroup " << bs << " roots/thread " << rpt_for(b->dev->rmax, bs) << ": " << ms << " ms\n";
-		b->tuned_ms.push_back(ms);
-		if (ms < best)
+		if (verbose >= 2)
+			std::cout << "batch autotune: workgroup " << bs << " roots/thread " << rpt_for(b->dev->rmax, bs) << ": centres " << ms[0]
+					  << " ms, chains " << ms[1] << " ms\n";
+		for (int q = 0; q < 2; q++)
 		{
-			best = ms;
-			pick = bs;
+			b->tuned_ms.push_back(ms[q]);
+			if (ms[q] < best[q])
+			{
+				best[q] = ms[q];
+				pick[q] = bs;
+			}
 		}
 	}
 	(void)hipEventDestroy(e0);
 	(void)hipEventDestroy(e1);
-	if (!pick)
+	(void)hipEventDestroy(e2);
+	if (!pick[0] || !pick[1])
 	{
 		m->fail("Error. no workgroup size fits the batch\n");
 		return false;
 	}
-	b->block = pick;
+	b->block = pick[0];
+	b->block1 = pick[1];
 	return true;
 }
 
@@ -740,9 +759,12 @@ bool batch_solve(ccf_batch* b, int verbose)
 	ccf_ctx* m = b->master;
 	hipStream_t s = m->stream;
 	BatchDevice& d = *b->dev;
+	// tuned once per batch; again when a rebuild's largest row no longer fits
+	if (b->block && (rpt_for(b->dev->rmax, b->block) > MAX_BATCH_RPT || rpt_for(b->dev->rmax, b->block1) > MAX_BATCH_RPT))
+		b->block = b->block1 = 0;
 	if (b->block == 0 && !batch_autotune(b, verbose)) return false;
 	b->grid0 = batch_grid(b, 0, b->block);
-	b->grid1 = batch_grid(b, 1, b->block);
+	b->grid1 = batch_grid(b, 1, b->block1);
 	auto t0 = std::chrono::steady_clock::now();
 	if (!b->graph_exec)
 	{

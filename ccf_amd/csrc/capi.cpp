@@ -1082,7 +1082,7 @@ int ccf_batch_get_array(ccf_batch* b, int field, int which, void* dst, long long
 	return ccf_get_array(b->fields[field], which, dst, capacity, nrows, ncols);
 }
 
-int ccf_batch_workgroup(const ccf_batch* b) { return b ? b->block : 0; }
+int ccf_batch_workgroup(const ccf_batch* b) { return b ? b->block1 : 0; }
 
 const char* ccf_batch_last_error(const ccf_batch* b) { return b && b->master ? b->master->last_error.c_str() : ""; }
 

@@ -252,7 +252,7 @@ struct ccf_batch
 	ccf::BatchDevice* dev = nullptr;
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t graph_exec = nullptr;
-	int block = 0;                  // autotuned workgroup size
+	int block = 0, block1 = 0;      // autotuned workgroup sizes of the branch-centre and phi-chain phases
 	int grid0 = 0, grid1 = 0;       // persistent grids of the two phases
 	std::vector<float> tuned_ms;
 	ccf_info info{};
@@ -272,6 +272,8 @@ struct ForestSpec
 	std::vector<int> soff, nstars;
 	std::vector<double> root_hl, theta, alpha_error;
 	std::vector<int> levels; // out: depth of each field's tree
+	std::vector<size_t> lookup_off; // out (forests): offset of each field's point-location table in tree.lookup
+	std::vector<int> lookup_level;  // out (forests): its level (0: none)
 	int order = 0;           // out: the expansion order used
 };
 bool build_forest(ccf_ctx* ctx, ForestSpec& spec, int verbose);

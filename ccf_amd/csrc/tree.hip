@@ -276,13 +276,13 @@ __global__ void k_node_box(int nnodes, const double2* __restrict__ center, const
 // cell's centre after at most lv levels
 constexpr int MAX_LOOKUP_LEVEL = 10;
 
-__global__ void k_lookup(int lv, double x0, double cell, const double4* __restrict__ nbox, int* lookup)
+__global__ void k_lookup(int lv, int root, double x0, double cell, const double4* __restrict__ nbox, int* lookup)
 {
 	const int n = 1 << lv;
 	const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (c >= (long long)n * n) return;
 	const double px = x0 + ((int)(c % n) + 0.5) * cell, py = x0 + ((int)(c / n) + 0.5) * cell;
-	int node = 0;
+	int node = root;
 	for (int l = 0; l < lv; l++)
 	{
 		const double4 b = nbox[node];
@@ -701,7 +701,7 @@ bool build_forest(ccf_ctx* ctx, ForestSpec& spec, int verbose)
 			T.lookup_x0 = -H;
 			T.lookup_inv = n / (2 * H);
 			if (!CCF_HIP(ctx, hipMalloc(&T.lookup, sizeof(int) * (size_t)n * n))) return false;
-			k_lookup<<<grid((long long)n * n), BLOCK, 0, s>>>(T.lookup_level, -H, 2 * H / n, T.nbox, T.lookup);
+			k_lookup<<<grid((long long)n * n), BLOCK, 0, s>>>(T.lookup_level, 0, -H, 2 * H / n, T.nbox, T.lookup);
 		}
 		k_near_count<<<grid(nn), BLOCK, 0, s>>>(nn, T.child, T.nf, T.nnf, T.nf_cnt);
 		std::vector<int> cnt(nn), off(nn);
@@ -730,6 +730,35 @@ bool build_forest(ccf_ctx* ctx, ForestSpec& spec, int verbose)
 		if (!CCF_HIP(ctx, hipMemcpy(fid.data(), T.field, sizeof(int) * T.nnodes, hipMemcpyDeviceToHost))) return false;
 		for (int l = 0; l <= T.levels; l++)
 			for (int n = T.level_off[l]; n < T.level_off[l] + T.level_cnt[l]; n++) spec.levels[fid[n]] = l;
+	}
+	// a forest's point-location tables: one per field over its own root box
+	// (node f at level 0, centre 0), at the field's depth, back to back
+	spec.lookup_off.assign(F, 0);
+	spec.lookup_level.assign(F, 0);
+	if (F > 1)
+	{
+		size_t tot = 0;
+		for (int f = 0; f < F; f++)
+		{
+			const int lv = std::min(spec.levels[f], MAX_LOOKUP_LEVEL);
+			if (lv < 2) continue;
+			spec.lookup_level[f] = lv;
+			spec.lookup_off[f] = tot;
+			tot += (size_t)1 << (2 * lv);
+		}
+		if (tot)
+		{
+			if (!CCF_HIP(ctx, hipMalloc(&T.lookup, sizeof(int) * tot))) return false;
+			for (int f = 0; f < F; f++)
+			{
+				const int lv = spec.lookup_level[f];
+				if (!lv) continue;
+				const int n = 1 << lv;
+				const double H = spec.root_hl[f];
+				k_lookup<<<grid((long long)n * n), BLOCK, 0, s>>>(lv, f, -H, 2 * H / n, T.nbox, T.lookup + spec.lookup_off[f]);
+			}
+			if (!CCF_LAUNCH(ctx, "forest point-location tables")) return false;
+		}
 	}
 	// expansion order (ccf.cuh:840-852)
 	int p = 0;

@@ -213,7 +213,8 @@ int ccf_batch_save(ccf_batch* b, int field, int verbose);
 /* info of one field, or of the whole batch for field = -1 (totals; t_ccs = solve time) */
 int ccf_batch_get_info(const ccf_batch* b, int field, ccf_info* info);
 int ccf_batch_get_array(ccf_batch* b, int field, int which, void* dst, long long capacity, int* nrows, int* ncols);
-/* workgroup size the autotuner picked (0 before the first solve) */
+/* workgroup size the autotuner picked for the phi chains, the longer of the two persistent
+   launches (0 before the first solve) */
 int ccf_batch_workgroup(const ccf_batch* b);
 const char* ccf_batch_last_error(const ccf_batch* b);
 
