@@ -106,6 +106,14 @@ __device__ inline int centre_row(const BField& F, int c) { return F.np / (2 * F.
 // ends there when the step is below the convergence threshold, as in the
 // single-field path.
 constexpr int BCNT = 6; // full, early, converged, near-field stars, sweeps, Newton
+// a sweep that lists at most COOP_MAX roots for the Cauchy sums sums them
+// cooperatively: wave w takes listed roots w, w + waves, ... and its lanes
+// split the row (and the stars), so a few listed roots do not leave the
+// workgroup waiting on their owners' serial loops
+constexpr int COOP_MAX = 256;
+// static LDS of k_batch (the cooperative list, counters, reductions), kept
+// out of the dynamic limit
+constexpr int BATCH_STATIC_LDS = 8192;
 
 template <int BS, int RPT, bool MIXED>
 __device__ void solve_row(const BField& F, int row, int cap, bool newton, double2* zc, const double2* st, int* la, int* lb,
@@ -115,11 +123,17 @@ __device__ void solve_row(const BField& F, int row, int cap, bool newton, double
 	const int R = F.R;
 	const double2 e = F.cs[row];
 	const bool pole_fmm = MIXED && L.order > 0;
+	__shared__ int s_nf;                  // roots listed for the sums this sweep
+	__shared__ int s_fl[COOP_MAX];        // their row indices (cooperative sweeps)
+	__shared__ double2 s_fs[COOP_MAX];    // their sums 2 sum 1/(z - s) - sum 1/(z - z_j), (re, -im)
+	constexpr int NW = BS / 64;
+	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 	for (int q = threadIdx.x; q < R; q += BS) la[q] = F.order[q];
 	if (threadIdx.x == 0)
 	{
 		s_n[0] = R;
 		s_n[1] = 0;
+		s_nf = 0;
 	}
 	__syncthreads();
 	int* cur = la;
@@ -130,7 +144,7 @@ __device__ void solve_row(const BField& F, int row, int cap, bool newton, double
 		if (n == 0) break;
 		cnt[4] += threadIdx.x == 0 ? 1 : 0;
 		// Witt function and 1/mu early exit for the listed roots
-		int idx[RPT];
+		int idx[RPT], slot[RPT];
 		bool full[RPT];
 		cd f0[RPT], f1[RPT], ex[RPT];
 #pragma unroll
@@ -177,11 +191,49 @@ __device__ void solve_row(const BField& F, int row, int cap, bool newton, double
 				ex[k] = cd();
 				if (F.circ_exact) ex[k] = 2.0 * z.inv(); // ccf_functions.cuh:223-227
 				if (pole_fmm) ex[k] += 2.0 * (w.nr.s0 + far_pole32(L, node, z));
+				slot[k] = atomicAdd(&s_nf, 1);
+				if (slot[k] < COOP_MAX) s_fl[slot[k]] = i;
 				cnt[0]++;
 				cnt[3] += w.nr.count;
 			}
 		}
 		__syncthreads();
+		const int nf = s_nf;
+		if (nf <= COOP_MAX)
+		{
+			// cooperative sums, one wave per listed root, lanes over the sources
+			for (int g = wv; g < nf; g += NW)
+			{
+				const double2 zg = zc[s_fl[g]];
+				double ax = 0, ay = 0, bx = 0, by = 0;
+				if (!pole_fmm)
+					for (int s = lane; s < F.N; s += 64)
+					{
+						const double2 w = st[s];
+						const double dx = zg.x - w.x, dy = zg.y - w.y;
+						const double iv = rcp64(fma(dx, dx, dy * dy));
+						ax = fma(dx, iv, ax);
+						ay = fma(dy, iv, ay);
+					}
+				for (int j = lane; j < R; j += 64)
+				{
+					// the own root gives dx = dy = 0 and, floored, a zero term
+					const double2 w = zc[j];
+					const double dx = zg.x - w.x, dy = zg.y - w.y;
+					const double iv = rcp64(fmax(fma(dx, dx, dy * dy), 1e-300));
+					bx = fma(dx, iv, bx);
+					by = fma(dy, iv, by);
+				}
+				double vx = 2 * ax - bx, vy = 2 * ay - by;
+				for (int o = 32; o >= 1; o >>= 1)
+				{
+					vx += __shfl_xor(vx, o, 64);
+					vy += __shfl_xor(vy, o, 64);
+				}
+				if (lane == 0) s_fs[g] = make_double2(vx, -vy);
+			}
+			__syncthreads();
+		}
 		// Cauchy sums 2 sum 1/(z - s_i) - sum_{j != k} 1/(z - z_j) and the update
 		double zx[RPT], zy[RPT], ax[RPT], ay[RPT], bx[RPT], by[RPT];
 		bool any = false;
@@ -194,7 +246,7 @@ __device__ void solve_row(const BField& F, int row, int cap, bool newton, double
 			zy[k] = v.y;
 			ax[k] = ay[k] = bx[k] = by[k] = 0;
 		}
-		if (any)
+		if (any && nf > COOP_MAX)
 		{
 			if (!pole_fmm)
 				for (int s = 0; s < F.N; s++)
@@ -231,7 +283,7 @@ __device__ void solve_row(const BField& F, int row, int cap, bool newton, double
 			moved[k] = false;
 			if (!full[k]) continue;
 			const cd z(zx[k], zy[k]);
-			const cd sum = cd(2 * ax[k] - bx[k], -(2 * ay[k] - by[k])) + ex[k];
+			const cd sum = (nf <= COOP_MAX ? cd(s_fs[slot[k]].x, s_fs[slot[k]].y) : cd(2 * ax[k] - bx[k], -(2 * ay[k] - by[k]))) + ex[k];
 			zn[k] = z - f0[k] / (f1[k] + f0[k] * sum);
 			moved[k] = !((zn[k] - z).abs() / L.theta < CONV_TOL); // a non-finite step stays active
 		}
@@ -250,6 +302,7 @@ __device__ void solve_row(const BField& F, int row, int cap, bool newton, double
 		{
 			s_n[0] = s_n[1];
 			s_n[1] = 0;
+			s_nf = 0;
 		}
 		int* t = cur;
 		cur = nxt;
@@ -749,7 +802,7 @@ int batch_lds_limit(ccf_ctx* m)
 {
 	int v = 0;
 	if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, 0) != hipSuccess || v <= 0) v = 65536;
-	return std::min(v, 65536);
+	return std::min(v, 65536) - BATCH_STATIC_LDS;
 }
 
 // solve, errors, transpose and caustics of every field: captured once into a
