@@ -120,6 +120,12 @@ def set_error(a: np.ndarray, b: np.ndarray, R: int, nb: int, theta: float = 1.0)
     return worst / theta, pairs
 
 
+def relative_error(pairs, theta: float = 1.0) -> float:
+    """max |a - b| / max(|b|, theta_E) over the differing pairs (the parity
+    bar's relative form; pairs within 1e-9 theta_E absolute are within it)"""
+    return max((abs(x - y) / max(abs(y), theta) for _, _, x, y in pairs), default=0.0)
+
+
 def polished_error(starfile: str, run_kw: dict, pairs):
     """Over the differing pairs (a, b), with p = two FP64 Newton steps
     z - f0/f1 on the GPU (the exact root of Witt's equation to rounding):
@@ -222,6 +228,9 @@ def summarize(times, infos):
                 sums_tflops=achieved, sums_flops=inf["flops_sums"], update_flops=inf["flops_update"],
                 witt_ms_total=w_ms, witt_launches=w_launches, witt_ms_avg=w_ms / w_launches,
                 witt_tflops=inf["flops_witt"] / (w_ms * 1e-3) / 1e12 if w_ms > 0 else 0.0, witt_flops=inf["flops_witt"],
+                tail_launches=inf["tail_launches"], tail_sweeps=inf["tail_sweeps"], tail_ms_total=inf["tail_kernel_ms"],
+                tail_flops=inf["flops_tail"],
+                tail_tflops=inf["flops_tail"] / (inf["tail_kernel_ms"] * 1e-3) / 1e12 if inf["tail_kernel_ms"] > 0 else 0.0,
                 expansion_order=inf["expansion_order"], tree_levels=inf["tree_levels"], num_roots=inf["num_roots"],
                 max_error_1_over_mu=inf["max_error"], max_witt_residual=inf["max_witt_residual"])
 
@@ -299,15 +308,26 @@ def bench_batch(args):
     fields = batch_fields(tmp)
     fp64, ccs64 = run_batch(fields, 0, 0, args.steps, args.warmup)
     mixed, ccsm = run_batch(fields, 1, args.predictor, args.steps, args.warmup)
-    # the mixed batch against the FP64 batch, field by field
-    worst = 0.0
+    # the mixed batch against the FP64 batch, field by field; where they
+    # differ, each form's distance to the exact root (two FP64 Newton steps
+    # on the field at the batch's expansion order)
+    worst = rel = pol = da = db = 0.0
     ndiff = 0
+    order = max(mixed["expansion_orders"])
     for (path, kw, n), a, b in zip(fields, ccsm, ccs64):
         err, pairs = set_error(a, b, 2 * n, BATCH_RUN["num_branches"])
         worst = max(worst, err)
+        rel = max(rel, relative_error(pairs))
         ndiff += len(pairs)
+        if pairs:
+            p, x, y = polished_error(path, dict(kw, fmm_order=order, **BATCH_RUN), pairs[:200])
+            pol, da, db = max(pol, p), max(da, x), max(db, y)
     mixed["max_root_err_vs_fp64_thetaE"] = worst
+    mixed["max_root_err_vs_fp64_rel"] = rel
     mixed["points_differing_over_1e-9"] = ndiff
+    mixed["max_root_err_vs_fp64_after_newton_polish_thetaE"] = pol
+    mixed["differing_points_max_dist_to_exact_root_thetaE"] = da
+    mixed["differing_points_fp64_max_dist_to_exact_root_thetaE"] = db
     seq = run_sequential(fields, min(args.seq_fields, len(fields)))
     line = {
         "metric": "converged cc points/s over 256 uneven fields (batched form) on 1 MI355X",
@@ -406,6 +426,7 @@ def main():
                 continue
             err, pairs = set_error(ccs[name], ccs["fp64"], R, run_kw["num_branches"])
             res[name]["max_root_err_vs_fp64_thetaE"] = err
+            res[name]["max_root_err_vs_fp64_rel"] = relative_error(pairs)
             res[name]["points_differing_over_1e-9"] = len(pairs)
             pol, da, db = polished_error(sf, run_kw, pairs[:2000])
             res[name]["max_root_err_vs_fp64_after_newton_polish_thetaE"] = pol
@@ -425,10 +446,17 @@ def main():
                     ("k_sums32 + k_sums32_sparse" if r is res.get("mixed") else "k_sums64"))
             return dict(kernel=name, achieved=r["sums_tflops"], total_ms=r["aberth_ms_total"],
                         avg_launch_ms=r["aberth_ms_avg"], launches=r["aberth_launches"])
+        if which == "tail":
+            n = max(1, r.get("tail_launches", 0))
+            return dict(kernel="k_tail (phase tails: Witt evaluations and Cauchy sums)", achieved=r.get("tail_tflops", 0.0),
+                        total_ms=r.get("tail_ms_total", 0.0), avg_launch_ms=r.get("tail_ms_total", 0.0) / n,
+                        launches=r.get("tail_launches", 0))
         return dict(kernel="k_witt + k_witt_far", achieved=r["witt_tflops"], total_ms=r["witt_ms_total"],
                     avg_launch_ms=r["witt_ms_avg"], launches=r["witt_launches"])
 
     regs = [region(head, "sums"), region(head, "witt")]
+    if head.get("tail_launches"):
+        regs.append(region(head, "tail"))
     dom = max(regs, key=lambda g: g["total_ms"])
     pk = PEAK_FP64_TFLOPS
     roof = {"bound": "mfma",

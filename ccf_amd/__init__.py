@@ -107,6 +107,10 @@ class Info(ctypes.Structure):
         ("witt_launches", ctypes.c_longlong),
         ("flops_witt", ctypes.c_double),
         ("updates_newton", ctypes.c_longlong),
+        ("tail_launches", ctypes.c_longlong),
+        ("tail_sweeps", ctypes.c_longlong),
+        ("tail_kernel_ms", ctypes.c_double),
+        ("flops_tail", ctypes.c_double),
     ]
 
     def as_dict(self):

@@ -99,6 +99,7 @@ void ccf_ctx::free_device()
 	// the events themselves live until ccf_destroy
 	timers.clear();
 	witt_timers.clear();
+	tail_timers.clear();
 	events_used = 0;
 	loaded = prepared = have_roots = have_ccs = have_caustics = have_mu = false;
 	caustics_done = mu_done = false;

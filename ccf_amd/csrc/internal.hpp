@@ -209,6 +209,7 @@ struct ccf_ctx
 	unsigned long long* counters = nullptr; // device statistics, see solve.hip
 	std::vector<ccf::Timer> timers;
 	std::vector<ccf::Timer> witt_timers;
+	std::vector<ccf::Timer> tail_timers;
 	// the timers' events, created once and reused once the timers are read
 	std::vector<hipEvent_t> events;
 	size_t events_used = 0;

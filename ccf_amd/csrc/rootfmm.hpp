@@ -43,6 +43,7 @@ constexpr int ROOT_FMM_MIN_ROOTS = 64;     // smaller rows are always summed dir
 constexpr int ROOT_FMM_INIT_SWEEPS = 24;
 constexpr int ROOT_FMM_STEP_SWEEPS = 2;
 constexpr int ROOT_FMM_PROBE = 8;
+constexpr int OUT_BLOCKS = 8; // workgroups per row summing the roots outside the grid
 
 // listed roots from which a row's sums are cheaper by the FMM (its build
 // costs about as much as R/32 direct updates)
@@ -489,6 +490,7 @@ __global__ __launch_bounds__(BLOCK) void k_rsum(RootView V, const double2* __res
 __global__ __launch_bounds__(BLOCK) void k_rsum_out(RootView V, const double2* __restrict__ roots, RowMap M,
 	unsigned long long* __restrict__ stats)
 {
+	// grid (rows, OUT_BLOCKS): the outliers of a row split over its blocks
 	const int r = blockIdx.x;
 	if (!V.rmode[r]) return;
 	const int no = V.ocnt[r];
@@ -499,7 +501,7 @@ __global__ __launch_bounds__(BLOCK) void k_rsum_out(RootView V, const double2* _
 	M.get(r, row, phi);
 	const double2* zr = roots + (size_t)row * R;
 	__shared__ double red[2][BLOCK / 64];
-	for (int o = 0; o < no; o++)
+	for (int o = blockIdx.y; o < no; o += gridDim.y)
 	{
 		const int j = V.oidx[(size_t)r * R + o];
 		const int s = V.slot[(size_t)r * R + j];
@@ -558,7 +560,7 @@ void root_fmm_pass_p(const SweepBufs& B, const double2* roots, const RowMap& M, 
 	for (int l = 0; l <= g.leaf; l++)
 		k_rdown32<P><<<dim3(grid((g.nx << l) * (g.ny << l), TPB), nrows), 32 * TPB, 0, s>>>(V, l, stats);
 	k_rsum<P><<<dim3(grid(R), nrows), BLOCK, 0, s>>>(V, roots, M, stats);
-	k_rsum_out<<<nrows, BLOCK, 0, s>>>(V, roots, M, stats);
+	k_rsum_out<<<dim3(nrows, OUT_BLOCKS), BLOCK, 0, s>>>(V, roots, M, stats);
 }
 
 // compiled expansion orders; a requested order runs at the next one up

@@ -57,7 +57,12 @@ enum Counter
 	C_RESID = 11,  // max |f0| over the solved points (bits of a non-negative double)
 	C_NEWTON = 12, // mixed form: roots ended by a Newton step below the convergence threshold
 	C_NEAR_EARLY = 13, // near-field stars summed by updates that returned on the 1/mu bound
-	C_COUNT = 16
+	C_TAIL_FULL = 14,  // k_tail: Aberth updates (not in C_FULL)
+	C_TAIL_EARLY = 15, // k_tail: 1/mu exits (not in C_EARLY)
+	C_TAIL_NEWTON = 16, // k_tail: Newton steps (not in C_NEWTON)
+	C_TAIL_NEAR = 17,  // k_tail: near-field stars of its evaluations
+	C_TAIL_SWEEPS = 18, // k_tail: sweeps, summed over rows
+	C_COUNT = 24
 };
 
 namespace
@@ -107,11 +112,11 @@ __device__ inline unsigned long long* stat_slot(unsigned long long* stats, int c
 }
 
 // adds v[k] of every thread of the block to counter which[k]; every thread
-// of a BLOCK-sized workgroup calls this
-template <int NC>
+// of a BS-sized workgroup calls this
+template <int NC, int BS = BLOCK>
 __device__ inline void block_stats(unsigned long long* stats, const int (&which)[NC], unsigned long long (&v)[NC])
 {
-	__shared__ unsigned long long red[NC][BLOCK / 64];
+	__shared__ unsigned long long red[NC][BS / 64];
 	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
 	for (int k = 0; k < NC; k++)
@@ -123,7 +128,7 @@ __device__ inline void block_stats(unsigned long long* stats, const int (&which)
 	if (threadIdx.x < NC)
 	{
 		unsigned long long s = 0;
-		for (int q = 0; q < BLOCK / 64; q++) s += red[threadIdx.x][q];
+		for (int q = 0; q < BS / 64; q++) s += red[threadIdx.x][q];
 		if (s) atomicAdd(stat_slot(stats, which[threadIdx.x]), s);
 	}
 	__syncthreads();
@@ -955,6 +960,146 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(double theta, double2* __res
 	block_stats(stats, which, v);
 }
 
+// The tail of a phase in one launch: once every row has at most TAIL_MAX
+// active roots, one workgroup per row runs the remaining sweeps (at most
+// cap) with the sweep of the kernels above: the Witt evaluation with its
+// early exit and Newton steps (witt_root, roots outside the tree through
+// p2p's star ranges), the Cauchy sums of the listed roots, the Aberth update
+// and the |dz| / theta < 1e-9 test, Jacobi within the row. The sums take one
+// wave per listed root with its lanes over the row's roots (FP32, a close
+// root's term in FP64, the own root skipped by index) and, without the FMM
+// pole sum, the stars. Mixed form only; the active lists live in LDS.
+constexpr int TAIL_MAX = 128;
+constexpr int TAIL_BLOCK = 512;
+
+__global__ __launch_bounds__(TAIL_BLOCK) void k_tail(LensView L, double2* __restrict__ roots, RowMap M, SweepView V,
+	unsigned long long* __restrict__ stats, int circ_exact, int fmm_pole, int cap, float cluster_d2,
+	const float2* __restrict__ spos32, int Np)
+{
+	__shared__ int s_a[2][TAIL_MAX];
+	__shared__ int s_cnt[2]; // listed for the sums this sweep, active for the next
+	__shared__ int s_fl[TAIL_MAX];
+	__shared__ double2 s_f0[TAIL_MAX], s_f1[TAIL_MAX], s_ex[TAIL_MAX], s_sum[TAIL_MAX];
+	constexpr int NW = TAIL_BLOCK / 64;
+	const int r = blockIdx.x, R = M.R;
+	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+	int row;
+	double phi;
+	M.get(r, row, phi);
+	double cp, sp;
+	M.phase(row, phi, cp, sp);
+	double2* __restrict__ xrow = roots + (size_t)row * R;
+	int n = min(V.cnt[r], TAIL_MAX);
+	for (int q = threadIdx.x; q < n; q += TAIL_BLOCK) s_a[0][q] = V.act[(size_t)r * R + q];
+	if (threadIdx.x == 0) s_cnt[0] = s_cnt[1] = 0;
+	__syncthreads();
+	unsigned long long nfull = 0, nearly = 0, nnewton = 0, nconv = 0, nnear = 0, nclus = 0, nsweeps = 0;
+	int p = 0;
+	for (int it = 0; it < cap && n > 0; it++)
+	{
+		nsweeps++;
+		for (int q = threadIdx.x; q < n; q += TAIL_BLOCK)
+		{
+			const int a = s_a[p][q];
+			double2* pz = xrow + a;
+			const cd z(pz->x, pz->y);
+			const int node = nearest_node(L, z);
+			const Witt w = witt_from(L, z, cp, sp, node, p2p(L, node, z));
+			cd f1, ex;
+			const int st = witt_root<true>(L, pz, z, w, circ_exact, fmm_pole, M.j > 0, f1, ex);
+			nnear += w.nr.count;
+			if (st == ROOT_FULL)
+			{
+				const int k = atomicAdd(&s_cnt[0], 1);
+				s_fl[k] = a;
+				s_f0[k] = make_double2(w.f0.re, w.f0.im);
+				s_f1[k] = make_double2(f1.re, f1.im);
+				s_ex[k] = make_double2(ex.re, ex.im);
+				nfull++;
+			}
+			else if (st == ROOT_STEPPED)
+			{
+				s_a[p ^ 1][atomicAdd(&s_cnt[1], 1)] = a;
+				nnewton++;
+			}
+			else
+			{
+				nconv++;
+				if (st == ROOT_NEWTON) nnewton++;
+				else nearly++;
+			}
+		}
+		__syncthreads();
+		const int nf = s_cnt[0];
+		for (int g = wv; g < nf; g += NW)
+		{
+			const int a = s_fl[g];
+			const double2 zg = xrow[a];
+			const float fx = (float)zg.x, fy = (float)zg.y;
+			float ax = 0, ay = 0, bx = 0, by = 0;
+			double cx = 0, cy = 0;
+			for (int q = lane; q < Np; q += 64)
+			{
+				const float2 w = spos32[q];
+				const float dx = fx - w.x, dy = fy - w.y;
+				const float iv = __builtin_amdgcn_rcpf(fmaf(dx, dx, dy * dy));
+				ax = fmaf(dx, iv, ax);
+				ay = fmaf(dy, iv, ay);
+			}
+			for (int q = lane; q < R; q += 64)
+			{
+				const double2 wd = xrow[q];
+				const float dx = fx - (float)wd.x, dy = fy - (float)wd.y;
+				const float d2 = fmaf(dx, dx, dy * dy);
+				if (d2 < cluster_d2 && q != a)
+				{
+					const double ex = zg.x - wd.x, ey = zg.y - wd.y;
+					const double iv = rcp64(fmax(fma(ex, ex, ey * ey), 1e-300));
+					cx = fma(ex, iv, cx);
+					cy = fma(ey, iv, cy);
+					nclus++;
+				}
+				else
+				{
+					// the own root gives dx = dy = 0 and, floored, a zero term
+					const float iv = __builtin_amdgcn_rcpf(fmaxf(d2, 1e-30f));
+					bx = fmaf(dx, iv, bx);
+					by = fmaf(dy, iv, by);
+				}
+			}
+			double vx = 2.0 * (double)ax - ((double)bx + cx), vy = 2.0 * (double)ay - ((double)by + cy);
+			for (int o = 32; o >= 1; o >>= 1)
+			{
+				vx += __shfl_xor(vx, o, 64);
+				vy += __shfl_xor(vy, o, 64);
+			}
+			if (lane == 0) s_sum[g] = make_double2(vx, -vy);
+		}
+		__syncthreads();
+		for (int g = threadIdx.x; g < nf; g += TAIL_BLOCK)
+		{
+			const int a = s_fl[g];
+			const double2 zz = xrow[a];
+			const cd z(zz.x, zz.y), f0(s_f0[g].x, s_f0[g].y), f1(s_f1[g].x, s_f1[g].y);
+			const cd sum = cd(s_sum[g].x, s_sum[g].y) + cd(s_ex[g].x, s_ex[g].y);
+			const cd zn = z - f0 / (f1 + f0 * sum);
+			xrow[a] = make_double2(zn.re, zn.im);
+			if ((zn - z).abs() / L.theta < CONV_TOL) nconv++;
+			else s_a[p ^ 1][atomicAdd(&s_cnt[1], 1)] = a;
+		}
+		__syncthreads();
+		n = s_cnt[1];
+		__syncthreads();
+		if (threadIdx.x == 0) s_cnt[0] = s_cnt[1] = 0;
+		p ^= 1;
+		__syncthreads();
+	}
+	if (threadIdx.x != 0) nsweeps = 0;
+	const int which[8] = {C_TAIL_FULL, C_TAIL_EARLY, C_CONV, C_TAIL_NEAR, C_TAIL_NEWTON, C_CLUSTER, C_TAIL_SWEEPS, C_TAIL_SWEEPS};
+	unsigned long long v[8] = {nfull, nearly, nconv, nnear, nnewton, nclus, nsweeps, 0};
+	block_stats<8, TAIL_BLOCK>(stats, which, v);
+}
+
 // initial roots of a phase: row centre+sgn*j from the rows of the previous
 // steps (prepare_roots_kernel, ccf_functions.cuh:261-295; order > 0
 // extrapolates), every root active, the sweep counters cleared
@@ -1263,6 +1408,31 @@ bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int
 			if (B.grid.order > 0 && amax < root_fmm_min(R)) fmm_mask = 0;
 			ctx->fmm_rows_sweep[0] = ctx->host_counts[ctx->max_rows];
 			ctx->fmm_rows_sweep[1] = ctx->host_counts[ctx->max_rows + 1];
+			// a handful per row left (mixed form, continuation): the rest of
+			// the phase in one launch
+			if (f32 && M.j > 0 && active > 0 && amax <= TAIL_MAX && it + 1 < cap)
+			{
+				const SweepView Vn = view(B, (it + 1) & 1);
+				const double d = CLUSTER_REL * ctx->root_half_length;
+				ccf::Timer tt;
+				if (ctx->P.timing)
+				{
+					tt.a = ctx->next_event();
+					tt.b = ctx->next_event();
+					if (tt.a && tt.b) (void)hipEventRecord(tt.a, s);
+				}
+				k_tail<<<nrows, TAIL_BLOCK, 0, s>>>(L, roots, M, Vn, ctx->counters, circ_exact, fmm_pole ? 1 : 0, cap - (it + 1),
+					(float)(d * d), ctx->spos32, Np);
+				if (ctx->P.timing && tt.a && tt.b)
+				{
+					(void)hipEventRecord(tt.b, s);
+					ctx->tail_timers.push_back(tt);
+				}
+				if (!CCF_LAUNCH(ctx, "find_critical_curve_roots_kernel")) return false;
+				ctx->info.tail_launches++;
+				ctx->phase_sweeps = it + 1;
+				break;
+			}
 			if (active == 0)
 			{
 				ctx->phase_sweeps = it + 1;
@@ -1412,7 +1582,10 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 	// the pass, whose launches cost more than its empty rows, and every
 	// ROOT_FMM_PROBE-th step tries again
 	int prev[2] = {-1, -1};
-	int first = 3; // sweeps the previous step needed
+	// sweeps the previous step needed; the mixed form checks from sweep 2,
+	// where the few roots still active go to the tail launch
+	const int first0 = ctx->P.precision == 1 ? 2 : 3;
+	int first = first0;
 	for (int j = 1; j <= steps; j++)
 	{
 		const RowMap M{R, np, nb, j, 2, 0.0, ctx->phase};
@@ -1430,7 +1603,7 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 		prev[1] = ctx->fmm_rows_sweep[1];
 		// a step checked first at k can end no earlier than k: every
 		// ROOT_FMM_PROBE-th step starts from 3 again
-		first = (j + 1) % ROOT_FMM_PROBE == 1 ? 3 : std::max(2, std::min(ctx->phase_sweeps, 6));
+		first = (j + 1) % ROOT_FMM_PROBE == 1 ? first0 : std::max(2, std::min(ctx->phase_sweeps, 6));
 		if (verbose >= 1 && j * 100 / steps > (j - 1) * 100 / steps)
 		{
 			if (!CCF_HIP(ctx, hipStreamSynchronize(s))) return false;
@@ -1487,9 +1660,10 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 	if (verbose >= 1) std::cout << "Maximum error in 1/mu: " << e << "\n\n";
 	ctx->info.points_total = total;
 	ctx->info.points_converged = (long long)c[C_CONV];
-	ctx->info.updates_full = (long long)c[C_FULL];
-	ctx->info.updates_early = (long long)c[C_EARLY];
-	ctx->info.updates_newton = (long long)c[C_NEWTON];
+	ctx->info.updates_full = (long long)(c[C_FULL] + c[C_TAIL_FULL]);
+	ctx->info.updates_early = (long long)(c[C_EARLY] + c[C_TAIL_EARLY]);
+	ctx->info.updates_newton = (long long)(c[C_NEWTON] + c[C_TAIL_NEWTON]);
+	ctx->info.tail_sweeps = (long long)c[C_TAIL_SWEEPS];
 	ctx->info.cluster_chunks = (long long)c[C_CLUSTER];
 	{
 		// algorithmic flops: 10 per Cauchy term of the sums; per Witt
@@ -1516,6 +1690,11 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 		ctx->info.root_fmm_rows = (long long)c[C_RROWS];
 		ctx->info.root_fmm_order = g.order;
 		ctx->info.root_fmm_leaf_level = g.leaf;
+		// the tails: every evaluation forms f0 and f1 (mixed form), every
+		// full update sums the row (and the stars without the FMM pole sum)
+		const double tail_evals = (double)(c[C_TAIL_FULL] + c[C_TAIL_EARLY] + c[C_TAIL_NEWTON]);
+		ctx->info.flops_tail = 36.0 * (double)c[C_TAIL_NEAR] + tail_evals * (16.0 * p + sm) +
+			(double)c[C_TAIL_FULL] * (10.0 * (Np + R - 1) + (fmm_pole ? 8.0 * p : 0.0));
 		const double ef = ctx->P.precision == 1 ? 1.0 : 0.5; // share of f0 + f1 in an early exit
 		const double early = (double)c[C_EARLY];
 		ctx->info.flops_update = ctx->info.flops_sums + 36.0 * (double)c[C_NEAR] + (full + (double)c[C_NEWTON]) * (16.0 * p + sm) +
@@ -1549,6 +1728,15 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 		ctx->info.witt_kernel_ms = mw;
 		ctx->info.witt_launches = (long long)ctx->witt_timers.size();
 		ctx->witt_timers.clear();
+		double mt = 0;
+		for (auto& t : ctx->tail_timers)
+		{
+			float x = 0;
+			(void)hipEventElapsedTime(&x, t.a, t.b);
+			mt += x;
+		}
+		ctx->info.tail_kernel_ms = mt;
+		ctx->tail_timers.clear();
 		ctx->events_used = 0;
 	}
 	ctx->info.flops_witt = ctx->info.flops_update - ctx->info.flops_sums;

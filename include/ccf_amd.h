@@ -112,6 +112,10 @@ typedef struct ccf_info
 	long long updates_newton;  /* precision 1, continuation steps: FP64 Newton steps z - f0/f1 taken
 	                              instead of an Aberth update (no Cauchy sums) where |f0/f1| < 1e-6 theta;
 	                              below 1e-9 theta the root ends there */
+	long long tail_launches;   /* precision 1: phase tails run in one launch (all counts above include them) */
+	long long tail_sweeps;     /* their sweeps, summed over rows */
+	double tail_kernel_ms;     /* sum of timed tail launches (params.timing = 1) */
+	double flops_tail;         /* algorithmic flops of the tails (Witt evaluations and Cauchy sums) */
 } ccf_info;
 
 /* Arrays readable with ccf_get_array. */
