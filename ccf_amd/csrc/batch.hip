@@ -508,8 +508,36 @@ int rpt_for(int rmax, int bs)
 
 // ---------------------------------------------------------------------------
 
+// one field's stars in its own buffers and at off in the batch's
+struct StarSlice
+{
+	double2* pos;
+	double* mass;
+	int off, n;
+};
+
+// the fields' stars into the batch's arrays (to_fields = 0) or back
+__global__ __launch_bounds__(BLOCK) void k_batch_stars(const StarSlice* __restrict__ sl, double2* pos, double* mass, int to_fields)
+{
+	const StarSlice t = sl[blockIdx.y];
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < t.n; i += gridDim.x * blockDim.x)
+	{
+		if (to_fields)
+		{
+			t.pos[i] = pos[t.off + i];
+			t.mass[i] = mass[t.off + i];
+		}
+		else
+		{
+			pos[t.off + i] = t.pos[i];
+			mass[t.off + i] = t.mass[i];
+		}
+	}
+}
+
 struct BatchDevice
 {
+	StarSlice* slices = nullptr;
 	BField* fields = nullptr;
 	BTask* tasks0 = nullptr;
 	BTask* tasks1 = nullptr;
@@ -523,10 +551,10 @@ struct BatchDevice
 namespace
 {
 
-void free_bdev(BatchDevice& d)
+void free_bdev(ccf_ctx* m, BatchDevice& d)
 {
-	auto F = [](auto*& p) { if (p) (void)hipFree(p); p = nullptr; };
-	F(d.fields); F(d.tasks0); F(d.tasks1); F(d.next); F(d.stats);
+	auto F = [m](auto*& p) { m->dfree(p); p = nullptr; };
+	F(d.slices); F(d.fields); F(d.tasks0); F(d.tasks1); F(d.next); F(d.stats);
 	d = BatchDevice{};
 }
 
@@ -591,7 +619,7 @@ void free_batch_device(ccf_batch* b)
 	b->graph = nullptr;
 	if (b->dev)
 	{
-		free_bdev(*b->dev);
+		free_bdev(b->master, *b->dev);
 		delete b->dev;
 		b->dev = nullptr;
 	}
@@ -605,7 +633,15 @@ bool batch_build(ccf_batch* b, int verbose)
 	ccf_ctx* m = b->master;
 	hipStream_t s = m->stream;
 	const int nf = (int)b->fields.size();
+	const auto tb0 = std::chrono::steady_clock::now();
+	auto lap = [&](const char* what) {
+		if (verbose < 2) return;
+		(void)hipStreamSynchronize(s);
+		std::cout << "batch build: " << what << " at "
+				  << std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tb0).count() << " ms\n";
+	};
 	free_batch_device(b);
+	lap("previous batch freed");
 	b->dev = new BatchDevice();
 	BatchDevice& d = *b->dev;
 	d.nf = nf;
@@ -623,26 +659,33 @@ bool batch_build(ccf_batch* b, int verbose)
 		N += f->num_stars;
 	}
 	m->free_device();
+	lap("master freed");
 	m->num_stars = N;
-	if (!CCF_HIP(m, hipMalloc(&m->spos, (size_t)N * sizeof(double2))) || !CCF_HIP(m, hipMalloc(&m->smass, (size_t)N * sizeof(double))) ||
-		!CCF_HIP(m, hipMalloc(&m->spos_tmp, (size_t)N * sizeof(double2))) || !CCF_HIP(m, hipMalloc(&m->smass_tmp, (size_t)N * sizeof(double))) ||
-		!CCF_HIP(m, hipMalloc(&m->spos32, (size_t)N * sizeof(float2))))
-		return false;
+	std::vector<StarSlice> slices(nf);
+	int nmax_stars = 1;
 	for (int f = 0; f < nf; f++)
 	{
 		ccf_ctx* c = b->fields[f];
-		if (!CCF_HIP(m, hipMemcpyAsync(m->spos + spec.soff[f], c->spos, (size_t)c->num_stars * sizeof(double2), hipMemcpyDeviceToDevice, s)) ||
-			!CCF_HIP(m, hipMemcpyAsync(m->smass + spec.soff[f], c->smass, (size_t)c->num_stars * sizeof(double), hipMemcpyDeviceToDevice, s)))
-			return false;
+		slices[f] = StarSlice{c->spos, c->smass, spec.soff[f], c->num_stars};
+		nmax_stars = std::max(nmax_stars, c->num_stars);
 	}
+	const dim3 g_stars(std::min(grid(nmax_stars), 64), nf);
+	if (!CCF_HIP(m, m->dmalloc(&m->spos, (size_t)N * sizeof(double2))) || !CCF_HIP(m, m->dmalloc(&m->smass, (size_t)N * sizeof(double))) ||
+		!CCF_HIP(m, m->dmalloc(&m->spos_tmp, (size_t)N * sizeof(double2))) || !CCF_HIP(m, m->dmalloc(&m->smass_tmp, (size_t)N * sizeof(double))) ||
+		!CCF_HIP(m, m->dmalloc(&m->spos32, (size_t)N * sizeof(float2))) || !CCF_HIP(m, m->dmalloc(&d.slices, sizeof(StarSlice) * nf)) ||
+		!CCF_HIP(m, hipMemcpyAsync(d.slices, slices.data(), sizeof(StarSlice) * nf, hipMemcpyHostToDevice, s)))
+		return false;
+	k_batch_stars<<<g_stars, BLOCK, 0, s>>>(d.slices, m->spos, m->smass, 0);
+	if (!CCF_LAUNCH(m, "batched stars")) return false;
+	lap("stars gathered");
 	if (!build_forest(m, spec, verbose)) return false;
+	lap("forest built");
 	// tree-order stars back into each field (ccf_stars.bin is written in tree order)
+	k_batch_stars<<<g_stars, BLOCK, 0, s>>>(d.slices, m->spos, m->smass, 1);
+	if (!CCF_LAUNCH(m, "batched stars")) return false;
 	for (int f = 0; f < nf; f++)
 	{
 		ccf_ctx* c = b->fields[f];
-		if (!CCF_HIP(m, hipMemcpyAsync(c->spos, m->spos + spec.soff[f], (size_t)c->num_stars * sizeof(double2), hipMemcpyDeviceToDevice, s)) ||
-			!CCF_HIP(m, hipMemcpyAsync(c->smass, m->smass + spec.soff[f], (size_t)c->num_stars * sizeof(double), hipMemcpyDeviceToDevice, s)))
-			return false;
 		c->expansion_order = spec.order;
 		c->info.tree_levels = spec.levels[f];
 		c->info.expansion_order = m->tree.order;
@@ -650,6 +693,58 @@ bool batch_build(ccf_batch* b, int verbose)
 		c->info.t_tree = m->info.t_tree;
 		c->prepared = true;
 	}
+	lap("stars scattered");
+	// the fields' results: slices of one arena, kept while it is large enough
+	{
+		auto room = [](size_t n) { return (n + 255) / 256 * 256; };
+		size_t need = 0;
+		for (ccf_ctx* c : b->fields)
+		{
+			const size_t pts = (size_t)(c->P.num_phi + c->P.num_branches) * c->num_roots;
+			need += room(pts * sizeof(double2)) + (c->P.write_caustics ? room(pts * sizeof(double2)) : 0) +
+				(c->P.write_mu_length_scales ? room(pts * sizeof(double)) : 0);
+		}
+		for (ccf_ctx* c : b->fields)
+		{
+			// a field's own buffers (from before it was borrowing) go first
+			if (!c->results_borrowed)
+			{
+				c->dfree(c->ccs);
+				c->dfree(c->caustics);
+				c->dfree(c->mu_scales);
+			}
+			c->ccs = c->caustics = nullptr;
+			c->mu_scales = nullptr;
+			c->results_borrowed = false;
+		}
+		if (need > b->results_bytes)
+		{
+			if (b->results) (void)hipFree(b->results);
+			b->results = nullptr;
+			b->results_bytes = 0;
+			if (!CCF_HIP(m, hipMalloc(&b->results, need))) return false;
+			b->results_bytes = need;
+		}
+		char* q = (char*)b->results;
+		for (ccf_ctx* c : b->fields)
+		{
+			const size_t pts = (size_t)(c->P.num_phi + c->P.num_branches) * c->num_roots;
+			c->ccs = (double2*)q;
+			q += room(pts * sizeof(double2));
+			if (c->P.write_caustics)
+			{
+				c->caustics = (double2*)q;
+				q += room(pts * sizeof(double2));
+			}
+			if (c->P.write_mu_length_scales)
+			{
+				c->mu_scales = (double*)q;
+				q += room(pts * sizeof(double));
+			}
+			c->results_borrowed = true;
+		}
+	}
+	lap("result arena");
 	// device tables
 	std::vector<BField> hf(nf);
 	std::vector<BTask> t0, t1;
@@ -687,9 +782,6 @@ bool batch_build(ccf_batch* b, int verbose)
 		}
 		const int R = c->num_roots, np = c->P.num_phi, nb = c->P.num_branches;
 		const size_t pts = (size_t)(np + nb) * R;
-		if (!c->ccs && !CCF_HIP(m, hipMalloc(&c->ccs, pts * sizeof(double2)))) return false;
-		if (c->P.write_caustics && !c->caustics && !CCF_HIP(m, hipMalloc(&c->caustics, pts * sizeof(double2)))) return false;
-		if (c->P.write_mu_length_scales && !c->mu_scales && !CCF_HIP(m, hipMalloc(&c->mu_scales, pts * sizeof(double)))) return false;
 		BField& F = hf[f];
 		F.L = L;
 		F.roots = c->ccs_init;
@@ -739,10 +831,11 @@ bool batch_build(ccf_batch* b, int verbose)
 	// the mixed form takes the pole sum from the star FMM: no star is read
 	// from LDS, which then holds only the row (more workgroups per CU)
 	if (m->P.precision == 1 && m->tree.order > 0) d.nlds = 0;
-	if (!CCF_HIP(m, hipMalloc(&d.stats, sizeof(unsigned long long) * nf * BSTAT))) return false;
+	lap("field tables");
+	if (!CCF_HIP(m, m->dmalloc(&d.stats, sizeof(unsigned long long) * nf * BSTAT))) return false;
 	for (int f = 0; f < nf; f++) hf[f].stats = d.stats + (size_t)f * BSTAT;
-	return CCF_HIP(m, hipMalloc(&d.fields, sizeof(BField) * nf)) && CCF_HIP(m, hipMalloc(&d.tasks0, sizeof(BTask) * d.n0)) &&
-		CCF_HIP(m, hipMalloc(&d.tasks1, sizeof(BTask) * d.n1)) && CCF_HIP(m, hipMalloc(&d.next, sizeof(int))) &&
+	return CCF_HIP(m, m->dmalloc(&d.fields, sizeof(BField) * nf)) && CCF_HIP(m, m->dmalloc(&d.tasks0, sizeof(BTask) * d.n0)) &&
+		CCF_HIP(m, m->dmalloc(&d.tasks1, sizeof(BTask) * d.n1)) && CCF_HIP(m, m->dmalloc(&d.next, sizeof(int))) &&
 		CCF_HIP(m, hipMemcpyAsync(d.fields, hf.data(), sizeof(BField) * nf, hipMemcpyHostToDevice, s)) &&
 		CCF_HIP(m, hipMemcpyAsync(d.tasks0, t0.data(), sizeof(BTask) * d.n0, hipMemcpyHostToDevice, s)) &&
 		CCF_HIP(m, hipMemcpyAsync(d.tasks1, t1.data(), sizeof(BTask) * d.n1, hipMemcpyHostToDevice, s)) &&
@@ -828,6 +921,9 @@ bool batch_solve(ccf_batch* b, int verbose)
 		if (!ok || !CCF_HIP(m, e)) return false;
 		b->graph = g;
 		if (!CCF_HIP(m, hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0))) return false;
+		if (verbose >= 2)
+			std::cout << "batch solve: graph captured and instantiated in "
+					  << std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() << " ms\n";
 	}
 	if (!CCF_HIP(m, hipGraphLaunch(b->graph_exec, s)) || !CCF_HIP(m, hipStreamSynchronize(s))) return false;
 	auto t1 = std::chrono::steady_clock::now();

@@ -82,17 +82,74 @@ bool ccf_ctx::hip_ok(hipError_t e, const char* what, const char* file, int line)
 	return false;
 }
 
+hipError_t ccf_ctx::dalloc(void** p, size_t bytes)
+{
+	*p = nullptr;
+	if (bytes == 0) bytes = 1;
+	const auto it = pool.lower_bound(bytes);
+	if (it != pool.end() && it->first <= bytes + bytes / 4 + (size_t(1) << 16))
+	{
+		*p = it->second;
+		pool.erase(it);
+		return hipSuccess;
+	}
+	hipError_t e = hipMalloc(p, bytes);
+	if (e == hipErrorOutOfMemory && !pool.empty())
+	{
+		(void)hipGetLastError();
+		pool_release();
+		e = hipMalloc(p, bytes);
+	}
+	if (e == hipSuccess) owned[*p] = bytes;
+	return e;
+}
+
+void ccf_ctx::dfree(void* p)
+{
+	if (!p) return;
+	const auto it = owned.find(p);
+	if (it == owned.end())
+	{
+		(void)hipFree(p);
+		return;
+	}
+	pool.emplace(it->second, p);
+}
+
+void ccf_ctx::pool_release()
+{
+	if (pool.empty()) return;
+	// work queued on the stream may still use a pooled buffer
+	if (stream) (void)hipStreamSynchronize(stream);
+	for (auto& [n, p] : pool)
+	{
+		owned.erase(p);
+		(void)hipFree(p);
+	}
+	pool.clear();
+}
+
 void ccf_ctx::free_device()
 {
-	auto F = [](auto*& p) { if (p) hipFree(p); p = nullptr; };
+	auto F = [this](auto*& p) { dfree(p); p = nullptr; };
 	ccf::DeviceTree& T = tree;
 	F(T.center); F(T.hl); F(T.parent); F(T.child); F(T.sstart); F(T.scount);
 	F(T.nbr); F(T.nnbr); F(T.sl); F(T.nsl); F(T.dl); F(T.ndl); F(T.split); F(T.rank);
 	F(T.field); F(T.nf); F(T.nnf); F(T.nbox); F(T.nf_off); F(T.nf_cnt); F(T.nfs); F(T.mp); F(T.loc); F(T.loc32); F(T.mpu); F(T.locu); F(T.locu32); F(T.binom); F(T.lookup);
 	T = ccf::DeviceTree{};
 	F(spos); F(smass); F(spos_tmp); F(smass_tmp); F(spos32); F(ts_a); F(ts_b); F(ones);
-	F(ccs_init); F(ccs); F(caustics); F(mu_scales); F(phase); F(dseeds); F(dorder);
-	ccf::free_sweep(sw);
+	F(ccs_init); F(phase); F(dseeds); F(dorder);
+	if (results_borrowed)
+	{
+		ccs = caustics = nullptr;
+		mu_scales = nullptr;
+		results_borrowed = false;
+	}
+	else
+	{
+		F(ccs); F(caustics); F(mu_scales);
+	}
+	ccf::free_sweep(this, sw);
 	F(counters);
 	if (host_counts) (void)hipHostFree(host_counts);
 	host_counts = nullptr;
@@ -413,9 +470,9 @@ bool populate(ccf_ctx* c, int verbose)
 	ccf_params& P = c->P;
 	const int N = c->num_stars;
 	hipStream_t s = c->stream;
-	if (!CCF_HIP(c, hipMalloc(&c->spos, (size_t)N * sizeof(double2))) || !CCF_HIP(c, hipMalloc(&c->smass, (size_t)N * sizeof(double))) ||
-		!CCF_HIP(c, hipMalloc(&c->spos_tmp, (size_t)N * sizeof(double2))) || !CCF_HIP(c, hipMalloc(&c->smass_tmp, (size_t)N * sizeof(double))) ||
-		!CCF_HIP(c, hipMalloc(&c->spos32, (size_t)N * sizeof(float2))))
+	if (!CCF_HIP(c, c->dmalloc(&c->spos, (size_t)N * sizeof(double2))) || !CCF_HIP(c, c->dmalloc(&c->smass, (size_t)N * sizeof(double))) ||
+		!CCF_HIP(c, c->dmalloc(&c->spos_tmp, (size_t)N * sizeof(double2))) || !CCF_HIP(c, c->dmalloc(&c->smass_tmp, (size_t)N * sizeof(double))) ||
+		!CCF_HIP(c, c->dmalloc(&c->spos32, (size_t)N * sizeof(float2))))
 		return false;
 	if (!c->from_file)
 	{
@@ -541,7 +598,7 @@ bool populate(ccf_ctx* c, int verbose)
 		c->nts = (int)a.size();
 		if (c->nts > 0)
 		{
-			if (!CCF_HIP(c, hipMalloc(&c->ts_a, a.size() * sizeof(double2))) || !CCF_HIP(c, hipMalloc(&c->ts_b, b.size() * sizeof(double2))) ||
+			if (!CCF_HIP(c, c->dmalloc(&c->ts_a, a.size() * sizeof(double2))) || !CCF_HIP(c, c->dmalloc(&c->ts_b, b.size() * sizeof(double2))) ||
 				!CCF_HIP(c, hipMemcpy(c->ts_a, a.data(), a.size() * sizeof(double2), hipMemcpyHostToDevice)) ||
 				!CCF_HIP(c, hipMemcpy(c->ts_b, b.data(), b.size() * sizeof(double2), hipMemcpyHostToDevice)))
 				return false;
@@ -595,7 +652,9 @@ void ccf_destroy(ccf_ctx* c)
 {
 	if (!c) return;
 	c->free_device();
+	c->pool_release();
 	for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
+	if (c->readback) (void)hipEventDestroy(c->readback);
 	if (c->stream && c->own_stream) hipStreamDestroy(c->stream);
 	delete c;
 }
@@ -966,6 +1025,7 @@ void ccf_batch_destroy(ccf_batch* b)
 	if (!b) return;
 	ccf::free_batch_device(b);
 	for (ccf_ctx* c : b->fields) ccf_destroy(c);
+	if (b->results) (void)hipFree(b->results);
 	ccf_destroy(b->master);
 	delete b;
 }

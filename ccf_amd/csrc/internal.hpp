@@ -8,7 +8,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <sstream>
+#include <unordered_map>
 #include <string>
 #include <vector>
 
@@ -224,12 +226,14 @@ struct ccf_ctx
 		return events[events_used++];
 	}
 	int* host_counts = nullptr;   // pinned: active roots per row, then the two FMM row counts, read back between sweeps
+	hipEvent_t readback = nullptr; // recorded after the read-back copies when work is queued behind them (no timing)
 	int fmm_rows_sweep[2] = {-1, -1}; // root-sum FMM rows of the last phase's first and second sweep (-1: unknown)
 	int phase_sweeps = 0;         // sweeps the last phase ran (its cap when no read-back saw it end)
 	bool caustics_done = false;   // caustics / length scales formed by the error pass of find_ccs
 	bool mu_done = false;
 	double2* phase = nullptr;     // (cos phi, sin phi) of every phi row
 	bool batch_member = false;    // a field of a ccf_batch: no sweep buffers, seeds on the device
+	bool results_borrowed = false; // ccs, caustics and mu_scales are slices of the batch's result arena (not freed here)
 	bool own_stream = true;       // ccf_destroy destroys the stream
 	double2* dseeds = nullptr;    // [R] initial roots (batch members)
 	int* dorder = nullptr;        // [R] processing order of the roots
@@ -238,6 +242,26 @@ struct ccf_ctx
 	void fail(const std::string& msg);
 	bool hip_ok(hipError_t e, const char* what, const char* file, int line);
 	void free_device();
+	// Device memory of this context. Buffers that free_device or a resize
+	// gives up stay in a pool and serve later requests of about the same
+	// size (at most a quarter plus 64 KiB larger), so that rebuilding a
+	// field or a batch of the same shape allocates nothing; pool_release
+	// (also on an allocation that fails) and ccf_destroy return them.
+	// Reuse is ordered by the context's stream, which every user of these
+	// buffers works on.
+	hipError_t dalloc(void** p, size_t bytes);
+	template <typename T>
+	hipError_t dmalloc(T** p, size_t bytes)
+	{
+		void* v = nullptr;
+		const hipError_t e = dalloc(&v, bytes);
+		*p = (T*)v;
+		return e;
+	}
+	void dfree(void* p);
+	void pool_release();
+	std::multimap<size_t, void*> pool;        // free buffers by capacity
+	std::unordered_map<void*, size_t> owned;  // every buffer from dalloc, by capacity
 };
 
 namespace ccf
@@ -256,6 +280,8 @@ struct ccf_batch
 	int block = 0, block1 = 0;      // autotuned workgroup sizes of the branch-centre and phi-chain phases
 	int grid0 = 0, grid1 = 0;       // persistent grids of the two phases
 	std::vector<float> tuned_ms;
+	void* results = nullptr;        // the fields' ccs, caustics and length scales, one allocation kept across builds
+	size_t results_bytes = 0;
 	ccf_info info{};
 	bool loaded = false, built = false, solved = false;
 };
@@ -280,7 +306,7 @@ struct ForestSpec
 bool build_forest(ccf_ctx* ctx, ForestSpec& spec, int verbose);
 // solve.hip
 bool alloc_sweep(ccf_ctx* ctx, SweepBufs& b, int rows, int R);
-void free_sweep(SweepBufs& b);
+void free_sweep(ccf_ctx* ctx, SweepBufs& b);
 bool alloc_solver(ccf_ctx* ctx);
 bool find_initial_roots_dev(ccf_ctx* ctx, int verbose);
 bool find_ccs_dev(ccf_ctx* ctx, int verbose);

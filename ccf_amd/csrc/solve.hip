@@ -960,9 +960,10 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(double theta, double2* __res
 	block_stats(stats, which, v);
 }
 
-// The tail of a phase in one launch: once every row has at most TAIL_MAX
-// active roots, one workgroup per row runs the remaining sweeps (at most
-// cap) with the sweep of the kernels above: the Witt evaluation with its
+// The tail of a phase in one launch: every row with at most TAIL_MAX active
+// roots gets one workgroup that runs its remaining sweeps (at most cap) and
+// then empties the row's list; rows with more return at once and stay with
+// the sweep kernels. A workgroup does the sweep of the kernels above: the Witt evaluation with its
 // early exit and Newton steps (witt_root, roots outside the tree through
 // p2p's star ranges), the Cauchy sums of the listed roots, the Aberth update
 // and the |dz| / theta < 1e-9 test, Jacobi within the row. The sums take one
@@ -989,7 +990,10 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_tail(LensView L, double2* __rest
 	double cp, sp;
 	M.phase(row, phi, cp, sp);
 	double2* __restrict__ xrow = roots + (size_t)row * R;
-	int n = min(V.cnt[r], TAIL_MAX);
+	int n = V.cnt[r];
+	// a row with more active roots stays with the sweep kernels (uniform
+	// over the block)
+	if (n > TAIL_MAX) return;
 	for (int q = threadIdx.x; q < n; q += TAIL_BLOCK) s_a[0][q] = V.act[(size_t)r * R + q];
 	if (threadIdx.x == 0) s_cnt[0] = s_cnt[1] = 0;
 	__syncthreads();
@@ -1094,6 +1098,9 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_tail(LensView L, double2* __rest
 		p ^= 1;
 		__syncthreads();
 	}
+	// the row is finished (or out of sweeps): sweeps that follow this launch
+	// for other rows find its list empty
+	if (threadIdx.x == 0) V.cnt[r] = 0;
 	if (threadIdx.x != 0) nsweeps = 0;
 	const int which[8] = {C_TAIL_FULL, C_TAIL_EARLY, C_CONV, C_TAIL_NEAR, C_TAIL_NEWTON, C_CLUSTER, C_TAIL_SWEEPS, C_TAIL_SWEEPS};
 	unsigned long long v[8] = {nfull, nearly, nconv, nnear, nnewton, nclus, nsweeps, 0};
@@ -1393,25 +1400,18 @@ bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int
 		if (ctx->host_counts && nrows <= ctx->max_rows && sweep_check(it + 1, cap, first_check))
 		{
 			if (!CCF_HIP(ctx, hipMemcpyAsync(ctx->host_counts, V.cnt_next, nrows * sizeof(int), hipMemcpyDeviceToHost, s)) ||
-				!CCF_HIP(ctx, hipMemcpyAsync(ctx->host_counts + ctx->max_rows, B.plan + 2, 2 * sizeof(int), hipMemcpyDeviceToHost, s)) ||
-				!CCF_HIP(ctx, hipStreamSynchronize(s)))
+				!CCF_HIP(ctx, hipMemcpyAsync(ctx->host_counts + ctx->max_rows, B.plan + 2, 2 * sizeof(int), hipMemcpyDeviceToHost, s)))
 				return false;
-			long long active = 0;
-			int amax = 0;
-			for (int r = 0; r < nrows; r++)
+			// mixed form, continuation: the tail launch goes in behind the
+			// copies before the host has seen them. It finishes every row with
+			// at most TAIL_MAX active roots and leaves the others to the
+			// sweeps above; the host waits for the copies only, so that its
+			// decision overlaps the tail.
+			const bool tail = f32 && M.j > 0 && it + 1 < cap;
+			if (tail)
 			{
-				active += ctx->host_counts[r];
-				amax = std::max(amax, ctx->host_counts[r]);
-			}
-			// a row's listed roots are among its active ones: once no row has
-			// as many as the FMM takes, the remaining sweeps skip its pass
-			if (B.grid.order > 0 && amax < root_fmm_min(R)) fmm_mask = 0;
-			ctx->fmm_rows_sweep[0] = ctx->host_counts[ctx->max_rows];
-			ctx->fmm_rows_sweep[1] = ctx->host_counts[ctx->max_rows + 1];
-			// a handful per row left (mixed form, continuation): the rest of
-			// the phase in one launch
-			if (f32 && M.j > 0 && active > 0 && amax <= TAIL_MAX && it + 1 < cap)
-			{
+				if (!ctx->readback && !CCF_HIP(ctx, hipEventCreateWithFlags(&ctx->readback, hipEventDisableTiming))) return false;
+				if (!CCF_HIP(ctx, hipEventRecord(ctx->readback, s))) return false;
 				const SweepView Vn = view(B, (it + 1) & 1);
 				const double d = CLUSTER_REL * ctx->root_half_length;
 				ccf::Timer tt;
@@ -1430,6 +1430,24 @@ bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int
 				}
 				if (!CCF_LAUNCH(ctx, "find_critical_curve_roots_kernel")) return false;
 				ctx->info.tail_launches++;
+			}
+			if (!CCF_HIP(ctx, tail ? hipEventSynchronize(ctx->readback) : hipStreamSynchronize(s))) return false;
+			long long active = 0;
+			int amax = 0;
+			for (int r = 0; r < nrows; r++)
+			{
+				active += ctx->host_counts[r];
+				amax = std::max(amax, ctx->host_counts[r]);
+			}
+			// a row's listed roots are among its active ones: once no row has
+			// as many as the FMM takes, the remaining sweeps skip its pass
+			if (B.grid.order > 0 && amax < root_fmm_min(R)) fmm_mask = 0;
+			ctx->fmm_rows_sweep[0] = ctx->host_counts[ctx->max_rows];
+			ctx->fmm_rows_sweep[1] = ctx->host_counts[ctx->max_rows + 1];
+			// every row had at most TAIL_MAX active roots: the tail launch
+			// finishes the phase
+			if (tail && amax <= TAIL_MAX)
+			{
 				ctx->phase_sweeps = it + 1;
 				break;
 			}
@@ -1464,14 +1482,14 @@ bool alloc_root_fmm(ccf_ctx* ctx, SweepBufs& b)
 	b.grid = plan_root_grid(hx, hy, R, order, leaf);
 	const RootGrid& g = b.grid;
 	const size_t ce = (size_t)rows * g.ncells * (g.order + 1);
-	return CCF_HIP(ctx, hipMalloc(&b.slot, list * sizeof(int))) && CCF_HIP(ctx, hipMalloc(&b.rmode, rows * sizeof(int))) &&
-		CCF_HIP(ctx, hipMalloc(&b.rsum, list * sizeof(double2))) &&
-		CCF_HIP(ctx, hipMalloc(&b.ccnt, (size_t)rows * g.ncells * sizeof(int))) &&
-		CCF_HIP(ctx, hipMalloc(&b.cstart, (size_t)rows * (g.nleaf + 1) * sizeof(int))) &&
-		CCF_HIP(ctx, hipMalloc(&b.ccur, (size_t)rows * g.nleaf * sizeof(int))) && CCF_HIP(ctx, hipMalloc(&b.cell, list * sizeof(int))) &&
-		CCF_HIP(ctx, hipMalloc(&b.sidx, list * sizeof(int))) && CCF_HIP(ctx, hipMalloc(&b.spos, list * sizeof(double2))) &&
-		CCF_HIP(ctx, hipMalloc(&b.ocnt, rows * sizeof(int))) && CCF_HIP(ctx, hipMalloc(&b.oidx, list * sizeof(int))) &&
-		CCF_HIP(ctx, hipMalloc(&b.mp, ce * sizeof(double2))) && CCF_HIP(ctx, hipMalloc(&b.lc, ce * sizeof(double2))) &&
+	return CCF_HIP(ctx, ctx->dmalloc(&b.slot, list * sizeof(int))) && CCF_HIP(ctx, ctx->dmalloc(&b.rmode, rows * sizeof(int))) &&
+		CCF_HIP(ctx, ctx->dmalloc(&b.rsum, list * sizeof(double2))) &&
+		CCF_HIP(ctx, ctx->dmalloc(&b.ccnt, (size_t)rows * g.ncells * sizeof(int))) &&
+		CCF_HIP(ctx, ctx->dmalloc(&b.cstart, (size_t)rows * (g.nleaf + 1) * sizeof(int))) &&
+		CCF_HIP(ctx, ctx->dmalloc(&b.ccur, (size_t)rows * g.nleaf * sizeof(int))) && CCF_HIP(ctx, ctx->dmalloc(&b.cell, list * sizeof(int))) &&
+		CCF_HIP(ctx, ctx->dmalloc(&b.sidx, list * sizeof(int))) && CCF_HIP(ctx, ctx->dmalloc(&b.spos, list * sizeof(double2))) &&
+		CCF_HIP(ctx, ctx->dmalloc(&b.ocnt, rows * sizeof(int))) && CCF_HIP(ctx, ctx->dmalloc(&b.oidx, list * sizeof(int))) &&
+		CCF_HIP(ctx, ctx->dmalloc(&b.mp, ce * sizeof(double2))) && CCF_HIP(ctx, ctx->dmalloc(&b.lc, ce * sizeof(double2))) &&
 		CCF_HIP(ctx, hipMemsetAsync(b.slot, 0xff, list * sizeof(int), ctx->stream)) &&
 		CCF_HIP(ctx, hipMemsetAsync(b.rmode, 0, rows * sizeof(int), ctx->stream)) &&
 		CCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -1479,9 +1497,9 @@ bool alloc_root_fmm(ccf_ctx* ctx, SweepBufs& b)
 
 } // namespace
 
-void free_sweep(SweepBufs& b)
+void free_sweep(ccf_ctx* ctx, SweepBufs& b)
 {
-	auto F = [](auto*& p) { if (p) (void)hipFree(p); p = nullptr; };
+	auto F = [ctx](auto*& p) { ctx->dfree(p); p = nullptr; };
 	F(b.act); F(b.cnt); F(b.fl); F(b.fcnt); F(b.dl); F(b.dcnt); F(b.f0); F(b.f1); F(b.ex); F(b.tile_off); F(b.plan); F(b.part);
 	F(b.slot); F(b.rmode); F(b.rsum); F(b.ccnt); F(b.cstart); F(b.ccur); F(b.cell); F(b.sidx); F(b.spos); F(b.ocnt);
 	F(b.oidx); F(b.mp); F(b.lc);
@@ -1490,7 +1508,7 @@ void free_sweep(SweepBufs& b)
 
 bool alloc_sweep(ccf_ctx* ctx, SweepBufs& b, int rows, int R)
 {
-	free_sweep(b);
+	free_sweep(ctx, b);
 	const SumsLaunch sl = sums_launch(ctx, ctx->P.precision == 1);
 	b.rows = rows;
 	b.R = R;
@@ -1499,14 +1517,14 @@ bool alloc_sweep(ccf_ctx* ctx, SweepBufs& b, int rows, int R)
 	const long long tiles_max = (long long)rows * ((R + b.tile - 1) / b.tile);
 	b.max_items = (int)(tiles_max + sl.target + tiles_max);
 	const size_t list = (size_t)rows * R;
-	const bool ok = CCF_HIP(ctx, hipMalloc(&b.act, 2 * list * sizeof(int))) && CCF_HIP(ctx, hipMalloc(&b.cnt, 2 * rows * sizeof(int))) &&
-		CCF_HIP(ctx, hipMalloc(&b.fl, list * sizeof(int))) && CCF_HIP(ctx, hipMalloc(&b.fcnt, 2 * rows * sizeof(int))) &&
-		CCF_HIP(ctx, hipMalloc(&b.dl, list * sizeof(int))) && CCF_HIP(ctx, hipMalloc(&b.dcnt, 2 * rows * sizeof(int))) &&
+	const bool ok = CCF_HIP(ctx, ctx->dmalloc(&b.act, 2 * list * sizeof(int))) && CCF_HIP(ctx, ctx->dmalloc(&b.cnt, 2 * rows * sizeof(int))) &&
+		CCF_HIP(ctx, ctx->dmalloc(&b.fl, list * sizeof(int))) && CCF_HIP(ctx, ctx->dmalloc(&b.fcnt, 2 * rows * sizeof(int))) &&
+		CCF_HIP(ctx, ctx->dmalloc(&b.dl, list * sizeof(int))) && CCF_HIP(ctx, ctx->dmalloc(&b.dcnt, 2 * rows * sizeof(int))) &&
 		CCF_HIP(ctx, hipMemsetAsync(b.dcnt, 0, 2 * rows * sizeof(int), ctx->stream)) &&
-		CCF_HIP(ctx, hipMalloc(&b.f0, list * sizeof(double2))) && CCF_HIP(ctx, hipMalloc(&b.f1, list * sizeof(double2))) &&
-		CCF_HIP(ctx, hipMalloc(&b.ex, list * sizeof(double2))) && CCF_HIP(ctx, hipMalloc(&b.tile_off, (rows + 1) * sizeof(int))) &&
-		CCF_HIP(ctx, hipMalloc(&b.plan, 4 * sizeof(int))) &&
-		CCF_HIP(ctx, hipMalloc(&b.part, (size_t)b.max_items * b.tile * sizeof(double2))) &&
+		CCF_HIP(ctx, ctx->dmalloc(&b.f0, list * sizeof(double2))) && CCF_HIP(ctx, ctx->dmalloc(&b.f1, list * sizeof(double2))) &&
+		CCF_HIP(ctx, ctx->dmalloc(&b.ex, list * sizeof(double2))) && CCF_HIP(ctx, ctx->dmalloc(&b.tile_off, (rows + 1) * sizeof(int))) &&
+		CCF_HIP(ctx, ctx->dmalloc(&b.plan, 4 * sizeof(int))) &&
+		CCF_HIP(ctx, ctx->dmalloc(&b.part, (size_t)b.max_items * b.tile * sizeof(double2))) &&
 		CCF_HIP(ctx, hipMemsetAsync(b.cnt, 0, 2 * rows * sizeof(int), ctx->stream)) &&
 		CCF_HIP(ctx, hipMemsetAsync(b.fcnt, 0, 2 * rows * sizeof(int), ctx->stream));
 	if (!ok) return false;
@@ -1526,20 +1544,20 @@ bool alloc_solver(ccf_ctx* ctx)
 	const int R = ctx->num_roots, np = ctx->P.num_phi, nb = ctx->P.num_branches;
 	const size_t pts = (size_t)(np + nb) * R;
 	ctx->max_rows = 2 * nb;
-	if (!CCF_HIP(ctx, hipMalloc(&ctx->dorder, (size_t)R * sizeof(int))) ||
+	if (!CCF_HIP(ctx, ctx->dmalloc(&ctx->dorder, (size_t)R * sizeof(int))) ||
 		!CCF_HIP(ctx, hipMemcpyAsync(ctx->dorder, ctx->order.data(), (size_t)R * sizeof(int), hipMemcpyHostToDevice, ctx->stream)))
 		return false;
 	if (ctx->batch_member)
-		return CCF_HIP(ctx, hipMalloc(&ctx->ccs_init, pts * sizeof(double2))) &&
-			CCF_HIP(ctx, hipMalloc(&ctx->phase, (size_t)(np + nb) * sizeof(double2))) && launch_phase(ctx) &&
-			CCF_HIP(ctx, hipMalloc(&ctx->dseeds, (size_t)R * sizeof(double2))) &&
+		return CCF_HIP(ctx, ctx->dmalloc(&ctx->ccs_init, pts * sizeof(double2))) &&
+			CCF_HIP(ctx, ctx->dmalloc(&ctx->phase, (size_t)(np + nb) * sizeof(double2))) && launch_phase(ctx) &&
+			CCF_HIP(ctx, ctx->dmalloc(&ctx->dseeds, (size_t)R * sizeof(double2))) &&
 			CCF_HIP(ctx, hipMemcpyAsync(ctx->dseeds, ctx->seeds.data(), (size_t)R * sizeof(double2), hipMemcpyHostToDevice, ctx->stream)) &&
 			CCF_HIP(ctx, hipMemsetAsync(ctx->ccs_init, 0, pts * sizeof(double2), ctx->stream)) &&
 			CCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
-	return CCF_HIP(ctx, hipMalloc(&ctx->ccs_init, pts * sizeof(double2))) && alloc_sweep(ctx, ctx->sw, ctx->max_rows, R) &&
+	return CCF_HIP(ctx, ctx->dmalloc(&ctx->ccs_init, pts * sizeof(double2))) && alloc_sweep(ctx, ctx->sw, ctx->max_rows, R) &&
 		CCF_HIP(ctx, hipHostMalloc((void**)&ctx->host_counts, (ctx->max_rows + 2) * sizeof(int), hipHostMallocDefault)) &&
-		CCF_HIP(ctx, hipMalloc(&ctx->phase, (size_t)(np + nb) * sizeof(double2))) && launch_phase(ctx) &&
-		CCF_HIP(ctx, hipMalloc(&ctx->counters, STAT_WORDS * sizeof(unsigned long long))) &&
+		CCF_HIP(ctx, ctx->dmalloc(&ctx->phase, (size_t)(np + nb) * sizeof(double2))) && launch_phase(ctx) &&
+		CCF_HIP(ctx, ctx->dmalloc(&ctx->counters, STAT_WORDS * sizeof(unsigned long long))) &&
 		CCF_HIP(ctx, hipMemsetAsync(ctx->ccs_init, 0, pts * sizeof(double2), ctx->stream)) &&
 		CCF_HIP(ctx, hipMemsetAsync(ctx->counters, 0, STAT_WORDS * sizeof(unsigned long long), ctx->stream));
 }
@@ -1624,8 +1642,8 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 	// same evaluation of the field; the caustics stage then has them
 	const bool wc = ctx->P.write_caustics != 0, wm = ctx->P.write_mu_length_scales != 0;
 	ctx->caustics_done = ctx->mu_done = false;
-	if (wc && !ctx->caustics && !CCF_HIP(ctx, hipMalloc(&ctx->caustics, (size_t)total * sizeof(double2)))) return false;
-	if (wm && !ctx->mu_scales && !CCF_HIP(ctx, hipMalloc(&ctx->mu_scales, (size_t)total * sizeof(double)))) return false;
+	if (wc && !ctx->caustics && !CCF_HIP(ctx, ctx->dmalloc(&ctx->caustics, (size_t)total * sizeof(double2)))) return false;
+	if (wm && !ctx->mu_scales && !CCF_HIP(ctx, ctx->dmalloc(&ctx->mu_scales, (size_t)total * sizeof(double)))) return false;
 	k_errors<<<dim3(xcd_round(grid(R)), np + nb), BLOCK, 0, s>>>(L, ctx->ccs_init, R, ctx->phase, ctx->dorder, ctx->counters,
 		wc ? ctx->caustics : nullptr, wm ? ctx->mu_scales : nullptr);
 	if (!CCF_LAUNCH(ctx, "find_errors_kernel")) return false;
@@ -1702,7 +1720,7 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 	}
 
 	// transpose to file order (ccf.cuh:1072-1080)
-	if (!ctx->ccs && !CCF_HIP(ctx, hipMalloc(&ctx->ccs, (size_t)total * sizeof(double2)))) return false;
+	if (!ctx->ccs && !CCF_HIP(ctx, ctx->dmalloc(&ctx->ccs, (size_t)total * sizeof(double2)))) return false;
 	k_transpose<<<dim3(grid(R, 32), grid(np + nb, 32)), BLOCK, 0, s>>>(ctx->ccs_init, np + nb, R, ctx->ccs);
 	if (!CCF_LAUNCH(ctx, "transpose_array_kernel")) return false;
 	if (!CCF_HIP(ctx, hipStreamSynchronize(s))) return false;
@@ -1751,8 +1769,8 @@ bool find_caustics_dev(ccf_ctx* ctx, bool caustics, bool mu)
 	if (ctx->caustics_done) caustics = false;
 	if (ctx->mu_done) mu = false;
 	if (!caustics && !mu) return true;
-	if (caustics && !ctx->caustics && !CCF_HIP(ctx, hipMalloc(&ctx->caustics, (size_t)n * sizeof(double2)))) return false;
-	if (mu && !ctx->mu_scales && !CCF_HIP(ctx, hipMalloc(&ctx->mu_scales, (size_t)n * sizeof(double)))) return false;
+	if (caustics && !ctx->caustics && !CCF_HIP(ctx, ctx->dmalloc(&ctx->caustics, (size_t)n * sizeof(double2)))) return false;
+	if (mu && !ctx->mu_scales && !CCF_HIP(ctx, ctx->dmalloc(&ctx->mu_scales, (size_t)n * sizeof(double)))) return false;
 	k_caustics<<<xcd_round(grid(n)), BLOCK, 0, s>>>(ctx->lens(), ctx->ccs, n, caustics ? ctx->caustics : nullptr, mu ? ctx->mu_scales : nullptr);
 	return CCF_LAUNCH(ctx, "find_caustics_kernel") && CCF_HIP(ctx, hipStreamSynchronize(s));
 }
@@ -1771,7 +1789,7 @@ bool eval_points_dev(ccf_ctx* ctx, const double* z, int n, double phi, double* o
 	double2* dz = nullptr;
 	double* dout = nullptr;
 	hipStream_t s = ctx->stream;
-	bool ok = CCF_HIP(ctx, hipMalloc(&dz, (size_t)n * sizeof(double2))) && CCF_HIP(ctx, hipMalloc(&dout, (size_t)n * 8 * sizeof(double))) &&
+	bool ok = CCF_HIP(ctx, ctx->dmalloc(&dz, (size_t)n * sizeof(double2))) && CCF_HIP(ctx, ctx->dmalloc(&dout, (size_t)n * 8 * sizeof(double))) &&
 		CCF_HIP(ctx, hipMemcpyAsync(dz, z, (size_t)n * sizeof(double2), hipMemcpyHostToDevice, s));
 	if (ok)
 	{
@@ -1779,8 +1797,8 @@ bool eval_points_dev(ccf_ctx* ctx, const double* z, int n, double phi, double* o
 		ok = CCF_LAUNCH(ctx, "eval kernel") && CCF_HIP(ctx, hipMemcpyAsync(out, dout, (size_t)n * 8 * sizeof(double), hipMemcpyDeviceToHost, s)) &&
 			CCF_HIP(ctx, hipStreamSynchronize(s));
 	}
-	(void)hipFree(dz);
-	(void)hipFree(dout);
+	ctx->dfree(dz);
+	ctx->dfree(dout);
 	return ok;
 }
 
@@ -1790,7 +1808,7 @@ bool aberth_sweep_dev(ccf_ctx* ctx, const double* roots, int n, double phi, doub
 	hipStream_t s = ctx->stream;
 	double2* row = nullptr;
 	SweepBufs B;
-	bool ok = CCF_HIP(ctx, hipMalloc(&row, (size_t)n * sizeof(double2))) && alloc_sweep(ctx, B, 1, n) &&
+	bool ok = CCF_HIP(ctx, ctx->dmalloc(&row, (size_t)n * sizeof(double2))) && alloc_sweep(ctx, B, 1, n) &&
 		CCF_HIP(ctx, hipMemcpyAsync(row, roots, (size_t)n * sizeof(double2), hipMemcpyHostToDevice, s));
 	if (ok)
 	{
@@ -1800,8 +1818,8 @@ bool aberth_sweep_dev(ccf_ctx* ctx, const double* roots, int n, double phi, doub
 			CCF_HIP(ctx, hipMemcpyAsync(out, row, (size_t)n * sizeof(double2), hipMemcpyDeviceToHost, s)) &&
 			CCF_HIP(ctx, hipStreamSynchronize(s));
 	}
-	(void)hipFree(row);
-	free_sweep(B);
+	ctx->dfree(row);
+	free_sweep(ctx, B);
 	return ok;
 }
 
@@ -1817,8 +1835,8 @@ bool root_sums_dev(ccf_ctx* ctx, const double* roots, int n, double* out)
 	std::vector<int> iota(n), one{1};
 	for (int i = 0; i < n; i++) iota[i] = i;
 	std::vector<double2> res(n);
-	bool ok = CCF_HIP(ctx, hipMalloc(&row, (size_t)n * sizeof(double2))) && alloc_root_fmm(ctx, B) &&
-		CCF_HIP(ctx, hipMalloc(&stats, STAT_WORDS * sizeof(unsigned long long))) &&
+	bool ok = CCF_HIP(ctx, ctx->dmalloc(&row, (size_t)n * sizeof(double2))) && alloc_root_fmm(ctx, B) &&
+		CCF_HIP(ctx, ctx->dmalloc(&stats, STAT_WORDS * sizeof(unsigned long long))) &&
 		CCF_HIP(ctx, hipMemcpyAsync(row, roots, (size_t)n * sizeof(double2), hipMemcpyHostToDevice, s)) &&
 		CCF_HIP(ctx, hipMemcpyAsync(B.slot, iota.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, s)) &&
 		CCF_HIP(ctx, hipMemcpyAsync(B.rmode, one.data(), sizeof(int), hipMemcpyHostToDevice, s));
@@ -1836,9 +1854,9 @@ bool root_sums_dev(ccf_ctx* ctx, const double* roots, int n, double* out)
 			out[2 * i] = -res[i].x;
 			out[2 * i + 1] = -res[i].y;
 		}
-	(void)hipFree(row);
-	(void)hipFree(stats);
-	free_sweep(B);
+	ctx->dfree(row);
+	ctx->dfree(stats);
+	free_sweep(ctx, B);
 	return ok;
 }
 

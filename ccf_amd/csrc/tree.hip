@@ -509,12 +509,12 @@ template <typename V>
 bool grow(ccf_ctx* ctx, V*& ptr, int old_n, int new_n, int width)
 {
 	V* p = nullptr;
-	if (!CCF_HIP(ctx, hipMalloc(&p, sizeof(V) * (size_t)new_n * width))) return false;
+	if (!CCF_HIP(ctx, ctx->dmalloc(&p, sizeof(V) * (size_t)new_n * width))) return false;
 	if (ptr)
 	{
 		if (old_n > 0 && !CCF_HIP(ctx, hipMemcpyAsync(p, ptr, sizeof(V) * (size_t)old_n * width, hipMemcpyDeviceToDevice, ctx->stream))) return false;
 		if (!CCF_HIP(ctx, hipStreamSynchronize(ctx->stream))) return false;
-		hipFree(ptr);
+		ctx->dfree(ptr);
 	}
 	ptr = p;
 	return true;
@@ -602,7 +602,7 @@ bool build_forest(ccf_ctx* ctx, ForestSpec& spec, int verbose)
 	T.capacity = 0;
 	if (!ensure_capacity(ctx, 4096 + N / 2 + 4 * F)) return false;
 	int* scal = nullptr;
-	if (!CCF_HIP(ctx, hipMalloc(&scal, 4 * sizeof(int)))) return false;
+	if (!CCF_HIP(ctx, ctx->dmalloc(&scal, 4 * sizeof(int)))) return false;
 
 	// root nodes
 	{
@@ -620,7 +620,7 @@ bool build_forest(ccf_ctx* ctx, ForestSpec& spec, int verbose)
 			CCF_HIP(ctx, hipMemcpyAsync(T.ndl, z.data(), F * sizeof(int), hipMemcpyHostToDevice, s)) &&
 			CCF_HIP(ctx, hipMemcpyAsync(T.field, fid.data(), F * sizeof(int), hipMemcpyHostToDevice, s)) &&
 			CCF_HIP(ctx, hipStreamSynchronize(s));
-		if (!ok) { hipFree(scal); return false; }
+		if (!ok) { ctx->dfree(scal); return false; }
 	}
 	T.nnodes = F;
 	T.level_off = {0};
@@ -669,7 +669,7 @@ bool build_forest(ccf_ctx* ctx, ForestSpec& spec, int verbose)
 		T.level_cnt.push_back(new_cnt);
 		T.levels++;
 	}
-	hipFree(scal);
+	ctx->dfree(scal);
 	if (!ok) return false;
 	if (verbose >= 2) std::cout << "tree_levels set to: " << T.levels << "\n\n";
 
@@ -678,19 +678,19 @@ bool build_forest(ccf_ctx* ctx, ForestSpec& spec, int verbose)
 		std::cout << "Done creating children and sorting stars. Elapsed time: " << std::chrono::duration<double>(t1 - t0).count() << " seconds.\n\n";
 
 	// near-field ranges of every node
-	if (!CCF_HIP(ctx, hipMalloc(&T.nf, sizeof(int2) * (size_t)T.nnodes * MAX_NEAR_RANGES))) return false;
-	if (!CCF_HIP(ctx, hipMalloc(&T.nnf, sizeof(int) * (size_t)T.nnodes))) return false;
+	if (!CCF_HIP(ctx, ctx->dmalloc(&T.nf, sizeof(int2) * (size_t)T.nnodes * MAX_NEAR_RANGES))) return false;
+	if (!CCF_HIP(ctx, ctx->dmalloc(&T.nnf, sizeof(int) * (size_t)T.nnodes))) return false;
 	k_near_field<<<grid(T.nnodes), BLOCK, 0, s>>>(T.nnodes, T.sstart, T.scount, T.nbr, T.nnbr, T.nf, T.nnf);
 	if (!CCF_LAUNCH(ctx, "near field ranges")) return false;
 	// packed boxes and contiguous near-field copies of the leaves
 	{
 		const int nn = T.nnodes;
-		if (!CCF_HIP(ctx, hipMalloc(&T.nbox, sizeof(double4) * nn)) || !CCF_HIP(ctx, hipMalloc(&T.nf_off, sizeof(int) * nn)) ||
-			!CCF_HIP(ctx, hipMalloc(&T.nf_cnt, sizeof(int) * nn)))
+		if (!CCF_HIP(ctx, ctx->dmalloc(&T.nbox, sizeof(double4) * nn)) || !CCF_HIP(ctx, ctx->dmalloc(&T.nf_off, sizeof(int) * nn)) ||
+			!CCF_HIP(ctx, ctx->dmalloc(&T.nf_cnt, sizeof(int) * nn)))
 			return false;
 		k_node_box<<<grid(nn), BLOCK, 0, s>>>(nn, T.center, T.hl, T.child, T.nbox);
 		// point-location table over the root box (centre 0, half-length root_half_length)
-		if (T.lookup) (void)hipFree(T.lookup);
+		ctx->dfree(T.lookup);
 		T.lookup = nullptr;
 		T.lookup_level = 0;
 		if (F == 1 && T.levels >= 2)
@@ -700,7 +700,7 @@ bool build_forest(ccf_ctx* ctx, ForestSpec& spec, int verbose)
 			const double H = spec.root_hl[0];
 			T.lookup_x0 = -H;
 			T.lookup_inv = n / (2 * H);
-			if (!CCF_HIP(ctx, hipMalloc(&T.lookup, sizeof(int) * (size_t)n * n))) return false;
+			if (!CCF_HIP(ctx, ctx->dmalloc(&T.lookup, sizeof(int) * (size_t)n * n))) return false;
 			k_lookup<<<grid((long long)n * n), BLOCK, 0, s>>>(T.lookup_level, 0, -H, 2 * H / n, T.nbox, T.lookup);
 		}
 		k_near_count<<<grid(nn), BLOCK, 0, s>>>(nn, T.child, T.nf, T.nnf, T.nf_cnt);
@@ -716,7 +716,7 @@ bool build_forest(ccf_ctx* ctx, ForestSpec& spec, int verbose)
 			off[n] = child[n] < 0 ? (int)tot : -1;
 			tot += cnt[n];
 		}
-		if (!CCF_HIP(ctx, hipMalloc(&T.nfs, sizeof(double4) * (size_t)std::max(tot, 1LL))) ||
+		if (!CCF_HIP(ctx, ctx->dmalloc(&T.nfs, sizeof(double4) * (size_t)std::max(tot, 1LL))) ||
 			!CCF_HIP(ctx, hipMemcpyAsync(T.nf_off, off.data(), sizeof(int) * nn, hipMemcpyHostToDevice, s)))
 			return false;
 		k_near_fill<<<grid(nn), BLOCK, 0, s>>>(nn, T.nf_off, T.nf, T.nnf, ctx->spos, ctx->smass, T.nfs);
@@ -748,7 +748,7 @@ bool build_forest(ccf_ctx* ctx, ForestSpec& spec, int verbose)
 		}
 		if (tot)
 		{
-			if (!CCF_HIP(ctx, hipMalloc(&T.lookup, sizeof(int) * tot))) return false;
+			if (!CCF_HIP(ctx, ctx->dmalloc(&T.lookup, sizeof(int) * tot))) return false;
 			for (int f = 0; f < F; f++)
 			{
 				const int lv = spec.lookup_level[f];
@@ -783,15 +783,15 @@ bool build_forest(ccf_ctx* ctx, ForestSpec& spec, int verbose)
 	// FMM: upward from the deepest level, downward from level 2 (ccf.cuh:866-893)
 	if (verbose >= 1) std::cout << "Calculating multipole and local coefficients...\n";
 	const int P1 = T.order + 1;
-	if (!CCF_HIP(ctx, hipMalloc(&T.mp, sizeof(double2) * (size_t)T.nnodes * P1))) return false;
-	if (!CCF_HIP(ctx, hipMalloc(&T.loc, sizeof(double2) * (size_t)T.nnodes * P1))) return false;
-	if (!CCF_HIP(ctx, hipMalloc(&T.loc32, sizeof(float2) * (size_t)T.nnodes * P1))) return false;
+	if (!CCF_HIP(ctx, ctx->dmalloc(&T.mp, sizeof(double2) * (size_t)T.nnodes * P1))) return false;
+	if (!CCF_HIP(ctx, ctx->dmalloc(&T.loc, sizeof(double2) * (size_t)T.nnodes * P1))) return false;
+	if (!CCF_HIP(ctx, ctx->dmalloc(&T.loc32, sizeof(float2) * (size_t)T.nnodes * P1))) return false;
 	if (!CCF_HIP(ctx, hipMemsetAsync(T.mp, 0, sizeof(double2) * (size_t)T.nnodes * P1, s))) return false;
 	if (!CCF_HIP(ctx, hipMemsetAsync(T.loc, 0, sizeof(double2) * (size_t)T.nnodes * P1, s))) return false;
 	if (T.order > 0)
 	{
 		std::vector<double> B = binomial_table(2 * p, P.binomial_exact != 0);
-		if (!CCF_HIP(ctx, hipMalloc(&T.binom, sizeof(double) * B.size()))) return false;
+		if (!CCF_HIP(ctx, ctx->dmalloc(&T.binom, sizeof(double) * B.size()))) return false;
 		if (!CCF_HIP(ctx, hipMemcpyAsync(T.binom, B.data(), sizeof(double) * B.size(), hipMemcpyHostToDevice, s))) return false;
 		for (int l = T.levels; l >= 0; l--)
 		{
@@ -814,11 +814,11 @@ bool build_forest(ccf_ctx* ctx, ForestSpec& spec, int verbose)
 		// the same passes with unit masses: local expansions of sum log(z - s_i),
 		// whose first derivative is the far part of the Aberth pole sum
 		std::vector<double> one(N, 1.0);
-		bool ok2 = CCF_HIP(ctx, hipMalloc(&ctx->ones, sizeof(double) * N)) &&
+		bool ok2 = CCF_HIP(ctx, ctx->dmalloc(&ctx->ones, sizeof(double) * N)) &&
 			CCF_HIP(ctx, hipMemcpyAsync(ctx->ones, one.data(), sizeof(double) * N, hipMemcpyHostToDevice, s)) &&
-			CCF_HIP(ctx, hipMalloc(&T.mpu, sizeof(double2) * (size_t)T.nnodes * P1)) &&
-			CCF_HIP(ctx, hipMalloc(&T.locu, sizeof(double2) * (size_t)T.nnodes * P1)) &&
-			CCF_HIP(ctx, hipMalloc(&T.locu32, sizeof(float2) * (size_t)T.nnodes * P1)) &&
+			CCF_HIP(ctx, ctx->dmalloc(&T.mpu, sizeof(double2) * (size_t)T.nnodes * P1)) &&
+			CCF_HIP(ctx, ctx->dmalloc(&T.locu, sizeof(double2) * (size_t)T.nnodes * P1)) &&
+			CCF_HIP(ctx, ctx->dmalloc(&T.locu32, sizeof(float2) * (size_t)T.nnodes * P1)) &&
 			CCF_HIP(ctx, hipMemsetAsync(T.mpu, 0, sizeof(double2) * (size_t)T.nnodes * P1, s)) &&
 			CCF_HIP(ctx, hipMemsetAsync(T.locu, 0, sizeof(double2) * (size_t)T.nnodes * P1, s));
 		if (!ok2) return false;
