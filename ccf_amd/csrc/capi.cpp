@@ -654,8 +654,9 @@ void ccf_destroy(ccf_ctx* c)
 	c->free_device();
 	c->pool_release();
 	for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
-	if (c->readback) (void)hipEventDestroy(c->readback);
-	if (c->stream && c->own_stream) hipStreamDestroy(c->stream);
+	for (hipEvent_t e : c->fmm_rows_event)
+		if (e) (void)hipEventDestroy(e);
+	if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
 	delete c;
 }
 

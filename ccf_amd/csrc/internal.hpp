@@ -225,8 +225,8 @@ struct ccf_ctx
 		}
 		return events[events_used++];
 	}
-	int* host_counts = nullptr;   // pinned: active roots per row, then the two FMM row counts, read back between sweeps
-	hipEvent_t readback = nullptr; // recorded after the read-back copies when work is queued behind them (no timing)
+	int* host_counts = nullptr;   // pinned: active roots per row, the two FMM row counts of a read-back, then two slots of two for the mixed continuation
+	hipEvent_t fmm_rows_event[2] = {nullptr, nullptr}; // mixed continuation: after the copy of a step's FMM row count (no timing)
 	int fmm_rows_sweep[2] = {-1, -1}; // root-sum FMM rows of the last phase's first and second sweep (-1: unknown)
 	int phase_sweeps = 0;         // sweeps the last phase ran (its cap when no read-back saw it end)
 	bool caustics_done = false;   // caustics / length scales formed by the error pass of find_ccs

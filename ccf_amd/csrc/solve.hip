@@ -960,27 +960,35 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(double theta, double2* __res
 	block_stats(stats, which, v);
 }
 
-// The tail of a phase in one launch: every row with at most TAIL_MAX active
-// roots gets one workgroup that runs its remaining sweeps (at most cap) and
-// then empties the row's list; rows with more return at once and stay with
-// the sweep kernels. A workgroup does the sweep of the kernels above: the Witt evaluation with its
-// early exit and Newton steps (witt_root, roots outside the tree through
-// p2p's star ranges), the Cauchy sums of the listed roots, the Aberth update
-// and the |dz| / theta < 1e-9 test, Jacobi within the row. The sums take one
-// wave per listed root with its lanes over the row's roots (FP32, a close
-// root's term in FP64, the own root skipped by index) and, without the FMM
-// pole sum, the stars. Mixed form only; the active lists live in LDS.
-constexpr int TAIL_MAX = 128;
+// The sweeps after the first few of a continuation step in one launch
+// (mixed form): one workgroup per row runs the row's remaining sweeps (at
+// most cap) with the sweep of the kernels above: the Witt evaluation with
+// its early exit and Newton steps (witt_root, roots outside the tree
+// through p2p's star ranges), the Cauchy sums of the listed roots, the
+// Aberth update and the |dz| / theta < 1e-9 test, Jacobi within the row.
+// The sums take one wave per listed root with its lanes over the row's
+// roots (FP32, a close root's term in FP64, the own root skipped by index)
+// and, without the FMM pole sum, the stars. The active lists ping-pong
+// between two buffers, and the listed roots carry f0, f1 and their Aberth
+// terms; entry k of each lives in LDS for k < TAIL_LDS and otherwise in the
+// row's slice of act (both halves), fl, f0, f1 and ex (the sweep kernels are
+// done with them), so a row of any size fits. The waves of the workgroup
+// see each other's global writes across its barriers.
 constexpr int TAIL_BLOCK = 512;
+constexpr int TAIL_LDS = 128;
+// sweeps of a continuation step before the tail launch: the second sweep
+// still lists hundreds of roots per row, which the whole device takes
+// faster than one workgroup per row
+constexpr int TAIL_AT = 2;
 
 __global__ __launch_bounds__(TAIL_BLOCK) void k_tail(LensView L, double2* __restrict__ roots, RowMap M, SweepView V,
 	unsigned long long* __restrict__ stats, int circ_exact, int fmm_pole, int cap, float cluster_d2,
 	const float2* __restrict__ spos32, int Np)
 {
-	__shared__ int s_a[2][TAIL_MAX];
-	__shared__ int s_cnt[2]; // listed for the sums this sweep, active for the next
-	__shared__ int s_fl[TAIL_MAX];
-	__shared__ double2 s_f0[TAIL_MAX], s_f1[TAIL_MAX], s_ex[TAIL_MAX], s_sum[TAIL_MAX];
+	__shared__ int s_a[2][TAIL_LDS];
+	__shared__ int s_fl[TAIL_LDS];
+	__shared__ double2 s_f0[TAIL_LDS], s_f1[TAIL_LDS], s_ex[TAIL_LDS];
+	__shared__ int s_nf, s_nn; // listed for the sums this sweep, active for the next
 	constexpr int NW = TAIL_BLOCK / 64;
 	const int r = blockIdx.x, R = M.R;
 	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -990,12 +998,16 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_tail(LensView L, double2* __rest
 	double cp, sp;
 	M.phase(row, phi, cp, sp);
 	double2* __restrict__ xrow = roots + (size_t)row * R;
+	const size_t base = (size_t)r * R;
+	int* const ga[2] = {const_cast<int*>(V.act) + base, V.act_next + base};
+	int* __restrict__ gfl = V.fl + base;
+	double2* __restrict__ gf0 = V.f0 + base;
+	double2* __restrict__ gf1 = V.f1 + base;
+	double2* __restrict__ gex = V.ex + base;
 	int n = V.cnt[r];
-	// a row with more active roots stays with the sweep kernels (uniform
-	// over the block)
-	if (n > TAIL_MAX) return;
-	for (int q = threadIdx.x; q < n; q += TAIL_BLOCK) s_a[0][q] = V.act[(size_t)r * R + q];
-	if (threadIdx.x == 0) s_cnt[0] = s_cnt[1] = 0;
+	// the first list's head into LDS; its other entries stay where they are
+	for (int q = threadIdx.x; q < min(n, TAIL_LDS); q += TAIL_BLOCK) s_a[0][q] = ga[0][q];
+	if (threadIdx.x == 0) s_nf = s_nn = 0;
 	__syncthreads();
 	unsigned long long nfull = 0, nearly = 0, nnewton = 0, nconv = 0, nnear = 0, nclus = 0, nsweeps = 0;
 	int p = 0;
@@ -1004,7 +1016,7 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_tail(LensView L, double2* __rest
 		nsweeps++;
 		for (int q = threadIdx.x; q < n; q += TAIL_BLOCK)
 		{
-			const int a = s_a[p][q];
+			const int a = q < TAIL_LDS ? s_a[p][q] : ga[p][q];
 			double2* pz = xrow + a;
 			const cd z(pz->x, pz->y);
 			const int node = nearest_node(L, z);
@@ -1014,16 +1026,29 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_tail(LensView L, double2* __rest
 			nnear += w.nr.count;
 			if (st == ROOT_FULL)
 			{
-				const int k = atomicAdd(&s_cnt[0], 1);
-				s_fl[k] = a;
-				s_f0[k] = make_double2(w.f0.re, w.f0.im);
-				s_f1[k] = make_double2(f1.re, f1.im);
-				s_ex[k] = make_double2(ex.re, ex.im);
+				const int k = atomicAdd(&s_nf, 1);
+				const double2 v0 = make_double2(w.f0.re, w.f0.im), v1 = make_double2(f1.re, f1.im), ve = make_double2(ex.re, ex.im);
+				if (k < TAIL_LDS)
+				{
+					s_fl[k] = a;
+					s_f0[k] = v0;
+					s_f1[k] = v1;
+					s_ex[k] = ve;
+				}
+				else
+				{
+					gfl[k] = a;
+					gf0[k] = v0;
+					gf1[k] = v1;
+					gex[k] = ve;
+				}
 				nfull++;
 			}
 			else if (st == ROOT_STEPPED)
 			{
-				s_a[p ^ 1][atomicAdd(&s_cnt[1], 1)] = a;
+				const int k = atomicAdd(&s_nn, 1);
+				if (k < TAIL_LDS) s_a[p ^ 1][k] = a;
+				else ga[p ^ 1][k] = a;
 				nnewton++;
 			}
 			else
@@ -1034,10 +1059,10 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_tail(LensView L, double2* __rest
 			}
 		}
 		__syncthreads();
-		const int nf = s_cnt[0];
+		const int nf = s_nf;
 		for (int g = wv; g < nf; g += NW)
 		{
-			const int a = s_fl[g];
+			const int a = g < TAIL_LDS ? s_fl[g] : gfl[g];
 			const double2 zg = xrow[a];
 			const float fx = (float)zg.x, fy = (float)zg.y;
 			float ax = 0, ay = 0, bx = 0, by = 0;
@@ -1077,29 +1102,41 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_tail(LensView L, double2* __rest
 				vx += __shfl_xor(vx, o, 64);
 				vy += __shfl_xor(vy, o, 64);
 			}
-			if (lane == 0) s_sum[g] = make_double2(vx, -vy);
+			// the whole Aberth sum: the root part (conjugated) and the terms formed per root
+			if (lane == 0)
+			{
+				double2* e = g < TAIL_LDS ? &s_ex[g] : &gex[g];
+				*e = make_double2(vx + e->x, -vy + e->y);
+			}
 		}
 		__syncthreads();
 		for (int g = threadIdx.x; g < nf; g += TAIL_BLOCK)
 		{
-			const int a = s_fl[g];
+			const bool lds = g < TAIL_LDS;
+			const int a = lds ? s_fl[g] : gfl[g];
+			const double2 v0 = lds ? s_f0[g] : gf0[g], v1 = lds ? s_f1[g] : gf1[g], ve = lds ? s_ex[g] : gex[g];
 			const double2 zz = xrow[a];
-			const cd z(zz.x, zz.y), f0(s_f0[g].x, s_f0[g].y), f1(s_f1[g].x, s_f1[g].y);
-			const cd sum = cd(s_sum[g].x, s_sum[g].y) + cd(s_ex[g].x, s_ex[g].y);
+			const cd z(zz.x, zz.y), f0(v0.x, v0.y), f1(v1.x, v1.y);
+			const cd sum(ve.x, ve.y);
 			const cd zn = z - f0 / (f1 + f0 * sum);
 			xrow[a] = make_double2(zn.re, zn.im);
 			if ((zn - z).abs() / L.theta < CONV_TOL) nconv++;
-			else s_a[p ^ 1][atomicAdd(&s_cnt[1], 1)] = a;
+			else
+			{
+				const int k = atomicAdd(&s_nn, 1);
+				if (k < TAIL_LDS) s_a[p ^ 1][k] = a;
+				else ga[p ^ 1][k] = a;
+			}
 		}
 		__syncthreads();
-		n = s_cnt[1];
-		__syncthreads();
-		if (threadIdx.x == 0) s_cnt[0] = s_cnt[1] = 0;
+		n = s_nn;
 		p ^= 1;
 		__syncthreads();
+		if (threadIdx.x == 0) s_nf = s_nn = 0;
+		__syncthreads();
 	}
-	// the row is finished (or out of sweeps): sweeps that follow this launch
-	// for other rows find its list empty
+	// the row is finished (or out of sweeps): its list is empty for any
+	// sweep kernel that follows
 	if (threadIdx.x == 0) V.cnt[r] = 0;
 	if (threadIdx.x != 0) nsweeps = 0;
 	const int which[8] = {C_TAIL_FULL, C_TAIL_EARLY, C_CONV, C_TAIL_NEAR, C_TAIL_NEWTON, C_CLUSTER, C_TAIL_SWEEPS, C_TAIL_SWEEPS};
@@ -1314,7 +1351,7 @@ inline bool sweep_check(int done, int cap, int first)
 // The first fmm_sweeps sweeps may take the root sums of their dense rows
 // from the FMM pass (when the buffers carry a root grid).
 bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int nrows, int cap, unsigned fmm_mask,
-	int first_check = 3)
+	int first_check = 3, int tail_at = 0)
 {
 	ctx->phase_sweeps = cap;
 	const LensView L = ctx->lens();
@@ -1394,44 +1431,51 @@ bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int
 		k_finalize<<<g_rows, BLOCK, 0, s>>>(ctx->theta, roots, M, V, ctx->counters);
 		if (!CCF_LAUNCH(ctx, "find_critical_curve_roots_kernel")) return false;
 		ctx->info.sweeps++;
+		// mixed form, continuation: the sweeps after the first tail_at of
+		// the step in one launch, with no read-back. The rows the root-sum
+		// FMM took in the step's first two sweeps go to the host for the
+		// choice two steps later.
+		if (tail_at > 0 && it + 1 == tail_at && it + 1 < cap)
+		{
+			if (ctx->host_counts && nrows <= ctx->max_rows)
+			{
+				hipEvent_t& e = ctx->fmm_rows_event[M.j & 1];
+				if (!e && !CCF_HIP(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming))) return false;
+				if (!CCF_HIP(ctx, hipMemcpyAsync(ctx->host_counts + ctx->max_rows + 2 + 2 * (M.j & 1), B.plan + 2, 2 * sizeof(int),
+						hipMemcpyDeviceToHost, s)) ||
+					!CCF_HIP(ctx, hipEventRecord(e, s)))
+					return false;
+			}
+			const SweepView Vn = view(B, (it + 1) & 1);
+			const double d = CLUSTER_REL * ctx->root_half_length;
+			ccf::Timer tt;
+			if (ctx->P.timing)
+			{
+				tt.a = ctx->next_event();
+				tt.b = ctx->next_event();
+				if (tt.a && tt.b) (void)hipEventRecord(tt.a, s);
+			}
+			k_tail<<<nrows, TAIL_BLOCK, 0, s>>>(L, roots, M, Vn, ctx->counters, circ_exact, fmm_pole ? 1 : 0, cap - (it + 1),
+				(float)(d * d), ctx->spos32, Np);
+			if (ctx->P.timing && tt.a && tt.b)
+			{
+				(void)hipEventRecord(tt.b, s);
+				ctx->tail_timers.push_back(tt);
+			}
+			if (!CCF_LAUNCH(ctx, "find_critical_curve_roots_kernel")) return false;
+			ctx->info.tail_launches++;
+			ctx->phase_sweeps = it + 1;
+			break;
+		}
 		// stop once no root of any row is active: the remaining sweeps up to
 		// the cap would do nothing (checked at a few sweeps only, each check
 		// waits for the device)
 		if (ctx->host_counts && nrows <= ctx->max_rows && sweep_check(it + 1, cap, first_check))
 		{
 			if (!CCF_HIP(ctx, hipMemcpyAsync(ctx->host_counts, V.cnt_next, nrows * sizeof(int), hipMemcpyDeviceToHost, s)) ||
-				!CCF_HIP(ctx, hipMemcpyAsync(ctx->host_counts + ctx->max_rows, B.plan + 2, 2 * sizeof(int), hipMemcpyDeviceToHost, s)))
+				!CCF_HIP(ctx, hipMemcpyAsync(ctx->host_counts + ctx->max_rows, B.plan + 2, 2 * sizeof(int), hipMemcpyDeviceToHost, s)) ||
+				!CCF_HIP(ctx, hipStreamSynchronize(s)))
 				return false;
-			// mixed form, continuation: the tail launch goes in behind the
-			// copies before the host has seen them. It finishes every row with
-			// at most TAIL_MAX active roots and leaves the others to the
-			// sweeps above; the host waits for the copies only, so that its
-			// decision overlaps the tail.
-			const bool tail = f32 && M.j > 0 && it + 1 < cap;
-			if (tail)
-			{
-				if (!ctx->readback && !CCF_HIP(ctx, hipEventCreateWithFlags(&ctx->readback, hipEventDisableTiming))) return false;
-				if (!CCF_HIP(ctx, hipEventRecord(ctx->readback, s))) return false;
-				const SweepView Vn = view(B, (it + 1) & 1);
-				const double d = CLUSTER_REL * ctx->root_half_length;
-				ccf::Timer tt;
-				if (ctx->P.timing)
-				{
-					tt.a = ctx->next_event();
-					tt.b = ctx->next_event();
-					if (tt.a && tt.b) (void)hipEventRecord(tt.a, s);
-				}
-				k_tail<<<nrows, TAIL_BLOCK, 0, s>>>(L, roots, M, Vn, ctx->counters, circ_exact, fmm_pole ? 1 : 0, cap - (it + 1),
-					(float)(d * d), ctx->spos32, Np);
-				if (ctx->P.timing && tt.a && tt.b)
-				{
-					(void)hipEventRecord(tt.b, s);
-					ctx->tail_timers.push_back(tt);
-				}
-				if (!CCF_LAUNCH(ctx, "find_critical_curve_roots_kernel")) return false;
-				ctx->info.tail_launches++;
-			}
-			if (!CCF_HIP(ctx, tail ? hipEventSynchronize(ctx->readback) : hipStreamSynchronize(s))) return false;
 			long long active = 0;
 			int amax = 0;
 			for (int r = 0; r < nrows; r++)
@@ -1444,13 +1488,6 @@ bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int
 			if (B.grid.order > 0 && amax < root_fmm_min(R)) fmm_mask = 0;
 			ctx->fmm_rows_sweep[0] = ctx->host_counts[ctx->max_rows];
 			ctx->fmm_rows_sweep[1] = ctx->host_counts[ctx->max_rows + 1];
-			// every row had at most TAIL_MAX active roots: the tail launch
-			// finishes the phase
-			if (tail && amax <= TAIL_MAX)
-			{
-				ctx->phase_sweeps = it + 1;
-				break;
-			}
 			if (active == 0)
 			{
 				ctx->phase_sweeps = it + 1;
@@ -1555,7 +1592,7 @@ bool alloc_solver(ccf_ctx* ctx)
 			CCF_HIP(ctx, hipMemsetAsync(ctx->ccs_init, 0, pts * sizeof(double2), ctx->stream)) &&
 			CCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
 	return CCF_HIP(ctx, ctx->dmalloc(&ctx->ccs_init, pts * sizeof(double2))) && alloc_sweep(ctx, ctx->sw, ctx->max_rows, R) &&
-		CCF_HIP(ctx, hipHostMalloc((void**)&ctx->host_counts, (ctx->max_rows + 2) * sizeof(int), hipHostMallocDefault)) &&
+		CCF_HIP(ctx, hipHostMalloc((void**)&ctx->host_counts, (ctx->max_rows + 6) * sizeof(int), hipHostMallocDefault)) &&
 		CCF_HIP(ctx, ctx->dmalloc(&ctx->phase, (size_t)(np + nb) * sizeof(double2))) && launch_phase(ctx) &&
 		CCF_HIP(ctx, ctx->dmalloc(&ctx->counters, STAT_WORDS * sizeof(unsigned long long))) &&
 		CCF_HIP(ctx, hipMemsetAsync(ctx->ccs_init, 0, pts * sizeof(double2), ctx->stream)) &&
@@ -1600,23 +1637,44 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 	// the pass, whose launches cost more than its empty rows, and every
 	// ROOT_FMM_PROBE-th step tries again
 	int prev[2] = {-1, -1};
-	// sweeps the previous step needed; the mixed form checks from sweep 2,
-	// where the few roots still active go to the tail launch
-	const int first0 = ctx->P.precision == 1 ? 2 : 3;
+	// sweeps the previous step needed (FP64 form: its first read-back)
+	const int first0 = 3;
 	int first = first0;
+	// the mixed form runs the sweeps after the first TAIL_AT of a step (the
+	// first TAIL_AT + 1 while the predictor still has fewer than two earlier
+	// steps) in the tail launch, without read-backs: the FMM choice of its
+	// first sweeps uses the row counts of the step before the previous one,
+	// copied back while the previous step ran (a fixed lag, so the choice
+	// does not depend on timing)
+	const bool tailed = ctx->P.precision == 1;
 	for (int j = 1; j <= steps; j++)
 	{
 		const RowMap M{R, np, nb, j, 2, 0.0, ctx->phase};
 		unsigned mask = 0;
-		for (int q = 0; q < ROOT_FMM_STEP_SWEEPS; q++)
-			if (prev[q] != 0 || j % ROOT_FMM_PROBE == 1) mask |= 1u << q;
+		if (tailed)
+		{
+			int rows2[2] = {-1, -1};
+			if (j >= 3 && ctx->fmm_rows_event[j & 1])
+			{
+				if (!CCF_HIP(ctx, hipEventSynchronize(ctx->fmm_rows_event[j & 1]))) return false;
+				rows2[0] = ctx->host_counts[ctx->max_rows + 2 + 2 * (j & 1)];
+				rows2[1] = ctx->host_counts[ctx->max_rows + 3 + 2 * (j & 1)];
+			}
+			for (int q = 0; q < ROOT_FMM_STEP_SWEEPS; q++)
+				if (rows2[q] != 0 || j % ROOT_FMM_PROBE == 1) mask |= 1u << q;
+		}
+		else
+			for (int q = 0; q < ROOT_FMM_STEP_SWEEPS; q++)
+				if (prev[q] != 0 || j % ROOT_FMM_PROBE == 1) mask |= 1u << q;
 		// the exact smooth sheets are discontinuous at the field's edge
 		// (boxcar / heaviside terms), where extrapolated roots could cross
 		// onto another branch: those keep the previous step's roots
 		const int order = ctx->P.approx ? ctx->P.predictor : 0;
 		k_prepare<<<dim3(grid(R), 2 * nb), BLOCK, 0, s>>>(ctx->ccs_init, M, ctx->sw, order, ctx->dorder);
 		if (!CCF_LAUNCH(ctx, "prepare_roots_kernel")) return false;
-		if (!run_sweeps(ctx, ctx->sw, ctx->ccs_init, M, 2 * nb, cap, mask, first)) return false;
+		if (!run_sweeps(ctx, ctx->sw, ctx->ccs_init, M, 2 * nb, cap, mask, tailed ? cap + 1 : first,
+				tailed ? (j < 3 ? TAIL_AT + 1 : TAIL_AT) : 0))
+			return false;
 		prev[0] = ctx->fmm_rows_sweep[0];
 		prev[1] = ctx->fmm_rows_sweep[1];
 		// a step checked first at k can end no earlier than k: every
