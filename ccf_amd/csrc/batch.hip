@@ -401,15 +401,30 @@ __global__ void k_zero_u64(unsigned long long* p, int n)
 // (find_errors_kernel + has_nan_err_kernel, ccf_functions.cuh:419-489), and
 // the caustic map and length scales of the fields that write them
 // (find_caustics_kernel, find_mu_length_scales_kernel)
-__global__ __launch_bounds__(BLOCK) void k_batch_errors(const BField* __restrict__ fields)
+__global__ __launch_bounds__(BLOCK) void k_batch_errors(const BField* __restrict__ fields, const int* __restrict__ boff, int nf)
 {
-	const BField& F = fields[blockIdx.y];
-	const long long total = (long long)(F.np + F.nb) * F.R;
+	// the field of this block: blocks boff[f] .. boff[f + 1] - 1 cover field
+	// f's points (uniform binary search)
+	const int blk = blockIdx.x;
+	int lo = 0, hi = nf;
+	while (hi - lo > 1)
+	{
+		const int mid = (lo + hi) >> 1;
+		if (boff[mid] <= blk) lo = mid;
+		else hi = mid;
+	}
+	const BField& F = fields[lo];
+	const unsigned R = (unsigned)F.R;
+	// points in the rows' processing (Morton) order, so that a wave's
+	// evaluations share tree nodes (fewer than 2^31 points per field)
+	const unsigned total = (unsigned)(F.np + F.nb) * R;
+	const unsigned t0 = (unsigned)(blk - boff[lo]) * BLOCK + threadIdx.x;
 	double e = 0, res = 0;
 	unsigned long long bad = 0;
-	for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (long long)gridDim.x * blockDim.x)
+	if (t0 < total)
 	{
-		const int row = (int)(idx / F.R);
+		const unsigned row = t0 / R;
+		const long long idx = (long long)row * R + F.order[t0 - row * R];
 		const double2 t = F.cs[row];
 		const double2 v = F.roots[idx];
 		const cd z(v.x, v.y);
@@ -419,8 +434,8 @@ __global__ __launch_bounds__(BLOCK) void k_batch_errors(const BField* __restrict
 		if (!isfinite(b)) bad = 1;
 		else
 		{
-			e = fmax(e, b);
-			res = fmax(res, w.f0.abs());
+			e = b;
+			res = w.f0.abs();
 		}
 		// the caustic point and length scale from the same sums, in file
 		// order (root a, phi row r at a * rows + r)
@@ -440,8 +455,25 @@ __global__ __launch_bounds__(BLOCK) void k_batch_errors(const BField* __restrict
 		res = fmax(res, __shfl_xor(res, d, 64));
 	}
 	bad = wave_sum(bad);
+	// one set of atomics per block
+	__shared__ double s_e[BLOCK / 64], s_r[BLOCK / 64];
+	__shared__ unsigned long long s_b[BLOCK / 64];
+	const int wv = threadIdx.x >> 6;
 	if ((threadIdx.x & 63) == 0)
 	{
+		s_e[wv] = e;
+		s_r[wv] = res;
+		s_b[wv] = bad;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0)
+	{
+		for (int k = 1; k < BLOCK / 64; k++)
+		{
+			e = fmax(e, s_e[k]);
+			res = fmax(res, s_r[k]);
+			bad |= s_b[k];
+		}
 		atomicMax(&F.stats[B_ERRBITS], (unsigned long long)__double_as_longlong(e));
 		atomicMax(&F.stats[B_RESID], (unsigned long long)__double_as_longlong(res));
 		if (bad) atomicOr(&F.stats[B_NAN], 1ull);
@@ -538,6 +570,8 @@ __global__ __launch_bounds__(BLOCK) void k_batch_stars(const StarSlice* __restri
 struct BatchDevice
 {
 	StarSlice* slices = nullptr;
+	int* eoff = nullptr;     // [nf] first block of each field in the error pass
+	int eblocks = 0;
 	BField* fields = nullptr;
 	BTask* tasks0 = nullptr;
 	BTask* tasks1 = nullptr;
@@ -554,7 +588,7 @@ namespace
 void free_bdev(ccf_ctx* m, BatchDevice& d)
 {
 	auto F = [m](auto*& p) { m->dfree(p); p = nullptr; };
-	F(d.slices); F(d.fields); F(d.tasks0); F(d.tasks1); F(d.next); F(d.stats);
+	F(d.slices); F(d.eoff); F(d.fields); F(d.tasks0); F(d.tasks1); F(d.next); F(d.stats);
 	d = BatchDevice{};
 }
 
@@ -595,8 +629,7 @@ bool launch_post(ccf_batch* b, hipStream_t s)
 {
 	ccf_ctx* m = b->master;
 	BatchDevice& d = *b->dev;
-	const int blocks = std::max(1, std::min(grid(d.maxpts), 1024));
-	k_batch_errors<<<dim3(blocks, d.nf), BLOCK, 0, s>>>(d.fields);
+	k_batch_errors<<<d.eblocks, BLOCK, 0, s>>>(d.fields, d.eoff, d.nf);
 	k_batch_transpose<<<dim3(std::min(grid(d.rmax, 32), 64), std::min(grid(d.maxrows, 32), 64), d.nf), BLOCK, 0, s>>>(d.fields);
 	return CCF_LAUNCH(m, "batched errors, transpose, caustics");
 }
@@ -832,6 +865,29 @@ bool batch_build(ccf_batch* b, int verbose)
 	// from LDS, which then holds only the row (more workgroups per CU)
 	if (m->P.precision == 1 && m->tree.order > 0) d.nlds = 0;
 	lap("field tables");
+	// error-pass blocks of each field (fewer than 2^31 points per field and blocks in all)
+	std::vector<int> eoff(nf);
+	long long eb = 0;
+	for (int f = 0; f < nf; f++)
+	{
+		const long long pts = (long long)(hf[f].np + hf[f].nb) * hf[f].R;
+		if (pts >= (1LL << 31))
+		{
+			m->fail("Error. A batched field has 2^31 critical curve points or more\n");
+			return false;
+		}
+		eoff[f] = (int)eb;
+		eb += grid(pts);
+	}
+	if (eb >= (1LL << 31))
+	{
+		m->fail("Error. The batch has too many critical curve points for one error pass\n");
+		return false;
+	}
+	d.eblocks = (int)std::max(1LL, eb);
+	if (!CCF_HIP(m, m->dmalloc(&d.eoff, sizeof(int) * nf)) ||
+		!CCF_HIP(m, hipMemcpyAsync(d.eoff, eoff.data(), sizeof(int) * nf, hipMemcpyHostToDevice, s)))
+		return false;
 	if (!CCF_HIP(m, m->dmalloc(&d.stats, sizeof(unsigned long long) * nf * BSTAT))) return false;
 	for (int f = 0; f < nf; f++) hf[f].stats = d.stats + (size_t)f * BSTAT;
 	return CCF_HIP(m, m->dmalloc(&d.fields, sizeof(BField) * nf)) && CCF_HIP(m, m->dmalloc(&d.tasks0, sizeof(BTask) * d.n0)) &&
