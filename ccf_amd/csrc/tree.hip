@@ -328,17 +328,24 @@ __device__ inline double bin(const double* __restrict__ B, int n, int k) { retur
 __device__ inline double pow2_divisor(int l) { return (double)(int)(1u << l); } // (1 << l) as the reference's int
 
 // upward pass for one level: P2M at leaves with stars (fmm.cuh:21-49),
-// M2M from the four children elsewhere (fmm.cuh:109-139)
-__global__ void k_upward(int off, int cnt, int p, const double2* __restrict__ center, const double* __restrict__ hl,
+// M2M from the four children elsewhere (fmm.cuh:109-139). With NS = 2 the
+// same pass also builds a second expansion set from the second masses
+// (the unit masses of the pole sum): the geometry is shared, and each set
+// takes exactly the operations it would take alone.
+template <int NS>
+__global__ __launch_bounds__(BLOCK) void k_upward(int off, int cnt, int p, const double2* __restrict__ center, const double* __restrict__ hl,
 	const int* __restrict__ child, const int* __restrict__ sstart, const int* __restrict__ scount,
-	const double2* __restrict__ spos, const double* __restrict__ smass, const double* __restrict__ B, double2* mp)
+	const double2* __restrict__ spos, const double* __restrict__ smass, const double* __restrict__ smass2,
+	const double* __restrict__ B, double2* mp, double2* mp2)
 {
 	const int idx = blockIdx.x * blockDim.x + threadIdx.x;
 	const int i = idx / (p + 1), l = idx % (p + 1);
 	if (i >= cnt) return;
 	const int n = off + i;
 	const int P1 = p + 1;
-	cd r;
+	const double* ms[2] = {smass, smass2};
+	double2* out[2] = {mp, mp2};
+	cd r[NS];
 	if (child[n] < 0)
 	{
 		const int s0 = sstart[n], ns = scount[n];
@@ -347,54 +354,68 @@ __global__ void k_upward(int off, int cnt, int p, const double2* __restrict__ ce
 		const double h = hl[n];
 		if (l == 0)
 		{
-			for (int s = 0; s < ns; s++) r += cd(smass[s0 + s]);
+			for (int s = 0; s < ns; s++)
+#pragma unroll
+				for (int q = 0; q < NS; q++) r[q] += cd(ms[q][s0 + s]);
 		}
 		else
 		{
 			for (int s = 0; s < ns; s++)
 			{
-				cd t = (ld(spos, s0 + s) - c) / h;
-				r -= t.ipow(l) * smass[s0 + s];
+				const cd t = (ld(spos, s0 + s) - c) / h;
+				const cd tl = t.ipow(l);
+#pragma unroll
+				for (int q = 0; q < NS; q++) r[q] -= tl * ms[q][s0 + s];
 			}
-			r /= (double)l;
+#pragma unroll
+			for (int q = 0; q < NS; q++) r[q] /= (double)l;
 		}
 	}
 	else
 	{
-		for (int q = 0; q < 4; q++)
+		for (int c4 = 0; c4 < 4; c4++)
 		{
-			const int ch = child[n] + q;
-			const double2* a = mp + (size_t)ch * P1;
+			const int ch = child[n] + c4;
 			if (l == 0)
 			{
-				r += ld(a, 0);
+#pragma unroll
+				for (int q = 0; q < NS; q++) r[q] += ld(out[q] + (size_t)ch * P1, 0);
 				continue;
 			}
 			const cd dz = (ld(center, ch) - ld(center, n)) / hl[ch];
 			const cd idz = dz.inv();
-			cd t;
-			for (int k = l; k >= 1; k--)
+			const cd dzl = dz.ipow(l);
+			const double dv = pow2_divisor(l);
+#pragma unroll
+			for (int q = 0; q < NS; q++)
 			{
-				t += ld(a, k) * bin(B, l - 1, k - 1);
-				t *= idz;
+				const double2* a = out[q] + (size_t)ch * P1;
+				cd t;
+				for (int k = l; k >= 1; k--)
+				{
+					t += ld(a, k) * bin(B, l - 1, k - 1);
+					t *= idz;
+				}
+				t -= ld(a, 0) / (double)l;
+				t *= dzl;
+				t /= dv;
+				r[q] += t;
 			}
-			t -= ld(a, 0) / (double)l;
-			t *= dz.ipow(l);
-			t /= pow2_divisor(l);
-			r += t;
 		}
 	}
-	st(mp, (size_t)n * P1 + l, r);
+#pragma unroll
+	for (int q = 0; q < NS; q++) st(out[q], (size_t)n * P1 + l, r[q]);
 }
 
 // downward pass for one level >= 2: L2L from the parent (fmm.cuh:204-223),
 // M2L from the same-level list (fmm.cuh:281-322), P2L from the coarser
-// leaves (fmm.cuh:387-413)
-__global__ void k_downward(int off, int cnt, int p, const double2* __restrict__ center, const double* __restrict__ hl,
+// leaves (fmm.cuh:387-413); NS = 2 as in k_upward
+template <int NS>
+__global__ __launch_bounds__(BLOCK) void k_downward(int off, int cnt, int p, const double2* __restrict__ center, const double* __restrict__ hl,
 	const int* __restrict__ parent, const int* __restrict__ sl, const int* __restrict__ nsl,
 	const int* __restrict__ dl, const int* __restrict__ ndl, const int* __restrict__ sstart, const int* __restrict__ scount,
-	const double2* __restrict__ spos, const double* __restrict__ smass, const double* __restrict__ B,
-	const double2* __restrict__ mp, double2* loc)
+	const double2* __restrict__ spos, const double* __restrict__ smass, const double* __restrict__ smass2,
+	const double* __restrict__ B, const double2* __restrict__ mp, const double2* __restrict__ mp2, double2* loc, double2* loc2)
 {
 	// the binomial rows 0..2p in LDS (dynamic, launched with their size);
 	// the Horner loops below take their coefficients and binomials four at a
@@ -408,72 +429,102 @@ __global__ void k_downward(int off, int cnt, int p, const double2* __restrict__ 
 	if (i >= cnt) return;
 	const int n = off + i;
 	const int P1 = p + 1;
+	const double* ms[2] = {smass, smass2};
+	const double2* mps[2] = {mp, mp2};
+	double2* locs[2] = {loc, loc2};
 	const cd cn = ld(center, n);
 	const double hn = hl[n];
-	cd r;
+	cd r[NS];
 	{
 		const int pa = parent[n];
-		const double2* a = loc + (size_t)pa * P1;
 		const cd dz = (cn - ld(center, pa)) / hl[pa];
 		for (int k0 = p; k0 >= 1; k0 -= 4)
 		{
-			cd av[4];
+			cd av[NS][4];
 			double bv[4];
 #pragma unroll
 			for (int u = 0; u < 4; u++)
 			{
 				const int k = max(k0 - u, 1);
-				av[u] = ld(a, k);
+#pragma unroll
+				for (int q = 0; q < NS; q++) av[q][u] = ld(locs[q] + (size_t)pa * P1, k);
 				bv[u] = bin(sB, k, l);
 			}
 #pragma unroll
 			for (int u = 0; u < 4; u++)
 				if (k0 - u >= 1)
 				{
-					r += av[u] * bv[u];
-					r *= dz;
+#pragma unroll
+					for (int q = 0; q < NS; q++)
+					{
+						r[q] += av[q][u] * bv[u];
+						r[q] *= dz;
+					}
 				}
 		}
-		r *= dz.inv().ipow(l);
-		r /= pow2_divisor(l);
+		const cd w = dz.inv().ipow(l);
+		const double dv = pow2_divisor(l);
+#pragma unroll
+		for (int q = 0; q < NS; q++)
+		{
+			r[q] *= w;
+			r[q] /= dv;
+		}
 	}
 	for (int j = 0; j < nsl[n]; j++)
 	{
 		const int m = sl[n * MAX_SAME_LEVEL + j];
-		const double2* b = mp + (size_t)m * P1;
 		const cd dz = (ld(center, m) - cn) / hl[m];
 		const cd imdz = (-dz).inv();
-		cd t;
+		cd t[NS];
 		for (int k0 = p; k0 >= 1; k0 -= 4)
 		{
-			cd bv[4];
+			cd bv[NS][4];
 			double cv[4];
 #pragma unroll
 			for (int u = 0; u < 4; u++)
 			{
 				const int k = max(k0 - u, 1);
-				bv[u] = ld(b, k);
+#pragma unroll
+				for (int q = 0; q < NS; q++) bv[q][u] = ld(mps[q] + (size_t)m * P1, k);
 				cv[u] = l == 0 ? 1.0 : bin(sB, l + k - 1, k - 1);
 			}
 #pragma unroll
 			for (int u = 0; u < 4; u++)
 				if (k0 - u >= 1)
 				{
-					t += l == 0 ? bv[u] : bv[u] * cv[u];
-					t *= imdz;
+#pragma unroll
+					for (int q = 0; q < NS; q++)
+					{
+						t[q] += l == 0 ? bv[q][u] : bv[q][u] * cv[u];
+						t[q] *= imdz;
+					}
 				}
 		}
 		if (l == 0)
 		{
-			t += ld(b, 0) * (-dz).log();
-			t += ld(b, 0) * log(hl[m]);
+			const cd lg = (-dz).log();
+			const double lh = log(hl[m]);
+#pragma unroll
+			for (int q = 0; q < NS; q++)
+			{
+				const cd b0 = ld(mps[q] + (size_t)m * P1, 0);
+				t[q] += b0 * lg;
+				t[q] += b0 * lh;
+			}
 		}
 		else
 		{
-			t -= ld(b, 0) / (double)l;
-			t *= dz.inv().ipow(l);
+			const cd w = dz.inv().ipow(l);
+#pragma unroll
+			for (int q = 0; q < NS; q++)
+			{
+				t[q] -= ld(mps[q] + (size_t)m * P1, 0) / (double)l;
+				t[q] *= w;
+			}
 		}
-		r += t;
+#pragma unroll
+		for (int q = 0; q < NS; q++) r[q] += t[q];
 	}
 	for (int j = 0; j < ndl[n]; j++)
 	{
@@ -482,19 +533,28 @@ __global__ void k_downward(int off, int cnt, int p, const double2* __restrict__ 
 		for (int s = 0; s < ns; s++)
 		{
 			const cd dz = (ld(spos, s0 + s) - cn) / hn;
-			const double ms = smass[s0 + s];
 			if (l == 0)
 			{
-				r += ms * (-dz).log();
-				r += cd(ms * log(hn));
+				const cd lg = (-dz).log();
+				const double lh = log(hn);
+#pragma unroll
+				for (int q = 0; q < NS; q++)
+				{
+					const double mq = ms[q][s0 + s];
+					r[q] += mq * lg;
+					r[q] += cd(mq * lh);
+				}
 			}
 			else
 			{
-				r -= (ms / (double)l) * dz.inv().ipow(l);
+				const cd w = dz.inv().ipow(l);
+#pragma unroll
+				for (int q = 0; q < NS; q++) r[q] -= (ms[q][s0 + s] / (double)l) * w;
 			}
 		}
 	}
-	st(loc, (size_t)n * P1 + l, r);
+#pragma unroll
+	for (int q = 0; q < NS; q++) st(locs[q], (size_t)n * P1 + l, r[q]);
 }
 
 __global__ void k_to_float(const double2* __restrict__ a, float2* b, size_t n)
@@ -788,6 +848,22 @@ bool build_forest(ccf_ctx* ctx, ForestSpec& spec, int verbose)
 	if (!CCF_HIP(ctx, ctx->dmalloc(&T.loc32, sizeof(float2) * (size_t)T.nnodes * P1))) return false;
 	if (!CCF_HIP(ctx, hipMemsetAsync(T.mp, 0, sizeof(double2) * (size_t)T.nnodes * P1, s))) return false;
 	if (!CCF_HIP(ctx, hipMemsetAsync(T.loc, 0, sizeof(double2) * (size_t)T.nnodes * P1, s))) return false;
+	// the mixed form also needs the expansions of the unit masses: local
+	// expansions of sum log(z - s_i), whose first derivative is the far part
+	// of the Aberth pole sum; both sets come from the same passes
+	const bool dual = P.precision == 1 && T.order > 0;
+	if (dual)
+	{
+		std::vector<double> one(N, 1.0);
+		const bool ok2 = CCF_HIP(ctx, ctx->dmalloc(&ctx->ones, sizeof(double) * N)) &&
+			CCF_HIP(ctx, hipMemcpyAsync(ctx->ones, one.data(), sizeof(double) * N, hipMemcpyHostToDevice, s)) &&
+			CCF_HIP(ctx, ctx->dmalloc(&T.mpu, sizeof(double2) * (size_t)T.nnodes * P1)) &&
+			CCF_HIP(ctx, ctx->dmalloc(&T.locu, sizeof(double2) * (size_t)T.nnodes * P1)) &&
+			CCF_HIP(ctx, ctx->dmalloc(&T.locu32, sizeof(float2) * (size_t)T.nnodes * P1)) &&
+			CCF_HIP(ctx, hipMemsetAsync(T.mpu, 0, sizeof(double2) * (size_t)T.nnodes * P1, s)) &&
+			CCF_HIP(ctx, hipMemsetAsync(T.locu, 0, sizeof(double2) * (size_t)T.nnodes * P1, s));
+		if (!ok2) return false;
+	}
 	if (T.order > 0)
 	{
 		std::vector<double> B = binomial_table(2 * p, P.binomial_exact != 0);
@@ -796,44 +872,29 @@ bool build_forest(ccf_ctx* ctx, ForestSpec& spec, int verbose)
 		for (int l = T.levels; l >= 0; l--)
 		{
 			const long long work = (long long)T.level_cnt[l] * P1;
-			k_upward<<<grid(work), BLOCK, 0, s>>>(T.level_off[l], T.level_cnt[l], p, T.center, T.hl, T.child, T.sstart, T.scount,
-				ctx->spos, ctx->smass, T.binom, T.mp);
+			if (dual)
+				k_upward<2><<<grid(work), BLOCK, 0, s>>>(T.level_off[l], T.level_cnt[l], p, T.center, T.hl, T.child, T.sstart,
+					T.scount, ctx->spos, ctx->smass, ctx->ones, T.binom, T.mp, T.mpu);
+			else
+				k_upward<1><<<grid(work), BLOCK, 0, s>>>(T.level_off[l], T.level_cnt[l], p, T.center, T.hl, T.child, T.sstart,
+					T.scount, ctx->spos, ctx->smass, nullptr, T.binom, T.mp, nullptr);
 		}
 		for (int l = 2; l <= T.levels; l++)
 		{
 			const long long work = (long long)T.level_cnt[l] * P1;
-			k_downward<<<grid(work), BLOCK, binom_lds(p), s>>>(T.level_off[l], T.level_cnt[l], p, T.center, T.hl,
-				T.parent, T.sl, T.nsl, T.dl, T.ndl, T.sstart, T.scount, ctx->spos, ctx->smass, T.binom, T.mp, T.loc);
+			if (dual)
+				k_downward<2><<<grid(work), BLOCK, binom_lds(p), s>>>(T.level_off[l], T.level_cnt[l], p, T.center, T.hl, T.parent,
+					T.sl, T.nsl, T.dl, T.ndl, T.sstart, T.scount, ctx->spos, ctx->smass, ctx->ones, T.binom, T.mp, T.mpu, T.loc, T.locu);
+			else
+				k_downward<1><<<grid(work), BLOCK, binom_lds(p), s>>>(T.level_off[l], T.level_cnt[l], p, T.center, T.hl, T.parent,
+					T.sl, T.nsl, T.dl, T.ndl, T.sstart, T.scount, ctx->spos, ctx->smass, nullptr, T.binom, T.mp, nullptr, T.loc, nullptr);
 		}
 		if (!CCF_LAUNCH(ctx, "calculate_coeffs_kernels")) return false;
 	}
 	k_to_float<<<grid((long long)T.nnodes * P1), BLOCK, 0, s>>>(T.loc, T.loc32, (size_t)T.nnodes * P1);
 	if (!CCF_LAUNCH(ctx, "local coefficients to float")) return false;
-	if (P.precision == 1 && T.order > 0)
+	if (dual)
 	{
-		// the same passes with unit masses: local expansions of sum log(z - s_i),
-		// whose first derivative is the far part of the Aberth pole sum
-		std::vector<double> one(N, 1.0);
-		bool ok2 = CCF_HIP(ctx, ctx->dmalloc(&ctx->ones, sizeof(double) * N)) &&
-			CCF_HIP(ctx, hipMemcpyAsync(ctx->ones, one.data(), sizeof(double) * N, hipMemcpyHostToDevice, s)) &&
-			CCF_HIP(ctx, ctx->dmalloc(&T.mpu, sizeof(double2) * (size_t)T.nnodes * P1)) &&
-			CCF_HIP(ctx, ctx->dmalloc(&T.locu, sizeof(double2) * (size_t)T.nnodes * P1)) &&
-			CCF_HIP(ctx, ctx->dmalloc(&T.locu32, sizeof(float2) * (size_t)T.nnodes * P1)) &&
-			CCF_HIP(ctx, hipMemsetAsync(T.mpu, 0, sizeof(double2) * (size_t)T.nnodes * P1, s)) &&
-			CCF_HIP(ctx, hipMemsetAsync(T.locu, 0, sizeof(double2) * (size_t)T.nnodes * P1, s));
-		if (!ok2) return false;
-		for (int l = T.levels; l >= 0; l--)
-		{
-			const long long work = (long long)T.level_cnt[l] * P1;
-			k_upward<<<grid(work), BLOCK, 0, s>>>(T.level_off[l], T.level_cnt[l], p, T.center, T.hl, T.child, T.sstart, T.scount,
-				ctx->spos, ctx->ones, T.binom, T.mpu);
-		}
-		for (int l = 2; l <= T.levels; l++)
-		{
-			const long long work = (long long)T.level_cnt[l] * P1;
-			k_downward<<<grid(work), BLOCK, binom_lds(p), s>>>(T.level_off[l], T.level_cnt[l], p, T.center, T.hl, T.parent, T.sl,
-				T.nsl, T.dl, T.ndl, T.sstart, T.scount, ctx->spos, ctx->ones, T.binom, T.mpu, T.locu);
-		}
 		k_to_float<<<grid((long long)T.nnodes * P1), BLOCK, 0, s>>>(T.locu, T.locu32, (size_t)T.nnodes * P1);
 		if (!CCF_LAUNCH(ctx, "unit-mass expansions")) return false;
 	}
