@@ -69,12 +69,14 @@ struct BField
 	const double2* seeds;    // [R] initial roots
 	const double2* cs;       // (cos phi, sin phi) per phi row
 	const double2* stars;    // the field's star positions, tree order
+	const float2* stars32;   // the same in FP32 (mixed form)
 	const int* order;        // [R] processing order of the roots
 	double2* ccs;            // [R nb][np/nb + 1] output (file order)
 	double2* caustics;       // same layout
 	double* mu;              // same layout
 	unsigned long long* stats; // [BSTAT]
 	int N, R, np, nb, circ_exact, predictor, cap_init, cap_step;
+	float cluster_d2;        // mixed form: squared root separation below which a Cauchy term is FP64
 };
 
 // per-field counters
@@ -204,27 +206,71 @@ __device__ void solve_row(const BField& F, int row, int cap, bool newton, double
 			// cooperative sums, one wave per listed root, lanes over the sources
 			for (int g = wv; g < nf; g += NW)
 			{
-				const double2 zg = zc[s_fl[g]];
-				double ax = 0, ay = 0, bx = 0, by = 0;
-				if (!pole_fmm)
-					for (int s = lane; s < F.N; s += 64)
-					{
-						const double2 w = st[s];
-						const double dx = zg.x - w.x, dy = zg.y - w.y;
-						const double iv = rcp64(fma(dx, dx, dy * dy));
-						ax = fma(dx, iv, ax);
-						ay = fma(dy, iv, ay);
-					}
-				for (int j = lane; j < R; j += 64)
+				const int ig = s_fl[g];
+				const double2 zg = zc[ig];
+				double vx, vy;
+				if (MIXED)
 				{
-					// the own root gives dx = dy = 0 and, floored, a zero term
-					const double2 w = zc[j];
-					const double dx = zg.x - w.x, dy = zg.y - w.y;
-					const double iv = rcp64(fmax(fma(dx, dx, dy * dy), 1e-300));
-					bx = fma(dx, iv, bx);
-					by = fma(dy, iv, by);
+					// FP32 terms, a close root's term in FP64
+					const float fx = (float)zg.x, fy = (float)zg.y;
+					float ax = 0, ay = 0, bx = 0, by = 0;
+					double cx = 0, cy = 0;
+					if (!pole_fmm)
+						for (int s = lane; s < F.N; s += 64)
+						{
+							const float2 w = F.stars32[s];
+							const float dx = fx - w.x, dy = fy - w.y;
+							const float iv = __builtin_amdgcn_rcpf(fmaf(dx, dx, dy * dy));
+							ax = fmaf(dx, iv, ax);
+							ay = fmaf(dy, iv, ay);
+						}
+					for (int j = lane; j < R; j += 64)
+					{
+						const double2 w = zc[j];
+						const float dx = fx - (float)w.x, dy = fy - (float)w.y;
+						const float d2 = fmaf(dx, dx, dy * dy);
+						if (d2 < F.cluster_d2 && j != ig)
+						{
+							const double ex = zg.x - w.x, ey = zg.y - w.y;
+							const double iv = rcp64(fmax(fma(ex, ex, ey * ey), 1e-300));
+							cx = fma(ex, iv, cx);
+							cy = fma(ey, iv, cy);
+						}
+						else
+						{
+							// the own root gives dx = dy = 0 and, floored, a zero term
+							const float iv = __builtin_amdgcn_rcpf(fmaxf(d2, 1e-30f));
+							bx = fmaf(dx, iv, bx);
+							by = fmaf(dy, iv, by);
+						}
+					}
+					vx = 2.0 * (double)ax - ((double)bx + cx);
+					vy = 2.0 * (double)ay - ((double)by + cy);
 				}
-				double vx = 2 * ax - bx, vy = 2 * ay - by;
+				else
+				{
+					double ax = 0, ay = 0, bx = 0, by = 0;
+					if (!pole_fmm)
+						for (int s = lane; s < F.N; s += 64)
+						{
+							const double2 w = st[s];
+							const double dx = zg.x - w.x, dy = zg.y - w.y;
+							const double iv = rcp64(fma(dx, dx, dy * dy));
+							ax = fma(dx, iv, ax);
+							ay = fma(dy, iv, ay);
+						}
+					for (int j = lane; j < R; j += 64)
+					{
+						// the own root gives dx = dy = 0 and, floored, a zero term
+						const double2 w = zc[j];
+						const double dx = zg.x - w.x, dy = zg.y - w.y;
+						const double iv = rcp64(fmax(fma(dx, dx, dy * dy), 1e-300));
+						bx = fma(dx, iv, bx);
+						by = fma(dy, iv, by);
+					}
+					vx = 2 * ax - bx;
+					vy = 2 * ay - by;
+				}
 				for (int o = 32; o >= 1; o >>= 1)
 				{
 					vx += __shfl_xor(vx, o, 64);
@@ -246,7 +292,66 @@ __device__ void solve_row(const BField& F, int row, int cap, bool newton, double
 			zy[k] = v.y;
 			ax[k] = ay[k] = bx[k] = by[k] = 0;
 		}
-		if (any && nf > COOP_MAX)
+		if (MIXED && any && nf > COOP_MAX)
+		{
+			// FP32 terms, a close root's term in FP64, as in the cooperative sums
+			float fx[RPT], fy[RPT], sax[RPT], say[RPT], sbx[RPT], sby[RPT];
+#pragma unroll
+			for (int k = 0; k < RPT; k++)
+			{
+				fx[k] = (float)zx[k];
+				fy[k] = (float)zy[k];
+				sax[k] = say[k] = sbx[k] = sby[k] = 0;
+			}
+			if (!pole_fmm)
+				for (int s = 0; s < F.N; s++)
+				{
+					const float2 w = F.stars32[s];
+#pragma unroll
+					for (int k = 0; k < RPT; k++)
+					{
+						const float dx = fx[k] - w.x, dy = fy[k] - w.y;
+						const float iv = __builtin_amdgcn_rcpf(fmaf(dx, dx, dy * dy));
+						sax[k] = fmaf(dx, iv, sax[k]);
+						say[k] = fmaf(dy, iv, say[k]);
+					}
+				}
+			for (int j = 0; j < R; j++)
+			{
+				const double2 w = zc[j];
+				const float wx = (float)w.x, wy = (float)w.y;
+#pragma unroll
+				for (int k = 0; k < RPT; k++)
+				{
+					const float dx = fx[k] - wx, dy = fy[k] - wy;
+					const float d2 = fmaf(dx, dx, dy * dy);
+					if (d2 < F.cluster_d2 && j != idx[k])
+					{
+						const double ex = zx[k] - w.x, ey = zy[k] - w.y;
+						const double iv = rcp64(fmax(fma(ex, ex, ey * ey), 1e-300));
+						bx[k] = fma(ex, iv, bx[k]);
+						by[k] = fma(ey, iv, by[k]);
+					}
+					else
+					{
+						// the own root gives dx = dy = 0 and, floored, a zero term
+						const float iv = __builtin_amdgcn_rcpf(fmaxf(d2, 1e-30f));
+						sbx[k] = fmaf(dx, iv, sbx[k]);
+						sby[k] = fmaf(dy, iv, sby[k]);
+					}
+				}
+			}
+			// as the cooperative sums: 2 a - (b32 + b64)
+#pragma unroll
+			for (int k = 0; k < RPT; k++)
+			{
+				ax[k] = (double)sax[k];
+				ay[k] = (double)say[k];
+				bx[k] += (double)sbx[k];
+				by[k] += (double)sby[k];
+			}
+		}
+		else if (any && nf > COOP_MAX)
 		{
 			if (!pole_fmm)
 				for (int s = 0; s < F.N; s++)
@@ -821,6 +926,11 @@ bool batch_build(ccf_batch* b, int verbose)
 		F.seeds = c->dseeds;
 		F.cs = c->phase;
 		F.stars = m->spos + spec.soff[f];
+		F.stars32 = m->spos32 + spec.soff[f];
+		{
+			const double dc = CLUSTER_REL * c->root_half_length;
+			F.cluster_d2 = (float)(dc * dc);
+		}
 		F.order = c->dorder;
 		F.ccs = c->ccs;
 		F.caustics = c->P.write_caustics ? c->caustics : nullptr;
@@ -861,9 +971,10 @@ bool batch_build(ccf_batch* b, int verbose)
 	int nmax = 0;
 	for (ccf_ctx* c : b->fields) nmax = std::max(nmax, c->num_stars);
 	d.nlds = (int)std::max(0LL, std::min<long long>(nmax, room / (long long)sizeof(double2)));
-	// the mixed form takes the pole sum from the star FMM: no star is read
-	// from LDS, which then holds only the row (more workgroups per CU)
-	if (m->P.precision == 1 && m->tree.order > 0) d.nlds = 0;
+	// the mixed form takes the pole sum from the star FMM, or else its FP32
+	// terms from the FP32 star copy: no star is read from LDS, which then
+	// holds only the row (more workgroups per CU)
+	if (m->P.precision == 1) d.nlds = 0;
 	lap("field tables");
 	// error-pass blocks of each field (fewer than 2^31 points per field and blocks in all)
 	std::vector<int> eoff(nf);

@@ -349,6 +349,10 @@ __device__ inline int xcd_x(int j, int used)
 
 // |dz| / theta below which an Aberth update ends a root (ccf_functions.cuh:384)
 constexpr double CONV_TOL = 1e-9;
+// mixed form: root separation, relative to the field's half-length, below
+// which an FP32 Cauchy term is redone in FP64 (an FP32 difference of two
+// positions that close keeps fewer than ~12 bits)
+constexpr double CLUSTER_REL = 2e-4;
 // mixed form, continuation steps: |f0/f1| / theta below which a root takes
 // a plain FP64 Newton step instead of an Aberth update. The root is then
 // within a millionth of theta_E of a root of Witt's equation, where Newton

@@ -73,7 +73,6 @@ constexpr int CHUNK64 = 1024;         // FP64 complex values per LDS chunk (16 K
 constexpr int CHUNK32 = 2048;         // FP32 complex values per LDS chunk (16 KiB)
 constexpr int MAX_SLICES = 64;
 constexpr int MIN_SLICE = 1024;       // fewest terms per slice
-constexpr double CLUSTER_REL = 2e-4;  // root separation, relative to the field, below which FP32 terms are redone in FP64
 
 inline int grid(long long n, int b = BLOCK) { return (int)((n + b - 1) / b); }
 
