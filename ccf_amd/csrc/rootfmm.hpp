@@ -288,11 +288,42 @@ __global__ __launch_bounds__(BLOCK) void k_rm2m(RootView V, int l)
 }
 
 // L2L and M2L of level l with 32 lanes per target cell: lane k translates
-// the k-th (and k+32-th) source of the target's interaction list into a
-// partial local expansion, the partials are added through LDS in lane
-// order, and lane m adds coefficient m of the parent's shifted expansion.
+// the k-th source of the target's interaction list into a partial local
+// expansion, the partials are added through LDS in lane order, and lane m
+// adds coefficient m of the parent's shifted expansion.
 template <int P>
 constexpr int down_targets() { return (P + 1) <= 14 ? 8 : 4; } // 32 (P+1) double2 of LDS per target
+
+// The interaction list of a cell below level 0: the children of its
+// parent's 3 x 3 block of cells (a 6 x 6 block, parent block row-major,
+// four children each) that are not its neighbours. Entry k of list o (the
+// cell's child position x + 2 y in its parent) is x + 6 y in that block; 27
+// entries, so one per lane.
+struct FarList
+{
+	unsigned char v[4][27];
+};
+
+constexpr FarList make_far_list()
+{
+	FarList t{};
+	for (int o = 0; o < 4; o++)
+	{
+		const int tx = 2 + (o & 1), ty = 2 + (o >> 1);
+		int n = 0;
+		for (int b = 0; b < 9; b++)
+			for (int q = 0; q < 4; q++)
+			{
+				const int jx = 2 * (b % 3) + (q & 1), jy = 2 * (b / 3) + (q >> 1);
+				const int dx = jx > tx ? jx - tx : tx - jx, dy = jy > ty ? jy - ty : ty - jy;
+				if (dx >= 2 || dy >= 2) t.v[o][n++] = (unsigned char)(jx + 6 * jy);
+			}
+	}
+	return t;
+}
+
+__constant__ const FarList kFarList = make_far_list();
+static_assert(make_far_list().v[3][26] != 0, "27 far cells per child position");
 
 template <int P>
 __global__ __launch_bounds__(32 * down_targets<P>()) void k_rdown32(RootView V, int l, unsigned long long* __restrict__ stats)
@@ -356,15 +387,12 @@ __global__ __launch_bounds__(32 * down_targets<P>()) void k_rdown32(RootView V, 
 		}
 		else
 		{
-			// the 36 children of the parent's 3 x 3 block
-			const int px = ix >> 1, py = iy >> 1, npx = nxl >> 1, npy = nyl >> 1;
-			for (int k = lane; k < 36; k += 32)
+			// the 27 far children of the parent's 3 x 3 block, one per lane
+			if (lane < 27)
 			{
-				const int b = k >> 2, q = k & 3;
-				const int qx = px - 1 + b % 3, qy = py - 1 + b / 3;
-				if (qx < 0 || qy < 0 || qx >= npx || qy >= npy) continue;
-				const int jx = 2 * qx + (q & 1), jy = 2 * qy + (q >> 1);
-				if (abs(jx - ix) >= 2 || abs(jy - iy) >= 2) m2l(jx, jy);
+				const int v = kFarList.v[(ix & 1) + 2 * (iy & 1)][lane];
+				const int jx = 2 * ((ix >> 1) - 1) + v % 6, jy = 2 * ((iy >> 1) - 1) + v / 6;
+				if (jx >= 0 && jy >= 0 && jx < nxl && jy < nyl) m2l(jx, jy);
 			}
 		}
 	}
