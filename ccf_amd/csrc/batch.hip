@@ -548,7 +548,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_errors(const BField* __restrict
 		{
 			cd ws;
 			double m;
-			lens_map_from(F.L, z, w.nr, w.fr, ws, m);
+			lens_map_from(F.L, z, w.nr, w.fr, w.d1, w.d2, ws, m);
 			const size_t o = (size_t)(idx - (long long)row * F.R) * (F.np + F.nb) + row;
 			if (F.caustics) F.caustics[o] = make_double2(ws.re, ws.im);
 			if (F.mu) F.mu[o] = m;
@@ -904,6 +904,7 @@ bool batch_build(ccf_batch* b, int verbose)
 		L.corner_abs = Lc.corner_abs;
 		L.corner_abs2 = Lc.corner_abs2;
 		L.corner_arg = Lc.corner_arg;
+		for (int k = 0; k < 4; k++) L.clog[k] = Lc.clog[k];
 		L.nstars = c->num_stars;
 		L.nts = Lc.nts;
 		L.ts_a = Lc.ts_a;

@@ -42,6 +42,11 @@ ccf::LensView ccf_ctx::lens() const
 	L.order = tree.order;
 	L.nstars = num_stars;
 	L.nts = nts;
+	{
+		const ccf::cd c(corner.re, corner.im);
+		const ccf::cd lg[4] = {c.log(), c.conj().log(), (-c).log(), (-c).conj().log()};
+		for (int k = 0; k < 4; k++) L.clog[k] = make_double2(lg[k].re, lg[k].im);
+	}
 	L.ts_a = ts_a;
 	L.ts_b = ts_b;
 	L.spos = spos;
@@ -137,7 +142,7 @@ void ccf_ctx::free_device()
 	F(T.nbr); F(T.nnbr); F(T.sl); F(T.nsl); F(T.dl); F(T.ndl); F(T.split); F(T.rank);
 	F(T.field); F(T.nf); F(T.nnf); F(T.nbox); F(T.nf_off); F(T.nf_cnt); F(T.nfs); F(T.mp); F(T.loc); F(T.loc32); F(T.mpu); F(T.locu); F(T.locu32); F(T.binom); F(T.lookup);
 	T = ccf::DeviceTree{};
-	F(spos); F(smass); F(spos_tmp); F(smass_tmp); F(spos32); F(ts_a); F(ts_b); F(ones);
+	F(spos); F(smass); F(spos_tmp); F(smass_tmp); F(spos32); F(ts_a); F(ts_b); F(ones); F(far_points);
 	F(ccs_init); F(phase); F(dseeds); F(dorder);
 	if (results_borrowed)
 	{

@@ -86,6 +86,7 @@ struct LensView
 	int nts;                   // number of precomputed smooth-matter series terms
 	const double2* ts_a;       // (1 - e^{2 i k arg c}) / k for k = ts-1, ts-3, ..., 2
 	const double2* ts_b;       // (1 - e^{2 i k arg c})
+	double2 clog[4];           // log c, log conj(c), log(-c), log(-conj(c)) of the corner c (rectangular fields)
 	const double2* spos;       // star positions, tree order
 	const double* smass;
 	const double2* center;
@@ -208,6 +209,7 @@ struct ccf_ctx
 	int max_rows = 0;
 	ccf::SweepBufs sw;
 	double* ones = nullptr;       // unit masses for the pole-sum expansions
+	long long* far_points = nullptr; // error pass: count of the points outside the tree, then their indices
 	unsigned long long* counters = nullptr; // device statistics, see solve.hip
 	std::vector<ccf::Timer> timers;
 	std::vector<ccf::Timer> witt_timers;
@@ -229,6 +231,7 @@ struct ccf_ctx
 	hipEvent_t fmm_rows_event[2] = {nullptr, nullptr}; // mixed continuation: after the copy of a step's FMM row count (no timing)
 	int fmm_rows_sweep[2] = {-1, -1}; // root-sum FMM rows of the last phase's first and second sweep (-1: unknown)
 	int phase_sweeps = 0;         // sweeps the last phase ran (its cap when no read-back saw it end)
+	int sweep_verbose = 0;        // verbose level of the running solve stage (3: a read-back and a line after every sweep)
 	bool caustics_done = false;   // caustics / length scales formed by the error pass of find_ccs
 	bool mu_done = false;
 	double2* phase = nullptr;     // (cos phi, sin phi) of every phi row

@@ -243,18 +243,31 @@ __device__ inline cd smooth_alpha(const LensView& L, cd z)
 		const cd c(L.corner_re, L.corner_im);
 		if (L.approx)
 		{
-			cd s1, s2, s3, s4;
-			const cd r1 = zb / c, r2 = zb / c.conj(), r3 = zb / (-c), r4 = zb / (-c.conj());
+			// s(r) = sum_{i=1..T} r^i / i at r1 = zbar / c, r2 = zbar / conj(c)
+			// and at -r1, -r2 (the corners -c, -conj(c)): with u = r^2,
+			// s(+-r) = B(u) +- r A(u), A over the odd powers, B over the even
+			const cd r1 = zb / c, r2 = zb / c.conj();
+			const cd u1 = r1 * r1, u2 = r2 * r2;
+			cd a1, b1, a2, b2;
 			for (int i = L.taylor_smooth; i >= 1; i--)
 			{
 				const double q = 1.0 / i;
-				s1 = (s1 + q) * r1;
-				s2 = (s2 + q) * r2;
-				s3 = (s3 + q) * r3;
-				s4 = (s4 + q) * r4;
+				if (i & 1)
+				{
+					a1 = a1 * u1 + q;
+					a2 = a2 * u2 + q;
+				}
+				else
+				{
+					b1 = (b1 + q) * u1;
+					b2 = (b2 + q) * u2;
+				}
 			}
-			cd a = (c - zb) * (c.log() - s1) - (c.conj() - zb) * (c.conj().log() - s2)
-				+ (-c - zb) * ((-c).log() - s3) - (-c.conj() - zb) * ((-c).conj().log() - s4);
+			const cd o1 = r1 * a1, o2 = r2 * a2;
+			const cd s1 = b1 + o1, s3 = b1 - o1, s2 = b2 + o2, s4 = b2 - o2;
+			const cd l1(L.clog[0].x, L.clog[0].y), l2(L.clog[1].x, L.clog[1].y), l3(L.clog[2].x, L.clog[2].y),
+				l4(L.clog[3].x, L.clog[3].y);
+			cd a = (c - zb) * (l1 - s1) - (c.conj() - zb) * (l2 - s2) + (-c - zb) * (l3 - s3) - (-c.conj() - zb) * (l4 - s4);
 			return a * cd(0, -ks / PI) - cd(ks * 2 * (c.re + z.re));
 		}
 		const cd c1 = c - zb, c2 = c.conj() - zb, c3 = -c - zb, c4 = -c.conj() - zb;
@@ -278,6 +291,7 @@ struct Witt
 	int node = 0;
 	Near nr;
 	Far fr;
+	cd d1, d2;      // d alpha_smooth / d zbar and its zbar-derivative
 };
 
 // the rest of witt_f0 once the near-field sums of z's node are known
@@ -287,11 +301,10 @@ __device__ inline Witt witt_from(const LensView& L, cd z, double cp, double sp, 
 	w.node = node;
 	w.nr = nr;
 	w.fr = l2p(L, w.node, z);
-	cd d1, d2;
-	smooth_dzbar(L, z, d1, d2);
+	smooth_dzbar(L, z, w.d1, w.d2);
 	w.a = 1 - L.kappa - smooth_dz(L, z);
 	const double ih2 = 1.0 / (w.fr.hl * w.fr.hl);
-	w.f0 = cd(L.gamma) + w.nr.s2 * L.theta2 - w.fr.h2 * (L.theta2 * ih2) - d1.conj() - w.a * cd(cp, -sp);
+	w.f0 = cd(L.gamma) + w.nr.s2 * L.theta2 - w.fr.h2 * (L.theta2 * ih2) - w.d1.conj() - w.a * cd(cp, -sp);
 	const double a0 = w.f0.abs();
 	w.early = fabs(a0 * (a0 + 2 * w.a)) < 1e-9 && fabs(a0 * (a0 - 2 * w.a)) < 1e-9;
 	return w;
@@ -308,10 +321,8 @@ __device__ inline Witt witt_f0(const LensView& L, cd z, double phi)
 // f1 = d f0 / dz from the sums already formed for f0 (ccf_functions.cuh:108-121)
 __device__ inline cd witt_f1(const LensView& L, cd z, const Witt& w)
 {
-	cd d1, d2;
-	smooth_dzbar(L, z, d1, d2);
 	const double ih3 = 1.0 / (w.fr.hl * w.fr.hl * w.fr.hl);
-	return w.nr.s3 * (-2 * L.theta2) - w.fr.h3 * (L.theta2 * ih3) - d2.conj();
+	return w.nr.s3 * (-2 * L.theta2) - w.fr.h3 * (L.theta2 * ih3) - w.d2.conj();
 }
 
 // The device hands a launch's workgroups to its XCDs round-robin by linear
@@ -415,8 +426,10 @@ __device__ inline double err_bound(const Witt& w)
 }
 
 // lens map and magnification length scale (ccf_functions.cuh:33-45, 144-166)
-// from the near- and far-field sums of z's node
-__device__ inline void lens_map_from(const LensView& L, cd z, const Near& nr, const Far& fr, cd& wsrc, double& mu_scale)
+// from the near- and far-field sums of z's node and the smooth-matter
+// derivatives d1, d2 (smooth_dzbar) at z
+__device__ inline void lens_map_from(const LensView& L, cd z, const Near& nr, const Far& fr, cd d1, cd d2, cd& wsrc,
+	double& mu_scale)
 {
 	const double t2 = L.theta2, h = fr.hl;
 	const cd a_star = (nr.s1 * t2).conj();
@@ -425,8 +438,6 @@ __device__ inline void lens_map_from(const LensView& L, cd z, const Near& nr, co
 	const cd a_loc = (fr.h1 * (t2 / h)).conj();
 	const cd da_loc = (fr.h2 * (t2 / (h * h))).conj();
 	const cd d2a_loc = (fr.h3 * (t2 / (h * h * h))).conj();
-	cd d1, d2;
-	smooth_dzbar(L, z, d1, d2);
 	wsrc = (1 - L.kappa) * z + L.gamma * z.conj() - a_star - a_loc - smooth_alpha(L, z);
 	const double dwdz = 1 - L.kappa - smooth_dz(L, z);
 	const cd dwdzb = cd(L.gamma) - da_star - da_loc - d1;
@@ -439,7 +450,9 @@ __device__ inline void lens_map_from(const LensView& L, cd z, const Near& nr, co
 __device__ inline void lens_map(const LensView& L, cd z, cd& wsrc, double& mu_scale)
 {
 	const int node = nearest_node(L, z);
-	lens_map_from(L, z, p2p(L, node, z), l2p(L, node, z), wsrc, mu_scale);
+	cd d1, d2;
+	smooth_dzbar(L, z, d1, d2);
+	lens_map_from(L, z, p2p(L, node, z), l2p(L, node, z), d1, d2, wsrc, mu_scale);
 }
 
 } // namespace ccf

@@ -45,9 +45,11 @@ constexpr int ROOT_FMM_STEP_SWEEPS = 2;
 constexpr int ROOT_FMM_PROBE = 8;
 constexpr int OUT_BLOCKS = 8; // workgroups per row summing the roots outside the grid
 
-// listed roots from which a row's sums are cheaper by the FMM (its build
-// costs about as much as R/32 direct updates)
-inline int root_fmm_min(int R) { return std::max(32, R / 32); }
+// listed roots from which a row's sums are cheaper by the FMM: its pass
+// costs about as much as R/32 direct updates on a row of 20 000 roots; on
+// larger rows the pass's cost per root falls while every direct update
+// sums R terms, and 512 updates outweigh it
+inline int root_fmm_min(int R) { return std::max(32, std::min(R / 32, 512)); }
 
 struct RootView
 {

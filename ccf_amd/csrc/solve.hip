@@ -110,6 +110,20 @@ __device__ inline unsigned long long* stat_slot(unsigned long long* stats, int c
 	return stats + ((size_t)c * STAT_SLOTS + ((blockIdx.x + 31u * blockIdx.y) & (STAT_SLOTS - 1))) * STAT_STRIDE;
 }
 
+// points of the error pass whose node is not a leaf (outside the tree),
+// listed for a cooperative pass: count (zeroed before the pass), then up to
+// cap global point indices
+struct FarPoints
+{
+	int* count;
+	long long* list;
+	int cap;
+};
+constexpr int FAR_POINTS_CAP = 1 << 20;
+constexpr int FAR_POINTS_BLOCKS = 1024;
+
+__global__ void k_zero_count(int* p) { *p = 0; }
+
 // adds v[k] of every thread of the block to counter which[k]; every thread
 // of a BS-sized workgroup calls this
 template <int NC, int BS = BLOCK>
@@ -398,8 +412,57 @@ __global__ __launch_bounds__(BLOCK) void k_witt(LensView L, double2* __restrict_
 
 // K1 for the deferred roots: one workgroup per root sums the star ranges of
 // its node (the same terms as p2p's range loop, split over the lanes and
-// reduced in a fixed order), then finishes the root as k_witt does
-constexpr int WITT_FAR_BLOCKS = 64;
+// reduced in a fixed order), then finishes the root as k_witt does. Few
+// roots lie outside the tree, and the launch comes every sweep whether a
+// row has any or not: 16 workgroups per row, each taking every 16th
+constexpr int WITT_FAR_BLOCKS = 16;
+
+// The near-field sums of z's node (a node that is not a leaf: z lies
+// outside the tree, whose root then holds every star) over its star
+// ranges, the terms of p2p's range loop split over the workgroup's lanes
+// and reduced in a fixed order; the result is valid in thread 0. Every
+// thread of the workgroup calls it.
+__device__ inline Near coop_near(const LensView& L, int node, cd z, double (*red)[8])
+{
+	Near nr;
+	const int ranges = L.nnf[node];
+	for (int k = 0; k < ranges; k++)
+	{
+		const int2 rg = L.nf[node * MAX_NEAR_RANGES + k];
+		for (int s = rg.x + threadIdx.x; s < rg.x + rg.y; s += BLOCK)
+		{
+			const double2 q = L.spos[s];
+			const cd t = cd(z.re - q.x, z.im - q.y).inv();
+			nr.s0 += t;
+			const cd t1 = t * L.smass[s];
+			nr.s1 += t1;
+			const cd t2 = t1 * t;
+			nr.s2 += t2;
+			nr.s3 += t2 * t;
+		}
+		nr.count += rg.y;
+	}
+	double v[8] = {nr.s0.re, nr.s0.im, nr.s1.re, nr.s1.im, nr.s2.re, nr.s2.im, nr.s3.re, nr.s3.im};
+#pragma unroll
+	for (int q = 0; q < 8; q++)
+		for (int o = 32; o >= 1; o >>= 1) v[q] += __shfl_xor(v[q], o, 64);
+	if ((threadIdx.x & 63) == 0)
+		for (int q = 0; q < 8; q++) red[threadIdx.x >> 6][q] = v[q];
+	__syncthreads();
+	Near sum;
+	sum.count = nr.count;
+	if (threadIdx.x == 0)
+	{
+		double t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+		for (int w = 0; w < BLOCK / 64; w++)
+			for (int q = 0; q < 8; q++) t[q] += red[w][q];
+		sum.s0 = cd(t[0], t[1]);
+		sum.s1 = cd(t[2], t[3]);
+		sum.s2 = cd(t[4], t[5]);
+		sum.s3 = cd(t[6], t[7]);
+	}
+	return sum;
+}
 
 template <bool MIXED>
 __global__ __launch_bounds__(BLOCK) void k_witt_far(LensView L, double2* __restrict__ roots, RowMap M, SweepView V,
@@ -418,42 +481,10 @@ __global__ __launch_bounds__(BLOCK) void k_witt_far(LensView L, double2* __restr
 		double2* p = roots + (size_t)row * R + a;
 		const cd z(p->x, p->y);
 		const int node = nearest_node(L, z);
-		Near nr;
-		const int ranges = L.nnf[node];
-		for (int k = 0; k < ranges; k++)
-		{
-			const int2 rg = L.nf[node * MAX_NEAR_RANGES + k];
-			for (int s = rg.x + threadIdx.x; s < rg.x + rg.y; s += BLOCK)
-			{
-				const double2 q = L.spos[s];
-				const cd t = cd(z.re - q.x, z.im - q.y).inv();
-				nr.s0 += t;
-				const cd t1 = t * L.smass[s];
-				nr.s1 += t1;
-				const cd t2 = t1 * t;
-				nr.s2 += t2;
-				nr.s3 += t2 * t;
-			}
-			nr.count += rg.y;
-		}
-		double v[8] = {nr.s0.re, nr.s0.im, nr.s1.re, nr.s1.im, nr.s2.re, nr.s2.im, nr.s3.re, nr.s3.im};
-#pragma unroll
-		for (int q = 0; q < 8; q++)
-			for (int o = 32; o >= 1; o >>= 1) v[q] += __shfl_xor(v[q], o, 64);
-		if ((threadIdx.x & 63) == 0)
-			for (int q = 0; q < 8; q++) red[threadIdx.x >> 6][q] = v[q];
-		__syncthreads();
+		const Near nr = coop_near(L, node, z, red);
 		if (threadIdx.x == 0)
 		{
-			double t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-			for (int w = 0; w < BLOCK / 64; w++)
-				for (int q = 0; q < 8; q++) t[q] += red[w][q];
-			Near sum;
-			sum.s0 = cd(t[0], t[1]);
-			sum.s1 = cd(t[2], t[3]);
-			sum.s2 = cd(t[4], t[5]);
-			sum.s3 = cd(t[6], t[7]);
-			sum.count = nr.count;
+			const Near& sum = nr;
 			double cp, sp;
 			M.phase(row, phi, cp, sp);
 			const Witt w = witt_from(L, z, cp, sp, node, sum);
@@ -1187,7 +1218,8 @@ __global__ void k_prepare(double2* roots, RowMap M, SweepBufs B, int order, cons
 // (find_caustics_kernel, find_mu_length_scales_kernel), written in file
 // order: root a, phi row r at a * rows + r.
 __global__ void k_errors(LensView L, const double2* __restrict__ roots, int R, const double2* __restrict__ cs,
-	const int* __restrict__ perm, unsigned long long* stats, double2* __restrict__ caus, double* __restrict__ mu)
+	const int* __restrict__ perm, unsigned long long* stats, double2* __restrict__ caus, double* __restrict__ mu,
+	FarPoints far)
 {
 	// grid: (xcd_round(blocks per row), rows); a row's points in the roots'
 	// processing order
@@ -1199,22 +1231,37 @@ __global__ void k_errors(LensView L, const double2* __restrict__ roots, int R, c
 	unsigned long long bad = 0;
 	if (k < R)
 	{
-		const double2 t = cs[row];
 		const double2 v = roots[idx];
 		const cd z(v.x, v.y);
 		const int node = nearest_node(L, z);
-		Witt w = witt_from(L, z, t.x, t.y, node, p2p(L, node, z));
-		e = err_bound(w);
-		res = w.f0.abs();
-		if (!isfinite(e)) { bad = 1; e = 0; res = 0; }
-		if (caus || mu)
+		// a point outside the tree sums every star: k_errors_far takes it
+		// with a whole workgroup (inline here only past the list's capacity)
+		bool deferred = false;
+		if (L.nf_off[node] < 0)
 		{
-			cd ws;
-			double m;
-			lens_map_from(L, z, w.nr, w.fr, ws, m);
-			const size_t o = (size_t)(idx - (long long)row * R) * gridDim.y + row;
-			if (caus) caus[o] = make_double2(ws.re, ws.im);
-			if (mu) mu[o] = m;
+			const int q = atomicAdd(far.count, 1);
+			if (q < far.cap)
+			{
+				far.list[q] = idx;
+				deferred = true;
+			}
+		}
+		if (!deferred)
+		{
+			const double2 t = cs[row];
+			Witt w = witt_from(L, z, t.x, t.y, node, p2p(L, node, z));
+			e = err_bound(w);
+			res = w.f0.abs();
+			if (!isfinite(e)) { bad = 1; e = 0; res = 0; }
+			if (caus || mu)
+			{
+				cd ws;
+				double m;
+				lens_map_from(L, z, w.nr, w.fr, w.d1, w.d2, ws, m);
+				const size_t o = (size_t)(idx - (long long)row * R) * gridDim.y + row;
+				if (caus) caus[o] = make_double2(ws.re, ws.im);
+				if (mu) mu[o] = m;
+			}
 		}
 	}
 	for (int d = 32; d >= 1; d >>= 1)
@@ -1228,6 +1275,47 @@ __global__ void k_errors(LensView L, const double2* __restrict__ roots, int R, c
 		atomicMax(stat_slot(stats, C_ERRBITS), (unsigned long long)__double_as_longlong(e));
 		atomicMax(stat_slot(stats, C_RESID), (unsigned long long)__double_as_longlong(res));
 		if (bad) atomicOr(stat_slot(stats, C_NAN), 1ull);
+	}
+}
+
+// the points k_errors deferred: one workgroup per point, the near-field sums
+// over the lanes (coop_near), then the same bound, residual, caustic point
+// and length scale
+__global__ __launch_bounds__(BLOCK) void k_errors_far(LensView L, const double2* __restrict__ roots, int R, int rows,
+	const double2* __restrict__ cs, unsigned long long* stats, double2* __restrict__ caus, double* __restrict__ mu, FarPoints far)
+{
+	__shared__ double red[BLOCK / 64][8];
+	const int n = min(*far.count, far.cap);
+	for (int d = blockIdx.x; d < n; d += gridDim.x)
+	{
+		const long long idx = far.list[d];
+		const int row = (int)(idx / R);
+		const double2 v = roots[idx];
+		const cd z(v.x, v.y);
+		const int node = nearest_node(L, z);
+		const Near nr = coop_near(L, node, z, red);
+		if (threadIdx.x == 0)
+		{
+			const double2 t = cs[row];
+			const Witt w = witt_from(L, z, t.x, t.y, node, nr);
+			const double e = err_bound(w);
+			if (!isfinite(e)) atomicOr(stat_slot(stats, C_NAN), 1ull);
+			else
+			{
+				atomicMax(stat_slot(stats, C_ERRBITS), (unsigned long long)__double_as_longlong(e));
+				atomicMax(stat_slot(stats, C_RESID), (unsigned long long)__double_as_longlong(w.f0.abs()));
+			}
+			if (caus || mu)
+			{
+				cd ws;
+				double m;
+				lens_map_from(L, z, w.nr, w.fr, w.d1, w.d2, ws, m);
+				const size_t o = (size_t)(idx - (long long)row * R) * rows + row;
+				if (caus) caus[o] = make_double2(ws.re, ws.im);
+				if (mu) mu[o] = m;
+			}
+		}
+		__syncthreads();
 	}
 }
 
@@ -1469,7 +1557,7 @@ bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int
 		// stop once no root of any row is active: the remaining sweeps up to
 		// the cap would do nothing (checked at a few sweeps only, each check
 		// waits for the device)
-		if (ctx->host_counts && nrows <= ctx->max_rows && sweep_check(it + 1, cap, first_check))
+		if (ctx->host_counts && nrows <= ctx->max_rows && (ctx->sweep_verbose >= 3 || sweep_check(it + 1, cap, first_check)))
 		{
 			if (!CCF_HIP(ctx, hipMemcpyAsync(ctx->host_counts, V.cnt_next, nrows * sizeof(int), hipMemcpyDeviceToHost, s)) ||
 				!CCF_HIP(ctx, hipMemcpyAsync(ctx->host_counts + ctx->max_rows, B.plan + 2, 2 * sizeof(int), hipMemcpyDeviceToHost, s)) ||
@@ -1487,6 +1575,9 @@ bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int
 			if (B.grid.order > 0 && amax < root_fmm_min(R)) fmm_mask = 0;
 			ctx->fmm_rows_sweep[0] = ctx->host_counts[ctx->max_rows];
 			ctx->fmm_rows_sweep[1] = ctx->host_counts[ctx->max_rows + 1];
+			if (ctx->sweep_verbose >= 3)
+				std::cout << "sweep " << it + 1 << ": " << active << " roots active, at most " << amax << " in a row, "
+						  << (fmm ? "root-sum FMM" : "direct sums") << "\n";
 			if (active == 0)
 			{
 				ctx->phase_sweeps = it + 1;
@@ -1612,6 +1703,7 @@ bool find_initial_roots_dev(ccf_ctx* ctx, int verbose)
 			return false;
 	}
 	const RowMap M{R, np, nb, 0, 1, 0.0, ctx->phase};
+	ctx->sweep_verbose = verbose;
 	k_prepare<<<dim3(grid(R), nb), BLOCK, 0, s>>>(ctx->ccs_init, M, ctx->sw, 0, ctx->dorder);
 	if (!CCF_LAUNCH(ctx, "prepare_roots_kernel")) return false;
 	const int cap = ctx->P.init_iters > 0 ? ctx->P.init_iters : 60;
@@ -1646,6 +1738,7 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 	// copied back while the previous step ran (a fixed lag, so the choice
 	// does not depend on timing)
 	const bool tailed = ctx->P.precision == 1;
+	ctx->sweep_verbose = verbose;
 	for (int j = 1; j <= steps; j++)
 	{
 		const RowMap M{R, np, nb, j, 2, 0.0, ctx->phase};
@@ -1701,17 +1794,28 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 	ctx->caustics_done = ctx->mu_done = false;
 	if (wc && !ctx->caustics && !CCF_HIP(ctx, ctx->dmalloc(&ctx->caustics, (size_t)total * sizeof(double2)))) return false;
 	if (wm && !ctx->mu_scales && !CCF_HIP(ctx, ctx->dmalloc(&ctx->mu_scales, (size_t)total * sizeof(double)))) return false;
+	// the list of points outside the tree: a count, then up to FAR_POINTS_CAP indices
+	const int far_cap = (int)std::min<long long>(total, FAR_POINTS_CAP);
+	if (!ctx->far_points && !CCF_HIP(ctx, ctx->dmalloc(&ctx->far_points, sizeof(long long) * (1 + (size_t)FAR_POINTS_CAP))))
+		return false;
+	const FarPoints far{(int*)ctx->far_points, (long long*)ctx->far_points + 1, far_cap};
+	k_zero_count<<<1, 1, 0, s>>>(far.count);
 	k_errors<<<dim3(xcd_round(grid(R)), np + nb), BLOCK, 0, s>>>(L, ctx->ccs_init, R, ctx->phase, ctx->dorder, ctx->counters,
-		wc ? ctx->caustics : nullptr, wm ? ctx->mu_scales : nullptr);
+		wc ? ctx->caustics : nullptr, wm ? ctx->mu_scales : nullptr, far);
+	k_errors_far<<<FAR_POINTS_BLOCKS, BLOCK, 0, s>>>(L, ctx->ccs_init, R, np + nb, ctx->phase, ctx->counters,
+		wc ? ctx->caustics : nullptr, wm ? ctx->mu_scales : nullptr, far);
 	if (!CCF_LAUNCH(ctx, "find_errors_kernel")) return false;
 	unsigned long long c[C_COUNT] = {};
 	{
 		// fold the per-slot copies: max for the error bits, or for the NaN
 		// flag, sums otherwise
 		std::vector<unsigned long long> raw(STAT_WORDS);
+		int nfar = 0;
 		if (!CCF_HIP(ctx, hipMemcpyAsync(raw.data(), ctx->counters, STAT_WORDS * sizeof(unsigned long long), hipMemcpyDeviceToHost, s)) ||
+			!CCF_HIP(ctx, hipMemcpyAsync(&nfar, far.count, sizeof(int), hipMemcpyDeviceToHost, s)) ||
 			!CCF_HIP(ctx, hipStreamSynchronize(s)))
 			return false;
+		if (verbose >= 3) std::cout << "error pass: " << nfar << " points outside the tree\n";
 		for (int k = 0; k < C_COUNT; k++)
 			for (int q = 0; q < STAT_SLOTS; q++)
 			{
