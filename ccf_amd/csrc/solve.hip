@@ -422,6 +422,7 @@ constexpr int WITT_FAR_BLOCKS = 16;
 // ranges, the terms of p2p's range loop split over the workgroup's lanes
 // and reduced in a fixed order; the result is valid in thread 0. Every
 // thread of the workgroup calls it.
+template <int BS = BLOCK>
 __device__ inline Near coop_near(const LensView& L, int node, cd z, double (*red)[8])
 {
 	Near nr;
@@ -429,7 +430,7 @@ __device__ inline Near coop_near(const LensView& L, int node, cd z, double (*red
 	for (int k = 0; k < ranges; k++)
 	{
 		const int2 rg = L.nf[node * MAX_NEAR_RANGES + k];
-		for (int s = rg.x + threadIdx.x; s < rg.x + rg.y; s += BLOCK)
+		for (int s = rg.x + threadIdx.x; s < rg.x + rg.y; s += BS)
 		{
 			const double2 q = L.spos[s];
 			const cd t = cd(z.re - q.x, z.im - q.y).inv();
@@ -454,7 +455,7 @@ __device__ inline Near coop_near(const LensView& L, int node, cd z, double (*red
 	if (threadIdx.x == 0)
 	{
 		double t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-		for (int w = 0; w < BLOCK / 64; w++)
+		for (int w = 0; w < BS / 64; w++)
 			for (int q = 0; q < 8; q++) t[q] += red[w][q];
 		sum.s0 = cd(t[0], t[1]);
 		sum.s1 = cd(t[2], t[3]);
@@ -1006,10 +1007,57 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(double theta, double2* __res
 // see each other's global writes across its barriers.
 constexpr int TAIL_BLOCK = 512;
 constexpr int TAIL_LDS = 128;
+constexpr int TAIL_DEFER = 64; // roots outside the tree per sweep taken by the whole workgroup
 // sweeps of a continuation step before the tail launch: the second sweep
 // still lists hundreds of roots per row, which the whole device takes
 // faster than one workgroup per row
 constexpr int TAIL_AT = 2;
+
+// k_tail: the outcome of root a's evaluation w, as in k_witt: listed for
+// the sums (entry k of the listed roots: LDS for k < TAIL_LDS, else the
+// global slices), active again next sweep (the next list), or done
+__device__ __forceinline__ void tail_settle(const LensView& L, int a, double2* pz, cd z, const Witt& w, int circ_exact, int fmm_pole,
+	bool newton, int& s_nf, int& s_nn, int* s_fl, double2* s_f0, double2* s_f1, double2* s_ex, int* s_next, int* gfl, double2* gf0,
+	double2* gf1, double2* gex, int* gnxt, unsigned long long& nfull, unsigned long long& nearly, unsigned long long& nnewton,
+	unsigned long long& nconv, unsigned long long& nnear)
+{
+	cd f1, ex;
+	const int st = witt_root<true>(L, pz, z, w, circ_exact, fmm_pole, newton, f1, ex);
+	nnear += w.nr.count;
+	if (st == ROOT_FULL)
+	{
+		const int k = atomicAdd(&s_nf, 1);
+		const double2 v0 = make_double2(w.f0.re, w.f0.im), v1 = make_double2(f1.re, f1.im), ve = make_double2(ex.re, ex.im);
+		if (k < TAIL_LDS)
+		{
+			s_fl[k] = a;
+			s_f0[k] = v0;
+			s_f1[k] = v1;
+			s_ex[k] = ve;
+		}
+		else
+		{
+			gfl[k] = a;
+			gf0[k] = v0;
+			gf1[k] = v1;
+			gex[k] = ve;
+		}
+		nfull++;
+	}
+	else if (st == ROOT_STEPPED)
+	{
+		const int k = atomicAdd(&s_nn, 1);
+		if (k < TAIL_LDS) s_next[k] = a;
+		else gnxt[k] = a;
+		nnewton++;
+	}
+	else
+	{
+		nconv++;
+		if (st == ROOT_NEWTON) nnewton++;
+		else nearly++;
+	}
+}
 
 __global__ __launch_bounds__(TAIL_BLOCK) void k_tail(LensView L, double2* __restrict__ roots, RowMap M, SweepView V,
 	unsigned long long* __restrict__ stats, int circ_exact, int fmm_pole, int cap, float cluster_d2,
@@ -1019,6 +1067,8 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_tail(LensView L, double2* __rest
 	__shared__ int s_fl[TAIL_LDS];
 	__shared__ double2 s_f0[TAIL_LDS], s_f1[TAIL_LDS], s_ex[TAIL_LDS];
 	__shared__ int s_nf, s_nn; // listed for the sums this sweep, active for the next
+	__shared__ int s_def[TAIL_DEFER], s_nd; // roots outside the tree, evaluated by the whole workgroup
+	__shared__ double s_red[TAIL_BLOCK / 64][8];
 	constexpr int NW = TAIL_BLOCK / 64;
 	const int r = blockIdx.x, R = M.R;
 	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -1029,15 +1079,17 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_tail(LensView L, double2* __rest
 	M.phase(row, phi, cp, sp);
 	double2* __restrict__ xrow = roots + (size_t)row * R;
 	const size_t base = (size_t)r * R;
-	int* const ga[2] = {const_cast<int*>(V.act) + base, V.act_next + base};
+	// the lists' global parts: cur this sweep, nxt the next (swapped with p)
+	int* gcur = const_cast<int*>(V.act) + base;
+	int* gnxt = V.act_next + base;
 	int* __restrict__ gfl = V.fl + base;
 	double2* __restrict__ gf0 = V.f0 + base;
 	double2* __restrict__ gf1 = V.f1 + base;
 	double2* __restrict__ gex = V.ex + base;
 	int n = V.cnt[r];
 	// the first list's head into LDS; its other entries stay where they are
-	for (int q = threadIdx.x; q < min(n, TAIL_LDS); q += TAIL_BLOCK) s_a[0][q] = ga[0][q];
-	if (threadIdx.x == 0) s_nf = s_nn = 0;
+	for (int q = threadIdx.x; q < min(n, TAIL_LDS); q += TAIL_BLOCK) s_a[0][q] = gcur[q];
+	if (threadIdx.x == 0) s_nf = s_nn = s_nd = 0;
 	__syncthreads();
 	unsigned long long nfull = 0, nearly = 0, nnewton = 0, nconv = 0, nnear = 0, nclus = 0, nsweeps = 0;
 	int p = 0;
@@ -1046,49 +1098,42 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_tail(LensView L, double2* __rest
 		nsweeps++;
 		for (int q = threadIdx.x; q < n; q += TAIL_BLOCK)
 		{
-			const int a = q < TAIL_LDS ? s_a[p][q] : ga[p][q];
+			const int a = q < TAIL_LDS ? s_a[p][q] : gcur[q];
 			double2* pz = xrow + a;
 			const cd z(pz->x, pz->y);
 			const int node = nearest_node(L, z);
+			// outside the tree (every star in the near field): below, with
+			// the whole workgroup (inline only past TAIL_DEFER of them)
+			if (L.nf_off[node] < 0)
+			{
+				const int k = atomicAdd(&s_nd, 1);
+				if (k < TAIL_DEFER)
+				{
+					s_def[k] = a;
+					continue;
+				}
+			}
 			const Witt w = witt_from(L, z, cp, sp, node, p2p(L, node, z));
-			cd f1, ex;
-			const int st = witt_root<true>(L, pz, z, w, circ_exact, fmm_pole, M.j > 0, f1, ex);
-			nnear += w.nr.count;
-			if (st == ROOT_FULL)
-			{
-				const int k = atomicAdd(&s_nf, 1);
-				const double2 v0 = make_double2(w.f0.re, w.f0.im), v1 = make_double2(f1.re, f1.im), ve = make_double2(ex.re, ex.im);
-				if (k < TAIL_LDS)
-				{
-					s_fl[k] = a;
-					s_f0[k] = v0;
-					s_f1[k] = v1;
-					s_ex[k] = ve;
-				}
-				else
-				{
-					gfl[k] = a;
-					gf0[k] = v0;
-					gf1[k] = v1;
-					gex[k] = ve;
-				}
-				nfull++;
-			}
-			else if (st == ROOT_STEPPED)
-			{
-				const int k = atomicAdd(&s_nn, 1);
-				if (k < TAIL_LDS) s_a[p ^ 1][k] = a;
-				else ga[p ^ 1][k] = a;
-				nnewton++;
-			}
-			else
-			{
-				nconv++;
-				if (st == ROOT_NEWTON) nnewton++;
-				else nearly++;
-			}
+			tail_settle(L, a, pz, z, w, circ_exact, fmm_pole, M.j > 0, s_nf, s_nn, s_fl, s_f0, s_f1, s_ex, s_a[p ^ 1], gfl, gf0, gf1, gex,
+				gnxt, nfull, nearly, nnewton, nconv, nnear);
 		}
 		__syncthreads();
+		const int nd = min(s_nd, TAIL_DEFER);
+		for (int d = 0; d < nd; d++)
+		{
+			const int a = s_def[d];
+			double2* pz = xrow + a;
+			const cd z(pz->x, pz->y);
+			const int node = nearest_node(L, z);
+			const Near nr = coop_near<TAIL_BLOCK>(L, node, z, s_red);
+			if (threadIdx.x == 0)
+			{
+				const Witt w = witt_from(L, z, cp, sp, node, nr);
+				tail_settle(L, a, pz, z, w, circ_exact, fmm_pole, M.j > 0, s_nf, s_nn, s_fl, s_f0, s_f1, s_ex, s_a[p ^ 1], gfl, gf0, gf1,
+					gex, gnxt, nfull, nearly, nnewton, nconv, nnear);
+			}
+			__syncthreads();
+		}
 		const int nf = s_nf;
 		for (int g = wv; g < nf; g += NW)
 		{
@@ -1155,14 +1200,17 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_tail(LensView L, double2* __rest
 			{
 				const int k = atomicAdd(&s_nn, 1);
 				if (k < TAIL_LDS) s_a[p ^ 1][k] = a;
-				else ga[p ^ 1][k] = a;
+				else gnxt[k] = a;
 			}
 		}
 		__syncthreads();
 		n = s_nn;
 		p ^= 1;
+		int* const t = gcur;
+		gcur = gnxt;
+		gnxt = t;
 		__syncthreads();
-		if (threadIdx.x == 0) s_nf = s_nn = 0;
+		if (threadIdx.x == 0) s_nf = s_nn = s_nd = 0;
 		__syncthreads();
 	}
 	// the row is finished (or out of sweeps): its list is empty for any
