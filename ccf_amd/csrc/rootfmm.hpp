@@ -449,6 +449,21 @@ __global__ __launch_bounds__(32 * down_targets<P>()) void k_rdown32(RootView V, 
 // per listed root of a row in FMM mode, in cell order: far field by L2P,
 // near field and outliers directly; rsum = -S (the root part of the
 // Aberth correction, the sign that the direct kernels' partials carry)
+// one direct Cauchy term (z - u) / |z - u|^2 added to (ax, ay), in FP64.
+// At orders up to 10, whose far field carries far larger relative errors,
+// the reciprocal is the hardware estimate alone (relative error about
+// 2^-23); above, rcp64 refines it to full precision. The own root gives a
+// zero difference and, floored, a zero term.
+template <int P>
+__device__ __forceinline__ void direct_term(cd z, double2 u, double& ax, double& ay)
+{
+	const double dx = z.re - u.x, dy = z.im - u.y;
+	const double d2 = fmax(fma(dx, dx, dy * dy), 1e-300);
+	const double iv = P <= 10 ? __builtin_amdgcn_rcp(d2) : rcp64(d2);
+	ax = fma(dx, iv, ax);
+	ay = fma(dy, iv, ay);
+}
+
 template <int P>
 __global__ __launch_bounds__(BLOCK) void k_rsum(RootView V, const double2* __restrict__ roots, RowMap M,
 	unsigned long long* __restrict__ stats)
@@ -481,14 +496,7 @@ __global__ __launch_bounds__(BLOCK) void k_rsum(RootView V, const double2* __res
 			{
 				const int q0 = st[y * nxl + max(ix - 1, 0)], q1 = st[y * nxl + min(ix + 1, nxl - 1) + 1];
 				nn += q1 - q0;
-				for (int q = q0; q < q1; q++)
-				{
-					const double2 u = sp[q];
-					const double dx = z.re - u.x, dy = z.im - u.y;
-					const double iv = rcp64(fmax(fma(dx, dx, dy * dy), 1e-300));
-					ax = fma(dx, iv, ax);
-					ay = fma(dy, iv, ay);
-				}
+				for (int q = q0; q < q1; q++) direct_term<P>(z, sp[q], ax, ay);
 			}
 			const int no = V.ocnt[r];
 			if (no > 0)
@@ -496,14 +504,7 @@ __global__ __launch_bounds__(BLOCK) void k_rsum(RootView V, const double2* __res
 				int row;
 				double phi;
 				M.get(r, row, phi);
-				for (int o = 0; o < no; o++)
-				{
-					const double2 u = roots[(size_t)row * R + V.oidx[(size_t)r * R + o]];
-					const double dx = z.re - u.x, dy = z.im - u.y;
-					const double iv = rcp64(fmax(fma(dx, dx, dy * dy), 1e-300));
-					ax = fma(dx, iv, ax);
-					ay = fma(dy, iv, ay);
-				}
+				for (int o = 0; o < no; o++) direct_term<P>(z, roots[(size_t)row * R + V.oidx[(size_t)r * R + o]], ax, ay);
 			}
 			S += cd(ax, -ay);
 			V.rsum[(size_t)r * R + s] = make_double2(-S.re, -S.im);
@@ -515,8 +516,12 @@ __global__ __launch_bounds__(BLOCK) void k_rsum(RootView V, const double2* __res
 	block_stats(stats, which, v);
 }
 
-// listed roots outside the grid: direct sums over the whole row, one block
-// per row, one root at a time
+// listed roots outside the grid: direct sums over the whole row. Block b
+// of a row takes its outliers b, b + OUT_BLOCKS, ..., OUT_GROUP at a time,
+// so that each pass over the row's roots serves a group of them.
+constexpr int OUT_GROUP = 8;
+
+template <int P>
 __global__ __launch_bounds__(BLOCK) void k_rsum_out(RootView V, const double2* __restrict__ roots, RowMap M,
 	unsigned long long* __restrict__ stats)
 {
@@ -530,42 +535,55 @@ __global__ __launch_bounds__(BLOCK) void k_rsum_out(RootView V, const double2* _
 	double phi;
 	M.get(r, row, phi);
 	const double2* zr = roots + (size_t)row * R;
-	__shared__ double red[2][BLOCK / 64];
-	for (int o = blockIdx.y; o < no; o += gridDim.y)
+	__shared__ double red[OUT_GROUP][2][BLOCK / 64];
+	for (int o0 = blockIdx.y; o0 < no; o0 += gridDim.y * OUT_GROUP)
 	{
-		const int j = V.oidx[(size_t)r * R + o];
-		const int s = V.slot[(size_t)r * R + j];
-		if (s < 0) continue;
-		const double2 z = zr[j];
-		double ax = 0, ay = 0;
+		// the group: outliers o0 + k gridDim.y that are listed (slot >= 0)
+		int j[OUT_GROUP], sl[OUT_GROUP];
+		cd z[OUT_GROUP];
+		double ax[OUT_GROUP], ay[OUT_GROUP];
+#pragma unroll
+		for (int k = 0; k < OUT_GROUP; k++)
+		{
+			const int o = o0 + k * gridDim.y;
+			j[k] = o < no ? V.oidx[(size_t)r * R + o] : -1;
+			sl[k] = j[k] >= 0 ? V.slot[(size_t)r * R + j[k]] : -1;
+			const double2 v = sl[k] >= 0 ? zr[j[k]] : make_double2(0, 0);
+			z[k] = cd(v.x, v.y);
+			ax[k] = ay[k] = 0;
+		}
 		for (int q = threadIdx.x; q < R; q += BLOCK)
 		{
-			if (q == j) continue;
-			const double dx = z.x - zr[q].x, dy = z.y - zr[q].y;
-			const double iv = rcp64(fmax(fma(dx, dx, dy * dy), 1e-300));
-			ax = fma(dx, iv, ax);
-			ay = fma(dy, iv, ay);
+			const double2 u = zr[q];
+#pragma unroll
+			for (int k = 0; k < OUT_GROUP; k++)
+				if (sl[k] >= 0 && q != j[k]) direct_term<P>(z[k], u, ax[k], ay[k]);
 		}
-		for (int d = 32; d >= 1; d >>= 1)
+#pragma unroll
+		for (int k = 0; k < OUT_GROUP; k++)
 		{
-			ax += __shfl_xor(ax, d, 64);
-			ay += __shfl_xor(ay, d, 64);
-		}
-		if ((threadIdx.x & 63) == 0)
-		{
-			red[0][threadIdx.x >> 6] = ax;
-			red[1][threadIdx.x >> 6] = ay;
+			for (int d = 32; d >= 1; d >>= 1)
+			{
+				ax[k] += __shfl_xor(ax[k], d, 64);
+				ay[k] += __shfl_xor(ay[k], d, 64);
+			}
+			if ((threadIdx.x & 63) == 0)
+			{
+				red[k][0][threadIdx.x >> 6] = ax[k];
+				red[k][1][threadIdx.x >> 6] = ay[k];
+			}
 		}
 		__syncthreads();
-		if (threadIdx.x == 0)
+		if (threadIdx.x < OUT_GROUP && sl[threadIdx.x] >= 0)
 		{
+			const int k = threadIdx.x;
 			double sx = 0, sy = 0;
 			for (int w = 0; w < BLOCK / 64; w++)
 			{
-				sx += red[0][w];
-				sy += red[1][w];
+				sx += red[k][0][w];
+				sy += red[k][1][w];
 			}
-			V.rsum[(size_t)r * R + s] = make_double2(-sx, sy);
+			V.rsum[(size_t)r * R + sl[k]] = make_double2(-sx, sy);
 			atomicAdd(stat_slot(stats, C_RTGT), 1ull);
 		}
 		__syncthreads();
@@ -590,7 +608,7 @@ void root_fmm_pass_p(const SweepBufs& B, const double2* roots, const RowMap& M, 
 	for (int l = 0; l <= g.leaf; l++)
 		k_rdown32<P><<<dim3(grid((g.nx << l) * (g.ny << l), TPB), nrows), 32 * TPB, 0, s>>>(V, l, stats);
 	k_rsum<P><<<dim3(grid(R), nrows), BLOCK, 0, s>>>(V, roots, M, stats);
-	k_rsum_out<<<dim3(nrows, OUT_BLOCKS), BLOCK, 0, s>>>(V, roots, M, stats);
+	k_rsum_out<P><<<dim3(nrows, OUT_BLOCKS), BLOCK, 0, s>>>(V, roots, M, stats);
 }
 
 // compiled expansion orders; a requested order runs at the next one up
