@@ -99,6 +99,7 @@ __global__ __launch_bounds__(BLOCK) void k_rbin(RootView V, const double2* __res
 	const int R = M.R;
 	const int j = blockIdx.x * BLOCK + threadIdx.x;
 	bool out = false;
+	int c = -1;
 	if (j < R)
 	{
 		int row;
@@ -108,18 +109,12 @@ __global__ __launch_bounds__(BLOCK) void k_rbin(RootView V, const double2* __res
 		const double sl = ldexp(V.g.s0, -V.g.leaf);
 		const double fx = (z.x - V.g.x0) / sl, fy = (z.y - V.g.y0) / sl;
 		const int nxl = V.g.nx << V.g.leaf, nyl = V.g.ny << V.g.leaf;
-		if (fx >= 0 && fx < nxl && fy >= 0 && fy < nyl)
-		{
-			const int c = (int)fy * nxl + (int)fx;
-			V.cell[(size_t)r * R + j] = c;
-			atomicAdd(&V.ccur[(size_t)r * V.g.nleaf + c], 1);
-		}
-		else
-		{
-			V.cell[(size_t)r * R + j] = -1;
-			out = true;
-		}
+		if (fx >= 0 && fx < nxl && fy >= 0 && fy < nyl) c = (int)fy * nxl + (int)fx;
+		else out = true;
+		V.cell[(size_t)r * R + j] = c;
 	}
+	// neighbouring roots mostly share a cell: one atomic per run of them
+	(void)run_add(V.ccur + (size_t)r * V.g.nleaf, c);
 	const int pos = wave_append(&V.ocnt[r], out);
 	if (out) V.oidx[(size_t)r * R + pos] = j;
 }
@@ -174,13 +169,9 @@ __global__ __launch_bounds__(BLOCK) void k_rscatter(RootView V, int R)
 	const int r = blockIdx.y;
 	if (!V.rmode[r]) return;
 	const int j = blockIdx.x * BLOCK + threadIdx.x;
-	if (j >= R) return;
-	const int c = V.cell[(size_t)r * R + j];
-	if (c >= 0)
-	{
-		const int pos = atomicAdd(&V.ccur[(size_t)r * V.g.nleaf + c], 1);
-		V.sidx[(size_t)r * R + pos] = j;
-	}
+	const int c = j < R ? V.cell[(size_t)r * R + j] : -1;
+	const int pos = run_add(V.ccur + (size_t)r * V.g.nleaf, c);
+	if (c >= 0) V.sidx[(size_t)r * R + pos] = j;
 }
 
 // per leaf cell: the cell's roots in index order, their positions copied

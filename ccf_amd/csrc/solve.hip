@@ -98,6 +98,27 @@ __device__ inline int wave_append(int* counter, bool pred)
 	return pred ? base + (int)__popcll(m & below) : -1;
 }
 
+// counters[key] += 1 for every lane with key >= 0, one atomic per run of
+// equal keys in consecutive lanes (the run's first lane adds the run's
+// length); returns the counter's value before the lane's own increment
+// (-1 for key < 0). Every lane of the wave calls it.
+__device__ inline int run_add(int* counters, int key)
+{
+	const int lane = threadIdx.x & 63;
+	const int prev = __shfl_up(key, 1, 64);
+	const unsigned long long starts = __ballot(lane == 0 || prev != key);
+	// the lane's run: from the highest start at or below the lane to the
+	// next start above it (or the wave's end)
+	const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+	const int first = 63 - __clzll(starts & upto);
+	const unsigned long long above = starts & ~upto;
+	const int end = above ? __ffsll(above) - 1 : 64;
+	int base = 0;
+	if (lane == first && key >= 0) base = atomicAdd(&counters[key], end - first);
+	base = __shfl(base, first, 64);
+	return key >= 0 ? base + (lane - first) : -1;
+}
+
 // Device statistics are spread over STAT_SLOTS copies of each counter, one
 // per 128-byte line, and added once per workgroup: atomics from thousands of
 // workgroups on one address serialise at the L2. The host folds the copies.
