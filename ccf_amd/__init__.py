@@ -111,6 +111,7 @@ class Info(ctypes.Structure):
         ("tail_sweeps", ctypes.c_longlong),
         ("tail_kernel_ms", ctypes.c_double),
         ("flops_tail", ctypes.c_double),
+        ("points_outside_tree", ctypes.c_longlong),
     ]
 
     def as_dict(self):

@@ -90,15 +90,14 @@ __global__ void k_rzero(RootView V)
 	if (c == 0) V.ocnt[r] = 0;
 }
 
-// leaf cell of every root of the row; roots outside the grid (or not finite)
-// go to the row's outlier list
+// leaf cell of every root of the row (-1 outside the grid or not finite;
+// k_rscan lists those)
 __global__ __launch_bounds__(BLOCK) void k_rbin(RootView V, const double2* __restrict__ roots, RowMap M)
 {
 	const int r = blockIdx.y;
 	if (!V.rmode[r]) return;
 	const int R = M.R;
 	const int j = blockIdx.x * BLOCK + threadIdx.x;
-	bool out = false;
 	int c = -1;
 	if (j < R)
 	{
@@ -110,17 +109,14 @@ __global__ __launch_bounds__(BLOCK) void k_rbin(RootView V, const double2* __res
 		const double fx = (z.x - V.g.x0) / sl, fy = (z.y - V.g.y0) / sl;
 		const int nxl = V.g.nx << V.g.leaf, nyl = V.g.ny << V.g.leaf;
 		if (fx >= 0 && fx < nxl && fy >= 0 && fy < nyl) c = (int)fy * nxl + (int)fx;
-		else out = true;
 		V.cell[(size_t)r * R + j] = c;
 	}
 	// neighbouring roots mostly share a cell: one atomic per run of them
 	(void)run_add(V.ccur + (size_t)r * V.g.nleaf, c);
-	const int pos = wave_append(&V.ocnt[r], out);
-	if (out) V.oidx[(size_t)r * R + pos] = j;
 }
 
 // exclusive prefix of the leaf counts: first sorted position of each cell
-__global__ __launch_bounds__(1024) void k_rscan(RootView V, unsigned long long* __restrict__ stats)
+__global__ __launch_bounds__(1024) void k_rscan(RootView V, int R, unsigned long long* __restrict__ stats)
 {
 	const int r = blockIdx.x;
 	if (!V.rmode[r]) return;
@@ -162,6 +158,32 @@ __global__ __launch_bounds__(1024) void k_rscan(RootView V, unsigned long long* 
 		__syncthreads();
 	}
 	if (threadIdx.x == 0) st[nl] = carry;
+	// the row's roots outside the grid (cell -1), listed in index order so
+	// that every sum over them runs in one order
+	const int* cell = V.cell + (size_t)r * R;
+	int* oidx = V.oidx + (size_t)r * R;
+	if (threadIdx.x == 0) carry = 0;
+	__syncthreads();
+	const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+	for (int base = 0; base < R; base += blockDim.x)
+	{
+		const int j = base + threadIdx.x;
+		const bool o = j < R && cell[j] < 0;
+		const unsigned long long m = __ballot(o);
+		if (lane == 0) wsum[w] = (int)__popcll(m);
+		__syncthreads();
+		int woff = 0, tot = 0;
+		for (int k = 0; k < nw; k++)
+		{
+			if (k < w) woff += wsum[k];
+			tot += wsum[k];
+		}
+		if (o) oidx[carry + woff + (int)__popcll(m & below)] = j;
+		__syncthreads();
+		if (threadIdx.x == 0) carry += tot;
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) V.ocnt[r] = carry;
 }
 
 __global__ __launch_bounds__(BLOCK) void k_rscatter(RootView V, int R)
@@ -590,7 +612,7 @@ void root_fmm_pass_p(const SweepBufs& B, const double2* roots, const RowMap& M, 
 	const int R = M.R;
 	k_rzero<<<dim3(grid(g.nleaf), nrows), BLOCK, 0, s>>>(V);
 	k_rbin<<<dim3(grid(R), nrows), BLOCK, 0, s>>>(V, roots, M);
-	k_rscan<<<nrows, 1024, 0, s>>>(V, stats);
+	k_rscan<<<nrows, 1024, 0, s>>>(V, R, stats);
 	k_rscatter<<<dim3(grid(R), nrows), BLOCK, 0, s>>>(V, R);
 	k_rleaf<P><<<dim3(grid(g.nleaf), nrows), BLOCK, 0, s>>>(V, roots, M);
 	for (int l = g.leaf - 1; l >= 0; l--)

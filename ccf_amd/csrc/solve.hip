@@ -82,22 +82,6 @@ __device__ inline unsigned long long wave_sum(unsigned long long v)
 	return v;
 }
 
-// Append to a list whose length is *counter: every lane of the wave calls
-// this; lanes with pred get consecutive slots from a single atomic per
-// wave (per-lane atomics on one address serialise at the L2).
-__device__ inline int wave_append(int* counter, bool pred)
-{
-	const unsigned long long m = __ballot(pred);
-	if (m == 0) return -1;
-	const int lane = threadIdx.x & 63;
-	const int leader = __ffsll((unsigned long long)m) - 1;
-	int base = 0;
-	if (lane == leader) base = atomicAdd(counter, (int)__popcll(m));
-	base = __shfl(base, leader, 64);
-	const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-	return pred ? base + (int)__popcll(m & below) : -1;
-}
-
 // counters[key] += 1 for every lane with key >= 0, one atomic per run of
 // equal keys in consecutive lanes (the run's first lane adds the run's
 // length); returns the counter's value before the lane's own increment
@@ -1884,6 +1868,7 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 			!CCF_HIP(ctx, hipMemcpyAsync(&nfar, far.count, sizeof(int), hipMemcpyDeviceToHost, s)) ||
 			!CCF_HIP(ctx, hipStreamSynchronize(s)))
 			return false;
+		ctx->info.points_outside_tree = nfar;
 		if (verbose >= 3) std::cout << "error pass: " << nfar << " points outside the tree\n";
 		for (int k = 0; k < C_COUNT; k++)
 			for (int q = 0; q < STAT_SLOTS; q++)

@@ -116,6 +116,8 @@ typedef struct ccf_info
 	long long tail_sweeps;     /* their sweeps, summed over rows */
 	double tail_kernel_ms;     /* sum of timed tail launches (params.timing = 1) */
 	double flops_tail;         /* algorithmic flops of the tails (Witt evaluations and Cauchy sums) */
+	long long points_outside_tree; /* error pass: points outside the star tree's root box, whose near
+	                              field is every star (evaluated with a whole workgroup each) */
 } ccf_info;
 
 /* Arrays readable with ccf_get_array. */

@@ -124,3 +124,34 @@ def test_mixed_batch_with_predictor_within_band(workdir, predictor):
         assert not over, over[:5]
         assert nbad <= max(5, 1e-3 * R * (RUN["num_phi"] + nb)), nbad
         assert gi["points_converged"] == R * (RUN["num_phi"] + nb)
+
+
+def test_batch_rebuild_with_more_fields_is_exact(workdir):
+    """A batch solved, grown by more fields and solved again (a larger result
+    arena, a new forest and graph, buffers from the pool) gives its first
+    fields bit-identical results, and the new fields those of a batch of
+    their own (one expansion order for all, so that it does not change)."""
+    files = _star_files(workdir)
+    kw = dict(RUN, fmm_order=20, precision=1, predictor=4)
+
+    def add(b, group, tag):
+        for i, (sf, spec) in group:
+            b.add(starfile=sf, outfile_prefix=os.path.join(str(workdir), f"{tag}{i}_"), **_field_kw(spec), **kw)
+
+    first = list(enumerate(files))[:3]
+    rest = list(enumerate(files))[3:]
+    b = ccf_amd.Batch()
+    add(b, first, "rb")
+    assert b.run(0), b.last_error()
+    before = [b.array(k, ccf_amd.ARRAY_CCS) for k in range(len(first))]
+    add(b, rest, "rb")
+    assert b.run(0), b.last_error()
+    for k in range(len(first)):
+        assert np.array_equal(b.array(k, ccf_amd.ARRAY_CCS), before[k])
+    alone = ccf_amd.Batch()
+    add(alone, rest, "ra")
+    assert alone.run(0), alone.last_error()
+    for k in range(len(rest)):
+        assert np.array_equal(b.array(len(first) + k, ccf_amd.ARRAY_CCS), alone.array(k, ccf_amd.ARRAY_CCS))
+    b.close()
+    alone.close()

@@ -286,3 +286,57 @@ def test_config2_full_size_matches_oracle(workdir):
         worst, nbad = compare_arrays(io.read_array(gp + f), io.read_array(op + f), R, 10)
         assert nbad == 0, (f, worst, nbad)
     assert gi["max_error"] < 1e-8 and np.isfinite(gi["max_witt_residual"])
+
+
+@pytest.mark.parametrize("gkw", [{}, dict(precision=1, predictor=4, root_fmm=1)])
+def test_context_reuse_is_exact(workdir, gkw):
+    """One context solving a field, a larger one, then the first again (its
+    device buffers come back from the context's pool in other roles) gives
+    bit-identical results to fresh contexts: the solve is deterministic and
+    nothing it reads survives from an earlier run."""
+    runs = [("salpeter300", dict(kappa_tot=0.45, shear=0.35, num_phi=60, num_branches=3, fmm_order=20)),
+            ("rect200", dict(kappa_tot=0.7, shear=0.25, num_phi=40, num_branches=2, fmm_order=0))]
+
+    def solve(c):
+        assert c.run(0), c.last_error()
+        return c.array(ccf_amd.ARRAY_CCS), c.array(ccf_amd.ARRAY_CAUSTICS), c.info()
+
+    fresh = []
+    for name, kw in runs:
+        with _ctx(make_field(workdir, name), str(workdir) + "/reuse_", write_stars=0, write_critical_curves=0,
+                  write_caustics=1, **kw, **gkw) as c:
+            fresh.append(solve(c))
+    (n0, kw0), (n1, kw1) = runs
+    with _ctx(make_field(workdir, n0), str(workdir) + "/reuse_", write_stars=0, write_critical_curves=0,
+              write_caustics=1, **kw0, **gkw) as c:
+        seq = [solve(c)]
+        c.set(starfile=make_field(workdir, n1), **kw1)
+        seq.append(solve(c))
+        c.set(starfile=make_field(workdir, n0), **kw0)
+        seq.append(solve(c))
+    for (a, ac, ai), (b, bc, bi) in zip(seq, fresh + fresh[:1]):
+        assert a.shape == b.shape and np.array_equal(a, b)
+        assert np.array_equal(ac, bc)
+        assert ai["max_error"] == bi["max_error"] and ai["points_converged"] == bi["points_converged"]
+
+
+def test_error_pass_takes_points_outside_the_tree(workdir):
+    """The rectangular field's critical curves leave the star tree's root box:
+    the error pass evaluates those points with a whole workgroup each, and
+    their caustics and the error maximum match the oracle (the full file
+    comparison is test_run_matches_oracle)."""
+    sf = make_field(workdir, "rect200")
+    kw = dict(kappa_tot=0.7, shear=0.25, num_phi=80, num_branches=2)
+    with _ctx(sf, str(workdir) + "/out_", write_stars=0, write_critical_curves=0, write_caustics=1, **kw) as c:
+        assert c.run(0), c.last_error()
+        inf = c.info()
+        ccs = c.array(ccf_amd.ARRAY_CCS).ravel()
+        cau = c.array(ccf_amd.ARRAY_CAUSTICS).ravel()
+    nstar, rect, corner, _, _ = io.read_stars(sf)
+    hl = 1.1 * max(abs(corner.real), abs(corner.imag))
+    outside = (np.abs(ccs.real) > hl) | (np.abs(ccs.imag) > hl)
+    assert inf["points_outside_tree"] == int(outside.sum()) > 0
+    okw = {k: v for k, v in kw.items() if k not in ("num_phi", "num_branches")}
+    o, _ = oracle.evaluate(oracle.params(starfile=sf, **okw), ccs[outside], 0.0)
+    w = o[:, 4] + 1j * o[:, 5]
+    assert np.max(np.abs(cau[outside] - w) / np.maximum(np.abs(w), 1.0)) < 1e-11

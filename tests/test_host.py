@@ -2,6 +2,7 @@
 loads and exports every declared symbol, parameter validation and derived
 parameters follow the reference, files round-trip, and the ccf_gpu command
 line parses options like src/main_ccf.cu."""
+import ctypes
 import os
 import re
 import subprocess
@@ -28,6 +29,28 @@ def test_library_exports_every_declared_symbol():
     lib = ccf_amd.load()
     for n in names:
         assert hasattr(lib, n)
+
+
+def _header_struct(hdr: str, name: str):
+    """(type, field) pairs of `typedef struct name { ... } name;` in the header"""
+    body = re.search(r"typedef struct %s\s*\{(.*?)\}\s*%s;" % (name, name), hdr, re.S).group(1)
+    return re.findall(r"^\s*((?:unsigned\s+)?(?:long long|int|double|char|float))\s+(\w+)(?:\[\w+\])?\s*;", body, re.M)
+
+
+@pytest.mark.parametrize("cname,pyname", [("ccf_params", "Params"), ("ccf_info", "Info")])
+def test_python_structs_mirror_the_header(cname, pyname):
+    """The ctypes mirrors list the header's fields in its order with its
+    types, so that ccf_get_info / ccf_set_params read and write in place."""
+    hdr = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "ccf_amd.h")).read(), flags=re.S)
+    fields = _header_struct(hdr, cname)
+    py = getattr(ccf_amd, pyname)._fields_
+    assert [f for _, f in fields] == [f for f, _ in py]
+    ctype = {"int": ctypes.c_int, "double": ctypes.c_double, "long long": ctypes.c_longlong, "float": ctypes.c_float}
+    for (t, f), (_, pt) in zip(fields, py):
+        if t == "char":
+            assert issubclass(pt, ctypes.Array) and pt._type_ is ctypes.c_char, f
+        else:
+            assert pt is ctype[t], (f, t, pt)
 
 
 def test_default_params_are_the_reference_defaults():
