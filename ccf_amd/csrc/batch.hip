@@ -42,7 +42,8 @@ namespace
 {
 
 constexpr int BLOCK = 256;
-constexpr int MAX_BATCH_RPT = 8;
+// workgroups per CU the list buffers of the persistent grid are sized for
+constexpr int MAX_BATCH_PER_CU = 8;
 
 inline int grid(long long n, int b = BLOCK) { return (int)((n + b - 1) / b); }
 
@@ -102,7 +103,10 @@ struct BTask
 __device__ inline int centre_row(const BField& F, int c) { return F.np / (2 * F.nb) + c * (F.np / F.nb + 1); }
 
 // All sweeps of one row (at most cap): zc holds the row's roots, la/lb are
-// the ping-pong active lists. Returns after the first sweep that leaves no
+// the ping-pong active lists (LDS). The threads take the listed roots of a
+// sweep in strides, so the row's length costs no registers; a root that
+// needs the Cauchy sums takes a slot k in the workgroup's list (fl, f0, f1,
+// ex: global, L2-resident). Returns after the first sweep that leaves no
 // root active. newton (MIXED, continuation steps): a root whose FP64 Newton
 // step is below NEWTON_STEP_TOL takes it instead of an Aberth update, and
 // ends there when the step is below the convergence threshold, as in the
@@ -113,21 +117,29 @@ constexpr int BCNT = 6; // full, early, converged, near-field stars, sweeps, New
 // split the row (and the stars), so a few listed roots do not leave the
 // workgroup waiting on their owners' serial loops
 constexpr int COOP_MAX = 256;
-// static LDS of k_batch (the cooperative list, counters, reductions), kept
+// static LDS of k_batch (the cooperative sums, counters, reductions), kept
 // out of the dynamic limit
 constexpr int BATCH_STATIC_LDS = 8192;
 
-template <int BS, int RPT, bool MIXED>
+// a workgroup's list of the roots that take the Cauchy sums in a sweep
+struct ListSlice
+{
+	int* fl;      // row index of slot k
+	double2* f0;  // Witt function
+	double2* f1;  // its derivative
+	double2* ex;  // the Aberth terms formed per root, then (serial sums) the whole sum
+};
+
+template <int BS, bool MIXED>
 __device__ void solve_row(const BField& F, int row, int cap, bool newton, double2* zc, const double2* st, int* la, int* lb,
-	int* s_n, unsigned long long (&cnt)[BCNT])
+	const ListSlice& ls, int* s_n, unsigned long long (&cnt)[BCNT])
 {
 	const LensView& L = F.L;
 	const int R = F.R;
 	const double2 e = F.cs[row];
 	const bool pole_fmm = MIXED && L.order > 0;
 	__shared__ int s_nf;                  // roots listed for the sums this sweep
-	__shared__ int s_fl[COOP_MAX];        // their row indices (cooperative sweeps)
-	__shared__ double2 s_fs[COOP_MAX];    // their sums 2 sum 1/(z - s) - sum 1/(z - z_j), (re, -im)
+	__shared__ double2 s_fs[COOP_MAX];    // cooperative sums 2 sum 1/(z - s) - sum 1/(z - z_j), (re, -im)
 	constexpr int NW = BS / 64;
 	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 	for (int q = threadIdx.x; q < R; q += BS) la[q] = F.order[q];
@@ -145,19 +157,10 @@ __device__ void solve_row(const BField& F, int row, int cap, bool newton, double
 		const int n = s_n[0];
 		if (n == 0) break;
 		cnt[4] += threadIdx.x == 0 ? 1 : 0;
-		// Witt function and 1/mu early exit for the listed roots
-		int idx[RPT], slot[RPT];
-		bool full[RPT];
-		cd f0[RPT], f1[RPT], ex[RPT];
-#pragma unroll
-		for (int k = 0; k < RPT; k++)
+		// Witt function and 1/mu early exit of the listed roots
+		for (int q = threadIdx.x; q < n; q += BS)
 		{
-			const int q = threadIdx.x + k * BS;
-			idx[k] = -1;
-			full[k] = false;
-			if (q >= n) continue;
 			const int i = cur[q];
-			idx[k] = i;
 			const cd z(zc[i].x, zc[i].y);
 			const int node = nearest_node(L, z);
 			const Witt w = witt_from(L, z, e.x, e.y, node, p2p(L, node, z));
@@ -170,34 +173,33 @@ __device__ void solve_row(const BField& F, int row, int cap, bool newton, double
 					const cd dz = w.f0 / witt_f1(L, z, w);
 					if (dz.abs() < 1e-6 * L.theta) zc[i] = make_double2(z.re - dz.re, z.im - dz.im);
 				}
+				continue;
 			}
-			else
+			const cd f1 = witt_f1(L, z, w);
+			if (MIXED && newton)
 			{
-				f0[k] = w.f0;
-				f1[k] = witt_f1(L, z, w);
-				if (MIXED && newton)
+				const cd dz = w.f0 / f1;
+				const double adz = dz.abs();
+				if (adz < NEWTON_STEP_TOL * L.theta) // false for a non-finite step
 				{
-					const cd dz = f0[k] / f1[k];
-					const double adz = dz.abs();
-					if (adz < NEWTON_STEP_TOL * L.theta) // false for a non-finite step
-					{
-						zc[i] = make_double2(z.re - dz.re, z.im - dz.im);
-						if (adz < CONV_TOL * L.theta) cnt[2]++;
-						else nxt[atomicAdd(&s_n[1], 1)] = i; // stepped: checked again next sweep
-						cnt[3] += w.nr.count;
-						cnt[5]++;
-						continue;
-					}
+					zc[i] = make_double2(z.re - dz.re, z.im - dz.im);
+					if (adz < CONV_TOL * L.theta) cnt[2]++;
+					else nxt[atomicAdd(&s_n[1], 1)] = i; // stepped: checked again next sweep
+					cnt[3] += w.nr.count;
+					cnt[5]++;
+					continue;
 				}
-				full[k] = true;
-				ex[k] = cd();
-				if (F.circ_exact) ex[k] = 2.0 * z.inv(); // ccf_functions.cuh:223-227
-				if (pole_fmm) ex[k] += 2.0 * (w.nr.s0 + far_pole32(L, node, z));
-				slot[k] = atomicAdd(&s_nf, 1);
-				if (slot[k] < COOP_MAX) s_fl[slot[k]] = i;
-				cnt[0]++;
-				cnt[3] += w.nr.count;
 			}
+			cd ex;
+			if (F.circ_exact) ex = 2.0 * z.inv(); // ccf_functions.cuh:223-227
+			if (pole_fmm) ex += 2.0 * (w.nr.s0 + far_pole32(L, node, z));
+			const int k = atomicAdd(&s_nf, 1);
+			ls.fl[k] = i;
+			ls.f0[k] = make_double2(w.f0.re, w.f0.im);
+			ls.f1[k] = make_double2(f1.re, f1.im);
+			ls.ex[k] = make_double2(ex.re, ex.im);
+			cnt[0]++;
+			cnt[3] += w.nr.count;
 		}
 		__syncthreads();
 		const int nf = s_nf;
@@ -206,7 +208,7 @@ __device__ void solve_row(const BField& F, int row, int cap, bool newton, double
 			// cooperative sums, one wave per listed root, lanes over the sources
 			for (int g = wv; g < nf; g += NW)
 			{
-				const int ig = s_fl[g];
+				const int ig = ls.fl[g];
 				const double2 zg = zc[ig];
 				double vx, vy;
 				if (MIXED)
@@ -278,128 +280,92 @@ __device__ void solve_row(const BField& F, int row, int cap, bool newton, double
 				}
 				if (lane == 0) s_fs[g] = make_double2(vx, -vy);
 			}
-			__syncthreads();
 		}
-		// Cauchy sums 2 sum 1/(z - s_i) - sum_{j != k} 1/(z - z_j) and the update
-		double zx[RPT], zy[RPT], ax[RPT], ay[RPT], bx[RPT], by[RPT];
-		bool any = false;
-#pragma unroll
-		for (int k = 0; k < RPT; k++)
+		else
 		{
-			any |= full[k];
-			const double2 v = full[k] ? zc[idx[k]] : make_double2(0, 0);
-			zx[k] = v.x;
-			zy[k] = v.y;
-			ax[k] = ay[k] = bx[k] = by[k] = 0;
-		}
-		if (MIXED && any && nf > COOP_MAX)
-		{
-			// FP32 terms, a close root's term in FP64, as in the cooperative sums
-			float fx[RPT], fy[RPT], sax[RPT], say[RPT], sbx[RPT], sby[RPT];
-#pragma unroll
-			for (int k = 0; k < RPT; k++)
+			// each thread sums its listed roots over the stars and the row
+			for (int g = threadIdx.x; g < nf; g += BS)
 			{
-				fx[k] = (float)zx[k];
-				fy[k] = (float)zy[k];
-				sax[k] = say[k] = sbx[k] = sby[k] = 0;
-			}
-			if (!pole_fmm)
-				for (int s = 0; s < F.N; s++)
+				const int ig = ls.fl[g];
+				const double2 zg = zc[ig];
+				double ax = 0, ay = 0, bx = 0, by = 0;
+				if (MIXED)
 				{
-					const float2 w = F.stars32[s];
-#pragma unroll
-					for (int k = 0; k < RPT; k++)
+					// FP32 terms, a close root's term in FP64, as in the cooperative sums
+					const float fx = (float)zg.x, fy = (float)zg.y;
+					float sax = 0, say = 0, sbx = 0, sby = 0;
+					if (!pole_fmm)
+						for (int s = 0; s < F.N; s++)
+						{
+							const float2 w = F.stars32[s];
+							const float dx = fx - w.x, dy = fy - w.y;
+							const float iv = __builtin_amdgcn_rcpf(fmaf(dx, dx, dy * dy));
+							sax = fmaf(dx, iv, sax);
+							say = fmaf(dy, iv, say);
+						}
+					for (int j = 0; j < R; j++)
 					{
-						const float dx = fx[k] - w.x, dy = fy[k] - w.y;
-						const float iv = __builtin_amdgcn_rcpf(fmaf(dx, dx, dy * dy));
-						sax[k] = fmaf(dx, iv, sax[k]);
-						say[k] = fmaf(dy, iv, say[k]);
+						const double2 w = zc[j];
+						const float dx = fx - (float)w.x, dy = fy - (float)w.y;
+						const float d2 = fmaf(dx, dx, dy * dy);
+						if (d2 < F.cluster_d2 && j != ig)
+						{
+							const double ex = zg.x - w.x, ey = zg.y - w.y;
+							const double iv = rcp64(fmax(fma(ex, ex, ey * ey), 1e-300));
+							bx = fma(ex, iv, bx);
+							by = fma(ey, iv, by);
+						}
+						else
+						{
+							// the own root gives dx = dy = 0 and, floored, a zero term
+							const float iv = __builtin_amdgcn_rcpf(fmaxf(d2, 1e-30f));
+							sbx = fmaf(dx, iv, sbx);
+							sby = fmaf(dy, iv, sby);
+						}
 					}
+					ax = (double)sax;
+					ay = (double)say;
+					bx += (double)sbx;
+					by += (double)sby;
 				}
-			for (int j = 0; j < R; j++)
-			{
-				const double2 w = zc[j];
-				const float wx = (float)w.x, wy = (float)w.y;
-#pragma unroll
-				for (int k = 0; k < RPT; k++)
+				else
 				{
-					const float dx = fx[k] - wx, dy = fy[k] - wy;
-					const float d2 = fmaf(dx, dx, dy * dy);
-					if (d2 < F.cluster_d2 && j != idx[k])
-					{
-						const double ex = zx[k] - w.x, ey = zy[k] - w.y;
-						const double iv = rcp64(fmax(fma(ex, ex, ey * ey), 1e-300));
-						bx[k] = fma(ex, iv, bx[k]);
-						by[k] = fma(ey, iv, by[k]);
-					}
-					else
+					if (!pole_fmm)
+						for (int s = 0; s < F.N; s++)
+						{
+							const double2 w = st[s];
+							const double dx = zg.x - w.x, dy = zg.y - w.y;
+							const double iv = rcp64(fma(dx, dx, dy * dy));
+							ax = fma(dx, iv, ax);
+							ay = fma(dy, iv, ay);
+						}
+					for (int j = 0; j < R; j++)
 					{
 						// the own root gives dx = dy = 0 and, floored, a zero term
-						const float iv = __builtin_amdgcn_rcpf(fmaxf(d2, 1e-30f));
-						sbx[k] = fmaf(dx, iv, sbx[k]);
-						sby[k] = fmaf(dy, iv, sby[k]);
+						const double2 w = zc[j];
+						const double dx = zg.x - w.x, dy = zg.y - w.y;
+						const double iv = rcp64(fmax(fma(dx, dx, dy * dy), 1e-300));
+						bx = fma(dx, iv, bx);
+						by = fma(dy, iv, by);
 					}
 				}
+				const double2 t = ls.ex[g];
+				ls.ex[g] = make_double2(2 * ax - bx + t.x, -(2 * ay - by) + t.y);
 			}
-			// as the cooperative sums: 2 a - (b32 + b64)
-#pragma unroll
-			for (int k = 0; k < RPT; k++)
-			{
-				ax[k] = (double)sax[k];
-				ay[k] = (double)say[k];
-				bx[k] += (double)sbx[k];
-				by[k] += (double)sby[k];
-			}
-		}
-		else if (any && nf > COOP_MAX)
-		{
-			if (!pole_fmm)
-				for (int s = 0; s < F.N; s++)
-				{
-					const double2 w = st[s];
-#pragma unroll
-					for (int k = 0; k < RPT; k++)
-					{
-						const double dx = zx[k] - w.x, dy = zy[k] - w.y;
-						const double iv = rcp64(fma(dx, dx, dy * dy));
-						ax[k] = fma(dx, iv, ax[k]);
-						ay[k] = fma(dy, iv, ay[k]);
-					}
-				}
-			for (int j = 0; j < R; j++)
-			{
-				const double2 w = zc[j];
-#pragma unroll
-				for (int k = 0; k < RPT; k++)
-				{
-					// the own root gives dx = dy = 0 and, floored, a zero term
-					const double dx = zx[k] - w.x, dy = zy[k] - w.y;
-					const double iv = rcp64(fmax(fma(dx, dx, dy * dy), 1e-300));
-					bx[k] = fma(dx, iv, bx[k]);
-					by[k] = fma(dy, iv, by[k]);
-				}
-			}
-		}
-		cd zn[RPT];
-		bool moved[RPT];
-#pragma unroll
-		for (int k = 0; k < RPT; k++)
-		{
-			moved[k] = false;
-			if (!full[k]) continue;
-			const cd z(zx[k], zy[k]);
-			const cd sum = (nf <= COOP_MAX ? cd(s_fs[slot[k]].x, s_fs[slot[k]].y) : cd(2 * ax[k] - bx[k], -(2 * ay[k] - by[k]))) + ex[k];
-			zn[k] = z - f0[k] / (f1[k] + f0[k] * sum);
-			moved[k] = !((zn[k] - z).abs() / L.theta < CONV_TOL); // a non-finite step stays active
 		}
 		__syncthreads();
-		// commit, and list the roots that moved by 1e-9 theta or more
-#pragma unroll
-		for (int k = 0; k < RPT; k++)
+		// the Aberth update (every sum is formed, so the roots may move), and
+		// the roots that moved by 1e-9 theta or more listed for the next sweep
+		for (int g = threadIdx.x; g < nf; g += BS)
 		{
-			if (!full[k]) continue;
-			zc[idx[k]] = make_double2(zn[k].re, zn[k].im);
-			if (moved[k]) nxt[atomicAdd(&s_n[1], 1)] = idx[k];
+			const int i = ls.fl[g];
+			const cd z(zc[i].x, zc[i].y);
+			const double2 a0 = ls.f0[g], a1 = ls.f1[g], ae = ls.ex[g];
+			const cd f0(a0.x, a0.y), f1(a1.x, a1.y);
+			const cd sum = nf <= COOP_MAX ? cd(s_fs[g].x, s_fs[g].y) + cd(ae.x, ae.y) : cd(ae.x, ae.y);
+			const cd zn = z - f0 / (f1 + f0 * sum);
+			zc[i] = make_double2(zn.re, zn.im);
+			if (!((zn - z).abs() / L.theta < CONV_TOL)) nxt[atomicAdd(&s_n[1], 1)] = i; // a non-finite step stays active
 			else cnt[2]++;
 		}
 		__syncthreads();
@@ -418,10 +384,11 @@ __device__ void solve_row(const BField& F, int row, int cap, bool newton, double
 
 // The persistent solve of one phase (0: branch centres, 1: phi chains).
 // Dynamic LDS: rmax roots, nlds stars (fields with more stars read them from
-// global memory), two lists of rmax indices.
-template <int BS, int RPT, bool MIXED>
+// global memory), two lists of rmax indices. lists: rmax entries per
+// workgroup of the grid.
+template <int BS, bool MIXED>
 __global__ __launch_bounds__(BS) void k_batch(const BField* __restrict__ fields, const BTask* __restrict__ tasks, int ntasks,
-	int* __restrict__ next, int phase, int rmax, int nlds)
+	int* __restrict__ next, int phase, int rmax, int nlds, ListSlice lists)
 {
 	extern __shared__ double2 smem[];
 	double2* zc = smem;
@@ -431,6 +398,8 @@ __global__ __launch_bounds__(BS) void k_batch(const BField* __restrict__ fields,
 	__shared__ int s_task;
 	__shared__ int s_n[2];
 	__shared__ unsigned long long s_red[BCNT][BS / 64];
+	const size_t lo = (size_t)blockIdx.x * rmax;
+	const ListSlice ls{lists.fl + lo, lists.f0 + lo, lists.f1 + lo, lists.ex + lo};
 	for (;;)
 	{
 		__syncthreads();
@@ -452,7 +421,7 @@ __global__ __launch_bounds__(BS) void k_batch(const BField* __restrict__ fields,
 		{
 			const int row = centre_row(F, tk.branch);
 			for (int i = threadIdx.x; i < R; i += BS) zc[i] = F.seeds[i];
-			solve_row<BS, RPT, MIXED>(F, row, F.cap_init, false, zc, st, la, lb, s_n, cnt);
+			solve_row<BS, MIXED>(F, row, F.cap_init, false, zc, st, la, lb, ls, s_n, cnt);
 			for (int i = threadIdx.x; i < R; i += BS) F.roots[(size_t)row * R + i] = zc[i];
 		}
 		else
@@ -468,7 +437,7 @@ __global__ __launch_bounds__(BS) void k_batch(const BField* __restrict__ fields,
 				const int q = min(F.predictor, j - 1);
 				for (int i = threadIdx.x; i < R; i += BS)
 					zc[i] = predict_root(F.roots + (size_t)(row - sgn) * R + i, (long long)sgn * R, q);
-				solve_row<BS, RPT, MIXED>(F, row, F.cap_step, true, zc, st, la, lb, s_n, cnt);
+				solve_row<BS, MIXED>(F, row, F.cap_step, true, zc, st, la, lb, ls, s_n, cnt);
 				for (int i = threadIdx.x; i < R; i += BS) F.roots[(size_t)row * R + i] = zc[i];
 			}
 		}
@@ -610,35 +579,17 @@ __global__ __launch_bounds__(BLOCK) void k_batch_transpose(const BField* __restr
 		}
 }
 
-using BatchKernel = void (*)(const BField*, const BTask*, int, int*, int, int, int);
-
-template <int BS, bool MIXED>
-BatchKernel pick_rpt(int rpt)
-{
-	switch (rpt)
-	{
-	case 1: return k_batch<BS, 1, MIXED>;
-	case 2: return k_batch<BS, 2, MIXED>;
-	case 4: return k_batch<BS, 4, MIXED>;
-	default: return k_batch<BS, 8, MIXED>;
-	}
-}
+using BatchKernel = void (*)(const BField*, const BTask*, int, int*, int, int, int, ListSlice);
 
 template <bool MIXED>
-BatchKernel pick_kernel(int bs, int rpt)
+BatchKernel pick_kernel(int bs)
 {
 	switch (bs)
 	{
-	case 256: return pick_rpt<256, MIXED>(rpt);
-	case 512: return pick_rpt<512, MIXED>(rpt);
-	default: return pick_rpt<1024, MIXED>(rpt);
+	case 256: return k_batch<256, MIXED>;
+	case 512: return k_batch<512, MIXED>;
+	default: return k_batch<1024, MIXED>;
 	}
-}
-
-int rpt_for(int rmax, int bs)
-{
-	const int need = (rmax + bs - 1) / bs;
-	return need <= 1 ? 1 : need <= 2 ? 2 : need <= 4 ? 4 : 8;
 }
 
 } // namespace
@@ -675,6 +626,9 @@ __global__ __launch_bounds__(BLOCK) void k_batch_stars(const StarSlice* __restri
 struct BatchDevice
 {
 	StarSlice* slices = nullptr;
+	ListSlice lists{};       // the listed roots of a sweep, rmax entries per workgroup of the grid
+	char* list_mem = nullptr;
+	int list_wgs = 0;        // workgroups the lists hold
 	int* eoff = nullptr;     // [nf] first block of each field in the error pass
 	int eblocks = 0;
 	BField* fields = nullptr;
@@ -693,7 +647,7 @@ namespace
 void free_bdev(ccf_ctx* m, BatchDevice& d)
 {
 	auto F = [m](auto*& p) { m->dfree(p); p = nullptr; };
-	F(d.slices); F(d.eoff); F(d.fields); F(d.tasks0); F(d.tasks1); F(d.next); F(d.stats);
+	F(d.slices); F(d.eoff); F(d.list_mem); F(d.fields); F(d.tasks0); F(d.tasks1); F(d.next); F(d.stats);
 	d = BatchDevice{};
 }
 
@@ -707,12 +661,13 @@ int batch_grid(ccf_batch* b, int phase, int bs)
 	ccf_ctx* m = b->master;
 	BatchDevice& d = *b->dev;
 	const bool mixed = m->P.precision == 1;
-	const BatchKernel fn = mixed ? pick_kernel<true>(bs, rpt_for(d.rmax, bs)) : pick_kernel<false>(bs, rpt_for(d.rmax, bs));
+	const BatchKernel fn = mixed ? pick_kernel<true>(bs) : pick_kernel<false>(bs);
 	int per_cu = 0;
 	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)fn, bs, batch_lds(d.rmax, d.nlds)) != hipSuccess || per_cu < 1)
 		per_cu = 1;
 	const int cus = m->num_cus > 0 ? m->num_cus : 256;
-	return std::max(1, std::min(phase == 0 ? d.n0 : d.n1, cus * per_cu));
+	// the list buffers hold d.list_wgs workgroups
+	return std::max(1, std::min(std::min(phase == 0 ? d.n0 : d.n1, cus * per_cu), d.list_wgs));
 }
 
 bool launch_batch_phase(ccf_batch* b, int phase, int bs, int g, hipStream_t s)
@@ -720,13 +675,17 @@ bool launch_batch_phase(ccf_batch* b, int phase, int bs, int g, hipStream_t s)
 	ccf_ctx* m = b->master;
 	BatchDevice& d = *b->dev;
 	const bool mixed = m->P.precision == 1;
-	const int rpt = rpt_for(d.rmax, bs);
-	const BatchKernel fn = mixed ? pick_kernel<true>(bs, rpt) : pick_kernel<false>(bs, rpt);
+	const BatchKernel fn = mixed ? pick_kernel<true>(bs) : pick_kernel<false>(bs);
 	const size_t lds = batch_lds(d.rmax, d.nlds);
 	const int ntasks = phase == 0 ? d.n0 : d.n1;
+	if (g > d.list_wgs)
+	{
+		m->fail("Error. The batched grid exceeds its list buffers\n");
+		return false;
+	}
 	k_zero_int<<<1, 64, 0, s>>>(d.next, 1);
 	hipLaunchKernelGGL(fn, dim3(g), dim3(bs), lds, s, (const BField*)d.fields, (const BTask*)(phase == 0 ? d.tasks0 : d.tasks1),
-		ntasks, d.next, phase, d.rmax, d.nlds);
+		ntasks, d.next, phase, d.rmax, d.nlds, d.lists);
 	return CCF_LAUNCH(m, "batched Aberth solve");
 }
 
@@ -957,7 +916,7 @@ bool batch_build(ccf_batch* b, int verbose)
 	}
 	// LDS of a workgroup: the row, its two active lists, then as many stars as fit
 	const long long lim = batch_lds_limit(m);
-	if ((long long)batch_lds(d.rmax, 0) > lim || d.rmax > MAX_BATCH_RPT * 1024)
+	if ((long long)batch_lds(d.rmax, 0) > lim)
 	{
 		m->fail("Error. A batched field has more roots than fit one workgroup (" + std::to_string(d.rmax) + ")\n");
 		return false;
@@ -976,6 +935,17 @@ bool batch_build(ccf_batch* b, int verbose)
 	// terms from the FP32 star copy: no star is read from LDS, which then
 	// holds only the row (more workgroups per CU)
 	if (m->P.precision == 1) d.nlds = 0;
+	// list buffers for the largest persistent grid: rmax slots per workgroup
+	{
+		const int cus = m->num_cus > 0 ? m->num_cus : 256;
+		d.list_wgs = cus * MAX_BATCH_PER_CU;
+		const size_t slots = (size_t)d.list_wgs * d.rmax;
+		if (!CCF_HIP(m, m->dmalloc(&d.list_mem, slots * (sizeof(int) + 3 * sizeof(double2))))) return false;
+		d.lists.f0 = (double2*)d.list_mem;
+		d.lists.f1 = d.lists.f0 + slots;
+		d.lists.ex = d.lists.f1 + slots;
+		d.lists.fl = (int*)(d.lists.ex + slots);
+	}
 	lap("field tables");
 	// error-pass blocks of each field (fewer than 2^31 points per field and blocks in all)
 	std::vector<int> eoff(nf);
@@ -1025,7 +995,6 @@ bool batch_autotune(ccf_batch* b, int verbose)
 	b->tuned_ms.clear();
 	for (int bs : cands)
 	{
-		if (rpt_for(b->dev->rmax, bs) > MAX_BATCH_RPT) continue;
 		float ms[2] = {0, 0};
 		const int g0 = batch_grid(b, 0, bs), g1 = batch_grid(b, 1, bs);
 		if (!launch_batch_phase(b, 0, bs, g0, s) || !CCF_HIP(m, hipEventRecord(e0, s)) || !launch_batch_phase(b, 0, bs, g0, s) ||
@@ -1034,7 +1003,7 @@ bool batch_autotune(ccf_batch* b, int verbose)
 			!CCF_HIP(m, hipEventElapsedTime(&ms[1], e1, e2)))
 			return false;
 		if (verbose >= 2)
-			std::cout << "batch autotune: workgroup " << bs << " roots/thread " << rpt_for(b->dev->rmax, bs) << ": centres " << ms[0]
+			std::cout << "batch autotune: workgroup " << bs << ": centres " << ms[0]
 					  << " ms, chains " << ms[1] << " ms\n";
 		for (int q = 0; q < 2; q++)
 		{
@@ -1073,9 +1042,7 @@ bool batch_solve(ccf_batch* b, int verbose)
 	ccf_ctx* m = b->master;
 	hipStream_t s = m->stream;
 	BatchDevice& d = *b->dev;
-	// tuned once per batch; again when a rebuild's largest row no longer fits
-	if (b->block && (rpt_for(b->dev->rmax, b->block) > MAX_BATCH_RPT || rpt_for(b->dev->rmax, b->block1) > MAX_BATCH_RPT))
-		b->block = b->block1 = 0;
+	// tuned once per batch
 	if (b->block == 0 && !batch_autotune(b, verbose)) return false;
 	b->grid0 = batch_grid(b, 0, b->block);
 	b->grid1 = batch_grid(b, 1, b->block1);
