@@ -124,6 +124,8 @@ constexpr int BATCH_STATIC_LDS = 8192;
 // a workgroup's list of the roots that take the Cauchy sums in a sweep
 struct ListSlice
 {
+	int* la;      // the ping-pong active lists
+	int* lb;
 	int* fl;      // row index of slot k
 	double2* f0;  // Witt function
 	double2* f1;  // its derivative
@@ -383,23 +385,26 @@ __device__ void solve_row(const BField& F, int row, int cap, bool newton, double
 }
 
 // The persistent solve of one phase (0: branch centres, 1: phi chains).
-// Dynamic LDS: rmax roots, nlds stars (fields with more stars read them from
-// global memory), two lists of rmax indices. lists: rmax entries per
-// workgroup of the grid.
+// Dynamic LDS: rmax roots and nlds stars (fields with more stars read them
+// from global memory). lists: the active lists and the listed roots, rmax
+// entries each per workgroup of the grid (L2-resident; in LDS they would
+// leave room for fewer workgroups per CU). At most 128 VGPRs: four waves
+// per SIMD.
+constexpr int BATCH_WAVES_PER_SIMD = 4;
 template <int BS, bool MIXED>
-__global__ __launch_bounds__(BS) void k_batch(const BField* __restrict__ fields, const BTask* __restrict__ tasks, int ntasks,
-	int* __restrict__ next, int phase, int rmax, int nlds, ListSlice lists)
+__global__ __launch_bounds__(BS, BATCH_WAVES_PER_SIMD) void k_batch(const BField* __restrict__ fields, const BTask* __restrict__ tasks,
+	int ntasks, int* __restrict__ next, int phase, int rmax, int nlds, ListSlice lists)
 {
 	extern __shared__ double2 smem[];
 	double2* zc = smem;
 	double2* sl = smem + rmax;
-	int* la = (int*)(sl + nlds);
-	int* lb = la + rmax;
 	__shared__ int s_task;
 	__shared__ int s_n[2];
 	__shared__ unsigned long long s_red[BCNT][BS / 64];
 	const size_t lo = (size_t)blockIdx.x * rmax;
-	const ListSlice ls{lists.fl + lo, lists.f0 + lo, lists.f1 + lo, lists.ex + lo};
+	const ListSlice ls{lists.la + lo, lists.lb + lo, lists.fl + lo, lists.f0 + lo, lists.f1 + lo, lists.ex + lo};
+	int* la = ls.la;
+	int* lb = ls.lb;
 	for (;;)
 	{
 		__syncthreads();
@@ -652,7 +657,7 @@ void free_bdev(ccf_ctx* m, BatchDevice& d)
 }
 
 // LDS bytes of a workgroup: roots, staged stars, two index lists
-size_t batch_lds(int rmax, int nlds) { return (size_t)(rmax + nlds) * sizeof(double2) + 2 * (size_t)rmax * sizeof(int); }
+size_t batch_lds(int rmax, int nlds) { return (size_t)(rmax + nlds) * sizeof(double2); }
 
 // persistent grid: as many workgroups as fit the device at once, at most
 // one per task (queried outside any graph capture)
@@ -914,7 +919,7 @@ bool batch_build(ccf_batch* b, int verbose)
 			t1.push_back(BTask{f, br, 1, cost});
 		}
 	}
-	// LDS of a workgroup: the row, its two active lists, then as many stars as fit
+	// LDS of a workgroup: the row, then as many stars as fit
 	const long long lim = batch_lds_limit(m);
 	if ((long long)batch_lds(d.rmax, 0) > lim)
 	{
@@ -940,11 +945,13 @@ bool batch_build(ccf_batch* b, int verbose)
 		const int cus = m->num_cus > 0 ? m->num_cus : 256;
 		d.list_wgs = cus * MAX_BATCH_PER_CU;
 		const size_t slots = (size_t)d.list_wgs * d.rmax;
-		if (!CCF_HIP(m, m->dmalloc(&d.list_mem, slots * (sizeof(int) + 3 * sizeof(double2))))) return false;
+		if (!CCF_HIP(m, m->dmalloc(&d.list_mem, slots * (3 * sizeof(int) + 3 * sizeof(double2))))) return false;
 		d.lists.f0 = (double2*)d.list_mem;
 		d.lists.f1 = d.lists.f0 + slots;
 		d.lists.ex = d.lists.f1 + slots;
 		d.lists.fl = (int*)(d.lists.ex + slots);
+		d.lists.la = d.lists.fl + slots;
+		d.lists.lb = d.lists.la + slots;
 	}
 	lap("field tables");
 	// error-pass blocks of each field (fewer than 2^31 points per field and blocks in all)
