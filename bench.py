@@ -358,7 +358,7 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="cc10k", choices=sorted(WORKLOADS) + ["batch256"])
     ap.add_argument("--seq-fields", type=int, default=64,
                     help="batch256: fields of the sample timed one by one on the single-field FP64 path")
