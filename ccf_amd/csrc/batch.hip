@@ -12,20 +12,19 @@
 //      phase 1, one task per (field, branch, side): the chain of phi steps of
 //               that side toward 0 or 2 pi (ccf.cuh:979-1083), each step
 //               predicted from the previous ones as in the single-field path.
-//    One workgroup runs a task. The R roots of its row live in LDS with the
-//    field's stars; a Jacobi sweep is three barrier-separated phases over a
-//    compacted list of the active roots: the Witt function and the 1/mu
-//    early exit (ccf_functions.cuh:197-213), the Cauchy sums over stars and
-//    roots and the Aberth update kept in registers (ccf_functions.cuh:
-//    214-242), then the commit and the |dz| / theta < 1e-9 test
-//    (ccf_functions.cuh:379-389). Workgroups take tasks from the queue,
-//    largest rows first, until it is empty.
+//    One workgroup runs a task. The R roots of its row live in LDS (with the
+//    field's stars in the FP64 form, when they fit); a Jacobi sweep is three
+//    barrier-separated phases over a compacted list of the active roots:
+//    the Witt function and the 1/mu early exit (ccf_functions.cuh:197-213),
+//    the Cauchy sums over stars and roots (ccf_functions.cuh:214-242), then
+//    the Aberth update and the |dz| / theta < 1e-9 test (ccf_functions.cuh:
+//    379-389). Workgroups take tasks from the queue, largest rows first,
+//    until it is empty.
 //  * the 1/mu error pass (which also forms the caustic map from the same
-//    field evaluation) and the transpose run once over all fields (one grid
-//    dimension per field);
+//    field evaluation) and the transpose run once over all fields;
 //  * the launches of a solve are captured in a HIP graph and replayed.
-// An autotuner picks the workgroup size (roots per thread follow from the
-// largest row) by timing the phase-0 launch of the batch itself.
+// An autotuner picks the workgroup size of each phase by timing the phase's
+// launch on the batch itself.
 #include "lens.hpp"
 
 #include <algorithm>
@@ -102,11 +101,11 @@ struct BTask
 
 __device__ inline int centre_row(const BField& F, int c) { return F.np / (2 * F.nb) + c * (F.np / F.nb + 1); }
 
-// All sweeps of one row (at most cap): zc holds the row's roots, la/lb are
-// the ping-pong active lists (LDS). The threads take the listed roots of a
-// sweep in strides, so the row's length costs no registers; a root that
-// needs the Cauchy sums takes a slot k in the workgroup's list (fl, f0, f1,
-// ex: global, L2-resident). Returns after the first sweep that leaves no
+// All sweeps of one row (at most cap): zc holds the row's roots (LDS),
+// la/lb are the ping-pong active lists (the workgroup's global slice). The
+// threads take the listed roots of a sweep in strides, so the row's length
+// costs no registers; a root that needs the Cauchy sums takes a slot k in
+// the workgroup's list (fl, f0, f1, ex: global, L2-resident). Returns after the first sweep that leaves no
 // root active. newton (MIXED, continuation steps): a root whose FP64 Newton
 // step is below NEWTON_STEP_TOL takes it instead of an Aberth update, and
 // ends there when the step is below the convergence threshold, as in the
