@@ -124,8 +124,9 @@ void ccf_ctx::dfree(void* p)
 void ccf_ctx::pool_release()
 {
 	if (pool.empty()) return;
-	// work queued on the stream may still use a pooled buffer
+	// work queued on the streams may still use a pooled buffer
 	if (stream) (void)hipStreamSynchronize(stream);
+	if (err_stream) (void)hipStreamSynchronize(err_stream);
 	for (auto& [n, p] : pool)
 	{
 		owned.erase(p);
@@ -136,6 +137,8 @@ void ccf_ctx::pool_release()
 
 void ccf_ctx::free_device()
 {
+	// a failed solve can leave error-pass work queued on the second stream
+	if (err_stream) (void)hipStreamSynchronize(err_stream);
 	auto F = [this](auto*& p) { dfree(p); p = nullptr; };
 	ccf::DeviceTree& T = tree;
 	F(T.center); F(T.hl); F(T.parent); F(T.child); F(T.sstart); F(T.scount);
@@ -661,6 +664,8 @@ void ccf_destroy(ccf_ctx* c)
 	for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
 	for (hipEvent_t e : c->fmm_rows_event)
 		if (e) (void)hipEventDestroy(e);
+	if (c->err_event) (void)hipEventDestroy(c->err_event);
+	if (c->err_stream) (void)hipStreamDestroy(c->err_stream);
 	if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
 	delete c;
 }

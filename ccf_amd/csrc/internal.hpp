@@ -122,6 +122,7 @@ struct RootGrid
 	int order = 0;       // expansion order (0: no FMM)
 	int ncells = 0;      // cells of all levels
 	int nleaf = 0;       // leaf cells
+	int bin_row = 0;     // a row's leaf counts fit in one workgroup's LDS (one-launch binning)
 	double x0 = 0, y0 = 0, s0 = 0;
 	int off[MAX_ROOT_GRID_LEVELS + 1] = {};
 };
@@ -228,6 +229,8 @@ struct ccf_ctx
 		return events[events_used++];
 	}
 	int* host_counts = nullptr;   // pinned: active roots per row, the two FMM row counts of a read-back, then two slots of two for the mixed continuation
+	hipStream_t err_stream = nullptr; // the error pass of find_ccs, beside the continuation (default priority)
+	hipEvent_t err_event = nullptr;   // a row set's roots are final (no timing)
 	hipEvent_t fmm_rows_event[2] = {nullptr, nullptr}; // mixed continuation: after the copy of a step's FMM row count (no timing)
 	int fmm_rows_sweep[2] = {-1, -1}; // root-sum FMM rows of the last phase's first and second sweep (-1: unknown)
 	int phase_sweeps = 0;         // sweeps the last phase ran (its cap when no read-back saw it end)

@@ -196,6 +196,138 @@ __global__ __launch_bounds__(BLOCK) void k_rscatter(RootView V, int R)
 	if (c >= 0) V.sidx[(size_t)r * R + pos] = j;
 }
 
+// exclusive prefix of v over the workgroup (blockDim.x a multiple of 64, at
+// most 64 waves); tot: the sum of all. wsum: one int per wave, reusable on
+// return.
+__device__ inline int block_scan_excl(int v, int* wsum, int& tot)
+{
+	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+	int x = v;
+	for (int d = 1; d < 64; d <<= 1)
+	{
+		const int y = __shfl_up(x, d, 64);
+		if (lane >= d) x += y;
+	}
+	if (lane == 63) wsum[w] = x;
+	__syncthreads();
+	int woff = 0;
+	tot = 0;
+	for (int k = 0; k < nw; k++)
+	{
+		if (k < w) woff += wsum[k];
+		tot += wsum[k];
+	}
+	__syncthreads();
+	return woff + x - v;
+}
+
+// The binning of a row in one workgroup when its leaf counts fit in LDS
+// (k_rzero, k_rbin, k_rscan and k_rscatter in one launch). The counts and
+// then the cells' cursors are LDS atomics; the outliers (cell -1) of every
+// 64 consecutive roots are a ballot mask in LDS, whose popcounts' prefix
+// places them in index order. Within a cell the order is the atomics'
+// (k_rleaf sorts it). LDS: nleaf ints, then per 64 roots a mask and an int.
+constexpr int BIN_THREADS = 1024;
+constexpr int BIN_STATIC_LDS = 256; // k_rbin_row's own LDS beside the dynamic part
+
+inline size_t bin_row_lds(int nleaf, int R)
+{
+	const size_t ng = ((size_t)R + 63) / 64;
+	return ((size_t)nleaf * sizeof(int) + 7) / 8 * 8 + ng * (sizeof(unsigned long long) + sizeof(int));
+}
+
+__global__ __launch_bounds__(BIN_THREADS) void k_rbin_row(RootView V, const double2* __restrict__ roots, RowMap M,
+	unsigned long long* __restrict__ stats)
+{
+	extern __shared__ unsigned long long bin_lds[];
+	__shared__ int wsum[BIN_THREADS / 64];
+	const int r = blockIdx.x;
+	if (!V.rmode[r]) return;
+	if (threadIdx.x == 0) atomicAdd(stat_slot(stats, C_RROWS), 1ull);
+	const int R = M.R, nl = V.g.nleaf, ng = (R + 63) / 64;
+	int* cnt = (int*)bin_lds;
+	unsigned long long* omask = bin_lds + (nl + 1) / 2;
+	int* obase = (int*)(omask + ng);
+	const int lane = threadIdx.x & 63;
+	for (int c = threadIdx.x; c < nl; c += BIN_THREADS) cnt[c] = 0;
+	__syncthreads();
+	int row;
+	double phi;
+	M.get(r, row, phi);
+	const double2* __restrict__ zr = roots + (size_t)row * R;
+	int* __restrict__ cell = V.cell + (size_t)r * R;
+	const double sl = ldexp(V.g.s0, -V.g.leaf);
+	const int nxl = V.g.nx << V.g.leaf, nyl = V.g.ny << V.g.leaf;
+	// every wave covers whole groups of 64: j - lane is a multiple of 64
+	const int nj = (R + BIN_THREADS - 1) / BIN_THREADS * BIN_THREADS;
+#pragma unroll 4
+	for (int j = threadIdx.x; j < nj; j += BIN_THREADS)
+	{
+		int c = -1;
+		if (j < R)
+		{
+			const double2 z = zr[j];
+			const double fx = (z.x - V.g.x0) / sl, fy = (z.y - V.g.y0) / sl;
+			if (fx >= 0 && fx < nxl && fy >= 0 && fy < nyl) c = (int)fy * nxl + (int)fx;
+			cell[j] = c;
+			if (c >= 0) atomicAdd(&cnt[c], 1);
+		}
+		const unsigned long long m = __ballot(j < R && c < 0);
+		if (lane == 0 && j < R) omask[j >> 6] = m;
+	}
+	__syncthreads();
+	// the outliers in index order: prefix of the groups' popcounts
+	int run = 0;
+	for (int base = 0; base < ng; base += BIN_THREADS)
+	{
+		const int g = base + threadIdx.x;
+		const int v = g < ng ? (int)__popcll(omask[g]) : 0;
+		int t;
+		const int e = run + block_scan_excl(v, wsum, t);
+		if (g < ng) obase[g] = e;
+		run += t;
+	}
+	if (threadIdx.x == 0) V.ocnt[r] = run;
+	__syncthreads();
+	if (run > 0)
+	{
+		int* oidx = V.oidx + (size_t)r * R;
+		const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+		for (int j = threadIdx.x; j < R; j += BIN_THREADS)
+		{
+			const unsigned long long m = omask[j >> 6];
+			if ((m >> lane) & 1ull) oidx[obase[j >> 6] + (int)__popcll(m & below)] = j;
+		}
+	}
+	// exclusive prefix of the leaf counts: cstart, and the cells' cursors
+	int* st = V.cstart + (size_t)r * (nl + 1);
+	run = 0;
+	for (int base = 0; base < nl; base += BIN_THREADS)
+	{
+		const int c = base + threadIdx.x;
+		const int v = c < nl ? cnt[c] : 0;
+		int t;
+		const int e = run + block_scan_excl(v, wsum, t);
+		if (c < nl)
+		{
+			st[c] = e;
+			cnt[c] = e;
+		}
+		run += t;
+	}
+	if (threadIdx.x == 0) st[nl] = run;
+	__syncthreads();
+	int* __restrict__ sidx = V.sidx + (size_t)r * R;
+#pragma unroll 4
+	for (int j = threadIdx.x; j < R; j += BIN_THREADS)
+	{
+		const int c = cell[j];
+		if (c >= 0) sidx[atomicAdd(&cnt[c], 1)] = j;
+	}
+}
+
+constexpr int LEAF_REG = 16; // cells up to this many roots sort in registers
+
 // per leaf cell: the cell's roots in index order, their positions copied
 // contiguously, and the multipole expansion (P2M)
 template <int P>
@@ -216,6 +348,56 @@ __global__ __launch_bounds__(BLOCK) void k_rleaf(RootView V, const double2* __re
 	V.ccnt[cid] = n;
 	if (n == 0) return;
 	int* ix = V.sidx + (size_t)r * R;
+	const int nxl = V.g.nx << L;
+	const cd cc = cell_center(V.g, L, c % nxl, c / nxl);
+	const double ih = 2.0 / ldexp(V.g.s0, -L);
+	cd A[P + 1];
+#pragma unroll
+	for (int k = 0; k <= P; k++) A[k] = cd(0, 0);
+	if (n <= LEAF_REG)
+	{
+		// a small cell sorts in registers: each index's rank among the
+		// cell's (the indices are distinct), then every load of the cell's
+		// roots at once instead of one dependent chain per root
+		int v[LEAF_REG], srt[LEAF_REG];
+#pragma unroll
+		for (int i = 0; i < LEAF_REG; i++) v[i] = i < n ? ix[b + i] : INT_MAX;
+#pragma unroll
+		for (int i = 0; i < LEAF_REG; i++) srt[i] = INT_MAX;
+#pragma unroll
+		for (int i = 0; i < LEAF_REG; i++)
+		{
+			int rk = 0;
+#pragma unroll
+			for (int j = 0; j < LEAF_REG; j++) rk += v[j] < v[i];
+#pragma unroll
+			for (int q = 0; q < LEAF_REG; q++)
+				if (rk == q && i < n) srt[q] = v[i];
+		}
+		double2 zz[LEAF_REG];
+#pragma unroll
+		for (int q = 0; q < LEAF_REG; q++)
+			if (q < n) zz[q] = roots[(size_t)row * R + srt[q]];
+#pragma unroll
+		for (int q = 0; q < LEAF_REG; q++)
+			if (q < n)
+			{
+				ix[b + q] = srt[q];
+				V.spos[(size_t)r * R + b + q] = zz[q];
+				const cd w = (cd(zz[q].x, zz[q].y) - cc) * ih;
+				cd t(1, 0);
+#pragma unroll
+				for (int k = 0; k <= P; k++)
+				{
+					A[k] += t;
+					t = t * w;
+				}
+			}
+		double2* out = V.mp + cid * (P + 1);
+#pragma unroll
+		for (int k = 0; k <= P; k++) out[k] = make_double2(A[k].re, A[k].im);
+		return;
+	}
 	if (n <= 128)
 	{
 		for (int q = b + 1; q < e; q++)
@@ -230,12 +412,6 @@ __global__ __launch_bounds__(BLOCK) void k_rleaf(RootView V, const double2* __re
 			ix[t + 1] = v;
 		}
 	}
-	const int nxl = V.g.nx << L;
-	const cd cc = cell_center(V.g, L, c % nxl, c / nxl);
-	const double ih = 2.0 / ldexp(V.g.s0, -L);
-	cd A[P + 1];
-#pragma unroll
-	for (int k = 0; k <= P; k++) A[k] = cd(0, 0);
 	for (int q = b; q < e; q++)
 	{
 		const double2 z = roots[(size_t)row * R + ix[q]];
@@ -340,6 +516,36 @@ constexpr FarList make_far_list()
 __constant__ const FarList kFarList = make_far_list();
 static_assert(make_far_list().v[3][26] != 0, "27 far cells per child position");
 
+// M2L: adds to Lc the local expansion about a target cell of the multipole
+// am of a source cell at offset (ix - jx, iy - jy) = (dx, dy) cells, on a
+// level of cell side 2 / ih
+template <int P>
+__device__ __forceinline__ void m2l_add(const double2* am, int dx, int dy, double ih, cd (&Lc)[P + 1])
+{
+	const cd rho = cd(2.0 * dx, 2.0 * dy).inv();
+	cd bq[P + 1];
+	cd t(1, 0);
+#pragma unroll
+	for (int n = 0; n <= P; n++)
+	{
+		const double2 a = am[n];
+		bq[n] = cd(a.x, a.y) * t;
+		t = t * rho;
+	}
+	// sum_n C(n+m, m) b_n is element 0 after the (m+1)-th suffix-sum pass
+	cd rp = rho * ih;
+#pragma unroll
+	for (int m = 0; m <= P; m++)
+	{
+#pragma unroll
+		for (int k = P - 1; k >= 0; k--) bq[k] += bq[k + 1];
+		const cd v = rp * bq[0];
+		if (m & 1) Lc[m] -= v;
+		else Lc[m] += v;
+		rp = rp * rho;
+	}
+}
+
 template <int P>
 __global__ __launch_bounds__(32 * down_targets<P>()) void k_rdown32(RootView V, int l, unsigned long long* __restrict__ stats)
 {
@@ -368,27 +574,7 @@ __global__ __launch_bounds__(32 * down_targets<P>()) void k_rdown32(RootView V, 
 		double2 am[P + 1];
 #pragma unroll
 		for (int n = 0; n <= P; n++) am[n] = src[n];
-		const cd rho = cd(2.0 * (ix - jx), 2.0 * (iy - jy)).inv();
-		cd bq[P + 1];
-		cd t(1, 0);
-#pragma unroll
-		for (int n = 0; n <= P; n++)
-		{
-			bq[n] = cd(am[n].x, am[n].y) * t;
-			t = t * rho;
-		}
-		// sum_n C(n+m, m) b_n is element 0 after the (m+1)-th suffix-sum pass
-		cd rp = rho * ih;
-#pragma unroll
-		for (int m = 0; m <= P; m++)
-		{
-#pragma unroll
-			for (int k = P - 1; k >= 0; k--) bq[k] += bq[k + 1];
-			const cd v = rp * bq[0];
-			if (m & 1) Lc[m] -= v;
-			else Lc[m] += v;
-			rp = rp * rho;
-		}
+		m2l_add<P>(am, ix - jx, iy - jy, ih, Lc);
 	};
 	if (live)
 	{
@@ -456,6 +642,118 @@ __global__ __launch_bounds__(32 * down_targets<P>()) void k_rdown32(RootView V, 
 		unsigned long long s = 0;
 		for (int w = 0; w < TPB * 32 / 64; w++) s += nsum[w];
 		if (s) atomicAdd(stat_slot(stats, C_RM2L), s);
+	}
+}
+
+// M2L and L2L of a level whose rows and columns of cells are multiples of
+// DOWN_TILE (levels 3 and finer): a workgroup takes a DOWN_TILE^2 tile of
+// target cells and stages the multipoles of the tile's whole interaction
+// region, (DOWN_TILE + 4)^2 cells, in LDS once, where the 32-lane kernel has
+// every target read its 27 sources through the cache. Four threads per
+// target take every fourth entry of its list; lane shuffles add their
+// partials, and each adds every fourth coefficient of the parent's shifted
+// expansion.
+constexpr int DOWN_TILE = 8;
+constexpr int DOWN_SRC = DOWN_TILE + 4;
+constexpr int DOWN_TILE_THREADS = DOWN_TILE * DOWN_TILE * 4;
+template <int P>
+constexpr bool down_tiled() { return (P + 1) <= 13; } // higher orders: the 32-lane kernel
+
+template <int P>
+__global__ __launch_bounds__(DOWN_TILE_THREADS) void k_rdown_tile(RootView V, int l, unsigned long long* __restrict__ stats)
+{
+	__shared__ double2 src[DOWN_SRC * DOWN_SRC][P + 1];
+	__shared__ int sok[DOWN_SRC * DOWN_SRC];
+	__shared__ unsigned long long nsum[DOWN_TILE_THREADS / 64];
+	const int r = blockIdx.y;
+	if (!V.rmode[r]) return;
+	const int nxl = V.g.nx << l, nyl = V.g.ny << l;
+	const int tiles_x = nxl / DOWN_TILE;
+	const int x0 = (blockIdx.x % tiles_x) * DOWN_TILE, y0 = (blockIdx.x / tiles_x) * DOWN_TILE;
+	// the region's first cell: the far lists reach two cells beyond the tile
+	const int sx0 = x0 - 2, sy0 = y0 - 2;
+	const size_t rowb = (size_t)r * V.g.ncells, lb = rowb + V.g.off[l];
+	for (int i = threadIdx.x; i < DOWN_SRC * DOWN_SRC; i += DOWN_TILE_THREADS)
+	{
+		const int jx = sx0 + i % DOWN_SRC, jy = sy0 + i / DOWN_SRC;
+		sok[i] = jx >= 0 && jy >= 0 && jx < nxl && jy < nyl && V.ccnt[lb + (size_t)jy * nxl + jx] > 0;
+	}
+	__syncthreads();
+	for (int i = threadIdx.x; i < DOWN_SRC * DOWN_SRC * (P + 1); i += DOWN_TILE_THREADS)
+	{
+		const int c = i / (P + 1), n = i - c * (P + 1);
+		if (sok[c])
+		{
+			const int jx = sx0 + c % DOWN_SRC, jy = sy0 + c / DOWN_SRC;
+			src[c][n] = V.mp[(lb + (size_t)jy * nxl + jx) * (P + 1) + n];
+		}
+	}
+	__syncthreads();
+	const int t = threadIdx.x >> 2, q = threadIdx.x & 3;
+	const int ix = x0 + t % DOWN_TILE, iy = y0 + t / DOWN_TILE;
+	const size_t cid = lb + (size_t)iy * nxl + ix;
+	const bool live = ix < nxl && iy < nyl && V.ccnt[cid] > 0;
+	const double ih = 2.0 / ldexp(V.g.s0, -l);
+	cd Lc[P + 1];
+#pragma unroll
+	for (int k = 0; k <= P; k++) Lc[k] = cd(0, 0);
+	unsigned long long nt = 0;
+	if (live)
+	{
+		// entry v of the far list is cell (v % 6, v / 6) of the 6 x 6 block
+		// whose first cell is (2 (ix / 2) - 2, 2 (iy / 2) - 2): local
+		// coordinates bx + v % 6, by + v / 6 in [0, DOWN_SRC)
+		const int o = (ix & 1) + 2 * (iy & 1);
+		const int bx = 2 * (ix >> 1) - x0, by = 2 * (iy >> 1) - y0;
+		for (int k = q; k < 27; k += 4)
+		{
+			const int v = kFarList.v[o][k];
+			const int lx = bx + v % 6, ly = by + v / 6;
+			const int c = ly * DOWN_SRC + lx;
+			if (!sok[c]) continue;
+			nt++;
+			m2l_add<P>(src[c], (ix - sx0) - lx, (iy - sy0) - ly, ih, Lc);
+		}
+	}
+	// the four threads of a target: (q0 + q1) + (q2 + q3) in every one of them
+#pragma unroll
+	for (int k = 0; k <= P; k++)
+		for (int d = 1; d <= 2; d <<= 1)
+		{
+			Lc[k].re += __shfl_xor(Lc[k].re, d, 64);
+			Lc[k].im += __shfl_xor(Lc[k].im, d, 64);
+		}
+	{
+		const unsigned long long sn = wave_sum(nt);
+		if ((threadIdx.x & 63) == 0) nsum[threadIdx.x >> 6] = sn;
+	}
+	if (live)
+	{
+		// L2L: 2^-m sum_{n >= m} C(n, m) L_n(parent) e^(n - m), the parent's
+		// polynomial shifted by e (repeated synthetic division)
+		const double2* lp = V.lc + (rowb + V.g.off[l - 1] + (size_t)(iy >> 1) * (nxl >> 1) + (ix >> 1)) * (P + 1);
+		const cd e((ix & 1) - 0.5, (iy & 1) - 0.5);
+		cd c[P + 1];
+#pragma unroll
+		for (int n = 0; n <= P; n++) c[n] = cd(lp[n].x, lp[n].y);
+#pragma unroll
+		for (int i = 0; i < P; i++)
+#pragma unroll
+			for (int j = P - 1; j >= i; j--) c[j] += e * c[j + 1];
+#pragma unroll
+		for (int m = 0; m <= P; m++)
+			if ((m & 3) == q)
+			{
+				const double sc = ldexp(1.0, -m);
+				V.lc[cid * (P + 1) + m] = make_double2(Lc[m].re + c[m].re * sc, Lc[m].im + c[m].im * sc);
+			}
+	}
+	__syncthreads();
+	if (threadIdx.x == 0)
+	{
+		unsigned long long sn = 0;
+		for (int w = 0; w < DOWN_TILE_THREADS / 64; w++) sn += nsum[w];
+		if (sn) atomicAdd(stat_slot(stats, C_RM2L), sn);
 	}
 }
 
@@ -610,16 +908,32 @@ void root_fmm_pass_p(const SweepBufs& B, const double2* roots, const RowMap& M, 
 	const RootView V = root_view(B);
 	const RootGrid& g = B.grid;
 	const int R = M.R;
-	k_rzero<<<dim3(grid(g.nleaf), nrows), BLOCK, 0, s>>>(V);
-	k_rbin<<<dim3(grid(R), nrows), BLOCK, 0, s>>>(V, roots, M);
-	k_rscan<<<nrows, 1024, 0, s>>>(V, R, stats);
-	k_rscatter<<<dim3(grid(R), nrows), BLOCK, 0, s>>>(V, R);
+	if (g.bin_row)
+		k_rbin_row<<<nrows, BIN_THREADS, bin_row_lds(g.nleaf, R), s>>>(V, roots, M, stats);
+	else
+	{
+		k_rzero<<<dim3(grid(g.nleaf), nrows), BLOCK, 0, s>>>(V);
+		k_rbin<<<dim3(grid(R), nrows), BLOCK, 0, s>>>(V, roots, M);
+		k_rscan<<<nrows, 1024, 0, s>>>(V, R, stats);
+		k_rscatter<<<dim3(grid(R), nrows), BLOCK, 0, s>>>(V, R);
+	}
 	k_rleaf<P><<<dim3(grid(g.nleaf), nrows), BLOCK, 0, s>>>(V, roots, M);
 	for (int l = g.leaf - 1; l >= 0; l--)
 		k_rm2m<P><<<dim3(grid((g.nx << l) * (g.ny << l)), nrows), BLOCK, 0, s>>>(V, l);
 	constexpr int TPB = down_targets<P>();
 	for (int l = 0; l <= g.leaf; l++)
+	{
+		// from 1024 cells (16 tiles) per row, with nx 2^l and ny 2^l multiples
+		// of DOWN_TILE (l >= 3); on smaller levels a few workgroups would
+		// each carry a whole tile's lists
+		if constexpr (down_tiled<P>())
+			if (l >= 3 && (g.nx << l) * (g.ny << l) >= 1024)
+			{
+				k_rdown_tile<P><<<dim3((g.nx << l) / DOWN_TILE * ((g.ny << l) / DOWN_TILE), nrows), DOWN_TILE_THREADS, 0, s>>>(V, l, stats);
+				continue;
+			}
 		k_rdown32<P><<<dim3(grid((g.nx << l) * (g.ny << l), TPB), nrows), 32 * TPB, 0, s>>>(V, l, stats);
+	}
 	k_rsum<P><<<dim3(grid(R), nrows), BLOCK, 0, s>>>(V, roots, M, stats);
 	k_rsum_out<P><<<dim3(nrows, OUT_BLOCKS), BLOCK, 0, s>>>(V, roots, M, stats);
 }

@@ -34,6 +34,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <iostream>
 #include <vector>
 
@@ -116,8 +117,9 @@ __device__ inline unsigned long long* stat_slot(unsigned long long* stats, int c
 }
 
 // points of the error pass whose node is not a leaf (outside the tree),
-// listed for a cooperative pass: count (zeroed before the pass), then up to
-// cap global point indices
+// listed for a cooperative pass: count (zeroed before each pass) and the
+// running total over the passes of a run (count[1]), then up to cap global
+// point indices
 struct FarPoints
 {
 	int* count;
@@ -125,9 +127,13 @@ struct FarPoints
 	int cap;
 };
 constexpr int FAR_POINTS_CAP = 1 << 20;
-constexpr int FAR_POINTS_BLOCKS = 1024;
+constexpr int FAR_POINTS_BLOCKS = 256; // workgroups of one row set's pass over its list
 
-__global__ void k_zero_count(int* p) { *p = 0; }
+__global__ void k_zero_count(int* p, int both)
+{
+	p[0] = 0;
+	if (both) p[1] = 0;
+}
 
 // adds v[k] of every thread of the block to counter which[k]; every thread
 // of a BS-sized workgroup calls this
@@ -1270,14 +1276,16 @@ __global__ void k_prepare(double2* roots, RowMap M, SweepBufs B, int order, cons
 // same evaluation also gives the caustic point and the length scale
 // (find_caustics_kernel, find_mu_length_scales_kernel), written in file
 // order: root a, phi row r at a * rows + r.
-__global__ void k_errors(LensView L, const double2* __restrict__ roots, int R, const double2* __restrict__ cs,
+__global__ void k_errors(LensView L, const double2* __restrict__ roots, int R, RowMap M, int rows,
 	const int* __restrict__ perm, unsigned long long* stats, double2* __restrict__ caus, double* __restrict__ mu,
 	FarPoints far)
 {
-	// grid: (xcd_round(blocks per row), rows); a row's points in the roots'
-	// processing order
+	// grid: (xcd_round(blocks per row), rows of M); a row's points in the
+	// roots' processing order; rows: the phi rows of the output
 	const int2 sl = xcd_slot();
-	const int row = sl.y;
+	int row;
+	double phi;
+	M.get(sl.y, row, phi);
 	const int k = xcd_x(sl.x, (R + BLOCK - 1) / BLOCK) * BLOCK + threadIdx.x;
 	const long long idx = (long long)row * R + (k < R ? perm[k] : 0);
 	double e = 0, res = 0;
@@ -1301,8 +1309,9 @@ __global__ void k_errors(LensView L, const double2* __restrict__ roots, int R, c
 		}
 		if (!deferred)
 		{
-			const double2 t = cs[row];
-			Witt w = witt_from(L, z, t.x, t.y, node, p2p(L, node, z));
+			double cp, sp;
+			M.phase(row, phi, cp, sp);
+			Witt w = witt_from(L, z, cp, sp, node, p2p(L, node, z));
 			e = err_bound(w);
 			res = w.f0.abs();
 			if (!isfinite(e)) { bad = 1; e = 0; res = 0; }
@@ -1311,7 +1320,7 @@ __global__ void k_errors(LensView L, const double2* __restrict__ roots, int R, c
 				cd ws;
 				double m;
 				lens_map_from(L, z, w.nr, w.fr, w.d1, w.d2, ws, m);
-				const size_t o = (size_t)(idx - (long long)row * R) * gridDim.y + row;
+				const size_t o = (size_t)(idx - (long long)row * R) * rows + row;
 				if (caus) caus[o] = make_double2(ws.re, ws.im);
 				if (mu) mu[o] = m;
 			}
@@ -1339,6 +1348,7 @@ __global__ __launch_bounds__(BLOCK) void k_errors_far(LensView L, const double2*
 {
 	__shared__ double red[BLOCK / 64][8];
 	const int n = min(*far.count, far.cap);
+	if (blockIdx.x == 0 && threadIdx.x == 0) far.count[1] += *far.count;
 	for (int d = blockIdx.x; d < n; d += gridDim.x)
 	{
 		const long long idx = far.list[d];
@@ -1489,9 +1499,10 @@ inline bool sweep_check(int done, int cap, int first)
 }
 
 // The first fmm_sweeps sweeps may take the root sums of their dense rows
-// from the FMM pass (when the buffers carry a root grid).
+// from the FMM pass (when the buffers carry a root grid). before_tail, when
+// set, runs once the tail_at sweeps are queued, before the tail launch.
 bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int nrows, int cap, unsigned fmm_mask,
-	int first_check = 3, int tail_at = 0)
+	int first_check = 3, int tail_at = 0, const std::function<bool()>& before_tail = {})
 {
 	ctx->phase_sweeps = cap;
 	const LensView L = ctx->lens();
@@ -1586,6 +1597,7 @@ bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int
 					!CCF_HIP(ctx, hipEventRecord(e, s)))
 					return false;
 			}
+			if (before_tail && !before_tail()) return false;
 			const SweepView Vn = view(B, (it + 1) & 1);
 			const double d = CLUSTER_REL * ctx->root_half_length;
 			ccf::Timer tt;
@@ -1660,6 +1672,9 @@ bool alloc_root_fmm(ccf_ctx* ctx, SweepBufs& b)
 	const int order = root_order_compiled(ctx->P.root_fmm_order > 0 ? ctx->P.root_fmm_order : 8);
 	const double leaf = ctx->P.root_fmm_leaf > 0 ? ctx->P.root_fmm_leaf : 6.0;
 	b.grid = plan_root_grid(hx, hy, R, order, leaf);
+	int lds = 0;
+	if (!CCF_HIP(ctx, hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, 0))) return false;
+	b.grid.bin_row = bin_row_lds(b.grid.nleaf, R) + BIN_STATIC_LDS <= (size_t)lds;
 	const RootGrid& g = b.grid;
 	const size_t ce = (size_t)rows * g.ncells * (g.order + 1);
 	return CCF_HIP(ctx, ctx->dmalloc(&b.slot, list * sizeof(int))) && CCF_HIP(ctx, ctx->dmalloc(&b.rmode, rows * sizeof(int))) &&
@@ -1792,6 +1807,47 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 	// does not depend on timing)
 	const bool tailed = ctx->P.precision == 1;
 	ctx->sweep_verbose = verbose;
+
+	// the error pass (max 1/mu error and the NaN abort, ccf.cuh:1039-1069)
+	// takes each row set once its roots are final: the branch centres now,
+	// every phi step's rows after the step, on a second stream beside the
+	// later steps (whose tails and launch gaps leave most of the device
+	// idle). Its statistics are integer sums and maxima and every point
+	// writes its own outputs, so the results do not depend on the overlap.
+	const LensView L = ctx->lens();
+	// the caustic map and length scales the run will write come from the
+	// same evaluation of the field; the caustics stage then has them
+	const long long total = (long long)(np + nb) * R;
+	const bool wc = ctx->P.write_caustics != 0, wm = ctx->P.write_mu_length_scales != 0;
+	ctx->caustics_done = ctx->mu_done = false;
+	if (wc && !ctx->caustics && !CCF_HIP(ctx, ctx->dmalloc(&ctx->caustics, (size_t)total * sizeof(double2)))) return false;
+	if (wm && !ctx->mu_scales && !CCF_HIP(ctx, ctx->dmalloc(&ctx->mu_scales, (size_t)total * sizeof(double)))) return false;
+	// the list of points outside the tree: two counts, then up to FAR_POINTS_CAP indices
+	const int far_cap = (int)std::min<long long>(total, FAR_POINTS_CAP);
+	if (!ctx->far_points && !CCF_HIP(ctx, ctx->dmalloc(&ctx->far_points, sizeof(long long) * (1 + (size_t)FAR_POINTS_CAP))))
+		return false;
+	const FarPoints far{(int*)ctx->far_points, (long long*)ctx->far_points + 1, far_cap};
+	if (!ctx->err_stream && !CCF_HIP(ctx, hipStreamCreateWithFlags(&ctx->err_stream, hipStreamNonBlocking))) return false;
+	if (!ctx->err_event && !CCF_HIP(ctx, hipEventCreateWithFlags(&ctx->err_event, hipEventDisableTiming))) return false;
+	hipStream_t es = ctx->err_stream;
+	int nfar = 0;
+	bool first_pass = true;
+	auto errors = [&](const RowMap& Me, int nrows) -> bool {
+		if (!CCF_HIP(ctx, hipEventRecord(ctx->err_event, s)) || !CCF_HIP(ctx, hipStreamWaitEvent(es, ctx->err_event, 0)))
+			return false;
+		k_zero_count<<<1, 1, 0, es>>>(far.count, first_pass ? 1 : 0);
+		first_pass = false;
+		k_errors<<<dim3(xcd_round(grid(R)), nrows), BLOCK, 0, es>>>(L, ctx->ccs_init, R, Me, np + nb, ctx->dorder, ctx->counters,
+			wc ? ctx->caustics : nullptr, wm ? ctx->mu_scales : nullptr, far);
+		k_errors_far<<<FAR_POINTS_BLOCKS, BLOCK, 0, es>>>(L, ctx->ccs_init, R, np + nb, ctx->phase, ctx->counters,
+			wc ? ctx->caustics : nullptr, wm ? ctx->mu_scales : nullptr, far);
+		return CCF_LAUNCH(ctx, "find_errors_kernel");
+	};
+	// the rows of the previous step (the centres before the first) take
+	// their pass while this step's tail runs, a launch of one workgroup per
+	// row; without a tail, after the step
+	auto prev_rows = [&](int j) { return RowMap{R, np, nb, j - 1, j == 1 ? 1 : 2, 0.0, ctx->phase}; };
+	bool prev_done = false;
 	for (int j = 1; j <= steps; j++)
 	{
 		const RowMap M{R, np, nb, j, 2, 0.0, ctx->phase};
@@ -1817,9 +1873,15 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 		const int order = ctx->P.approx ? ctx->P.predictor : 0;
 		k_prepare<<<dim3(grid(R), 2 * nb), BLOCK, 0, s>>>(ctx->ccs_init, M, ctx->sw, order, ctx->dorder);
 		if (!CCF_LAUNCH(ctx, "prepare_roots_kernel")) return false;
+		prev_done = false;
+		const auto at_tail = [&]() {
+			prev_done = true;
+			return errors(prev_rows(j), j == 1 ? nb : 2 * nb);
+		};
 		if (!run_sweeps(ctx, ctx->sw, ctx->ccs_init, M, 2 * nb, cap, mask, tailed ? cap + 1 : first,
-				tailed ? (j < 3 ? TAIL_AT + 1 : TAIL_AT) : 0))
+				tailed ? (j < 3 ? TAIL_AT + 1 : TAIL_AT) : 0, at_tail))
 			return false;
+		if (!prev_done && !errors(prev_rows(j), j == 1 ? nb : 2 * nb)) return false;
 		prev[0] = ctx->fmm_rows_sweep[0];
 		prev[1] = ctx->fmm_rows_sweep[1];
 		// a step checked first at k can end no earlier than k: every
@@ -1833,39 +1895,20 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 			std::cout << "] " << j * 100 / steps << " %" << std::flush;
 		}
 	}
+	if (!errors(prev_rows(steps + 1), steps == 0 ? nb : 2 * nb)) return false;
 	if (!CCF_HIP(ctx, hipStreamSynchronize(s))) return false;
 	auto t1 = std::chrono::steady_clock::now();
 	ctx->info.t_ccs = std::chrono::duration<double>(t1 - t0).count();
 	if (verbose >= 1) std::cout << "\nDone finding critical curve positions. Elapsed time: " << ctx->info.t_ccs << " seconds.\n";
 
-	// max 1/mu error and the NaN abort (ccf.cuh:1039-1069)
-	const long long total = (long long)(np + nb) * R;
-	const LensView L = ctx->lens();
-	// the caustic map and length scales the run will write come from the
-	// same evaluation of the field; the caustics stage then has them
-	const bool wc = ctx->P.write_caustics != 0, wm = ctx->P.write_mu_length_scales != 0;
-	ctx->caustics_done = ctx->mu_done = false;
-	if (wc && !ctx->caustics && !CCF_HIP(ctx, ctx->dmalloc(&ctx->caustics, (size_t)total * sizeof(double2)))) return false;
-	if (wm && !ctx->mu_scales && !CCF_HIP(ctx, ctx->dmalloc(&ctx->mu_scales, (size_t)total * sizeof(double)))) return false;
-	// the list of points outside the tree: a count, then up to FAR_POINTS_CAP indices
-	const int far_cap = (int)std::min<long long>(total, FAR_POINTS_CAP);
-	if (!ctx->far_points && !CCF_HIP(ctx, ctx->dmalloc(&ctx->far_points, sizeof(long long) * (1 + (size_t)FAR_POINTS_CAP))))
-		return false;
-	const FarPoints far{(int*)ctx->far_points, (long long*)ctx->far_points + 1, far_cap};
-	k_zero_count<<<1, 1, 0, s>>>(far.count);
-	k_errors<<<dim3(xcd_round(grid(R)), np + nb), BLOCK, 0, s>>>(L, ctx->ccs_init, R, ctx->phase, ctx->dorder, ctx->counters,
-		wc ? ctx->caustics : nullptr, wm ? ctx->mu_scales : nullptr, far);
-	k_errors_far<<<FAR_POINTS_BLOCKS, BLOCK, 0, s>>>(L, ctx->ccs_init, R, np + nb, ctx->phase, ctx->counters,
-		wc ? ctx->caustics : nullptr, wm ? ctx->mu_scales : nullptr, far);
-	if (!CCF_LAUNCH(ctx, "find_errors_kernel")) return false;
 	unsigned long long c[C_COUNT] = {};
 	{
 		// fold the per-slot copies: max for the error bits, or for the NaN
 		// flag, sums otherwise
 		std::vector<unsigned long long> raw(STAT_WORDS);
-		int nfar = 0;
-		if (!CCF_HIP(ctx, hipMemcpyAsync(raw.data(), ctx->counters, STAT_WORDS * sizeof(unsigned long long), hipMemcpyDeviceToHost, s)) ||
-			!CCF_HIP(ctx, hipMemcpyAsync(&nfar, far.count, sizeof(int), hipMemcpyDeviceToHost, s)) ||
+		if (!CCF_HIP(ctx, hipStreamSynchronize(es)) ||
+			!CCF_HIP(ctx, hipMemcpyAsync(raw.data(), ctx->counters, STAT_WORDS * sizeof(unsigned long long), hipMemcpyDeviceToHost, s)) ||
+			!CCF_HIP(ctx, hipMemcpyAsync(&nfar, far.count + 1, sizeof(int), hipMemcpyDeviceToHost, s)) ||
 			!CCF_HIP(ctx, hipStreamSynchronize(s)))
 			return false;
 		ctx->info.points_outside_tree = nfar;
