@@ -260,7 +260,6 @@ __global__ __launch_bounds__(BIN_THREADS) void k_rbin_row(RootView V, const doub
 	const int nxl = V.g.nx << V.g.leaf, nyl = V.g.ny << V.g.leaf;
 	// every wave covers whole groups of 64: j - lane is a multiple of 64
 	const int nj = (R + BIN_THREADS - 1) / BIN_THREADS * BIN_THREADS;
-#pragma unroll 4
 	for (int j = threadIdx.x; j < nj; j += BIN_THREADS)
 	{
 		int c = -1;
@@ -831,9 +830,10 @@ __global__ __launch_bounds__(BLOCK) void k_rsum(RootView V, const double2* __res
 // of a row takes its outliers b, b + OUT_BLOCKS, ..., OUT_GROUP at a time,
 // so that each pass over the row's roots serves a group of them.
 constexpr int OUT_GROUP = 8;
+constexpr int OUT_THREADS = 1024; // a pass over the row's roots: R / 1024 loads per lane
 
 template <int P>
-__global__ __launch_bounds__(BLOCK) void k_rsum_out(RootView V, const double2* __restrict__ roots, RowMap M,
+__global__ __launch_bounds__(OUT_THREADS) void k_rsum_out(RootView V, const double2* __restrict__ roots, RowMap M,
 	unsigned long long* __restrict__ stats)
 {
 	// grid (rows, OUT_BLOCKS): the outliers of a row split over its blocks
@@ -846,7 +846,7 @@ __global__ __launch_bounds__(BLOCK) void k_rsum_out(RootView V, const double2* _
 	double phi;
 	M.get(r, row, phi);
 	const double2* zr = roots + (size_t)row * R;
-	__shared__ double red[OUT_GROUP][2][BLOCK / 64];
+	__shared__ double red[OUT_GROUP][2][OUT_THREADS / 64];
 	for (int o0 = blockIdx.y; o0 < no; o0 += gridDim.y * OUT_GROUP)
 	{
 		// the group: outliers o0 + k gridDim.y that are listed (slot >= 0)
@@ -863,7 +863,7 @@ __global__ __launch_bounds__(BLOCK) void k_rsum_out(RootView V, const double2* _
 			z[k] = cd(v.x, v.y);
 			ax[k] = ay[k] = 0;
 		}
-		for (int q = threadIdx.x; q < R; q += BLOCK)
+		for (int q = threadIdx.x; q < R; q += OUT_THREADS)
 		{
 			const double2 u = zr[q];
 #pragma unroll
@@ -889,7 +889,7 @@ __global__ __launch_bounds__(BLOCK) void k_rsum_out(RootView V, const double2* _
 		{
 			const int k = threadIdx.x;
 			double sx = 0, sy = 0;
-			for (int w = 0; w < BLOCK / 64; w++)
+			for (int w = 0; w < OUT_THREADS / 64; w++)
 			{
 				sx += red[k][0][w];
 				sy += red[k][1][w];
@@ -935,7 +935,7 @@ void root_fmm_pass_p(const SweepBufs& B, const double2* roots, const RowMap& M, 
 		k_rdown32<P><<<dim3(grid((g.nx << l) * (g.ny << l), TPB), nrows), 32 * TPB, 0, s>>>(V, l, stats);
 	}
 	k_rsum<P><<<dim3(grid(R), nrows), BLOCK, 0, s>>>(V, roots, M, stats);
-	k_rsum_out<P><<<dim3(nrows, OUT_BLOCKS), BLOCK, 0, s>>>(V, roots, M, stats);
+	k_rsum_out<P><<<dim3(nrows, OUT_BLOCKS), OUT_THREADS, 0, s>>>(V, roots, M, stats);
 }
 
 // compiled expansion orders; a requested order runs at the next one up

@@ -425,8 +425,10 @@ __global__ __launch_bounds__(BLOCK) void k_witt(LensView L, double2* __restrict_
 // its node (the same terms as p2p's range loop, split over the lanes and
 // reduced in a fixed order), then finishes the root as k_witt does. Few
 // roots lie outside the tree, and the launch comes every sweep whether a
-// row has any or not: 16 workgroups per row, each taking every 16th
+// row has any or not: 16 workgroups per row, each taking every 16th, of
+// WITT_FAR_THREADS lanes (a root's sum over N stars: N / 512 loads per lane)
 constexpr int WITT_FAR_BLOCKS = 16;
+constexpr int WITT_FAR_THREADS = 512;
 
 // The near-field sums of z's node (a node that is not a leaf: z lies
 // outside the tree, whose root then holds every star) over its star
@@ -477,13 +479,13 @@ __device__ inline Near coop_near(const LensView& L, int node, cd z, double (*red
 }
 
 template <bool MIXED>
-__global__ __launch_bounds__(BLOCK) void k_witt_far(LensView L, double2* __restrict__ roots, RowMap M, SweepView V,
+__global__ __launch_bounds__(WITT_FAR_THREADS) void k_witt_far(LensView L, double2* __restrict__ roots, RowMap M, SweepView V,
 	unsigned long long* __restrict__ stats, int circ_exact, int fmm_pole)
 {
 	const int r = blockIdx.y;
 	const int R = M.R;
 	const int nd = V.dcnt[r];
-	__shared__ double red[BLOCK / 64][8];
+	__shared__ double red[WITT_FAR_THREADS / 64][8];
 	for (int d = blockIdx.x; d < nd; d += gridDim.x)
 	{
 		int row;
@@ -493,7 +495,7 @@ __global__ __launch_bounds__(BLOCK) void k_witt_far(LensView L, double2* __restr
 		double2* p = roots + (size_t)row * R + a;
 		const cd z(p->x, p->y);
 		const int node = nearest_node(L, z);
-		const Near nr = coop_near(L, node, z, red);
+		const Near nr = coop_near<WITT_FAR_THREADS>(L, node, z, red);
 		if (threadIdx.x == 0)
 		{
 			const Near& sum = nr;
@@ -1535,12 +1537,12 @@ bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int
 		if (f32)
 		{
 			k_witt<true><<<g_witt, BLOCK, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, fmm_pole ? 1 : 0);
-			k_witt_far<true><<<g_far, BLOCK, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, fmm_pole ? 1 : 0);
+			k_witt_far<true><<<g_far, WITT_FAR_THREADS, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, fmm_pole ? 1 : 0);
 		}
 		else
 		{
 			k_witt<false><<<g_witt, BLOCK, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, 0);
-			k_witt_far<false><<<g_far, BLOCK, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, 0);
+			k_witt_far<false><<<g_far, WITT_FAR_THREADS, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, 0);
 		}
 		if (ctx->P.timing && tw.a && tw.b)
 		{
