@@ -572,8 +572,9 @@ bool grow(ccf_ctx* ctx, V*& ptr, int old_n, int new_n, int width)
 	if (!CCF_HIP(ctx, ctx->dmalloc(&p, sizeof(V) * (size_t)new_n * width))) return false;
 	if (ptr)
 	{
+		// the old buffer goes back to the context's pool, whose next user
+		// queues on the same stream after this copy
 		if (old_n > 0 && !CCF_HIP(ctx, hipMemcpyAsync(p, ptr, sizeof(V) * (size_t)old_n * width, hipMemcpyDeviceToDevice, ctx->stream))) return false;
-		if (!CCF_HIP(ctx, hipStreamSynchronize(ctx->stream))) return false;
 		ctx->dfree(ptr);
 	}
 	ptr = p;
@@ -660,7 +661,11 @@ bool build_forest(ccf_ctx* ctx, ForestSpec& spec, int verbose)
 
 	T.nnodes = 0;
 	T.capacity = 0;
-	if (!ensure_capacity(ctx, 4096 + N / 2 + 4 * F)) return false;
+	// the node count of the last tree of the same star and field counts, if
+	// any, so that a repeated build does not grow its arrays level by level
+	int cap0 = 4096 + N / 2 + 4 * F;
+	if (ctx->tree_hint_stars == N && ctx->tree_hint_fields == F) cap0 = std::max(cap0, ctx->tree_hint_nodes);
+	if (!ensure_capacity(ctx, cap0)) return false;
 	int* scal = nullptr;
 	if (!CCF_HIP(ctx, ctx->dmalloc(&scal, 4 * sizeof(int)))) return false;
 
@@ -732,6 +737,9 @@ bool build_forest(ccf_ctx* ctx, ForestSpec& spec, int verbose)
 	ctx->dfree(scal);
 	if (!ok) return false;
 	if (verbose >= 2) std::cout << "tree_levels set to: " << T.levels << "\n\n";
+	ctx->tree_hint_stars = N;
+	ctx->tree_hint_fields = F;
+	ctx->tree_hint_nodes = T.nnodes;
 
 	auto t1 = std::chrono::steady_clock::now();
 	if (verbose >= 1)
