@@ -692,18 +692,22 @@ bool build_forest(ccf_ctx* ctx, ForestSpec& spec, int verbose)
 	T.level_cnt = {F};
 	T.levels = 0;
 
-	bool ok = true;
-	while (!P.direct)
+	// scal[2], the list overflow flag, stays set once a level's (capped)
+	// lists overflow: it is read with the next level's counts (the loop
+	// ends at such a read-back), one read-back per level
+	bool ok = CCF_HIP(ctx, hipMemsetAsync(scal, 0, 4 * sizeof(int), s));
+	int h_scal[4] = {0, 0, 0, 0};
+	while (ok && !P.direct)
 	{
 		const int off = T.level_off[T.levels], cnt = T.level_cnt[T.levels];
-		int h_scal[4] = {0, 0, 0, 0};
-		ok = CCF_HIP(ctx, hipMemsetAsync(scal, 0, 4 * sizeof(int), s));
+		ok = CCF_HIP(ctx, hipMemsetAsync(scal, 0, 2 * sizeof(int), s));
 		if (!ok) break;
 		k_level_count<<<grid(cnt), BLOCK, 0, s>>>(off, cnt, T.scount, T.nbr, T.nnbr, T.split, scal);
 		k_scan<<<1, 1024, 0, s>>>(T.split, cnt, T.rank, scal + 1);
 		ok = CCF_LAUNCH(ctx, "tree level count") && CCF_HIP(ctx, hipMemcpyAsync(h_scal, scal, 4 * sizeof(int), hipMemcpyDeviceToHost, s)) &&
 			CCF_HIP(ctx, hipStreamSynchronize(s));
 		if (!ok) break;
+		if (h_scal[2]) break;
 		if (verbose >= 3)
 		{
 			std::cout << "\nProcessing level " << T.levels << "\n";
@@ -720,19 +724,17 @@ bool build_forest(ccf_ctx* ctx, ForestSpec& spec, int verbose)
 			ctx->spos_tmp, ctx->smass_tmp);
 		k_set_neighbors<<<grid(new_cnt), BLOCK, 0, s>>>(new_off, new_cnt, T.center, T.hl, T.parent, T.child, T.scount,
 			T.nbr, T.nnbr, T.sl, T.nsl, T.dl, T.ndl, scal);
-		ok = CCF_LAUNCH(ctx, "tree split level") && CCF_HIP(ctx, hipMemcpyAsync(h_scal, scal, 4 * sizeof(int), hipMemcpyDeviceToHost, s)) &&
-			CCF_HIP(ctx, hipStreamSynchronize(s));
+		ok = CCF_LAUNCH(ctx, "tree split level");
 		if (!ok) break;
-		if (h_scal[2])
-		{
-			ctx->fail("Error. A node's neighbour or interaction list exceeded its maximum size.\n");
-			ok = false;
-			break;
-		}
 		T.nnodes = new_off + new_cnt;
 		T.level_off.push_back(new_off);
 		T.level_cnt.push_back(new_cnt);
 		T.levels++;
+	}
+	if (ok && h_scal[2])
+	{
+		ctx->fail("Error. A node's neighbour or interaction list exceeded its maximum size.\n");
+		ok = false;
 	}
 	ctx->dfree(scal);
 	if (!ok) return false;
