@@ -795,6 +795,9 @@ bool build_forest(ccf_ctx* ctx, ForestSpec& spec, int verbose)
 
 	// depth of every field's tree
 	spec.levels.assign(F, 0);
+	if (F == 1)
+		spec.levels[0] = T.levels;
+	else
 	{
 		std::vector<int> fid(T.nnodes);
 		if (!CCF_HIP(ctx, hipMemcpy(fid.data(), T.field, sizeof(int) * T.nnodes, hipMemcpyDeviceToHost))) return false;
