@@ -228,6 +228,7 @@ __device__ inline int block_scan_excl(int v, int* wsum, int& tot)
 // places them in index order. Within a cell the order is the atomics'
 // (k_rleaf sorts it). LDS: nleaf ints, then per 64 roots a mask and an int.
 constexpr int BIN_THREADS = 1024;
+constexpr int BIN_BATCH = 4;        // roots per thread whose loads k_rbin_row issues at once
 constexpr int BIN_STATIC_LDS = 256; // k_rbin_row's own LDS beside the dynamic part
 
 inline size_t bin_row_lds(int nleaf, int R)
@@ -258,21 +259,34 @@ __global__ __launch_bounds__(BIN_THREADS) void k_rbin_row(RootView V, const doub
 	int* __restrict__ cell = V.cell + (size_t)r * R;
 	const double sl = ldexp(V.g.s0, -V.g.leaf);
 	const int nxl = V.g.nx << V.g.leaf, nyl = V.g.ny << V.g.leaf;
-	// every wave covers whole groups of 64: j - lane is a multiple of 64
-	const int nj = (R + BIN_THREADS - 1) / BIN_THREADS * BIN_THREADS;
-	for (int j = threadIdx.x; j < nj; j += BIN_THREADS)
+	// every wave covers whole groups of 64: j - lane is a multiple of 64.
+	// BIN_BATCH roots per thread and pass, their loads issued together.
+	constexpr int step = BIN_THREADS * BIN_BATCH;
+	const int nj = (R + step - 1) / step * step;
+	for (int j0 = threadIdx.x; j0 < nj; j0 += step)
 	{
-		int c = -1;
-		if (j < R)
+		double2 z[BIN_BATCH];
+#pragma unroll
+		for (int k = 0; k < BIN_BATCH; k++)
 		{
-			const double2 z = zr[j];
-			const double fx = (z.x - V.g.x0) / sl, fy = (z.y - V.g.y0) / sl;
-			if (fx >= 0 && fx < nxl && fy >= 0 && fy < nyl) c = (int)fy * nxl + (int)fx;
-			cell[j] = c;
-			if (c >= 0) atomicAdd(&cnt[c], 1);
+			const int j = j0 + k * BIN_THREADS;
+			z[k] = j < R ? zr[j] : make_double2(0, 0);
 		}
-		const unsigned long long m = __ballot(j < R && c < 0);
-		if (lane == 0 && j < R) omask[j >> 6] = m;
+#pragma unroll
+		for (int k = 0; k < BIN_BATCH; k++)
+		{
+			const int j = j0 + k * BIN_THREADS;
+			int c = -1;
+			if (j < R)
+			{
+				const double fx = (z[k].x - V.g.x0) / sl, fy = (z[k].y - V.g.y0) / sl;
+				if (fx >= 0 && fx < nxl && fy >= 0 && fy < nyl) c = (int)fy * nxl + (int)fx;
+				cell[j] = c;
+				if (c >= 0) atomicAdd(&cnt[c], 1);
+			}
+			const unsigned long long m = __ballot(j < R && c < 0);
+			if (lane == 0 && j < R) omask[j >> 6] = m;
+		}
 	}
 	__syncthreads();
 	// the outliers in index order: prefix of the groups' popcounts
@@ -317,11 +331,18 @@ __global__ __launch_bounds__(BIN_THREADS) void k_rbin_row(RootView V, const doub
 	if (threadIdx.x == 0) st[nl] = run;
 	__syncthreads();
 	int* __restrict__ sidx = V.sidx + (size_t)r * R;
-#pragma unroll 4
-	for (int j = threadIdx.x; j < R; j += BIN_THREADS)
+	for (int j0 = threadIdx.x; j0 < R; j0 += step)
 	{
-		const int c = cell[j];
-		if (c >= 0) sidx[atomicAdd(&cnt[c], 1)] = j;
+		int c[BIN_BATCH];
+#pragma unroll
+		for (int k = 0; k < BIN_BATCH; k++)
+		{
+			const int j = j0 + k * BIN_THREADS;
+			c[k] = j < R ? cell[j] : -1;
+		}
+#pragma unroll
+		for (int k = 0; k < BIN_BATCH; k++)
+			if (c[k] >= 0) sidx[atomicAdd(&cnt[c[k]], 1)] = j0 + k * BIN_THREADS;
 	}
 }
 
