@@ -346,7 +346,7 @@ __global__ __launch_bounds__(BIN_THREADS) void k_rbin_row(RootView V, const doub
 	}
 }
 
-constexpr int LEAF_REG = 16; // cells up to this many roots sort in registers
+constexpr int LEAF_REG = 32; // cells up to this many roots sort in registers
 
 // per leaf cell: the cell's roots in index order, their positions copied
 // contiguously, and the multipole expansion (P2M)
