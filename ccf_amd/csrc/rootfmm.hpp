@@ -827,6 +827,7 @@ __global__ __launch_bounds__(BLOCK) void k_rsum(RootView V, const double2* __res
 			{
 				const int q0 = st[y * nxl + max(ix - 1, 0)], q1 = st[y * nxl + min(ix + 1, nxl - 1) + 1];
 				nn += q1 - q0;
+#pragma unroll 4
 				for (int q = q0; q < q1; q++) direct_term<P>(z, sp[q], ax, ay);
 			}
 			const int no = V.ocnt[r];
