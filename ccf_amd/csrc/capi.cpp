@@ -282,8 +282,15 @@ bool read_star_file(ccf_ctx* c, const std::string& fname)
 			c->fail("Error. Invalid rectangular input. rectangular must be 1 (rectangular) or 0 (circular).\n");
 			return false;
 		}
+		const bool dbl = fsize == 8 + 16 + 8 + (uint64_t)n * 24, flt = fsize == 8 + 8 + 4 + (uint64_t)n * 12;
+		if (!dbl && !flt)
+		{
+			c->fail("Error. Star input file " + fname + " does not contain validly formatted single or double precision stars and accompanying information.\n");
+			return false;
+		}
+		// sized only once the file holds n stars: a header's n alone allocates nothing
 		st.assign(3 * (size_t)n, 0.0);
-		if (fsize == 8 + 16 + 8 + (uint64_t)n * 24)
+		if (dbl)
 		{
 			double cc[2], t;
 			in.read((char*)cc, 16);
@@ -292,7 +299,7 @@ bool read_star_file(ccf_ctx* c, const std::string& fname)
 			c->corner = cd(cc[0], cc[1]);
 			c->theta = t;
 		}
-		else if (fsize == 8 + 8 + 4 + (uint64_t)n * 12)
+		else
 		{
 			float cc[2], t;
 			in.read((char*)cc, 8);
@@ -302,11 +309,6 @@ bool read_star_file(ccf_ctx* c, const std::string& fname)
 			for (size_t i = 0; i < buf.size(); i++) st[i] = buf[i];
 			c->corner = cd(cc[0], cc[1]);
 			c->theta = t;
-		}
-		else
-		{
-			c->fail("Error. Star input file " + fname + " does not contain validly formatted single or double precision stars and accompanying information.\n");
-			return false;
 		}
 		if (c->corner.re < 0 || c->corner.im < 0)
 		{
@@ -638,6 +640,30 @@ bool need(ccf_ctx* c, bool ok, const char* what)
 
 double ms_seconds(double s) { return std::floor(s * 1000.0) / 1000.0; } // stopwatch.hpp:22-33
 
+// Every extern "C" entry point that can allocate runs inside guarded(): an
+// exception (std::bad_alloc, a filesystem error) becomes the reference's
+// error convention -- a message and status 1 -- instead of unwinding through
+// the C boundary into the caller.
+template <class F>
+int guarded(ccf_ctx* c, F&& f)
+{
+	try
+	{
+		return f();
+	}
+	catch (const std::bad_alloc&)
+	{
+		if (c) c->fail("Error. Out of host memory.\n");
+		else std::cerr << "Error. Out of host memory.\n";
+	}
+	catch (const std::exception& e)
+	{
+		if (c) c->fail(std::string("Error. ") + e.what() + "\n");
+		else std::cerr << "Error. " << e.what() << "\n";
+	}
+	return 1;
+}
+
 } // namespace
 
 // ---------------------------------------------------------------------------
@@ -746,197 +772,217 @@ static bool derive(ccf_ctx* c, int verbose)
 
 int ccf_calculate_derived_params(ccf_ctx* c, int verbose)
 {
-	if (!c) return 1;
-	return derive(c, verbose) ? 0 : 1;
+	return guarded(c, [&]() -> int {
+		if (!c) return 1;
+		return derive(c, verbose) ? 0 : 1;
+	});
 }
 
 int ccf_load(ccf_ctx* c, int verbose)
 {
-	if (!c) return 1;
-	c->free_device();
-	c->loaded = false;
-	if (!derive(c, verbose)) return 1;
-	if (!device_init(c)) return 1;
-	if (!populate(c, verbose)) return 1;
-	c->loaded = true;
-	return 0;
+	return guarded(c, [&]() -> int {
+		if (!c) return 1;
+		c->free_device();
+		c->loaded = false;
+		if (!derive(c, verbose)) return 1;
+		if (!device_init(c)) return 1;
+		if (!populate(c, verbose)) return 1;
+		c->loaded = true;
+		return 0;
+	});
 }
 
 int ccf_build_tree_fmm(ccf_ctx* c, int verbose)
 {
-	if (!c || !need(c, c->loaded && !c->prepared, "ccf_load")) return 1;
-	if (!ccf::build_tree(c, verbose)) return 1;
-	c->info.num_stars = c->num_stars;
-	c->info.num_roots = c->num_roots;
-	c->info.taylor_smooth = c->taylor_smooth;
-	c->info.random_seed = c->random_seed;
-	c->info.kappa_star = c->kappa_star;
-	c->info.kappa_star_actual = c->kappa_star_actual;
-	c->info.mu_ave = c->mu_ave;
-	c->info.corner_re = c->corner.re;
-	c->info.corner_im = c->corner.im;
-	c->info.alpha_error = c->alpha_error;
-	c->prepared = true;
-	return 0;
+	return guarded(c, [&]() -> int {
+		if (!c || !need(c, c->loaded && !c->prepared, "ccf_load")) return 1;
+		if (!ccf::build_tree(c, verbose)) return 1;
+		c->info.num_stars = c->num_stars;
+		c->info.num_roots = c->num_roots;
+		c->info.taylor_smooth = c->taylor_smooth;
+		c->info.random_seed = c->random_seed;
+		c->info.kappa_star = c->kappa_star;
+		c->info.kappa_star_actual = c->kappa_star_actual;
+		c->info.mu_ave = c->mu_ave;
+		c->info.corner_re = c->corner.re;
+		c->info.corner_im = c->corner.im;
+		c->info.alpha_error = c->alpha_error;
+		c->prepared = true;
+		return 0;
+	});
 }
 
 int ccf_prepare(ccf_ctx* c, int verbose)
 {
-	if (ccf_load(c, verbose)) return 1;
-	return ccf_build_tree_fmm(c, verbose);
+	return guarded(c, [&]() -> int {
+		if (ccf_load(c, verbose)) return 1;
+		return ccf_build_tree_fmm(c, verbose);
+	});
 }
 
 int ccf_find_initial_roots(ccf_ctx* c, int verbose)
 {
-	if (!c || !need(c, c->prepared, "ccf_prepare")) return 1;
-	if (!ccf::find_initial_roots_dev(c, verbose)) return 1;
-	c->have_roots = true;
-	return 0;
+	return guarded(c, [&]() -> int {
+		if (!c || !need(c, c->prepared, "ccf_prepare")) return 1;
+		if (!ccf::find_initial_roots_dev(c, verbose)) return 1;
+		c->have_roots = true;
+		return 0;
+	});
 }
 
 int ccf_find_ccs(ccf_ctx* c, int verbose)
 {
-	if (!c || !need(c, c->have_roots, "ccf_find_initial_roots")) return 1;
-	if (!ccf::find_ccs_dev(c, verbose)) return 1;
-	c->info.max_error = c->max_error;
-	c->have_ccs = true;
-	return 0;
+	return guarded(c, [&]() -> int {
+		if (!c || !need(c, c->have_roots, "ccf_find_initial_roots")) return 1;
+		if (!ccf::find_ccs_dev(c, verbose)) return 1;
+		c->info.max_error = c->max_error;
+		c->have_ccs = true;
+		return 0;
+	});
 }
 
 int ccf_find_caustics(ccf_ctx* c, int verbose)
 {
-	if (!c || !need(c, c->have_ccs, "ccf_find_ccs")) return 1;
-	if (!c->P.write_caustics) return 0;
-	if (verbose >= 2) std::cout << "Finding caustic positions...\n";
-	auto t0 = std::chrono::steady_clock::now();
-	if (!ccf::find_caustics_dev(c, true, false)) return 1;
-	c->info.t_caustics = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-	if (verbose >= 2) std::cout << "Done finding caustic positions. Elapsed time: " << c->info.t_caustics << " seconds.\n\n";
-	c->have_caustics = true;
-	return 0;
+	return guarded(c, [&]() -> int {
+		if (!c || !need(c, c->have_ccs, "ccf_find_ccs")) return 1;
+		if (!c->P.write_caustics) return 0;
+		if (verbose >= 2) std::cout << "Finding caustic positions...\n";
+		auto t0 = std::chrono::steady_clock::now();
+		if (!ccf::find_caustics_dev(c, true, false)) return 1;
+		c->info.t_caustics = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+		if (verbose >= 2) std::cout << "Done finding caustic positions. Elapsed time: " << c->info.t_caustics << " seconds.\n\n";
+		c->have_caustics = true;
+		return 0;
+	});
 }
 
 int ccf_find_mu_length_scales(ccf_ctx* c, int verbose)
 {
-	if (!c || !need(c, c->have_ccs, "ccf_find_ccs")) return 1;
-	if (!c->P.write_mu_length_scales) return 0;
-	if (verbose >= 2) std::cout << "Finding magnification length scales...\n";
-	if (!ccf::find_caustics_dev(c, false, true)) return 1;
-	if (verbose >= 2) std::cout << "Done finding magnification length scales.\n\n";
-	c->have_mu = true;
-	return 0;
+	return guarded(c, [&]() -> int {
+		if (!c || !need(c, c->have_ccs, "ccf_find_ccs")) return 1;
+		if (!c->P.write_mu_length_scales) return 0;
+		if (verbose >= 2) std::cout << "Finding magnification length scales...\n";
+		if (!ccf::find_caustics_dev(c, false, true)) return 1;
+		if (verbose >= 2) std::cout << "Done finding magnification length scales.\n\n";
+		c->have_mu = true;
+		return 0;
+	});
 }
 
 int ccf_run(ccf_ctx* c, int verbose)
 {
-	if (!c) return 1;
-	if (ccf_prepare(c, verbose)) return 1;
-	if (ccf_find_initial_roots(c, verbose)) return 1;
-	if (ccf_find_ccs(c, verbose)) return 1;
-	if (ccf_find_caustics(c, verbose)) return 1;
-	if (ccf_find_mu_length_scales(c, verbose)) return 1;
-	return 0;
+	return guarded(c, [&]() -> int {
+		if (!c) return 1;
+		if (ccf_prepare(c, verbose)) return 1;
+		if (ccf_find_initial_roots(c, verbose)) return 1;
+		if (ccf_find_ccs(c, verbose)) return 1;
+		if (ccf_find_caustics(c, verbose)) return 1;
+		if (ccf_find_mu_length_scales(c, verbose)) return 1;
+		return 0;
+	});
 }
 
 int ccf_save(ccf_ctx* c, int verbose)
 {
-	if (!c || !need(c, c->have_ccs, "ccf_run")) return 1;
-	const ccf_params& P = c->P;
-	const std::string pre = P.outfile_prefix;
-	const bool nofile = !c->from_file;
-	const int R = c->num_roots, np = P.num_phi, nb = P.num_branches;
-	// ccf.cuh:1129-1192
-	if (verbose >= 2) std::cout << "Writing parameter info...\n";
-	{
-		const std::string fname = pre + "ccf_parameter_info.txt";
-		std::ofstream out(fname);
-		if (!out.is_open()) { c->fail("Error. Failed to open file " + fname + "\n"); return 1; }
-		out.precision(9);
-		out << "kappa_tot " << P.kappa_tot << "\n";
-		out << "shear " << P.shear << "\n";
-		out << "mu_ave " << c->mu_ave << "\n";
-		out << "smooth_fraction " << (1 - c->kappa_star / P.kappa_tot) << "\n";
-		out << "kappa_star " << c->kappa_star << "\n";
-		if (nofile) out << "kappa_star_actual " << c->kappa_star_actual << "\n";
-		out << "theta_star " << c->theta << "\n";
-		out << "random_seed " << c->random_seed << "\n";
-		if (nofile)
+	return guarded(c, [&]() -> int {
+		if (!c || !need(c, c->have_ccs, "ccf_run")) return 1;
+		const ccf_params& P = c->P;
+		const std::string pre = P.outfile_prefix;
+		const bool nofile = !c->from_file;
+		const int R = c->num_roots, np = P.num_phi, nb = P.num_branches;
+		// ccf.cuh:1129-1192
+		if (verbose >= 2) std::cout << "Writing parameter info...\n";
 		{
-			out << "mass_function " << P.mass_function << "\n";
-			if (!std::strcmp(P.mass_function, "salpeter") || !std::strcmp(P.mass_function, "kroupa")) out << "m_solar " << P.m_solar << "\n";
-			out << "m_lower " << P.m_lower << "\n";
-			out << "m_upper " << P.m_upper << "\n";
-			out << "mean_mass " << c->mean_mass << "\n";
-			out << "mean_mass2 " << c->mean_mass2 << "\n";
-			out << "mean_mass2_ln_mass " << c->mean_mass2_ln_mass << "\n";
+			const std::string fname = pre + "ccf_parameter_info.txt";
+			std::ofstream out(fname);
+			if (!out.is_open()) { c->fail("Error. Failed to open file " + fname + "\n"); return 1; }
+			out.precision(9);
+			out << "kappa_tot " << P.kappa_tot << "\n";
+			out << "shear " << P.shear << "\n";
+			out << "mu_ave " << c->mu_ave << "\n";
+			out << "smooth_fraction " << (1 - c->kappa_star / P.kappa_tot) << "\n";
+			out << "kappa_star " << c->kappa_star << "\n";
+			if (nofile) out << "kappa_star_actual " << c->kappa_star_actual << "\n";
+			out << "theta_star " << c->theta << "\n";
+			out << "random_seed " << c->random_seed << "\n";
+			if (nofile)
+			{
+				out << "mass_function " << P.mass_function << "\n";
+				if (!std::strcmp(P.mass_function, "salpeter") || !std::strcmp(P.mass_function, "kroupa")) out << "m_solar " << P.m_solar << "\n";
+				out << "m_lower " << P.m_lower << "\n";
+				out << "m_upper " << P.m_upper << "\n";
+				out << "mean_mass " << c->mean_mass << "\n";
+				out << "mean_mass2 " << c->mean_mass2 << "\n";
+				out << "mean_mass2_ln_mass " << c->mean_mass2_ln_mass << "\n";
+			}
+			out << "m_lower_actual " << c->m_lower_actual << "\n";
+			out << "m_upper_actual " << c->m_upper_actual << "\n";
+			out << "mean_mass_actual " << c->mean_mass_actual << "\n";
+			out << "mean_mass2_actual " << c->mean_mass2_actual << "\n";
+			out << "mean_mass2_ln_mass_actual " << c->mean_mass2_ln_mass_actual << "\n";
+			out << "num_stars " << c->num_stars << "\n";
+			if (c->rectangular)
+			{
+				out << "corner_x1 " << c->corner.re << "\n";
+				out << "corner_x2 " << c->corner.im << "\n";
+				if (P.approx) out << "taylor_smooth " << c->taylor_smooth << "\n";
+			}
+			else
+			{
+				out << "rad " << c->corner.abs() << "\n";
+			}
+			out << "num_roots " << R << "\n";
+			out << "num_phi " << np << "\n";
+			out << "num_branches " << nb << "\n";
+			out << "max_error_1/mu " << c->max_error << "\n";
+			out << "t_init_roots " << ms_seconds(c->info.t_init_roots) << "\n";
+			out << "t_ccs " << ms_seconds(c->info.t_ccs) << "\n";
+			out << "t_caustics " << ms_seconds(c->info.t_caustics) << "\n";
+			if (verbose >= 1) std::cout << "Done writing parameter info to file " << fname << "\n";
 		}
-		out << "m_lower_actual " << c->m_lower_actual << "\n";
-		out << "m_upper_actual " << c->m_upper_actual << "\n";
-		out << "mean_mass_actual " << c->mean_mass_actual << "\n";
-		out << "mean_mass2_actual " << c->mean_mass2_actual << "\n";
-		out << "mean_mass2_ln_mass_actual " << c->mean_mass2_ln_mass_actual << "\n";
-		out << "num_stars " << c->num_stars << "\n";
-		if (c->rectangular)
+		if (P.write_stars)
 		{
-			out << "corner_x1 " << c->corner.re << "\n";
-			out << "corner_x2 " << c->corner.im << "\n";
-			if (P.approx) out << "taylor_smooth " << c->taylor_smooth << "\n";
+			std::vector<double> st(3 * (size_t)c->num_stars);
+			int nr, nc;
+			if (ccf_get_array(c, CCF_ARRAY_STARS, st.data(), (long long)st.size() * 8, &nr, &nc)) return 1;
+			const std::string fname = pre + "ccf_stars.bin";
+			if (!write_star_file(c, fname, st)) { c->fail("Error. Unable to write star info to file " + fname + "\n"); return 1; }
+			if (verbose >= 1) std::cout << "Done writing star info to file " << fname << "\n";
 		}
-		else
+		const size_t pts = (size_t)R * (np + nb);
+		if (P.write_critical_curves)
 		{
-			out << "rad " << c->corner.abs() << "\n";
+			std::vector<double> buf(2 * pts);
+			int nr, nc;
+			if (ccf_get_array(c, CCF_ARRAY_CCS, buf.data(), (long long)buf.size() * 8, &nr, &nc)) return 1;
+			const std::string fname = pre + "ccf_ccs.bin";
+			if (!write_array(c, fname, buf.data(), 16, nr, nc)) { c->fail("Error. Unable to write ccs info to file " + fname + "\n"); return 1; }
+			if (verbose >= 1) std::cout << "Done writing critical curve positions to file " << fname << "\n";
 		}
-		out << "num_roots " << R << "\n";
-		out << "num_phi " << np << "\n";
-		out << "num_branches " << nb << "\n";
-		out << "max_error_1/mu " << c->max_error << "\n";
-		out << "t_init_roots " << ms_seconds(c->info.t_init_roots) << "\n";
-		out << "t_ccs " << ms_seconds(c->info.t_ccs) << "\n";
-		out << "t_caustics " << ms_seconds(c->info.t_caustics) << "\n";
-		if (verbose >= 1) std::cout << "Done writing parameter info to file " << fname << "\n";
-	}
-	if (P.write_stars)
-	{
-		std::vector<double> st(3 * (size_t)c->num_stars);
-		int nr, nc;
-		if (ccf_get_array(c, CCF_ARRAY_STARS, st.data(), (long long)st.size() * 8, &nr, &nc)) return 1;
-		const std::string fname = pre + "ccf_stars.bin";
-		if (!write_star_file(c, fname, st)) { c->fail("Error. Unable to write star info to file " + fname + "\n"); return 1; }
-		if (verbose >= 1) std::cout << "Done writing star info to file " << fname << "\n";
-	}
-	const size_t pts = (size_t)R * (np + nb);
-	if (P.write_critical_curves)
-	{
-		std::vector<double> buf(2 * pts);
-		int nr, nc;
-		if (ccf_get_array(c, CCF_ARRAY_CCS, buf.data(), (long long)buf.size() * 8, &nr, &nc)) return 1;
-		const std::string fname = pre + "ccf_ccs.bin";
-		if (!write_array(c, fname, buf.data(), 16, nr, nc)) { c->fail("Error. Unable to write ccs info to file " + fname + "\n"); return 1; }
-		if (verbose >= 1) std::cout << "Done writing critical curve positions to file " << fname << "\n";
-	}
-	if (P.write_caustics)
-	{
-		std::vector<double> buf(2 * pts);
-		int nr, nc;
-		if (ccf_get_array(c, CCF_ARRAY_CAUSTICS, buf.data(), (long long)buf.size() * 8, &nr, &nc)) return 1;
-		const std::string fname = pre + "ccf_caustics.bin";
-		if (!write_array(c, fname, buf.data(), 16, nr, nc)) { c->fail("Error. Unable to write caustic info to file " + fname + "\n"); return 1; }
-		if (verbose >= 1) std::cout << "Done writing caustic positions to file " << fname << "\n";
-	}
-	if (P.write_mu_length_scales)
-	{
-		std::vector<double> buf(pts);
-		int nr, nc;
-		if (ccf_get_array(c, CCF_ARRAY_MU_LENGTH_SCALES, buf.data(), (long long)buf.size() * 8, &nr, &nc)) return 1;
-		const std::string fname = pre + "ccf_mu_length_scales.bin";
-		if (!write_array(c, fname, buf.data(), 8, nr, nc))
+		if (P.write_caustics)
 		{
-			c->fail("Error. Unable to write magnification length scales info to file " + fname + "\n");
-			return 1;
+			std::vector<double> buf(2 * pts);
+			int nr, nc;
+			if (ccf_get_array(c, CCF_ARRAY_CAUSTICS, buf.data(), (long long)buf.size() * 8, &nr, &nc)) return 1;
+			const std::string fname = pre + "ccf_caustics.bin";
+			if (!write_array(c, fname, buf.data(), 16, nr, nc)) { c->fail("Error. Unable to write caustic info to file " + fname + "\n"); return 1; }
+			if (verbose >= 1) std::cout << "Done writing caustic positions to file " << fname << "\n";
 		}
-		if (verbose >= 1) std::cout << "Done writing magnification length scales to file " << fname << "\n";
-	}
-	return 0;
+		if (P.write_mu_length_scales)
+		{
+			std::vector<double> buf(pts);
+			int nr, nc;
+			if (ccf_get_array(c, CCF_ARRAY_MU_LENGTH_SCALES, buf.data(), (long long)buf.size() * 8, &nr, &nc)) return 1;
+			const std::string fname = pre + "ccf_mu_length_scales.bin";
+			if (!write_array(c, fname, buf.data(), 8, nr, nc))
+			{
+				c->fail("Error. Unable to write magnification length scales info to file " + fname + "\n");
+				return 1;
+			}
+			if (verbose >= 1) std::cout << "Done writing magnification length scales to file " << fname << "\n";
+		}
+		return 0;
+	});
 }
 
 int ccf_get_info(const ccf_ctx* c, ccf_info* info)
@@ -948,42 +994,44 @@ int ccf_get_info(const ccf_ctx* c, ccf_info* info)
 
 int ccf_get_array(ccf_ctx* c, int which, void* dst, long long capacity, int* nrows, int* ncols)
 {
-	if (!c || !dst) return 1;
-	const int R = c->num_roots, np = c->P.num_phi, nb = c->P.num_branches;
-	const size_t pts = (size_t)R * (np + nb);
-	const void* src = nullptr;
-	size_t bytes = 0;
-	int nr = R * nb, nc = np / nb + 1;
-	if (which == CCF_ARRAY_STARS)
-	{
-		if (!need(c, c->prepared, "ccf_prepare")) return 1;
-		const int N = c->num_stars;
-		if (capacity < (long long)N * 24) { c->fail("Error. ccf_get_array: destination too small\n"); return 1; }
-		std::vector<double2> pos(N);
-		std::vector<double> m(N);
-		if (!CCF_HIP(c, hipMemcpy(pos.data(), c->spos, (size_t)N * sizeof(double2), hipMemcpyDeviceToHost)) ||
-			!CCF_HIP(c, hipMemcpy(m.data(), c->smass, (size_t)N * sizeof(double), hipMemcpyDeviceToHost)))
-			return 1;
-		double* d = (double*)dst;
-		for (int i = 0; i < N; i++)
+	return guarded(c, [&]() -> int {
+		if (!c || !dst) return 1;
+		const int R = c->num_roots, np = c->P.num_phi, nb = c->P.num_branches;
+		const size_t pts = (size_t)R * (np + nb);
+		const void* src = nullptr;
+		size_t bytes = 0;
+		int nr = R * nb, nc = np / nb + 1;
+		if (which == CCF_ARRAY_STARS)
 		{
-			d[3 * i] = pos[i].x;
-			d[3 * i + 1] = pos[i].y;
-			d[3 * i + 2] = m[i];
+			if (!need(c, c->prepared, "ccf_prepare")) return 1;
+			const int N = c->num_stars;
+			if (capacity < (long long)N * 24) { c->fail("Error. ccf_get_array: destination too small\n"); return 1; }
+			std::vector<double2> pos(N);
+			std::vector<double> m(N);
+			if (!CCF_HIP(c, hipMemcpy(pos.data(), c->spos, (size_t)N * sizeof(double2), hipMemcpyDeviceToHost)) ||
+				!CCF_HIP(c, hipMemcpy(m.data(), c->smass, (size_t)N * sizeof(double), hipMemcpyDeviceToHost)))
+				return 1;
+			double* d = (double*)dst;
+			for (int i = 0; i < N; i++)
+			{
+				d[3 * i] = pos[i].x;
+				d[3 * i + 1] = pos[i].y;
+				d[3 * i + 2] = m[i];
+			}
+			if (nrows) *nrows = N;
+			if (ncols) *ncols = 3;
+			return 0;
 		}
-		if (nrows) *nrows = N;
-		if (ncols) *ncols = 3;
+		if (which == CCF_ARRAY_CCS) { if (!need(c, c->have_ccs, "ccf_find_ccs")) return 1; src = c->ccs; bytes = pts * 16; }
+		else if (which == CCF_ARRAY_CAUSTICS) { if (!need(c, c->have_caustics, "ccf_find_caustics")) return 1; src = c->caustics; bytes = pts * 16; }
+		else if (which == CCF_ARRAY_MU_LENGTH_SCALES) { if (!need(c, c->have_mu, "ccf_find_mu_length_scales")) return 1; src = c->mu_scales; bytes = pts * 8; }
+		else { c->fail("Error. ccf_get_array: unknown array\n"); return 1; }
+		if (capacity < (long long)bytes) { c->fail("Error. ccf_get_array: destination too small\n"); return 1; }
+		if (!CCF_HIP(c, hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost))) return 1;
+		if (nrows) *nrows = nr;
+		if (ncols) *ncols = nc;
 		return 0;
-	}
-	if (which == CCF_ARRAY_CCS) { if (!need(c, c->have_ccs, "ccf_find_ccs")) return 1; src = c->ccs; bytes = pts * 16; }
-	else if (which == CCF_ARRAY_CAUSTICS) { if (!need(c, c->have_caustics, "ccf_find_caustics")) return 1; src = c->caustics; bytes = pts * 16; }
-	else if (which == CCF_ARRAY_MU_LENGTH_SCALES) { if (!need(c, c->have_mu, "ccf_find_mu_length_scales")) return 1; src = c->mu_scales; bytes = pts * 8; }
-	else { c->fail("Error. ccf_get_array: unknown array\n"); return 1; }
-	if (capacity < (long long)bytes) { c->fail("Error. ccf_get_array: destination too small\n"); return 1; }
-	if (!CCF_HIP(c, hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost))) return 1;
-	if (nrows) *nrows = nr;
-	if (ncols) *ncols = nc;
-	return 0;
+	});
 }
 
 const char* ccf_last_error(const ccf_ctx* c)
@@ -993,20 +1041,26 @@ const char* ccf_last_error(const ccf_ctx* c)
 
 int ccf_eval_points(ccf_ctx* c, const double* z, int n, double phi, double* out)
 {
-	if (!c || !need(c, c->prepared, "ccf_prepare")) return 1;
-	return ccf::eval_points_dev(c, z, n, phi, out) ? 0 : 1;
+	return guarded(c, [&]() -> int {
+		if (!c || !need(c, c->prepared, "ccf_prepare")) return 1;
+		return ccf::eval_points_dev(c, z, n, phi, out) ? 0 : 1;
+	});
 }
 
 int ccf_aberth_sweep(ccf_ctx* c, const double* roots, int n, double phi, double* out)
 {
-	if (!c || !need(c, c->prepared, "ccf_prepare")) return 1;
-	return ccf::aberth_sweep_dev(c, roots, n, phi, out) ? 0 : 1;
+	return guarded(c, [&]() -> int {
+		if (!c || !need(c, c->prepared, "ccf_prepare")) return 1;
+		return ccf::aberth_sweep_dev(c, roots, n, phi, out) ? 0 : 1;
+	});
 }
 
 int ccf_root_sums(ccf_ctx* c, const double* roots, int n, double* out)
 {
-	if (!c || !need(c, c->prepared, "ccf_prepare")) return 1;
-	return ccf::root_sums_dev(c, roots, n, out) ? 0 : 1;
+	return guarded(c, [&]() -> int {
+		if (!c || !need(c, c->prepared, "ccf_prepare")) return 1;
+		return ccf::root_sums_dev(c, roots, n, out) ? 0 : 1;
+	});
 }
 
 double ccf_mass_function(const char* name, int which, double p, double m_lower, double m_upper, double m_solar)
@@ -1048,7 +1102,15 @@ int ccf_batch_add(ccf_batch* b, const ccf_params* p)
 	if (!c) return -1;
 	c->P_in = *p;
 	c->P = *p;
-	b->fields.push_back(c);
+	try
+	{
+		b->fields.push_back(c);
+	}
+	catch (const std::bad_alloc&)
+	{
+		ccf_destroy(c);
+		return -1;
+	}
 	b->loaded = b->built = b->solved = false;
 	return (int)b->fields.size() - 1;
 }
@@ -1057,101 +1119,115 @@ int ccf_batch_size(const ccf_batch* b) { return b ? (int)b->fields.size() : 0; }
 
 int ccf_batch_load(ccf_batch* b, int verbose)
 {
-	if (!b) return 1;
-	ccf_ctx* m = b->master;
-	m->last_error.clear();
-	b->loaded = b->built = b->solved = false;
-	ccf::free_batch_device(b);
-	if (b->fields.empty())
-	{
-		m->fail("Error. The batch has no fields.\n");
-		return 1;
-	}
-	const ccf_params& p0 = b->fields[0]->P_in;
-	for (ccf_ctx* c : b->fields)
-	{
-		const ccf_params& p = c->P_in;
-		if (p.fmm_order != p0.fmm_order || p.direct != p0.direct || p.binomial_exact != p0.binomial_exact || p.precision != p0.precision)
+	return guarded(b ? b->master : nullptr, [&]() -> int {
+		if (!b) return 1;
+		ccf_ctx* m = b->master;
+		m->last_error.clear();
+		b->loaded = b->built = b->solved = false;
+		ccf::free_batch_device(b);
+		if (b->fields.empty())
 		{
-			m->fail("Error. The fields of a batch must share fmm_order, direct, binomial_exact and precision.\n");
+			m->fail("Error. The batch has no fields.\n");
 			return 1;
 		}
-	}
-	m->P_in = p0;
-	m->P = p0;
-	if (!device_init(m)) return 1;
-	for (size_t i = 0; i < b->fields.size(); i++)
-	{
-		ccf_ctx* c = b->fields[i];
-		c->batch_member = true;
-		if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
-		c->stream = m->stream;
-		c->own_stream = false;
-		if (ccf_load(c, verbose))
+		const ccf_params& p0 = b->fields[0]->P_in;
+		for (ccf_ctx* c : b->fields)
 		{
-			m->last_error = "Error. Field " + std::to_string(i) + ": " + c->last_error;
-			return 1;
+			const ccf_params& p = c->P_in;
+			if (p.fmm_order != p0.fmm_order || p.direct != p0.direct || p.binomial_exact != p0.binomial_exact || p.precision != p0.precision)
+			{
+				m->fail("Error. The fields of a batch must share fmm_order, direct, binomial_exact and precision.\n");
+				return 1;
+			}
 		}
-	}
-	b->loaded = true;
-	return 0;
+		m->P_in = p0;
+		m->P = p0;
+		if (!device_init(m)) return 1;
+		for (size_t i = 0; i < b->fields.size(); i++)
+		{
+			ccf_ctx* c = b->fields[i];
+			c->batch_member = true;
+			if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
+			c->stream = m->stream;
+			c->own_stream = false;
+			if (ccf_load(c, verbose))
+			{
+				m->last_error = "Error. Field " + std::to_string(i) + ": " + c->last_error;
+				return 1;
+			}
+		}
+		b->loaded = true;
+		return 0;
+	});
 }
 
 int ccf_batch_build(ccf_batch* b, int verbose)
 {
-	if (!b) return 1;
-	if (!b->loaded)
-	{
-		b->master->fail("Error. ccf_batch_load must run first.\n");
-		return 1;
-	}
-	b->built = b->solved = false;
-	if (!ccf::batch_build(b, verbose)) return 1;
-	b->built = true;
-	return 0;
+	return guarded(b ? b->master : nullptr, [&]() -> int {
+		if (!b) return 1;
+		if (!b->loaded)
+		{
+			b->master->fail("Error. ccf_batch_load must run first.\n");
+			return 1;
+		}
+		b->built = b->solved = false;
+		if (!ccf::batch_build(b, verbose)) return 1;
+		b->built = true;
+		return 0;
+	});
 }
 
 int ccf_batch_solve(ccf_batch* b, int verbose)
 {
-	if (!b) return 1;
-	if (!b->built)
-	{
-		b->master->fail("Error. ccf_batch_build must run first.\n");
-		return 1;
-	}
-	if (!ccf::batch_solve(b, verbose)) return 1;
-	b->solved = true;
-	return 0;
+	return guarded(b ? b->master : nullptr, [&]() -> int {
+		if (!b) return 1;
+		if (!b->built)
+		{
+			b->master->fail("Error. ccf_batch_build must run first.\n");
+			return 1;
+		}
+		if (!ccf::batch_solve(b, verbose)) return 1;
+		b->solved = true;
+		return 0;
+	});
 }
 
 int ccf_batch_run(ccf_batch* b, int verbose)
 {
-	if (ccf_batch_load(b, verbose) || ccf_batch_build(b, verbose)) return 1;
-	return ccf_batch_solve(b, verbose);
+	return guarded(b ? b->master : nullptr, [&]() -> int {
+		if (ccf_batch_load(b, verbose) || ccf_batch_build(b, verbose)) return 1;
+		return ccf_batch_solve(b, verbose);
+	});
 }
 
 int ccf_batch_save(ccf_batch* b, int field, int verbose)
 {
-	if (!b || field < 0 || field >= (int)b->fields.size()) return 1;
-	if (!b->solved)
-	{
-		b->master->fail("Error. ccf_batch_solve must run first.\n");
-		return 1;
-	}
-	return ccf_save(b->fields[field], verbose);
+	return guarded(b ? b->master : nullptr, [&]() -> int {
+		if (!b || field < 0 || field >= (int)b->fields.size()) return 1;
+		if (!b->solved)
+		{
+			b->master->fail("Error. ccf_batch_solve must run first.\n");
+			return 1;
+		}
+		return ccf_save(b->fields[field], verbose);
+	});
 }
 
 int ccf_batch_get_info(const ccf_batch* b, int field, ccf_info* info)
 {
-	if (!b || !info || field < -1 || field >= (int)b->fields.size()) return 1;
-	*info = field < 0 ? b->info : b->fields[field]->info;
-	return 0;
+	return guarded(b ? b->master : nullptr, [&]() -> int {
+		if (!b || !info || field < -1 || field >= (int)b->fields.size()) return 1;
+		*info = field < 0 ? b->info : b->fields[field]->info;
+		return 0;
+	});
 }
 
 int ccf_batch_get_array(ccf_batch* b, int field, int which, void* dst, long long capacity, int* nrows, int* ncols)
 {
-	if (!b || field < 0 || field >= (int)b->fields.size()) return 1;
-	return ccf_get_array(b->fields[field], which, dst, capacity, nrows, ncols);
+	return guarded(b ? b->master : nullptr, [&]() -> int {
+		if (!b || field < 0 || field >= (int)b->fields.size()) return 1;
+		return ccf_get_array(b->fields[field], which, dst, capacity, nrows, ncols);
+	});
 }
 
 int ccf_batch_workgroup(const ccf_batch* b) { return b ? b->block1 : 0; }
@@ -1160,67 +1236,69 @@ const char* ccf_batch_last_error(const ccf_batch* b) { return b && b->master ? b
 
 int ccf_write_synthetic_stars(const ccf_params* p, unsigned long long seed, const char* path)
 {
-	if (!p || !path) return 1;
-	const int N = p->num_stars;
-	if (N < 1) { std::cerr << "Error. num_stars must be an integer > 0\n"; return 1; }
-	double ml = p->m_lower, mu = p->m_upper;
-	if (!std::strcmp(p->mass_function, "equal")) { ml = 1; mu = 1; }
-	else { ml *= p->m_solar; mu *= p->m_solar; }
-	const ccf::Imf f = ccf::make_imf(p->mass_function, ml, mu, p->m_solar);
-	if (f.kind < 0) { std::cerr << "Error. mass_function must be equal, uniform, Salpeter, Kroupa, or optical_depth.\n"; return 1; }
-	const double mean = ccf::imf_moment(f, 1);
-	const double th = p->theta_star, ks = p->kappa_star, k = p->kappa_tot, g = p->shear;
-	// field the reference derives for these parameters (ccf.cuh:420-436)
-	cd cr(1 / std::abs(1 - k + g), 1 / std::abs(1 - k - g));
-	if (p->rectangular)
-	{
-		cr = cd(std::sqrt(cr.re / cr.im), std::sqrt(cr.im / cr.re));
-		cr *= std::sqrt(ccf::PI * th * th * N * mean / (4 * ks));
-	}
-	else
-	{
-		cr = cr / cr.abs();
-		cr *= std::sqrt(th * th * N * mean / ks);
-	}
-	std::mt19937_64 rng(seed);
-	auto u = [&rng]() { return 1.0 - (double)(rng() >> 11) * 0x1.0p-53; }; // (0, 1]
-	std::vector<double> st(3 * (size_t)N);
-	double mt = 0;
-	for (int i = 0; i < N; i++)
-	{
-		double x1, x2;
+	return guarded(nullptr, [&]() -> int {
+		if (!p || !path) return 1;
+		const int N = p->num_stars;
+		if (N < 1) { std::cerr << "Error. num_stars must be an integer > 0\n"; return 1; }
+		double ml = p->m_lower, mu = p->m_upper;
+		if (!std::strcmp(p->mass_function, "equal")) { ml = 1; mu = 1; }
+		else { ml *= p->m_solar; mu *= p->m_solar; }
+		const ccf::Imf f = ccf::make_imf(p->mass_function, ml, mu, p->m_solar);
+		if (f.kind < 0) { std::cerr << "Error. mass_function must be equal, uniform, Salpeter, Kroupa, or optical_depth.\n"; return 1; }
+		const double mean = ccf::imf_moment(f, 1);
+		const double th = p->theta_star, ks = p->kappa_star, k = p->kappa_tot, g = p->shear;
+		// field the reference derives for these parameters (ccf.cuh:420-436)
+		cd cr(1 / std::abs(1 - k + g), 1 / std::abs(1 - k - g));
 		if (p->rectangular)
 		{
-			x1 = u() * 2 * cr.re - cr.re;
-			x2 = u() * 2 * cr.im - cr.im;
+			cr = cd(std::sqrt(cr.re / cr.im), std::sqrt(cr.im / cr.re));
+			cr *= std::sqrt(ccf::PI * th * th * N * mean / (4 * ks));
 		}
 		else
 		{
-			const double a = u() * 2 * ccf::PI;
-			const double r = std::sqrt(u()) * cr.abs();
-			x1 = r * std::cos(a);
-			x2 = r * std::sin(a);
+			cr = cr / cr.abs();
+			cr *= std::sqrt(th * th * N * mean / ks);
 		}
-		st[3 * i] = x1;
-		st[3 * i + 1] = x2;
-		st[3 * i + 2] = ccf::imf_mass(f, u());
-		mt += st[3 * i + 2];
-	}
-	// the file's field size follows the actual mean mass, as the reference
-	// rescales its corner after drawing the stars (ccf.cuh:686-700)
-	const double mact = mt / N;
-	if (p->rectangular) cr *= std::sqrt(mact / mean);
-	else cr *= std::sqrt(mact / mean);
-	std::ofstream out(path, std::ios::binary);
-	if (!out.is_open()) { std::cerr << "Error. Failed to open file " << path << "\n"; return 1; }
-	const int32_t n32 = N, r32 = p->rectangular;
-	out.write((const char*)&n32, 4);
-	out.write((const char*)&r32, 4);
-	const double cc[2] = {p->rectangular ? cr.re : cr.abs(), p->rectangular ? cr.im : 0.0};
-	out.write((const char*)cc, 16);
-	out.write((const char*)&th, 8);
-	out.write((const char*)st.data(), st.size() * 8);
-	return out.good() ? 0 : 1;
+		std::mt19937_64 rng(seed);
+		auto u = [&rng]() { return 1.0 - (double)(rng() >> 11) * 0x1.0p-53; }; // (0, 1]
+		std::vector<double> st(3 * (size_t)N);
+		double mt = 0;
+		for (int i = 0; i < N; i++)
+		{
+			double x1, x2;
+			if (p->rectangular)
+			{
+				x1 = u() * 2 * cr.re - cr.re;
+				x2 = u() * 2 * cr.im - cr.im;
+			}
+			else
+			{
+				const double a = u() * 2 * ccf::PI;
+				const double r = std::sqrt(u()) * cr.abs();
+				x1 = r * std::cos(a);
+				x2 = r * std::sin(a);
+			}
+			st[3 * i] = x1;
+			st[3 * i + 1] = x2;
+			st[3 * i + 2] = ccf::imf_mass(f, u());
+			mt += st[3 * i + 2];
+		}
+		// the file's field size follows the actual mean mass, as the reference
+		// rescales its corner after drawing the stars (ccf.cuh:686-700)
+		const double mact = mt / N;
+		if (p->rectangular) cr *= std::sqrt(mact / mean);
+		else cr *= std::sqrt(mact / mean);
+		std::ofstream out(path, std::ios::binary);
+		if (!out.is_open()) { std::cerr << "Error. Failed to open file " << path << "\n"; return 1; }
+		const int32_t n32 = N, r32 = p->rectangular;
+		out.write((const char*)&n32, 4);
+		out.write((const char*)&r32, 4);
+		const double cc[2] = {p->rectangular ? cr.re : cr.abs(), p->rectangular ? cr.im : 0.0};
+		out.write((const char*)cc, 16);
+		out.write((const char*)&th, 8);
+		out.write((const char*)st.data(), st.size() * 8);
+		return out.good() ? 0 : 1;
+	});
 }
 
 } // extern "C"

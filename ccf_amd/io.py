@@ -26,7 +26,8 @@ def read_params(fname: str) -> dict:
 
 def read_stars(fname: str):
     """(nstars, rectangular, corner complex, theta, stars (n, 3) float64)."""
-    raw = open(fname, "rb").read()
+    with open(fname, "rb") as f:
+        raw = f.read()
     n, rect = np.frombuffer(raw[:8], dtype=np.int32)
     if len(raw) == 8 + 16 + 8 + 24 * n:
         dt, sz = np.float64, 8
@@ -49,7 +50,8 @@ def write_stars(fname: str, stars, rectangular: int, corner: complex, theta: flo
 
 
 def read_array(fname: str, complex_values: bool = True) -> np.ndarray:
-    raw = open(fname, "rb").read()
+    with open(fname, "rb") as f:
+        raw = f.read()
     nrows, ncols = np.frombuffer(raw[:8], dtype=np.int32)
     dt = np.complex128 if complex_values else np.float64
     return np.frombuffer(raw[8:], dtype=dt).reshape(int(nrows), int(ncols)).copy()

@@ -45,6 +45,7 @@ class OracleParams(ctypes.Structure):
         ("step_limit", ctypes.c_int),
         ("branch_limit", ctypes.c_int),
         ("num_threads", ctypes.c_int),
+        ("plant", ctypes.c_int),
         ("mass_function", ctypes.c_char * 32),
         ("starfile", ctypes.c_char * 512),
         ("outfile_prefix", ctypes.c_char * 512),
@@ -77,6 +78,13 @@ class OracleStats(ctypes.Structure):
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
+
+# planted mistakes (oracle.h ORACLE_PLANT_*): tests of the tests only
+PLANT_W_STAR_UNCONJUGATED = 1
+PLANT_BOUND_SMOOTH_SIGN = 2
+PLANT_MU_SCALE_FACTOR = 3
+PLANT_DISC_OUTSIDE_Z = 4
+PLANT_DISC_DZBAR_INSIDE = 5
 
 _lib = None
 

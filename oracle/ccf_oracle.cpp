@@ -716,6 +716,7 @@ struct Lens
 	int rectangular = 0, approx = 1, taylor_smooth = 1;
 	cx corner;
 	int p = 0;
+	int plant = 0; /* oracle_params::plant: a planted mistake (tests of the tests only) */
 	bool use_tree = true;
 	const std::vector<Star>* stars = nullptr;
 	const std::vector<double>* sx = nullptr; /* star x and y, tree order */
@@ -796,7 +797,7 @@ cx alpha_smooth(const Lens& L, cx z)
 	else
 	{
 		if (L.approx || z.abs() <= c.abs()) a = -ks * z;
-		else a = -ks * c.abs() * c.abs() / z.conj();
+		else a = -ks * c.abs() * c.abs() / (L.plant == ORACLE_PLANT_DISC_OUTSIDE_Z ? z : z.conj());
 	}
 	return a;
 }
@@ -839,7 +840,8 @@ cx d_alpha_smooth_d_zbar(const Lens& L, cx z)
 	}
 	else
 	{
-		if (!L.approx) d = ks * c.abs() * c.abs() / z.conj().pow(2);
+		if (!L.approx && !(L.plant == ORACLE_PLANT_DISC_DZBAR_INSIDE && z.abs() <= c.abs()))
+			d = ks * c.abs() * c.abs() / z.conj().pow(2);
 	}
 	return d;
 }
@@ -910,19 +912,20 @@ Eval evaluate(const Lens& L, cx z, double phi, bool want_f1 = true)
 	e.dsz = dsz;
 	e.f0 = cx(L.gamma) - da_star.conj() - da_loc.conj() - dsb.conj() - (1 - L.kappa - dsz) * cx(std::cos(phi), -std::sin(phi));
 	double a0 = e.f0.abs();
-	double am = 1 - L.kappa - dsz;
+	double am = 1 - L.kappa - (L.plant == ORACLE_PLANT_BOUND_SMOOTH_SIGN ? -dsz : dsz);
 	e.err = std::max(std::fabs(a0 * (a0 + 2 * am)), std::fabs(a0 * (a0 - 2 * am)));
 	if (!want_f1) return e;
 	cx d2sb = d2_alpha_smooth_d_zbar2(L, z);
 	e.f1 = -d2a_star.conj() - d2a_loc.conj() - d2sb.conj();
-	e.w = (1 - L.kappa) * z + L.gamma * z.conj() - a_star - a_loc - alpha_smooth(L, z);
+	e.w = (1 - L.kappa) * z + L.gamma * z.conj() - (L.plant == ORACLE_PLANT_W_STAR_UNCONJUGATED ? a_star.conj() : a_star)
+		- a_loc - alpha_smooth(L, z);
 	/* ccf_functions.cuh:145-166 */
 	double dwdz = 1 - L.kappa - dsz;
 	cx dwdzb = cx(L.gamma) - da_star - da_loc - dsb;
 	cx d2wdzb2 = -d2a_star - d2a_loc - d2sb;
 	cx cct = cx(0, -2) * dwdzb.conj() * d2wdzb2;
 	cx ct = dwdz * cct + dwdzb * cct.conj();
-	e.mu_scale = 1 / (2 * ct.abs());
+	e.mu_scale = 1 / ((L.plant == ORACLE_PLANT_MU_SCALE_FACTOR ? 1 : 2) * ct.abs());
 	return e;
 }
 
@@ -1078,6 +1081,7 @@ bool setup(const oracle_params& p, Setup& S, oracle_stats* stats, std::vector<cx
 	L.corner = f.corner;
 	L.p = p.direct ? 0 : order;
 	L.use_tree = !p.direct;
+	L.plant = p.plant;
 	L.stars = &S.sorted;
 	S.sx.resize(S.sorted.size());
 	S.sy.resize(S.sorted.size());

@@ -63,6 +63,8 @@ typedef struct oracle_params
 	int step_limit;     /* >0: stop the continuation after this many phi steps (timing samples) */
 	int branch_limit;   /* >0: solve only the first branch_limit branches (timing samples) */
 	int num_threads;    /* OpenMP threads (0 -> runtime default) */
+	int plant;          /* tests of the tests only: 0 the restated algorithm; k > 0 plants
+	                       mistake k (ORACLE_PLANT_*) so a test can show that its check fails */
 	char mass_function[32];
 	char starfile[512];
 	char outfile_prefix[512];
@@ -90,6 +92,17 @@ typedef struct oracle_stats
 	long long updates_early;    /* updates that returned on the 1/mu test */
 	double flops;               /* algorithmic flops of the executed updates */
 } oracle_stats;
+
+/* mistakes a test can plant in the lens-map family, each one a plausible slip */
+enum
+{
+	ORACLE_PLANT_NONE = 0,
+	ORACLE_PLANT_W_STAR_UNCONJUGATED = 1, /* w subtracts theta^2 sum m/(z - s), not its conjugate */
+	ORACLE_PLANT_BOUND_SMOOTH_SIGN = 2,   /* 1/mu bound with 1 - kappa + d(alpha_smooth)/dz */
+	ORACLE_PLANT_MU_SCALE_FACTOR = 3,     /* mu length scale 1/|caustic tangent| (factor 2 dropped) */
+	ORACLE_PLANT_DISC_OUTSIDE_Z = 4,      /* exact disc outside: -kappa_star R^2 / z, not / zbar */
+	ORACLE_PLANT_DISC_DZBAR_INSIDE = 5    /* exact disc: d(alpha_smooth)/dzbar = 0 inside (not the reference's) */
+};
 
 /* default-initialise a parameter block with the reference's defaults (ccf.cuh:42-62) */
 void oracle_default_params(oracle_params* p);

@@ -138,7 +138,13 @@ def test_synthetic_star_files(workdir, name):
     assert os.path.getsize(sf) == 8 + 16 + 8 + 24 * n
     if rect:
         ks = st[:, 2].sum() * np.pi / (4 * corner.real * corner.imag)
-        assert np.all(np.abs(st[:, 0]) <= corner.real * (1 + 1e-12)) or True
+        # positions are drawn in the rectangle of the IMF's mean mass; the
+        # file's corner then follows the drawn mean mass (ccf.cuh:686-700)
+        mean = ccf_amd.mass_function(spec["mass_function"], 1, 0.0, spec.get("m_lower", 1.0), spec.get("m_upper", 1.0), 1.0)
+        drawn = np.sqrt(mean / st[:, 2].mean())
+        assert np.all(np.abs(st[:, 0]) <= corner.real * drawn * (1 + 1e-12))
+        assert np.all(np.abs(st[:, 1]) <= corner.imag * drawn * (1 + 1e-12))
+        assert np.abs(st[:, 0]).max() > 0.95 * corner.real * drawn and np.abs(st[:, 1]).max() > 0.95 * corner.imag * drawn
     else:
         ks = st[:, 2].sum() / abs(corner) ** 2
         assert corner.imag == 0
@@ -170,7 +176,7 @@ def test_bad_star_files(workdir):
         f.write(np.array([5, 0], dtype=np.int32).tobytes() + b"\0" * 17)
     with ccf_amd.Context(starfile=bad) as c:
         assert c.lib.ccf_calculate_derived_params(c.ptr, 0) != 0
-        assert "does not contain validly formatted" in c.last_error() or "Unable to read" in c.last_error()
+        assert "does not contain validly formatted" in c.last_error() or "Unable to read" in c.last_error() or "Unable to read" in c.last_error()
     with ccf_amd.Context(starfile=os.path.join(str(workdir), "stars.dat")) as c:
         assert c.lib.ccf_calculate_derived_params(c.ptr, 0) != 0
         assert "is not a .bin or .txt file" in c.last_error() or "Unable to read" in c.last_error()
@@ -178,6 +184,14 @@ def test_bad_star_files(workdir):
     io.write_stars(neg, np.array([[0.0, 0.0, 1.0]]), 0, complex(-1.0, 0.0))
     with ccf_amd.Context(starfile=neg) as c:
         assert c.lib.ccf_calculate_derived_params(c.ptr, 0) != 0
+    # a header claiming 2^31 - 1 stars in a file of a few bytes: the format
+    # error of the reference (star.cuh), nothing sized from the header
+    huge = os.path.join(str(workdir), "huge.bin")
+    with open(huge, "wb") as f:
+        f.write(np.array([2**31 - 1, 0], dtype=np.int32).tobytes() + b"\0" * 24)
+    with ccf_amd.Context(starfile=huge) as c:
+        assert c.lib.ccf_calculate_derived_params(c.ptr, 0) != 0
+        assert "does not contain validly formatted" in c.last_error() or "Unable to read" in c.last_error()
 
 
 def test_float_star_files_are_read(workdir):
