@@ -12,8 +12,9 @@ converged critical-curve points per second.
 Prints one JSON line. The Aberth kernel's time comes from HIP events
 recorded on the stream it is launched on; its algorithmic flops from the
 per-update counts (10 per Cauchy term, 36 per near-field star, 16 per FMM
-order). The CPU baseline is the FP64 oracle with OpenMP on the host cores,
-timed on one branch over a few phi steps and scaled to the whole job.
+order). The CPU baseline is the FP64 oracle with OpenMP on the process's
+host cores, timed on whole branches (median of three runs) and scaled to the
+whole job; that leg also takes the accuracy gate against the oracle.
 """
 from __future__ import annotations
 
@@ -126,70 +127,104 @@ def relative_error(pairs, theta: float = 1.0) -> float:
     return max((abs(x - y) / max(abs(y), theta) for _, _, x, y in pairs), default=0.0)
 
 
-def polished_error(starfile: str, run_kw: dict, pairs):
-    """Over the differing pairs (a, b), with p = two FP64 Newton steps
-    z - f0/f1 on the GPU (the exact root of Witt's equation to rounding):
-    max |p(a) - p(b)| (the two forms converge to the same root), max |a - p(a)|
-    and max |b - p(b)| (how far each form stopped from that root; the
-    reference's 1/mu early exit bounds |f0|, not the distance)."""
-    if not pairs:
-        return 0.0, 0.0, 0.0
-    npp, nb = run_kw["num_phi"], run_kw["num_branches"]
-    worst = da = db = 0.0
-    with ccf_amd.Context(starfile=starfile, outfile_prefix="./", **run_kw) as ctx:
-        ctx.prepare()
-        by_col = {}
-        for c, k, x, y in pairs:
-            by_col.setdefault((c, k), []).append((x, y))
-        for (c, k), zz in by_col.items():
-            phi = 2 * np.pi * (c * npp // nb + k) / npp
-            z0 = np.array([p for pr in zz for p in pr], dtype=np.complex128)
-            z = z0
-            for _ in range(2):
-                v = ctx.eval_points(z, phi)
-                z = z - (v[:, 0] + 1j * v[:, 1]) / (v[:, 2] + 1j * v[:, 3])
-            worst = max(worst, float(np.abs(z[0::2] - z[1::2]).max()))
-            da = max(da, float(np.abs(z0[0::2] - z[0::2]).max()))
-            db = max(db, float(np.abs(z0[1::2] - z[1::2]).max()))
-    return worst, da, db
+def gate_summary(pairs_x, pairs_y, exact):
+    """distances of two forms' roots to the exact roots at the given points"""
+    x, y = np.asarray(pairs_x), np.asarray(pairs_y)
+    return dict(points=int(x.size), form_max_dist_to_exact_thetaE=float(np.abs(x - exact).max(initial=0.0)),
+                fp64_max_dist_to_exact_thetaE=float(np.abs(y - exact).max(initial=0.0)))
 
 
-def cpu_baseline(starfile: str, run_kw: dict, branches: int, threads: int, gpu_ccs: np.ndarray | None, R: int):
-    """Oracle (FP64, OpenMP) on the first `branches` branches, every phi step
-    (initial solve + the whole continuation of each), scaled to all branches;
-    also checks those rows against the GPU FP64 result."""
+def cpu_threads() -> dict:
+    """the host threads the CPU baseline uses: the process's CPU share
+    (OMP_NUM_THREADS, else the CPUs this process may run on), and what the
+    host reports besides"""
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return dict(threads=omp or affinity, omp_num_threads=omp or None, sched_affinity=affinity, os_cpu_count=os.cpu_count())
+
+
+def cpu_baseline(starfile: str, run_kw: dict, branches: int, threads: int, repeats: int, gpu_ccs: dict, R: int,
+                 job_pairs: dict):
+    """The CPU leg: the oracle (FP64, OpenMP) on the first `branches`
+    branches, every phi step (initial solve + the whole continuation of
+    each), timed `repeats` times (median) and scaled to all branches. As the
+    checker it also compares the GPU rows of those branches with the
+    oracle's, and gives the exact roots (the oracle's roots polished by FP64
+    Newton steps of the oracle's field) at which the accuracy gate is taken:
+      - every GPU form's roots on the sampled branches against the oracle's;
+      - the points of the whole job where the mixed and FP64 GPU forms differ
+        by more than 1e-9 theta_E (job_pairs)."""
     import oracle
+    from scipy.optimize import linear_sum_assignment
 
     nb, npp = run_kw["num_branches"], run_kw["num_phi"]
     branches = min(branches, nb)
+    walls, stats = [], []
     with tempfile.TemporaryDirectory() as d:
-        p = oracle.params(starfile=starfile, outfile_prefix=d + "/", branch_limit=branches,
-                          num_threads=threads, write_stars=0, write_caustics=0, **run_kw)
-        t0 = time.perf_counter()
-        s = oracle.run(p)
-        wall = time.perf_counter() - t0
+        for i in range(repeats):
+            p = oracle.params(starfile=starfile, outfile_prefix=d + "/", branch_limit=branches, compact_output=1,
+                              num_threads=threads, write_stars=0, write_caustics=0, **run_kw)
+            t0 = time.perf_counter()
+            st = oracle.run(p)
+            walls.append(time.perf_counter() - t0)
+            stats.append(st)
+            print(f"# cpu baseline run {i}: {walls[-1]:.1f} s", file=sys.stderr, flush=True)
         occs = io.read_array(d + "/ccf_ccs.bin")
+    order = np.argsort([s_["t_init_roots"] + s_["t_ccs"] for s_ in stats])
+    s = stats[order[len(order) // 2]]
     t_job = (s["t_init_roots"] + s["t_ccs"]) * nb / branches
     points = R * (npp + nb)
     res = dict(value=points / t_job, unit="points/s", cores=threads, kind="port",
-               sample=f"FP64 oracle, branches 0..{branches - 1} of {nb}, every phi step ({s['points_solved']} points, "
-                      f"{wall:.1f} s wall); job time = {nb}/{branches} x (t_init + t_ccs)",
-               sample_seconds=wall, t_init=s["t_init_roots"], t_ccs=s["t_ccs"])
-    if gpu_ccs is not None:
-        ca = io.columns_by_phi(gpu_ccs, R, nb)[:branches]
-        cb = io.columns_by_phi(occs, R, nb)[:branches]
-        worst = 0.0
-        from scipy.optimize import linear_sum_assignment
-        for c in range(ca.shape[0]):
-            for k in range(ca.shape[1]):
+               sample=f"FP64 oracle, branches 0..{branches - 1} of {nb}, every phi step ({s['points_solved']} points), "
+                      f"median of {repeats} runs ({', '.join(f'{w:.1f}' for w in walls)} s wall); "
+                      f"job time = {nb}/{branches} x (t_init + t_ccs)",
+               sample_seconds=float(np.median(walls)), t_init=s["t_init_roots"], t_ccs=s["t_ccs"], repeats=repeats)
+    okw = {k: v for k, v in run_kw.items() if k not in ("num_phi", "num_branches")}
+    p = oracle.params(starfile=starfile, **okw)
+    cols = npp // nb + 1
+
+    def phi(c, k):
+        return 2 * np.pi * (c * npp // nb + k) / npp
+
+    gate = {}
+    for name, arr in gpu_ccs.items():
+        ca = io.columns_by_phi(arr, R, nb)[:branches]
+        cb = io.columns_by_phi(occs, R, branches)
+        worst, xs, ys, phs = 0.0, [], [], []
+        for c in range(branches):
+            for k in range(cols):
                 dd = np.abs(ca[c, k] - cb[c, k])
-                idx = np.nonzero(dd > 1e-9 * np.maximum(np.abs(cb[c, k]), 1))[0]
+                # absolute 1e-9 theta_E: the gate's unit (the parity bar's
+                # relative form is the smaller test)
+                idx = np.nonzero(dd > 1e-9)[0]
                 if idx.size:
                     cost = np.abs(ca[c, k][idx][:, None] - cb[c, k][idx][None, :])
                     r, cc = linear_sum_assignment(cost)
                     dd[idx[r]] = cost[r, cc]
+                    for i, j in zip(r, cc):
+                        if cost[i, j] > 1e-9:
+                            xs.append(ca[c, k][idx[i]])
+                            ys.append(cb[c, k][idx[j]])
+                            phs.append(phi(c, k))
                 worst = max(worst, float((dd / np.maximum(np.abs(cb[c, k]), 1)).max()))
-        res["oracle_max_rel_diff"] = worst
+        g = dict(max_rel_diff_vs_oracle=worst, points_differing_over_1e9_thetaE=len(xs))
+        if xs:
+            exact = oracle.newton_polish(p, ys, phs)
+            g.update(gate_summary(xs, ys, exact))
+        gate[name] = g
+    res["oracle_max_rel_diff"] = gate.get("fp64", {}).get("max_rel_diff_vs_oracle")
+    res["sampled_branches_vs_oracle"] = gate
+    if job_pairs:
+        # whole job: where the mixed form differs from the FP64 form, both
+        # against the exact root
+        xs = [x for _, _, x, _ in job_pairs["pairs"]]
+        ys = [y for _, _, _, y in job_pairs["pairs"]]
+        phs = [phi(c, k) for c, k, _, _ in job_pairs["pairs"]]
+        res["job_mixed_vs_fp64_differing_points"] = gate_summary(xs, ys, oracle.newton_polish(p, ys, phs)) if xs else \
+            dict(points=0, form_max_dist_to_exact_thetaE=0.0, fp64_max_dist_to_exact_thetaE=0.0)
     return res
 
 
@@ -308,26 +343,18 @@ def bench_batch(args):
     fields = batch_fields(tmp)
     fp64, ccs64 = run_batch(fields, 0, 0, args.steps, args.warmup)
     mixed, ccsm = run_batch(fields, 1, args.predictor, args.steps, args.warmup)
-    # the mixed batch against the FP64 batch, field by field; where they
-    # differ, each form's distance to the exact root (two FP64 Newton steps
-    # on the field at the batch's expansion order)
-    worst = rel = pol = da = db = 0.0
+    # the mixed batch against the FP64 batch, field by field (the gate
+    # against the oracle's exact roots is tests/test_gpu_configs.py's)
+    worst = rel = 0.0
     ndiff = 0
-    order = max(mixed["expansion_orders"])
     for (path, kw, n), a, b in zip(fields, ccsm, ccs64):
         err, pairs = set_error(a, b, 2 * n, BATCH_RUN["num_branches"])
         worst = max(worst, err)
         rel = max(rel, relative_error(pairs))
         ndiff += len(pairs)
-        if pairs:
-            p, x, y = polished_error(path, dict(kw, fmm_order=order, **BATCH_RUN), pairs[:200])
-            pol, da, db = max(pol, p), max(da, x), max(db, y)
     mixed["max_root_err_vs_fp64_thetaE"] = worst
     mixed["max_root_err_vs_fp64_rel"] = rel
     mixed["points_differing_over_1e-9"] = ndiff
-    mixed["max_root_err_vs_fp64_after_newton_polish_thetaE"] = pol
-    mixed["differing_points_max_dist_to_exact_root_thetaE"] = da
-    mixed["differing_points_fp64_max_dist_to_exact_root_thetaE"] = db
     seq = run_sequential(fields, min(args.seq_fields, len(fields)))
     line = {
         "metric": "converged cc points/s over 256 uneven fields (batched form) on 1 MI355X",
@@ -363,15 +390,17 @@ def main():
     ap.add_argument("--seq-fields", type=int, default=64,
                     help="batch256: fields of the sample timed one by one on the single-field FP64 path")
     ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"])
-    ap.add_argument("--cpu-branches", type=int, default=6, help="branches in the CPU sample, all phi steps (0: skip)")
+    ap.add_argument("--cpu-branches", type=int, default=2, help="branches in the CPU sample, all phi steps (0: skip)")
     ap.add_argument("--cpu-steps", type=int, default=None, help="0: skip the CPU sample (same as --cpu-branches 0)")
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: the process's CPU share (OMP_NUM_THREADS or affinity)")
+    ap.add_argument("--cpu-repeats", type=int, default=3, help="oracle runs of the CPU sample (the median is reported)")
+    ap.add_argument("--ablation-steps", type=int, default=2,
+                    help="timed steps of the default ablation (FP64 + predictor, mixed with direct root sums; 0: none)")
     ap.add_argument("--predictor", type=int, default=4, help="continuation predictor of the mixed form (0..4)")
     ap.add_argument("--root-fmm", type=int, default=1, help="root sums of dense sweeps by the FMM in the mixed form")
     ap.add_argument("--root-fmm-order", type=int, default=0, help="its expansion order (0: the library default)")
     ap.add_argument("--root-fmm-leaf", type=int, default=0, help="its mean roots per leaf cell (0: the library default)")
-    ap.add_argument("--ablation", action="store_true",
-                    help="also time FP64 + predictor, mixed without predictor, mixed with direct root sums")
+    ap.add_argument("--ablation", action="store_true", help="also time the mixed form without the predictor")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device: the known-size kernel that scales the HBM byte counters")
     args = ap.parse_args()
@@ -410,16 +439,22 @@ def main():
     mixed = ("mixed", 1, args.predictor, args.root_fmm)
     modes = {"fp64": [("fp64", 0, 0, 0)], "mixed": [mixed],
              "both": [("fp64", 0, 0, 0), mixed]}[args.precision]
-    if args.ablation:
-        modes += [("fp64_predictor", 0, args.predictor, 0), ("mixed_no_predictor", 1, 0, args.root_fmm),
-                  ("mixed_direct_root_sums", 1, args.predictor, 0)]
-    for name, prec, pred, rfmm in modes:
+    # the ablation: what the FP64 form gains from the predictor alone, and
+    # what the mixed form loses without the root-sum FMM (short runs)
+    ablation = []
+    if args.ablation_steps > 0 and args.precision == "both":
+        ablation = [("fp64_predictor", 0, args.predictor, 0), ("mixed_direct_root_sums", 1, args.predictor, 0)]
+        if args.ablation:
+            ablation.append(("mixed_no_predictor", 1, 0, args.root_fmm))
+    for name, prec, pred, rfmm in modes + ablation:
         fmm_kw = dict(root_fmm_order=args.root_fmm_order, root_fmm_leaf=args.root_fmm_leaf)
-        times, infos, ccs[name] = run_gpu(sf, run_kw, prec, args.steps, args.warmup, pred, rfmm, fmm_kw)
+        steps, warm = (args.steps, args.warmup) if (name, prec, pred, rfmm) in modes else (args.ablation_steps, 1)
+        times, infos, ccs[name] = run_gpu(sf, run_kw, prec, steps, warm, pred, rfmm, fmm_kw)
         res[name] = summarize(times, infos)
         res[name]["predictor"] = pred
         res[name]["root_fmm"] = rfmm
     R = res[next(iter(res))]["num_roots"]
+    job_pairs = None
     if "fp64" in ccs:
         for name in res:
             if name == "fp64":
@@ -428,11 +463,8 @@ def main():
             res[name]["max_root_err_vs_fp64_thetaE"] = err
             res[name]["max_root_err_vs_fp64_rel"] = relative_error(pairs)
             res[name]["points_differing_over_1e-9"] = len(pairs)
-            pol, da, db = polished_error(sf, run_kw, pairs[:2000])
-            res[name]["max_root_err_vs_fp64_after_newton_polish_thetaE"] = pol
-            # over the same pairs: each form's distance to the exact root
-            res[name]["differing_points_max_dist_to_exact_root_thetaE"] = da
-            res[name]["differing_points_fp64_max_dist_to_exact_root_thetaE"] = db
+            if name == "mixed":
+                job_pairs = dict(pairs=pairs)
             res[name]["speedup_vs_fp64"] = res[name]["points_per_s"] / res["fp64"]["points_per_s"]
 
     head = res["mixed"] if "mixed" in res else res["fp64"]
@@ -459,9 +491,10 @@ def main():
         regs.append(region(head, "tail"))
     dom = max(regs, key=lambda g: g["total_ms"])
     pk = PEAK_FP64_TFLOPS
-    roof = {"bound": "mfma",
-            "pipe": "FP64 VALU (v_fma_f64, v_rcp_f64): MI355X FP64 vector peak = FP64 matrix peak; the mixed form's "
-                    "FP32 parts (sparse-sweep root sums, far pole field) issue at the same non-packed rate",
+    roof = {"bound": "valu",
+            "pipe": "FP64 VALU (v_fma_f64, v_rcp_f64): dense reciprocal sums and tree gathers with no contraction, so "
+                    "no MFMA work; MI355X FP64 vector peak = FP64 matrix peak; the mixed form's FP32 parts "
+                    "(sparse-sweep root sums, far pole field) issue at the same non-packed rate",
             "kernel": dom["kernel"], "achieved": dom["achieved"], "peak": pk, "unit": "TFLOP/s",
             "frac": dom["achieved"] / pk,
             "traffic": measured_traffic(args.workload, dom["kernel"]),
@@ -476,9 +509,25 @@ def main():
                                "traffic": measured_traffic(args.workload, "k_sums64")}
 
     cpu = None
+    gate = None
     if args.cpu_branches > 0:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(sf, run_kw, args.cpu_branches, threads, ccs.get("fp64"), R)
+        host = cpu_threads()
+        threads = args.cpu_threads or host["threads"]
+        cpu = cpu_baseline(sf, run_kw, args.cpu_branches, threads, args.cpu_repeats,
+                           {k: v for k, v in ccs.items() if k in ("fp64", "mixed")}, R, job_pairs)
+        cpu["host"] = host
+        m = cpu["sampled_branches_vs_oracle"].get("mixed")
+        if m is not None:
+            # north-star accuracy gate: max root error <= 1e-9 theta_E against
+            # the FP64 oracle's exact roots, on the sampled branches and at
+            # every point of the job where the mixed and FP64 forms differ
+            jm = cpu.get("job_mixed_vs_fp64_differing_points") or {}
+            err = max(m.get("form_max_dist_to_exact_thetaE", 0.0), jm.get("form_max_dist_to_exact_thetaE", 0.0))
+            gate = {"gate_max_root_err_le_1e-9_thetaE": bool(err <= 1e-9), "max_root_err_vs_exact_thetaE": err,
+                    "sampled_branches": args.cpu_branches,
+                    "mixed_points_differing_from_oracle_over_1e-9": m["points_differing_over_1e9_thetaE"],
+                    "mixed_max_rel_diff_vs_oracle": m["max_rel_diff_vs_oracle"],
+                    "oracle_max_dist_to_exact_thetaE": m.get("fp64_max_dist_to_exact_thetaE", 0.0)}
 
     line = {
         "metric": METRIC,
@@ -500,6 +549,7 @@ def main():
                    "num_roots": R, "form": "mixed" if "mixed" in res else "fp64"},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "accuracy_gate": gate,
         "fp64": res.get("fp64"),
         "mixed": res.get("mixed"),
         "ablation": {k: v for k, v in res.items() if k not in ("fp64", "mixed")} or None,

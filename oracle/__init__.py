@@ -45,6 +45,7 @@ class OracleParams(ctypes.Structure):
         ("step_limit", ctypes.c_int),
         ("branch_limit", ctypes.c_int),
         ("num_threads", ctypes.c_int),
+        ("compact_output", ctypes.c_int),
         ("plant", ctypes.c_int),
         ("mass_function", ctypes.c_char * 32),
         ("starfile", ctypes.c_char * 512),
@@ -73,6 +74,7 @@ class OracleStats(ctypes.Structure):
         ("updates_full", ctypes.c_longlong),
         ("updates_early", ctypes.c_longlong),
         ("flops", ctypes.c_double),
+        ("max_witt_residual", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -108,6 +110,8 @@ def load():
         lib.oracle_mass_function.restype = ctypes.c_double
         lib.oracle_eval.argtypes = [P, ctypes.c_int, dp, ctypes.c_int, ctypes.c_double, dp, S]
         lib.oracle_eval.restype = ctypes.c_int
+        lib.oracle_eval_phis.argtypes = [P, ctypes.c_int, dp, dp, ctypes.c_int, dp]
+        lib.oracle_eval_phis.restype = ctypes.c_int
         lib.oracle_aberth_sweep.argtypes = [P, dp, ctypes.c_int, ctypes.c_double, dp]
         lib.oracle_aberth_sweep.restype = ctypes.c_int
         _lib = lib
@@ -154,6 +158,26 @@ def evaluate(p: OracleParams, z, phi: float, use_tree: bool = True):
     if rc != 0:
         raise RuntimeError("oracle_eval failed")
     return out, s.as_dict()
+
+
+def evaluate_phis(p: OracleParams, z, phis, use_tree: bool = True) -> np.ndarray:
+    """as evaluate, with one angle per point: values (n, 8)"""
+    z = np.ascontiguousarray(np.asarray(z, dtype=np.complex128).ravel())
+    ph = np.ascontiguousarray(np.broadcast_to(np.asarray(phis, dtype=np.float64), z.shape))
+    out = np.empty((z.size, 8))
+    if load().oracle_eval_phis(ctypes.byref(p), int(use_tree), _dp(z.view(np.float64)), _dp(ph), z.size, _dp(out)) != 0:
+        raise RuntimeError("oracle_eval_phis failed")
+    return out
+
+
+def newton_polish(p: OracleParams, z, phis, steps: int = 3) -> np.ndarray:
+    """the roots of Witt's function nearest z (at angles phis), by FP64
+    Newton steps z - f0/f1 of the oracle's field: the exact root to rounding"""
+    z = np.asarray(z, dtype=np.complex128).ravel().copy()
+    for _ in range(steps):
+        v = evaluate_phis(p, z, phis)
+        z -= (v[:, 0] + 1j * v[:, 1]) / (v[:, 2] + 1j * v[:, 3])
+    return z
 
 
 def aberth_sweep(p: OracleParams, roots, phi: float) -> np.ndarray:

@@ -1212,7 +1212,7 @@ int run(const oracle_params& p, oracle_stats* stats)
 	}
 	auto t2 = std::chrono::steady_clock::now();
 	/* 1/mu error bound at every solved point (ccf_functions.cuh:419-467) */
-	double max_err = 0;
+	double max_err = 0, max_res = 0;
 	bool nan = false;
 	for (int c = 0; c < nb; c++)
 	{
@@ -1223,16 +1223,22 @@ int run(const oracle_params& p, oracle_stats* stats)
 			int r = center + k;
 			if (!solved[r]) continue;
 			double phi = phi0 + 2 * PI / np * k;
-			double lmax = 0;
+			double lmax = 0, lres = 0;
 			bool lnan = false;
-#pragma omp parallel for reduction(max : lmax) reduction(|| : lnan)
+#pragma omp parallel for reduction(max : lmax, lres) reduction(|| : lnan)
 			for (int a = 0; a < R; a++)
 			{
-				double e = evaluate(L, ccs_init[(size_t)r * R + a], phi, false).err;
+				const Eval ev = evaluate(L, ccs_init[(size_t)r * R + a], phi, false);
+				const double e = ev.err;
 				if (!std::isfinite(e)) lnan = true;
-				else lmax = std::max(lmax, e);
+				else
+				{
+					lmax = std::max(lmax, e);
+					lres = std::max(lres, ev.f0.abs());
+				}
 			}
 			max_err = std::max(max_err, lmax);
+			max_res = std::max(max_res, lres);
 			nan = nan || lnan;
 		}
 	}
@@ -1241,10 +1247,19 @@ int run(const oracle_params& p, oracle_stats* stats)
 		std::cerr << "Error. Errors in 1/mu contain values which are not positive real numbers.\n";
 		return 1;
 	}
-	/* transpose [row][root] -> [root][row] (ccf.cuh:1072-1080) */
-	std::vector<cx> ccs((size_t)rows * R);
-	for (int r = 0; r < rows; r++)
-		for (int a = 0; a < R; a++) ccs[(size_t)a * rows + r] = ccs_init[(size_t)r * R + a];
+	/* transpose [row][root] -> [root][row] (ccf.cuh:1072-1080); with
+	   compact output only the first branch_limit branches' rows, in the same
+	   layout for that many branches */
+	const int nb_out = (p.compact_output && p.branch_limit > 0) ? std::min(nb, p.branch_limit) : nb;
+	const int cols = np / nb + 1, rows_out = nb_out * cols;
+	std::vector<cx> ccs((size_t)rows_out * R);
+	for (int c = 0; c < nb_out; c++)
+		for (int k = 0; k < cols; k++)
+		{
+			const int r = c * (np / nb + 1) + k;
+#pragma omp parallel for schedule(static, 4096)
+			for (int a = 0; a < R; a++) ccs[((size_t)a * nb_out + c) * cols + k] = ccs_init[(size_t)r * R + a];
+		}
 	/* caustics and length scales (ccf.cuh:1085-1121) */
 	auto t3 = std::chrono::steady_clock::now();
 	std::vector<cx> caus;
@@ -1264,6 +1279,7 @@ int run(const oracle_params& p, oracle_stats* stats)
 	auto t4 = std::chrono::steady_clock::now();
 
 	stats->max_error = max_err;
+	stats->max_witt_residual = max_res;
 	stats->t_init_roots = std::chrono::duration<double>(t1 - t0).count();
 	stats->t_ccs = std::chrono::duration<double>(t2 - t1).count();
 	stats->t_caustics = std::chrono::duration<double>(t4 - t3).count();
@@ -1314,9 +1330,9 @@ int run(const oracle_params& p, oracle_stats* stats)
 		out << "t_caustics " << stats->t_caustics << "\n";
 	}
 	if (p.write_stars && !write_star_file(pre + "ccf_stars.bin", f, S.sorted)) return 1;
-	if (p.write_critical_curves && !write_array(pre + "ccf_ccs.bin", ccs.data(), R * nb, np / nb + 1)) return 1;
-	if (p.write_caustics && !write_array(pre + "ccf_caustics.bin", caus.data(), R * nb, np / nb + 1)) return 1;
-	if (p.write_mu_length_scales && !write_array(pre + "ccf_mu_length_scales.bin", mus.data(), R * nb, np / nb + 1)) return 1;
+	if (p.write_critical_curves && !write_array(pre + "ccf_ccs.bin", ccs.data(), R * nb_out, cols)) return 1;
+	if (p.write_caustics && !write_array(pre + "ccf_caustics.bin", caus.data(), R * nb_out, cols)) return 1;
+	if (p.write_mu_length_scales && !write_array(pre + "ccf_mu_length_scales.bin", mus.data(), R * nb_out, cols)) return 1;
 	return 0;
 }
 
@@ -1384,6 +1400,26 @@ int oracle_eval(const oracle_params* p, int use_tree, const double* z, int n, do
 	for (int i = 0; i < n; i++)
 	{
 		orc::Eval e = orc::evaluate(S.lens, orc::cx(z[2 * i], z[2 * i + 1]), phi, true);
+		double* o = out + 8 * (size_t)i;
+		o[0] = e.f0.re; o[1] = e.f0.im;
+		o[2] = e.f1.re; o[3] = e.f1.im;
+		o[4] = e.w.re; o[5] = e.w.im;
+		o[6] = e.mu_scale;
+		o[7] = e.err;
+	}
+	return 0;
+}
+
+int oracle_eval_phis(const oracle_params* p, int use_tree, const double* z, const double* phis, int n, double* out)
+{
+	orc::Setup S;
+	oracle_params q = *p;
+	if (!use_tree) q.direct = 1;
+	if (!orc::setup(q, S, nullptr, nullptr)) return 1;
+#pragma omp parallel for schedule(static, 64)
+	for (int i = 0; i < n; i++)
+	{
+		orc::Eval e = orc::evaluate(S.lens, orc::cx(z[2 * i], z[2 * i + 1]), phis[i], true);
 		double* o = out + 8 * (size_t)i;
 		o[0] = e.f0.re; o[1] = e.f0.im;
 		o[2] = e.f1.re; o[3] = e.f1.im;

@@ -63,6 +63,8 @@ typedef struct oracle_params
 	int step_limit;     /* >0: stop the continuation after this many phi steps (timing samples) */
 	int branch_limit;   /* >0: solve only the first branch_limit branches (timing samples) */
 	int num_threads;    /* OpenMP threads (0 -> runtime default) */
+	int compact_output; /* 1 with branch_limit > 0: the output arrays hold only the solved branches
+	                       (the file layout for num_branches = branch_limit) */
 	int plant;          /* tests of the tests only: 0 the restated algorithm; k > 0 plants
 	                       mistake k (ORACLE_PLANT_*) so a test can show that its check fails */
 	char mass_function[32];
@@ -91,6 +93,7 @@ typedef struct oracle_stats
 	long long updates_full;     /* executed full Aberth updates */
 	long long updates_early;    /* updates that returned on the 1/mu test */
 	double flops;               /* algorithmic flops of the executed updates */
+	double max_witt_residual;   /* max |f0| over the solved points */
 } oracle_stats;
 
 /* mistakes a test can plant in the lens-map family, each one a plausible slip */
@@ -128,6 +131,10 @@ double oracle_mass_function(const char* name, int which, double p, double m_lowe
      out[8*i+7]    1/mu error bound.
    use_tree=0 sums every star directly (no FMM) for the same field. */
 int oracle_eval(const oracle_params* p, int use_tree, const double* z, int n, double phi, double* out, oracle_stats* stats);
+
+/* as oracle_eval, with its own angle phis[i] for each point (one tree build
+   for points of many phi columns) */
+int oracle_eval_phis(const oracle_params* p, int use_tree, const double* z, const double* phis, int n, double* out);
 
 /* one Aberth update of every root in roots[0..n) (Jacobi), writing the
    updated roots to out; uses the same field/tree as oracle_eval. */

@@ -76,12 +76,65 @@ def compare_arrays(x: np.ndarray, y: np.ndarray, num_roots: int, num_branches: i
     return worst, nbad
 
 
+def phi_of(c: int, k: int, num_phi: int, num_branches: int) -> float:
+    """angle of column k of branch c (the file layout of ccf.cuh:1072-1080)"""
+    return 2 * np.pi * (c * num_phi // num_branches + k) / num_phi
+
+
 def newton_step(starfile: str, points, num_phi: int, num_branches: int, **kw) -> np.ndarray:
     """|f0 / f1| of the oracle at (branch, column, z) points: how far a root
     accepted by the reference's convergence tests can sit from the exact root."""
-    out = np.empty(len(points))
-    for i, (c, k, z, _) in enumerate(points):
-        phi = 2 * np.pi * (c * num_phi // num_branches + k) / num_phi
-        v, _ = oracle.evaluate(oracle.params(starfile=starfile, num_phi=num_phi, num_branches=num_branches, **kw), [z], phi)
-        out[i] = abs(complex(v[0, 0], v[0, 1]) / complex(v[0, 2], v[0, 3]))
+    if not len(points):
+        return np.empty(0)
+    z = np.array([p[2] for p in points], dtype=np.complex128)
+    ph = np.array([phi_of(c, k, num_phi, num_branches) for c, k, _, _ in points])
+    v = oracle.evaluate_phis(oracle.params(starfile=starfile, num_phi=num_phi, num_branches=num_branches, **kw), z, ph)
+    return np.abs((v[:, 0] + 1j * v[:, 1]) / (v[:, 2] + 1j * v[:, 3]))
+
+
+def compare_param_files(gpath: str, opath: str):
+    """The product's parameter file against the oracle's (ccf.cuh:1135-1192):
+    the same keys in the same order and the same printed values, except the
+    timings (t_*) and max_error_1/mu, returned as (product, oracle) for a
+    numeric comparison."""
+    def lines(p):
+        with open(p) as f:
+            return [ln.split() for ln in f if ln.strip()]
+
+    a, b = lines(gpath), lines(opath)
+    assert [k for k, _ in a] == [k for k, _ in b], ([k for k, _ in a], [k for k, _ in b])
+    err = None
+    for (k, va), (_, vb) in zip(a, b):
+        if k.startswith("t_"):
+            continue
+        if k == "max_error_1/mu":
+            err = (float(va), float(vb))
+            continue
+        assert va == vb, (k, va, vb)
+    return err
+
+
+def differing_pairs(x: np.ndarray, y: np.ndarray, num_roots: int, num_branches: int, rel: float = 1e-9):
+    """Matched pairs (branch, column, x root, y root) of two root arrays whose
+    distance exceeds rel * max(|y|, 1) after set matching per phi column
+    (index match, then Hungarian assignment on the entries that disagree)."""
+    cx = io.columns_by_phi(x, num_roots, num_branches)
+    cy = io.columns_by_phi(y, num_roots, num_branches)
+    out = []
+    d = np.abs(cx - cy)
+    for c, k in np.argwhere((d / np.maximum(np.abs(cy), 1.0)).max(axis=2) > rel):
+        a, b = cx[c, k], cy[c, k]
+        bad = np.nonzero(np.abs(a - b) > rel * np.maximum(np.abs(b), 1.0))[0]
+        cost = np.abs(a[bad][:, None] - b[bad][None, :])
+        r, s = linear_sum_assignment(cost)
+        for i, j in zip(r, s):
+            if cost[i, j] > rel * max(abs(b[bad[j]]), 1.0):
+                out.append((int(c), int(k), a[bad[i]], b[bad[j]]))
     return out
+
+
+def branch_slice(arr: np.ndarray, num_roots: int, num_branches: int, branches: int) -> np.ndarray:
+    """the rows of the first `branches` branches of a file-layout array, in
+    the layout of a run with that many branches"""
+    cols = arr.shape[1]
+    return arr.reshape(num_roots, num_branches, cols)[:, :branches, :].reshape(num_roots * branches, cols)

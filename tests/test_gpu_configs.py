@@ -1,0 +1,237 @@
+"""The product at the benchmark workloads' own sizes, against the FP64 oracle.
+
+- 10,000 stars (Salpeter on [0.08, 1], kappa = kappa_star = 0.5, shear 0.2,
+  num_phi 2000, num_branches 20, FMM order 24): the whole run on the device,
+  FP64 and mixed (root-sum FMM, degree-4 predictor), against the oracle's
+  solve of branch 0 (its initial roots and both continuations, 2.02 M points).
+- 50,000 stars (Kroupa, rectangular field, kappa_star 0.35, kappa 0.7, shear
+  0.25, num_phi 2000, num_branches 100, FMM order 28): the mixed form, branch 0.
+- 256 uneven fields (num_phi 504, num_branches 4) in the batched form: the
+  oracle on 8 fields spread over the batch, every field against the
+  single-field path, and properties of all of them.
+
+Parity: the FP64 path within 1e-9 relative of the oracle at every point.
+The mixed form may stop elsewhere inside the reference's convergence band
+(within twice the oracle's own Newton step |f0/f1| of its root), and every
+mixed root that differs from the oracle's by more than 1e-9 theta_E lies
+within 1e-9 theta_E of the exact root (the oracle's root polished by FP64
+Newton steps of the oracle's field): the oracle itself stops up to about
+2e-9 theta_E from that root where the 1/mu early exit ends it
+(ccf_functions.cuh:207-211).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import ccf_amd
+import oracle
+from ccf_amd import io
+from helpers import branch_slice, compare_arrays, differing_pairs, newton_step, phi_of
+
+pytestmark = pytest.mark.gpu
+
+WORKLOADS = {
+    "cc10k": dict(stars=dict(num_stars=10000, kappa_tot=0.5, kappa_star=0.5, shear=0.2, mass_function="salpeter",
+                             m_lower=0.08, m_upper=1.0), seed=2,
+                  run=dict(kappa_tot=0.5, shear=0.2, num_phi=2000, num_branches=20, fmm_order=24)),
+    "cc50k": dict(stars=dict(num_stars=50000, kappa_tot=0.7, kappa_star=0.35, shear=0.25, mass_function="kroupa",
+                             m_lower=0.08, m_upper=1.0, rectangular=1), seed=3,
+                  run=dict(kappa_tot=0.7, shear=0.25, num_phi=2000, num_branches=100, fmm_order=28)),
+}
+MIXED = dict(precision=1, predictor=4, root_fmm=1)
+_oracle_cache = {}
+
+
+def _field(workdir, name):
+    W = WORKLOADS[name]
+    sf = os.path.join(str(workdir), f"{name}.bin")
+    if not os.path.exists(sf):
+        ccf_amd.write_synthetic_stars(sf, W["seed"], **W["stars"])
+    return sf, W["run"]
+
+
+def _oracle_branch0(workdir, name):
+    """the oracle's solve of branch 0 (compact files: that branch only)"""
+    if name not in _oracle_cache:
+        sf, run = _field(workdir, name)
+        pre = os.path.join(str(workdir), f"{name}_o_")
+        print(f"# oracle {name}: branch 0", flush=True)
+        st = oracle.run(oracle.params(starfile=sf, outfile_prefix=pre, branch_limit=1, compact_output=1, write_stars=0,
+                                      **run))
+        _oracle_cache[name] = (st, io.read_array(pre + "ccf_ccs.bin"), io.read_array(pre + "ccf_caustics.bin"))
+    return _oracle_cache[name]
+
+
+def _product(sf, run, gkw):
+    with ccf_amd.Context(starfile=sf, outfile_prefix="./", write_stars=0, write_critical_curves=0, write_caustics=1,
+                         **run, **gkw) as c:
+        assert c.run(0), c.last_error()
+        return c.array(ccf_amd.ARRAY_CCS), c.array(ccf_amd.ARRAY_CAUSTICS), c.info()
+
+
+def _check_against_oracle(workdir, name, gkw):
+    sf, run = _field(workdir, name)
+    ost, occ, ocau = _oracle_branch0(workdir, name)
+    print(f"# product {name} {gkw or 'fp64'}", flush=True)
+    gcc, gcau, gi = _product(sf, run, gkw)
+    R, nb = ost["num_roots"], run["num_branches"]
+    assert gi["num_roots"] == R and gi["tree_levels"] == ost["tree_levels"] and gi["expansion_order"] == run["fmm_order"]
+    assert gi["points_converged"] == gi["points_total"] == R * (run["num_phi"] + nb)
+    assert ost["points_converged"] == ost["points_solved"] == R * (run["num_phi"] // nb + 1)
+    g0, c0 = branch_slice(gcc, R, nb, 1), branch_slice(gcau, R, nb, 1)
+    assert g0.shape == occ.shape
+    okw = {k: v for k, v in run.items() if k not in ("num_phi", "num_branches")}
+    p = oracle.params(starfile=sf, **okw)
+    if not gkw:
+        worst, nbad = compare_arrays(g0, occ, R, 1)
+        assert nbad == 0, (worst, nbad)
+        worst_c, nbad_c = compare_arrays(c0, ocau, R, 1)
+        assert nbad_c == 0, (worst_c, nbad_c)
+        return gi, ost, 0
+    pairs = differing_pairs(g0, occ, R, 1)
+    # within the reference's convergence band: twice the oracle's Newton step
+    bad = [(c, k, y, abs(x - y)) for c, k, x, y in pairs]
+    allowed = 2 * newton_step(sf, bad, run["num_phi"], nb, **okw)
+    over = [(b, a) for b, a in zip(bad, allowed) if b[3] > 1e-9 * max(abs(b[2]), 1) + a]
+    assert not over, over[:5]
+    assert len(pairs) <= 1e-4 * g0.size, len(pairs)
+    # the accuracy gate: each differing mixed root is within 1e-9 theta_E of
+    # the exact root (the oracle's root polished by its own Newton steps)
+    if pairs:
+        x = np.array([q[2] for q in pairs])
+        y = np.array([q[3] for q in pairs])
+        ph = np.array([phi_of(c, k, run["num_phi"], nb) for c, k, _, _ in pairs])
+        exact = oracle.newton_polish(p, y, ph)
+        assert np.abs(x - exact).max() <= 1e-9, np.abs(x - exact).max()
+        print(f"# {name} mixed: {len(pairs)} points differ from the oracle by > 1e-9 (max {np.abs(x - y).max():.3e}); "
+              f"mixed -> exact root max {np.abs(x - exact).max():.3e}, oracle -> exact root max {np.abs(y - exact).max():.3e}",
+              flush=True)
+    # caustics: the oracle's lens map of the product's own critical points
+    pts = g0.ravel()
+    v, _ = oracle.evaluate(p, pts, 0.0)
+    w = v[:, 4] + 1j * v[:, 5]
+    rel = np.abs(c0.ravel() - w) / np.maximum(np.abs(w), 1.0)
+    assert rel.max() < 1e-11, rel.max()
+    return gi, ost, len(pairs)
+
+
+@pytest.mark.parametrize("gkw", [{}, MIXED], ids=["fp64", "mixed"])
+def test_cc10k_matches_oracle(workdir, gkw):
+    gi, ost, _ = _check_against_oracle(workdir, "cc10k", gkw)
+    assert np.isfinite(gi["max_witt_residual"]) and gi["max_error"] < 1e-7
+
+
+def test_cc50k_mixed_matches_oracle(workdir):
+    gi, ost, _ = _check_against_oracle(workdir, "cc50k", MIXED)
+    assert np.isfinite(gi["max_witt_residual"]) and np.isfinite(gi["max_error"])
+    assert gi["points_outside_tree"] > 0   # the long rectangle's curves leave the tree's box
+
+
+# ---------------------------------------------------------------- batch256
+BATCH_RUN = dict(num_phi=504, num_branches=4)
+
+
+def _batch_fields(workdir):
+    """256 fields: N uniform in [50, 1000], equal masses, kappa = kappa_star
+    uniform in [0.1, 0.6], shear uniform in [0, 0.5]; field k from seed 100 + k"""
+    out = []
+    for k in range(256):
+        seed = 100 + k
+        r = np.random.default_rng(seed)
+        n = int(r.integers(50, 1001))
+        ks = float(r.uniform(0.1, 0.6))
+        g = float(r.uniform(0.0, 0.5))
+        path = os.path.join(str(workdir), f"b256_{k}.bin")
+        if not os.path.exists(path):
+            ccf_amd.write_synthetic_stars(path, seed, num_stars=n, kappa_tot=ks, kappa_star=ks, shear=g, mass_function="equal")
+        out.append((path, dict(kappa_tot=ks, shear=g)))
+    return out
+
+
+@pytest.fixture(scope="module")
+def batch256(workdir):
+    fields = _batch_fields(workdir)
+    res = {}
+    for tag, extra in (("fp64", {}), ("mixed", dict(precision=1, predictor=4))):
+        print(f"# batch256 {tag}", flush=True)
+        b = ccf_amd.Batch()
+        for sf, kw in fields:
+            b.add(starfile=sf, outfile_prefix="./", write_stars=0, write_critical_curves=0, write_caustics=1,
+                  **kw, **BATCH_RUN, **extra)
+        assert b.run(0), b.last_error()
+        res[tag] = ([b.info(i) for i in range(len(fields))], [b.array(i, ccf_amd.ARRAY_CCS) for i in range(len(fields))],
+                    b.info(-1))
+        b.close()
+    return fields, res
+
+
+def test_batch256_properties(batch256):
+    """every field of both batched forms: all points converged and finite,
+    a finite Witt residual and 1/mu error (their values are compared with
+    the oracle's and the single-field path's below: near a star a root that
+    meets |dz| < 1e-9 theta_E can leave |f0| of order 1e-6), one expansion
+    order for the batch"""
+    fields, res = batch256
+    for tag in ("fp64", "mixed"):
+        infos, ccs, tot = res[tag]
+        assert tot["points_converged"] == tot["points_total"] == sum(i["points_total"] for i in infos)
+        assert len({i["expansion_order"] for i in infos}) == 1
+        for (sf, kw), inf, a in zip(fields, infos, ccs):
+            R = inf["num_roots"]
+            assert a.shape == (R * 4, 504 // 4 + 1) and np.all(np.isfinite(a))
+            assert inf["points_converged"] == inf["points_total"] == R * 508
+            assert 0 < inf["max_error"] < 1e-3 and 0 < inf["max_witt_residual"] < 1e-3, (sf, inf["max_error"], inf["max_witt_residual"])
+
+
+SPREAD = [0, 37, 73, 110, 146, 183, 219, 255]
+
+
+@pytest.mark.parametrize("k", SPREAD)
+def test_batch256_field_matches_oracle(workdir, batch256, k):
+    """8 fields spread over the batch against the oracle at the batch's
+    expansion order: the FP64 batch at every point, the mixed batch within
+    the band and the 1e-9 theta_E gate to the exact root"""
+    fields, res = batch256
+    sf, kw = fields[k]
+    order = res["fp64"][0][k]["expansion_order"]
+    pre = os.path.join(str(workdir), f"b256o_{k}_")
+    ost = oracle.run(oracle.params(starfile=sf, outfile_prefix=pre, fmm_order=order, write_stars=0, write_caustics=0,
+                                   **kw, **BATCH_RUN))
+    R, nb = ost["num_roots"], BATCH_RUN["num_branches"]
+    o = io.read_array(pre + "ccf_ccs.bin")
+    worst, nbad = compare_arrays(res["fp64"][1][k], o, R, nb)
+    assert nbad == 0, (k, worst, nbad)
+    gi = res["fp64"][0][k]
+    # the error maxima agree to the rounding of f0 at the worst point (an
+    # early exit just under 1e-9, or a root beside a star whose |f1| turns
+    # rounding-level differences of z into 1e-5 of |f0|)
+    assert gi["max_error"] == pytest.approx(ost["max_error"], rel=1e-3, abs=1e-12), (gi["max_error"], ost["max_error"])
+    assert gi["max_witt_residual"] == pytest.approx(ost["max_witt_residual"], rel=1e-3, abs=1e-12)
+    pairs = differing_pairs(res["mixed"][1][k], o, R, nb)
+    assert len(pairs) <= max(5, 1e-3 * o.size), len(pairs)
+    if pairs:
+        x = np.array([q[2] for q in pairs])
+        y = np.array([q[3] for q in pairs])
+        ph = np.array([phi_of(c, kk, BATCH_RUN["num_phi"], nb) for c, kk, _, _ in pairs])
+        exact = oracle.newton_polish(oracle.params(starfile=sf, fmm_order=order, **kw), y, ph)
+        assert np.abs(x - exact).max() <= 1e-9, (k, np.abs(x - exact).max())
+
+
+def test_batch256_matches_single_field_path(workdir, batch256):
+    """every one of the 256 fields of the FP64 batch against the product's
+    single-field path at the batch's expansion order"""
+    fields, res = batch256
+    order = res["fp64"][0][0]["expansion_order"]
+    for k, (sf, kw) in enumerate(fields):
+        with ccf_amd.Context(starfile=sf, outfile_prefix="./", write_stars=0, write_critical_curves=0, write_caustics=0,
+                             fmm_order=order, **kw, **BATCH_RUN) as c:
+            assert c.run(0), c.last_error()
+            s = c.array(ccf_amd.ARRAY_CCS)
+            si = c.info()
+        R = si["num_roots"]
+        worst, nbad = compare_arrays(res["fp64"][1][k], s, R, BATCH_RUN["num_branches"])
+        assert nbad == 0, (k, worst, nbad)
+        bi = res["fp64"][0][k]
+        assert bi["max_error"] == pytest.approx(si["max_error"], rel=1e-3, abs=1e-12), (k, bi["max_error"], si["max_error"])
+        assert bi["max_witt_residual"] == pytest.approx(si["max_witt_residual"], rel=1e-3, abs=1e-12)

@@ -17,7 +17,7 @@ import pytest
 import ccf_amd
 import oracle
 from ccf_amd import io
-from helpers import FIELDS, compare_arrays, make_field, newton_step, oracle_run
+from helpers import FIELDS, compare_arrays, compare_param_files, make_field, newton_step, oracle_run
 
 pytestmark = pytest.mark.gpu
 
@@ -176,6 +176,31 @@ def test_root_sums_fmm_binning_without_lds(workdir):
     assert rel.max() <= 5e-5, rel.max()
 
 
+def check_error_maxima(ginfo, ostats, file_err=None, oracle_file_err=None, exact=True):
+    """max_error_1/mu (ccf.cuh:972,1068) and the max Witt residual |f0| of a
+    run against the oracle's. Where the product follows the oracle's iterates
+    (the FP64 path) they agree to the rounding of f0 at the worst point: that
+    |f0| is 1e-9 to 1e-5, and the two last iterates differ by rounding
+    (1e-15 |z|), which moves f0 by |f1| times that (|f1| reaches 1e3 beside
+    a star), so 1e-3 relative or 1e-12 absolute. The other forms take
+    other paths to the roots and stop elsewhere inside the convergence band:
+    an early exit anywhere below the 1/mu bound's 1e-9 (ccf_functions.cuh:
+    207-211), where |f0| < 1e-9 / (2 (1 - kappa - d alpha_smooth/dz)), below
+    1e-8 on these fields; their maxima are held to the larger of the
+    oracle's and that band."""
+    g, o = ginfo["max_error"], ostats["max_error"]
+    gr, orr = ginfo["max_witt_residual"], ostats["max_witt_residual"]
+    assert np.isfinite(g) and np.isfinite(gr) and 0 < o < 1e-7 and 0 < orr
+    if file_err is not None:
+        assert file_err == pytest.approx(g, rel=1e-8) and oracle_file_err == pytest.approx(o, rel=1e-8)
+    if exact:
+        assert g == pytest.approx(o, rel=1e-3, abs=1e-12), (g, o)
+        assert gr == pytest.approx(orr, rel=1e-3, abs=1e-12), (gr, orr)
+    else:
+        assert g <= max(o, 1e-9) * (1 + 1e-2) + 1e-12, (g, o)
+        assert gr <= max(orr, 1e-8) * (1 + 1e-2) + 1e-12, (gr, orr)
+
+
 RUNS = [
     ("cfg1", dict(kappa_tot=0.3, shear=0.1, num_phi=100, num_branches=1, direct=1)),
     ("cfg1", dict(kappa_tot=0.3, shear=0.1, num_phi=100, num_branches=1)),
@@ -253,12 +278,10 @@ def test_run_matches_oracle(workdir, name, kw, gkw):
         assert nbad_c <= max(5, 1e-3 * R * (kw["num_phi"] + nb)) and worst_c < 1e-6, (worst_c, nbad_c)
     gm = io.read_array(gp + "ccf_mu_length_scales.bin", complex_values=False)
     assert gm.shape == gccs.shape and np.all(np.isfinite(gm)) and np.all(gm > 0)
-    # parameter files agree except for the timings
-    a = io.read_params(gp + "ccf_parameter_info.txt")
-    b = io.read_params(op + "ccf_parameter_info.txt")
-    for k in ("kappa_tot", "shear", "mu_ave", "kappa_star", "num_stars", "num_roots", "num_phi", "num_branches"):
-        assert a[k] == b[k], k
-    assert a["max_error_1/mu"] < 1e-8 and b["max_error_1/mu"] < 1e-8
+    # parameter files agree line by line except for the timings; the
+    # error maximum is compared numerically
+    ge, oe = compare_param_files(gp + "ccf_parameter_info.txt", op + "ccf_parameter_info.txt")
+    check_error_maxima(ginfo, ostats, ge, oe, exact=not gkw)
     # both write the same star set (tree order may differ)
     _, _, _, _, gs = io.read_stars(gp + "ccf_stars.bin")
     _, _, _, _, os_ = io.read_stars(op + "ccf_stars.bin")
@@ -365,3 +388,139 @@ def test_error_pass_takes_points_outside_the_tree(workdir):
     o, _ = oracle.evaluate(oracle.params(starfile=sf, **okw), ccs[outside], 0.0)
     w = o[:, 4] + 1j * o[:, 5]
     assert np.max(np.abs(cau[outside] - w) / np.maximum(np.abs(w), 1.0)) < 1e-11
+
+
+def test_non_finite_errors_abort_the_run(workdir):
+    """ccf.cuh:960-967: a non-finite 1/mu error aborts the run with the
+    reference's message and no files are written. Two stars exactly one
+    theta_E apart put one star's seed root (star + 1, ccf.cuh:716) on the
+    other star, where Witt's function is infinite; the NaN that the Aberth
+    update makes of it spreads through the root sums of its row. The oracle
+    fails the same way on the same file."""
+    import subprocess
+
+    rng = np.random.default_rng(41)
+    st = np.column_stack([rng.uniform(-6, 6, 30), rng.uniform(-6, 6, 30), np.ones(30)])
+    st[0, :2] = [0.25, 0.5]
+    st[1, :2] = [1.25, 0.5]
+    sf = os.path.join(str(workdir), "nanfield.bin")
+    io.write_stars(sf, st, 0, complex(np.sqrt(30 / 0.4), 0.0))
+    kw = dict(kappa_tot=0.4, shear=0.2, num_phi=8, num_branches=1)
+    msg = "Error. Errors in 1/mu contain values which are not positive real numbers."
+    pre = os.path.join(str(workdir), "nan_g_")
+    with _ctx(sf, pre, **kw) as c:
+        assert not c.run(0)
+        assert c.last_error().strip() == msg
+        assert not c.save(0)
+    r = subprocess.run([ccf_amd.CLI_PATH, "-k", "0.4", "-y", "0.2", "-sf", sf, "-np", "8", "-nb", "1", "-o", pre + "cli_",
+                        "-v", "0"], capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0 and msg in r.stderr
+    assert not [f for f in os.listdir(str(workdir)) if f.startswith("nan_g_")]
+    with pytest.raises(RuntimeError):
+        oracle_run(sf, os.path.join(str(workdir), "nan_o_"), **kw)
+
+
+RS_FIELDS = [
+    dict(num_stars=4000, kappa_tot=0.45, kappa_star=0.45, shear=0.35, mass_function="salpeter", m_lower=0.08, m_upper=1.0),
+    dict(num_stars=4000, kappa_tot=0.7, kappa_star=0.35, shear=0.25, mass_function="kroupa", m_lower=0.08, m_upper=1.0,
+         rectangular=1),
+    dict(num_stars=3000, kappa_tot=0.3, kappa_star=0.3, shear=0.1, mass_function="uniform", m_lower=0.1, m_upper=2.0),
+]
+
+
+def _generated(seed, **kw):
+    with ccf_amd.Context(random_seed=seed, num_phi=10, outfile_prefix="./", **kw) as c:
+        c.prepare()
+        inf = c.info()
+        st = c.array(ccf_amd.ARRAY_STARS)   # tree order: a deterministic permutation of the drawn stars
+    return st, inf
+
+
+@pytest.mark.parametrize("spec", RS_FIELDS)
+def test_random_star_field_generation(spec):
+    """-rs star fields (star.cuh:40-107, ccf.cuh:602-704) drawn on the
+    device: every position inside the disc or rectangle the reference draws
+    in (the field of the mass function's mean mass, before the corner
+    follows the drawn mean mass, ccf.cuh:686-700), positions uniform in it
+    and masses distributed as the mass function (Kolmogorov-Smirnov at
+    p > 1e-4, moments within 5 sigma of the closed forms), the same seed
+    giving the same bits and another seed another field."""
+    from scipy import stats
+
+    st, inf = _generated(1234, **spec)
+    n = spec["num_stars"]
+    assert st.shape == (n, 3) and np.all(np.isfinite(st)) and inf["random_seed"] == 1234
+    ml, mu = spec["m_lower"], spec["m_upper"]
+    mean = ccf_amd.mass_function(spec["mass_function"], 1, 0.0, ml, mu, 1.0)
+    mean2 = ccf_amd.mass_function(spec["mass_function"], 2, 0.0, ml, mu, 1.0)
+    m = st[:, 2]
+    assert m.min() >= ml * (1 - 1e-12) and m.max() <= mu * (1 + 1e-12)
+    sd = np.sqrt(mean2 - mean ** 2)
+    assert abs(m.mean() - mean) < 5 * sd / np.sqrt(n), (m.mean(), mean)
+    # quantiles of the inverse CDF: the fraction of masses below m(p) is p
+    cdf_pts = [ccf_amd.mass_function(spec["mass_function"], 0, p, ml, mu, 1.0) for p in (0.1, 0.3, 0.5, 0.7, 0.9)]
+    for p, mp in zip((0.1, 0.3, 0.5, 0.7, 0.9), cdf_pts):
+        assert abs((m < mp).mean() - p) < 5 * np.sqrt(p * (1 - p) / n), (p, (m < mp).mean())
+    corner = complex(inf["corner_re"], inf["corner_im"])
+    drawn = np.sqrt(mean / m.mean())   # the corner before it followed the drawn mean mass
+    if spec.get("rectangular"):
+        cx, cy = corner.real * drawn, corner.imag * drawn
+        assert np.all(np.abs(st[:, 0]) <= cx * (1 + 1e-12)) and np.all(np.abs(st[:, 1]) <= cy * (1 + 1e-12))
+        assert stats.kstest(st[:, 0] / cx, "uniform", args=(-1, 2)).pvalue > 1e-4
+        assert stats.kstest(st[:, 1] / cy, "uniform", args=(-1, 2)).pvalue > 1e-4
+    else:
+        R = abs(corner) * drawn
+        r2 = (st[:, 0] ** 2 + st[:, 1] ** 2) / R ** 2
+        assert r2.max() <= 1 + 1e-12 and r2.max() > 0.99
+        assert stats.kstest(r2, "uniform").pvalue > 1e-4
+        ang = np.mod(np.arctan2(st[:, 1], st[:, 0]), 2 * np.pi) / (2 * np.pi)
+        assert stats.kstest(ang, "uniform").pvalue > 1e-4
+    # kappa_star_actual is taken before the corner follows the drawn mean
+    # mass (ccf.cuh:676-700): kappa_star scaled by drawn / expected mean mass
+    assert inf["kappa_star_actual"] == pytest.approx(spec["kappa_star"] * m.mean() / mean, rel=1e-9)
+    again, _ = _generated(1234, **spec)
+    assert np.array_equal(st, again)
+    other, _ = _generated(1235, **spec)
+    assert not np.array_equal(st[:, :2], other[:, :2])
+
+
+def test_random_seed_zero_takes_the_clock():
+    """ccf.cuh:619-622: seed 0 is replaced by the clock (never 0), and the
+    field is the one that seed gives when it is set explicitly"""
+    spec = RS_FIELDS[0]
+    st, inf = _generated(0, **spec)
+    seed = inf["random_seed"]
+    assert seed != 0
+    again, inf2 = _generated(seed, **spec)
+    assert np.array_equal(st, again)
+    import time
+    time.sleep(0.01)
+    _, inf3 = _generated(0, **spec)
+    assert inf3["random_seed"] not in (0, seed)
+
+
+def test_cli_random_field_matches_oracle(workdir):
+    """ccf_gpu -rs: a seeded field generated on the device, solved, and the
+    files it writes; the oracle solves the star file that run wrote, and the
+    critical curves, caustics and parameter file agree (the parameter file's
+    random_seed and mass-function lines are the generated run's own)."""
+    import subprocess
+
+    gp = os.path.join(str(workdir), "rs_")
+    op = os.path.join(str(workdir), "rs_o_")
+    r = subprocess.run([ccf_amd.CLI_PATH, "-k", "0.45", "-y", "0.35", "-ks", "0.45", "-ns", "300", "-mf", "salpeter",
+                        "-ml", "0.08", "-mh", "1", "-rs", "4242", "-np", "60", "-nb", "3", "-ws", "1", "-wcc", "1",
+                        "-wc", "1", "-o", gp, "-v", "1"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    kw = dict(kappa_tot=0.45, shear=0.35, num_phi=60, num_branches=3)
+    ost = oracle_run(gp + "ccf_stars.bin", op, **kw)
+    R = ost["num_roots"]
+    assert R == 600
+    for f in ("ccf_ccs.bin", "ccf_caustics.bin"):
+        worst, nbad = compare_arrays(io.read_array(gp + f), io.read_array(op + f), R, 3)
+        assert nbad == 0, (f, worst, nbad)
+    a = io.read_params(gp + "ccf_parameter_info.txt")
+    b = io.read_params(op + "ccf_parameter_info.txt")
+    assert a["random_seed"] == 4242 and a["mass_function"] == "salpeter" and "kappa_star_actual" in a
+    for k in ("kappa_tot", "shear", "num_stars", "num_roots", "num_phi", "num_branches", "rad", "mean_mass_actual"):
+        assert a[k] == pytest.approx(b[k], rel=1e-8), k
