@@ -472,10 +472,10 @@ def main():
 
     def region(r, which):
         # the two event-timed regions of a sweep: the root sums (FMM pass and
-        # direct sums) and the Witt evaluation (k_witt + k_witt_far)
+        # direct sums) and the Witt evaluation (k_witt, which also plans the sums)
         if which == "sums":
-            name = ("root sums: FMM pass (k_r*) + k_sums32 + k_sums32_sparse" if r.get("root_fmm") else
-                    ("k_sums32 + k_sums32_sparse" if r is res.get("mixed") else "k_sums64"))
+            name = ("root sums: FMM pass (k_r*) + k_sums32" if r.get("root_fmm") else
+                    ("k_sums32" if r is res.get("mixed") else "k_sums64"))
             return dict(kernel=name, achieved=r["sums_tflops"], total_ms=r["aberth_ms_total"],
                         avg_launch_ms=r["aberth_ms_avg"], launches=r["aberth_launches"])
         if which == "tail":
@@ -483,7 +483,7 @@ def main():
             return dict(kernel="k_tail (phase tails: Witt evaluations and Cauchy sums)", achieved=r.get("tail_tflops", 0.0),
                         total_ms=r.get("tail_ms_total", 0.0), avg_launch_ms=r.get("tail_ms_total", 0.0) / n,
                         launches=r.get("tail_launches", 0))
-        return dict(kernel="k_witt + k_witt_far", achieved=r["witt_tflops"], total_ms=r["witt_ms_total"],
+        return dict(kernel="k_witt", achieved=r["witt_tflops"], total_ms=r["witt_ms_total"],
                     avg_launch_ms=r["witt_ms_avg"], launches=r["witt_launches"])
 
     regs = [region(head, "sums"), region(head, "witt")]

@@ -127,6 +127,7 @@ void ccf_ctx::pool_release()
 	// work queued on the streams may still use a pooled buffer
 	if (stream) (void)hipStreamSynchronize(stream);
 	if (err_stream) (void)hipStreamSynchronize(err_stream);
+	if (group_stream) (void)hipStreamSynchronize(group_stream);
 	for (auto& [n, p] : pool)
 	{
 		owned.erase(p);
@@ -137,8 +138,10 @@ void ccf_ctx::pool_release()
 
 void ccf_ctx::free_device()
 {
-	// a failed solve can leave error-pass work queued on the second stream
+	// a failed solve can leave error-pass or continuation work queued on
+	// the other streams
 	if (err_stream) (void)hipStreamSynchronize(err_stream);
+	if (group_stream) (void)hipStreamSynchronize(group_stream);
 	auto F = [this](auto*& p) { dfree(p); p = nullptr; };
 	ccf::DeviceTree& T = tree;
 	F(T.center); F(T.hl); F(T.parent); F(T.child); F(T.sstart); F(T.scount);
@@ -158,6 +161,7 @@ void ccf_ctx::free_device()
 		F(ccs); F(caustics); F(mu_scales);
 	}
 	ccf::free_sweep(this, sw);
+	ccf::free_sweep(this, sw2);
 	F(counters);
 	if (host_counts) (void)hipHostFree(host_counts);
 	host_counts = nullptr;
@@ -692,6 +696,8 @@ void ccf_destroy(ccf_ctx* c)
 		if (e) (void)hipEventDestroy(e);
 	if (c->err_event) (void)hipEventDestroy(c->err_event);
 	if (c->err_stream) (void)hipStreamDestroy(c->err_stream);
+	if (c->group_event) (void)hipEventDestroy(c->group_event);
+	if (c->group_stream) (void)hipStreamDestroy(c->group_stream);
 	if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
 	delete c;
 }

@@ -28,6 +28,10 @@ constexpr int MAX_ROOT_FMM_ORDER = 24;
 // highest degree of the continuation predictor
 constexpr int MAX_PREDICTOR = 4;
 constexpr int MAX_ROOT_GRID_LEVELS = 10;
+// groups of branches whose mixed-form continuations run on their own streams
+// side by side (one group's latency-bound sweeps beside the other's
+// device-wide ones)
+constexpr int CONT_GROUPS = 2;
 
 // Quadtree over the stars, structure of arrays indexed by node id. Nodes of
 // a level are contiguous ([level_off[l], level_off[l] + level_cnt[l])) and
@@ -142,6 +146,7 @@ struct SweepBufs
 	double2* ex = nullptr;   // [rows][R] terms of the Aberth sum formed per root
 	int* tile_off = nullptr; // [rows + 1] first tile of each row
 	int* plan = nullptr;     // [4] tiles, slices per tile; rows summed by the root-sum FMM in a phase's first and second sweep
+	int* ticket = nullptr;   // [1] workgroups of the sweep's Witt kernel done (the last forms the plan and resets it)
 	double2* part = nullptr; // [max_items][tile] partial Cauchy sums
 	// root-sum FMM (grid.order > 0 only)
 	RootGrid grid;
@@ -228,11 +233,14 @@ struct ccf_ctx
 		}
 		return events[events_used++];
 	}
-	int* host_counts = nullptr;   // pinned: active roots per row, the two FMM row counts of a read-back, then two slots of two for the mixed continuation
+	int* host_counts = nullptr;   // pinned: active roots per row, the two FMM row counts of a read-back, then per continuation group two slots of two for the mixed continuation
 	int tree_hint_stars = -1, tree_hint_fields = -1, tree_hint_nodes = 0; // the last tree build's node count
 	hipStream_t err_stream = nullptr; // the error pass of find_ccs, beside the continuation (default priority)
 	hipEvent_t err_event = nullptr;   // a row set's roots are final (no timing)
-	hipEvent_t fmm_rows_event[2] = {nullptr, nullptr}; // mixed continuation: after the copy of a step's FMM row count (no timing)
+	hipEvent_t fmm_rows_event[2 * ccf::CONT_GROUPS] = {}; // mixed continuation: after the copy of a step's FMM row count, per group (no timing)
+	hipStream_t group_stream = nullptr; // mixed continuation: the second group of branches' stream (default priority)
+	hipEvent_t group_event = nullptr;   // orders the group streams around the continuation (no timing)
+	ccf::SweepBufs sw2;                 // mixed continuation: the second group's sweep buffers
 	int fmm_rows_sweep[2] = {-1, -1}; // root-sum FMM rows of the last phase's first and second sweep (-1: unknown)
 	int phase_sweeps = 0;         // sweeps the last phase ran (its cap when no read-back saw it end)
 	int sweep_verbose = 0;        // verbose level of the running solve stage (3: a read-back and a line after every sweep)

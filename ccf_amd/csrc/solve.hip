@@ -10,8 +10,9 @@
 //   K1 k_witt      per active root: Witt's function f0, its derivative f1
 //                  (near-field P2P + far-field L2P + smooth sheet) and the
 //                  1/mu early exit; roots that need the Cauchy sums are
-//                  appended to a per-row list;
-//   K2 k_plan      cuts the (stars + roots) term range of every tile of
+//                  appended to a per-row list; k_witt_far does the same for
+//                  the few roots outside the tree and, in its last workgroup,
+//   K2 plan_sums   cuts the (stars + roots) term range of every tile of
 //                  RPT*256 listed roots into S slices so that dense and sparse
 //                  sweeps alike give enough equal work items to fill the chip;
 //   K3 k_sums      one work item = one tile x one slice: the slice's terms are
@@ -214,6 +215,7 @@ struct RowMap
 	int R, np, nb, j, rows_per_branch;
 	double phi_fixed;
 	const double2* cs = nullptr; // (cos phi, sin phi) of every phi row, or null
+	int c0 = 0;                  // the first branch of the rows (a group of branches solved on its own stream)
 	__device__ inline void get(int r, int& row, double& phi) const
 	{
 		if (rows_per_branch == 0)
@@ -222,7 +224,7 @@ struct RowMap
 			phi = phi_fixed;
 			return;
 		}
-		const int c = r / rows_per_branch;
+		const int c = c0 + r / rows_per_branch;
 		const int sgn = rows_per_branch == 1 ? 0 : ((r % rows_per_branch) == 0 ? -1 : 1);
 		const int center = np / (2 * nb) + c * (np / nb + 1);
 		row = center + sgn * j;
@@ -262,7 +264,7 @@ struct SweepView
 	int R, rows, tile;
 	const int* act;
 	int* act_next;
-	int* cnt;      // cleared by k_plan once read
+	int* cnt;      // cleared by the sweep's plan once read
 	int* cnt_next;
 	int* fl;
 	int* fcnt;
@@ -275,6 +277,7 @@ struct SweepView
 	double2* ex;
 	int* tile_off;
 	int* plan;
+	int* ticket;
 	double2* part;
 	int* slot;     // root-sum FMM only (else null)
 	int* rmode;
@@ -303,6 +306,7 @@ SweepView view(const SweepBufs& B, int parity)
 	v.ex = B.ex;
 	v.tile_off = B.tile_off;
 	v.plan = B.plan;
+	v.ticket = B.ticket;
 	v.part = B.part;
 	v.slot = B.slot;
 	v.rmode = B.rmode;
@@ -346,6 +350,122 @@ __device__ inline int witt_root(const LensView& L, double2* p, cd z, const Witt&
 	if (circ_exact) ex = 2.0 * z.inv(); // the disc's pole at 0 (ccf_functions.cuh:223-227)
 	if (MIXED && fmm_pole) ex += 2.0 * (w.nr.s0 + far_pole32(L, w.node, z));
 	return ROOT_FULL;
+}
+
+// The near-field sums of z's node (a node that is not a leaf: z lies
+// outside the tree, whose root then holds every star) over its star
+// ranges, the terms of p2p's range loop split over the workgroup's lanes
+// and reduced in a fixed order; the result is valid in thread 0. Every
+// thread of the workgroup calls it.
+template <int BS = BLOCK>
+__device__ inline Near coop_near(const LensView& L, int node, cd z, double (*red)[8])
+{
+	Near nr;
+	const int ranges = L.nnf[node];
+	for (int k = 0; k < ranges; k++)
+	{
+		const int2 rg = L.nf[node * MAX_NEAR_RANGES + k];
+		for (int s = rg.x + threadIdx.x; s < rg.x + rg.y; s += BS)
+		{
+			const double2 q = L.spos[s];
+			const cd t = cd(z.re - q.x, z.im - q.y).inv();
+			nr.s0 += t;
+			const cd t1 = t * L.smass[s];
+			nr.s1 += t1;
+			const cd t2 = t1 * t;
+			nr.s2 += t2;
+			nr.s3 += t2 * t;
+		}
+		nr.count += rg.y;
+	}
+	double v[8] = {nr.s0.re, nr.s0.im, nr.s1.re, nr.s1.im, nr.s2.re, nr.s2.im, nr.s3.re, nr.s3.im};
+#pragma unroll
+	for (int q = 0; q < 8; q++)
+		for (int o = 32; o >= 1; o >>= 1) v[q] += __shfl_xor(v[q], o, 64);
+	if ((threadIdx.x & 63) == 0)
+		for (int q = 0; q < 8; q++) red[threadIdx.x >> 6][q] = v[q];
+	__syncthreads();
+	Near sum;
+	sum.count = nr.count;
+	if (threadIdx.x == 0)
+	{
+		double t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+		for (int w = 0; w < BS / 64; w++)
+			for (int q = 0; q < 8; q++) t[q] += red[w][q];
+		sum.s0 = cd(t[0], t[1]);
+		sum.s1 = cd(t[2], t[3]);
+		sum.s2 = cd(t[4], t[5]);
+		sum.s3 = cd(t[6], t[7]);
+	}
+	return sum;
+}
+
+// K2: tiles per row (exclusive prefix in tile_off) and slices per tile; a
+// row with at least fmm_min listed roots (fmm_min > 0) takes its root sums
+// from the FMM pass instead and gets no tiles. Formed by the last workgroup
+// of k_witt to finish, once every row's listed count is final.
+constexpr int WITT_FAR_BLOCKS = 16;
+constexpr int WITT_FAR_THREADS = 512;
+
+struct PlanArgs
+{
+	int nrows, terms, target, fmm_min, sweep;
+};
+
+__device__ inline void plan_sums(SweepView V, const PlanArgs& P)
+{
+	const int nrows = P.nrows, terms = P.terms, target = P.target, fmm_min = P.fmm_min, sweep = P.sweep;
+	__shared__ int wsum[16];
+	__shared__ int carry, nfmm;
+	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+	if (threadIdx.x == 0) carry = nfmm = 0;
+	__syncthreads();
+	for (int base = 0; base < nrows; base += blockDim.x)
+	{
+		const int r = base + threadIdx.x;
+		const int nf = r < nrows ? __hip_atomic_load(&V.fcnt[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+		const bool fmm = r < nrows && fmm_min > 0 && nf >= fmm_min;
+		if (r < nrows && V.rmode) V.rmode[r] = fmm ? 1 : 0;
+		if (fmm) atomicAdd(&nfmm, 1);
+		// this sweep's active count is consumed (k_witt read it): cleared
+		// here, it is the list the next sweep's k_witt appends stepped roots to
+		if (r < nrows) V.cnt[r] = 0;
+		const int v = (r < nrows && !fmm) ? (nf + V.tile - 1) / V.tile : 0;
+		int x = v;
+		for (int d = 1; d < 64; d <<= 1)
+		{
+			const int y = __shfl_up(x, d, 64);
+			if (lane >= d) x += y;
+		}
+		if (lane == 63) wsum[w] = x;
+		__syncthreads();
+		int woff = 0, tot = 0;
+		for (int k = 0; k < nw; k++)
+		{
+			if (k < w) woff += wsum[k];
+			tot += wsum[k];
+		}
+		if (r < nrows) V.tile_off[r] = carry + woff + x - v;
+		__syncthreads();
+		if (threadIdx.x == 0) carry += tot;
+		__syncthreads();
+	}
+	if (threadIdx.x == 0)
+	{
+		const int tiles = carry;
+		int S = 1;
+		if (tiles > 0 && tiles < target)
+		{
+			S = (target + tiles - 1) / tiles;
+			S = min(S, MAX_SLICES);
+			S = min(S, max(1, terms / MIN_SLICE));
+		}
+		V.tile_off[nrows] = tiles;
+		V.plan[0] = tiles;
+		V.plan[1] = S;
+		// FMM rows of the phase's first and second sweep (cleared by k_prepare)
+		if (sweep < 2 && nfmm) V.plan[2 + sweep] += nfmm;
+	}
 }
 
 // K1 (ccf_functions.cuh:197-213 and the early return at :207-211). A root
@@ -426,62 +546,13 @@ __global__ __launch_bounds__(BLOCK) void k_witt(LensView L, double2* __restrict_
 // reduced in a fixed order), then finishes the root as k_witt does. Few
 // roots lie outside the tree, and the launch comes every sweep whether a
 // row has any or not: 16 workgroups per row, each taking every 16th, of
-// WITT_FAR_THREADS lanes (a root's sum over N stars: N / 512 loads per lane)
-constexpr int WITT_FAR_BLOCKS = 16;
-constexpr int WITT_FAR_THREADS = 512;
-
-// The near-field sums of z's node (a node that is not a leaf: z lies
-// outside the tree, whose root then holds every star) over its star
-// ranges, the terms of p2p's range loop split over the workgroup's lanes
-// and reduced in a fixed order; the result is valid in thread 0. Every
-// thread of the workgroup calls it.
-template <int BS = BLOCK>
-__device__ inline Near coop_near(const LensView& L, int node, cd z, double (*red)[8])
-{
-	Near nr;
-	const int ranges = L.nnf[node];
-	for (int k = 0; k < ranges; k++)
-	{
-		const int2 rg = L.nf[node * MAX_NEAR_RANGES + k];
-		for (int s = rg.x + threadIdx.x; s < rg.x + rg.y; s += BS)
-		{
-			const double2 q = L.spos[s];
-			const cd t = cd(z.re - q.x, z.im - q.y).inv();
-			nr.s0 += t;
-			const cd t1 = t * L.smass[s];
-			nr.s1 += t1;
-			const cd t2 = t1 * t;
-			nr.s2 += t2;
-			nr.s3 += t2 * t;
-		}
-		nr.count += rg.y;
-	}
-	double v[8] = {nr.s0.re, nr.s0.im, nr.s1.re, nr.s1.im, nr.s2.re, nr.s2.im, nr.s3.re, nr.s3.im};
-#pragma unroll
-	for (int q = 0; q < 8; q++)
-		for (int o = 32; o >= 1; o >>= 1) v[q] += __shfl_xor(v[q], o, 64);
-	if ((threadIdx.x & 63) == 0)
-		for (int q = 0; q < 8; q++) red[threadIdx.x >> 6][q] = v[q];
-	__syncthreads();
-	Near sum;
-	sum.count = nr.count;
-	if (threadIdx.x == 0)
-	{
-		double t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-		for (int w = 0; w < BS / 64; w++)
-			for (int q = 0; q < 8; q++) t[q] += red[w][q];
-		sum.s0 = cd(t[0], t[1]);
-		sum.s1 = cd(t[2], t[3]);
-		sum.s2 = cd(t[4], t[5]);
-		sum.s3 = cd(t[6], t[7]);
-	}
-	return sum;
-}
-
+// WITT_FAR_THREADS lanes (a root's sum over N stars: N / 512 loads per lane).
+// Its last workgroup to finish forms the plan of the sweep's Cauchy sums.
 template <bool MIXED>
 __global__ __launch_bounds__(WITT_FAR_THREADS) void k_witt_far(LensView L, double2* __restrict__ roots, RowMap M, SweepView V,
-	unsigned long long* __restrict__ stats, int circ_exact, int fmm_pole)
+	unsigned long long* __restrict__ stats, int circ_exact, int fmm_pole, PlanArgs P)
 {
+	__shared__ int s_last;
 	const int r = blockIdx.y;
 	const int R = M.R;
 	const int nd = V.dcnt[r];
@@ -531,62 +602,21 @@ __global__ __launch_bounds__(WITT_FAR_THREADS) void k_witt_far(LensView L, doubl
 		}
 		__syncthreads();
 	}
-}
-
-// K2: tiles per row (exclusive prefix in tile_off) and slices per tile; a
-// row with at least fmm_min listed roots (fmm_min > 0) takes its root sums
-// from the FMM pass instead and gets no tiles
-__global__ void k_plan(SweepView V, int nrows, int terms, int target, int fmm_min, int sweep)
-{
-	__shared__ int wsum[16];
-	__shared__ int carry, nfmm;
-	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-	if (threadIdx.x == 0) carry = nfmm = 0;
+	// every workgroup takes a ticket once its appends are issued; the last
+	// one sees every row's listed count (k_witt's and these) and forms the
+	// sums' plan, then resets the ticket for the next sweep
 	__syncthreads();
-	for (int base = 0; base < nrows; base += blockDim.x)
-	{
-		const int r = base + threadIdx.x;
-		const bool fmm = r < nrows && fmm_min > 0 && V.fcnt[r] >= fmm_min;
-		if (r < nrows && V.rmode) V.rmode[r] = fmm ? 1 : 0;
-		if (fmm) atomicAdd(&nfmm, 1);
-		// this sweep's active count is consumed (k_witt read it): cleared
-		// here, it is the list the next sweep's k_witt appends stepped roots to
-		if (r < nrows) V.cnt[r] = 0;
-		const int v = (r < nrows && !fmm) ? (V.fcnt[r] + V.tile - 1) / V.tile : 0;
-		int x = v;
-		for (int d = 1; d < 64; d <<= 1)
-		{
-			const int y = __shfl_up(x, d, 64);
-			if (lane >= d) x += y;
-		}
-		if (lane == 63) wsum[w] = x;
-		__syncthreads();
-		int woff = 0, tot = 0;
-		for (int k = 0; k < nw; k++)
-		{
-			if (k < w) woff += wsum[k];
-			tot += wsum[k];
-		}
-		if (r < nrows) V.tile_off[r] = carry + woff + x - v;
-		__syncthreads();
-		if (threadIdx.x == 0) carry += tot;
-		__syncthreads();
-	}
 	if (threadIdx.x == 0)
 	{
-		const int tiles = carry;
-		int S = 1;
-		if (tiles > 0 && tiles < target)
-		{
-			S = (target + tiles - 1) / tiles;
-			S = min(S, MAX_SLICES);
-			S = min(S, max(1, terms / MIN_SLICE));
-		}
-		V.tile_off[nrows] = tiles;
-		V.plan[0] = tiles;
-		V.plan[1] = S;
-		// FMM rows of the phase's first and second sweep (cleared by k_prepare)
-		if (sweep < 2 && nfmm) V.plan[2 + sweep] += nfmm;
+		__threadfence();
+		s_last = atomicAdd(V.ticket, 1) == (int)(gridDim.x * gridDim.y) - 1;
+	}
+	__syncthreads();
+	if (s_last)
+	{
+		__threadfence();
+		plan_sums(V, P);
+		if (threadIdx.x == 0) *V.ticket = 0;
 	}
 }
 
@@ -690,9 +720,9 @@ __global__ __launch_bounds__(BLOCK) void k_sums64(const double2* __restrict__ sp
 }
 
 // K3 in FP32 with the FP64 fallback for clustered roots
-// The items of tiles with few listed roots (at most SPARSE_TARGETS), in a
-// kernel of their own (k_sums32 skips them) so that neither path's registers
-// limit the other's occupancy: the lanes run over the slice's sources and
+// The items of tiles with few listed roots (at most SPARSE_TARGETS), in the
+// same launch as the dense items (a branch uniform per item): the lanes run
+// over the slice's sources and
 // groups of SPARSE_GROUP targets are held uniform across the workgroup, so
 // no lane idles on an empty target slot. Each target's partial sum is
 // reduced over the lanes in a fixed order and lands where the tile's own
@@ -818,7 +848,14 @@ __global__ __launch_bounds__(BLOCK) void k_sums32(const float2* __restrict__ spo
 		double phi;
 		M.get(r, row, phi);
 		const double2* __restrict__ xrow = roots + (size_t)row * R;
-		if (nfr - t * T <= SPARSE_TARGETS) continue; // k_sums32_sparse's item
+		const long long lo = s * Lt / S, hi = (s + 1) * Lt / S;
+		if (nfr - t * T <= SPARSE_TARGETS)
+		{
+			// a tile of few listed roots: lanes over the sources (uniform per item)
+			sums32_sparse(spos32, Np, xrow, V.fl + (size_t)r * R + (size_t)t * T, nfr - t * T, lo, hi, cluster_d2,
+				V.part + (size_t)item * T, nclus);
+			continue;
+		}
 		double zx[RPT], zy[RPT], bx[RPT], by[RPT];
 		float fx[RPT], fy[RPT], ax[RPT], ay[RPT];
 		int ia[RPT];
@@ -836,7 +873,6 @@ __global__ __launch_bounds__(BLOCK) void k_sums32(const float2* __restrict__ spo
 			ax[k] = ay[k] = 0;
 			bx[k] = by[k] = 0;
 		}
-		const long long lo = s * Lt / S, hi = (s + 1) * Lt / S;
 		const int s1 = (int)min(hi, (long long)Np);
 		for (int c0 = (int)lo; c0 < s1; c0 += CHUNK32)
 		{
@@ -924,32 +960,6 @@ __global__ __launch_bounds__(BLOCK) void k_sums32(const float2* __restrict__ spo
 	if ((threadIdx.x & 63) == 0 && nclus) atomicAdd(stat_slot(stats, C_CLUSTER), nclus);
 }
 
-template <int RPT>
-__global__ __launch_bounds__(BLOCK) void k_sums32_sparse(const float2* __restrict__ spos32, int Np, const double2* __restrict__ roots,
-	RowMap M, SweepView V, int nrows, float cluster_d2, unsigned long long* __restrict__ stats)
-{
-	const int tiles = V.plan[0], S = V.plan[1];
-	const long long items = (long long)tiles * S;
-	const int R = M.R, T = BLOCK * RPT;
-	const long long Lt = (long long)Np + R;
-	unsigned long long nclus = 0;
-	for (long long item = blockIdx.x; item < items; item += gridDim.x)
-	{
-		const int g = (int)(item / S), s = (int)(item % S);
-		const int r = row_of_tile(V.tile_off, nrows, g);
-		const int t = g - V.tile_off[r];
-		const int nt = V.fcnt[r] - t * T;
-		if (nt > SPARSE_TARGETS) continue; // k_sums32's item
-		int row;
-		double phi;
-		M.get(r, row, phi);
-		sums32_sparse(spos32, Np, roots + (size_t)row * R, V.fl + (size_t)r * R + (size_t)t * T, nt, s * Lt / S,
-			(s + 1) * Lt / S, cluster_d2, V.part + (size_t)item * T, nclus);
-	}
-	nclus = wave_sum(nclus);
-	if ((threadIdx.x & 63) == 0 && nclus) atomicAdd(stat_slot(stats, C_CLUSTER), nclus);
-}
-
 // K4 (ccf_functions.cuh:241-242 and 379-389)
 __global__ __launch_bounds__(BLOCK) void k_finalize(double theta, double2* __restrict__ roots, RowMap M, SweepView V,
 	unsigned long long* __restrict__ stats)
@@ -1021,10 +1031,12 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(double theta, double2* __res
 constexpr int TAIL_BLOCK = 512;
 constexpr int TAIL_LDS = 128;
 constexpr int TAIL_DEFER = 64; // roots outside the tree per sweep taken by the whole workgroup
-// sweeps of a continuation step before the tail launch: the second sweep
-// still lists hundreds of roots per row, which the whole device takes
-// faster than one workgroup per row
-constexpr int TAIL_AT = 2;
+// sweeps of a continuation step before the tail launch: the first sweep
+// evaluates every root of every row, which takes the whole device; after it
+// a row has a few per cent of its roots left, mostly ending by Newton steps,
+// which one workgroup per row finishes while the other group of branches'
+// first sweep fills the device
+constexpr int TAIL_AT = 1;
 
 // k_tail: the outcome of root a's evaluation w, as in k_witt: listed for
 // the sums (entry k of the listed roots: LDS for k < TAIL_LDS, else the
@@ -1148,14 +1160,20 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_tail(LensView L, double2* __rest
 			__syncthreads();
 		}
 		const int nf = s_nf;
-		for (int g = wv; g < nf; g += NW)
+		// waves per listed root: all of them share the row when few roots
+		// are listed (a power of two, lanes strided over the row)
+		const int wpt = nf >= NW ? 1 : (nf >= NW / 2 ? 2 : (nf >= NW / 4 ? 4 : NW));
+		const int sub = wv % wpt, lanes = 64 * wpt, ln = sub * 64 + lane;
+		for (int g0 = 0; g0 < nf; g0 += NW / wpt)
 		{
-			const int a = g < TAIL_LDS ? s_fl[g] : gfl[g];
+			const int g = g0 + wv / wpt;
+			const bool live = g < nf;
+			const int a = live ? (g < TAIL_LDS ? s_fl[g] : gfl[g]) : 0;
 			const double2 zg = xrow[a];
 			const float fx = (float)zg.x, fy = (float)zg.y;
 			float ax = 0, ay = 0, bx = 0, by = 0;
 			double cx = 0, cy = 0;
-			for (int q = lane; q < Np; q += 64)
+			for (int q = live ? ln : Np; q < Np; q += lanes)
 			{
 				const float2 w = spos32[q];
 				const float dx = fx - w.x, dy = fy - w.y;
@@ -1163,7 +1181,7 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_tail(LensView L, double2* __rest
 				ax = fmaf(dx, iv, ax);
 				ay = fmaf(dy, iv, ay);
 			}
-			for (int q = lane; q < R; q += 64)
+			for (int q = live ? ln : R; q < R; q += lanes)
 			{
 				const double2 wd = xrow[q];
 				const float dx = fx - (float)wd.x, dy = fy - (float)wd.y;
@@ -1190,8 +1208,25 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_tail(LensView L, double2* __rest
 				vx += __shfl_xor(vx, o, 64);
 				vy += __shfl_xor(vy, o, 64);
 			}
+			if (wpt > 1)
+			{
+				// the waves of a root add their sums in wave order
+				if (lane == 0)
+				{
+					s_red[wv][0] = vx;
+					s_red[wv][1] = vy;
+				}
+				__syncthreads();
+				if (lane == 0 && sub == 0)
+					for (int k = 1; k < wpt; k++)
+					{
+						vx += s_red[wv + k][0];
+						vy += s_red[wv + k][1];
+					}
+				__syncthreads();
+			}
 			// the whole Aberth sum: the root part (conjugated) and the terms formed per root
-			if (lane == 0)
+			if (lane == 0 && sub == 0 && live)
 			{
 				double2* e = g < TAIL_LDS ? &s_ex[g] : &gex[g];
 				*e = make_double2(vx + e->x, -vy + e->y);
@@ -1500,15 +1535,23 @@ inline bool sweep_check(int done, int cap, int first)
 	return done < first + 3 || done % 4 == 0;
 }
 
+// pinned host slot of the root-sum FMM row counts of step j's first two
+// sweeps for a group of branches (two slots per group, by the parity of j)
+inline int* fmm_lag_slot(ccf_ctx* ctx, int group, int j)
+{
+	return ctx->host_counts + ctx->max_rows + 2 + 4 * group + 2 * (j & 1);
+}
+
 // The first fmm_sweeps sweeps may take the root sums of their dense rows
 // from the FMM pass (when the buffers carry a root grid). before_tail, when
 // set, runs once the tail_at sweeps are queued, before the tail launch.
 bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int nrows, int cap, unsigned fmm_mask,
-	int first_check = 3, int tail_at = 0, const std::function<bool()>& before_tail = {})
+	int first_check = 3, int tail_at = 0, const std::function<bool()>& before_tail = {}, hipStream_t stream = nullptr,
+	int group = 0)
 {
 	ctx->phase_sweeps = cap;
 	const LensView L = ctx->lens();
-	hipStream_t s = ctx->stream;
+	hipStream_t s = stream ? stream : ctx->stream;
 	const int R = M.R;
 	const bool f32 = ctx->P.precision == 1;
 	const bool fmm_pole = f32 && ctx->tree.order > 0;
@@ -1523,10 +1566,13 @@ bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int
 	}
 	const dim3 g_rows(grid(R), nrows);
 	const dim3 g_witt(xcd_round(grid(R)), nrows);
+	bool fmm_any = false; // a sweep of this phase launched the root-sum FMM pass
 	for (int it = 0; it < cap; it++)
 	{
 		const SweepView V = view(B, it & 1);
-		const dim3 g_far(WITT_FAR_BLOCKS, nrows);
+		const bool fmm = B.grid.order > 0 && it < 32 && ((fmm_mask >> it) & 1u);
+		fmm_any = fmm_any || fmm;
+		const PlanArgs plan{nrows, Np + R, sl.target, fmm ? root_fmm_min(R) : 0, it};
 		ccf::Timer tw;
 		if (ctx->P.timing)
 		{
@@ -1534,23 +1580,22 @@ bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int
 			tw.b = ctx->next_event();
 			if (tw.a) (void)hipEventRecord(tw.a, s);
 		}
+		const dim3 g_far(WITT_FAR_BLOCKS, nrows);
 		if (f32)
 		{
 			k_witt<true><<<g_witt, BLOCK, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, fmm_pole ? 1 : 0);
-			k_witt_far<true><<<g_far, WITT_FAR_THREADS, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, fmm_pole ? 1 : 0);
+			k_witt_far<true><<<g_far, WITT_FAR_THREADS, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, fmm_pole ? 1 : 0, plan);
 		}
 		else
 		{
 			k_witt<false><<<g_witt, BLOCK, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, 0);
-			k_witt_far<false><<<g_far, WITT_FAR_THREADS, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, 0);
+			k_witt_far<false><<<g_far, WITT_FAR_THREADS, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, 0, plan);
 		}
 		if (ctx->P.timing && tw.a && tw.b)
 		{
 			(void)hipEventRecord(tw.b, s);
 			ctx->witt_timers.push_back(tw);
 		}
-		const bool fmm = B.grid.order > 0 && it < 32 && ((fmm_mask >> it) & 1u);
-		k_plan<<<1, 1024, 0, s>>>(V, nrows, Np + R, sl.target, fmm ? root_fmm_min(R) : 0, it);
 		ccf::Timer tm;
 		if (ctx->P.timing)
 		{
@@ -1566,9 +1611,6 @@ bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int
 			if (sl.rpt == 1) k_sums32<1><<<sl.grid, BLOCK, 0, s>>>(ctx->spos32, Np, roots, M, V, nrows, cd2, ctx->counters);
 			else if (sl.rpt == 4) k_sums32<4><<<sl.grid, BLOCK, 0, s>>>(ctx->spos32, Np, roots, M, V, nrows, cd2, ctx->counters);
 			else k_sums32<2><<<sl.grid, BLOCK, 0, s>>>(ctx->spos32, Np, roots, M, V, nrows, cd2, ctx->counters);
-			if (sl.rpt == 1) k_sums32_sparse<1><<<sl.grid, BLOCK, 0, s>>>(ctx->spos32, Np, roots, M, V, nrows, cd2, ctx->counters);
-			else if (sl.rpt == 4) k_sums32_sparse<4><<<sl.grid, BLOCK, 0, s>>>(ctx->spos32, Np, roots, M, V, nrows, cd2, ctx->counters);
-			else k_sums32_sparse<2><<<sl.grid, BLOCK, 0, s>>>(ctx->spos32, Np, roots, M, V, nrows, cd2, ctx->counters);
 		}
 		else
 		{
@@ -1592,12 +1634,22 @@ bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int
 		{
 			if (ctx->host_counts && nrows <= ctx->max_rows)
 			{
-				hipEvent_t& e = ctx->fmm_rows_event[M.j & 1];
-				if (!e && !CCF_HIP(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming))) return false;
-				if (!CCF_HIP(ctx, hipMemcpyAsync(ctx->host_counts + ctx->max_rows + 2 + 2 * (M.j & 1), B.plan + 2, 2 * sizeof(int),
-						hipMemcpyDeviceToHost, s)) ||
-					!CCF_HIP(ctx, hipEventRecord(e, s)))
-					return false;
+				if (fmm_any)
+				{
+					hipEvent_t& e = ctx->fmm_rows_event[2 * group + (M.j & 1)];
+					if (!e && !CCF_HIP(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming))) return false;
+					if (!CCF_HIP(ctx, hipMemcpyAsync(fmm_lag_slot(ctx, group, M.j), B.plan + 2, 2 * sizeof(int),
+							hipMemcpyDeviceToHost, s)) ||
+						!CCF_HIP(ctx, hipEventRecord(e, s)))
+						return false;
+				}
+				else
+				{
+					// no pass ran: its row counts are 0 (the slot's previous copy,
+					// two steps back, was waited for when this step was planned)
+					int* slot = fmm_lag_slot(ctx, group, M.j);
+					slot[0] = slot[1] = 0;
+				}
 			}
 			if (before_tail && !before_tail()) return false;
 			const SweepView Vn = view(B, (it + 1) & 1);
@@ -1655,12 +1707,51 @@ bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int
 	return true;
 }
 
-bool alloc_root_fmm(ccf_ctx* ctx, SweepBufs& b)
+// The sweep buffers of `rows` rows of R roots (and, when the root-sum FMM
+// applies, its grid over the field): take(p, count) provides each array of
+// count elements (from the context's device pool, or any other provider of
+// device arrays). exact_items sizes the split-K partials for the plan
+// plan_sums can make for these terms (at most MAX_SLICES slices of at least
+// MIN_SLICE terms per tile, ceil(target / tiles) wanted); otherwise for any
+// slice count up to the target.
+template <class Take>
+bool root_layout(ccf_ctx* ctx, SweepBufs& b, Take&& take);
+
+template <class Take>
+bool sweep_layout(ccf_ctx* ctx, SweepBufs& b, int rows, int R, bool exact_items, Take&& take)
+{
+	const bool f32 = ctx->P.precision == 1;
+	const SumsLaunch sl = sums_launch(ctx, f32);
+	b.rows = rows;
+	b.R = R;
+	b.tile = BLOCK * sl.rpt;
+	// items per sweep <= max(tiles, target) + tiles
+	const long long tiles_max = (long long)rows * ((R + b.tile - 1) / b.tile);
+	b.max_items = (int)(tiles_max + sl.target + tiles_max);
+	if (exact_items)
+	{
+		const int Np = (f32 && ctx->tree.order > 0) ? 0 : ctx->num_stars;
+		const long long scap = std::min(MAX_SLICES, std::max(1, (Np + R) / MIN_SLICE));
+		b.max_items = (int)(std::min(tiles_max * scap, tiles_max + sl.target) + tiles_max);
+	}
+	const size_t list = (size_t)rows * R;
+	if (!(take(b.act, 2 * list) && take(b.cnt, 2 * (size_t)rows) && take(b.fl, list) && take(b.fcnt, 2 * (size_t)rows) &&
+			take(b.dl, list) && take(b.dcnt, 2 * (size_t)rows) && take(b.f0, list) && take(b.f1, list) && take(b.ex, list) &&
+			take(b.tile_off, (size_t)rows + 1) && take(b.plan, 4) && take(b.ticket, 1) &&
+			take(b.part, (size_t)b.max_items * b.tile)))
+		return false;
+	if (ctx->P.root_fmm != 1 || R < ROOT_FMM_MIN_ROOTS) return true;
+	return root_layout(ctx, b, take);
+}
+
+// the root-sum FMM's arrays of b.rows rows of b.R roots and its grid over
+// the field (with a tenth of margin, as the star tree's root node,
+// ccf.cuh:738-746)
+template <class Take>
+bool root_layout(ccf_ctx* ctx, SweepBufs& b, Take&& take)
 {
 	const int rows = b.rows, R = b.R;
 	const size_t list = (size_t)rows * R;
-	// root-sum FMM: grid over the field (with a tenth of margin, as the
-	// star tree's root node, ccf.cuh:738-746)
 	double hx, hy;
 	if (ctx->rectangular)
 	{
@@ -1679,16 +1770,19 @@ bool alloc_root_fmm(ccf_ctx* ctx, SweepBufs& b)
 	b.grid.bin_row = bin_row_lds(b.grid.nleaf, R) + BIN_STATIC_LDS <= (size_t)lds;
 	const RootGrid& g = b.grid;
 	const size_t ce = (size_t)rows * g.ncells * (g.order + 1);
-	return CCF_HIP(ctx, ctx->dmalloc(&b.slot, list * sizeof(int))) && CCF_HIP(ctx, ctx->dmalloc(&b.rmode, rows * sizeof(int))) &&
-		CCF_HIP(ctx, ctx->dmalloc(&b.rsum, list * sizeof(double2))) &&
-		CCF_HIP(ctx, ctx->dmalloc(&b.ccnt, (size_t)rows * g.ncells * sizeof(int))) &&
-		CCF_HIP(ctx, ctx->dmalloc(&b.cstart, (size_t)rows * (g.nleaf + 1) * sizeof(int))) &&
-		CCF_HIP(ctx, ctx->dmalloc(&b.ccur, (size_t)rows * g.nleaf * sizeof(int))) && CCF_HIP(ctx, ctx->dmalloc(&b.cell, list * sizeof(int))) &&
-		CCF_HIP(ctx, ctx->dmalloc(&b.sidx, list * sizeof(int))) && CCF_HIP(ctx, ctx->dmalloc(&b.spos, list * sizeof(double2))) &&
-		CCF_HIP(ctx, ctx->dmalloc(&b.ocnt, rows * sizeof(int))) && CCF_HIP(ctx, ctx->dmalloc(&b.oidx, list * sizeof(int))) &&
-		CCF_HIP(ctx, ctx->dmalloc(&b.mp, ce * sizeof(double2))) && CCF_HIP(ctx, ctx->dmalloc(&b.lc, ce * sizeof(double2))) &&
-		CCF_HIP(ctx, hipMemsetAsync(b.slot, 0xff, list * sizeof(int), ctx->stream)) &&
-		CCF_HIP(ctx, hipMemsetAsync(b.rmode, 0, rows * sizeof(int), ctx->stream)) &&
+	return take(b.slot, list) && take(b.rmode, (size_t)rows) && take(b.rsum, list) && take(b.ccnt, (size_t)rows * g.ncells) &&
+		take(b.cstart, (size_t)rows * (g.nleaf + 1)) && take(b.ccur, (size_t)rows * g.nleaf) && take(b.cell, list) &&
+		take(b.sidx, list) && take(b.spos, list) && take(b.ocnt, (size_t)rows) && take(b.oidx, list) && take(b.mp, ce) &&
+		take(b.lc, ce);
+}
+
+// the root-sum FMM's arrays alone (ccf_root_sums), from the context's pool
+bool alloc_root_fmm(ccf_ctx* ctx, SweepBufs& b)
+{
+	auto take = [ctx](auto*& p, size_t count) { return CCF_HIP(ctx, ctx->dmalloc(&p, count * sizeof(*p))); };
+	return root_layout(ctx, b, take) &&
+		CCF_HIP(ctx, hipMemsetAsync(b.slot, 0xff, (size_t)b.rows * b.R * sizeof(int), ctx->stream)) &&
+		CCF_HIP(ctx, hipMemsetAsync(b.rmode, 0, b.rows * sizeof(int), ctx->stream)) &&
 		CCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
 }
 
@@ -1697,7 +1791,8 @@ bool alloc_root_fmm(ccf_ctx* ctx, SweepBufs& b)
 void free_sweep(ccf_ctx* ctx, SweepBufs& b)
 {
 	auto F = [ctx](auto*& p) { ctx->dfree(p); p = nullptr; };
-	F(b.act); F(b.cnt); F(b.fl); F(b.fcnt); F(b.dl); F(b.dcnt); F(b.f0); F(b.f1); F(b.ex); F(b.tile_off); F(b.plan); F(b.part);
+	F(b.act); F(b.cnt); F(b.fl); F(b.fcnt); F(b.dl); F(b.dcnt); F(b.f0); F(b.f1); F(b.ex); F(b.tile_off); F(b.plan); F(b.ticket);
+	F(b.part);
 	F(b.slot); F(b.rmode); F(b.rsum); F(b.ccnt); F(b.cstart); F(b.ccur); F(b.cell); F(b.sidx); F(b.spos); F(b.ocnt);
 	F(b.oidx); F(b.mp); F(b.lc);
 	b = SweepBufs{};
@@ -1706,27 +1801,18 @@ void free_sweep(ccf_ctx* ctx, SweepBufs& b)
 bool alloc_sweep(ccf_ctx* ctx, SweepBufs& b, int rows, int R)
 {
 	free_sweep(ctx, b);
-	const SumsLaunch sl = sums_launch(ctx, ctx->P.precision == 1);
-	b.rows = rows;
-	b.R = R;
-	b.tile = BLOCK * sl.rpt;
-	// items per sweep <= max(tiles, target) + tiles
-	const long long tiles_max = (long long)rows * ((R + b.tile - 1) / b.tile);
-	b.max_items = (int)(tiles_max + sl.target + tiles_max);
+	auto take = [ctx](auto*& p, size_t count) { return CCF_HIP(ctx, ctx->dmalloc(&p, count * sizeof(*p))); };
+	if (!sweep_layout(ctx, b, rows, R, false, take)) return false;
 	const size_t list = (size_t)rows * R;
-	const bool ok = CCF_HIP(ctx, ctx->dmalloc(&b.act, 2 * list * sizeof(int))) && CCF_HIP(ctx, ctx->dmalloc(&b.cnt, 2 * rows * sizeof(int))) &&
-		CCF_HIP(ctx, ctx->dmalloc(&b.fl, list * sizeof(int))) && CCF_HIP(ctx, ctx->dmalloc(&b.fcnt, 2 * rows * sizeof(int))) &&
-		CCF_HIP(ctx, ctx->dmalloc(&b.dl, list * sizeof(int))) && CCF_HIP(ctx, ctx->dmalloc(&b.dcnt, 2 * rows * sizeof(int))) &&
-		CCF_HIP(ctx, hipMemsetAsync(b.dcnt, 0, 2 * rows * sizeof(int), ctx->stream)) &&
-		CCF_HIP(ctx, ctx->dmalloc(&b.f0, list * sizeof(double2))) && CCF_HIP(ctx, ctx->dmalloc(&b.f1, list * sizeof(double2))) &&
-		CCF_HIP(ctx, ctx->dmalloc(&b.ex, list * sizeof(double2))) && CCF_HIP(ctx, ctx->dmalloc(&b.tile_off, (rows + 1) * sizeof(int))) &&
-		CCF_HIP(ctx, ctx->dmalloc(&b.plan, 4 * sizeof(int))) &&
-		CCF_HIP(ctx, ctx->dmalloc(&b.part, (size_t)b.max_items * b.tile * sizeof(double2))) &&
-		CCF_HIP(ctx, hipMemsetAsync(b.cnt, 0, 2 * rows * sizeof(int), ctx->stream)) &&
-		CCF_HIP(ctx, hipMemsetAsync(b.fcnt, 0, 2 * rows * sizeof(int), ctx->stream));
-	if (!ok) return false;
-	if (ctx->P.root_fmm != 1 || R < ROOT_FMM_MIN_ROOTS) return true;
-	return alloc_root_fmm(ctx, b);
+	if (!CCF_HIP(ctx, hipMemsetAsync(b.dcnt, 0, 2 * rows * sizeof(int), ctx->stream)) ||
+		!CCF_HIP(ctx, hipMemsetAsync(b.cnt, 0, 2 * rows * sizeof(int), ctx->stream)) ||
+		!CCF_HIP(ctx, hipMemsetAsync(b.fcnt, 0, 2 * rows * sizeof(int), ctx->stream)) ||
+		!CCF_HIP(ctx, hipMemsetAsync(b.ticket, 0, sizeof(int), ctx->stream)))
+		return false;
+	if (!b.slot) return true;
+	return CCF_HIP(ctx, hipMemsetAsync(b.slot, 0xff, list * sizeof(int), ctx->stream)) &&
+		CCF_HIP(ctx, hipMemsetAsync(b.rmode, 0, rows * sizeof(int), ctx->stream)) &&
+		CCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
 }
 
 bool launch_phase(ccf_ctx* ctx)
@@ -1751,8 +1837,12 @@ bool alloc_solver(ccf_ctx* ctx)
 			CCF_HIP(ctx, hipMemcpyAsync(ctx->dseeds, ctx->seeds.data(), (size_t)R * sizeof(double2), hipMemcpyHostToDevice, ctx->stream)) &&
 			CCF_HIP(ctx, hipMemsetAsync(ctx->ccs_init, 0, pts * sizeof(double2), ctx->stream)) &&
 			CCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+	// the mixed form's continuation runs CONT_GROUPS groups of branches
+	// side by side: the first group's rows in sw, the others' in sw2
+	const bool groups = ctx->P.precision == 1 && nb >= CONT_GROUPS;
 	return CCF_HIP(ctx, ctx->dmalloc(&ctx->ccs_init, pts * sizeof(double2))) && alloc_sweep(ctx, ctx->sw, ctx->max_rows, R) &&
-		CCF_HIP(ctx, hipHostMalloc((void**)&ctx->host_counts, (ctx->max_rows + 6) * sizeof(int), hipHostMallocDefault)) &&
+		(!groups || alloc_sweep(ctx, ctx->sw2, 2 * (nb - nb / CONT_GROUPS), R)) &&
+		CCF_HIP(ctx, hipHostMalloc((void**)&ctx->host_counts, (ctx->max_rows + 2 + 4 * CONT_GROUPS) * sizeof(int), hipHostMallocDefault)) &&
 		CCF_HIP(ctx, ctx->dmalloc(&ctx->phase, (size_t)(np + nb) * sizeof(double2))) && launch_phase(ctx) &&
 		CCF_HIP(ctx, ctx->dmalloc(&ctx->counters, STAT_WORDS * sizeof(unsigned long long))) &&
 		CCF_HIP(ctx, hipMemsetAsync(ctx->ccs_init, 0, pts * sizeof(double2), ctx->stream)) &&
@@ -1834,8 +1924,8 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 	hipStream_t es = ctx->err_stream;
 	int nfar = 0;
 	bool first_pass = true;
-	auto errors = [&](const RowMap& Me, int nrows) -> bool {
-		if (!CCF_HIP(ctx, hipEventRecord(ctx->err_event, s)) || !CCF_HIP(ctx, hipStreamWaitEvent(es, ctx->err_event, 0)))
+	auto errors = [&](const RowMap& Me, int nrows, hipStream_t after) -> bool {
+		if (!CCF_HIP(ctx, hipEventRecord(ctx->err_event, after)) || !CCF_HIP(ctx, hipStreamWaitEvent(es, ctx->err_event, 0)))
 			return false;
 		k_zero_count<<<1, 1, 0, es>>>(far.count, first_pass ? 1 : 0);
 		first_pass = false;
@@ -1845,45 +1935,83 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 			wc ? ctx->caustics : nullptr, wm ? ctx->mu_scales : nullptr, far);
 		return CCF_LAUNCH(ctx, "find_errors_kernel");
 	};
-	// the rows of the previous step (the centres before the first) take
-	// their pass while this step's tail runs, a launch of one workgroup per
-	// row; without a tail, after the step
-	auto prev_rows = [&](int j) { return RowMap{R, np, nb, j - 1, j == 1 ? 1 : 2, 0.0, ctx->phase}; };
-	bool prev_done = false;
+	// The mixed form (no read-backs) runs its branches in CONT_GROUPS groups,
+	// each on its own stream with its own sweep buffers: a group's steps are
+	// a chain of short, latency-bound launches (the later sweeps, the tail of
+	// one workgroup per row) that leaves most of the device idle, which the
+	// other group's first sweep over all of its roots fills. The groups share
+	// only read-only inputs, the statistics (integer atomics) and the error
+	// pass's stream; each writes its own rows.
+	const int G = (tailed && nb >= CONT_GROUPS && ctx->sw2.act) ? CONT_GROUPS : 1;
+	int gb[CONT_GROUPS + 1] = {};
+	hipStream_t gs[CONT_GROUPS] = {};
+	SweepBufs* gsw[CONT_GROUPS] = {};
+	for (int g = 0; g <= G; g++) gb[g] = g * nb / G;
+	gs[0] = s;
+	gsw[0] = &ctx->sw;
+	if (G > 1)
+	{
+		if (!ctx->group_stream && !CCF_HIP(ctx, hipStreamCreateWithFlags(&ctx->group_stream, hipStreamNonBlocking))) return false;
+		if (!ctx->group_event && !CCF_HIP(ctx, hipEventCreateWithFlags(&ctx->group_event, hipEventDisableTiming))) return false;
+		// the other group starts after everything queued before the continuation
+		if (!CCF_HIP(ctx, hipEventRecord(ctx->group_event, s)) || !CCF_HIP(ctx, hipStreamWaitEvent(ctx->group_stream, ctx->group_event, 0)))
+			return false;
+		for (int g = 1; g < G; g++)
+		{
+			gs[g] = ctx->group_stream;
+			gsw[g] = &ctx->sw2;
+		}
+	}
+	// the rows of a group's previous step (the centres before the first)
+	// take their pass while this step's tail runs, a launch of one workgroup
+	// per row; without a tail, after the step
+	auto prev_rows = [&](int j, int g) { return RowMap{R, np, nb, j - 1, j == 1 ? 1 : 2, 0.0, ctx->phase, gb[g]}; };
+	auto prev_count = [&](int j, int g) { return (j == 1 ? 1 : 2) * (gb[g + 1] - gb[g]); };
 	for (int j = 1; j <= steps; j++)
 	{
-		const RowMap M{R, np, nb, j, 2, 0.0, ctx->phase};
-		unsigned mask = 0;
-		if (tailed)
+		for (int g = 0; g < G; g++)
 		{
-			int rows2[2] = {-1, -1};
-			if (j >= 3 && ctx->fmm_rows_event[j & 1])
+			const int nbg = gb[g + 1] - gb[g];
+			const RowMap M{R, np, nb, j, 2, 0.0, ctx->phase, gb[g]};
+			unsigned mask = 0;
+			if (tailed)
 			{
-				if (!CCF_HIP(ctx, hipEventSynchronize(ctx->fmm_rows_event[j & 1]))) return false;
-				rows2[0] = ctx->host_counts[ctx->max_rows + 2 + 2 * (j & 1)];
-				rows2[1] = ctx->host_counts[ctx->max_rows + 3 + 2 * (j & 1)];
+				int rows2[2] = {-1, -1};
+				hipEvent_t e = ctx->fmm_rows_event[2 * g + (j & 1)];
+				if (j >= 3 && e)
+				{
+					if (!CCF_HIP(ctx, hipEventSynchronize(e))) return false;
+					rows2[0] = fmm_lag_slot(ctx, g, j)[0];
+					rows2[1] = fmm_lag_slot(ctx, g, j)[1];
+				}
+				// the first steps (a predictor without history) can list
+				// thousands of roots a row; later the pass runs while the
+				// step before the previous one gave it rows. No periodic
+				// probe: its launches over empty rows cost more than the
+				// direct FP32 sums of a continuation sweep, O(listed R),
+				// that it could replace
+				for (int q = 0; q < ROOT_FMM_STEP_SWEEPS; q++)
+					if (j < 3 || rows2[q] > 0) mask |= 1u << q;
 			}
-			for (int q = 0; q < ROOT_FMM_STEP_SWEEPS; q++)
-				if (rows2[q] != 0 || j % ROOT_FMM_PROBE == 1) mask |= 1u << q;
+			else
+				for (int q = 0; q < ROOT_FMM_STEP_SWEEPS; q++)
+					if (prev[q] != 0 || j % ROOT_FMM_PROBE == 1) mask |= 1u << q;
+			// the exact smooth sheets are discontinuous at the field's edge
+			// (boxcar / heaviside terms), where extrapolated roots could cross
+			// onto another branch: those keep the previous step's roots
+			const int order = ctx->P.approx ? ctx->P.predictor : 0;
+			k_prepare<<<dim3(grid(R), 2 * nbg), BLOCK, 0, gs[g]>>>(ctx->ccs_init, M, *gsw[g], order, ctx->dorder);
+			if (!CCF_LAUNCH(ctx, "prepare_roots_kernel")) return false;
+			bool prev_done = false;
+			const auto at_tail = [&]() {
+				prev_done = true;
+				return errors(prev_rows(j, g), prev_count(j, g), gs[g]);
+			};
+			if (!run_sweeps(ctx, *gsw[g], ctx->ccs_init, M, 2 * nbg, cap, mask, tailed ? cap + 1 : first,
+					tailed ? (j < 3 ? TAIL_AT + 1 : TAIL_AT) : 0, at_tail, gs[g], g))
+				return false;
+			if (!prev_done && !errors(prev_rows(j, g), prev_count(j, g), gs[g])) return false;
 		}
-		else
-			for (int q = 0; q < ROOT_FMM_STEP_SWEEPS; q++)
-				if (prev[q] != 0 || j % ROOT_FMM_PROBE == 1) mask |= 1u << q;
-		// the exact smooth sheets are discontinuous at the field's edge
-		// (boxcar / heaviside terms), where extrapolated roots could cross
-		// onto another branch: those keep the previous step's roots
-		const int order = ctx->P.approx ? ctx->P.predictor : 0;
-		k_prepare<<<dim3(grid(R), 2 * nb), BLOCK, 0, s>>>(ctx->ccs_init, M, ctx->sw, order, ctx->dorder);
-		if (!CCF_LAUNCH(ctx, "prepare_roots_kernel")) return false;
-		prev_done = false;
-		const auto at_tail = [&]() {
-			prev_done = true;
-			return errors(prev_rows(j), j == 1 ? nb : 2 * nb);
-		};
-		if (!run_sweeps(ctx, ctx->sw, ctx->ccs_init, M, 2 * nb, cap, mask, tailed ? cap + 1 : first,
-				tailed ? (j < 3 ? TAIL_AT + 1 : TAIL_AT) : 0, at_tail))
-			return false;
-		if (!prev_done && !errors(prev_rows(j), j == 1 ? nb : 2 * nb)) return false;
 		prev[0] = ctx->fmm_rows_sweep[0];
 		prev[1] = ctx->fmm_rows_sweep[1];
 		// a step checked first at k can end no earlier than k: every
@@ -1897,7 +2025,12 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 			std::cout << "] " << j * 100 / steps << " %" << std::flush;
 		}
 	}
-	if (!errors(prev_rows(steps + 1), steps == 0 ? nb : 2 * nb)) return false;
+	for (int g = 0; g < G; g++)
+		if (!errors(prev_rows(steps + 1, g), steps == 0 ? gb[g + 1] - gb[g] : 2 * (gb[g + 1] - gb[g]), gs[g])) return false;
+	// the stages after the continuation run on the context's stream
+	if (G > 1 && (!CCF_HIP(ctx, hipEventRecord(ctx->group_event, ctx->group_stream)) ||
+					 !CCF_HIP(ctx, hipStreamWaitEvent(s, ctx->group_event, 0))))
+		return false;
 	if (!CCF_HIP(ctx, hipStreamSynchronize(s))) return false;
 	auto t1 = std::chrono::steady_clock::now();
 	ctx->info.t_ccs = std::chrono::duration<double>(t1 - t0).count();
