@@ -155,6 +155,7 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         "ccf_last_error": (ctypes.c_char_p, [vp]),
         "ccf_eval_points": (ctypes.c_int, [vp, dp, ctypes.c_int, ctypes.c_double, dp]),
         "ccf_aberth_sweep": (ctypes.c_int, [vp, dp, ctypes.c_int, ctypes.c_double, dp]),
+        "ccf_aberth_sweep_noalloc": (ctypes.c_int, [vp, dp, ctypes.c_int, ctypes.c_double, dp]),
         "ccf_root_sums": (ctypes.c_int, [vp, dp, ctypes.c_int, dp]),
         "ccf_batch_create": (vp, []),
         "ccf_batch_destroy": (None, [vp]),
@@ -304,6 +305,14 @@ class Context:
         out = np.empty_like(r)
         self._check(self.lib.ccf_aberth_sweep(self.ptr, _dptr(r.view(np.float64)), r.size, phi,
                                               _dptr(out.view(np.float64))), "ccf_aberth_sweep")
+        return out
+
+    def aberth_sweep_noalloc(self, roots, phi: float) -> np.ndarray:
+        """aberth_sweep without device allocations (the library's fixed scratch array)"""
+        r = np.ascontiguousarray(np.asarray(roots, dtype=np.complex128).ravel())
+        out = np.empty_like(r)
+        self._check(self.lib.ccf_aberth_sweep_noalloc(self.ptr, _dptr(r.view(np.float64)), r.size, phi,
+                                                      _dptr(out.view(np.float64))), "ccf_aberth_sweep_noalloc")
         return out
 
     def root_sums(self, roots) -> np.ndarray:

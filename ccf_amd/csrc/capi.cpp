@@ -633,7 +633,8 @@ bool device_init(ccf_ctx* c)
 	if (!CCF_HIP(c, hipSetDevice(0))) return false;
 	if (!c->stream && !CCF_HIP(c, hipStreamCreate(&c->stream))) return false;
 	if (!CCF_HIP(c, hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, 0))) return false;
-	return true;
+	// the no-allocation sweep's scratch array on this device (once per device)
+	return ccf::scratch_setup(c);
 }
 
 bool need(ccf_ctx* c, bool ok, const char* what)
@@ -1058,6 +1059,14 @@ int ccf_aberth_sweep(ccf_ctx* c, const double* roots, int n, double phi, double*
 	return guarded(c, [&]() -> int {
 		if (!c || !need(c, c->prepared, "ccf_prepare")) return 1;
 		return ccf::aberth_sweep_dev(c, roots, n, phi, out) ? 0 : 1;
+	});
+}
+
+int ccf_aberth_sweep_noalloc(ccf_ctx* c, const double* roots, int n, double phi, double* out)
+{
+	return guarded(c, [&]() -> int {
+		if (!c || !need(c, c->prepared, "ccf_prepare")) return 1;
+		return ccf::aberth_sweep_noalloc_dev(c, roots, n, phi, out) ? 0 : 1;
 	});
 }
 

@@ -330,6 +330,10 @@ bool generate_stars_dev(ccf_ctx* ctx);
 bool eval_points_dev(ccf_ctx* ctx, const double* z, int n, double phi, double* out);
 bool aberth_sweep_dev(ccf_ctx* ctx, const double* roots, int n, double phi, double* out);
 bool root_sums_dev(ccf_ctx* ctx, const double* roots, int n, double* out);
+// the no-allocation sweep: its scratch array's address on the current
+// device (once per device), and the sweep over slices of that array
+bool scratch_setup(ccf_ctx* ctx);
+bool aberth_sweep_noalloc_dev(ccf_ctx* ctx, const double* roots, int n, double phi, double* out);
 // batch.hip
 bool batch_build(ccf_batch* b, int verbose);
 bool batch_solve(ccf_batch* b, int verbose);

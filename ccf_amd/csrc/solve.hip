@@ -37,6 +37,8 @@
 #include <cstring>
 #include <functional>
 #include <iostream>
+#include <mutex>
+#include <string>
 #include <vector>
 
 namespace ccf
@@ -1709,8 +1711,8 @@ bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int
 
 // The sweep buffers of `rows` rows of R roots (and, when the root-sum FMM
 // applies, its grid over the field): take(p, count) provides each array of
-// count elements (from the context's device pool, or any other provider of
-// device arrays). exact_items sizes the split-K partials for the plan
+// count elements, from the context's device pool or from a slice of the
+// fixed scratch array. exact_items sizes the split-K partials for the plan
 // plan_sums can make for these terms (at most MAX_SLICES slices of at least
 // MIN_SLICE terms per tile, ceil(target / tiles) wanted); otherwise for any
 // slice count up to the target.
@@ -2214,6 +2216,134 @@ bool aberth_sweep_dev(ccf_ctx* ctx, const double* roots, int n, double phi, doub
 	ctx->dfree(row);
 	free_sweep(ctx, B);
 	return ok;
+}
+
+// ---------------------------------------------------------------------------
+// One Aberth sweep that allocates nothing (ccf_aberth_sweep_noalloc): the
+// row's staging copy and every buffer of the sweep are slices of `scratch`,
+// a file-scope device array that each device the library runs on holds
+// until the process ends. In host code the array's name is no device
+// address: its address on a device comes from hipGetSymbolAddress, taken
+// once per device with that device current (scratch_setup, called when a
+// context initialises the device), and only that address is handed to
+// kernels and copies. Every call on a device shares that device's array and
+// finds the last call's contents: what must start at a fixed value is set
+// by each call, and calls that overlap on one device race.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace
+{
+constexpr int SCRATCH_DEVICES = 64;
+std::mutex scratch_mu;
+unsigned char* scratch_addr[SCRATCH_DEVICES] = {};
+hipError_t scratch_err[SCRATCH_DEVICES] = {}; // a failed look-up of the address, reported by the sweep
+
+// slices of the scratch array, 256-byte aligned; with a null base it only
+// adds up the bytes the slices take
+struct ScratchCarve
+{
+	unsigned char* base;
+	size_t used = 0;
+	template <class T>
+	bool operator()(T*& p, size_t count)
+	{
+		used = (used + 255) & ~size_t(255);
+		p = base ? reinterpret_cast<T*>(base + used) : nullptr;
+		used += count * sizeof(T);
+		return true;
+	}
+};
+
+__global__ void k_fill_int(int* __restrict__ p, size_t n, int v)
+{
+	const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) p[i] = v;
+}
+
+bool fill_int(ccf_ctx* ctx, int* p, size_t n, int v)
+{
+	k_fill_int<<<(unsigned)((n + BLOCK - 1) / BLOCK), BLOCK, 0, ctx->stream>>>(p, n, v);
+	return CCF_LAUNCH(ctx, "fill kernel");
+}
+} // namespace
+
+bool scratch_setup(ccf_ctx* ctx)
+{
+	int dev = 0;
+	if (!CCF_HIP(ctx, hipGetDevice(&dev))) return false;
+	if (dev < 0 || dev >= SCRATCH_DEVICES) return true; // the no-allocation sweep refuses to run there
+	std::lock_guard<std::mutex> lock(scratch_mu);
+	if (scratch_addr[dev]) return true;
+	void* p = nullptr;
+	const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+	if (e != hipSuccess || !p)
+	{
+		// only the no-allocation sweep needs the array: it reports this
+		scratch_err[dev] = e != hipSuccess ? e : hipErrorInvalidSymbol;
+		(void)hipGetLastError();
+		return true;
+	}
+	scratch_addr[dev] = static_cast<unsigned char*>(p);
+	scratch_err[dev] = hipSuccess;
+	return true;
+}
+
+bool aberth_sweep_noalloc_dev(ccf_ctx* ctx, const double* roots, int n, double phi, double* out)
+{
+	if (n <= 0) return true;
+	int dev = 0;
+	if (!CCF_HIP(ctx, hipGetDevice(&dev))) return false;
+	unsigned char* base = nullptr;
+	hipError_t why = hipSuccess;
+	{
+		std::lock_guard<std::mutex> lock(scratch_mu);
+		if (dev >= 0 && dev < SCRATCH_DEVICES)
+		{
+			base = scratch_addr[dev];
+			why = scratch_err[dev];
+		}
+	}
+	if (!base)
+	{
+		ctx->fail("Error. ccf_aberth_sweep_noalloc: no scratch array on device " + std::to_string(dev) +
+			(why != hipSuccess ? std::string(" (hipGetSymbolAddress: ") + hipGetErrorString(why) + ")" : std::string()) + "\n");
+		return false;
+	}
+	// the bytes first: the row, then the sweep buffers (and the root-sum
+	// FMM's when the context takes it)
+	SweepBufs B;
+	double2* row = nullptr;
+	ScratchCarve size{nullptr};
+	size(row, (size_t)n);
+	if (!sweep_layout(ctx, B, 1, n, true, size)) return false;
+	if (size.used > NOALLOC_SCRATCH_BYTES)
+	{
+		ctx->fail("Error. ccf_aberth_sweep_noalloc: a sweep of " + std::to_string(n) + " roots needs " + std::to_string(size.used) +
+			" bytes of scratch, more than the " + std::to_string(NOALLOC_SCRATCH_BYTES) + " it has\n");
+		return false;
+	}
+	B = SweepBufs{};
+	ScratchCarve carve{base};
+	carve(row, (size_t)n);
+	if (!sweep_layout(ctx, B, 1, n, true, carve)) return false;
+	hipStream_t s = ctx->stream;
+	// the Witt kernel's ticket starts at 0, the slots of listed roots at -1
+	// and the row's FMM mode at 0; k_prepare sets the active list and clears
+	// the counts
+	if (!fill_int(ctx, B.ticket, 1, 0)) return false;
+	if (B.slot && !(fill_int(ctx, B.slot, (size_t)n, -1) && fill_int(ctx, B.rmode, 1, 0))) return false;
+	if (!CCF_HIP(ctx, hipMemcpyAsync(row, roots, (size_t)n * sizeof(double2), hipMemcpyHostToDevice, s))) return false;
+	const RowMap M{n, 2, 1, 0, 0, phi};
+	k_prepare<<<dim3(grid(n), 1), BLOCK, 0, s>>>(row, M, B, 0, nullptr);
+	if (!CCF_LAUNCH(ctx, "prepare")) return false;
+	// no timing events (creating them would allocate)
+	const int timing = ctx->P.timing;
+	ctx->P.timing = 0;
+	const bool ok = run_sweeps(ctx, B, row, M, 1, 1, 1);
+	ctx->P.timing = timing;
+	return ok && CCF_HIP(ctx, hipMemcpyAsync(out, row, (size_t)n * sizeof(double2), hipMemcpyDeviceToHost, s)) &&
+		CCF_HIP(ctx, hipStreamSynchronize(s));
 }
 
 bool root_sums_dev(ccf_ctx* ctx, const double* roots, int n, double* out)

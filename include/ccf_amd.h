@@ -184,6 +184,17 @@ int ccf_eval_points(ccf_ctx* ctx, const double* z, int n, double phi, double* ou
    the device: out[i] = update of roots[i] given all of roots[0..n). */
 int ccf_aberth_sweep(ccf_ctx* ctx, const double* roots, int n, double phi, double* out);
 
+/* ccf_aberth_sweep without allocating: no counterpart in the reference.
+   For callers that cannot allocate device memory (code under HIP graph
+   capture, tight serving loops, harnesses that forbid hipMalloc). The row
+   and every buffer of the sweep live in a fixed scratch array of 32 MiB that
+   the library holds on each device it runs on until the process ends
+   (set up when a context initialises the device); a sweep that needs more
+   (about 30 000 roots; fewer with root_fmm on an elongated field) is
+   refused with an error. Results are those of ccf_aberth_sweep. Calls on
+   one device share the array: calls that overlap on one device race. */
+int ccf_aberth_sweep_noalloc(ccf_ctx* ctx, const double* roots, int n, double phi, double* out);
+
 /* The root sums S_k = sum_{j != k} 1/(z_k - z_j) of n host roots (the root
    part of ccf_functions.cuh:232-239) by the root-sum FMM of a prepared
    context, with its root_fmm_order and root_fmm_leaf, whether root_fmm is
