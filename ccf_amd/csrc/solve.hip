@@ -1541,172 +1541,244 @@ inline bool sweep_check(int done, int cap, int first)
 // sweeps for a group of branches (two slots per group, by the parity of j)
 inline int* fmm_lag_slot(ccf_ctx* ctx, int group, int j)
 {
-	return ctx->host_counts + ctx->max_rows + 2 + 4 * group + 2 * (j & 1);
+	return ctx->host_counts + ctx->max_rows + 2 * CONT_GROUPS + 4 * group + 2 * (j & 1);
 }
 
-// The first fmm_sweeps sweeps may take the root sums of their dense rows
-// from the FMM pass (when the buffers carry a root grid). before_tail, when
-// set, runs once the tail_at sweeps are queued, before the tail launch.
-bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int nrows, int cap, unsigned fmm_mask,
-	int first_check = 3, int tail_at = 0, const std::function<bool()>& before_tail = {}, hipStream_t stream = nullptr,
-	int group = 0)
+// pinned host slot of a read-back's two root-sum FMM row counts for a group
+inline int* fmm_read_slot(ccf_ctx* ctx, int group) { return ctx->host_counts + ctx->max_rows + 2 * group; }
+
+// One group of rows of a phase, solved on its own stream with its own sweep
+// buffers: the rows of M (nrows of them).
+struct SweepGroup
+{
+	SweepBufs* B;
+	RowMap M;
+	int nrows;
+	hipStream_t s;
+	unsigned fmm_mask;  // sweeps that may take the root sums of dense rows from the FMM pass
+	int id = 0;         // its host slots (FMM row counts) and events, one set per group of branches
+	int row0 = 0;       // first slot of its rows' active counts in a read-back
+	bool done = false;  // no root active, or handed to the tail launch
+	bool fmm_any = false;
+};
+
+// `cap` sweeps over the rows of every group (ccf.cuh:929-939 and 1013-1018).
+// Each sweep queues the groups' launches on their own streams, so that one
+// group's short, latency-bound launches run beside the others'. The first
+// sweeps may take the root sums of dense rows from the FMM pass (when the
+// buffers carry a root grid). A read-back (after sweep_check's sweeps)
+// waits for every group still running and stops the groups whose rows have
+// no active root. With tail_at, the sweeps after the first tail_at of each
+// group run in one tail launch, with no read-back; before_tail(g), when
+// set, runs once group g's first tail_at sweeps are queued.
+bool run_sweeps_groups(ccf_ctx* ctx, SweepGroup* gs, int G, double2* roots, int cap, int first_check = 3, int tail_at = 0,
+	const std::function<bool(int)>& before_tail = {})
 {
 	ctx->phase_sweeps = cap;
 	const LensView L = ctx->lens();
-	hipStream_t s = stream ? stream : ctx->stream;
-	const int R = M.R;
 	const bool f32 = ctx->P.precision == 1;
 	const bool fmm_pole = f32 && ctx->tree.order > 0;
 	const int circ_exact = (!ctx->rectangular && !ctx->P.approx) ? 1 : 0;
 	const int Np = fmm_pole ? 0 : ctx->num_stars;
 	const SumsLaunch sl = sums_launch(ctx, f32);
 	ctx->fmm_rows_sweep[0] = ctx->fmm_rows_sweep[1] = -1; // unknown until a read-back
-	if (B.tile != BLOCK * sl.rpt)
+	int rows_total = 0, left = 0;
+	for (int g = 0; g < G; g++)
 	{
-		ctx->fail("Error. sweep buffers were sized for another tile\n");
-		return false;
+		if (gs[g].id < 0 || gs[g].id >= CONT_GROUPS)
+		{
+			ctx->fail("Error. sweep group out of range\n");
+			return false;
+		}
+		left += !gs[g].done;
+		if (gs[g].B->tile != BLOCK * sl.rpt)
+		{
+			ctx->fail("Error. sweep buffers were sized for another tile\n");
+			return false;
+		}
+		gs[g].row0 = rows_total;
+		rows_total += gs[g].nrows;
 	}
-	const dim3 g_rows(grid(R), nrows);
-	const dim3 g_witt(xcd_round(grid(R)), nrows);
-	bool fmm_any = false; // a sweep of this phase launched the root-sum FMM pass
-	for (int it = 0; it < cap; it++)
+	const bool readback = ctx->host_counts && rows_total <= ctx->max_rows;
+	int swept = 0;
+	for (int it = 0; it < cap && left > 0; it++)
 	{
-		const SweepView V = view(B, it & 1);
-		const bool fmm = B.grid.order > 0 && it < 32 && ((fmm_mask >> it) & 1u);
-		fmm_any = fmm_any || fmm;
-		const PlanArgs plan{nrows, Np + R, sl.target, fmm ? root_fmm_min(R) : 0, it};
-		ccf::Timer tw;
-		if (ctx->P.timing)
+		swept = it + 1;
+		for (int g = 0; g < G; g++)
 		{
-			tw.a = ctx->next_event();
-			tw.b = ctx->next_event();
-			if (tw.a) (void)hipEventRecord(tw.a, s);
-		}
-		const dim3 g_far(WITT_FAR_BLOCKS, nrows);
-		if (f32)
-		{
-			k_witt<true><<<g_witt, BLOCK, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, fmm_pole ? 1 : 0);
-			k_witt_far<true><<<g_far, WITT_FAR_THREADS, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, fmm_pole ? 1 : 0, plan);
-		}
-		else
-		{
-			k_witt<false><<<g_witt, BLOCK, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, 0);
-			k_witt_far<false><<<g_far, WITT_FAR_THREADS, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, 0, plan);
-		}
-		if (ctx->P.timing && tw.a && tw.b)
-		{
-			(void)hipEventRecord(tw.b, s);
-			ctx->witt_timers.push_back(tw);
-		}
-		ccf::Timer tm;
-		if (ctx->P.timing)
-		{
-			tm.a = ctx->next_event();
-			tm.b = ctx->next_event();
-			if (tm.a) (void)hipEventRecord(tm.a, s);
-		}
-		if (fmm) root_fmm_pass(B, roots, M, nrows, ctx->counters, s);
-		if (f32)
-		{
-			const double d = CLUSTER_REL * ctx->root_half_length;
-			const float cd2 = (float)(d * d);
-			if (sl.rpt == 1) k_sums32<1><<<sl.grid, BLOCK, 0, s>>>(ctx->spos32, Np, roots, M, V, nrows, cd2, ctx->counters);
-			else if (sl.rpt == 4) k_sums32<4><<<sl.grid, BLOCK, 0, s>>>(ctx->spos32, Np, roots, M, V, nrows, cd2, ctx->counters);
-			else k_sums32<2><<<sl.grid, BLOCK, 0, s>>>(ctx->spos32, Np, roots, M, V, nrows, cd2, ctx->counters);
-		}
-		else
-		{
-			if (sl.rpt == 1) k_sums64<1><<<sl.grid, BLOCK, 0, s>>>(ctx->spos, Np, roots, M, V, nrows);
-			else if (sl.rpt == 4) k_sums64<4><<<sl.grid, BLOCK, 0, s>>>(ctx->spos, Np, roots, M, V, nrows);
-			else k_sums64<2><<<sl.grid, BLOCK, 0, s>>>(ctx->spos, Np, roots, M, V, nrows);
-		}
-		if (ctx->P.timing && tm.a && tm.b)
-		{
-			(void)hipEventRecord(tm.b, s);
-			ctx->timers.push_back(tm);
-		}
-		k_finalize<<<g_rows, BLOCK, 0, s>>>(ctx->theta, roots, M, V, ctx->counters);
-		if (!CCF_LAUNCH(ctx, "find_critical_curve_roots_kernel")) return false;
-		ctx->info.sweeps++;
-		// mixed form, continuation: the sweeps after the first tail_at of
-		// the step in one launch, with no read-back. The rows the root-sum
-		// FMM took in the step's first two sweeps go to the host for the
-		// choice two steps later.
-		if (tail_at > 0 && it + 1 == tail_at && it + 1 < cap)
-		{
-			if (ctx->host_counts && nrows <= ctx->max_rows)
-			{
-				if (fmm_any)
-				{
-					hipEvent_t& e = ctx->fmm_rows_event[2 * group + (M.j & 1)];
-					if (!e && !CCF_HIP(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming))) return false;
-					if (!CCF_HIP(ctx, hipMemcpyAsync(fmm_lag_slot(ctx, group, M.j), B.plan + 2, 2 * sizeof(int),
-							hipMemcpyDeviceToHost, s)) ||
-						!CCF_HIP(ctx, hipEventRecord(e, s)))
-						return false;
-				}
-				else
-				{
-					// no pass ran: its row counts are 0 (the slot's previous copy,
-					// two steps back, was waited for when this step was planned)
-					int* slot = fmm_lag_slot(ctx, group, M.j);
-					slot[0] = slot[1] = 0;
-				}
-			}
-			if (before_tail && !before_tail()) return false;
-			const SweepView Vn = view(B, (it + 1) & 1);
-			const double d = CLUSTER_REL * ctx->root_half_length;
-			ccf::Timer tt;
+			SweepGroup& q = gs[g];
+			if (q.done) continue;
+			SweepBufs& B = *q.B;
+			const RowMap& M = q.M;
+			const int nrows = q.nrows, R = M.R;
+			hipStream_t s = q.s;
+			const dim3 g_rows(grid(R), nrows);
+			const dim3 g_witt(xcd_round(grid(R)), nrows);
+			const SweepView V = view(B, it & 1);
+			const bool fmm = B.grid.order > 0 && it < 32 && ((q.fmm_mask >> it) & 1u);
+			q.fmm_any = q.fmm_any || fmm;
+			const PlanArgs plan{nrows, Np + R, sl.target, fmm ? root_fmm_min(R) : 0, it};
+			ccf::Timer tw;
 			if (ctx->P.timing)
 			{
-				tt.a = ctx->next_event();
-				tt.b = ctx->next_event();
-				if (tt.a && tt.b) (void)hipEventRecord(tt.a, s);
+				tw.a = ctx->next_event();
+				tw.b = ctx->next_event();
+				if (tw.a) (void)hipEventRecord(tw.a, s);
 			}
-			k_tail<<<nrows, TAIL_BLOCK, 0, s>>>(L, roots, M, Vn, ctx->counters, circ_exact, fmm_pole ? 1 : 0, cap - (it + 1),
-				(float)(d * d), ctx->spos32, Np);
-			if (ctx->P.timing && tt.a && tt.b)
+			const dim3 g_far(WITT_FAR_BLOCKS, nrows);
+			if (f32)
 			{
-				(void)hipEventRecord(tt.b, s);
-				ctx->tail_timers.push_back(tt);
+				k_witt<true><<<g_witt, BLOCK, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, fmm_pole ? 1 : 0);
+				k_witt_far<true><<<g_far, WITT_FAR_THREADS, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, fmm_pole ? 1 : 0, plan);
 			}
+			else
+			{
+				k_witt<false><<<g_witt, BLOCK, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, 0);
+				k_witt_far<false><<<g_far, WITT_FAR_THREADS, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, 0, plan);
+			}
+			if (ctx->P.timing && tw.a && tw.b)
+			{
+				(void)hipEventRecord(tw.b, s);
+				ctx->witt_timers.push_back(tw);
+			}
+			ccf::Timer tm;
+			if (ctx->P.timing)
+			{
+				tm.a = ctx->next_event();
+				tm.b = ctx->next_event();
+				if (tm.a) (void)hipEventRecord(tm.a, s);
+			}
+			if (fmm) root_fmm_pass(B, roots, M, nrows, ctx->counters, s);
+			if (f32)
+			{
+				const double d = CLUSTER_REL * ctx->root_half_length;
+				const float cd2 = (float)(d * d);
+				if (sl.rpt == 1) k_sums32<1><<<sl.grid, BLOCK, 0, s>>>(ctx->spos32, Np, roots, M, V, nrows, cd2, ctx->counters);
+				else if (sl.rpt == 4) k_sums32<4><<<sl.grid, BLOCK, 0, s>>>(ctx->spos32, Np, roots, M, V, nrows, cd2, ctx->counters);
+				else k_sums32<2><<<sl.grid, BLOCK, 0, s>>>(ctx->spos32, Np, roots, M, V, nrows, cd2, ctx->counters);
+			}
+			else
+			{
+				if (sl.rpt == 1) k_sums64<1><<<sl.grid, BLOCK, 0, s>>>(ctx->spos, Np, roots, M, V, nrows);
+				else if (sl.rpt == 4) k_sums64<4><<<sl.grid, BLOCK, 0, s>>>(ctx->spos, Np, roots, M, V, nrows);
+				else k_sums64<2><<<sl.grid, BLOCK, 0, s>>>(ctx->spos, Np, roots, M, V, nrows);
+			}
+			if (ctx->P.timing && tm.a && tm.b)
+			{
+				(void)hipEventRecord(tm.b, s);
+				ctx->timers.push_back(tm);
+			}
+			k_finalize<<<g_rows, BLOCK, 0, s>>>(ctx->theta, roots, M, V, ctx->counters);
 			if (!CCF_LAUNCH(ctx, "find_critical_curve_roots_kernel")) return false;
-			ctx->info.tail_launches++;
-			ctx->phase_sweeps = it + 1;
-			break;
-		}
-		// stop once no root of any row is active: the remaining sweeps up to
-		// the cap would do nothing (checked at a few sweeps only, each check
-		// waits for the device)
-		if (ctx->host_counts && nrows <= ctx->max_rows && (ctx->sweep_verbose >= 3 || sweep_check(it + 1, cap, first_check)))
-		{
-			if (!CCF_HIP(ctx, hipMemcpyAsync(ctx->host_counts, V.cnt_next, nrows * sizeof(int), hipMemcpyDeviceToHost, s)) ||
-				!CCF_HIP(ctx, hipMemcpyAsync(ctx->host_counts + ctx->max_rows, B.plan + 2, 2 * sizeof(int), hipMemcpyDeviceToHost, s)) ||
-				!CCF_HIP(ctx, hipStreamSynchronize(s)))
-				return false;
-			long long active = 0;
-			int amax = 0;
-			for (int r = 0; r < nrows; r++)
+			ctx->info.sweeps++;
+			// mixed form, continuation: the sweeps after the first tail_at of
+			// the step in one launch, with no read-back. The rows the root-sum
+			// FMM took in the step's first two sweeps go to the host for the
+			// choice two steps later.
+			if (tail_at > 0 && it + 1 == tail_at && it + 1 < cap)
 			{
-				active += ctx->host_counts[r];
-				amax = std::max(amax, ctx->host_counts[r]);
+				if (ctx->host_counts && nrows <= ctx->max_rows)
+				{
+					if (q.fmm_any)
+					{
+						hipEvent_t& e = ctx->fmm_rows_event[2 * q.id + (M.j & 1)];
+						if (!e && !CCF_HIP(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming))) return false;
+						if (!CCF_HIP(ctx, hipMemcpyAsync(fmm_lag_slot(ctx, q.id, M.j), B.plan + 2, 2 * sizeof(int),
+								hipMemcpyDeviceToHost, s)) ||
+							!CCF_HIP(ctx, hipEventRecord(e, s)))
+							return false;
+					}
+					else
+					{
+						// no pass ran: its row counts are 0 (the slot's previous
+						// copy, two steps back, was waited for when this step was
+						// planned)
+						int* slot = fmm_lag_slot(ctx, q.id, M.j);
+						slot[0] = slot[1] = 0;
+					}
+				}
+				if (before_tail && !before_tail(q.id)) return false;
+				const SweepView Vn = view(B, (it + 1) & 1);
+				const double d = CLUSTER_REL * ctx->root_half_length;
+				ccf::Timer tt;
+				if (ctx->P.timing)
+				{
+					tt.a = ctx->next_event();
+					tt.b = ctx->next_event();
+					if (tt.a && tt.b) (void)hipEventRecord(tt.a, s);
+				}
+				k_tail<<<nrows, TAIL_BLOCK, 0, s>>>(L, roots, M, Vn, ctx->counters, circ_exact, fmm_pole ? 1 : 0, cap - (it + 1),
+					(float)(d * d), ctx->spos32, Np);
+				if (ctx->P.timing && tt.a && tt.b)
+				{
+					(void)hipEventRecord(tt.b, s);
+					ctx->tail_timers.push_back(tt);
+				}
+				if (!CCF_LAUNCH(ctx, "find_critical_curve_roots_kernel")) return false;
+				ctx->info.tail_launches++;
+				q.done = true;
+				left--;
 			}
-			// a row's listed roots are among its active ones: once no row has
-			// as many as the FMM takes, the remaining sweeps skip its pass
-			if (B.grid.order > 0 && amax < root_fmm_min(R)) fmm_mask = 0;
-			ctx->fmm_rows_sweep[0] = ctx->host_counts[ctx->max_rows];
-			ctx->fmm_rows_sweep[1] = ctx->host_counts[ctx->max_rows + 1];
-			if (ctx->sweep_verbose >= 3)
-				std::cout << "sweep " << it + 1 << ": " << active << " roots active, at most " << amax << " in a row, "
-						  << (fmm ? "root-sum FMM" : "direct sums") << "\n";
-			if (active == 0)
+		}
+		// stop the groups whose rows have no active root: the remaining
+		// sweeps up to the cap would do nothing (checked at a few sweeps only,
+		// each check waits for the device)
+		if (left > 0 && readback && (ctx->sweep_verbose >= 3 || sweep_check(it + 1, cap, first_check)))
+		{
+			for (int g = 0; g < G; g++)
 			{
-				ctx->phase_sweeps = it + 1;
-				break;
+				SweepGroup& q = gs[g];
+				if (q.done) continue;
+				const SweepView V = view(*q.B, it & 1);
+				if (!CCF_HIP(ctx, hipMemcpyAsync(ctx->host_counts + q.row0, V.cnt_next, q.nrows * sizeof(int), hipMemcpyDeviceToHost, q.s)) ||
+					!CCF_HIP(ctx, hipMemcpyAsync(fmm_read_slot(ctx, q.id), q.B->plan + 2, 2 * sizeof(int), hipMemcpyDeviceToHost, q.s)))
+					return false;
+			}
+			for (int g = 0; g < G; g++)
+				if (!gs[g].done && !CCF_HIP(ctx, hipStreamSynchronize(gs[g].s))) return false;
+			for (int g = 0; g < G; g++)
+			{
+				SweepGroup& q = gs[g];
+				if (q.done) continue;
+				long long active = 0;
+				int amax = 0;
+				for (int r = 0; r < q.nrows; r++)
+				{
+					active += ctx->host_counts[q.row0 + r];
+					amax = std::max(amax, ctx->host_counts[q.row0 + r]);
+				}
+				// a row's listed roots are among its active ones: once no row has
+				// as many as the FMM takes, the remaining sweeps skip its pass
+				if (q.B->grid.order > 0 && amax < root_fmm_min(q.M.R)) q.fmm_mask = 0;
+				if (q.id == 0)
+				{
+					ctx->fmm_rows_sweep[0] = fmm_read_slot(ctx, 0)[0];
+					ctx->fmm_rows_sweep[1] = fmm_read_slot(ctx, 0)[1];
+				}
+				if (ctx->sweep_verbose >= 3)
+					std::cout << "sweep " << it + 1 << (G > 1 ? ", group " + std::to_string(g) : std::string()) << ": " << active
+							  << " roots active, at most " << amax << " in a row\n";
+				if (active == 0)
+				{
+					q.done = true;
+					left--;
+				}
 			}
 		}
 	}
+	ctx->phase_sweeps = swept;
 	return true;
+}
+
+// run_sweeps_groups for one group on `stream` (the context's by default),
+// with the host slots of group `group`
+bool run_sweeps(ccf_ctx* ctx, SweepBufs& B, double2* roots, const RowMap& M, int nrows, int cap, unsigned fmm_mask,
+	int first_check = 3, int tail_at = 0, const std::function<bool()>& before_tail = {}, hipStream_t stream = nullptr,
+	int group = 0)
+{
+	SweepGroup q{&B, M, nrows, stream ? stream : ctx->stream, fmm_mask, group};
+	return run_sweeps_groups(ctx, &q, 1, roots, cap, first_check, tail_at,
+		before_tail ? std::function<bool(int)>([&](int) { return before_tail(); }) : std::function<bool(int)>());
 }
 
 // The sweep buffers of `rows` rows of R roots (and, when the root-sum FMM
@@ -1844,11 +1916,28 @@ bool alloc_solver(ccf_ctx* ctx)
 	const bool groups = ctx->P.precision == 1 && nb >= CONT_GROUPS;
 	return CCF_HIP(ctx, ctx->dmalloc(&ctx->ccs_init, pts * sizeof(double2))) && alloc_sweep(ctx, ctx->sw, ctx->max_rows, R) &&
 		(!groups || alloc_sweep(ctx, ctx->sw2, 2 * (nb - nb / CONT_GROUPS), R)) &&
-		CCF_HIP(ctx, hipHostMalloc((void**)&ctx->host_counts, (ctx->max_rows + 2 + 4 * CONT_GROUPS) * sizeof(int), hipHostMallocDefault)) &&
+		CCF_HIP(ctx, hipHostMalloc((void**)&ctx->host_counts, (ctx->max_rows + 6 * CONT_GROUPS) * sizeof(int), hipHostMallocDefault)) &&
 		CCF_HIP(ctx, ctx->dmalloc(&ctx->phase, (size_t)(np + nb) * sizeof(double2))) && launch_phase(ctx) &&
 		CCF_HIP(ctx, ctx->dmalloc(&ctx->counters, STAT_WORDS * sizeof(unsigned long long))) &&
 		CCF_HIP(ctx, hipMemsetAsync(ctx->ccs_init, 0, pts * sizeof(double2), ctx->stream)) &&
 		CCF_HIP(ctx, hipMemsetAsync(ctx->counters, 0, STAT_WORDS * sizeof(unsigned long long), ctx->stream));
+}
+
+// the second group of branches' stream (created once, default priority)
+// starts after everything queued on the context's stream so far
+bool group_streams_start(ccf_ctx* ctx)
+{
+	if (!ctx->group_stream && !CCF_HIP(ctx, hipStreamCreateWithFlags(&ctx->group_stream, hipStreamNonBlocking))) return false;
+	if (!ctx->group_event && !CCF_HIP(ctx, hipEventCreateWithFlags(&ctx->group_event, hipEventDisableTiming))) return false;
+	return CCF_HIP(ctx, hipEventRecord(ctx->group_event, ctx->stream)) &&
+		CCF_HIP(ctx, hipStreamWaitEvent(ctx->group_stream, ctx->group_event, 0));
+}
+
+// what follows on the context's stream waits for the second group's work
+bool group_streams_join(ccf_ctx* ctx)
+{
+	return CCF_HIP(ctx, hipEventRecord(ctx->group_event, ctx->group_stream)) &&
+		CCF_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->group_event, 0));
 }
 
 bool find_initial_roots_dev(ccf_ctx* ctx, int verbose)
@@ -1864,12 +1953,29 @@ bool find_initial_roots_dev(ccf_ctx* ctx, int verbose)
 		if (!CCF_HIP(ctx, hipMemcpyAsync(ctx->ccs_init + (size_t)center * R, ctx->seeds.data(), (size_t)R * sizeof(double2), hipMemcpyHostToDevice, s)))
 			return false;
 	}
-	const RowMap M{R, np, nb, 0, 1, 0.0, ctx->phase};
 	ctx->sweep_verbose = verbose;
-	k_prepare<<<dim3(grid(R), nb), BLOCK, 0, s>>>(ctx->ccs_init, M, ctx->sw, 0, ctx->dorder);
-	if (!CCF_LAUNCH(ctx, "prepare_roots_kernel")) return false;
 	const int cap = ctx->P.init_iters > 0 ? ctx->P.init_iters : 60;
-	if (!run_sweeps(ctx, ctx->sw, ctx->ccs_init, M, nb, cap, (1u << ROOT_FMM_INIT_SWEEPS) - 1u)) return false;
+	// the mixed form solves its branch centres in CONT_GROUPS groups on their
+	// own streams, sweep by sweep side by side (the root-sum FMM pass of a
+	// dense sweep is a chain of short launches); each group's rows are
+	// independent
+	const int G = (ctx->sw2.act && nb >= CONT_GROUPS) ? CONT_GROUPS : 1;
+	if (G > 1 && !group_streams_start(ctx)) return false;
+	SweepGroup gs[CONT_GROUPS];
+	for (int g = 0; g < G; g++)
+	{
+		const int c0 = g * nb / G, nbg = (g + 1) * nb / G - c0;
+		const RowMap M{R, np, nb, 0, 1, 0.0, ctx->phase, c0};
+		gs[g] = SweepGroup{g == 0 ? &ctx->sw : &ctx->sw2, M, nbg, g == 0 ? s : ctx->group_stream,
+			(1u << ROOT_FMM_INIT_SWEEPS) - 1u, g};
+	}
+	for (int g = 0; g < G; g++)
+	{
+		k_prepare<<<dim3(grid(R), gs[g].nrows), BLOCK, 0, gs[g].s>>>(ctx->ccs_init, gs[g].M, *gs[g].B, 0, ctx->dorder);
+		if (!CCF_LAUNCH(ctx, "prepare_roots_kernel")) return false;
+	}
+	if (!run_sweeps_groups(ctx, gs, G, ctx->ccs_init, cap)) return false;
+	if (G > 1 && !group_streams_join(ctx)) return false;
 	if (!CCF_HIP(ctx, hipStreamSynchronize(s))) return false;
 	auto t1 = std::chrono::steady_clock::now();
 	ctx->info.t_init_roots = std::chrono::duration<double>(t1 - t0).count();
@@ -1953,11 +2059,7 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 	gsw[0] = &ctx->sw;
 	if (G > 1)
 	{
-		if (!ctx->group_stream && !CCF_HIP(ctx, hipStreamCreateWithFlags(&ctx->group_stream, hipStreamNonBlocking))) return false;
-		if (!ctx->group_event && !CCF_HIP(ctx, hipEventCreateWithFlags(&ctx->group_event, hipEventDisableTiming))) return false;
-		// the other group starts after everything queued before the continuation
-		if (!CCF_HIP(ctx, hipEventRecord(ctx->group_event, s)) || !CCF_HIP(ctx, hipStreamWaitEvent(ctx->group_stream, ctx->group_event, 0)))
-			return false;
+		if (!group_streams_start(ctx)) return false;
 		for (int g = 1; g < G; g++)
 		{
 			gs[g] = ctx->group_stream;
@@ -2030,9 +2132,7 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 	for (int g = 0; g < G; g++)
 		if (!errors(prev_rows(steps + 1, g), steps == 0 ? gb[g + 1] - gb[g] : 2 * (gb[g + 1] - gb[g]), gs[g])) return false;
 	// the stages after the continuation run on the context's stream
-	if (G > 1 && (!CCF_HIP(ctx, hipEventRecord(ctx->group_event, ctx->group_stream)) ||
-					 !CCF_HIP(ctx, hipStreamWaitEvent(s, ctx->group_event, 0))))
-		return false;
+	if (G > 1 && !group_streams_join(ctx)) return false;
 	if (!CCF_HIP(ctx, hipStreamSynchronize(s))) return false;
 	auto t1 = std::chrono::steady_clock::now();
 	ctx->info.t_ccs = std::chrono::duration<double>(t1 - t0).count();
