@@ -127,6 +127,7 @@ struct RootGrid
 	int ncells = 0;      // cells of all levels
 	int nleaf = 0;       // leaf cells
 	int bin_row = 0;     // a row's leaf counts fit in one workgroup's LDS (one-launch binning)
+	int rlev = 0;        // region level: its cells group the leaf cells in blocks of at most 8 x 8
 	double x0 = 0, y0 = 0, s0 = 0;
 	int off[MAX_ROOT_GRID_LEVELS + 1] = {};
 };
@@ -156,13 +157,14 @@ struct SweepBufs
 	int* ccnt = nullptr;     // [rows][ncells] roots in each cell's subtree
 	int* cstart = nullptr;   // [rows][nleaf + 1] first sorted position of each leaf cell
 	int* ccur = nullptr;     // [rows][nleaf] counts, then scatter cursors
-	int* cell = nullptr;     // [rows][R] leaf cell of each root, -1 outside the grid
-	int* sidx = nullptr;     // [rows][R] root indices sorted by leaf cell
+	int* cell = nullptr;     // [rows][R] leaf cell (sort key) of each root, -1 outside the grid
+	int* bidx = nullptr;     // [rows][R] root indices in leaf cell order, each cell's in any order
+	int* sidx = nullptr;     // [rows][R] root indices sorted by leaf cell, then by index
 	double2* spos = nullptr; // [rows][R] their positions
 	int* ocnt = nullptr;     // [rows] roots outside the grid
 	int* oidx = nullptr;     // [rows][R] their indices
 	double2* mp = nullptr;   // [rows][ncells][order + 1] multipole coefficients
-	double2* lc = nullptr;   // [rows][ncells][order + 1] local coefficients
+	int* rticket = nullptr;  // [rows] regions of the row done with their upward pass (the last resets it)
 };
 
 } // namespace ccf

@@ -10,7 +10,9 @@
 //    squares for an elongated rectangle) and level l splits every top cell
 //    into 2^l x 2^l. Its geometry is fixed per field; the roots of each row
 //    are binned into the leaf cells at every sweep (counting sort, then a
-//    sort by root index inside each cell so that sums are deterministic).
+//    sort by root index inside each cell so that sums are deterministic),
+//    the leaf cells numbered region by region (leaf_key) so that the roots
+//    of a block of up to 8 x 8 leaf cells are contiguous.
 //  * Expansions are of the Cauchy kernel 1/(z - w), scaled by the half side
 //    h of the cell's level (Greengard & Rokhlin 1987, written for the
 //    complex kernel rather than for log|z - w|):
@@ -20,11 +22,10 @@
 //      M2M         A_n(parent) = sum_k C(n,k) 2^-k A_k(child) e^(n-k),  e = (c_child - c_parent)/h_parent
 //      M2L         L_m += (-1)^m / h sum_n C(n+m,m) A_n d^-(n+m+1),    d = (c_target - c_source)/h
 //      L2L         L_k(child) = 2^-k sum_{m>=k} C(m,k) L_m(parent) e^(m-k)
-//    each formed without a table of binomials (whose constants would
-//    crowd the scalar registers): M2M as the in-place binomial transform
-//    a_n += e a_(n-1), L2L as a Taylor shift by repeated synthetic
-//    division, and M2L's sum_n C(n+m,m) b_n as element 0 after m + 1
-//    suffix-sum passes over b_n = A_n d^-n.
+//    M2M with a table of binomials in LDS, one coefficient per thread; L2L
+//    as a Taylor shift by repeated synthetic division, and M2L's
+//    sum_n C(n+m,m) b_n as element 0 after m + 1 suffix-sum passes over
+//    b_n = A_n d^-n (no table: its constants would crowd the registers).
 //    The interaction list of a cell at level >= 1 is the set of children of
 //    its parent's neighbours that are not its own neighbours; at level 0 it
 //    is every top cell that is not a neighbour. A leaf's near field is its
@@ -43,7 +44,16 @@ constexpr int ROOT_FMM_MIN_ROOTS = 64;     // smaller rows are always summed dir
 constexpr int ROOT_FMM_INIT_SWEEPS = 24;
 constexpr int ROOT_FMM_STEP_SWEEPS = 2;
 constexpr int ROOT_FMM_PROBE = 8;
-constexpr int OUT_BLOCKS = 8; // workgroups per row summing the roots outside the grid
+
+// regions: blocks of up to REGION_SIDE x REGION_SIDE leaf cells, one cell of
+// level rlev = max(0, leaf - REGION_LOG) each, and the workgroup size of the
+// kernels over them
+constexpr int REGION_LOG = 3;
+constexpr int REGION_SIDE = 1 << REGION_LOG;
+constexpr int REGION_CELLS = REGION_SIDE * REGION_SIDE;
+constexpr int REGION_TREE = REGION_CELLS + REGION_CELLS / 4 + REGION_CELLS / 16 + 1; // its cells on levels leaf .. rlev
+constexpr int REGION_THREADS = 256;
+constexpr int RUP_STAGE = 1024; // roots of a region that k_rup sorts and expands from LDS
 
 // listed roots from which a row's sums are cheaper by the FMM: its pass
 // costs about as much as R/32 direct updates on a row of 20 000 roots; on
@@ -61,23 +71,34 @@ struct RootView
 	int* cstart;
 	int* ccur;
 	int* cell;
+	int* bidx;
 	int* sidx;
 	double2* spos;
 	int* ocnt;
 	int* oidx;
 	double2* mp;
-	double2* lc;
+	int* rticket;
 };
 
 RootView root_view(const SweepBufs& B)
 {
-	return RootView{B.grid, B.slot, B.rmode, B.rsum, B.ccnt, B.cstart, B.ccur, B.cell, B.sidx, B.spos, B.ocnt, B.oidx, B.mp, B.lc};
+	return RootView{B.grid, B.slot, B.rmode, B.rsum, B.ccnt, B.cstart, B.ccur, B.cell, B.bidx, B.sidx, B.spos, B.ocnt, B.oidx,
+		B.mp, B.rticket};
 }
 
 __device__ inline cd cell_center(const RootGrid& g, int l, int ix, int iy)
 {
 	const double s = ldexp(g.s0, -l);
 	return cd(g.x0 + (ix + 0.5) * s, g.y0 + (iy + 0.5) * s);
+}
+
+// sort key of leaf cell (ix, iy): its region's row-major index over level
+// rlev, then the cell's row-major place in the region's block
+__device__ __forceinline__ int leaf_key(const RootGrid& g, int ix, int iy)
+{
+	const int s = g.leaf - g.rlev, m = (1 << s) - 1;
+	const int rg = (iy >> s) * (g.nx << g.rlev) + (ix >> s);
+	return (rg << (2 * s)) + ((iy & m) << s) + (ix & m);
 }
 
 // counts and the outlier count of the rows of this sweep
@@ -108,7 +129,7 @@ __global__ __launch_bounds__(BLOCK) void k_rbin(RootView V, const double2* __res
 		const double sl = ldexp(V.g.s0, -V.g.leaf);
 		const double fx = (z.x - V.g.x0) / sl, fy = (z.y - V.g.y0) / sl;
 		const int nxl = V.g.nx << V.g.leaf, nyl = V.g.ny << V.g.leaf;
-		if (fx >= 0 && fx < nxl && fy >= 0 && fy < nyl) c = (int)fy * nxl + (int)fx;
+		if (fx >= 0 && fx < nxl && fy >= 0 && fy < nyl) c = leaf_key(V.g, (int)fx, (int)fy);
 		V.cell[(size_t)r * R + j] = c;
 	}
 	// neighbouring roots mostly share a cell: one atomic per run of them
@@ -193,7 +214,7 @@ __global__ __launch_bounds__(BLOCK) void k_rscatter(RootView V, int R)
 	const int j = blockIdx.x * BLOCK + threadIdx.x;
 	const int c = j < R ? V.cell[(size_t)r * R + j] : -1;
 	const int pos = run_add(V.ccur + (size_t)r * V.g.nleaf, c);
-	if (c >= 0) V.sidx[(size_t)r * R + pos] = j;
+	if (c >= 0) V.bidx[(size_t)r * R + pos] = j;
 }
 
 // exclusive prefix of v over the workgroup (blockDim.x a multiple of 64, at
@@ -226,7 +247,7 @@ __device__ inline int block_scan_excl(int v, int* wsum, int& tot)
 // then the cells' cursors are LDS atomics; the outliers (cell -1) of every
 // 64 consecutive roots are a ballot mask in LDS, whose popcounts' prefix
 // places them in index order. Within a cell the order is the atomics'
-// (k_rleaf sorts it). LDS: nleaf ints, then per 64 roots a mask and an int.
+// (k_rup sorts it). LDS: nleaf ints, then per 64 roots a mask and an int.
 constexpr int BIN_THREADS = 1024;
 constexpr int BIN_BATCH = 4;        // roots per thread whose loads k_rbin_row issues at once
 constexpr int BIN_STATIC_LDS = 256; // k_rbin_row's own LDS beside the dynamic part
@@ -280,7 +301,7 @@ __global__ __launch_bounds__(BIN_THREADS) void k_rbin_row(RootView V, const doub
 			if (j < R)
 			{
 				const double fx = (z[k].x - V.g.x0) / sl, fy = (z[k].y - V.g.y0) / sl;
-				if (fx >= 0 && fx < nxl && fy >= 0 && fy < nyl) c = (int)fy * nxl + (int)fx;
+				if (fx >= 0 && fx < nxl && fy >= 0 && fy < nyl) c = leaf_key(V.g, (int)fx, (int)fy);
 				cell[j] = c;
 				if (c >= 0) atomicAdd(&cnt[c], 1);
 			}
@@ -330,7 +351,7 @@ __global__ __launch_bounds__(BIN_THREADS) void k_rbin_row(RootView V, const doub
 	}
 	if (threadIdx.x == 0) st[nl] = run;
 	__syncthreads();
-	int* __restrict__ sidx = V.sidx + (size_t)r * R;
+	int* __restrict__ bidx = V.bidx + (size_t)r * R;
 	for (int j0 = threadIdx.x; j0 < R; j0 += step)
 	{
 		int c[BIN_BATCH];
@@ -342,174 +363,15 @@ __global__ __launch_bounds__(BIN_THREADS) void k_rbin_row(RootView V, const doub
 		}
 #pragma unroll
 		for (int k = 0; k < BIN_BATCH; k++)
-			if (c[k] >= 0) sidx[atomicAdd(&cnt[c[k]], 1)] = j0 + k * BIN_THREADS;
+			if (c[k] >= 0) bidx[atomicAdd(&cnt[c[k]], 1)] = j0 + k * BIN_THREADS;
 	}
 }
-
-constexpr int LEAF_REG = 32; // cells up to this many roots sort in registers
-
-// per leaf cell: the cell's roots in index order, their positions copied
-// contiguously, and the multipole expansion (P2M)
-template <int P>
-__global__ __launch_bounds__(BLOCK) void k_rleaf(RootView V, const double2* __restrict__ roots, RowMap M)
-{
-	const int r = blockIdx.y;
-	if (!V.rmode[r]) return;
-	const int c = blockIdx.x * BLOCK + threadIdx.x;
-	if (c >= V.g.nleaf) return;
-	const int R = M.R;
-	int row;
-	double phi;
-	M.get(r, row, phi);
-	const int* st = V.cstart + (size_t)r * (V.g.nleaf + 1);
-	const int b = st[c], e = st[c + 1], n = e - b;
-	const int L = V.g.leaf;
-	const size_t cid = (size_t)r * V.g.ncells + V.g.off[L] + c;
-	V.ccnt[cid] = n;
-	if (n == 0) return;
-	int* ix = V.sidx + (size_t)r * R;
-	const int nxl = V.g.nx << L;
-	const cd cc = cell_center(V.g, L, c % nxl, c / nxl);
-	const double ih = 2.0 / ldexp(V.g.s0, -L);
-	cd A[P + 1];
-#pragma unroll
-	for (int k = 0; k <= P; k++) A[k] = cd(0, 0);
-	if (n <= LEAF_REG)
-	{
-		// a small cell sorts in registers: each index's rank among the
-		// cell's (the indices are distinct), then every load of the cell's
-		// roots at once instead of one dependent chain per root
-		int v[LEAF_REG], srt[LEAF_REG];
-#pragma unroll
-		for (int i = 0; i < LEAF_REG; i++) v[i] = i < n ? ix[b + i] : INT_MAX;
-#pragma unroll
-		for (int i = 0; i < LEAF_REG; i++) srt[i] = INT_MAX;
-#pragma unroll
-		for (int i = 0; i < LEAF_REG; i++)
-		{
-			int rk = 0;
-#pragma unroll
-			for (int j = 0; j < LEAF_REG; j++) rk += v[j] < v[i];
-#pragma unroll
-			for (int q = 0; q < LEAF_REG; q++)
-				if (rk == q && i < n) srt[q] = v[i];
-		}
-		double2 zz[LEAF_REG];
-#pragma unroll
-		for (int q = 0; q < LEAF_REG; q++)
-			if (q < n) zz[q] = roots[(size_t)row * R + srt[q]];
-#pragma unroll
-		for (int q = 0; q < LEAF_REG; q++)
-			if (q < n)
-			{
-				ix[b + q] = srt[q];
-				V.spos[(size_t)r * R + b + q] = zz[q];
-				const cd w = (cd(zz[q].x, zz[q].y) - cc) * ih;
-				cd t(1, 0);
-#pragma unroll
-				for (int k = 0; k <= P; k++)
-				{
-					A[k] += t;
-					t = t * w;
-				}
-			}
-		double2* out = V.mp + cid * (P + 1);
-#pragma unroll
-		for (int k = 0; k <= P; k++) out[k] = make_double2(A[k].re, A[k].im);
-		return;
-	}
-	if (n <= 128)
-	{
-		for (int q = b + 1; q < e; q++)
-		{
-			const int v = ix[q];
-			int t = q - 1;
-			while (t >= b && ix[t] > v)
-			{
-				ix[t + 1] = ix[t];
-				t--;
-			}
-			ix[t + 1] = v;
-		}
-	}
-	for (int q = b; q < e; q++)
-	{
-		const double2 z = roots[(size_t)row * R + ix[q]];
-		V.spos[(size_t)r * R + q] = z;
-		const cd w = (cd(z.x, z.y) - cc) * ih;
-		cd t(1, 0);
-#pragma unroll
-		for (int k = 0; k <= P; k++)
-		{
-			A[k] += t;
-			t = t * w;
-		}
-	}
-	double2* out = V.mp + cid * (P + 1);
-#pragma unroll
-	for (int k = 0; k <= P; k++) out[k] = make_double2(A[k].re, A[k].im);
-}
-
-// M2M: level l from its children at level l + 1
-template <int P>
-__global__ __launch_bounds__(BLOCK) void k_rm2m(RootView V, int l)
-{
-	const int r = blockIdx.y;
-	if (!V.rmode[r]) return;
-	const int nxl = V.g.nx << l, nyl = V.g.ny << l;
-	const int c = blockIdx.x * BLOCK + threadIdx.x;
-	if (c >= nxl * nyl) return;
-	const int ix = c % nxl, iy = c / nxl, nxc = 2 * nxl;
-	const size_t base = (size_t)r * V.g.ncells;
-	int tot = 0;
-	cd A[P + 1];
-#pragma unroll
-	for (int k = 0; k <= P; k++) A[k] = cd(0, 0);
-	for (int q = 0; q < 4; q++)
-	{
-		const int a = q & 1, b = q >> 1;
-		const size_t ch = base + V.g.off[l + 1] + (size_t)(2 * iy + b) * nxc + 2 * ix + a;
-		const int n = V.ccnt[ch];
-		if (n == 0) continue;
-		tot += n;
-		const double2* src = V.mp + ch * (P + 1);
-		cd ak[P + 1];
-		const cd e(a - 0.5, b - 0.5);
-		double sc = 1;
-#pragma unroll
-		for (int k = 0; k <= P; k++)
-		{
-			ak[k] = cd(src[k].x * sc, src[k].y * sc);
-			sc *= 0.5;
-		}
-		// binomial transform in place: a_n <- sum_k C(n, k) a_k e^(n-k)
-#pragma unroll
-		for (int i = 1; i <= P; i++)
-#pragma unroll
-			for (int n = P; n >= i; n--) ak[n] += e * ak[n - 1];
-#pragma unroll
-		for (int m = 0; m <= P; m++) A[m] += ak[m];
-	}
-	const size_t cid = base + V.g.off[l] + c;
-	V.ccnt[cid] = tot;
-	if (tot == 0) return;
-	double2* out = V.mp + cid * (P + 1);
-#pragma unroll
-	for (int k = 0; k <= P; k++) out[k] = make_double2(A[k].re, A[k].im);
-}
-
-// L2L and M2L of level l with 32 lanes per target cell: lane k translates
-// the k-th source of the target's interaction list into a partial local
-// expansion, the partials are added through LDS in lane order, and lane m
-// adds coefficient m of the parent's shifted expansion.
-template <int P>
-constexpr int down_targets() { return (P + 1) <= 14 ? 8 : 4; } // 32 (P+1) double2 of LDS per target
 
 // The interaction list of a cell below level 0: the children of its
 // parent's 3 x 3 block of cells (a 6 x 6 block, parent block row-major,
 // four children each) that are not its neighbours. Entry k of list o (the
 // cell's child position x + 2 y in its parent) is x + 6 y in that block; 27
-// entries, so one per lane.
+// entries.
 struct FarList
 {
 	unsigned char v[4][27];
@@ -566,220 +428,6 @@ __device__ __forceinline__ void m2l_add(const double2* am, int dx, int dy, doubl
 	}
 }
 
-template <int P>
-__global__ __launch_bounds__(32 * down_targets<P>()) void k_rdown32(RootView V, int l, unsigned long long* __restrict__ stats)
-{
-	constexpr int TPB = down_targets<P>();
-	__shared__ double2 part[TPB][32][P + 1];
-	__shared__ unsigned long long nsum[TPB * 32 / 64];
-	const int r = blockIdx.y;
-	if (!V.rmode[r]) return;
-	const int nxl = V.g.nx << l, nyl = V.g.ny << l;
-	const int tl = threadIdx.x >> 5, lane = threadIdx.x & 31;
-	const int c = blockIdx.x * TPB + tl;
-	const size_t base = (size_t)r * V.g.ncells;
-	const size_t cid = base + V.g.off[l] + c;
-	const bool live = c < nxl * nyl && V.ccnt[cid] > 0;
-	const int ix = live ? c % nxl : 0, iy = live ? c / nxl : 0;
-	const double ih = 2.0 / ldexp(V.g.s0, -l);
-	cd Lc[P + 1];
-#pragma unroll
-	for (int k = 0; k <= P; k++) Lc[k] = cd(0, 0);
-	unsigned long long nt = 0;
-	auto m2l = [&](int jx, int jy) {
-		const size_t sid = base + V.g.off[l] + (size_t)jy * nxl + jx;
-		if (V.ccnt[sid] == 0) return;
-		nt++;
-		const double2* src = V.mp + sid * (P + 1);
-		double2 am[P + 1];
-#pragma unroll
-		for (int n = 0; n <= P; n++) am[n] = src[n];
-		m2l_add<P>(am, ix - jx, iy - jy, ih, Lc);
-	};
-	if (live)
-	{
-		if (l == 0)
-		{
-			for (int k = lane; k < nxl * nyl; k += 32)
-			{
-				const int jx = k % nxl, jy = k / nxl;
-				if (abs(jx - ix) >= 2 || abs(jy - iy) >= 2) m2l(jx, jy);
-			}
-		}
-		else
-		{
-			// the 27 far children of the parent's 3 x 3 block, one per lane
-			if (lane < 27)
-			{
-				const int v = kFarList.v[(ix & 1) + 2 * (iy & 1)][lane];
-				const int jx = 2 * ((ix >> 1) - 1) + v % 6, jy = 2 * ((iy >> 1) - 1) + v / 6;
-				if (jx >= 0 && jy >= 0 && jx < nxl && jy < nyl) m2l(jx, jy);
-			}
-		}
-	}
-#pragma unroll
-	for (int k = 0; k <= P; k++) part[tl][lane][k] = make_double2(Lc[k].re, Lc[k].im);
-	{
-		const unsigned long long s = wave_sum(nt);
-		if ((threadIdx.x & 63) == 0) nsum[threadIdx.x >> 6] = s;
-	}
-	__syncthreads();
-	if (live && lane <= P)
-	{
-		const int m = lane;
-		double sx = 0, sy = 0;
-		for (int k = 0; k < 32; k++)
-		{
-			const double2 v = part[tl][k][m];
-			sx += v.x;
-			sy += v.y;
-		}
-		if (l >= 1)
-		{
-			// L2L: 2^-m sum_{n >= m} C(n, m) L_n(parent) e^(n - m), the
-			// parent's polynomial shifted by e (repeated synthetic division)
-			const double2* lp = V.lc + (base + V.g.off[l - 1] + (size_t)(iy >> 1) * (nxl >> 1) + (ix >> 1)) * (P + 1);
-			const cd e((ix & 1) - 0.5, (iy & 1) - 0.5);
-			cd c[P + 1];
-#pragma unroll
-			for (int n = 0; n <= P; n++) c[n] = cd(lp[n].x, lp[n].y);
-#pragma unroll
-			for (int i = 0; i < P; i++)
-#pragma unroll
-				for (int j = P - 1; j >= i; j--) c[j] += e * c[j + 1];
-			cd s(0, 0);
-#pragma unroll
-			for (int n = 0; n <= P; n++)
-				if (n == m) s = c[n];
-			const double sc = ldexp(1.0, -m);
-			sx += s.re * sc;
-			sy += s.im * sc;
-		}
-		V.lc[cid * (P + 1) + m] = make_double2(sx, sy);
-	}
-	if (threadIdx.x == 0)
-	{
-		unsigned long long s = 0;
-		for (int w = 0; w < TPB * 32 / 64; w++) s += nsum[w];
-		if (s) atomicAdd(stat_slot(stats, C_RM2L), s);
-	}
-}
-
-// M2L and L2L of a level whose rows and columns of cells are multiples of
-// DOWN_TILE (levels 3 and finer): a workgroup takes a DOWN_TILE^2 tile of
-// target cells and stages the multipoles of the tile's whole interaction
-// region, (DOWN_TILE + 4)^2 cells, in LDS once, where the 32-lane kernel has
-// every target read its 27 sources through the cache. Four threads per
-// target take every fourth entry of its list; lane shuffles add their
-// partials, and each adds every fourth coefficient of the parent's shifted
-// expansion.
-constexpr int DOWN_TILE = 8;
-constexpr int DOWN_SRC = DOWN_TILE + 4;
-constexpr int DOWN_TILE_THREADS = DOWN_TILE * DOWN_TILE * 4;
-template <int P>
-constexpr bool down_tiled() { return (P + 1) <= 13; } // higher orders: the 32-lane kernel
-
-template <int P>
-__global__ __launch_bounds__(DOWN_TILE_THREADS) void k_rdown_tile(RootView V, int l, unsigned long long* __restrict__ stats)
-{
-	__shared__ double2 src[DOWN_SRC * DOWN_SRC][P + 1];
-	__shared__ int sok[DOWN_SRC * DOWN_SRC];
-	__shared__ unsigned long long nsum[DOWN_TILE_THREADS / 64];
-	const int r = blockIdx.y;
-	if (!V.rmode[r]) return;
-	const int nxl = V.g.nx << l, nyl = V.g.ny << l;
-	const int tiles_x = nxl / DOWN_TILE;
-	const int x0 = (blockIdx.x % tiles_x) * DOWN_TILE, y0 = (blockIdx.x / tiles_x) * DOWN_TILE;
-	// the region's first cell: the far lists reach two cells beyond the tile
-	const int sx0 = x0 - 2, sy0 = y0 - 2;
-	const size_t rowb = (size_t)r * V.g.ncells, lb = rowb + V.g.off[l];
-	for (int i = threadIdx.x; i < DOWN_SRC * DOWN_SRC; i += DOWN_TILE_THREADS)
-	{
-		const int jx = sx0 + i % DOWN_SRC, jy = sy0 + i / DOWN_SRC;
-		sok[i] = jx >= 0 && jy >= 0 && jx < nxl && jy < nyl && V.ccnt[lb + (size_t)jy * nxl + jx] > 0;
-	}
-	__syncthreads();
-	for (int i = threadIdx.x; i < DOWN_SRC * DOWN_SRC * (P + 1); i += DOWN_TILE_THREADS)
-	{
-		const int c = i / (P + 1), n = i - c * (P + 1);
-		if (sok[c])
-		{
-			const int jx = sx0 + c % DOWN_SRC, jy = sy0 + c / DOWN_SRC;
-			src[c][n] = V.mp[(lb + (size_t)jy * nxl + jx) * (P + 1) + n];
-		}
-	}
-	__syncthreads();
-	const int t = threadIdx.x >> 2, q = threadIdx.x & 3;
-	const int ix = x0 + t % DOWN_TILE, iy = y0 + t / DOWN_TILE;
-	const size_t cid = lb + (size_t)iy * nxl + ix;
-	const bool live = ix < nxl && iy < nyl && V.ccnt[cid] > 0;
-	const double ih = 2.0 / ldexp(V.g.s0, -l);
-	cd Lc[P + 1];
-#pragma unroll
-	for (int k = 0; k <= P; k++) Lc[k] = cd(0, 0);
-	unsigned long long nt = 0;
-	if (live)
-	{
-		// entry v of the far list is cell (v % 6, v / 6) of the 6 x 6 block
-		// whose first cell is (2 (ix / 2) - 2, 2 (iy / 2) - 2): local
-		// coordinates bx + v % 6, by + v / 6 in [0, DOWN_SRC)
-		const int o = (ix & 1) + 2 * (iy & 1);
-		const int bx = 2 * (ix >> 1) - x0, by = 2 * (iy >> 1) - y0;
-		for (int k = q; k < 27; k += 4)
-		{
-			const int v = kFarList.v[o][k];
-			const int lx = bx + v % 6, ly = by + v / 6;
-			const int c = ly * DOWN_SRC + lx;
-			if (!sok[c]) continue;
-			nt++;
-			m2l_add<P>(src[c], (ix - sx0) - lx, (iy - sy0) - ly, ih, Lc);
-		}
-	}
-	// the four threads of a target: (q0 + q1) + (q2 + q3) in every one of them
-#pragma unroll
-	for (int k = 0; k <= P; k++)
-		for (int d = 1; d <= 2; d <<= 1)
-		{
-			Lc[k].re += __shfl_xor(Lc[k].re, d, 64);
-			Lc[k].im += __shfl_xor(Lc[k].im, d, 64);
-		}
-	{
-		const unsigned long long sn = wave_sum(nt);
-		if ((threadIdx.x & 63) == 0) nsum[threadIdx.x >> 6] = sn;
-	}
-	if (live)
-	{
-		// L2L: 2^-m sum_{n >= m} C(n, m) L_n(parent) e^(n - m), the parent's
-		// polynomial shifted by e (repeated synthetic division)
-		const double2* lp = V.lc + (rowb + V.g.off[l - 1] + (size_t)(iy >> 1) * (nxl >> 1) + (ix >> 1)) * (P + 1);
-		const cd e((ix & 1) - 0.5, (iy & 1) - 0.5);
-		cd c[P + 1];
-#pragma unroll
-		for (int n = 0; n <= P; n++) c[n] = cd(lp[n].x, lp[n].y);
-#pragma unroll
-		for (int i = 0; i < P; i++)
-#pragma unroll
-			for (int j = P - 1; j >= i; j--) c[j] += e * c[j + 1];
-#pragma unroll
-		for (int m = 0; m <= P; m++)
-			if ((m & 3) == q)
-			{
-				const double sc = ldexp(1.0, -m);
-				V.lc[cid * (P + 1) + m] = make_double2(Lc[m].re + c[m].re * sc, Lc[m].im + c[m].im * sc);
-			}
-	}
-	__syncthreads();
-	if (threadIdx.x == 0)
-	{
-		unsigned long long sn = 0;
-		for (int w = 0; w < DOWN_TILE_THREADS / 64; w++) sn += nsum[w];
-		if (sn) atomicAdd(stat_slot(stats, C_RM2L), sn);
-	}
-}
-
-// per listed root of a row in FMM mode, in cell order: far field by L2P,
-// near field and outliers directly; rsum = -S (the root part of the
-// Aberth correction, the sign that the direct kernels' partials carry)
 // one direct Cauchy term (z - u) / |z - u|^2 added to (ax, ay), in FP64.
 // At orders up to 10, whose far field carries far larger relative errors,
 // the reciprocal is the hardware estimate alone (relative error about
@@ -795,97 +443,36 @@ __device__ __forceinline__ void direct_term(cd z, double2 u, double& ax, double&
 	ay = fma(dy, iv, ay);
 }
 
-template <int P>
-__global__ __launch_bounds__(BLOCK) void k_rsum(RootView V, const double2* __restrict__ roots, RowMap M,
-	unsigned long long* __restrict__ stats)
-{
-	const int r = blockIdx.y;
-	if (!V.rmode[r]) return;
-	const int R = M.R;
-	const int k = blockIdx.x * BLOCK + threadIdx.x;
-	const int* st = V.cstart + (size_t)r * (V.g.nleaf + 1);
-	unsigned long long nn = 0, nt = 0;
-	if (k < st[V.g.nleaf])
-	{
-		const int j = V.sidx[(size_t)r * R + k];
-		const int s = V.slot[(size_t)r * R + j];
-		if (s >= 0)
-		{
-			const double2 zz = V.spos[(size_t)r * R + k];
-			const cd z(zz.x, zz.y);
-			const int L = V.g.leaf, nxl = V.g.nx << L, nyl = V.g.ny << L;
-			const int c = V.cell[(size_t)r * R + j];
-			const int ix = c % nxl, iy = c / nxl;
-			const cd w = (z - cell_center(V.g, L, ix, iy)) * (2.0 / ldexp(V.g.s0, -L));
-			const double2* lp = V.lc + ((size_t)r * V.g.ncells + V.g.off[L] + c) * (P + 1);
-			cd S(lp[P].x, lp[P].y);
-#pragma unroll
-			for (int m = P - 1; m >= 0; m--) S = S * w + cd(lp[m].x, lp[m].y);
-			double ax = 0, ay = 0;
-			const double2* sp = V.spos + (size_t)r * R;
-			for (int y = max(iy - 1, 0); y <= min(iy + 1, nyl - 1); y++)
-			{
-				const int q0 = st[y * nxl + max(ix - 1, 0)], q1 = st[y * nxl + min(ix + 1, nxl - 1) + 1];
-				nn += q1 - q0;
-#pragma unroll 4
-				for (int q = q0; q < q1; q++) direct_term<P>(z, sp[q], ax, ay);
-			}
-			const int no = V.ocnt[r];
-			if (no > 0)
-			{
-				int row;
-				double phi;
-				M.get(r, row, phi);
-				for (int o = 0; o < no; o++) direct_term<P>(z, roots[(size_t)row * R + V.oidx[(size_t)r * R + o]], ax, ay);
-			}
-			S += cd(ax, -ay);
-			V.rsum[(size_t)r * R + s] = make_double2(-S.re, -S.im);
-			nt = 1;
-		}
-	}
-	const int which[2] = {C_RNEAR, C_RTGT};
-	unsigned long long v[2] = {nn, nt};
-	block_stats(stats, which, v);
-}
-
-// listed roots outside the grid: direct sums over the whole row. Block b
-// of a row takes its outliers b, b + OUT_BLOCKS, ..., OUT_GROUP at a time,
-// so that each pass over the row's roots serves a group of them.
+// listed roots outside the grid: direct sums over the whole row. Workgroup
+// b of nb takes the row's outliers b, b + nb, ..., OUT_GROUP at a time, so
+// that each pass over the row's roots serves a group of them.
 constexpr int OUT_GROUP = 8;
-constexpr int OUT_THREADS = 1024; // a pass over the row's roots: R / 1024 loads per lane
+constexpr int RSUM_OUT_BLOCKS = 16;
 
 template <int P>
-__global__ __launch_bounds__(OUT_THREADS) void k_rsum_out(RootView V, const double2* __restrict__ roots, RowMap M,
+__device__ void rsum_outliers(const RootView& V, const double2* __restrict__ zr, int R, int r, int b, int nb,
 	unsigned long long* __restrict__ stats)
 {
-	// grid (rows, OUT_BLOCKS): the outliers of a row split over its blocks
-	const int r = blockIdx.x;
-	if (!V.rmode[r]) return;
+	constexpr int NT = REGION_THREADS;
+	__shared__ double red[OUT_GROUP][2][NT / 64];
 	const int no = V.ocnt[r];
-	if (no == 0) return;
-	const int R = M.R;
-	int row;
-	double phi;
-	M.get(r, row, phi);
-	const double2* zr = roots + (size_t)row * R;
-	__shared__ double red[OUT_GROUP][2][OUT_THREADS / 64];
-	for (int o0 = blockIdx.y; o0 < no; o0 += gridDim.y * OUT_GROUP)
+	for (int o0 = b; o0 < no; o0 += nb * OUT_GROUP)
 	{
-		// the group: outliers o0 + k gridDim.y that are listed (slot >= 0)
+		// the group: outliers o0 + k nb that are listed (slot >= 0)
 		int j[OUT_GROUP], sl[OUT_GROUP];
 		cd z[OUT_GROUP];
 		double ax[OUT_GROUP], ay[OUT_GROUP];
 #pragma unroll
 		for (int k = 0; k < OUT_GROUP; k++)
 		{
-			const int o = o0 + k * gridDim.y;
+			const int o = o0 + k * nb;
 			j[k] = o < no ? V.oidx[(size_t)r * R + o] : -1;
 			sl[k] = j[k] >= 0 ? V.slot[(size_t)r * R + j[k]] : -1;
 			const double2 v = sl[k] >= 0 ? zr[j[k]] : make_double2(0, 0);
 			z[k] = cd(v.x, v.y);
 			ax[k] = ay[k] = 0;
 		}
-		for (int q = threadIdx.x; q < R; q += OUT_THREADS)
+		for (int q = threadIdx.x; q < R; q += NT)
 		{
 			const double2 u = zr[q];
 #pragma unroll
@@ -911,7 +498,7 @@ __global__ __launch_bounds__(OUT_THREADS) void k_rsum_out(RootView V, const doub
 		{
 			const int k = threadIdx.x;
 			double sx = 0, sy = 0;
-			for (int w = 0; w < OUT_THREADS / 64; w++)
+			for (int w = 0; w < NT / 64; w++)
 			{
 				sx += red[k][0][w];
 				sy += red[k][1][w];
@@ -921,6 +508,477 @@ __global__ __launch_bounds__(OUT_THREADS) void k_rsum_out(RootView V, const doub
 		}
 		__syncthreads();
 	}
+}
+
+// After the binning a pass is two launches over regions (one cell of level
+// g.rlev each, a block of RS x RS leaf cells, RS = 2^(leaf - rlev) <=
+// REGION_SIDE), whose roots leaf_key keeps contiguous in the sorted arrays:
+//  k_rup   one workgroup per region: each leaf cell's roots sorted by index,
+//          P2M on the leaf cells and M2M up to level rlev in LDS; the row's
+//          last workgroup forms the levels above rlev.
+//  k_rdown one workgroup per region: M2L and L2L from level 0 to the leaf
+//          over the region's chain of ancestors and its own cells (the
+//          sources of a level's targets staged in LDS), then L2P and the
+//          direct near field of each listed root of the region.
+// k_rup's launch also carries RSUM_OUT_BLOCKS workgroups per row that sum
+// the roots outside the grid directly.
+
+// coefficient n of a child's multipole shifted to its parent,
+// sum_k C(n, k) 2^-k a_k e^(n - k) (the child's unit is half the parent's)
+template <int P>
+__device__ __forceinline__ cd m2m_term(const double2* a, int n, cd e, const double (*binom)[P + 1])
+{
+	cd sum(0, 0), ep(1, 0);
+	for (int k = n; k >= 0; k--)
+	{
+		const double2 v = a[k];
+		sum += cd(v.x, v.y) * ep * (binom[n][k] * ldexp(1.0, -k));
+		ep = ep * e;
+	}
+	return sum;
+}
+
+// the cell of sorted position q among nc <= 128 cells: the last k with
+// cst[k] <= q (cst non-decreasing, cst[0] <= q < cst[nc]), by a fixed
+// number of halving steps
+__device__ __forceinline__ int region_cell(const int* cst, int nc, int q)
+{
+	int lo = 0;
+#pragma unroll
+	for (int step = 64; step > 0; step >>= 1)
+	{
+		const int j = min(lo + step, nc); // cst[nc] > q: never taken
+		lo = cst[j] <= q ? j : lo;
+	}
+	return lo;
+}
+static_assert(REGION_CELLS <= 128 && (REGION_SIDE + 2) * (REGION_SIDE + 2) <= 128, "region_cell searches at most 128 cells");
+
+template <int P>
+__global__ __launch_bounds__(REGION_THREADS) void k_rup(RootView V, const double2* __restrict__ roots, RowMap M,
+	unsigned long long* __restrict__ stats)
+{
+	constexpr int NT = REGION_THREADS, NC = P + 1, NPT = (NC + 3) / 4;
+	__shared__ int cst[REGION_CELLS + 1];
+	__shared__ int tcnt[REGION_TREE];
+	__shared__ double2 am[REGION_TREE][NC];
+	__shared__ double binom[NC][NC];
+	__shared__ int sbi[RUP_STAGE];     // the region's binned root indices
+	__shared__ double2 sz[RUP_STAGE];  // and their positions in sorted order
+	__shared__ int s_last;
+	const int r = blockIdx.y;
+	if (!V.rmode[r]) return;
+	const RootGrid& g = V.g;
+	const int Q = g.rlev, L = g.leaf, s = L - Q, RS = 1 << s, RC = RS * RS;
+	const int nxq = g.nx << Q, nreg = nxq * (g.ny << Q), rg = blockIdx.x, rx = rg % nxq, ry = rg / nxq;
+	const int R = M.R;
+	int row;
+	double phi;
+	M.get(r, row, phi);
+	if (rg >= nreg)
+	{
+		rsum_outliers<P>(V, roots + (size_t)row * R, R, r, rg - nreg, gridDim.x - nreg, stats);
+		return;
+	}
+	const size_t rowc = (size_t)r * g.ncells;
+	for (int i = threadIdx.x; i <= RC; i += NT) cst[i] = V.cstart[(size_t)r * (g.nleaf + 1) + (size_t)rg * RC + i];
+	// C(n, k) for n, k <= P by the product formula: every partial product
+	// is an integer below 2^53, so each step is exact
+	for (int i = threadIdx.x; i < NC * NC; i += NT)
+	{
+		const int n = i / NC, k = i - n * NC;
+		double c = k <= n ? 1.0 : 0.0;
+		for (int t = 1; t <= k && k <= n; t++) c = c * (n - k + t) / t;
+		binom[n][k] = c;
+	}
+	__syncthreads();
+	// each cell's roots in index order (the binning leaves them in the order
+	// of its atomics): a root's rank among its cell's indices, then its
+	// position copied beside it. A region of at most RUP_STAGE roots works
+	// from a copy of its indices in LDS and leaves its positions there.
+	const int b0 = cst[0], e0 = cst[RC];
+	const bool staged = e0 - b0 <= RUP_STAGE;
+	const int* __restrict__ bi = V.bidx + (size_t)r * R;
+	if (staged)
+		for (int i = threadIdx.x; i < e0 - b0; i += NT) sbi[i] = bi[b0 + i];
+	__syncthreads();
+	{
+		int* __restrict__ si = V.sidx + (size_t)r * R;
+		double2* __restrict__ sp = V.spos + (size_t)r * R;
+		const double2* __restrict__ zr = roots + (size_t)row * R;
+		// binned root index at position q of the row's cell order
+		auto lb = [&](int q) { return staged ? sbi[q - b0] : bi[q]; };
+		for (int q = b0 + threadIdx.x; q < e0; q += NT)
+		{
+			const int k = region_cell(cst, RC, q);
+			const int cb = cst[k], ce = cst[k + 1], v = lb(q);
+			int rank = 0;
+			for (int t = cb; t < ce; t++) rank += lb(t) < v;
+			const double2 z = zr[v];
+			si[cb + rank] = v;
+			sp[cb + rank] = z;
+			if (staged) sz[cb + rank - b0] = z;
+		}
+	}
+	__syncthreads();
+	// P2M: NT / RC threads per leaf cell, thread p forming the coefficients
+	// n = p, p + tpc, ... as sums over the cell's roots in index order
+	{
+		const int tpc = NT / RC, k = threadIdx.x / tpc, p = threadIdx.x - k * tpc;
+		const int ix = (rx << s) + (k & (RS - 1)), iy = (ry << s) + (k >> s);
+		const int cb = cst[k], ce = cst[k + 1];
+		const cd cc = cell_center(g, L, ix, iy);
+		const double ih = 2.0 / ldexp(g.s0, -L);
+		const double2* __restrict__ sp = V.spos + (size_t)r * R;
+		cd acc[NPT];
+#pragma unroll
+		for (int i = 0; i < NPT; i++) acc[i] = cd(0, 0);
+		if (p <= P)
+			for (int q = cb; q < ce; q++)
+			{
+				const double2 z = staged ? sz[q - b0] : sp[q];
+				const cd w = (cd(z.x, z.y) - cc) * ih;
+				cd wp(1, 0), wt(1, 0);
+				for (int t = 0; t < p; t++) wp = wp * w;
+				if (tpc <= P)
+					for (int t = 0; t < tpc; t++) wt = wt * w;
+#pragma unroll
+				for (int i = 0; i < NPT; i++)
+				{
+					if (p + i * tpc <= P) acc[i] += wp;
+					wp = wp * wt;
+				}
+			}
+		const size_t cid = rowc + g.off[L] + (size_t)iy * (g.nx << L) + ix;
+#pragma unroll
+		for (int i = 0; i < NPT; i++)
+		{
+			const int n = p + i * tpc;
+			if (n <= P)
+			{
+				am[k][n] = make_double2(acc[i].re, acc[i].im);
+				V.mp[cid * NC + n] = am[k][n];
+			}
+		}
+		if (p == 0)
+		{
+			tcnt[k] = ce - cb;
+			V.ccnt[cid] = ce - cb;
+		}
+	}
+	__syncthreads();
+	// M2M inside the region, level by level up to rlev; a level of side sl
+	// follows its children's level (side 2 sl) in am and tcnt
+	int cofs = 0;
+	for (int l = L - 1; l >= Q; l--)
+	{
+		const int sl = 1 << (l - Q), nc = sl * sl, pofs = cofs + 4 * nc, nxl = g.nx << l;
+		for (int i = threadIdx.x; i < nc * NC; i += NT)
+		{
+			const int c = i / NC, n = i - c * NC, cx = c & (sl - 1), cy = c / sl;
+			cd sum(0, 0);
+			int tot = 0;
+			for (int qd = 0; qd < 4; qd++)
+			{
+				const int a = qd & 1, b = qd >> 1, ch = cofs + (2 * cy + b) * (2 * sl) + 2 * cx + a;
+				tot += tcnt[ch];
+				if (tcnt[ch] > 0) sum += m2m_term<P>(am[ch], n, cd(a - 0.5, b - 0.5), binom);
+			}
+			const size_t cid = rowc + g.off[l] + (size_t)((ry << (l - Q)) + cy) * nxl + (rx << (l - Q)) + cx;
+			am[pofs + c][n] = make_double2(sum.re, sum.im);
+			V.mp[cid * NC + n] = am[pofs + c][n];
+			if (n == 0)
+			{
+				tcnt[pofs + c] = tot;
+				V.ccnt[cid] = tot;
+			}
+		}
+		__syncthreads();
+		cofs = pofs;
+	}
+	// the row's last workgroup to finish forms the levels above rlev from
+	// every region's cell at rlev, then resets the row's ticket
+	if (threadIdx.x == 0)
+	{
+		__threadfence();
+		s_last = atomicAdd(V.rticket + r, 1) == nreg - 1;
+	}
+	__syncthreads();
+	if (!s_last) return;
+	__threadfence();
+	if (threadIdx.x == 0) V.rticket[r] = 0;
+	for (int l = Q - 1; l >= 0; l--)
+	{
+		const int nxl = g.nx << l, nyl = g.ny << l, nxc = 2 * nxl;
+		for (int i = threadIdx.x; i < nxl * nyl * NC; i += NT)
+		{
+			const int c = i / NC, n = i - c * NC, ix = c % nxl, iy = c / nxl;
+			cd sum(0, 0);
+			int tot = 0;
+			for (int qd = 0; qd < 4; qd++)
+			{
+				const int a = qd & 1, b = qd >> 1;
+				const size_t ch = rowc + g.off[l + 1] + (size_t)(2 * iy + b) * nxc + 2 * ix + a;
+				const int nch = V.ccnt[ch];
+				tot += nch;
+				if (nch > 0) sum += m2m_term<P>(V.mp + ch * NC, n, cd(a - 0.5, b - 0.5), binom);
+			}
+			const size_t cid = rowc + g.off[l] + c;
+			V.mp[cid * NC + n] = make_double2(sum.re, sum.im);
+			if (n == 0) V.ccnt[cid] = tot;
+		}
+		__syncthreads();
+	}
+}
+
+constexpr int HALO_SIDE = REGION_SIDE + 2;        // a region's leaf cells and their neighbours
+constexpr int HALO_CELLS = HALO_SIDE * HALO_SIDE;
+constexpr int STAGE_BATCH = 4;                    // loads per thread issued together when staging
+constexpr int ROOT_BATCH = 2;                     // roots per thread whose list loads k_rdown issues together
+
+// rsum = -S for every listed root of a row in FMM mode (the root part of
+// the Aberth correction, the sign that the direct kernels' partials carry)
+template <int P>
+__global__ __launch_bounds__(REGION_THREADS) void k_rdown(RootView V, const double2* __restrict__ roots, RowMap M,
+	unsigned long long* __restrict__ stats)
+{
+	constexpr int NT = REGION_THREADS, NC = P + 1, W = REGION_SIDE + 4, SRC = W * W * NC, SB = STAGE_BATCH;
+	// a level's sources (coefficient n of staged cell c at c * NC + n), then
+	// the positions of the roots of the region's cells and their neighbours
+	__shared__ double2 src[SRC];
+	__shared__ int sok[W * W];
+	__shared__ double2 lbuf[2][REGION_CELLS][NC]; // local expansions of a level and of its parents
+	__shared__ int cst[REGION_CELLS + 1];
+	__shared__ int hst[HALO_CELLS + 1], hsrc[HALO_CELLS];
+	__shared__ int wsum[NT / 64];
+	const int r = blockIdx.y;
+	if (!V.rmode[r]) return;
+	const RootGrid& g = V.g;
+	const int Q = g.rlev, L = g.leaf, s = L - Q, RS = 1 << s, RC = RS * RS;
+	const int nxq = g.nx << Q;
+	const int R = M.R;
+	int row;
+	double phi;
+	M.get(r, row, phi);
+	const double2* __restrict__ zr = roots + (size_t)row * R;
+	const int rg = blockIdx.x, rx = rg % nxq, ry = rg / nxq;
+	const size_t rowc = (size_t)r * g.ncells;
+	const int* __restrict__ cc = V.ccnt + rowc;
+	const int* __restrict__ st = V.cstart + (size_t)r * (g.nleaf + 1);
+	for (int i = threadIdx.x; i <= RC; i += NT) cst[i] = st[(size_t)rg * RC + i];
+	unsigned long long nm = 0;
+	for (int l = 0; l <= L; l++)
+	{
+		// the level's targets: the region's ancestor (l <= rlev) or its
+		// side x side cells, tpt threads per target
+		const int nxl = g.nx << l, nyl = g.ny << l, lofs = g.off[l];
+		const int side = l <= Q ? 1 : 1 << (l - Q);
+		const int x0 = l <= Q ? rx >> (Q - l) : rx << (l - Q), y0 = l <= Q ? ry >> (Q - l) : ry << (l - Q);
+		const int T = side * side, tpt = min(32, NT / T);
+		const int t = threadIdx.x / tpt, q = threadIdx.x - t * tpt;
+		const int ix = x0 + t % side, iy = y0 + t / side;
+		const bool live = t < T && cc[lofs + iy * nxl + ix] > 0;
+		const double ih = 2.0 / ldexp(g.s0, -l);
+		double2(*lcur)[NC] = lbuf[l & 1];
+		const double2(*lpar)[NC] = lbuf[(l & 1) ^ 1];
+		cd Lc[NC];
+#pragma unroll
+		for (int k = 0; k < NC; k++) Lc[k] = cd(0, 0);
+		// the sources, staged in LDS: at level 0 every top cell (W * W at a
+		// time); below, the block of cells that holds the targets' far lists,
+		// for side >= 2 (x0, y0 even) cells x0 - 2 .. x0 + side + 1, for a
+		// single target the 6 x 6 block of its parent's neighbourhood
+		const int bx0 = l == 0 ? 0 : (side >= 2 ? x0 - 2 : 2 * (x0 >> 1) - 2);
+		const int by0 = l == 0 ? 0 : (side >= 2 ? y0 - 2 : 2 * (y0 >> 1) - 2);
+		const int Wl = l == 0 ? nxl : max(side + 4, 6);
+		const int ncand = l == 0 ? nxl * nyl : Wl * Wl;
+		for (int c0 = 0; c0 < ncand; c0 += W * W)
+		{
+			const int nch = min(W * W, ncand - c0), ne = nch * NC;
+			for (int i0 = threadIdx.x; i0 < ne; i0 += NT * SB)
+			{
+				double2 v[SB];
+				int n0[SB];
+#pragma unroll
+				for (int b = 0; b < SB; b++)
+				{
+					const int i = i0 + b * NT, c = i / NC, n = i - c * NC, gi = c0 + c;
+					const int jx = bx0 + gi % Wl, jy = by0 + gi / Wl;
+					const bool in = i < ne && jx >= 0 && jy >= 0 && jx < nxl && jy < nyl;
+					const size_t cid = lofs + (size_t)jy * nxl + jx;
+					v[b] = in ? V.mp[(rowc + cid) * NC + n] : make_double2(0, 0);
+					n0[b] = in && n == 0 ? cc[cid] : 0;
+				}
+#pragma unroll
+				for (int b = 0; b < SB; b++)
+				{
+					const int i = i0 + b * NT, c = i / NC;
+					if (i < ne)
+					{
+						src[i] = v[b];
+						if (i == c * NC) sok[c] = n0[b] > 0;
+					}
+				}
+			}
+			__syncthreads();
+			if (live && l >= 1 && q == 0 && c0 == 0)
+			{
+				// L2L into the group's first lane (the parents' expansions
+				// were written before the barrier above): 2^-m sum_{n >= m}
+				// C(n, m) L_n(parent) e^(n - m), the parent's polynomial
+				// shifted by e (repeated synthetic division)
+				const int ps = max(side >> 1, 1);
+				const int pi = ((iy >> 1) - (y0 >> 1)) * ps + ((ix >> 1) - (x0 >> 1));
+#pragma unroll
+				for (int n = 0; n < NC; n++) Lc[n] = cd(lpar[pi][n].x, lpar[pi][n].y);
+				const cd e((ix & 1) - 0.5, (iy & 1) - 0.5);
+#pragma unroll
+				for (int i = 0; i < P; i++)
+#pragma unroll
+					for (int j = P - 1; j >= i; j--) Lc[j] += e * Lc[j + 1];
+#pragma unroll
+				for (int m = 0; m < NC; m++) Lc[m] *= ldexp(1.0, -m);
+			}
+			if (live)
+			{
+				if (l == 0)
+				{
+					// every top cell that is not a neighbour
+					for (int k = q; k < nch; k += tpt)
+					{
+						const int jx = (c0 + k) % nxl, jy = (c0 + k) / nxl;
+						if ((abs(jx - ix) >= 2 || abs(jy - iy) >= 2) && sok[k])
+						{
+							nm++;
+							m2l_add<P>(src + k * NC, ix - jx, iy - jy, ih, Lc);
+						}
+					}
+				}
+				else
+				{
+					// entry v of the far list is cell (v % 6, v / 6) of the 6 x 6
+					// block whose first cell is (2 (ix / 2) - 2, 2 (iy / 2) - 2)
+					const int o = (ix & 1) + 2 * (iy & 1), bx = 2 * (ix >> 1) - 2, by = 2 * (iy >> 1) - 2;
+					for (int k = q; k < 27; k += tpt)
+					{
+						const int v = kFarList.v[o][k];
+						const int jx = bx + v % 6, jy = by + v / 6;
+						const int c = (jy - by0) * Wl + (jx - bx0);
+						if (!sok[c]) continue;
+						nm++;
+						m2l_add<P>(src + c * NC, ix - jx, iy - jy, ih, Lc);
+					}
+				}
+			}
+			__syncthreads();
+		}
+		// the target's partials over its tpt threads (aligned groups of
+		// lanes; the butterfly leaves the same sum in each)
+		for (int d = 1; d < tpt; d <<= 1)
+#pragma unroll
+			for (int k = 0; k < NC; k++)
+			{
+				Lc[k].re += __shfl_xor(Lc[k].re, d, 64);
+				Lc[k].im += __shfl_xor(Lc[k].im, d, 64);
+			}
+		// read by the next level after its staging barrier; the buffer it
+		// overwrites held the parents, read before this level's barrier
+		if (live)
+#pragma unroll
+			for (int m = 0; m < NC; m++)
+				if ((m & (tpt - 1)) == q) lcur[t][m] = make_double2(Lc[m].re, Lc[m].im);
+	}
+	// the near-field sources: the roots of the region's leaf cells and of
+	// their neighbours, HS x HS cells from (hx0, hy0), copied into LDS when
+	// they fit (cell k's at hst[k] ..), else read where they lie
+	const int nxl = g.nx << L, nyl = g.ny << L, HS = RS + 2, hx0 = (rx << s) - 1, hy0 = (ry << s) - 1;
+	const double2* __restrict__ sp = V.spos + (size_t)r * R;
+	int hc = 0;
+	if ((int)threadIdx.x < HS * HS)
+	{
+		const int jx = hx0 + threadIdx.x % HS, jy = hy0 + threadIdx.x / HS;
+		if (jx >= 0 && jy >= 0 && jx < nxl && jy < nyl)
+		{
+			const int key = leaf_key(g, jx, jy), b = st[key];
+			hc = st[key + 1] - b;
+			hsrc[threadIdx.x] = b;
+		}
+	}
+	int htot;
+	const int ho = block_scan_excl(hc, wsum, htot);
+	if ((int)threadIdx.x < HS * HS) hst[threadIdx.x] = ho;
+	if (threadIdx.x == 0) hst[HS * HS] = htot;
+	const bool hlds = htot <= SRC;
+	__syncthreads();
+	if (hlds)
+#pragma unroll 4
+		for (int i = threadIdx.x; i < htot; i += NT)
+		{
+			const int k = region_cell(hst, HS * HS, i);
+			src[i] = sp[hsrc[k] + i - hst[k]];
+		}
+	__syncthreads();
+	// L2P, the near field (the 3 x 3 leaf cells around the root's) and the
+	// roots outside the grid, per listed root of the region, ROOT_BATCH at a
+	// time per thread
+	const double2(*lleaf)[NC] = lbuf[L & 1];
+	const double ih = 2.0 / ldexp(g.s0, -L);
+	const int no = V.ocnt[r];
+	const int* __restrict__ si = V.sidx + (size_t)r * R;
+	const int* __restrict__ oi = V.oidx + (size_t)r * R;
+	unsigned long long nn = 0, nt = 0;
+	for (int q0 = cst[0] + threadIdx.x; q0 < cst[RC]; q0 += NT * ROOT_BATCH)
+	{
+		int sl[ROOT_BATCH];
+#pragma unroll
+		for (int b = 0; b < ROOT_BATCH; b++)
+		{
+			const int qq = q0 + b * NT;
+			sl[b] = qq < cst[RC] ? si[qq] : -1;
+		}
+#pragma unroll
+		for (int b = 0; b < ROOT_BATCH; b++) sl[b] = sl[b] >= 0 ? V.slot[(size_t)r * R + sl[b]] : -1;
+#pragma unroll
+		for (int b = 0; b < ROOT_BATCH; b++)
+		{
+			const int qq = q0 + b * NT;
+			if (sl[b] < 0) continue;
+			const int k = region_cell(cst, RC, qq);
+			const int kx = k & (RS - 1), ky = k >> s, ix = (rx << s) + kx, iy = (ry << s) + ky;
+			// the root's own cell in the near-field block: (kx + 1, ky + 1)
+			const int hk = (ky + 1) * HS + kx + 1;
+			const double2 zz = hlds ? src[hst[hk] + qq - hsrc[hk]] : sp[qq];
+			const cd z(zz.x, zz.y);
+			const cd w = (z - cell_center(g, L, ix, iy)) * ih;
+			cd S(lleaf[k][P].x, lleaf[k][P].y);
+#pragma unroll
+			for (int m = P - 1; m >= 0; m--) S = S * w + cd(lleaf[k][m].x, lleaf[k][m].y);
+			double ax = 0, ay = 0;
+			for (int d = 0; d < 9; d++)
+			{
+				const int h = hk + (d / 3 - 1) * HS + d % 3 - 1;
+				const int u0 = hst[h], u1 = hst[h + 1];
+				nn += u1 - u0;
+				if (hlds)
+				{
+					for (int u = u0; u < u1; u++) direct_term<P>(z, src[u], ax, ay);
+				}
+				else
+				{
+					const int gb = hsrc[h] - u0;
+#pragma unroll 4
+					for (int u = u0; u < u1; u++) direct_term<P>(z, sp[gb + u], ax, ay);
+				}
+			}
+			for (int o = 0; o < no; o++) direct_term<P>(z, zr[oi[o]], ax, ay);
+			S += cd(ax, -ay);
+			V.rsum[(size_t)r * R + sl[b]] = make_double2(-S.re, -S.im);
+			nt++;
+		}
+	}
+	const int which[3] = {C_RM2L, C_RNEAR, C_RTGT};
+	unsigned long long v[3] = {nm, nn, nt};
+	block_stats<3, REGION_THREADS>(stats, which, v);
 }
 
 // the FMM pass of one sweep over nrows rows (rows not in FMM mode exit at once)
@@ -939,25 +997,9 @@ void root_fmm_pass_p(const SweepBufs& B, const double2* roots, const RowMap& M, 
 		k_rscan<<<nrows, 1024, 0, s>>>(V, R, stats);
 		k_rscatter<<<dim3(grid(R), nrows), BLOCK, 0, s>>>(V, R);
 	}
-	k_rleaf<P><<<dim3(grid(g.nleaf), nrows), BLOCK, 0, s>>>(V, roots, M);
-	for (int l = g.leaf - 1; l >= 0; l--)
-		k_rm2m<P><<<dim3(grid((g.nx << l) * (g.ny << l)), nrows), BLOCK, 0, s>>>(V, l);
-	constexpr int TPB = down_targets<P>();
-	for (int l = 0; l <= g.leaf; l++)
-	{
-		// from 1024 cells (16 tiles) per row, with nx 2^l and ny 2^l multiples
-		// of DOWN_TILE (l >= 3); on smaller levels a few workgroups would
-		// each carry a whole tile's lists
-		if constexpr (down_tiled<P>())
-			if (l >= 3 && (g.nx << l) * (g.ny << l) >= 1024)
-			{
-				k_rdown_tile<P><<<dim3((g.nx << l) / DOWN_TILE * ((g.ny << l) / DOWN_TILE), nrows), DOWN_TILE_THREADS, 0, s>>>(V, l, stats);
-				continue;
-			}
-		k_rdown32<P><<<dim3(grid((g.nx << l) * (g.ny << l), TPB), nrows), 32 * TPB, 0, s>>>(V, l, stats);
-	}
-	k_rsum<P><<<dim3(grid(R), nrows), BLOCK, 0, s>>>(V, roots, M, stats);
-	k_rsum_out<P><<<dim3(nrows, OUT_BLOCKS), OUT_THREADS, 0, s>>>(V, roots, M, stats);
+	const int nreg = (g.nx << g.rlev) * (g.ny << g.rlev);
+	k_rup<P><<<dim3(nreg + RSUM_OUT_BLOCKS, nrows), REGION_THREADS, 0, s>>>(V, roots, M, stats);
+	k_rdown<P><<<dim3(nreg, nrows), REGION_THREADS, 0, s>>>(V, roots, M, stats);
 }
 
 // compiled expansion orders; a requested order runs at the next one up
@@ -1010,6 +1052,7 @@ RootGrid plan_root_grid(double hx, double hy, int R, int order, double leaf_targ
 		}
 	}
 	g.leaf = best;
+	g.rlev = std::max(0, best - REGION_LOG);
 	g.order = order;
 	int tot = 0;
 	for (int l = 0; l <= g.leaf; l++)

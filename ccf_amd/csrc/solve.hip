@@ -156,7 +156,13 @@ __device__ inline void block_stats(unsigned long long* stats, const int (&which)
 	{
 		unsigned long long s = 0;
 		for (int q = 0; q < BS / 64; q++) s += red[threadIdx.x][q];
-		if (s) atomicAdd(stat_slot(stats, which[threadIdx.x]), s);
+		// the counter by a select chain: a local array indexed by the thread
+		// would be placed in scratch memory
+		int c = which[0];
+#pragma unroll
+		for (int k = 1; k < NC; k++)
+			if ((int)threadIdx.x == k) c = which[k];
+		if (s) atomicAdd(stat_slot(stats, c), s);
 	}
 	__syncthreads();
 }
@@ -1051,6 +1057,13 @@ __device__ __forceinline__ void tail_settle(const LensView& L, int a, double2* p
 	cd f1, ex;
 	const int st = witt_root<true>(L, pz, z, w, circ_exact, fmm_pole, newton, f1, ex);
 	nnear += w.nr.count;
+	// counted without branches: increments chosen by branch are merged into
+	// one through a pointer, which places the counters in scratch memory
+	const bool settled = st != ROOT_FULL && st != ROOT_STEPPED;
+	nfull += st == ROOT_FULL;
+	nnewton += st == ROOT_STEPPED || st == ROOT_NEWTON;
+	nconv += settled;
+	nearly += settled && st != ROOT_NEWTON;
 	if (st == ROOT_FULL)
 	{
 		const int k = atomicAdd(&s_nf, 1);
@@ -1069,20 +1082,12 @@ __device__ __forceinline__ void tail_settle(const LensView& L, int a, double2* p
 			gf1[k] = v1;
 			gex[k] = ve;
 		}
-		nfull++;
 	}
 	else if (st == ROOT_STEPPED)
 	{
 		const int k = atomicAdd(&s_nn, 1);
 		if (k < TAIL_LDS) s_next[k] = a;
 		else gnxt[k] = a;
-		nnewton++;
-	}
-	else
-	{
-		nconv++;
-		if (st == ROOT_NEWTON) nnewton++;
-		else nearly++;
 	}
 }
 
@@ -1846,8 +1851,8 @@ bool root_layout(ccf_ctx* ctx, SweepBufs& b, Take&& take)
 	const size_t ce = (size_t)rows * g.ncells * (g.order + 1);
 	return take(b.slot, list) && take(b.rmode, (size_t)rows) && take(b.rsum, list) && take(b.ccnt, (size_t)rows * g.ncells) &&
 		take(b.cstart, (size_t)rows * (g.nleaf + 1)) && take(b.ccur, (size_t)rows * g.nleaf) && take(b.cell, list) &&
-		take(b.sidx, list) && take(b.spos, list) && take(b.ocnt, (size_t)rows) && take(b.oidx, list) && take(b.mp, ce) &&
-		take(b.lc, ce);
+		take(b.bidx, list) && take(b.sidx, list) && take(b.spos, list) && take(b.ocnt, (size_t)rows) && take(b.oidx, list) &&
+		take(b.mp, ce) && take(b.rticket, (size_t)rows);
 }
 
 // the root-sum FMM's arrays alone (ccf_root_sums), from the context's pool
@@ -1857,6 +1862,7 @@ bool alloc_root_fmm(ccf_ctx* ctx, SweepBufs& b)
 	return root_layout(ctx, b, take) &&
 		CCF_HIP(ctx, hipMemsetAsync(b.slot, 0xff, (size_t)b.rows * b.R * sizeof(int), ctx->stream)) &&
 		CCF_HIP(ctx, hipMemsetAsync(b.rmode, 0, b.rows * sizeof(int), ctx->stream)) &&
+		CCF_HIP(ctx, hipMemsetAsync(b.rticket, 0, b.rows * sizeof(int), ctx->stream)) &&
 		CCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
 }
 
@@ -1867,8 +1873,8 @@ void free_sweep(ccf_ctx* ctx, SweepBufs& b)
 	auto F = [ctx](auto*& p) { ctx->dfree(p); p = nullptr; };
 	F(b.act); F(b.cnt); F(b.fl); F(b.fcnt); F(b.dl); F(b.dcnt); F(b.f0); F(b.f1); F(b.ex); F(b.tile_off); F(b.plan); F(b.ticket);
 	F(b.part);
-	F(b.slot); F(b.rmode); F(b.rsum); F(b.ccnt); F(b.cstart); F(b.ccur); F(b.cell); F(b.sidx); F(b.spos); F(b.ocnt);
-	F(b.oidx); F(b.mp); F(b.lc);
+	F(b.slot); F(b.rmode); F(b.rsum); F(b.ccnt); F(b.cstart); F(b.ccur); F(b.cell); F(b.bidx); F(b.sidx); F(b.spos);
+	F(b.ocnt); F(b.oidx); F(b.mp); F(b.rticket);
 	b = SweepBufs{};
 }
 
@@ -1886,6 +1892,7 @@ bool alloc_sweep(ccf_ctx* ctx, SweepBufs& b, int rows, int R)
 	if (!b.slot) return true;
 	return CCF_HIP(ctx, hipMemsetAsync(b.slot, 0xff, list * sizeof(int), ctx->stream)) &&
 		CCF_HIP(ctx, hipMemsetAsync(b.rmode, 0, rows * sizeof(int), ctx->stream)) &&
+		CCF_HIP(ctx, hipMemsetAsync(b.rticket, 0, rows * sizeof(int), ctx->stream)) &&
 		CCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
 }
 
@@ -2428,11 +2435,12 @@ bool aberth_sweep_noalloc_dev(ccf_ctx* ctx, const double* roots, int n, double p
 	carve(row, (size_t)n);
 	if (!sweep_layout(ctx, B, 1, n, true, carve)) return false;
 	hipStream_t s = ctx->stream;
-	// the Witt kernel's ticket starts at 0, the slots of listed roots at -1
-	// and the row's FMM mode at 0; k_prepare sets the active list and clears
-	// the counts
+	// the Witt kernel's ticket starts at 0, the slots of listed roots at -1,
+	// the row's FMM mode and its region ticket at 0; k_prepare sets the
+	// active list and clears the counts
 	if (!fill_int(ctx, B.ticket, 1, 0)) return false;
-	if (B.slot && !(fill_int(ctx, B.slot, (size_t)n, -1) && fill_int(ctx, B.rmode, 1, 0))) return false;
+	if (B.slot && !(fill_int(ctx, B.slot, (size_t)n, -1) && fill_int(ctx, B.rmode, 1, 0) && fill_int(ctx, B.rticket, 1, 0)))
+		return false;
 	if (!CCF_HIP(ctx, hipMemcpyAsync(row, roots, (size_t)n * sizeof(double2), hipMemcpyHostToDevice, s))) return false;
 	const RowMap M{n, 2, 1, 0, 0, phi};
 	k_prepare<<<dim3(grid(n), 1), BLOCK, 0, s>>>(row, M, B, 0, nullptr);
