@@ -1093,7 +1093,7 @@ __device__ __forceinline__ void tail_settle(const LensView& L, int a, double2* p
 
 __global__ __launch_bounds__(TAIL_BLOCK) void k_tail(LensView L, double2* __restrict__ roots, RowMap M, SweepView V,
 	unsigned long long* __restrict__ stats, int circ_exact, int fmm_pole, int cap, float cluster_d2,
-	const float2* __restrict__ spos32, int Np)
+	const float2* __restrict__ spos32, int Np, double theta)
 {
 	__shared__ int s_a[2][TAIL_LDS];
 	__shared__ int s_fl[TAIL_LDS];
@@ -1118,12 +1118,63 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_tail(LensView L, double2* __rest
 	double2* __restrict__ gf0 = V.f0 + base;
 	double2* __restrict__ gf1 = V.f1 + base;
 	double2* __restrict__ gex = V.ex + base;
-	int n = V.cnt[r];
-	// the first list's head into LDS; its other entries stay where they are
-	for (int q = threadIdx.x; q < min(n, TAIL_LDS); q += TAIL_BLOCK) s_a[0][q] = gcur[q];
-	if (threadIdx.x == 0) s_nf = s_nn = s_nd = 0;
-	__syncthreads();
 	unsigned long long nfull = 0, nearly = 0, nnewton = 0, nconv = 0, nnear = 0, nclus = 0, nsweeps = 0;
+	// the first list: the roots the previous sweep's Witt kernels stepped
+	// (its head into LDS; its other entries stay where they are), then the
+	// previous sweep's updates of the row (k_finalize's work): its listed
+	// roots (the other parity's count) take their sums and move, and those
+	// not converged join the list
+	const int n0 = V.cnt[r];
+	for (int q = threadIdx.x; q < min(n0, TAIL_LDS); q += TAIL_BLOCK) s_a[0][q] = gcur[q];
+	if (threadIdx.x == 0)
+	{
+		s_nf = s_nd = 0;
+		s_nn = n0;
+	}
+	__syncthreads();
+	{
+		const int nl = V.fcnt_next[r], S = V.plan[1];
+		for (int i = threadIdx.x; i < nl; i += TAIL_BLOCK)
+		{
+			const int a = gfl[i];
+			double2* pz = xrow + a;
+			const cd z(pz->x, pz->y);
+			const double2 e = gex[i];
+			cd sum(e.x, e.y);
+			if (V.rmode && V.rmode[r])
+			{
+				const double2 q = V.rsum[base + i];
+				sum += cd(q.x, q.y);
+			}
+			else
+			{
+				const size_t g = (size_t)(V.tile_off[r] + i / V.tile);
+				const int loc = i % V.tile;
+				for (int sl = 0; sl < S; sl++)
+				{
+					const double2 q = V.part[(g * S + sl) * V.tile + loc];
+					sum += cd(q.x, q.y);
+				}
+			}
+			if (V.slot) V.slot[base + a] = -1;
+			const double2 a0 = gf0[i], a1 = gf1[i];
+			const cd f0(a0.x, a0.y), f1(a1.x, a1.y);
+			const cd zn = z - f0 / (f1 + f0 * sum);
+			*pz = make_double2(zn.re, zn.im);
+			if ((zn - z).abs() / theta < CONV_TOL) nconv++;
+			else
+			{
+				const int k = atomicAdd(&s_nn, 1);
+				if (k < TAIL_LDS) s_a[0][k] = a;
+				else gcur[k] = a;
+			}
+		}
+	}
+	__syncthreads();
+	int n = s_nn;
+	__syncthreads();
+	if (threadIdx.x == 0) s_nn = 0;
+	__syncthreads();
 	int p = 0;
 	for (int it = 0; it < cap && n > 0; it++)
 	{
@@ -1673,14 +1724,16 @@ bool run_sweeps_groups(ccf_ctx* ctx, SweepGroup* gs, int G, double2* roots, int 
 				(void)hipEventRecord(tm.b, s);
 				ctx->timers.push_back(tm);
 			}
-			k_finalize<<<g_rows, BLOCK, 0, s>>>(ctx->theta, roots, M, V, ctx->counters);
+			// mixed form, continuation: the sweeps after the first tail_at of
+			// the step in one launch, with no read-back, which begins with
+			// this sweep's updates. The rows the root-sum FMM took in the
+			// step's first two sweeps go to the host for the choice two steps
+			// later.
+			const bool tail_now = tail_at > 0 && it + 1 == tail_at && it + 1 < cap;
+			if (!tail_now) k_finalize<<<g_rows, BLOCK, 0, s>>>(ctx->theta, roots, M, V, ctx->counters);
 			if (!CCF_LAUNCH(ctx, "find_critical_curve_roots_kernel")) return false;
 			ctx->info.sweeps++;
-			// mixed form, continuation: the sweeps after the first tail_at of
-			// the step in one launch, with no read-back. The rows the root-sum
-			// FMM took in the step's first two sweeps go to the host for the
-			// choice two steps later.
-			if (tail_at > 0 && it + 1 == tail_at && it + 1 < cap)
+			if (tail_now)
 			{
 				if (ctx->host_counts && nrows <= ctx->max_rows)
 				{
@@ -1713,7 +1766,7 @@ bool run_sweeps_groups(ccf_ctx* ctx, SweepGroup* gs, int G, double2* roots, int 
 					if (tt.a && tt.b) (void)hipEventRecord(tt.a, s);
 				}
 				k_tail<<<nrows, TAIL_BLOCK, 0, s>>>(L, roots, M, Vn, ctx->counters, circ_exact, fmm_pole ? 1 : 0, cap - (it + 1),
-					(float)(d * d), ctx->spos32, Np);
+					(float)(d * d), ctx->spos32, Np, ctx->theta);
 				if (ctx->P.timing && tt.a && tt.b)
 				{
 					(void)hipEventRecord(tt.b, s);
