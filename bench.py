@@ -320,6 +320,39 @@ def run_batch(fields, precision: int, predictor: int, steps: int, warmup: int):
                 expansion_orders=orders, t_tree=inf["t_tree"], t_solve=inf["t_ccs"]), ccs
 
 
+def run_batched_workload(W: dict, tmp: str, nfields: int, predictor: int, steps: int, warmup: int):
+    """the batched form (one persistent launch per phase, HIP graph) over
+    nfields fields of the workload's size and shape, seeds seed, seed + 1, ...
+    (mixed precision, the continuation predictor): whole-batch points/s"""
+    fields = []
+    for k in range(nfields):
+        path = os.path.join(tmp, f"batched_{k}.bin")
+        ccf_amd.write_synthetic_stars(path, W["seed"] + k, **W["stars"])
+        fields.append((path, W["run"], None))
+    b = ccf_amd.Batch()
+    for path, kw, _ in fields:
+        b.add(starfile=path, outfile_prefix="./", write_stars=0, write_critical_curves=0, write_caustics=1,
+              precision=1, predictor=predictor, **kw)
+    times = []
+    for i in range(warmup + steps):
+        b.load()
+        t0 = time.perf_counter()
+        b.build()
+        b.solve()
+        dt = time.perf_counter() - t0
+        print(f"# batched {nfields} fields step {i} ({'warmup' if i < warmup else 'timed'}): {dt:.3f} s "
+              f"(workgroup {b.workgroup()})", file=sys.stderr, flush=True)
+        if i >= warmup:
+            times.append(dt)
+    inf = b.info(-1)
+    wg = b.workgroup()
+    b.close()
+    t = float(np.mean(times))
+    return dict(fields=nfields, points_per_s=inf["points_converged"] / t, s_per_step=t,
+                points_converged=inf["points_converged"], points_total=inf["points_total"], sweeps=inf["sweeps"],
+                workgroup=wg, t_tree=inf["t_tree"], t_solve=inf["t_ccs"])
+
+
 def run_sequential(fields, nfields: int):
     """the plain FP64 single-field path, one field after another (stars in
     HBM before each timed field, as for the batch)"""
@@ -401,6 +434,8 @@ def main():
     ap.add_argument("--root-fmm-order", type=int, default=0, help="its expansion order (0: the library default)")
     ap.add_argument("--root-fmm-leaf", type=int, default=0, help="its mean roots per leaf cell (0: the library default)")
     ap.add_argument("--ablation", action="store_true", help="also time the mixed form without the predictor")
+    ap.add_argument("--batched-fields", type=int, default=8,
+                    help="the batched mixed form over this many fields of the workload's size (0: skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device: the known-size kernel that scales the HBM byte counters")
     args = ap.parse_args()
@@ -466,6 +501,12 @@ def main():
             if name == "mixed":
                 job_pairs = dict(pairs=pairs)
             res[name]["speedup_vs_fp64"] = res[name]["points_per_s"] / res["fp64"]["points_per_s"]
+
+    batched = None
+    if args.batched_fields > 0 and args.precision != "fp64":
+        batched = run_batched_workload(W, tmp, args.batched_fields, args.predictor, 2, 1)
+        if "fp64" in res:
+            batched["vs_fp64_single_field"] = batched["points_per_s"] / res["fp64"]["points_per_s"]
 
     head = res["mixed"] if "mixed" in res else res["fp64"]
     mixed_head = "mixed" in res
@@ -553,6 +594,7 @@ def main():
         "fp64": res.get("fp64"),
         "mixed": res.get("mixed"),
         "ablation": {k: v for k, v in res.items() if k not in ("fp64", "mixed")} or None,
+        "batched": batched,
     }
     print(json.dumps(line))
 

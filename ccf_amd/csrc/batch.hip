@@ -13,7 +13,10 @@
 //               that side toward 0 or 2 pi (ccf.cuh:979-1083), each step
 //               predicted from the previous ones as in the single-field path.
 //    One workgroup runs a task. The R roots of its row live in LDS (with the
-//    field's stars in the FP64 form, when they fit); a Jacobi sweep is three
+//    field's stars in the FP64 form, when they fit), or, for a field whose
+//    rows exceed the LDS a workgroup can hold (20 000 roots at 10 000 stars),
+//    in the output row itself in global memory, which the L2 serves; a
+//    Jacobi sweep is three
 //    barrier-separated phases over a compacted list of the active roots:
 //    the Witt function and the 1/mu early exit (ccf_functions.cuh:197-213),
 //    the Cauchy sums over stars and roots (ccf_functions.cuh:214-242), then
@@ -384,19 +387,19 @@ __device__ void solve_row(const BField& F, int row, int cap, bool newton, double
 }
 
 // The persistent solve of one phase (0: branch centres, 1: phi chains).
-// Dynamic LDS: rmax roots and nlds stars (fields with more stars read them
-// from global memory). lists: the active lists and the listed roots, rmax
-// entries each per workgroup of the grid (L2-resident; in LDS they would
+// Dynamic LDS: rlds roots and nlds stars (fields with more stars read them
+// from global memory; a field with more than rlds roots solves its rows in
+// place in global memory). lists: the active lists and the listed roots,
+// rmax entries each per workgroup of the grid (L2-resident; in LDS they would
 // leave room for fewer workgroups per CU). At most 128 VGPRs: four waves
 // per SIMD.
 constexpr int BATCH_WAVES_PER_SIMD = 4;
 template <int BS, bool MIXED>
 __global__ __launch_bounds__(BS, BATCH_WAVES_PER_SIMD) void k_batch(const BField* __restrict__ fields, const BTask* __restrict__ tasks,
-	int ntasks, int* __restrict__ next, int phase, int rmax, int nlds, ListSlice lists)
+	int ntasks, int* __restrict__ next, int phase, int rmax, int rlds, int nlds, ListSlice lists)
 {
 	extern __shared__ double2 smem[];
-	double2* zc = smem;
-	double2* sl = smem + rmax;
+	double2* sl = smem + rlds;
 	__shared__ int s_task;
 	__shared__ int s_n[2];
 	__shared__ unsigned long long s_red[BCNT][BS / 64];
@@ -421,12 +424,16 @@ __global__ __launch_bounds__(BS, BATCH_WAVES_PER_SIMD) void k_batch(const BField
 			st = sl;
 		}
 		unsigned long long cnt[BCNT] = {0, 0, 0, 0, 0, 0};
+		// the row being solved: in LDS, or in place in the output rows
+		const bool lds_row = R <= rlds;
 		if (phase == 0)
 		{
 			const int row = centre_row(F, tk.branch);
+			double2* zc = lds_row ? smem : F.roots + (size_t)row * R;
 			for (int i = threadIdx.x; i < R; i += BS) zc[i] = F.seeds[i];
 			solve_row<BS, MIXED>(F, row, F.cap_init, false, zc, st, la, lb, ls, s_n, cnt);
-			for (int i = threadIdx.x; i < R; i += BS) F.roots[(size_t)row * R + i] = zc[i];
+			if (lds_row)
+				for (int i = threadIdx.x; i < R; i += BS) F.roots[(size_t)row * R + i] = zc[i];
 		}
 		else
 		{
@@ -439,10 +446,13 @@ __global__ __launch_bounds__(BS, BATCH_WAVES_PER_SIMD) void k_batch(const BField
 				// prepare_roots_kernel (ccf_functions.cuh:261-295); a predictor
 				// extrapolates from the rows this thread itself wrote
 				const int q = min(F.predictor, j - 1);
+				double2* zc = lds_row ? smem : F.roots + (size_t)row * R;
+				// (in place: row is written, the rows before it are read)
 				for (int i = threadIdx.x; i < R; i += BS)
 					zc[i] = predict_root(F.roots + (size_t)(row - sgn) * R + i, (long long)sgn * R, q);
 				solve_row<BS, MIXED>(F, row, F.cap_step, true, zc, st, la, lb, ls, s_n, cnt);
-				for (int i = threadIdx.x; i < R; i += BS) F.roots[(size_t)row * R + i] = zc[i];
+				if (lds_row)
+					for (int i = threadIdx.x; i < R; i += BS) F.roots[(size_t)row * R + i] = zc[i];
 			}
 		}
 		// per-field counters, one atomic per counter and task
@@ -583,7 +593,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_transpose(const BField* __restr
 		}
 }
 
-using BatchKernel = void (*)(const BField*, const BTask*, int, int*, int, int, int, ListSlice);
+using BatchKernel = void (*)(const BField*, const BTask*, int, int*, int, int, int, int, ListSlice);
 
 template <bool MIXED>
 BatchKernel pick_kernel(int bs)
@@ -641,7 +651,7 @@ struct BatchDevice
 	int* next = nullptr;
 	unsigned long long* stats = nullptr;
 	int n0 = 0, n1 = 0, nf = 0;
-	int rmax = 0, nlds = 0, maxrows = 0;
+	int rmax = 0, rlds = 0, nlds = 0, maxrows = 0; // roots of the largest row, rows held in LDS, stars in LDS
 	long long maxpts = 0;
 };
 
@@ -655,8 +665,8 @@ void free_bdev(ccf_ctx* m, BatchDevice& d)
 	d = BatchDevice{};
 }
 
-// LDS bytes of a workgroup: roots, staged stars, two index lists
-size_t batch_lds(int rmax, int nlds) { return (size_t)(rmax + nlds) * sizeof(double2); }
+// LDS bytes of a workgroup: the row (up to rlds roots) and staged stars
+size_t batch_lds(int rlds, int nlds) { return (size_t)(rlds + nlds) * sizeof(double2); }
 
 // persistent grid: as many workgroups as fit the device at once, at most
 // one per task (queried outside any graph capture)
@@ -667,7 +677,7 @@ int batch_grid(ccf_batch* b, int phase, int bs)
 	const bool mixed = m->P.precision == 1;
 	const BatchKernel fn = mixed ? pick_kernel<true>(bs) : pick_kernel<false>(bs);
 	int per_cu = 0;
-	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)fn, bs, batch_lds(d.rmax, d.nlds)) != hipSuccess || per_cu < 1)
+	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)fn, bs, batch_lds(d.rlds, d.nlds)) != hipSuccess || per_cu < 1)
 		per_cu = 1;
 	const int cus = m->num_cus > 0 ? m->num_cus : 256;
 	// the list buffers hold d.list_wgs workgroups
@@ -680,7 +690,7 @@ bool launch_batch_phase(ccf_batch* b, int phase, int bs, int g, hipStream_t s)
 	BatchDevice& d = *b->dev;
 	const bool mixed = m->P.precision == 1;
 	const BatchKernel fn = mixed ? pick_kernel<true>(bs) : pick_kernel<false>(bs);
-	const size_t lds = batch_lds(d.rmax, d.nlds);
+	const size_t lds = batch_lds(d.rlds, d.nlds);
 	const int ntasks = phase == 0 ? d.n0 : d.n1;
 	if (g > d.list_wgs)
 	{
@@ -689,7 +699,7 @@ bool launch_batch_phase(ccf_batch* b, int phase, int bs, int g, hipStream_t s)
 	}
 	k_zero_int<<<1, 64, 0, s>>>(d.next, 1);
 	hipLaunchKernelGGL(fn, dim3(g), dim3(bs), lds, s, (const BField*)d.fields, (const BTask*)(phase == 0 ? d.tasks0 : d.tasks1),
-		ntasks, d.next, phase, d.rmax, d.nlds, d.lists);
+		ntasks, d.next, phase, d.rmax, d.rlds, d.nlds, d.lists);
 	return CCF_LAUNCH(m, "batched Aberth solve");
 }
 
@@ -918,20 +928,18 @@ bool batch_build(ccf_batch* b, int verbose)
 			t1.push_back(BTask{f, br, 1, cost});
 		}
 	}
-	// LDS of a workgroup: the row, then as many stars as fit
+	// LDS of a workgroup: the largest row that fits (larger rows are solved
+	// in place in global memory), then as many stars as fit
 	const long long lim = batch_lds_limit(m);
-	if ((long long)batch_lds(d.rmax, 0) > lim)
-	{
-		m->fail("Error. A batched field has more roots than fit one workgroup (" + std::to_string(d.rmax) + ")\n");
-		return false;
-	}
+	for (const BField& F : hf)
+		if ((long long)batch_lds(F.R, 0) <= lim) d.rlds = std::max(d.rlds, F.R);
 	// largest rows first (longest-processing-time order of the queue)
 	auto by_cost = [](const BTask& a, const BTask& b2) { return a.cost > b2.cost; };
 	std::stable_sort(t0.begin(), t0.end(), by_cost);
 	std::stable_sort(t1.begin(), t1.end(), by_cost);
 	d.n0 = (int)t0.size();
 	d.n1 = (int)t1.size();
-	const long long room = lim - (long long)batch_lds(d.rmax, 0);
+	const long long room = lim - (long long)batch_lds(d.rlds, 0);
 	int nmax = 0;
 	for (ccf_ctx* c : b->fields) nmax = std::max(nmax, c->num_stars);
 	d.nlds = (int)std::max(0LL, std::min<long long>(nmax, room / (long long)sizeof(double2)));
@@ -939,10 +947,11 @@ bool batch_build(ccf_batch* b, int verbose)
 	// terms from the FP32 star copy: no star is read from LDS, which then
 	// holds only the row (more workgroups per CU)
 	if (m->P.precision == 1) d.nlds = 0;
-	// list buffers for the largest persistent grid: rmax slots per workgroup
+	// list buffers for the largest persistent grid (no more workgroups than
+	// tasks): rmax slots per workgroup
 	{
 		const int cus = m->num_cus > 0 ? m->num_cus : 256;
-		d.list_wgs = cus * MAX_BATCH_PER_CU;
+		d.list_wgs = std::min(cus * MAX_BATCH_PER_CU, std::max(d.n0, d.n1));
 		const size_t slots = (size_t)d.list_wgs * d.rmax;
 		if (!CCF_HIP(m, m->dmalloc(&d.list_mem, slots * (3 * sizeof(int) + 3 * sizeof(double2))))) return false;
 		d.lists.f0 = (double2*)d.list_mem;
@@ -1038,7 +1047,7 @@ int batch_lds_limit(ccf_ctx* m)
 {
 	int v = 0;
 	if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, 0) != hipSuccess || v <= 0) v = 65536;
-	return std::min(v, 65536) - BATCH_STATIC_LDS;
+	return v - BATCH_STATIC_LDS;
 }
 
 // solve, errors, transpose and caustics of every field: captured once into a

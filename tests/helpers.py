@@ -133,6 +133,31 @@ def differing_pairs(x: np.ndarray, y: np.ndarray, num_roots: int, num_branches: 
     return out
 
 
+def set_mismatch(x: np.ndarray, y: np.ndarray, num_roots: int, num_branches: int, branches: int):
+    """For the first `branches` branches, every root of each phi column of x
+    that differs from y's root of the same index by more than 1e-9 relative,
+    matched to the nearest root of y's column (k-d tree: for columns whose
+    roots are permuted at scale, where match_column's assignment would not
+    finish). Returns (max relative distance of those matches, count over
+    1e-9)."""
+    from scipy.spatial import cKDTree
+
+    cx = io.columns_by_phi(x, num_roots, num_branches)
+    cy = io.columns_by_phi(y, num_roots, num_branches)
+    worst, nbad = 0.0, 0
+    for c in range(branches):
+        for k in range(cx.shape[1]):
+            a, b = cx[c, k], cy[c, k]
+            idx = np.nonzero(np.abs(a - b) > 1e-9 * np.maximum(np.abs(b), 1.0))[0]
+            if idx.size == 0:
+                continue
+            d, j = cKDTree(np.c_[b.real, b.imag]).query(np.c_[a[idx].real, a[idx].imag])
+            rel = d / np.maximum(np.abs(b[j]), 1.0)
+            worst = max(worst, float(rel.max()))
+            nbad += int((rel > 1e-9).sum())
+    return worst, nbad
+
+
 def branch_slice(arr: np.ndarray, num_roots: int, num_branches: int, branches: int) -> np.ndarray:
     """the rows of the first `branches` branches of a file-layout array, in
     the layout of a run with that many branches"""

@@ -9,6 +9,9 @@
 - 256 uneven fields (num_phi 504, num_branches 4) in the batched form: the
   oracle on 8 fields spread over the batch, every field against the
   single-field path, and properties of all of them.
+- the 10,000-star field inside a batch (its 20,000-root rows are solved in
+  place in global memory, beyond a workgroup's LDS), mixed form, against the
+  single-field path.
 
 Parity: the FP64 path within 1e-9 relative of the oracle at every point.
 The mixed form may stop elsewhere inside the reference's convergence band
@@ -27,7 +30,7 @@ import pytest
 import ccf_amd
 import oracle
 from ccf_amd import io
-from helpers import branch_slice, compare_arrays, differing_pairs, newton_step, phi_of
+from helpers import branch_slice, compare_arrays, differing_pairs, newton_step, phi_of, set_mismatch
 
 pytestmark = pytest.mark.gpu
 
@@ -235,3 +238,34 @@ def test_batch256_matches_single_field_path(workdir, batch256):
         bi = res["fp64"][0][k]
         assert bi["max_error"] == pytest.approx(si["max_error"], rel=1e-3, abs=1e-12), (k, bi["max_error"], si["max_error"])
         assert bi["max_witt_residual"] == pytest.approx(si["max_witt_residual"], rel=1e-3, abs=1e-12)
+
+
+def test_batch_holding_the_10k_field(workdir):
+    """a batch of the 10,000-star field and a 300-star field (one expansion
+    order, as a batch requires), mixed form:
+    every point converged, and each field's critical points, as sets per phi
+    column (the batched initial solve may hand the roots to other indices),
+    within 1e-9 relative of the single-field path's (the 10k field on four
+    of its 20 branches, 404 000 points)"""
+    from helpers import make_field
+
+    sf, run = _field(workdir, "cc10k")
+    small = make_field(workdir, "salpeter300")
+    skw = dict(kappa_tot=0.45, shear=0.35, num_phi=200, num_branches=4, fmm_order=run["fmm_order"])
+    out = dict(outfile_prefix="./", write_stars=0, write_critical_curves=0, write_caustics=0, precision=1, predictor=4)
+    print("# batch: 10k field + 300-star field", flush=True)
+    with ccf_amd.Batch() as b:
+        b.add(starfile=sf, **out, **run)
+        b.add(starfile=small, **out, **skw)
+        assert b.run(0), b.last_error()
+        xs = [b.array(f, ccf_amd.ARRAY_CCS) for f in range(2)]
+        infos = [b.info(f) for f in range(2)]
+    for (f, kw, nbr), x, inf in zip([(sf, run, 4), (small, skw, 4)], xs, infos):
+        assert inf["points_converged"] == inf["points_total"], inf
+        with ccf_amd.Context(starfile=f, **out, root_fmm=1, **kw) as c:
+            assert c.run(0), c.last_error()
+            y = c.array(ccf_amd.ARRAY_CCS)
+            R = c.info()["num_roots"]
+        worst, nbad = set_mismatch(x, y, R, kw["num_branches"], nbr)
+        print(f"# batch vs single-field path, {R} roots: worst {worst:.3e}, {nbad} unmatched", flush=True)
+        assert nbad == 0, (R, worst, nbad)
