@@ -58,6 +58,7 @@ def run_gpu(starfile: str, run_kw: dict, precision: int, steps: int, warmup: int
     ctx = ccf_amd.Context(starfile=starfile, outfile_prefix="./", write_stars=0, write_critical_curves=0,
                           write_caustics=1, precision=precision, timing=0, predictor=predictor, root_fmm=root_fmm,
                           **(fmm_kw or {}), **run_kw)
+    tune_s = 0.0
     for i in range(warmup + steps):
         if i == warmup + steps - 1:
             # the last timed step records HIP events around every sweep's
@@ -71,11 +72,15 @@ def run_gpu(starfile: str, run_kw: dict, precision: int, steps: int, warmup: int
         dt = time.perf_counter() - t0
         print(f"# precision={precision} predictor={predictor} root_fmm={root_fmm} step {i} ({'warmup' if i < warmup else 'timed'}): "
               f"{dt:.3f} s", file=sys.stderr, flush=True)
+        inf = ctx.info()
+        tune_s += inf["t_autotune"]
         if i >= warmup:
             times.append(dt)
-            infos.append(ctx.info())
+            infos.append(inf)
     ccs = ctx.array(ccf_amd.ARRAY_CCS)
     ctx.close()
+    # (params.autotune: the first warmup step tunes; the timed steps reuse it)
+    infos[-1]["t_autotune_first_step"] = tune_s
     return times, infos, ccs
 
 
@@ -267,7 +272,9 @@ def summarize(times, infos):
                 tail_flops=inf["flops_tail"],
                 tail_tflops=inf["flops_tail"] / (inf["tail_kernel_ms"] * 1e-3) / 1e12 if inf["tail_kernel_ms"] > 0 else 0.0,
                 expansion_order=inf["expansion_order"], tree_levels=inf["tree_levels"], num_roots=inf["num_roots"],
-                max_error_1_over_mu=inf["max_error"], max_witt_residual=inf["max_witt_residual"])
+                max_error_1_over_mu=inf["max_error"], max_witt_residual=inf["max_witt_residual"],
+                roots_per_thread=inf["roots_per_thread"], root_fmm_leaf=inf["root_fmm_leaf"],
+                autotune_s=inf.get("t_autotune_first_step", 0.0))
 
 
 # 256 independent fields: N uniform in [50, 1000], equal masses, kappa_star =
@@ -433,6 +440,9 @@ def main():
     ap.add_argument("--root-fmm", type=int, default=1, help="root sums of dense sweeps by the FMM in the mixed form")
     ap.add_argument("--root-fmm-order", type=int, default=0, help="its expansion order (0: the library default)")
     ap.add_argument("--root-fmm-leaf", type=int, default=0, help="its mean roots per leaf cell (0: the library default)")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="1: the first warmup step times the automatic choices (sums tile, root-sum FMM order and "
+                         "cells) and the timed steps keep the fastest")
     ap.add_argument("--ablation", action="store_true", help="also time the mixed form without the predictor")
     ap.add_argument("--batched-fields", type=int, default=8,
                     help="the batched mixed form over this many fields of the workload's size (0: skip)")
@@ -482,7 +492,7 @@ def main():
         if args.ablation:
             ablation.append(("mixed_no_predictor", 1, 0, args.root_fmm))
     for name, prec, pred, rfmm in modes + ablation:
-        fmm_kw = dict(root_fmm_order=args.root_fmm_order, root_fmm_leaf=args.root_fmm_leaf)
+        fmm_kw = dict(root_fmm_order=args.root_fmm_order, root_fmm_leaf=args.root_fmm_leaf, autotune=args.autotune)
         steps, warm = (args.steps, args.warmup) if (name, prec, pred, rfmm) in modes else (args.ablation_steps, 1)
         times, infos, ccs[name] = run_gpu(sf, run_kw, prec, steps, warm, pred, rfmm, fmm_kw)
         res[name] = summarize(times, infos)

@@ -63,6 +63,7 @@ class Params(ctypes.Structure):
         ("root_fmm", ctypes.c_int),
         ("root_fmm_order", ctypes.c_int),
         ("root_fmm_leaf", ctypes.c_int),
+        ("autotune", ctypes.c_int),
     ]
 
 
@@ -112,6 +113,9 @@ class Info(ctypes.Structure):
         ("tail_kernel_ms", ctypes.c_double),
         ("flops_tail", ctypes.c_double),
         ("points_outside_tree", ctypes.c_longlong),
+        ("roots_per_thread", ctypes.c_int),
+        ("root_fmm_leaf", ctypes.c_int),
+        ("t_autotune", ctypes.c_double),
     ]
 
     def as_dict(self):

@@ -239,6 +239,7 @@ bool check_input_params(ccf_ctx* c)
 	if (P.root_fmm_order < 0 || P.root_fmm_order > ccf::MAX_ROOT_FMM_ORDER || (P.root_fmm_order > 0 && P.root_fmm_order < 3))
 		return err("Error. root_fmm_order must be 0 (automatic) or in [3, " + std::to_string(ccf::MAX_ROOT_FMM_ORDER) + "]\n");
 	if (P.root_fmm_leaf < 0) return err("Error. root_fmm_leaf must be >= 0\n");
+	if (P.autotune != 0 && P.autotune != 1) return err("Error. autotune must be 0 or 1\n");
 	return true;
 }
 
@@ -731,11 +732,17 @@ void ccf_default_params(ccf_params* p)
 int ccf_set_params(ccf_ctx* c, const ccf_params* p)
 {
 	if (!c || !p) return 1;
+	const ccf_params before = c->P_in;
 	c->P_in = *p;
 	c->P_in.mass_function[sizeof(c->P_in.mass_function) - 1] = 0;
 	c->P_in.starfile[CCF_PATH_MAX - 1] = 0;
 	c->P_in.outfile_prefix[CCF_PATH_MAX - 1] = 0;
 	c->P = c->P_in;
+	// tuned choices hold for the field and parameters they were timed on
+	// (turning the timing events on or off changes neither)
+	ccf_params a = before, b = c->P_in;
+	a.timing = b.timing = 0;
+	if (std::memcmp(&a, &b, sizeof(a)) != 0) c->tuned = false;
 	return 0;
 }
 
@@ -765,6 +772,16 @@ static bool derive(ccf_ctx* c, int verbose)
 	c->stars_in.clear();
 	c->P = c->P_in;
 	if (!check_input_params(c)) return false;
+	// the choices an earlier solve of this context tuned (params.autotune)
+	if (c->P.autotune && c->tuned)
+	{
+		if (c->P.roots_per_thread == 0) c->P.roots_per_thread = c->tuned_rpt;
+		if (c->P.root_fmm_order == 0 && c->P.root_fmm_leaf == 0 && c->tuned_root_order > 0)
+		{
+			c->P.root_fmm_order = c->tuned_root_order;
+			c->P.root_fmm_leaf = c->tuned_root_leaf;
+		}
+	}
 	if (!calculate_derived_params(c, verbose)) return false;
 	c->info.num_stars = c->num_stars;
 	c->info.num_roots = c->num_roots;

@@ -245,6 +245,9 @@ struct ccf_ctx
 	ccf::SweepBufs sw2;                 // mixed continuation: the second group's sweep buffers
 	int fmm_rows_sweep[2] = {-1, -1}; // root-sum FMM rows of the last phase's first and second sweep (-1: unknown)
 	int phase_sweeps = 0;         // sweeps the last phase ran (its cap when no read-back saw it end)
+	// params.autotune: the choices the context's first solve timed fastest
+	bool tuned = false;
+	int tuned_rpt = 0, tuned_root_order = 0, tuned_root_leaf = 0;
 	int sweep_verbose = 0;        // verbose level of the running solve stage (3: a read-back and a line after every sweep)
 	bool caustics_done = false;   // caustics / length scales formed by the error pass of find_ccs
 	bool mu_done = false;
@@ -295,6 +298,7 @@ struct ccf_batch
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t graph_exec = nullptr;
 	int block = 0, block1 = 0;      // autotuned workgroup sizes of the branch-centre and phi-chain phases
+	int coop = 0, coop1 = 0;        // and their autotuned cooperative-sum limits (listed roots per sweep)
 	int grid0 = 0, grid1 = 0;       // persistent grids of the two phases
 	std::vector<float> tuned_ms;
 	void* results = nullptr;        // the fields' ccs, caustics and length scales, one allocation kept across builds

@@ -45,7 +45,7 @@ const std::string OPTS[] = {
 	"-o", "--outfile_prefix",
 	"--fmm_order", "--direct", "--binomial_exact", "--precision",
 	"--init_iters", "--step_iters", "--roots_per_thread", "--predictor",
-	"--root_fmm", "--root_fmm_order", "--root_fmm_leaf",
+	"--root_fmm", "--root_fmm_order", "--root_fmm_leaf", "--autotune",
 };
 
 int verbose = 1;
@@ -143,7 +143,11 @@ void usage(const char* name)
 		<< "  --root_fmm                 Root sums of dense sweeps by a fast multipole\n"
 		<< "                             method (1) or directly (0). Default value: 0\n"
 		<< "  --root_fmm_order           Expansion order of the root-sum FMM (0 automatic).\n"
-		<< "  --root_fmm_leaf            Mean roots per leaf cell of its grid (0 automatic).\n";
+		<< "  --root_fmm_leaf            Mean roots per leaf cell of its grid (0 automatic).\n"
+		<< "  --autotune                 1: time the automatic choices of roots_per_thread,\n"
+		<< "                             root_fmm_order and root_fmm_leaf on the first\n"
+		<< "                             branch centres and keep the fastest.\n"
+		<< "                             Default value: 0\n";
 }
 
 template <typename V, typename F>
@@ -278,6 +282,7 @@ int main(int argc, char* argv[])
 		else if (o == "--root_fmm") ok = parse("root_fmm", in, ccf.root_fmm, to_i);
 		else if (o == "--root_fmm_order") ok = parse("root_fmm_order", in, ccf.root_fmm_order, to_i);
 		else if (o == "--root_fmm_leaf") ok = parse("root_fmm_leaf", in, ccf.root_fmm_leaf, to_i);
+		else if (o == "--autotune") ok = parse("autotune", in, ccf.autotune, to_i);
 		if (!ok) return -1;
 	}
 	if (!has_sf && !has_ks && (exists(b, e, "-s") || exists(b, e, "--smooth_fraction")))

@@ -39,6 +39,8 @@
 // on average) steers the iteration, it does not move the roots it reaches.
 
 constexpr int ROOT_FMM_MIN_ROOTS = 64;     // smaller rows are always summed directly
+constexpr int ROOT_FMM_ORDER = 8;          // expansion order unless given
+constexpr int ROOT_FMM_LEAF = 6;           // mean roots per leaf cell the grid depth aims at unless given
 // sweeps of a phase that may use the FMM: the dense ones (the initial solve
 // converges over a few dozen sweeps; a continuation step mostly in one or two)
 constexpr int ROOT_FMM_INIT_SWEEPS = 24;

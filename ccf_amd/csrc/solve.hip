@@ -1894,8 +1894,8 @@ bool root_layout(ccf_ctx* ctx, SweepBufs& b, Take&& take)
 	{
 		hx = hy = 1.1 * ctx->corner.abs();
 	}
-	const int order = root_order_compiled(ctx->P.root_fmm_order > 0 ? ctx->P.root_fmm_order : 8);
-	const double leaf = ctx->P.root_fmm_leaf > 0 ? ctx->P.root_fmm_leaf : 6.0;
+	const int order = root_order_compiled(ctx->P.root_fmm_order > 0 ? ctx->P.root_fmm_order : ROOT_FMM_ORDER);
+	const double leaf = ctx->P.root_fmm_leaf > 0 ? ctx->P.root_fmm_leaf : ROOT_FMM_LEAF;
 	b.grid = plan_root_grid(hx, hy, R, order, leaf);
 	int lds = 0;
 	if (!CCF_HIP(ctx, hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, 0))) return false;
@@ -2000,10 +2000,120 @@ bool group_streams_join(ccf_ctx* ctx)
 		CCF_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->group_event, 0));
 }
 
+// params.autotune: the context's first solve times, on the initial solve of
+// the first group's branch centres (the rows, sweeps and FMM passes of the
+// real one, from the seeds), the automatic choices of the sums' roots per
+// thread and then, in the mixed form with the root-sum FMM, of its (order,
+// mean roots per leaf); the fastest stay for the context's later runs.
+// None of them moves a fixed point: the tile only splits the sums, the
+// FMM's truncation only steers the iteration. The trial solves leave the
+// run's counters and statistics as they found them.
+bool autotune_dev(ccf_ctx* ctx, int verbose)
+{
+	const int R = ctx->num_roots, np = ctx->P.num_phi, nb = ctx->P.num_branches;
+	hipStream_t s = ctx->stream;
+	const ccf_params& Pin = ctx->P_in;
+	const bool tune_rpt = Pin.roots_per_thread == 0;
+	const bool tune_fmm = ctx->P.precision == 1 && ctx->P.root_fmm == 1 && Pin.root_fmm_order == 0 && Pin.root_fmm_leaf == 0 &&
+		R >= ROOT_FMM_MIN_ROOTS;
+	auto t0 = std::chrono::steady_clock::now();
+	const int G = (ctx->sw2.act && nb >= CONT_GROUPS) ? CONT_GROUPS : 1;
+	const int nbg = nb / G;
+	const RowMap M{R, np, nb, 0, 1, 0.0, ctx->phase, 0};
+	const int cap = ctx->P.init_iters > 0 ? ctx->P.init_iters : 60;
+	const ccf_info keep = ctx->info;
+	const int timing = ctx->P.timing;
+	ctx->P.timing = 0;
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	bool ok = CCF_HIP(ctx, hipEventCreate(&e0)) && CCF_HIP(ctx, hipEventCreate(&e1));
+	auto trial = [&](int rpt, int order, int leaf, float& ms) -> bool {
+		ctx->P.roots_per_thread = rpt;
+		ctx->P.root_fmm_order = order;
+		ctx->P.root_fmm_leaf = leaf;
+		SweepBufs B;
+		bool good = alloc_sweep(ctx, B, nbg, R);
+		for (int c = 0; good && c < nbg; c++)
+		{
+			const int center = np / (2 * nb) + c * np / nb + c;
+			good = CCF_HIP(ctx, hipMemcpyAsync(ctx->ccs_init + (size_t)center * R, ctx->seeds.data(), (size_t)R * sizeof(double2),
+				hipMemcpyHostToDevice, s));
+		}
+		if (good)
+		{
+			k_prepare<<<dim3(grid(R), nbg), BLOCK, 0, s>>>(ctx->ccs_init, M, B, 0, ctx->dorder);
+			SweepGroup q{&B, M, nbg, s, (1u << ROOT_FMM_INIT_SWEEPS) - 1u, 0};
+			good = CCF_LAUNCH(ctx, "prepare_roots_kernel") && CCF_HIP(ctx, hipEventRecord(e0, s)) &&
+				run_sweeps_groups(ctx, &q, 1, ctx->ccs_init, cap) && CCF_HIP(ctx, hipEventRecord(e1, s)) &&
+				CCF_HIP(ctx, hipEventSynchronize(e1)) && CCF_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+		}
+		free_sweep(ctx, B);
+		if (good && verbose >= 2)
+			std::cout << "autotune: roots_per_thread " << rpt << ", root-sum FMM order " << order << " leaf " << leaf << ": " << ms
+					  << " ms\n";
+		return good;
+	};
+	int best_rpt = sums_launch(ctx, ctx->P.precision == 1).rpt;
+	int best_order = ctx->P.root_fmm_order > 0 ? ctx->P.root_fmm_order : ROOT_FMM_ORDER;
+	int best_leaf = ctx->P.root_fmm_leaf > 0 ? ctx->P.root_fmm_leaf : ROOT_FMM_LEAF;
+	float best = 1e30f;
+	if (ok && tune_rpt)
+		for (int rpt : {1, 2, 4})
+		{
+			float ms = 0;
+			if (!(ok = trial(rpt, best_order, best_leaf, ms))) break;
+			if (ms < best)
+			{
+				best = ms;
+				best_rpt = rpt;
+			}
+		}
+	if (ok && tune_fmm)
+	{
+		// the defaults first; lower orders and larger cells trade far-field
+		// terms for near-field ones
+		const int cand[][2] = {{ROOT_FMM_ORDER, ROOT_FMM_LEAF}, {6, 6}, {10, 6}, {8, 10}, {6, 10}, {8, 4}};
+		best = 1e30f;
+		for (const auto& c : cand)
+		{
+			float ms = 0;
+			if (!(ok = trial(best_rpt, c[0], c[1], ms))) break;
+			if (ms < best)
+			{
+				best = ms;
+				best_order = c[0];
+				best_leaf = c[1];
+			}
+		}
+	}
+	if (e0) (void)hipEventDestroy(e0);
+	if (e1) (void)hipEventDestroy(e1);
+	ctx->P.timing = timing;
+	if (!ok) return false;
+	// the tuned choices, and the solver's buffers laid out for them
+	ctx->P.roots_per_thread = best_rpt;
+	ctx->P.root_fmm_order = best_order;
+	ctx->P.root_fmm_leaf = best_leaf;
+	ctx->tuned = true;
+	ctx->tuned_rpt = best_rpt;
+	ctx->tuned_root_order = tune_fmm ? best_order : 0;
+	ctx->tuned_root_leaf = tune_fmm ? best_leaf : 0;
+	ctx->info = keep;
+	const bool groups = ctx->P.precision == 1 && nb >= CONT_GROUPS;
+	ok = alloc_sweep(ctx, ctx->sw, ctx->max_rows, R) && (!groups || alloc_sweep(ctx, ctx->sw2, 2 * (nb - nb / CONT_GROUPS), R)) &&
+		CCF_HIP(ctx, hipMemsetAsync(ctx->counters, 0, STAT_WORDS * sizeof(unsigned long long), s)) &&
+		CCF_HIP(ctx, hipStreamSynchronize(s));
+	ctx->info.t_autotune = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+	if (verbose >= 1)
+		std::cout << "Autotuned: roots_per_thread " << best_rpt << (tune_fmm ? ", root-sum FMM order " + std::to_string(best_order) +
+			" leaf " + std::to_string(best_leaf) : std::string()) << " (" << ctx->info.t_autotune << " s)\n";
+	return ok;
+}
+
 bool find_initial_roots_dev(ccf_ctx* ctx, int verbose)
 {
 	const int R = ctx->num_roots, np = ctx->P.num_phi, nb = ctx->P.num_branches;
 	hipStream_t s = ctx->stream;
+	if (ctx->P.autotune && !ctx->tuned && !autotune_dev(ctx, verbose)) return false;
 	if (verbose >= 1) std::cout << "Finding initial roots...\n";
 	auto t0 = std::chrono::steady_clock::now();
 	// seeds at every branch centre (ccf.cuh:711-726)
@@ -2263,6 +2373,8 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 		ctx->info.root_fmm_rows = (long long)c[C_RROWS];
 		ctx->info.root_fmm_order = g.order;
 		ctx->info.root_fmm_leaf_level = g.leaf;
+		ctx->info.root_fmm_leaf = g.order > 0 ? (ctx->P.root_fmm_leaf > 0 ? ctx->P.root_fmm_leaf : ROOT_FMM_LEAF) : 0;
+		ctx->info.roots_per_thread = sums_launch(ctx, ctx->P.precision == 1).rpt;
 		// the tails: every evaluation forms f0 and f1 (mixed form), every
 		// full update sums the row (and the stars without the FMM pole sum)
 		const double tail_evals = (double)(c[C_TAIL_FULL] + c[C_TAIL_EARLY] + c[C_TAIL_NEWTON]);

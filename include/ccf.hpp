@@ -57,6 +57,7 @@ public:
 	int root_fmm = 0;
 	int root_fmm_order = 0;
 	int root_fmm_leaf = 0;
+	int autotune = 0;
 
 	CCF() : ctx_(ccf_create()) {}
 	~CCF() { ccf_destroy(ctx_); }
@@ -125,6 +126,7 @@ public:
 		p.root_fmm = root_fmm;
 		p.root_fmm_order = root_fmm_order;
 		p.root_fmm_leaf = root_fmm_leaf;
+		p.autotune = autotune;
 		return p;
 	}
 

@@ -68,6 +68,11 @@ typedef struct ccf_params
 	                        sums as ccf_functions.cuh:232-239 */
 	int root_fmm_order;  /* expansion order of the root-sum FMM (0 -> 8; at most 24) */
 	int root_fmm_leaf;   /* mean roots per leaf cell the grid depth aims at (0 -> 6) */
+	int autotune;        /* 1: the context's first solve times the automatic (0) choices of
+	                        roots_per_thread (1, 2, 4) and, in the mixed form with root_fmm, of
+	                        (root_fmm_order, root_fmm_leaf) over a few pairs, on the initial solve
+	                        of the first group of branch centres, and keeps the fastest for its
+	                        later runs (until ccf_set_params); none of them moves a root */
 } ccf_params;
 
 /* Derived values and measurements of the last run. */
@@ -118,6 +123,9 @@ typedef struct ccf_info
 	double flops_tail;         /* algorithmic flops of the tails (Witt evaluations and Cauchy sums) */
 	long long points_outside_tree; /* error pass: points outside the star tree's root box, whose near
 	                              field is every star (evaluated with a whole workgroup each) */
+	int roots_per_thread;      /* the sums' roots per thread the run used (given, default or tuned) */
+	int root_fmm_leaf;         /* the mean roots per leaf cell its root-sum FMM grid aimed at */
+	double t_autotune;         /* seconds the run spent tuning (params.autotune; 0 once tuned) */
 } ccf_info;
 
 /* Arrays readable with ccf_get_array. */

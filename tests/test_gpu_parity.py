@@ -9,6 +9,7 @@ with the oracle (oracle/libccf_oracle.so) run on the same inputs:
     |dz| <= 1e-9 max(|z|, theta_E) (the parity bar), plus the parameter
     file, the star file and the 1/mu error.
 """
+import ctypes
 import os
 
 import numpy as np
@@ -227,6 +228,10 @@ MIXED = [
      dict(precision=1, predictor=4, root_fmm=1)),
     ("rect200", dict(kappa_tot=0.7, shear=0.25, num_phi=80, num_branches=2), dict(precision=1, predictor=3)),
     ("smooth150", dict(kappa_tot=0.6, shear=0.2, num_phi=60, num_branches=1, approx=0), dict(precision=1, root_fmm=1)),
+    # the autotuned choices (tile of the sums; root-sum FMM order and cells)
+    ("salpeter300", dict(kappa_tot=0.45, shear=0.35, num_phi=120, num_branches=3, fmm_order=20),
+     dict(precision=1, predictor=4, root_fmm=1, autotune=1)),
+    ("rect200", dict(kappa_tot=0.7, shear=0.25, num_phi=80, num_branches=2), dict(autotune=1)),
 ]
 
 
@@ -366,6 +371,35 @@ def test_context_reuse_is_exact(workdir, gkw):
         assert a.shape == b.shape and np.array_equal(a, b)
         assert np.array_equal(ac, bc)
         assert ai["max_error"] == bi["max_error"] and ai["points_converged"] == bi["points_converged"]
+
+
+def test_autotune_choices_persist(workdir):
+    """params.autotune: the first run tunes (time spent, choices among the
+    candidates), the next run of the context reuses them without tuning and
+    reproduces the first run's roots bit for bit; ccf_set_params keeps them
+    when only the timing switch changes and forgets them otherwise"""
+    sf = make_field(workdir, "salpeter300")
+    kw = dict(kappa_tot=0.45, shear=0.35, num_phi=120, num_branches=3, fmm_order=20, precision=1, predictor=4,
+              root_fmm=1, autotune=1)
+    with _ctx(sf, str(workdir) + "/", **kw) as c:
+        assert c.run(0), c.last_error()
+        a, ia = c.array(ccf_amd.ARRAY_CCS), c.info()
+        assert c.run(0), c.last_error()
+        b, ib = c.array(ccf_amd.ARRAY_CCS), c.info()
+        c.set(timing=1)
+        assert c.run(0), c.last_error()
+        ic = c.info()
+        c.set(timing=0, init_iters=59)
+        assert c.run(0), c.last_error()
+        idd = c.info()
+    assert ia["t_autotune"] > 0 and ib["t_autotune"] == 0 and ic["t_autotune"] == 0 and idd["t_autotune"] > 0
+    assert ia["roots_per_thread"] in (1, 2, 4) and ia["root_fmm_order"] in (6, 8, 10)
+    assert ia["root_fmm_leaf"] in (4, 6, 10)
+    assert (ib["roots_per_thread"], ib["root_fmm_order"], ib["root_fmm_leaf"]) == \
+        (ia["roots_per_thread"], ia["root_fmm_order"], ia["root_fmm_leaf"])
+    assert ia["points_converged"] == ia["points_total"] == ib["points_converged"]
+    assert ia["sweeps"] == ib["sweeps"] and ia["updates_full"] == ib["updates_full"]
+    assert np.array_equal(a, b)
 
 
 def test_error_pass_takes_points_outside_the_tree(workdir):

@@ -84,6 +84,8 @@ def test_default_params_are_the_reference_defaults():
     (dict(write_caustics=-1), "Error. write_caustics must be 1 (true) or 0 (false)."),
     (dict(write_mu_length_scales=5), "Error. write_mu_length_scales must be 1 (true) or 0 (false)."),
     (dict(fmm_order=2), "Error. fmm_order must be 0 (automatic) or in [3, 31]"),
+    (dict(roots_per_thread=3), "Error. roots_per_thread must be 0 (automatic), 1, 2 or 4."),
+    (dict(autotune=2), "Error. autotune must be 0 or 1"),
 ])
 def test_input_validation_messages(kw, msg):
     with ccf_amd.Context(**kw) as c:
