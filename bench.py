@@ -440,6 +440,7 @@ def main():
     ap.add_argument("--root-fmm", type=int, default=1, help="root sums of dense sweeps by the FMM in the mixed form")
     ap.add_argument("--root-fmm-order", type=int, default=0, help="its expansion order (0: the library default)")
     ap.add_argument("--root-fmm-leaf", type=int, default=0, help="its mean roots per leaf cell (0: the library default)")
+    ap.add_argument("--roots-per-thread", type=int, default=0, help="the sums' roots per thread (0: automatic)")
     ap.add_argument("--autotune", type=int, default=1,
                     help="1: the first warmup step times the automatic choices (sums tile, root-sum FMM order and "
                          "cells) and the timed steps keep the fastest")
@@ -492,7 +493,8 @@ def main():
         if args.ablation:
             ablation.append(("mixed_no_predictor", 1, 0, args.root_fmm))
     for name, prec, pred, rfmm in modes + ablation:
-        fmm_kw = dict(root_fmm_order=args.root_fmm_order, root_fmm_leaf=args.root_fmm_leaf, autotune=args.autotune)
+        fmm_kw = dict(root_fmm_order=args.root_fmm_order, root_fmm_leaf=args.root_fmm_leaf, autotune=args.autotune,
+                      roots_per_thread=args.roots_per_thread)
         steps, warm = (args.steps, args.warmup) if (name, prec, pred, rfmm) in modes else (args.ablation_steps, 1)
         times, infos, ccs[name] = run_gpu(sf, run_kw, prec, steps, warm, pred, rfmm, fmm_kw)
         res[name] = summarize(times, infos)
