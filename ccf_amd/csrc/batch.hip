@@ -26,9 +26,8 @@
 //  * the 1/mu error pass (which also forms the caustic map from the same
 //    field evaluation) and the transpose run once over all fields;
 //  * the launches of a solve are captured in a HIP graph and replayed.
-// An autotuner picks the workgroup size of each phase and the number of
-// listed roots up to which a sweep sums cooperatively (a wave per root
-// rather than a thread) by timing the phase's launch on the batch itself.
+// An autotuner picks the workgroup size of each phase by timing the phase's
+// launch on the batch itself.
 #include "lens.hpp"
 
 #include <algorithm>
@@ -115,13 +114,14 @@ __device__ inline int centre_row(const BField& F, int c) { return F.np / (2 * F.
 // ends there when the step is below the convergence threshold, as in the
 // single-field path.
 constexpr int BCNT = 6; // full, early, converged, near-field stars, sweeps, Newton
-// a sweep that lists at most coop roots for the Cauchy sums (autotuned,
-// at most COOP_MAX) sums them cooperatively: wave w takes listed roots w,
-// w + waves, ... and its lanes split the row (and the stars), so a few
-// listed roots do not leave the workgroup waiting on their owners' serial
-// loops; more listed roots take a thread each
+// a sweep that lists at most COOP_MAX roots for the Cauchy sums sums them
+// cooperatively: wave w takes listed roots w, w + waves, ... and
+// its lanes split the row (and the stars), so a few listed roots do not
+// leave the workgroup waiting on their owners' serial loops; more listed
+// roots take a thread each. The two forms add a root's terms in different
+// orders, so the limit is fixed: were it tuned by timing, results would
+// depend on the timing.
 constexpr int COOP_MAX = 256;
-constexpr int COOP_CANDIDATES[] = {64, COOP_MAX};
 // static LDS of k_batch (the cooperative sums, counters, reductions), kept
 // out of the dynamic limit
 constexpr int BATCH_STATIC_LDS = 8192;
@@ -139,7 +139,7 @@ struct ListSlice
 
 template <int BS, bool MIXED>
 __device__ void solve_row(const BField& F, int row, int cap, bool newton, double2* zc, const double2* st, int* la, int* lb,
-	const ListSlice& ls, int* s_n, unsigned long long (&cnt)[BCNT], int coop)
+	const ListSlice& ls, int* s_n, unsigned long long (&cnt)[BCNT])
 {
 	const LensView& L = F.L;
 	const int R = F.R;
@@ -210,7 +210,7 @@ __device__ void solve_row(const BField& F, int row, int cap, bool newton, double
 		}
 		__syncthreads();
 		const int nf = s_nf;
-		if (nf <= coop)
+		if (nf <= COOP_MAX)
 		{
 			// cooperative sums, one wave per listed root, lanes over the sources
 			for (int g = wv; g < nf; g += NW)
@@ -369,7 +369,7 @@ __device__ void solve_row(const BField& F, int row, int cap, bool newton, double
 			const cd z(zc[i].x, zc[i].y);
 			const double2 a0 = ls.f0[g], a1 = ls.f1[g], ae = ls.ex[g];
 			const cd f0(a0.x, a0.y), f1(a1.x, a1.y);
-			const cd sum = nf <= coop ? cd(s_fs[g].x, s_fs[g].y) + cd(ae.x, ae.y) : cd(ae.x, ae.y);
+			const cd sum = nf <= COOP_MAX ? cd(s_fs[g].x, s_fs[g].y) + cd(ae.x, ae.y) : cd(ae.x, ae.y);
 			const cd zn = z - f0 / (f1 + f0 * sum);
 			zc[i] = make_double2(zn.re, zn.im);
 			if (!((zn - z).abs() / L.theta < CONV_TOL)) nxt[atomicAdd(&s_n[1], 1)] = i; // a non-finite step stays active
@@ -399,7 +399,7 @@ __device__ void solve_row(const BField& F, int row, int cap, bool newton, double
 constexpr int BATCH_WAVES_PER_SIMD = 4;
 template <int BS, bool MIXED>
 __global__ __launch_bounds__(BS, BATCH_WAVES_PER_SIMD) void k_batch(const BField* __restrict__ fields, const BTask* __restrict__ tasks,
-	int ntasks, int* __restrict__ next, int phase, int rmax, int rlds, int nlds, ListSlice lists, int coop)
+	int ntasks, int* __restrict__ next, int phase, int rmax, int rlds, int nlds, ListSlice lists)
 {
 	extern __shared__ double2 smem[];
 	double2* sl = smem + rlds;
@@ -434,7 +434,7 @@ __global__ __launch_bounds__(BS, BATCH_WAVES_PER_SIMD) void k_batch(const BField
 			const int row = centre_row(F, tk.branch);
 			double2* zc = lds_row ? smem : F.roots + (size_t)row * R;
 			for (int i = threadIdx.x; i < R; i += BS) zc[i] = F.seeds[i];
-			solve_row<BS, MIXED>(F, row, F.cap_init, false, zc, st, la, lb, ls, s_n, cnt, coop);
+			solve_row<BS, MIXED>(F, row, F.cap_init, false, zc, st, la, lb, ls, s_n, cnt);
 			if (lds_row)
 				for (int i = threadIdx.x; i < R; i += BS) F.roots[(size_t)row * R + i] = zc[i];
 		}
@@ -453,7 +453,7 @@ __global__ __launch_bounds__(BS, BATCH_WAVES_PER_SIMD) void k_batch(const BField
 				// (in place: row is written, the rows before it are read)
 				for (int i = threadIdx.x; i < R; i += BS)
 					zc[i] = predict_root(F.roots + (size_t)(row - sgn) * R + i, (long long)sgn * R, q);
-				solve_row<BS, MIXED>(F, row, F.cap_step, true, zc, st, la, lb, ls, s_n, cnt, coop);
+				solve_row<BS, MIXED>(F, row, F.cap_step, true, zc, st, la, lb, ls, s_n, cnt);
 				if (lds_row)
 					for (int i = threadIdx.x; i < R; i += BS) F.roots[(size_t)row * R + i] = zc[i];
 			}
@@ -596,7 +596,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_transpose(const BField* __restr
 		}
 }
 
-using BatchKernel = void (*)(const BField*, const BTask*, int, int*, int, int, int, int, ListSlice, int);
+using BatchKernel = void (*)(const BField*, const BTask*, int, int*, int, int, int, int, ListSlice);
 
 template <bool MIXED>
 BatchKernel pick_kernel(int bs)
@@ -687,7 +687,7 @@ int batch_grid(ccf_batch* b, int phase, int bs)
 	return std::max(1, std::min(std::min(phase == 0 ? d.n0 : d.n1, cus * per_cu), d.list_wgs));
 }
 
-bool launch_batch_phase(ccf_batch* b, int phase, int bs, int coop, int g, hipStream_t s)
+bool launch_batch_phase(ccf_batch* b, int phase, int bs, int g, hipStream_t s)
 {
 	ccf_ctx* m = b->master;
 	BatchDevice& d = *b->dev;
@@ -702,7 +702,7 @@ bool launch_batch_phase(ccf_batch* b, int phase, int bs, int coop, int g, hipStr
 	}
 	k_zero_int<<<1, 64, 0, s>>>(d.next, 1);
 	hipLaunchKernelGGL(fn, dim3(g), dim3(bs), lds, s, (const BField*)d.fields, (const BTask*)(phase == 0 ? d.tasks0 : d.tasks1),
-		ntasks, d.next, phase, d.rmax, d.rlds, d.nlds, d.lists, std::min(coop, COOP_MAX));
+		ntasks, d.next, phase, d.rmax, d.rlds, d.nlds, d.lists);
 	return CCF_LAUNCH(m, "batched Aberth solve");
 }
 
@@ -720,7 +720,7 @@ bool enqueue_solve(ccf_batch* b, hipStream_t s)
 {
 	BatchDevice& d = *b->dev;
 	k_zero_u64<<<grid((long long)d.nf * BSTAT), BLOCK, 0, s>>>(d.stats, d.nf * BSTAT);
-	return launch_batch_phase(b, 0, b->block, b->coop, b->grid0, s) && launch_batch_phase(b, 1, b->block1, b->coop1, b->grid1, s) &&
+	return launch_batch_phase(b, 0, b->block, b->grid0, s) && launch_batch_phase(b, 1, b->block1, b->grid1, s) &&
 		launch_post(b, s);
 }
 
@@ -999,45 +999,42 @@ bool batch_build(ccf_batch* b, int verbose)
 		CCF_HIP(m, hipStreamSynchronize(s));
 }
 
-// workgroup size and cooperative-sum limit of each phase by timing its
-// launch on this batch at every candidate pair (idempotent: phase 0
-// recomputes the centre rows from the seeds, phase 1 the chains from those
-// rows)
+// workgroup size of each phase by timing its launch on this batch at every
+// candidate (idempotent: phase 0 recomputes the centre rows from the seeds,
+// phase 1 the chains from those rows). A root's arithmetic does not depend
+// on the workgroup size.
 bool batch_autotune(ccf_batch* b, int verbose)
 {
 	ccf_ctx* m = b->master;
 	hipStream_t s = m->stream;
 	const int cands[3] = {256, 512, 1024};
 	double best[2] = {1e300, 1e300};
-	int pick[2] = {0, 0}, pick_coop[2] = {COOP_MAX, COOP_MAX};
+	int pick[2] = {0, 0};
 	hipEvent_t e0, e1, e2;
 	if (!CCF_HIP(m, hipEventCreate(&e0)) || !CCF_HIP(m, hipEventCreate(&e1)) || !CCF_HIP(m, hipEventCreate(&e2))) return false;
 	b->tuned_ms.clear();
 	for (int bs : cands)
-		for (int coop : COOP_CANDIDATES)
+	{
+		float ms[2] = {0, 0};
+		const int g0 = batch_grid(b, 0, bs), g1 = batch_grid(b, 1, bs);
+		if (!launch_batch_phase(b, 0, bs, g0, s) || !CCF_HIP(m, hipEventRecord(e0, s)) ||
+			!launch_batch_phase(b, 0, bs, g0, s) || !CCF_HIP(m, hipEventRecord(e1, s)) ||
+			!launch_batch_phase(b, 1, bs, g1, s) || !CCF_HIP(m, hipEventRecord(e2, s)) ||
+			!CCF_HIP(m, hipEventSynchronize(e2)) || !CCF_HIP(m, hipEventElapsedTime(&ms[0], e0, e1)) ||
+			!CCF_HIP(m, hipEventElapsedTime(&ms[1], e1, e2)))
+			return false;
+		if (verbose >= 2)
+			std::cout << "batch autotune: workgroup " << bs << ": centres " << ms[0] << " ms, chains " << ms[1] << " ms\n";
+		for (int q = 0; q < 2; q++)
 		{
-			float ms[2] = {0, 0};
-			const int g0 = batch_grid(b, 0, bs), g1 = batch_grid(b, 1, bs);
-			if (!launch_batch_phase(b, 0, bs, coop, g0, s) || !CCF_HIP(m, hipEventRecord(e0, s)) ||
-				!launch_batch_phase(b, 0, bs, coop, g0, s) || !CCF_HIP(m, hipEventRecord(e1, s)) ||
-				!launch_batch_phase(b, 1, bs, coop, g1, s) || !CCF_HIP(m, hipEventRecord(e2, s)) ||
-				!CCF_HIP(m, hipEventSynchronize(e2)) || !CCF_HIP(m, hipEventElapsedTime(&ms[0], e0, e1)) ||
-				!CCF_HIP(m, hipEventElapsedTime(&ms[1], e1, e2)))
-				return false;
-			if (verbose >= 2)
-				std::cout << "batch autotune: workgroup " << bs << ", cooperative sums up to " << coop << " listed roots: centres "
-						  << ms[0] << " ms, chains " << ms[1] << " ms\n";
-			for (int q = 0; q < 2; q++)
+			b->tuned_ms.push_back(ms[q]);
+			if (ms[q] < best[q])
 			{
-				b->tuned_ms.push_back(ms[q]);
-				if (ms[q] < best[q])
-				{
-					best[q] = ms[q];
-					pick[q] = bs;
-					pick_coop[q] = coop;
-				}
+				best[q] = ms[q];
+				pick[q] = bs;
 			}
 		}
+	}
 	(void)hipEventDestroy(e0);
 	(void)hipEventDestroy(e1);
 	(void)hipEventDestroy(e2);
@@ -1048,8 +1045,6 @@ bool batch_autotune(ccf_batch* b, int verbose)
 	}
 	b->block = pick[0];
 	b->block1 = pick[1];
-	b->coop = pick_coop[0];
-	b->coop1 = pick_coop[1];
 	return true;
 }
 

@@ -298,7 +298,6 @@ struct ccf_batch
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t graph_exec = nullptr;
 	int block = 0, block1 = 0;      // autotuned workgroup sizes of the branch-centre and phi-chain phases
-	int coop = 0, coop1 = 0;        // and their autotuned cooperative-sum limits (listed roots per sweep)
 	int grid0 = 0, grid1 = 0;       // persistent grids of the two phases
 	std::vector<float> tuned_ms;
 	void* results = nullptr;        // the fields' ccs, caustics and length scales, one allocation kept across builds
