@@ -1949,6 +1949,21 @@ bool alloc_sweep(ccf_ctx* ctx, SweepBufs& b, int rows, int R)
 		CCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
 }
 
+// the seeds in the rows of branch centres c0 .. c0 + gridDim.y - 1
+// (ccf.cuh:711-726: every centre starts from the same seeds)
+__global__ void k_seed_rows(double2* __restrict__ roots, const double2* __restrict__ seeds, int R, int np, int nb, int c0)
+{
+	const int a = blockIdx.x * blockDim.x + threadIdx.x, c = c0 + (int)blockIdx.y;
+	if (a < R) roots[(size_t)(np / (2 * nb) + c * np / nb + c) * R + a] = seeds[a];
+}
+
+bool seed_rows(ccf_ctx* ctx, int c0, int n, hipStream_t s)
+{
+	const int R = ctx->num_roots;
+	k_seed_rows<<<dim3(grid(R), n), BLOCK, 0, s>>>(ctx->ccs_init, ctx->dseeds, R, ctx->P.num_phi, ctx->P.num_branches, c0);
+	return CCF_LAUNCH(ctx, "seeds");
+}
+
 bool launch_phase(ccf_ctx* ctx)
 {
 	const int rows = ctx->P.num_phi + ctx->P.num_branches;
@@ -1974,7 +1989,9 @@ bool alloc_solver(ccf_ctx* ctx)
 	// the mixed form's continuation runs CONT_GROUPS groups of branches
 	// side by side: the first group's rows in sw, the others' in sw2
 	const bool groups = ctx->P.precision == 1 && nb >= CONT_GROUPS;
-	return CCF_HIP(ctx, ctx->dmalloc(&ctx->ccs_init, pts * sizeof(double2))) && alloc_sweep(ctx, ctx->sw, ctx->max_rows, R) &&
+	return CCF_HIP(ctx, ctx->dmalloc(&ctx->dseeds, (size_t)R * sizeof(double2))) &&
+		CCF_HIP(ctx, hipMemcpyAsync(ctx->dseeds, ctx->seeds.data(), (size_t)R * sizeof(double2), hipMemcpyHostToDevice, ctx->stream)) &&
+		CCF_HIP(ctx, ctx->dmalloc(&ctx->ccs_init, pts * sizeof(double2))) && alloc_sweep(ctx, ctx->sw, ctx->max_rows, R) &&
 		(!groups || alloc_sweep(ctx, ctx->sw2, 2 * (nb - nb / CONT_GROUPS), R)) &&
 		CCF_HIP(ctx, hipHostMalloc((void**)&ctx->host_counts, (ctx->max_rows + 6 * CONT_GROUPS) * sizeof(int), hipHostMallocDefault)) &&
 		CCF_HIP(ctx, ctx->dmalloc(&ctx->phase, (size_t)(np + nb) * sizeof(double2))) && launch_phase(ctx) &&
@@ -2031,13 +2048,7 @@ bool autotune_dev(ccf_ctx* ctx, int verbose)
 		ctx->P.root_fmm_order = order;
 		ctx->P.root_fmm_leaf = leaf;
 		SweepBufs B;
-		bool good = alloc_sweep(ctx, B, nbg, R);
-		for (int c = 0; good && c < nbg; c++)
-		{
-			const int center = np / (2 * nb) + c * np / nb + c;
-			good = CCF_HIP(ctx, hipMemcpyAsync(ctx->ccs_init + (size_t)center * R, ctx->seeds.data(), (size_t)R * sizeof(double2),
-				hipMemcpyHostToDevice, s));
-		}
+		bool good = alloc_sweep(ctx, B, nbg, R) && seed_rows(ctx, 0, nbg, s);
 		if (good)
 		{
 			k_prepare<<<dim3(grid(R), nbg), BLOCK, 0, s>>>(ctx->ccs_init, M, B, 0, ctx->dorder);
@@ -2117,12 +2128,7 @@ bool find_initial_roots_dev(ccf_ctx* ctx, int verbose)
 	if (verbose >= 1) std::cout << "Finding initial roots...\n";
 	auto t0 = std::chrono::steady_clock::now();
 	// seeds at every branch centre (ccf.cuh:711-726)
-	for (int c = 0; c < nb; c++)
-	{
-		const int center = np / (2 * nb) + c * np / nb + c;
-		if (!CCF_HIP(ctx, hipMemcpyAsync(ctx->ccs_init + (size_t)center * R, ctx->seeds.data(), (size_t)R * sizeof(double2), hipMemcpyHostToDevice, s)))
-			return false;
-	}
+	if (!seed_rows(ctx, 0, nb, s)) return false;
 	ctx->sweep_verbose = verbose;
 	const int cap = ctx->P.init_iters > 0 ? ctx->P.init_iters : 60;
 	// the mixed form solves its branch centres in CONT_GROUPS groups on their
