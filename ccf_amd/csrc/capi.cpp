@@ -127,7 +127,8 @@ void ccf_ctx::pool_release()
 	// work queued on the streams may still use a pooled buffer
 	if (stream) (void)hipStreamSynchronize(stream);
 	if (err_stream) (void)hipStreamSynchronize(err_stream);
-	if (group_stream) (void)hipStreamSynchronize(group_stream);
+	for (hipStream_t g : group_streams)
+		if (g) (void)hipStreamSynchronize(g);
 	for (auto& [n, p] : pool)
 	{
 		owned.erase(p);
@@ -141,7 +142,8 @@ void ccf_ctx::free_device()
 	// a failed solve can leave error-pass or continuation work queued on
 	// the other streams
 	if (err_stream) (void)hipStreamSynchronize(err_stream);
-	if (group_stream) (void)hipStreamSynchronize(group_stream);
+	for (hipStream_t g : group_streams)
+		if (g) (void)hipStreamSynchronize(g);
 	auto F = [this](auto*& p) { dfree(p); p = nullptr; };
 	ccf::DeviceTree& T = tree;
 	F(T.center); F(T.hl); F(T.parent); F(T.child); F(T.sstart); F(T.scount);
@@ -161,7 +163,7 @@ void ccf_ctx::free_device()
 		F(ccs); F(caustics); F(mu_scales);
 	}
 	ccf::free_sweep(this, sw);
-	ccf::free_sweep(this, sw2);
+	for (ccf::SweepBufs& b : swg) ccf::free_sweep(this, b);
 	F(counters);
 	if (host_counts) (void)hipHostFree(host_counts);
 	host_counts = nullptr;
@@ -699,7 +701,8 @@ void ccf_destroy(ccf_ctx* c)
 	if (c->err_event) (void)hipEventDestroy(c->err_event);
 	if (c->err_stream) (void)hipStreamDestroy(c->err_stream);
 	if (c->group_event) (void)hipEventDestroy(c->group_event);
-	if (c->group_stream) (void)hipStreamDestroy(c->group_stream);
+	for (hipStream_t g : c->group_streams)
+		if (g) (void)hipStreamDestroy(g);
 	if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
 	delete c;
 }

@@ -28,9 +28,9 @@ constexpr int MAX_ROOT_FMM_ORDER = 24;
 // highest degree of the continuation predictor
 constexpr int MAX_PREDICTOR = 4;
 constexpr int MAX_ROOT_GRID_LEVELS = 10;
-// groups of branches whose mixed-form continuations run on their own streams
-// side by side (one group's latency-bound sweeps beside the other's
-// device-wide ones)
+// groups of branches whose mixed-form solves run on their own streams side
+// by side (one group's latency-bound sweeps beside the others' device-wide
+// ones): at most CONT_GROUPS, at most one per branch
 constexpr int CONT_GROUPS = 2;
 
 // Quadtree over the stars, structure of arrays indexed by node id. Nodes of
@@ -140,8 +140,6 @@ struct SweepBufs
 	int* cnt = nullptr;      // [2][rows]
 	int* fl = nullptr;       // [rows][R] roots that need the Cauchy sums this sweep
 	int* fcnt = nullptr;     // [2][rows]
-	int* dl = nullptr;       // [rows][R] roots outside the tree, deferred to the cooperative Witt kernel
-	int* dcnt = nullptr;     // [2][rows]
 	double2* f0 = nullptr;   // [rows][R] Witt function at the roots of fl
 	double2* f1 = nullptr;   // [rows][R] its z-derivative
 	double2* ex = nullptr;   // [rows][R] terms of the Aberth sum formed per root
@@ -240,9 +238,9 @@ struct ccf_ctx
 	hipStream_t err_stream = nullptr; // the error pass of find_ccs, beside the continuation (default priority)
 	hipEvent_t err_event = nullptr;   // a row set's roots are final (no timing)
 	hipEvent_t fmm_rows_event[2 * ccf::CONT_GROUPS] = {}; // mixed continuation: after the copy of a step's FMM row count, per group (no timing)
-	hipStream_t group_stream = nullptr; // mixed continuation: the second group of branches' stream (default priority)
+	hipStream_t group_streams[ccf::CONT_GROUPS - 1] = {}; // mixed form: the streams of the groups of branches after the first (default priority)
 	hipEvent_t group_event = nullptr;   // orders the group streams around the continuation (no timing)
-	ccf::SweepBufs sw2;                 // mixed continuation: the second group's sweep buffers
+	ccf::SweepBufs swg[ccf::CONT_GROUPS - 1]; // mixed form: the sweep buffers of the groups after the first
 	int fmm_rows_sweep[2] = {-1, -1}; // root-sum FMM rows of the last phase's first and second sweep (-1: unknown)
 	int phase_sweeps = 0;         // sweeps the last phase ran (its cap when no read-back saw it end)
 	// params.autotune: the choices the context's first solve timed fastest

@@ -5,13 +5,13 @@
 // write the roots in place (ccf_functions.cuh:321-394) and launches 20 of
 // them per phi step whatever the convergence state (ccf.cuh:1013-1018).
 // Here a "row" is one phi value of one branch side and a sweep is one Jacobi
-// update of every still-active root of every row of a phase, in four
+// update of every still-active root of every row of a phase, in three
 // launches:
 //   K1 k_witt      per active root: Witt's function f0, its derivative f1
 //                  (near-field P2P + far-field L2P + smooth sheet) and the
 //                  1/mu early exit; roots that need the Cauchy sums are
-//                  appended to a per-row list; k_witt_far does the same for
-//                  the few roots outside the tree and, in its last workgroup,
+//                  appended to a per-row list (the few roots outside the
+//                  tree by their whole workgroup); its last workgroup runs
 //   K2 plan_sums   cuts the (stars + roots) term range of every tile of
 //                  RPT*256 listed roots into S slices so that dense and sparse
 //                  sweeps alike give enough equal work items to fill the chip;
@@ -277,9 +277,6 @@ struct SweepView
 	int* fl;
 	int* fcnt;
 	int* fcnt_next;
-	int* dl;
-	int* dcnt;
-	int* dcnt_next;
 	double2* f0;
 	double2* f1;
 	double2* ex;
@@ -306,9 +303,6 @@ SweepView view(const SweepBufs& B, int parity)
 	v.fl = B.fl;
 	v.fcnt = B.fcnt + parity * B.rows;
 	v.fcnt_next = B.fcnt + (parity ^ 1) * B.rows;
-	v.dl = B.dl;
-	v.dcnt = B.dcnt + parity * B.rows;
-	v.dcnt_next = B.dcnt + (parity ^ 1) * B.rows;
 	v.f0 = B.f0;
 	v.f1 = B.f1;
 	v.ex = B.ex;
@@ -412,8 +406,6 @@ __device__ inline Near coop_near(const LensView& L, int node, cd z, double (*red
 // row with at least fmm_min listed roots (fmm_min > 0) takes its root sums
 // from the FMM pass instead and gets no tiles. Formed by the last workgroup
 // of k_witt to finish, once every row's listed count is final.
-constexpr int WITT_FAR_BLOCKS = 16;
-constexpr int WITT_FAR_THREADS = 512;
 
 struct PlanArgs
 {
@@ -476,48 +468,95 @@ __device__ inline void plan_sums(SweepView V, const PlanArgs& P)
 	}
 }
 
-// K1 (ccf_functions.cuh:197-213 and the early return at :207-211). A root
-// outside the tree (whose near field is every star, tree_node.cuh:529-561)
-// is deferred to k_witt_far, which sums those stars with a whole workgroup.
+// The near-field sums of a root outside the tree (its node is not a leaf:
+// every star of the node's ranges), by the lanes of one wave: lane l sums
+// the stars l, l + 64, ... of the ranges (p2p's terms), and a butterfly of
+// fixed order gives every lane the total. Every lane of the wave calls it
+// with the same z and node.
+__device__ inline Near wave_near(const LensView& L, int node, cd z)
+{
+	Near nr;
+	const int lane = threadIdx.x & 63;
+	const int ranges = L.nnf[node];
+	for (int k = 0; k < ranges; k++)
+	{
+		const int2 rg = L.nf[node * MAX_NEAR_RANGES + k];
+		for (int s = rg.x + lane; s < rg.x + rg.y; s += 64)
+		{
+			const double2 q = L.spos[s];
+			const cd t = cd(z.re - q.x, z.im - q.y).inv();
+			nr.s0 += t;
+			const cd t1 = t * L.smass[s];
+			nr.s1 += t1;
+			const cd t2 = t1 * t;
+			nr.s2 += t2;
+			nr.s3 += t2 * t;
+		}
+		nr.count += rg.y;
+	}
+	double v[8] = {nr.s0.re, nr.s0.im, nr.s1.re, nr.s1.im, nr.s2.re, nr.s2.im, nr.s3.re, nr.s3.im};
+#pragma unroll
+	for (int q = 0; q < 8; q++)
+		for (int o = 32; o >= 1; o >>= 1) v[q] += __shfl_xor(v[q], o, 64);
+	nr.s0 = cd(v[0], v[1]);
+	nr.s1 = cd(v[2], v[3]);
+	nr.s2 = cd(v[4], v[5]);
+	nr.s3 = cd(v[6], v[7]);
+	return nr;
+}
+
+// K1 (ccf_functions.cuh:197-213 and the early return at :207-211), one
+// thread per listed root. A root outside the tree (whose near field is every
+// star, tree_node.cuh:529-561) takes its near-field sums from its whole wave
+// (wave_near), then goes on as the others do. Every workgroup then takes a
+// ticket; the last to finish sees every row's listed count and forms the
+// plan of the sweep's Cauchy sums.
 template <bool MIXED>
 __global__ __launch_bounds__(BLOCK) void k_witt(LensView L, double2* __restrict__ roots, RowMap M, SweepView V,
-	unsigned long long* __restrict__ stats, int circ_exact, int fmm_pole)
+	unsigned long long* __restrict__ stats, int circ_exact, int fmm_pole, PlanArgs P)
 {
+	__shared__ int s_last;
 	const int2 sl = xcd_slot();
 	const int r = sl.y;
 	const int R = M.R;
-	// (cnt_next was cleared by the previous sweep's k_plan or by k_prepare:
+	// (cnt_next was cleared by the previous sweep's plan or by k_prepare:
 	// stepped roots are appended to it below)
-	if (xcd_row_leader(sl) && threadIdx.x == 0)
-	{
-		V.fcnt_next[r] = 0;
-		V.dcnt_next[r] = 0;
-	}
+	if (xcd_row_leader(sl) && threadIdx.x == 0) V.fcnt_next[r] = 0;
 	const int n = V.cnt[r];
 	const int i = xcd_x(sl.x, (n + BLOCK - 1) / BLOCK) * BLOCK + threadIdx.x;
-	if (i - (int)threadIdx.x >= n) return; // a block without listed roots (uniform)
-	unsigned long long nfull = 0, nearly = 0, nnewton = 0, nconv = 0, nnear = 0, nnear_early = 0;
-	bool full = false, deferred = false, stepped = false;
-	int a = 0;
-	cd f0, f1, ex;
-	if (i < n)
+	if (i - (int)threadIdx.x < n) // a block with listed roots (uniform)
 	{
-		int row;
+		unsigned long long nfull = 0, nearly = 0, nnewton = 0, nconv = 0, nnear = 0, nnear_early = 0;
+		bool full = false, stepped = false;
+		int a = 0;
+		cd f0, f1, ex;
+		int row = 0, node = 0;
 		double phi;
-		M.get(r, row, phi);
-		a = V.act[(size_t)r * R + i];
-		double2* p = roots + (size_t)row * R + a;
-		const cd z(p->x, p->y);
-		const int node = nearest_node(L, z);
-		if (L.nf_off[node] < 0)
+		double2* p = nullptr;
+		cd z;
+		if (i < n)
 		{
-			deferred = true;
+			M.get(r, row, phi);
+			a = V.act[(size_t)r * R + i];
+			p = roots + (size_t)row * R + a;
+			z = cd(p->x, p->y);
+			node = nearest_node(L, z);
 		}
-		else
+		// the wave's roots outside the tree, one at a time by the whole wave
+		const bool outside = i < n && L.nf_off[node] < 0;
+		Near nr;
+		for (unsigned long long m = __ballot(outside); m; m &= m - 1)
 		{
+			const int src = __ffsll(m) - 1;
+			const Near t = wave_near(L, __shfl(node, src, 64), cd(__shfl(z.re, src, 64), __shfl(z.im, src, 64)));
+			if ((int)(threadIdx.x & 63) == src) nr = t;
+		}
+		if (i < n)
+		{
+			if (!outside) nr = p2p(L, node, z);
 			double cp, sp;
 			M.phase(row, phi, cp, sp);
-			const Witt w = witt_from(L, z, cp, sp, node, p2p(L, node, z));
+			const Witt w = witt_from(L, z, cp, sp, node, nr);
 			const int st = witt_root<MIXED>(L, p, z, w, circ_exact, fmm_pole, M.j > 0, f1, ex);
 			full = st == ROOT_FULL;
 			if (full) f0 = w.f0;
@@ -529,96 +568,29 @@ __global__ __launch_bounds__(BLOCK) void k_witt(LensView L, double2* __restrict_
 			if (st != ROOT_EARLY) nnear = w.nr.count;
 			else nnear_early = w.nr.count;
 		}
-	}
-	const int dpos = block_append(&V.dcnt[r], deferred);
-	if (deferred) V.dl[(size_t)r * R + dpos] = a;
-	const int spos = block_append(&V.cnt_next[r], stepped);
-	if (stepped) V.act_next[(size_t)r * R + spos] = a;
-	const int pos = block_append(&V.fcnt[r], full);
-	if (full)
-	{
-		const size_t k = (size_t)r * R + pos;
-		V.fl[k] = a;
-		if (V.slot) V.slot[(size_t)r * R + a] = pos;
-		V.f0[k] = make_double2(f0.re, f0.im);
-		V.f1[k] = make_double2(f1.re, f1.im);
-		V.ex[k] = make_double2(ex.re, ex.im);
-	}
-	const int which[6] = {C_FULL, C_EARLY, C_CONV, C_NEAR, C_NEWTON, C_NEAR_EARLY};
-	unsigned long long v[6] = {nfull, nearly, nconv, nnear, nnewton, nnear_early};
-	block_stats(stats, which, v);
-}
-
-// K1 for the deferred roots: one workgroup per root sums the star ranges of
-// its node (the same terms as p2p's range loop, split over the lanes and
-// reduced in a fixed order), then finishes the root as k_witt does. Few
-// roots lie outside the tree, and the launch comes every sweep whether a
-// row has any or not: 16 workgroups per row, each taking every 16th, of
-// WITT_FAR_THREADS lanes (a root's sum over N stars: N / 512 loads per lane).
-// Its last workgroup to finish forms the plan of the sweep's Cauchy sums.
-template <bool MIXED>
-__global__ __launch_bounds__(WITT_FAR_THREADS) void k_witt_far(LensView L, double2* __restrict__ roots, RowMap M, SweepView V,
-	unsigned long long* __restrict__ stats, int circ_exact, int fmm_pole, PlanArgs P)
-{
-	__shared__ int s_last;
-	const int r = blockIdx.y;
-	const int R = M.R;
-	const int nd = V.dcnt[r];
-	__shared__ double red[WITT_FAR_THREADS / 64][8];
-	for (int d = blockIdx.x; d < nd; d += gridDim.x)
-	{
-		int row;
-		double phi;
-		M.get(r, row, phi);
-		const int a = V.dl[(size_t)r * R + d];
-		double2* p = roots + (size_t)row * R + a;
-		const cd z(p->x, p->y);
-		const int node = nearest_node(L, z);
-		const Near nr = coop_near<WITT_FAR_THREADS>(L, node, z, red);
-		if (threadIdx.x == 0)
+		const int spos = block_append(&V.cnt_next[r], stepped);
+		if (stepped) V.act_next[(size_t)r * R + spos] = a;
+		const int pos = block_append(&V.fcnt[r], full);
+		if (full)
 		{
-			const Near& sum = nr;
-			double cp, sp;
-			M.phase(row, phi, cp, sp);
-			const Witt w = witt_from(L, z, cp, sp, node, sum);
-			cd f1, ex;
-			const int st = witt_root<MIXED>(L, p, z, w, circ_exact, fmm_pole, M.j > 0, f1, ex);
-			if (st == ROOT_FULL)
-			{
-				const int pos = atomicAdd(&V.fcnt[r], 1);
-				const size_t k = (size_t)r * R + pos;
-				V.fl[k] = a;
-				if (V.slot) V.slot[(size_t)r * R + a] = pos;
-				V.f0[k] = make_double2(w.f0.re, w.f0.im);
-				V.f1[k] = make_double2(f1.re, f1.im);
-				V.ex[k] = make_double2(ex.re, ex.im);
-				atomicAdd(stat_slot(stats, C_FULL), 1ull);
-				atomicAdd(stat_slot(stats, C_NEAR), (unsigned long long)sum.count);
-			}
-			else if (st == ROOT_STEPPED)
-			{
-				V.act_next[(size_t)r * R + atomicAdd(&V.cnt_next[r], 1)] = a;
-				atomicAdd(stat_slot(stats, C_NEWTON), 1ull);
-				atomicAdd(stat_slot(stats, C_NEAR), (unsigned long long)sum.count);
-			}
-			else
-			{
-				atomicAdd(stat_slot(stats, st == ROOT_NEWTON ? C_NEWTON : C_EARLY), 1ull);
-				atomicAdd(stat_slot(stats, C_CONV), 1ull);
-				atomicAdd(stat_slot(stats, st == ROOT_NEWTON ? C_NEAR : C_NEAR_EARLY), (unsigned long long)sum.count);
-			}
+			const size_t k = (size_t)r * R + pos;
+			V.fl[k] = a;
+			if (V.slot) V.slot[(size_t)r * R + a] = pos;
+			V.f0[k] = make_double2(f0.re, f0.im);
+			V.f1[k] = make_double2(f1.re, f1.im);
+			V.ex[k] = make_double2(ex.re, ex.im);
 		}
-		__syncthreads();
+		const int which[6] = {C_FULL, C_EARLY, C_CONV, C_NEAR, C_NEWTON, C_NEAR_EARLY};
+		unsigned long long v[6] = {nfull, nearly, nconv, nnear, nnewton, nnear_early};
+		block_stats(stats, which, v);
 	}
-	// every workgroup takes a ticket once its appends are issued; the last
-	// one sees every row's listed count (k_witt's and these) and forms the
-	// sums' plan, then resets the ticket for the next sweep
-	__syncthreads();
-	if (threadIdx.x == 0)
-	{
-		__threadfence();
-		s_last = atomicAdd(V.ticket, 1) == (int)(gridDim.x * gridDim.y) - 1;
-	}
+	// every workgroup takes a ticket once its appends are done; the last one
+	// sees every row's listed count and forms the sums' plan, then resets
+	// the ticket for the next sweep. The plan reads only the list counts,
+	// device-scope atomics whose returned values block_append waited for
+	// before the ticket is taken, so no release fence precedes it (one per
+	// workgroup would write back the L2 of its XCD).
+	if (threadIdx.x == 0) s_last = atomicAdd(V.ticket, 1) == (int)(gridDim.x * gridDim.y) - 1;
 	__syncthreads();
 	if (s_last)
 	{
@@ -1355,8 +1327,6 @@ __global__ void k_prepare(double2* roots, RowMap M, SweepBufs B, int order, cons
 		B.cnt[B.rows + r] = 0;
 		B.fcnt[r] = 0;
 		B.fcnt[B.rows + r] = 0;
-		B.dcnt[r] = 0;
-		B.dcnt[B.rows + r] = 0;
 		if (r == 0)
 		{
 			B.plan[2] = 0;
@@ -1681,17 +1651,8 @@ bool run_sweeps_groups(ccf_ctx* ctx, SweepGroup* gs, int G, double2* roots, int 
 				tw.b = ctx->next_event();
 				if (tw.a) (void)hipEventRecord(tw.a, s);
 			}
-			const dim3 g_far(WITT_FAR_BLOCKS, nrows);
-			if (f32)
-			{
-				k_witt<true><<<g_witt, BLOCK, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, fmm_pole ? 1 : 0);
-				k_witt_far<true><<<g_far, WITT_FAR_THREADS, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, fmm_pole ? 1 : 0, plan);
-			}
-			else
-			{
-				k_witt<false><<<g_witt, BLOCK, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, 0);
-				k_witt_far<false><<<g_far, WITT_FAR_THREADS, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, 0, plan);
-			}
+			if (f32) k_witt<true><<<g_witt, BLOCK, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, fmm_pole ? 1 : 0, plan);
+			else k_witt<false><<<g_witt, BLOCK, 0, s>>>(L, roots, M, V, ctx->counters, circ_exact, 0, plan);
 			if (ctx->P.timing && tw.a && tw.b)
 			{
 				(void)hipEventRecord(tw.b, s);
@@ -1868,7 +1829,7 @@ bool sweep_layout(ccf_ctx* ctx, SweepBufs& b, int rows, int R, bool exact_items,
 	}
 	const size_t list = (size_t)rows * R;
 	if (!(take(b.act, 2 * list) && take(b.cnt, 2 * (size_t)rows) && take(b.fl, list) && take(b.fcnt, 2 * (size_t)rows) &&
-			take(b.dl, list) && take(b.dcnt, 2 * (size_t)rows) && take(b.f0, list) && take(b.f1, list) && take(b.ex, list) &&
+			take(b.f0, list) && take(b.f1, list) && take(b.ex, list) &&
 			take(b.tile_off, (size_t)rows + 1) && take(b.plan, 4) && take(b.ticket, 1) &&
 			take(b.part, (size_t)b.max_items * b.tile)))
 		return false;
@@ -1924,7 +1885,7 @@ bool alloc_root_fmm(ccf_ctx* ctx, SweepBufs& b)
 void free_sweep(ccf_ctx* ctx, SweepBufs& b)
 {
 	auto F = [ctx](auto*& p) { ctx->dfree(p); p = nullptr; };
-	F(b.act); F(b.cnt); F(b.fl); F(b.fcnt); F(b.dl); F(b.dcnt); F(b.f0); F(b.f1); F(b.ex); F(b.tile_off); F(b.plan); F(b.ticket);
+	F(b.act); F(b.cnt); F(b.fl); F(b.fcnt); F(b.f0); F(b.f1); F(b.ex); F(b.tile_off); F(b.plan); F(b.ticket);
 	F(b.part);
 	F(b.slot); F(b.rmode); F(b.rsum); F(b.ccnt); F(b.cstart); F(b.ccur); F(b.cell); F(b.bidx); F(b.sidx); F(b.spos);
 	F(b.ocnt); F(b.oidx); F(b.mp); F(b.rticket);
@@ -1937,8 +1898,7 @@ bool alloc_sweep(ccf_ctx* ctx, SweepBufs& b, int rows, int R)
 	auto take = [ctx](auto*& p, size_t count) { return CCF_HIP(ctx, ctx->dmalloc(&p, count * sizeof(*p))); };
 	if (!sweep_layout(ctx, b, rows, R, false, take)) return false;
 	const size_t list = (size_t)rows * R;
-	if (!CCF_HIP(ctx, hipMemsetAsync(b.dcnt, 0, 2 * rows * sizeof(int), ctx->stream)) ||
-		!CCF_HIP(ctx, hipMemsetAsync(b.cnt, 0, 2 * rows * sizeof(int), ctx->stream)) ||
+	if (!CCF_HIP(ctx, hipMemsetAsync(b.cnt, 0, 2 * rows * sizeof(int), ctx->stream)) ||
 		!CCF_HIP(ctx, hipMemsetAsync(b.fcnt, 0, 2 * rows * sizeof(int), ctx->stream)) ||
 		!CCF_HIP(ctx, hipMemsetAsync(b.ticket, 0, sizeof(int), ctx->stream)))
 		return false;
@@ -1971,6 +1931,40 @@ bool launch_phase(ccf_ctx* ctx)
 	return CCF_LAUNCH(ctx, "phase table");
 }
 
+// groups of branches solved side by side: the mixed form's (at most
+// CONT_GROUPS, at most one per branch); the FP64 form reads its sweeps back
+// and runs one
+inline int group_count(const ccf_ctx* ctx)
+{
+	return ctx->P.precision == 1 ? std::max(1, std::min(CONT_GROUPS, ctx->P.num_branches)) : 1;
+}
+
+// the groups a solve runs: group_count's, if their buffers were made for
+// the parameters the context has now (ccf_load makes them), else one
+inline int solve_groups(const ccf_ctx* ctx)
+{
+	const int G = group_count(ctx);
+	for (int g = 1; g < G; g++)
+		if (!ctx->swg[g - 1].act) return 1;
+	return G;
+}
+
+// group g's sweep buffers and stream: the context's own for group 0
+inline SweepBufs& group_bufs(ccf_ctx* ctx, int g) { return g == 0 ? ctx->sw : ctx->swg[g - 1]; }
+inline hipStream_t group_stream(ccf_ctx* ctx, int g) { return g == 0 ? ctx->stream : ctx->group_streams[g - 1]; }
+
+// the sweep buffers: the first group's (also every phase that runs one
+// group) for all rows of a step, each other group's for its 2 (c1 - c0)
+// rows, its branches c0 .. c1 - 1
+bool alloc_groups(ccf_ctx* ctx)
+{
+	const int R = ctx->num_roots, nb = ctx->P.num_branches, G = group_count(ctx);
+	if (!alloc_sweep(ctx, ctx->sw, ctx->max_rows, R)) return false;
+	for (int g = 1; g < G; g++)
+		if (!alloc_sweep(ctx, ctx->swg[g - 1], 2 * ((g + 1) * nb / G - g * nb / G), R)) return false;
+	return true;
+}
+
 bool alloc_solver(ccf_ctx* ctx)
 {
 	const int R = ctx->num_roots, np = ctx->P.num_phi, nb = ctx->P.num_branches;
@@ -1986,13 +1980,9 @@ bool alloc_solver(ccf_ctx* ctx)
 			CCF_HIP(ctx, hipMemcpyAsync(ctx->dseeds, ctx->seeds.data(), (size_t)R * sizeof(double2), hipMemcpyHostToDevice, ctx->stream)) &&
 			CCF_HIP(ctx, hipMemsetAsync(ctx->ccs_init, 0, pts * sizeof(double2), ctx->stream)) &&
 			CCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
-	// the mixed form's continuation runs CONT_GROUPS groups of branches
-	// side by side: the first group's rows in sw, the others' in sw2
-	const bool groups = ctx->P.precision == 1 && nb >= CONT_GROUPS;
 	return CCF_HIP(ctx, ctx->dmalloc(&ctx->dseeds, (size_t)R * sizeof(double2))) &&
 		CCF_HIP(ctx, hipMemcpyAsync(ctx->dseeds, ctx->seeds.data(), (size_t)R * sizeof(double2), hipMemcpyHostToDevice, ctx->stream)) &&
-		CCF_HIP(ctx, ctx->dmalloc(&ctx->ccs_init, pts * sizeof(double2))) && alloc_sweep(ctx, ctx->sw, ctx->max_rows, R) &&
-		(!groups || alloc_sweep(ctx, ctx->sw2, 2 * (nb - nb / CONT_GROUPS), R)) &&
+		CCF_HIP(ctx, ctx->dmalloc(&ctx->ccs_init, pts * sizeof(double2))) && alloc_groups(ctx) &&
 		CCF_HIP(ctx, hipHostMalloc((void**)&ctx->host_counts, (ctx->max_rows + 6 * CONT_GROUPS) * sizeof(int), hipHostMallocDefault)) &&
 		CCF_HIP(ctx, ctx->dmalloc(&ctx->phase, (size_t)(np + nb) * sizeof(double2))) && launch_phase(ctx) &&
 		CCF_HIP(ctx, ctx->dmalloc(&ctx->counters, STAT_WORDS * sizeof(unsigned long long))) &&
@@ -2000,21 +1990,29 @@ bool alloc_solver(ccf_ctx* ctx)
 		CCF_HIP(ctx, hipMemsetAsync(ctx->counters, 0, STAT_WORDS * sizeof(unsigned long long), ctx->stream));
 }
 
-// the second group of branches' stream (created once, default priority)
-// starts after everything queued on the context's stream so far
-bool group_streams_start(ccf_ctx* ctx)
+// the streams of groups 1 .. G - 1 (created once, default priority) start
+// after everything queued on the context's stream so far
+bool group_streams_start(ccf_ctx* ctx, int G)
 {
-	if (!ctx->group_stream && !CCF_HIP(ctx, hipStreamCreateWithFlags(&ctx->group_stream, hipStreamNonBlocking))) return false;
 	if (!ctx->group_event && !CCF_HIP(ctx, hipEventCreateWithFlags(&ctx->group_event, hipEventDisableTiming))) return false;
-	return CCF_HIP(ctx, hipEventRecord(ctx->group_event, ctx->stream)) &&
-		CCF_HIP(ctx, hipStreamWaitEvent(ctx->group_stream, ctx->group_event, 0));
+	if (!CCF_HIP(ctx, hipEventRecord(ctx->group_event, ctx->stream))) return false;
+	for (int g = 1; g < G; g++)
+	{
+		hipStream_t& q = ctx->group_streams[g - 1];
+		if (!q && !CCF_HIP(ctx, hipStreamCreateWithFlags(&q, hipStreamNonBlocking))) return false;
+		if (!CCF_HIP(ctx, hipStreamWaitEvent(q, ctx->group_event, 0))) return false;
+	}
+	return true;
 }
 
-// what follows on the context's stream waits for the second group's work
-bool group_streams_join(ccf_ctx* ctx)
+// what follows on the context's stream waits for the other groups' work
+bool group_streams_join(ccf_ctx* ctx, int G)
 {
-	return CCF_HIP(ctx, hipEventRecord(ctx->group_event, ctx->group_stream)) &&
-		CCF_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->group_event, 0));
+	for (int g = 1; g < G; g++)
+		if (!CCF_HIP(ctx, hipEventRecord(ctx->group_event, ctx->group_streams[g - 1])) ||
+			!CCF_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->group_event, 0)))
+			return false;
+	return true;
 }
 
 // params.autotune: the context's first solve times, on the initial solve of
@@ -2034,7 +2032,7 @@ bool autotune_dev(ccf_ctx* ctx, int verbose)
 	const bool tune_fmm = ctx->P.precision == 1 && ctx->P.root_fmm == 1 && Pin.root_fmm_order == 0 && Pin.root_fmm_leaf == 0 &&
 		R >= ROOT_FMM_MIN_ROOTS;
 	auto t0 = std::chrono::steady_clock::now();
-	const int G = (ctx->sw2.act && nb >= CONT_GROUPS) ? CONT_GROUPS : 1;
+	const int G = solve_groups(ctx);
 	const int nbg = nb / G;
 	const RowMap M{R, np, nb, 0, 1, 0.0, ctx->phase, 0};
 	const int cap = ctx->P.init_iters > 0 ? ctx->P.init_iters : 60;
@@ -2109,8 +2107,7 @@ bool autotune_dev(ccf_ctx* ctx, int verbose)
 	ctx->tuned_root_order = tune_fmm ? best_order : 0;
 	ctx->tuned_root_leaf = tune_fmm ? best_leaf : 0;
 	ctx->info = keep;
-	const bool groups = ctx->P.precision == 1 && nb >= CONT_GROUPS;
-	ok = alloc_sweep(ctx, ctx->sw, ctx->max_rows, R) && (!groups || alloc_sweep(ctx, ctx->sw2, 2 * (nb - nb / CONT_GROUPS), R)) &&
+	ok = alloc_groups(ctx) &&
 		CCF_HIP(ctx, hipMemsetAsync(ctx->counters, 0, STAT_WORDS * sizeof(unsigned long long), s)) &&
 		CCF_HIP(ctx, hipStreamSynchronize(s));
 	ctx->info.t_autotune = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -2131,19 +2128,17 @@ bool find_initial_roots_dev(ccf_ctx* ctx, int verbose)
 	if (!seed_rows(ctx, 0, nb, s)) return false;
 	ctx->sweep_verbose = verbose;
 	const int cap = ctx->P.init_iters > 0 ? ctx->P.init_iters : 60;
-	// the mixed form solves its branch centres in CONT_GROUPS groups on their
-	// own streams, sweep by sweep side by side (the root-sum FMM pass of a
-	// dense sweep is a chain of short launches); each group's rows are
-	// independent
-	const int G = (ctx->sw2.act && nb >= CONT_GROUPS) ? CONT_GROUPS : 1;
-	if (G > 1 && !group_streams_start(ctx)) return false;
+	// the mixed form solves its branch centres in groups on their own
+	// streams, sweep by sweep side by side (the root-sum FMM pass of a dense
+	// sweep is a chain of short launches); each group's rows are independent
+	const int G = solve_groups(ctx);
+	if (G > 1 && !group_streams_start(ctx, G)) return false;
 	SweepGroup gs[CONT_GROUPS];
 	for (int g = 0; g < G; g++)
 	{
 		const int c0 = g * nb / G, nbg = (g + 1) * nb / G - c0;
 		const RowMap M{R, np, nb, 0, 1, 0.0, ctx->phase, c0};
-		gs[g] = SweepGroup{g == 0 ? &ctx->sw : &ctx->sw2, M, nbg, g == 0 ? s : ctx->group_stream,
-			(1u << ROOT_FMM_INIT_SWEEPS) - 1u, g};
+		gs[g] = SweepGroup{&group_bufs(ctx, g), M, nbg, group_stream(ctx, g), (1u << ROOT_FMM_INIT_SWEEPS) - 1u, g};
 	}
 	for (int g = 0; g < G; g++)
 	{
@@ -2151,7 +2146,7 @@ bool find_initial_roots_dev(ccf_ctx* ctx, int verbose)
 		if (!CCF_LAUNCH(ctx, "prepare_roots_kernel")) return false;
 	}
 	if (!run_sweeps_groups(ctx, gs, G, ctx->ccs_init, cap)) return false;
-	if (G > 1 && !group_streams_join(ctx)) return false;
+	if (G > 1 && !group_streams_join(ctx, G)) return false;
 	if (!CCF_HIP(ctx, hipStreamSynchronize(s))) return false;
 	auto t1 = std::chrono::steady_clock::now();
 	ctx->info.t_init_roots = std::chrono::duration<double>(t1 - t0).count();
@@ -2226,21 +2221,16 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 	// other group's first sweep over all of its roots fills. The groups share
 	// only read-only inputs, the statistics (integer atomics) and the error
 	// pass's stream; each writes its own rows.
-	const int G = (tailed && nb >= CONT_GROUPS && ctx->sw2.act) ? CONT_GROUPS : 1;
+	const int G = tailed ? solve_groups(ctx) : 1;
 	int gb[CONT_GROUPS + 1] = {};
 	hipStream_t gs[CONT_GROUPS] = {};
 	SweepBufs* gsw[CONT_GROUPS] = {};
 	for (int g = 0; g <= G; g++) gb[g] = g * nb / G;
-	gs[0] = s;
-	gsw[0] = &ctx->sw;
-	if (G > 1)
+	if (G > 1 && !group_streams_start(ctx, G)) return false;
+	for (int g = 0; g < G; g++)
 	{
-		if (!group_streams_start(ctx)) return false;
-		for (int g = 1; g < G; g++)
-		{
-			gs[g] = ctx->group_stream;
-			gsw[g] = &ctx->sw2;
-		}
+		gs[g] = group_stream(ctx, g);
+		gsw[g] = &group_bufs(ctx, g);
 	}
 	// the rows of a group's previous step (the centres before the first)
 	// take their pass while this step's tail runs, a launch of one workgroup
@@ -2308,7 +2298,7 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 	for (int g = 0; g < G; g++)
 		if (!errors(prev_rows(steps + 1, g), steps == 0 ? gb[g + 1] - gb[g] : 2 * (gb[g + 1] - gb[g]), gs[g])) return false;
 	// the stages after the continuation run on the context's stream
-	if (G > 1 && !group_streams_join(ctx)) return false;
+	if (G > 1 && !group_streams_join(ctx, G)) return false;
 	if (!CCF_HIP(ctx, hipStreamSynchronize(s))) return false;
 	auto t1 = std::chrono::steady_clock::now();
 	ctx->info.t_ccs = std::chrono::duration<double>(t1 - t0).count();
