@@ -215,7 +215,6 @@ struct ccf_ctx
 	int max_rows = 0;
 	ccf::SweepBufs sw;
 	double* ones = nullptr;       // unit masses for the pole-sum expansions
-	long long* far_points = nullptr; // error pass: count of the points outside the tree, then their indices
 	unsigned long long* counters = nullptr; // device statistics, see solve.hip
 	std::vector<ccf::Timer> timers;
 	std::vector<ccf::Timer> witt_timers;

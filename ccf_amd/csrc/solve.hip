@@ -66,6 +66,7 @@ enum Counter
 	C_TAIL_NEWTON = 16, // k_tail: Newton steps (not in C_NEWTON)
 	C_TAIL_NEAR = 17,  // k_tail: near-field stars of its evaluations
 	C_TAIL_SWEEPS = 18, // k_tail: sweeps, summed over rows
+	C_OUTSIDE = 19,     // error pass: points outside the tree (their near field is every star)
 	C_COUNT = 24
 };
 
@@ -119,24 +120,6 @@ __device__ inline unsigned long long* stat_slot(unsigned long long* stats, int c
 	return stats + ((size_t)c * STAT_SLOTS + ((blockIdx.x + 31u * blockIdx.y) & (STAT_SLOTS - 1))) * STAT_STRIDE;
 }
 
-// points of the error pass whose node is not a leaf (outside the tree),
-// listed for a cooperative pass: count (zeroed before each pass) and the
-// running total over the passes of a run (count[1]), then up to cap global
-// point indices
-struct FarPoints
-{
-	int* count;
-	long long* list;
-	int cap;
-};
-constexpr int FAR_POINTS_CAP = 1 << 20;
-constexpr int FAR_POINTS_BLOCKS = 256; // workgroups of one row set's pass over its list
-
-__global__ void k_zero_count(int* p, int both)
-{
-	p[0] = 0;
-	if (both) p[1] = 0;
-}
 
 // adds v[k] of every thread of the block to counter which[k]; every thread
 // of a BS-sized workgroup calls this
@@ -1341,9 +1324,8 @@ __global__ void k_prepare(double2* roots, RowMap M, SweepBufs B, int order, cons
 // same evaluation also gives the caustic point and the length scale
 // (find_caustics_kernel, find_mu_length_scales_kernel), written in file
 // order: root a, phi row r at a * rows + r.
-__global__ void k_errors(LensView L, const double2* __restrict__ roots, int R, RowMap M, int rows,
-	const int* __restrict__ perm, unsigned long long* stats, double2* __restrict__ caus, double* __restrict__ mu,
-	FarPoints far)
+__global__ __launch_bounds__(BLOCK) void k_errors(LensView L, const double2* __restrict__ roots, int R, RowMap M, int rows,
+	const int* __restrict__ perm, unsigned long long* stats, double2* __restrict__ caus, double* __restrict__ mu)
 {
 	// grid: (xcd_round(blocks per row), rows of M); a row's points in the
 	// roots' processing order; rows: the phi rows of the output
@@ -1355,28 +1337,32 @@ __global__ void k_errors(LensView L, const double2* __restrict__ roots, int R, R
 	const long long idx = (long long)row * R + (k < R ? perm[k] : 0);
 	double e = 0, res = 0;
 	unsigned long long bad = 0;
+	cd z;
+	int node = 0;
 	if (k < R)
 	{
 		const double2 v = roots[idx];
-		const cd z(v.x, v.y);
-		const int node = nearest_node(L, z);
-		// a point outside the tree sums every star: k_errors_far takes it
-		// with a whole workgroup (inline here only past the list's capacity)
-		bool deferred = false;
-		if (L.nf_off[node] < 0)
+		z = cd(v.x, v.y);
+		node = nearest_node(L, z);
+	}
+	// a point outside the tree sums every star: its whole wave forms the
+	// sums (wave_near), one such point at a time
+	const bool outside = k < R && L.nf_off[node] < 0;
+	const unsigned long long mo = __ballot(outside);
+	Near nr;
+	for (unsigned long long m = mo; m; m &= m - 1)
+	{
+		const int src = __ffsll(m) - 1;
+		const Near t = wave_near(L, __shfl(node, src, 64), cd(__shfl(z.re, src, 64), __shfl(z.im, src, 64)));
+		if ((int)(threadIdx.x & 63) == src) nr = t;
+	}
+	if (k < R)
+	{
 		{
-			const int q = atomicAdd(far.count, 1);
-			if (q < far.cap)
-			{
-				far.list[q] = idx;
-				deferred = true;
-			}
-		}
-		if (!deferred)
-		{
+			if (!outside) nr = p2p(L, node, z);
 			double cp, sp;
 			M.phase(row, phi, cp, sp);
-			Witt w = witt_from(L, z, cp, sp, node, p2p(L, node, z));
+			Witt w = witt_from(L, z, cp, sp, node, nr);
 			e = err_bound(w);
 			res = w.f0.abs();
 			if (!isfinite(e)) { bad = 1; e = 0; res = 0; }
@@ -1402,48 +1388,7 @@ __global__ void k_errors(LensView L, const double2* __restrict__ roots, int R, R
 		atomicMax(stat_slot(stats, C_ERRBITS), (unsigned long long)__double_as_longlong(e));
 		atomicMax(stat_slot(stats, C_RESID), (unsigned long long)__double_as_longlong(res));
 		if (bad) atomicOr(stat_slot(stats, C_NAN), 1ull);
-	}
-}
-
-// the points k_errors deferred: one workgroup per point, the near-field sums
-// over the lanes (coop_near), then the same bound, residual, caustic point
-// and length scale
-__global__ __launch_bounds__(BLOCK) void k_errors_far(LensView L, const double2* __restrict__ roots, int R, int rows,
-	const double2* __restrict__ cs, unsigned long long* stats, double2* __restrict__ caus, double* __restrict__ mu, FarPoints far)
-{
-	__shared__ double red[BLOCK / 64][8];
-	const int n = min(*far.count, far.cap);
-	if (blockIdx.x == 0 && threadIdx.x == 0) far.count[1] += *far.count;
-	for (int d = blockIdx.x; d < n; d += gridDim.x)
-	{
-		const long long idx = far.list[d];
-		const int row = (int)(idx / R);
-		const double2 v = roots[idx];
-		const cd z(v.x, v.y);
-		const int node = nearest_node(L, z);
-		const Near nr = coop_near(L, node, z, red);
-		if (threadIdx.x == 0)
-		{
-			const double2 t = cs[row];
-			const Witt w = witt_from(L, z, t.x, t.y, node, nr);
-			const double e = err_bound(w);
-			if (!isfinite(e)) atomicOr(stat_slot(stats, C_NAN), 1ull);
-			else
-			{
-				atomicMax(stat_slot(stats, C_ERRBITS), (unsigned long long)__double_as_longlong(e));
-				atomicMax(stat_slot(stats, C_RESID), (unsigned long long)__double_as_longlong(w.f0.abs()));
-			}
-			if (caus || mu)
-			{
-				cd ws;
-				double m;
-				lens_map_from(L, z, w.nr, w.fr, w.d1, w.d2, ws, m);
-				const size_t o = (size_t)(idx - (long long)row * R) * rows + row;
-				if (caus) caus[o] = make_double2(ws.re, ws.im);
-				if (mu) mu[o] = m;
-			}
-		}
-		__syncthreads();
+		if (mo) atomicAdd(stat_slot(stats, C_OUTSIDE), (unsigned long long)__popcll(mo));
 	}
 }
 
@@ -2193,25 +2138,14 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 	ctx->caustics_done = ctx->mu_done = false;
 	if (wc && !ctx->caustics && !CCF_HIP(ctx, ctx->dmalloc(&ctx->caustics, (size_t)total * sizeof(double2)))) return false;
 	if (wm && !ctx->mu_scales && !CCF_HIP(ctx, ctx->dmalloc(&ctx->mu_scales, (size_t)total * sizeof(double)))) return false;
-	// the list of points outside the tree: two counts, then up to FAR_POINTS_CAP indices
-	const int far_cap = (int)std::min<long long>(total, FAR_POINTS_CAP);
-	if (!ctx->far_points && !CCF_HIP(ctx, ctx->dmalloc(&ctx->far_points, sizeof(long long) * (1 + (size_t)FAR_POINTS_CAP))))
-		return false;
-	const FarPoints far{(int*)ctx->far_points, (long long*)ctx->far_points + 1, far_cap};
 	if (!ctx->err_stream && !CCF_HIP(ctx, hipStreamCreateWithFlags(&ctx->err_stream, hipStreamNonBlocking))) return false;
 	if (!ctx->err_event && !CCF_HIP(ctx, hipEventCreateWithFlags(&ctx->err_event, hipEventDisableTiming))) return false;
 	hipStream_t es = ctx->err_stream;
-	int nfar = 0;
-	bool first_pass = true;
 	auto errors = [&](const RowMap& Me, int nrows, hipStream_t after) -> bool {
 		if (!CCF_HIP(ctx, hipEventRecord(ctx->err_event, after)) || !CCF_HIP(ctx, hipStreamWaitEvent(es, ctx->err_event, 0)))
 			return false;
-		k_zero_count<<<1, 1, 0, es>>>(far.count, first_pass ? 1 : 0);
-		first_pass = false;
 		k_errors<<<dim3(xcd_round(grid(R)), nrows), BLOCK, 0, es>>>(L, ctx->ccs_init, R, Me, np + nb, ctx->dorder, ctx->counters,
-			wc ? ctx->caustics : nullptr, wm ? ctx->mu_scales : nullptr, far);
-		k_errors_far<<<FAR_POINTS_BLOCKS, BLOCK, 0, es>>>(L, ctx->ccs_init, R, np + nb, ctx->phase, ctx->counters,
-			wc ? ctx->caustics : nullptr, wm ? ctx->mu_scales : nullptr, far);
+			wc ? ctx->caustics : nullptr, wm ? ctx->mu_scales : nullptr);
 		return CCF_LAUNCH(ctx, "find_errors_kernel");
 	};
 	// The mixed form (no read-backs) runs its branches in CONT_GROUPS groups,
@@ -2311,11 +2245,8 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 		std::vector<unsigned long long> raw(STAT_WORDS);
 		if (!CCF_HIP(ctx, hipStreamSynchronize(es)) ||
 			!CCF_HIP(ctx, hipMemcpyAsync(raw.data(), ctx->counters, STAT_WORDS * sizeof(unsigned long long), hipMemcpyDeviceToHost, s)) ||
-			!CCF_HIP(ctx, hipMemcpyAsync(&nfar, far.count + 1, sizeof(int), hipMemcpyDeviceToHost, s)) ||
 			!CCF_HIP(ctx, hipStreamSynchronize(s)))
 			return false;
-		ctx->info.points_outside_tree = nfar;
-		if (verbose >= 3) std::cout << "error pass: " << nfar << " points outside the tree\n";
 		for (int k = 0; k < C_COUNT; k++)
 			for (int q = 0; q < STAT_SLOTS; q++)
 			{
@@ -2339,6 +2270,8 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 	if (verbose >= 1) std::cout << "Maximum error in 1/mu: " << e << "\n\n";
 	ctx->info.points_total = total;
 	ctx->info.points_converged = (long long)c[C_CONV];
+	ctx->info.points_outside_tree = (long long)c[C_OUTSIDE];
+	if (verbose >= 3) std::cout << "error pass: " << c[C_OUTSIDE] << " points outside the tree\n";
 	ctx->info.updates_full = (long long)(c[C_FULL] + c[C_TAIL_FULL]);
 	ctx->info.updates_early = (long long)(c[C_EARLY] + c[C_TAIL_EARLY]);
 	ctx->info.updates_newton = (long long)(c[C_NEWTON] + c[C_TAIL_NEWTON]);

@@ -122,7 +122,7 @@ typedef struct ccf_info
 	double tail_kernel_ms;     /* sum of timed tail launches (params.timing = 1) */
 	double flops_tail;         /* algorithmic flops of the tails (Witt evaluations and Cauchy sums) */
 	long long points_outside_tree; /* error pass: points outside the star tree's root box, whose near
-	                              field is every star (evaluated with a whole workgroup each) */
+	                              field is every star (summed by a whole wave each) */
 	int roots_per_thread;      /* the sums' roots per thread the run used (given, default or tuned) */
 	int root_fmm_leaf;         /* the mean roots per leaf cell its root-sum FMM grid aimed at */
 	double t_autotune;         /* seconds the run spent tuning (params.autotune; 0 once tuned) */
