@@ -128,8 +128,8 @@ __global__ __launch_bounds__(BLOCK) void k_rbin(RootView V, const double2* __res
 		double phi;
 		M.get(r, row, phi);
 		const double2 z = roots[(size_t)row * R + j];
-		const double sl = ldexp(V.g.s0, -V.g.leaf);
-		const double fx = (z.x - V.g.x0) / sl, fy = (z.y - V.g.y0) / sl;
+		const double isl = ldexp(1.0 / V.g.s0, V.g.leaf);
+		const double fx = (z.x - V.g.x0) * isl, fy = (z.y - V.g.y0) * isl;
 		const int nxl = V.g.nx << V.g.leaf, nyl = V.g.ny << V.g.leaf;
 		if (fx >= 0 && fx < nxl && fy >= 0 && fy < nyl) c = leaf_key(V.g, (int)fx, (int)fy);
 		V.cell[(size_t)r * R + j] = c;
@@ -273,14 +273,14 @@ __global__ __launch_bounds__(BIN_THREADS) void k_rbin_row(RootView V, const doub
 	unsigned long long* omask = bin_lds + (nl + 1) / 2;
 	int* obase = (int*)(omask + ng);
 	const int lane = threadIdx.x & 63;
+	int* __restrict__ cell = V.cell + (size_t)r * R;
 	for (int c = threadIdx.x; c < nl; c += BIN_THREADS) cnt[c] = 0;
 	__syncthreads();
 	int row;
 	double phi;
 	M.get(r, row, phi);
 	const double2* __restrict__ zr = roots + (size_t)row * R;
-	int* __restrict__ cell = V.cell + (size_t)r * R;
-	const double sl = ldexp(V.g.s0, -V.g.leaf);
+	const double isl = ldexp(1.0 / V.g.s0, V.g.leaf);
 	const int nxl = V.g.nx << V.g.leaf, nyl = V.g.ny << V.g.leaf;
 	// every wave covers whole groups of 64: j - lane is a multiple of 64.
 	// BIN_BATCH roots per thread and pass, their loads issued together.
@@ -302,7 +302,7 @@ __global__ __launch_bounds__(BIN_THREADS) void k_rbin_row(RootView V, const doub
 			int c = -1;
 			if (j < R)
 			{
-				const double fx = (z[k].x - V.g.x0) / sl, fy = (z[k].y - V.g.y0) / sl;
+				const double fx = (z[k].x - V.g.x0) * isl, fy = (z[k].y - V.g.y0) * isl;
 				if (fx >= 0 && fx < nxl && fy >= 0 && fy < nyl) c = leaf_key(V.g, (int)fx, (int)fy);
 				cell[j] = c;
 				if (c >= 0) atomicAdd(&cnt[c], 1);
