@@ -30,8 +30,13 @@ constexpr int MAX_PREDICTOR = 4;
 constexpr int MAX_ROOT_GRID_LEVELS = 10;
 // groups of branches whose mixed-form solves run on their own streams side
 // by side (one group's latency-bound sweeps beside the others' device-wide
-// ones): at most CONT_GROUPS, at most one per branch
-constexpr int CONT_GROUPS = 2;
+// ones), at most one per branch: CONT_GROUPS in the initial solve (chains of
+// short FMM launches), CONT_STEP_GROUPS in the continuation (more were
+// slower: each group's chain slows with the contention). The group streams
+// and the error pass's stay within the 4 hardware queues a process gets by
+// default; streams that share a queue serialise.
+constexpr int CONT_GROUPS = 3;
+constexpr int CONT_STEP_GROUPS = 2;
 
 // Quadtree over the stars, structure of arrays indexed by node id. Nodes of
 // a level are contiguous ([level_off[l], level_off[l] + level_cnt[l])) and

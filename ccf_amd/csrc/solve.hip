@@ -1879,16 +1879,16 @@ bool launch_phase(ccf_ctx* ctx)
 // groups of branches solved side by side: the mixed form's (at most
 // CONT_GROUPS, at most one per branch); the FP64 form reads its sweeps back
 // and runs one
-inline int group_count(const ccf_ctx* ctx)
+inline int group_count(const ccf_ctx* ctx, bool init)
 {
-	return ctx->P.precision == 1 ? std::max(1, std::min(CONT_GROUPS, ctx->P.num_branches)) : 1;
+	return ctx->P.precision == 1 ? std::max(1, std::min(init ? CONT_GROUPS : CONT_STEP_GROUPS, ctx->P.num_branches)) : 1;
 }
 
 // the groups a solve runs: group_count's, if their buffers were made for
 // the parameters the context has now (ccf_load makes them), else one
-inline int solve_groups(const ccf_ctx* ctx)
+inline int solve_groups(const ccf_ctx* ctx, bool init)
 {
-	const int G = group_count(ctx);
+	const int G = group_count(ctx, init);
 	for (int g = 1; g < G; g++)
 		if (!ctx->swg[g - 1].act) return 1;
 	return G;
@@ -1903,10 +1903,13 @@ inline hipStream_t group_stream(ccf_ctx* ctx, int g) { return g == 0 ? ctx->stre
 // rows, its branches c0 .. c1 - 1
 bool alloc_groups(ccf_ctx* ctx)
 {
-	const int R = ctx->num_roots, nb = ctx->P.num_branches, G = group_count(ctx);
+	const int R = ctx->num_roots, nb = ctx->P.num_branches, Gi = group_count(ctx, true), Gc = group_count(ctx, false);
 	if (!alloc_sweep(ctx, ctx->sw, ctx->max_rows, R)) return false;
-	for (int g = 1; g < G; g++)
-		if (!alloc_sweep(ctx, ctx->swg[g - 1], 2 * ((g + 1) * nb / G - g * nb / G), R)) return false;
+	for (int g = 1; g < std::max(Gi, Gc); g++)
+	{
+		const int ri = g < Gi ? (g + 1) * nb / Gi - g * nb / Gi : 0, rc = g < Gc ? 2 * ((g + 1) * nb / Gc - g * nb / Gc) : 0;
+		if (!alloc_sweep(ctx, ctx->swg[g - 1], std::max(ri, rc), R)) return false;
+	}
 	return true;
 }
 
@@ -1977,7 +1980,7 @@ bool autotune_dev(ccf_ctx* ctx, int verbose)
 	const bool tune_fmm = ctx->P.precision == 1 && ctx->P.root_fmm == 1 && Pin.root_fmm_order == 0 && Pin.root_fmm_leaf == 0 &&
 		R >= ROOT_FMM_MIN_ROOTS;
 	auto t0 = std::chrono::steady_clock::now();
-	const int G = solve_groups(ctx);
+	const int G = solve_groups(ctx, true);
 	const int nbg = nb / G;
 	const RowMap M{R, np, nb, 0, 1, 0.0, ctx->phase, 0};
 	const int cap = ctx->P.init_iters > 0 ? ctx->P.init_iters : 60;
@@ -2076,7 +2079,7 @@ bool find_initial_roots_dev(ccf_ctx* ctx, int verbose)
 	// the mixed form solves its branch centres in groups on their own
 	// streams, sweep by sweep side by side (the root-sum FMM pass of a dense
 	// sweep is a chain of short launches); each group's rows are independent
-	const int G = solve_groups(ctx);
+	const int G = solve_groups(ctx, true);
 	if (G > 1 && !group_streams_start(ctx, G)) return false;
 	SweepGroup gs[CONT_GROUPS];
 	for (int g = 0; g < G; g++)
@@ -2148,14 +2151,14 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 			wc ? ctx->caustics : nullptr, wm ? ctx->mu_scales : nullptr);
 		return CCF_LAUNCH(ctx, "find_errors_kernel");
 	};
-	// The mixed form (no read-backs) runs its branches in CONT_GROUPS groups,
+	// The mixed form (no read-backs) runs its branches in CONT_STEP_GROUPS groups,
 	// each on its own stream with its own sweep buffers: a group's steps are
 	// a chain of short, latency-bound launches (the later sweeps, the tail of
 	// one workgroup per row) that leaves most of the device idle, which the
 	// other group's first sweep over all of its roots fills. The groups share
 	// only read-only inputs, the statistics (integer atomics) and the error
 	// pass's stream; each writes its own rows.
-	const int G = tailed ? solve_groups(ctx) : 1;
+	const int G = tailed ? solve_groups(ctx, false) : 1;
 	int gb[CONT_GROUPS + 1] = {};
 	hipStream_t gs[CONT_GROUPS] = {};
 	SweepBufs* gsw[CONT_GROUPS] = {};
