@@ -167,7 +167,7 @@ struct SweepBufs
 	int* ocnt = nullptr;     // [rows] roots outside the grid
 	int* oidx = nullptr;     // [rows][R] their indices
 	double2* mp = nullptr;   // [rows][ncells][order + 1] multipole coefficients
-	int* rticket = nullptr;  // [rows] regions of the row done with their upward pass (the last resets it)
+	double2* lcq = nullptr;  // [rows][cells of levels 0 .. rlev][order + 1] their local expansions (k_rcoarse)
 };
 
 } // namespace ccf

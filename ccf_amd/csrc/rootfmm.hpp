@@ -79,13 +79,13 @@ struct RootView
 	int* ocnt;
 	int* oidx;
 	double2* mp;
-	int* rticket;
+	double2* lcq;
 };
 
 RootView root_view(const SweepBufs& B)
 {
 	return RootView{B.grid, B.slot, B.rmode, B.rsum, B.ccnt, B.cstart, B.ccur, B.cell, B.bidx, B.sidx, B.spos, B.ocnt, B.oidx,
-		B.mp, B.rticket};
+		B.mp, B.lcq};
 }
 
 __device__ inline cd cell_center(const RootGrid& g, int l, int ix, int iy)
@@ -102,6 +102,10 @@ __device__ __forceinline__ int leaf_key(const RootGrid& g, int ix, int iy)
 	const int rg = (iy >> s) * (g.nx << g.rlev) + (ix >> s);
 	return (rg << (2 * s)) + ((iy & m) << s) + (ix & m);
 }
+
+// cells of levels 0 .. rlev (the coarse cells, whose local expansions
+// k_rcoarse forms once per row)
+__host__ __device__ inline int coarse_cells(const RootGrid& g) { return g.rlev < g.leaf ? g.off[g.rlev + 1] : g.ncells; }
 
 // counts and the outlier count of the rows of this sweep
 __global__ void k_rzero(RootView V)
@@ -512,16 +516,17 @@ __device__ void rsum_outliers(const RootView& V, const double2* __restrict__ zr,
 	}
 }
 
-// After the binning a pass is two launches over regions (one cell of level
-// g.rlev each, a block of RS x RS leaf cells, RS = 2^(leaf - rlev) <=
-// REGION_SIDE), whose roots leaf_key keeps contiguous in the sorted arrays:
-//  k_rup   one workgroup per region: each leaf cell's roots sorted by index,
-//          P2M on the leaf cells and M2M up to level rlev in LDS; the row's
-//          last workgroup forms the levels above rlev.
-//  k_rdown one workgroup per region: M2L and L2L from level 0 to the leaf
-//          over the region's chain of ancestors and its own cells (the
-//          sources of a level's targets staged in LDS), then L2P and the
-//          direct near field of each listed root of the region.
+// After the binning a pass is three launches. Regions are the cells of
+// level g.rlev, each a block of RS x RS leaf cells (RS = 2^(leaf - rlev) <=
+// REGION_SIDE) whose roots leaf_key keeps contiguous in the sorted arrays:
+//  k_rup     one workgroup per region: each leaf cell's roots sorted by
+//            index, P2M on the leaf cells and M2M up to level rlev in LDS.
+//  k_rcoarse one workgroup per row: M2M above rlev, then the local
+//            expansions of every cell of levels 0 .. rlev (M2L, L2L).
+//  k_rdown   one workgroup per region: from its own cell's local expansion,
+//            M2L and L2L over its cells down to the leaf (the sources of a
+//            level's targets staged in LDS), then L2P and the direct near
+//            field of each listed root of the region.
 // k_rup's launch also carries RSUM_OUT_BLOCKS workgroups per row that sum
 // the roots outside the grid directly.
 
@@ -538,6 +543,19 @@ __device__ __forceinline__ cd m2m_term(const double2* a, int n, cd e, const doub
 		ep = ep * e;
 	}
 	return sum;
+}
+
+// L2L: L_m(child) = 2^-m sum_{n >= m} C(n, m) L_n(parent) e^(n - m), the
+// parent's polynomial shifted by e (repeated synthetic division), in place
+template <int P>
+__device__ __forceinline__ void l2l_shift(cd (&Lc)[P + 1], cd e)
+{
+#pragma unroll
+	for (int i = 0; i < P; i++)
+#pragma unroll
+		for (int j = P - 1; j >= i; j--) Lc[j] += e * Lc[j + 1];
+#pragma unroll
+	for (int m = 0; m <= P; m++) Lc[m] *= ldexp(1.0, -m);
 }
 
 // the cell of sorted position q among nc <= 128 cells: the last k with
@@ -567,7 +585,6 @@ __global__ __launch_bounds__(REGION_THREADS) void k_rup(RootView V, const double
 	__shared__ double binom[NC][NC];
 	__shared__ int sbi[RUP_STAGE];     // the region's binned root indices
 	__shared__ double2 sz[RUP_STAGE];  // and their positions in sorted order
-	__shared__ int s_last;
 	const int r = blockIdx.y;
 	if (!V.rmode[r]) return;
 	const RootGrid& g = V.g;
@@ -698,17 +715,38 @@ __global__ __launch_bounds__(REGION_THREADS) void k_rup(RootView V, const double
 		__syncthreads();
 		cofs = pofs;
 	}
-	// the row's last workgroup to finish forms the levels above rlev from
-	// every region's cell at rlev, then resets the row's ticket
-	if (threadIdx.x == 0)
+}
+
+// One workgroup per row between k_rup and k_rdown: the multipoles of the
+// cells above the regions (levels rlev - 1 .. 0, M2M from their children),
+// then the local expansion of every cell of levels 0 .. rlev: L2L from its
+// parent (levels >= 1) and M2L from its interaction list (at level 0 every
+// top cell that is not a neighbour), tpt threads per cell. They go to lcq
+// (the row's coarse cells in the grid's cell order), where each region of
+// k_rdown finds its own cell's; before, every region formed this chain of
+// ancestors itself.
+constexpr int COARSE_THREADS = 256;
+
+template <int P>
+__global__ __launch_bounds__(COARSE_THREADS) void k_rcoarse(RootView V, unsigned long long* __restrict__ stats)
+{
+	constexpr int NT = COARSE_THREADS, NC = P + 1;
+	__shared__ double binom[NC][NC];
+	const int r = blockIdx.x;
+	if (!V.rmode[r]) return;
+	const RootGrid& g = V.g;
+	const int Q = g.rlev;
+	const size_t rowc = (size_t)r * g.ncells, rowq = (size_t)r * coarse_cells(g);
+	const int* __restrict__ cc = V.ccnt + rowc;
+	// C(n, k) for n, k <= P as in k_rup
+	for (int i = threadIdx.x; i < NC * NC; i += NT)
 	{
-		__threadfence();
-		s_last = atomicAdd(V.rticket + r, 1) == nreg - 1;
+		const int n = i / NC, k = i - n * NC;
+		double c = k <= n ? 1.0 : 0.0;
+		for (int t = 1; t <= k && k <= n; t++) c = c * (n - k + t) / t;
+		binom[n][k] = c;
 	}
 	__syncthreads();
-	if (!s_last) return;
-	__threadfence();
-	if (threadIdx.x == 0) V.rticket[r] = 0;
 	for (int l = Q - 1; l >= 0; l--)
 	{
 		const int nxl = g.nx << l, nyl = g.ny << l, nxc = 2 * nxl;
@@ -731,6 +769,75 @@ __global__ __launch_bounds__(REGION_THREADS) void k_rup(RootView V, const double
 		}
 		__syncthreads();
 	}
+	unsigned long long nm = 0;
+	for (int l = 0; l <= Q; l++)
+	{
+		const int nxl = g.nx << l, nyl = g.ny << l, T = nxl * nyl;
+		int tpt = 32;
+		while (tpt > 1 && T * tpt > NT) tpt >>= 1;
+		const double ih = 2.0 / ldexp(g.s0, -l);
+		for (int t0 = 0; t0 < T; t0 += NT / tpt)
+		{
+			const int t = t0 + (int)threadIdx.x / tpt, q = (int)threadIdx.x % tpt;
+			const int ix = t % nxl, iy = t / nxl;
+			const bool live = t < T && cc[g.off[l] + t] > 0;
+			cd Lc[NC];
+#pragma unroll
+			for (int k = 0; k < NC; k++) Lc[k] = cd(0, 0);
+			if (live)
+			{
+				if (l >= 1 && q == 0)
+				{
+					const double2* lp = V.lcq + (rowq + g.off[l - 1] + (size_t)(iy >> 1) * (nxl >> 1) + (ix >> 1)) * NC;
+#pragma unroll
+					for (int n = 0; n < NC; n++) Lc[n] = cd(lp[n].x, lp[n].y);
+					l2l_shift<P>(Lc, cd((ix & 1) - 0.5, (iy & 1) - 0.5));
+				}
+				if (l == 0)
+				{
+					for (int k = q; k < T; k += tpt)
+					{
+						const int jx = k % nxl, jy = k / nxl;
+						if ((abs(jx - ix) >= 2 || abs(jy - iy) >= 2) && cc[k] > 0)
+						{
+							nm++;
+							m2l_add<P>(V.mp + (rowc + k) * NC, ix - jx, iy - jy, ih, Lc);
+						}
+					}
+				}
+				else
+				{
+					const int o = (ix & 1) + 2 * (iy & 1), bx = 2 * (ix >> 1) - 2, by = 2 * (iy >> 1) - 2;
+					for (int k = q; k < 27; k += tpt)
+					{
+						const int v = kFarList.v[o][k];
+						const int jx = bx + v % 6, jy = by + v / 6;
+						if (jx < 0 || jy < 0 || jx >= nxl || jy >= nyl) continue;
+						const int c = g.off[l] + jy * nxl + jx;
+						if (cc[c] <= 0) continue;
+						nm++;
+						m2l_add<P>(V.mp + (rowc + c) * NC, ix - jx, iy - jy, ih, Lc);
+					}
+				}
+			}
+			for (int d = 1; d < tpt; d <<= 1)
+#pragma unroll
+				for (int k = 0; k < NC; k++)
+				{
+					Lc[k].re += __shfl_xor(Lc[k].re, d, 64);
+					Lc[k].im += __shfl_xor(Lc[k].im, d, 64);
+				}
+			if (live)
+#pragma unroll
+				for (int m = 0; m < NC; m++)
+					if ((m & (tpt - 1)) == q) V.lcq[(rowq + g.off[l] + t) * NC + m] = make_double2(Lc[m].re, Lc[m].im);
+		}
+		// the level's expansions before the next level's L2L reads them
+		__syncthreads();
+	}
+	const int which[1] = {C_RM2L};
+	unsigned long long v[1] = {nm};
+	block_stats<1, COARSE_THREADS>(stats, which, v);
 }
 
 constexpr int HALO_SIDE = REGION_SIDE + 2;        // a region's leaf cells and their neighbours
@@ -768,14 +875,16 @@ __global__ __launch_bounds__(REGION_THREADS) void k_rdown(RootView V, const doub
 	const int* __restrict__ cc = V.ccnt + rowc;
 	const int* __restrict__ st = V.cstart + (size_t)r * (g.nleaf + 1);
 	for (int i = threadIdx.x; i <= RC; i += NT) cst[i] = st[(size_t)rg * RC + i];
+	// the region's own cell (level rlev): its local expansion from k_rcoarse
+	if ((int)threadIdx.x < NC)
+		lbuf[Q & 1][0][threadIdx.x] = V.lcq[((size_t)r * coarse_cells(g) + g.off[Q] + (size_t)ry * nxq + rx) * NC + threadIdx.x];
 	unsigned long long nm = 0;
-	for (int l = 0; l <= L; l++)
+	for (int l = Q + 1; l <= L; l++)
 	{
-		// the level's targets: the region's ancestor (l <= rlev) or its
-		// side x side cells, tpt threads per target
+		// the level's targets: the region's side x side cells, tpt threads
+		// per target
 		const int nxl = g.nx << l, nyl = g.ny << l, lofs = g.off[l];
-		const int side = l <= Q ? 1 : 1 << (l - Q);
-		const int x0 = l <= Q ? rx >> (Q - l) : rx << (l - Q), y0 = l <= Q ? ry >> (Q - l) : ry << (l - Q);
+		const int side = 1 << (l - Q), x0 = rx << (l - Q), y0 = ry << (l - Q);
 		const int T = side * side, tpt = min(32, NT / T);
 		const int t = threadIdx.x / tpt, q = threadIdx.x - t * tpt;
 		const int ix = x0 + t % side, iy = y0 + t / side;
@@ -786,94 +895,61 @@ __global__ __launch_bounds__(REGION_THREADS) void k_rdown(RootView V, const doub
 		cd Lc[NC];
 #pragma unroll
 		for (int k = 0; k < NC; k++) Lc[k] = cd(0, 0);
-		// the sources, staged in LDS: at level 0 every top cell (W * W at a
-		// time); below, the block of cells that holds the targets' far lists,
-		// for side >= 2 (x0, y0 even) cells x0 - 2 .. x0 + side + 1, for a
-		// single target the 6 x 6 block of its parent's neighbourhood
-		const int bx0 = l == 0 ? 0 : (side >= 2 ? x0 - 2 : 2 * (x0 >> 1) - 2);
-		const int by0 = l == 0 ? 0 : (side >= 2 ? y0 - 2 : 2 * (y0 >> 1) - 2);
-		const int Wl = l == 0 ? nxl : max(side + 4, 6);
-		const int ncand = l == 0 ? nxl * nyl : Wl * Wl;
-		for (int c0 = 0; c0 < ncand; c0 += W * W)
+		// the sources, staged in LDS: the block of cells that holds the
+		// targets' far lists, cells x0 - 2 .. x0 + side + 1 (x0, y0 even)
+		const int bx0 = x0 - 2, by0 = y0 - 2, Wl = side + 4, ne = Wl * Wl * NC;
+		for (int i0 = threadIdx.x; i0 < ne; i0 += NT * SB)
 		{
-			const int nch = min(W * W, ncand - c0), ne = nch * NC;
-			for (int i0 = threadIdx.x; i0 < ne; i0 += NT * SB)
+			double2 v[SB];
+			int n0[SB];
+#pragma unroll
+			for (int b = 0; b < SB; b++)
 			{
-				double2 v[SB];
-				int n0[SB];
+				const int i = i0 + b * NT, c = i / NC, n = i - c * NC;
+				const int jx = bx0 + c % Wl, jy = by0 + c / Wl;
+				const bool in = i < ne && jx >= 0 && jy >= 0 && jx < nxl && jy < nyl;
+				const size_t cid = lofs + (size_t)jy * nxl + jx;
+				v[b] = in ? V.mp[(rowc + cid) * NC + n] : make_double2(0, 0);
+				n0[b] = in && n == 0 ? cc[cid] : 0;
+			}
 #pragma unroll
-				for (int b = 0; b < SB; b++)
+			for (int b = 0; b < SB; b++)
+			{
+				const int i = i0 + b * NT, c = i / NC;
+				if (i < ne)
 				{
-					const int i = i0 + b * NT, c = i / NC, n = i - c * NC, gi = c0 + c;
-					const int jx = bx0 + gi % Wl, jy = by0 + gi / Wl;
-					const bool in = i < ne && jx >= 0 && jy >= 0 && jx < nxl && jy < nyl;
-					const size_t cid = lofs + (size_t)jy * nxl + jx;
-					v[b] = in ? V.mp[(rowc + cid) * NC + n] : make_double2(0, 0);
-					n0[b] = in && n == 0 ? cc[cid] : 0;
-				}
-#pragma unroll
-				for (int b = 0; b < SB; b++)
-				{
-					const int i = i0 + b * NT, c = i / NC;
-					if (i < ne)
-					{
-						src[i] = v[b];
-						if (i == c * NC) sok[c] = n0[b] > 0;
-					}
+					src[i] = v[b];
+					if (i == c * NC) sok[c] = n0[b] > 0;
 				}
 			}
-			__syncthreads();
-			if (live && l >= 1 && q == 0 && c0 == 0)
+		}
+		__syncthreads();
+		if (live)
+		{
+			// L2L into the group's first lane (the parents' expansions were
+			// written before the barrier above), then the M2L partials
+			if (q == 0)
 			{
-				// L2L into the group's first lane (the parents' expansions
-				// were written before the barrier above): 2^-m sum_{n >= m}
-				// C(n, m) L_n(parent) e^(n - m), the parent's polynomial
-				// shifted by e (repeated synthetic division)
-				const int ps = max(side >> 1, 1);
+				const int ps = side >> 1;
 				const int pi = ((iy >> 1) - (y0 >> 1)) * ps + ((ix >> 1) - (x0 >> 1));
 #pragma unroll
 				for (int n = 0; n < NC; n++) Lc[n] = cd(lpar[pi][n].x, lpar[pi][n].y);
-				const cd e((ix & 1) - 0.5, (iy & 1) - 0.5);
-#pragma unroll
-				for (int i = 0; i < P; i++)
-#pragma unroll
-					for (int j = P - 1; j >= i; j--) Lc[j] += e * Lc[j + 1];
-#pragma unroll
-				for (int m = 0; m < NC; m++) Lc[m] *= ldexp(1.0, -m);
+				l2l_shift<P>(Lc, cd((ix & 1) - 0.5, (iy & 1) - 0.5));
 			}
-			if (live)
+			// entry v of the far list is cell (v % 6, v / 6) of the 6 x 6
+			// block whose first cell is (2 (ix / 2) - 2, 2 (iy / 2) - 2)
+			const int o = (ix & 1) + 2 * (iy & 1), bx = 2 * (ix >> 1) - 2, by = 2 * (iy >> 1) - 2;
+			for (int k = q; k < 27; k += tpt)
 			{
-				if (l == 0)
-				{
-					// every top cell that is not a neighbour
-					for (int k = q; k < nch; k += tpt)
-					{
-						const int jx = (c0 + k) % nxl, jy = (c0 + k) / nxl;
-						if ((abs(jx - ix) >= 2 || abs(jy - iy) >= 2) && sok[k])
-						{
-							nm++;
-							m2l_add<P>(src + k * NC, ix - jx, iy - jy, ih, Lc);
-						}
-					}
-				}
-				else
-				{
-					// entry v of the far list is cell (v % 6, v / 6) of the 6 x 6
-					// block whose first cell is (2 (ix / 2) - 2, 2 (iy / 2) - 2)
-					const int o = (ix & 1) + 2 * (iy & 1), bx = 2 * (ix >> 1) - 2, by = 2 * (iy >> 1) - 2;
-					for (int k = q; k < 27; k += tpt)
-					{
-						const int v = kFarList.v[o][k];
-						const int jx = bx + v % 6, jy = by + v / 6;
-						const int c = (jy - by0) * Wl + (jx - bx0);
-						if (!sok[c]) continue;
-						nm++;
-						m2l_add<P>(src + c * NC, ix - jx, iy - jy, ih, Lc);
-					}
-				}
+				const int v = kFarList.v[o][k];
+				const int jx = bx + v % 6, jy = by + v / 6;
+				const int c = (jy - by0) * Wl + (jx - bx0);
+				if (!sok[c]) continue;
+				nm++;
+				m2l_add<P>(src + c * NC, ix - jx, iy - jy, ih, Lc);
 			}
-			__syncthreads();
 		}
+		__syncthreads();
 		// the target's partials over its tpt threads (aligned groups of
 		// lanes; the butterfly leaves the same sum in each)
 		for (int d = 1; d < tpt; d <<= 1)
@@ -1001,6 +1077,7 @@ void root_fmm_pass_p(const SweepBufs& B, const double2* roots, const RowMap& M, 
 	}
 	const int nreg = (g.nx << g.rlev) * (g.ny << g.rlev);
 	k_rup<P><<<dim3(nreg + RSUM_OUT_BLOCKS, nrows), REGION_THREADS, 0, s>>>(V, roots, M, stats);
+	k_rcoarse<P><<<nrows, COARSE_THREADS, 0, s>>>(V, stats);
 	k_rdown<P><<<dim3(nreg, nrows), REGION_THREADS, 0, s>>>(V, roots, M, stats);
 }
 

@@ -11,7 +11,7 @@
 //                  (near-field P2P + far-field L2P + smooth sheet) and the
 //                  1/mu early exit; roots that need the Cauchy sums are
 //                  appended to a per-row list (the few roots outside the
-//                  tree by their whole workgroup); its last workgroup runs
+//                  tree by their whole wave); its last workgroup runs
 //   K2 plan_sums   cuts the (stars + roots) term range of every tile of
 //                  RPT*256 listed roots into S slices so that dense and sparse
 //                  sweeps alike give enough equal work items to fill the chip;
@@ -1811,7 +1811,7 @@ bool root_layout(ccf_ctx* ctx, SweepBufs& b, Take&& take)
 	return take(b.slot, list) && take(b.rmode, (size_t)rows) && take(b.rsum, list) && take(b.ccnt, (size_t)rows * g.ncells) &&
 		take(b.cstart, (size_t)rows * (g.nleaf + 1)) && take(b.ccur, (size_t)rows * g.nleaf) && take(b.cell, list) &&
 		take(b.bidx, list) && take(b.sidx, list) && take(b.spos, list) && take(b.ocnt, (size_t)rows) && take(b.oidx, list) &&
-		take(b.mp, ce) && take(b.rticket, (size_t)rows);
+		take(b.mp, ce) && take(b.lcq, (size_t)rows * coarse_cells(g) * (g.order + 1));
 }
 
 // the root-sum FMM's arrays alone (ccf_root_sums), from the context's pool
@@ -1821,7 +1821,6 @@ bool alloc_root_fmm(ccf_ctx* ctx, SweepBufs& b)
 	return root_layout(ctx, b, take) &&
 		CCF_HIP(ctx, hipMemsetAsync(b.slot, 0xff, (size_t)b.rows * b.R * sizeof(int), ctx->stream)) &&
 		CCF_HIP(ctx, hipMemsetAsync(b.rmode, 0, b.rows * sizeof(int), ctx->stream)) &&
-		CCF_HIP(ctx, hipMemsetAsync(b.rticket, 0, b.rows * sizeof(int), ctx->stream)) &&
 		CCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
 }
 
@@ -1833,7 +1832,7 @@ void free_sweep(ccf_ctx* ctx, SweepBufs& b)
 	F(b.act); F(b.cnt); F(b.fl); F(b.fcnt); F(b.f0); F(b.f1); F(b.ex); F(b.tile_off); F(b.plan); F(b.ticket);
 	F(b.part);
 	F(b.slot); F(b.rmode); F(b.rsum); F(b.ccnt); F(b.cstart); F(b.ccur); F(b.cell); F(b.bidx); F(b.sidx); F(b.spos);
-	F(b.ocnt); F(b.oidx); F(b.mp); F(b.rticket);
+	F(b.ocnt); F(b.oidx); F(b.mp); F(b.lcq);
 	b = SweepBufs{};
 }
 
@@ -1850,7 +1849,6 @@ bool alloc_sweep(ccf_ctx* ctx, SweepBufs& b, int rows, int R)
 	if (!b.slot) return true;
 	return CCF_HIP(ctx, hipMemsetAsync(b.slot, 0xff, list * sizeof(int), ctx->stream)) &&
 		CCF_HIP(ctx, hipMemsetAsync(b.rmode, 0, rows * sizeof(int), ctx->stream)) &&
-		CCF_HIP(ctx, hipMemsetAsync(b.rticket, 0, rows * sizeof(int), ctx->stream)) &&
 		CCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
 }
 
@@ -2533,11 +2531,10 @@ bool aberth_sweep_noalloc_dev(ccf_ctx* ctx, const double* roots, int n, double p
 	if (!sweep_layout(ctx, B, 1, n, true, carve)) return false;
 	hipStream_t s = ctx->stream;
 	// the Witt kernel's ticket starts at 0, the slots of listed roots at -1,
-	// the row's FMM mode and its region ticket at 0; k_prepare sets the
-	// active list and clears the counts
+	// the row's FMM mode at 0; k_prepare sets the active list and clears the
+	// counts
 	if (!fill_int(ctx, B.ticket, 1, 0)) return false;
-	if (B.slot && !(fill_int(ctx, B.slot, (size_t)n, -1) && fill_int(ctx, B.rmode, 1, 0) && fill_int(ctx, B.rticket, 1, 0)))
-		return false;
+	if (B.slot && !(fill_int(ctx, B.slot, (size_t)n, -1) && fill_int(ctx, B.rmode, 1, 0))) return false;
 	if (!CCF_HIP(ctx, hipMemcpyAsync(row, roots, (size_t)n * sizeof(double2), hipMemcpyHostToDevice, s))) return false;
 	const RowMap M{n, 2, 1, 0, 0, phi};
 	k_prepare<<<dim3(grid(n), 1), BLOCK, 0, s>>>(row, M, B, 0, nullptr);
