@@ -725,12 +725,17 @@ __global__ __launch_bounds__(REGION_THREADS) void k_rup(RootView V, const double
 // (the row's coarse cells in the grid's cell order), where each region of
 // k_rdown finds its own cell's; before, every region formed this chain of
 // ancestors itself.
-constexpr int COARSE_THREADS = 256;
+// (512 threads while the registers of an order allow two waves a SIMD)
+template <int P>
+constexpr int coarse_threads()
+{
+	return P <= 16 ? 512 : 256;
+}
 
 template <int P>
-__global__ __launch_bounds__(COARSE_THREADS) void k_rcoarse(RootView V, unsigned long long* __restrict__ stats)
+__global__ __launch_bounds__(coarse_threads<P>()) void k_rcoarse(RootView V, unsigned long long* __restrict__ stats)
 {
-	constexpr int NT = COARSE_THREADS, NC = P + 1;
+	constexpr int NT = coarse_threads<P>(), NC = P + 1;
 	__shared__ double binom[NC][NC];
 	const int r = blockIdx.x;
 	if (!V.rmode[r]) return;
@@ -837,7 +842,7 @@ __global__ __launch_bounds__(COARSE_THREADS) void k_rcoarse(RootView V, unsigned
 	}
 	const int which[1] = {C_RM2L};
 	unsigned long long v[1] = {nm};
-	block_stats<1, COARSE_THREADS>(stats, which, v);
+	block_stats<1, NT>(stats, which, v);
 }
 
 constexpr int HALO_SIDE = REGION_SIDE + 2;        // a region's leaf cells and their neighbours
@@ -1077,7 +1082,7 @@ void root_fmm_pass_p(const SweepBufs& B, const double2* roots, const RowMap& M, 
 	}
 	const int nreg = (g.nx << g.rlev) * (g.ny << g.rlev);
 	k_rup<P><<<dim3(nreg + RSUM_OUT_BLOCKS, nrows), REGION_THREADS, 0, s>>>(V, roots, M, stats);
-	k_rcoarse<P><<<nrows, COARSE_THREADS, 0, s>>>(V, stats);
+	k_rcoarse<P><<<nrows, coarse_threads<P>(), 0, s>>>(V, stats);
 	k_rdown<P><<<dim3(nreg, nrows), REGION_THREADS, 0, s>>>(V, roots, M, stats);
 }
 
