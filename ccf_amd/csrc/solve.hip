@@ -684,26 +684,27 @@ __global__ __launch_bounds__(BLOCK) void k_sums64(const double2* __restrict__ sp
 
 // K3 in FP32 with the FP64 fallback for clustered roots
 // The items of tiles with few listed roots (at most SPARSE_TARGETS), in the
-// same launch as the dense items (a branch uniform per item): the lanes run
-// over the slice's sources and
-// groups of SPARSE_GROUP targets are held uniform across the workgroup, so
-// no lane idles on an empty target slot. Each target's partial sum is
-// reduced over the lanes in a fixed order and lands where the tile's own
-// lane would have written it (part[i] for target i of the tile). A root
-// within the cluster distance has its one term formed in FP64; the target's
-// own index is skipped by index, not by distance.
+// same launch as the dense items (a branch uniform per item): each wave
+// takes groups of SPARSE_GROUP targets (wave w the groups w, w + waves, ...),
+// its lanes run over the slice's sources, SPARSE_LOADS at a time, and a
+// butterfly of fixed order over the wave gives each target's partial sum,
+// written where the tile's own lane would have written it (part[i] for
+// target i of the tile). A root within the cluster distance has its one term
+// formed in FP64; the target's own index is skipped by index, not by
+// distance.
 constexpr int SPARSE_TARGETS = 32;
 constexpr int SPARSE_GROUP = 4;
+constexpr int SPARSE_LOADS = 2;
 
 __device__ inline void sums32_sparse(const float2* __restrict__ spos32, int Np, const double2* __restrict__ xrow,
 	const int* __restrict__ fl, int nt, long long lo, long long hi, float cluster_d2, double2* __restrict__ part,
 	unsigned long long& nclus)
 {
-	__shared__ double red[BLOCK / 64][SPARSE_GROUP][2];
 	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 	const int s1 = (int)min(hi, (long long)Np);
 	const int r0 = (int)(max(lo, (long long)Np) - Np), r1 = (int)(hi - Np);
-	for (int g0 = 0; g0 < nt; g0 += SPARSE_GROUP)
+#pragma unroll 1
+	for (int g0 = wv * SPARSE_GROUP; g0 < nt; g0 += (BLOCK / 64) * SPARSE_GROUP)
 	{
 		int ia[SPARSE_GROUP];
 		double cx[SPARSE_GROUP], cy[SPARSE_GROUP];
@@ -720,43 +721,60 @@ __device__ inline void sums32_sparse(const float2* __restrict__ spos32, int Np, 
 			cx[g] = cy[g] = 0;
 		}
 #pragma unroll 1
-		for (int q = (int)lo + threadIdx.x; q < s1; q += BLOCK)
+		for (int q0 = (int)lo + lane; q0 < s1; q0 += 64 * SPARSE_LOADS)
 		{
-			const float2 w = spos32[q];
+			float2 w[SPARSE_LOADS];
 #pragma unroll
-			for (int g = 0; g < SPARSE_GROUP; g++)
-			{
-				const float dx = fx[g] - w.x, dy = fy[g] - w.y;
-				const float iv = __builtin_amdgcn_rcpf(fmaf(dx, dx, dy * dy));
-				ax[g] = fmaf(dx, iv, ax[g]);
-				ay[g] = fmaf(dy, iv, ay[g]);
-			}
+			for (int u = 0; u < SPARSE_LOADS; u++) w[u] = spos32[min(q0 + 64 * u, s1 - 1)];
+#pragma unroll
+			for (int u = 0; u < SPARSE_LOADS; u++)
+				if (q0 + 64 * u < s1)
+#pragma unroll
+					for (int g = 0; g < SPARSE_GROUP; g++)
+					{
+						const float dx = fx[g] - w[u].x, dy = fy[g] - w[u].y;
+						const float iv = __builtin_amdgcn_rcpf(fmaf(dx, dx, dy * dy));
+						ax[g] = fmaf(dx, iv, ax[g]);
+						ay[g] = fmaf(dy, iv, ay[g]);
+					}
 		}
 #pragma unroll 1
-		for (int q = r0 + threadIdx.x; q < r1; q += BLOCK)
+		for (int q0 = r0 + lane; q0 < r1; q0 += 64 * SPARSE_LOADS)
 		{
-			const double2 wd = xrow[q];
-			const float wx = (float)wd.x, wy = (float)wd.y;
+			float2 w[SPARSE_LOADS];
 #pragma unroll
-			for (int g = 0; g < SPARSE_GROUP; g++)
+			for (int u = 0; u < SPARSE_LOADS; u++)
 			{
-				const float dx = fx[g] - wx, dy = fy[g] - wy;
-				const float d2 = fmaf(dx, dx, dy * dy);
-				if (d2 < cluster_d2 && q != ia[g])
+				const double2 v = xrow[min(q0 + 64 * u, r1 - 1)];
+				w[u] = make_float2((float)v.x, (float)v.y);
+			}
+#pragma unroll
+			for (int u = 0; u < SPARSE_LOADS; u++)
+			{
+				const int q = q0 + 64 * u;
+				if (q >= r1) continue;
+#pragma unroll
+				for (int g = 0; g < SPARSE_GROUP; g++)
 				{
-					const double2 z = xrow[ia[g]]; // rare: the target in FP64 again
-					const double ex = z.x - wd.x, ey = z.y - wd.y;
-					const double iv = rcp64(fmax(fma(ex, ex, ey * ey), 1e-300));
-					cx[g] = fma(ex, iv, cx[g]);
-					cy[g] = fma(ey, iv, cy[g]);
-					nclus++;
-				}
-				else
-				{
-					// the own root gives dx = dy = 0 and, floored, a zero term
-					const float iv = __builtin_amdgcn_rcpf(fmaxf(d2, 1e-30f));
-					bx[g] = fmaf(dx, iv, bx[g]);
-					by[g] = fmaf(dy, iv, by[g]);
+					const float dx = fx[g] - w[u].x, dy = fy[g] - w[u].y;
+					const float d2 = fmaf(dx, dx, dy * dy);
+					if (d2 < cluster_d2 && q != ia[g])
+					{
+						// rare: the source and the target in FP64 again
+						const double2 z = xrow[ia[g]], wd = xrow[q];
+						const double ex = z.x - wd.x, ey = z.y - wd.y;
+						const double iv = rcp64(fmax(fma(ex, ex, ey * ey), 1e-300));
+						cx[g] = fma(ex, iv, cx[g]);
+						cy[g] = fma(ey, iv, cy[g]);
+						nclus++;
+					}
+					else
+					{
+						// the own root gives dx = dy = 0 and, floored, a zero term
+						const float iv = __builtin_amdgcn_rcpf(fmaxf(d2, 1e-30f));
+						bx[g] = fmaf(dx, iv, bx[g]);
+						by[g] = fmaf(dy, iv, by[g]);
+					}
 				}
 			}
 		}
@@ -770,24 +788,8 @@ __device__ inline void sums32_sparse(const float2* __restrict__ spos32, int Np, 
 				vx += __shfl_xor(vx, o, 64);
 				vy += __shfl_xor(vy, o, 64);
 			}
-			if (lane == 0)
-			{
-				red[wv][g][0] = vx;
-				red[wv][g][1] = vy;
-			}
+			if (lane == 0 && g0 + g < nt) part[g0 + g] = make_double2(vx, -vy);
 		}
-		__syncthreads();
-		if (threadIdx.x < SPARSE_GROUP && g0 + (int)threadIdx.x < nt)
-		{
-			double vx = 0, vy = 0;
-			for (int w = 0; w < BLOCK / 64; w++)
-			{
-				vx += red[w][threadIdx.x][0];
-				vy += red[w][threadIdx.x][1];
-			}
-			part[g0 + threadIdx.x] = make_double2(vx, -vy);
-		}
-		__syncthreads();
 	}
 }
 
