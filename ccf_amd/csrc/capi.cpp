@@ -698,6 +698,8 @@ void ccf_destroy(ccf_ctx* c)
 	for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
 	for (hipEvent_t e : c->fmm_rows_event)
 		if (e) (void)hipEventDestroy(e);
+	for (hipEvent_t e : c->read_event)
+		if (e) (void)hipEventDestroy(e);
 	if (c->err_event) (void)hipEventDestroy(c->err_event);
 	if (c->err_stream) (void)hipStreamDestroy(c->err_stream);
 	if (c->group_event) (void)hipEventDestroy(c->group_event);

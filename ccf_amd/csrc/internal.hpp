@@ -242,6 +242,7 @@ struct ccf_ctx
 	hipStream_t err_stream = nullptr; // the error pass of find_ccs, beside the continuation (default priority)
 	hipEvent_t err_event = nullptr;   // a row set's roots are final (no timing)
 	hipEvent_t fmm_rows_event[2 * ccf::CONT_GROUPS] = {}; // mixed continuation: after the copy of a step's FMM row count, per group (no timing)
+	hipEvent_t read_event[ccf::CONT_GROUPS] = {}; // after the copy of a sweep's active counts, per group (no timing)
 	hipStream_t group_streams[ccf::CONT_GROUPS - 1] = {}; // mixed form: the streams of the groups of branches after the first (default priority)
 	hipEvent_t group_event = nullptr;   // orders the group streams around the continuation (no timing)
 	ccf::SweepBufs swg[ccf::CONT_GROUPS - 1]; // mixed form: the sweep buffers of the groups after the first

@@ -1573,7 +1573,42 @@ bool run_sweeps_groups(ccf_ctx* ctx, SweepGroup* gs, int G, double2* roots, int 
 		rows_total += gs[g].nrows;
 	}
 	const bool readback = ctx->host_counts && rows_total <= ctx->max_rows;
-	int swept = 0;
+	int swept = 0, pend = -1, ended = -1;
+	// the counts read back after sweep it (pend): groups with no active root
+	// stop, and the FMM pass stops once no row has as many active roots as it
+	// takes
+	const auto take_counts = [&](int it) -> bool {
+		for (int g = 0; g < G; g++)
+			if (!gs[g].done && !CCF_HIP(ctx, hipEventSynchronize(ctx->read_event[gs[g].id]))) return false;
+		for (int g = 0; g < G; g++)
+		{
+			SweepGroup& q = gs[g];
+			if (q.done) continue;
+			long long active = 0;
+			int amax = 0;
+			for (int r = 0; r < q.nrows; r++)
+			{
+				active += ctx->host_counts[q.row0 + r];
+				amax = std::max(amax, ctx->host_counts[q.row0 + r]);
+			}
+			// a row's listed roots are among its active ones
+			if (q.B->grid.order > 0 && amax < root_fmm_min(q.M.R)) q.fmm_mask = 0;
+			if (q.id == 0)
+			{
+				ctx->fmm_rows_sweep[0] = fmm_read_slot(ctx, 0)[0];
+				ctx->fmm_rows_sweep[1] = fmm_read_slot(ctx, 0)[1];
+			}
+			if (ctx->sweep_verbose >= 3)
+				std::cout << "sweep " << it + 1 << (G > 1 ? ", group " + std::to_string(g) : std::string()) << ": " << active
+						  << " roots active, at most " << amax << " in a row\n";
+			if (active == 0)
+			{
+				q.done = true;
+				if (--left == 0) ended = it + 1;
+			}
+		}
+		return true;
+	};
 	for (int it = 0; it < cap && left > 0; it++)
 	{
 		swept = it + 1;
@@ -1687,8 +1722,18 @@ bool run_sweeps_groups(ccf_ctx* ctx, SweepGroup* gs, int G, double2* roots, int 
 			}
 		}
 		// stop the groups whose rows have no active root: the remaining
-		// sweeps up to the cap would do nothing (checked at a few sweeps only,
-		// each check waits for the device)
+		// sweeps up to the cap would do nothing (checked at a few sweeps
+		// only). A check's counts come back while the next sweep runs: they
+		// are copied after their sweep, and waited for once the next sweep is
+		// queued, so the device does not idle on the host's decision. The
+		// sweep queued meanwhile after a group's last active root finds none
+		// and changes nothing; a cleared FMM flag takes effect a sweep later,
+		// when no row lists enough roots for the pass any more either.
+		if (pend >= 0)
+		{
+			if (!take_counts(pend)) return false;
+			pend = -1;
+		}
 		if (left > 0 && readback && (ctx->sweep_verbose >= 3 || sweep_check(it + 1, cap, first_check)))
 		{
 			for (int g = 0; g < G; g++)
@@ -1696,43 +1741,18 @@ bool run_sweeps_groups(ccf_ctx* ctx, SweepGroup* gs, int G, double2* roots, int 
 				SweepGroup& q = gs[g];
 				if (q.done) continue;
 				const SweepView V = view(*q.B, it & 1);
+				hipEvent_t& e = ctx->read_event[q.id];
+				if (!e && !CCF_HIP(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming))) return false;
 				if (!CCF_HIP(ctx, hipMemcpyAsync(ctx->host_counts + q.row0, V.cnt_next, q.nrows * sizeof(int), hipMemcpyDeviceToHost, q.s)) ||
-					!CCF_HIP(ctx, hipMemcpyAsync(fmm_read_slot(ctx, q.id), q.B->plan + 2, 2 * sizeof(int), hipMemcpyDeviceToHost, q.s)))
+					!CCF_HIP(ctx, hipMemcpyAsync(fmm_read_slot(ctx, q.id), q.B->plan + 2, 2 * sizeof(int), hipMemcpyDeviceToHost, q.s)) ||
+					!CCF_HIP(ctx, hipEventRecord(e, q.s)))
 					return false;
 			}
-			for (int g = 0; g < G; g++)
-				if (!gs[g].done && !CCF_HIP(ctx, hipStreamSynchronize(gs[g].s))) return false;
-			for (int g = 0; g < G; g++)
-			{
-				SweepGroup& q = gs[g];
-				if (q.done) continue;
-				long long active = 0;
-				int amax = 0;
-				for (int r = 0; r < q.nrows; r++)
-				{
-					active += ctx->host_counts[q.row0 + r];
-					amax = std::max(amax, ctx->host_counts[q.row0 + r]);
-				}
-				// a row's listed roots are among its active ones: once no row has
-				// as many as the FMM takes, the remaining sweeps skip its pass
-				if (q.B->grid.order > 0 && amax < root_fmm_min(q.M.R)) q.fmm_mask = 0;
-				if (q.id == 0)
-				{
-					ctx->fmm_rows_sweep[0] = fmm_read_slot(ctx, 0)[0];
-					ctx->fmm_rows_sweep[1] = fmm_read_slot(ctx, 0)[1];
-				}
-				if (ctx->sweep_verbose >= 3)
-					std::cout << "sweep " << it + 1 << (G > 1 ? ", group " + std::to_string(g) : std::string()) << ": " << active
-							  << " roots active, at most " << amax << " in a row\n";
-				if (active == 0)
-				{
-					q.done = true;
-					left--;
-				}
-			}
+			pend = it;
 		}
 	}
-	ctx->phase_sweeps = swept;
+	if (pend >= 0 && !take_counts(pend)) return false;
+	ctx->phase_sweeps = ended > 0 ? ended : swept;
 	return true;
 }
 
