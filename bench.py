@@ -59,29 +59,34 @@ def run_gpu(starfile: str, run_kw: dict, precision: int, steps: int, warmup: int
                           write_caustics=1, precision=precision, timing=0, predictor=predictor, root_fmm=root_fmm,
                           **(fmm_kw or {}), **run_kw)
     tune_s = 0.0
-    for i in range(warmup + steps):
-        if i == warmup + steps - 1:
-            # the last timed step records HIP events around every sweep's
-            # regions (the roofline's launch durations); the events cost
-            # about 4 ms of this one step
+    probe = None
+    for i in range(warmup + steps + 1):
+        # one step after the timed ones records HIP events around every
+        # sweep's regions (the roofline's launch durations); the events cost
+        # about 4 ms of that step, which is therefore not a timed step
+        events = i == warmup + steps
+        if events:
             ctx.set(timing=1)
         ctx.stage("load")
         t0 = time.perf_counter()
         for st in ("build_tree_fmm", "find_initial_roots", "find_ccs", "find_caustics"):
             ctx.stage(st)
         dt = time.perf_counter() - t0
-        print(f"# precision={precision} predictor={predictor} root_fmm={root_fmm} step {i} ({'warmup' if i < warmup else 'timed'}): "
+        kind = "warmup" if i < warmup else ("event-timed, not timed" if events else "timed")
+        print(f"# precision={precision} predictor={predictor} root_fmm={root_fmm} step {i} ({kind}): "
               f"{dt:.3f} s", file=sys.stderr, flush=True)
         inf = ctx.info()
         tune_s += inf["t_autotune"]
-        if i >= warmup:
+        if events:
+            probe = inf
+        elif i >= warmup:
             times.append(dt)
             infos.append(inf)
     ccs = ctx.array(ccf_amd.ARRAY_CCS)
     ctx.close()
     # (params.autotune: the first warmup step tunes; the timed steps reuse it)
     infos[-1]["t_autotune_first_step"] = tune_s
-    return times, infos, ccs
+    return times, infos, ccs, probe
 
 
 def set_error(a: np.ndarray, b: np.ndarray, R: int, nb: int, theta: float = 1.0):
@@ -248,15 +253,17 @@ def measured_traffic(workload: str, kernel: str):
     return None
 
 
-def summarize(times, infos):
+def summarize(times, infos, probe):
+    """times and infos: the timed steps; probe: the info of the event-timed
+    step after them (launch durations of the roofline's regions)"""
     inf = infos[-1]
     t = float(np.mean(times))
     conv = inf["points_converged"]
-    k_ms = inf["aberth_kernel_ms"]
-    launches = max(1, inf["aberth_launches"])
-    achieved = inf["flops_sums"] / (k_ms * 1e-3) / 1e12 if k_ms > 0 else 0.0
-    w_ms = inf["witt_kernel_ms"]
-    w_launches = max(1, inf["witt_launches"])
+    k_ms = probe["aberth_kernel_ms"]
+    launches = max(1, probe["aberth_launches"])
+    achieved = probe["flops_sums"] / (k_ms * 1e-3) / 1e12 if k_ms > 0 else 0.0
+    w_ms = probe["witt_kernel_ms"]
+    w_launches = max(1, probe["witt_launches"])
     return dict(points_per_s=conv / t, s_per_step=t, points_converged=conv, points_total=inf["points_total"],
                 t_tree=inf["t_tree"], t_init_roots=inf["t_init_roots"], t_ccs=inf["t_ccs"], t_caustics=inf["t_caustics"],
                 updates_full=inf["updates_full"], updates_early=inf["updates_early"], updates_newton=inf["updates_newton"],
@@ -265,12 +272,13 @@ def summarize(times, infos):
                 root_fmm_order=inf["root_fmm_order"], root_fmm_leaf_level=inf["root_fmm_leaf_level"],
                 cluster_chunks=inf["cluster_chunks"],
                 aberth_ms_total=k_ms, aberth_launches=launches, aberth_ms_avg=k_ms / launches,
-                sums_tflops=achieved, sums_flops=inf["flops_sums"], update_flops=inf["flops_update"],
+                sums_tflops=achieved, sums_flops=probe["flops_sums"], update_flops=inf["flops_update"],
                 witt_ms_total=w_ms, witt_launches=w_launches, witt_ms_avg=w_ms / w_launches,
-                witt_tflops=inf["flops_witt"] / (w_ms * 1e-3) / 1e12 if w_ms > 0 else 0.0, witt_flops=inf["flops_witt"],
-                tail_launches=inf["tail_launches"], tail_sweeps=inf["tail_sweeps"], tail_ms_total=inf["tail_kernel_ms"],
-                tail_flops=inf["flops_tail"],
-                tail_tflops=inf["flops_tail"] / (inf["tail_kernel_ms"] * 1e-3) / 1e12 if inf["tail_kernel_ms"] > 0 else 0.0,
+                witt_tflops=probe["flops_witt"] / (w_ms * 1e-3) / 1e12 if w_ms > 0 else 0.0, witt_flops=probe["flops_witt"],
+                tail_launches=probe["tail_launches"], tail_sweeps=probe["tail_sweeps"], tail_ms_total=probe["tail_kernel_ms"],
+                tail_flops=probe["flops_tail"],
+                tail_tflops=(probe["flops_tail"] / (probe["tail_kernel_ms"] * 1e-3) / 1e12
+                             if probe["tail_kernel_ms"] > 0 else 0.0),
                 expansion_order=inf["expansion_order"], tree_levels=inf["tree_levels"], num_roots=inf["num_roots"],
                 max_error_1_over_mu=inf["max_error"], max_witt_residual=inf["max_witt_residual"],
                 roots_per_thread=inf["roots_per_thread"], root_fmm_leaf=inf["root_fmm_leaf"],
@@ -496,8 +504,8 @@ def main():
         fmm_kw = dict(root_fmm_order=args.root_fmm_order, root_fmm_leaf=args.root_fmm_leaf, autotune=args.autotune,
                       roots_per_thread=args.roots_per_thread)
         steps, warm = (args.steps, args.warmup) if (name, prec, pred, rfmm) in modes else (args.ablation_steps, 1)
-        times, infos, ccs[name] = run_gpu(sf, run_kw, prec, steps, warm, pred, rfmm, fmm_kw)
-        res[name] = summarize(times, infos)
+        times, infos, ccs[name], probe = run_gpu(sf, run_kw, prec, steps, warm, pred, rfmm, fmm_kw)
+        res[name] = summarize(times, infos, probe)
         res[name]["predictor"] = pred
         res[name]["root_fmm"] = rfmm
     R = res[next(iter(res))]["num_roots"]
@@ -553,7 +561,7 @@ def main():
             "traffic": measured_traffic(args.workload, dom["kernel"]),
             "traffic_source": f"profiles/{args.workload}_sums_hbm_traffic.md",
             "avg_launch_ms": dom["avg_launch_ms"], "launches": dom["launches"],
-            "timed_with": "HIP events on the context's stream around each sweep's regions, during the last timed step",
+            "timed_with": "HIP events on the context's stream around each sweep's regions, during one step run after the timed steps",
             "regions": [dict(g, frac=g["achieved"] / pk) for g in regs]}
     if "fp64" in res:
         roof["fp64_kernel"] = {"kernel": "k_sums64", "achieved": res["fp64"]["sums_tflops"], "peak": PEAK_FP64_TFLOPS,
