@@ -281,6 +281,7 @@ def summarize(times, infos, probe):
                              if probe["tail_kernel_ms"] > 0 else 0.0),
                 expansion_order=inf["expansion_order"], tree_levels=inf["tree_levels"], num_roots=inf["num_roots"],
                 max_error_1_over_mu=inf["max_error"], max_witt_residual=inf["max_witt_residual"],
+                points_outside_tree=inf["points_outside_tree"],
                 roots_per_thread=inf["roots_per_thread"], root_fmm_leaf=inf["root_fmm_leaf"],
                 autotune_s=inf.get("t_autotune_first_step", 0.0))
 

@@ -464,16 +464,31 @@ __device__ inline Near wave_near(const LensView& L, int node, cd z)
 	for (int k = 0; k < ranges; k++)
 	{
 		const int2 rg = L.nf[node * MAX_NEAR_RANGES + k];
-		for (int s = rg.x + lane; s < rg.x + rg.y; s += 64)
+		const int e = rg.x + rg.y;
+		// the lane's stars in order, two loads issued before their terms
+		for (int s0 = rg.x + lane; s0 < e; s0 += 2 * 64)
 		{
-			const double2 q = L.spos[s];
-			const cd t = cd(z.re - q.x, z.im - q.y).inv();
-			nr.s0 += t;
-			const cd t1 = t * L.smass[s];
-			nr.s1 += t1;
-			const cd t2 = t1 * t;
-			nr.s2 += t2;
-			nr.s3 += t2 * t;
+			double2 q[2];
+			double m[2];
+#pragma unroll
+			for (int u = 0; u < 2; u++)
+			{
+				const int s = min(s0 + u * 64, e - 1);
+				q[u] = L.spos[s];
+				m[u] = L.smass[s];
+			}
+#pragma unroll
+			for (int u = 0; u < 2; u++)
+				if (s0 + u * 64 < e)
+				{
+					const cd t = cd(z.re - q[u].x, z.im - q[u].y).inv();
+					nr.s0 += t;
+					const cd t1 = t * m[u];
+					nr.s1 += t1;
+					const cd t2 = t1 * t;
+					nr.s2 += t2;
+					nr.s3 += t2 * t;
+				}
 		}
 		nr.count += rg.y;
 	}
@@ -1954,7 +1969,6 @@ bool alloc_solver(ccf_ctx* ctx)
 		CCF_HIP(ctx, hipHostMalloc((void**)&ctx->host_counts, (ctx->max_rows + 6 * CONT_GROUPS) * sizeof(int), hipHostMallocDefault)) &&
 		CCF_HIP(ctx, ctx->dmalloc(&ctx->phase, (size_t)(np + nb) * sizeof(double2))) && launch_phase(ctx) &&
 		CCF_HIP(ctx, ctx->dmalloc(&ctx->counters, STAT_WORDS * sizeof(unsigned long long))) &&
-		CCF_HIP(ctx, hipMemsetAsync(ctx->ccs_init, 0, pts * sizeof(double2), ctx->stream)) &&
 		CCF_HIP(ctx, hipMemsetAsync(ctx->counters, 0, STAT_WORDS * sizeof(unsigned long long), ctx->stream));
 }
 
