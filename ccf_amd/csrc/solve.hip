@@ -124,7 +124,7 @@ __device__ inline unsigned long long* stat_slot(unsigned long long* stats, int c
 // adds v[k] of every thread of the block to counter which[k]; every thread
 // of a BS-sized workgroup calls this
 template <int NC, int BS = BLOCK>
-__device__ inline void block_stats(unsigned long long* stats, const int (&which)[NC], unsigned long long (&v)[NC])
+__device__ __forceinline__ void block_stats(unsigned long long* stats, const int (&which)[NC], unsigned long long (&v)[NC])
 {
 	__shared__ unsigned long long red[NC][BS / 64];
 	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -503,12 +503,77 @@ __device__ inline Near wave_near(const LensView& L, int node, cd z)
 	return nr;
 }
 
+// Counts of a workgroup's Witt evaluations (block_stats adds them to the
+// device counters)
+struct EvalCounts
+{
+	int full = 0, early = 0, newton = 0, conv = 0, nnear = 0, near_early = 0;
+};
+
+// One Witt evaluation of root a of the row xrow and its outcome
+// (witt_root), called by every lane of a wave (valid: the lane has a root).
+// A root outside the tree (whose near field is every star,
+// tree_node.cuh:529-561) takes its near-field sums from its whole wave
+// (wave_near), one such root at a time.
+template <bool MIXED>
+__device__ __forceinline__ int witt_eval(const LensView& L, double2* xrow, int a, bool valid, double cp, double sp, int circ_exact,
+	int fmm_pole, int newton, cd& f0, cd& f1, cd& ex, EvalCounts& c)
+{
+	double2* const p = xrow + a;
+	cd z;
+	int node = 0;
+	if (valid)
+	{
+		z = cd(p->x, p->y);
+		node = nearest_node(L, z);
+	}
+	const bool outside = valid && L.nf_off[node] < 0;
+	Near nr;
+	for (unsigned long long m = __ballot(outside); m; m &= m - 1)
+	{
+		const int src = __ffsll(m) - 1;
+		const Near t = wave_near(L, __shfl(node, src, 64), cd(__shfl(z.re, src, 64), __shfl(z.im, src, 64)));
+		if ((int)(threadIdx.x & 63) == src) nr = t;
+	}
+	if (!valid) return -1;
+	if (!outside) nr = p2p(L, node, z);
+	const Witt w = witt_from(L, z, cp, sp, node, nr);
+	const int st = witt_root<MIXED>(L, p, z, w, circ_exact, fmm_pole, newton, f1, ex);
+	if (st == ROOT_FULL) f0 = w.f0;
+	c.full += st == ROOT_FULL;
+	c.early += st == ROOT_EARLY;
+	c.newton += st == ROOT_NEWTON || st == ROOT_STEPPED;
+	c.conv += st == ROOT_EARLY || st == ROOT_NEWTON;
+	c.nnear += st != ROOT_EARLY ? w.nr.count : 0;
+	c.near_early += st == ROOT_EARLY ? w.nr.count : 0;
+	return st;
+}
+
+// appends root a (full) to row r's list for the sweep's Cauchy sums with its
+// f0, f1 and extra pole sums; every thread of the block calls it
+__device__ __forceinline__ void append_full(const SweepView& V, int r, int R, int a, bool full, cd f0, cd f1, cd ex)
+{
+	const int pos = block_append(&V.fcnt[r], full);
+	if (full)
+	{
+		const size_t k = (size_t)r * R + pos;
+		V.fl[k] = a;
+		if (V.slot) V.slot[(size_t)r * R + a] = pos;
+		V.f0[k] = make_double2(f0.re, f0.im);
+		V.f1[k] = make_double2(f1.re, f1.im);
+		V.ex[k] = make_double2(ex.re, ex.im);
+	}
+}
+
 // K1 (ccf_functions.cuh:197-213 and the early return at :207-211), one
-// thread per listed root. A root outside the tree (whose near field is every
-// star, tree_node.cuh:529-561) takes its near-field sums from its whole wave
-// (wave_near), then goes on as the others do. Every workgroup then takes a
-// ticket; the last to finish sees every row's listed count and forms the
-// plan of the sweep's Cauchy sums.
+// thread per listed root: the Witt evaluation, then the 1/mu early exit, a
+// Newton step or a listing for the sweep's Cauchy sums. In the mixed form a
+// root that took a Newton step without converging is evaluated once more in
+// the same launch: the workgroup gathers its stepped roots in LDS and its
+// first threads evaluate them, so that the roots left for the next sweep
+// are those that still need it. Every workgroup then takes a ticket; the
+// last to finish sees every row's listed count and forms the plan of the
+// sweep's Cauchy sums.
 template <bool MIXED>
 __global__ __launch_bounds__(BLOCK) void k_witt(LensView L, double2* __restrict__ roots, RowMap M, SweepView V,
 	unsigned long long* __restrict__ stats, int circ_exact, int fmm_pole, PlanArgs P)
@@ -524,62 +589,51 @@ __global__ __launch_bounds__(BLOCK) void k_witt(LensView L, double2* __restrict_
 	const int i = xcd_x(sl.x, (n + BLOCK - 1) / BLOCK) * BLOCK + threadIdx.x;
 	if (i - (int)threadIdx.x < n) // a block with listed roots (uniform)
 	{
-		unsigned long long nfull = 0, nearly = 0, nnewton = 0, nconv = 0, nnear = 0, nnear_early = 0;
-		bool full = false, stepped = false;
-		int a = 0;
+		EvalCounts c;
+		int row;
+		double phi, cp, sp;
+		M.get(r, row, phi);
+		M.phase(row, phi, cp, sp);
+		double2* const xrow = roots + (size_t)row * R;
+		const bool valid = i < n;
+		int a = valid ? V.act[(size_t)r * R + i] : 0;
 		cd f0, f1, ex;
-		int row = 0, node = 0;
-		double phi;
-		double2* p = nullptr;
-		cd z;
-		if (i < n)
+		const int st = witt_eval<MIXED>(L, xrow, a, valid, cp, sp, circ_exact, fmm_pole, M.j > 0, f0, f1, ex, c);
+		append_full(V, r, R, a, st == ROOT_FULL, f0, f1, ex);
+		bool stepped = st == ROOT_STEPPED;
+		if (MIXED)
 		{
-			M.get(r, row, phi);
-			a = V.act[(size_t)r * R + i];
-			p = roots + (size_t)row * R + a;
-			z = cd(p->x, p->y);
-			node = nearest_node(L, z);
-		}
-		// the wave's roots outside the tree, one at a time by the whole wave
-		const bool outside = i < n && L.nf_off[node] < 0;
-		Near nr;
-		for (unsigned long long m = __ballot(outside); m; m &= m - 1)
-		{
-			const int src = __ffsll(m) - 1;
-			const Near t = wave_near(L, __shfl(node, src, 64), cd(__shfl(z.re, src, 64), __shfl(z.im, src, 64)));
-			if ((int)(threadIdx.x & 63) == src) nr = t;
-		}
-		if (i < n)
-		{
-			if (!outside) nr = p2p(L, node, z);
-			double cp, sp;
-			M.phase(row, phi, cp, sp);
-			const Witt w = witt_from(L, z, cp, sp, node, nr);
-			const int st = witt_root<MIXED>(L, p, z, w, circ_exact, fmm_pole, M.j > 0, f1, ex);
-			full = st == ROOT_FULL;
-			if (full) f0 = w.f0;
-			stepped = st == ROOT_STEPPED;
-			nfull = full;
-			nearly = st == ROOT_EARLY;
-			nnewton = st == ROOT_NEWTON || stepped;
-			nconv = st == ROOT_EARLY || st == ROOT_NEWTON;
-			if (st != ROOT_EARLY) nnear = w.nr.count;
-			else nnear_early = w.nr.count;
+			// the workgroup's stepped roots, in thread order, to its first threads
+			__shared__ int s_a2[BLOCK];
+			__shared__ int s_w2[BLOCK / 64];
+			const unsigned long long m = __ballot(stepped);
+			const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+			if (lane == 0) s_w2[wv] = (int)__popcll(m);
+			__syncthreads();
+			int off = 0, n2 = 0;
+			for (int q = 0; q < BLOCK / 64; q++)
+			{
+				if (q < wv) off += s_w2[q];
+				n2 += s_w2[q];
+			}
+			const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+			if (stepped) s_a2[off + (int)__popcll(m & below)] = a;
+			__syncthreads();
+			stepped = false;
+			if (n2 > 0) // uniform
+			{
+				const bool v2 = (int)threadIdx.x < n2;
+				a = v2 ? s_a2[threadIdx.x] : 0;
+				const int st2 = witt_eval<MIXED>(L, xrow, a, v2, cp, sp, circ_exact, fmm_pole, 1, f0, f1, ex, c);
+				append_full(V, r, R, a, st2 == ROOT_FULL, f0, f1, ex);
+				stepped = st2 == ROOT_STEPPED;
+			}
 		}
 		const int spos = block_append(&V.cnt_next[r], stepped);
 		if (stepped) V.act_next[(size_t)r * R + spos] = a;
-		const int pos = block_append(&V.fcnt[r], full);
-		if (full)
-		{
-			const size_t k = (size_t)r * R + pos;
-			V.fl[k] = a;
-			if (V.slot) V.slot[(size_t)r * R + a] = pos;
-			V.f0[k] = make_double2(f0.re, f0.im);
-			V.f1[k] = make_double2(f1.re, f1.im);
-			V.ex[k] = make_double2(ex.re, ex.im);
-		}
 		const int which[6] = {C_FULL, C_EARLY, C_CONV, C_NEAR, C_NEWTON, C_NEAR_EARLY};
-		unsigned long long v[6] = {nfull, nearly, nconv, nnear, nnewton, nnear_early};
+		unsigned long long v[6] = {(unsigned long long)c.full, (unsigned long long)c.early, (unsigned long long)c.conv,
+			(unsigned long long)c.nnear, (unsigned long long)c.newton, (unsigned long long)c.near_early};
 		block_stats(stats, which, v);
 	}
 	// every workgroup takes a ticket once its appends are done; the last one
