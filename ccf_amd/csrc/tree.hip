@@ -36,6 +36,32 @@ __device__ inline int quadrant(double2 z, double2 c)
 __device__ inline cd ld(const double2* p, size_t i) { double2 v = p[i]; return cd(v.x, v.y); }
 __device__ inline void st(double2* p, size_t i, cd v) { p[i] = make_double2(v.re, v.im); }
 
+// near-field run offset of every node: its prefix rank for a leaf, -1
+// otherwise
+__global__ void k_leaf_off(int nn, const int* __restrict__ child, const int* __restrict__ rank, int* __restrict__ off)
+{
+	const int n = blockIdx.x * blockDim.x + threadIdx.x;
+	if (n < nn) off[n] = child[n] < 0 ? rank[n] : -1;
+}
+
+// level 0 of a one-field tree: the root node (centre 0, half-length hl0,
+// stars [s0, s0 + n)) and the level loop's four counters cleared
+__global__ void k_root_node(double hl0, int s0, int n, double2* center, double* hl, int* parent, int* child, int* sstart,
+	int* scount, int* nnbr, int* nsl, int* ndl, int* field, int* scal)
+{
+	center[0] = make_double2(0, 0);
+	hl[0] = hl0;
+	parent[0] = -1;
+	child[0] = -1;
+	sstart[0] = s0;
+	scount[0] = n;
+	nnbr[0] = 0;
+	nsl[0] = 0;
+	ndl[0] = 0;
+	field[0] = 0;
+	for (int k = 0; k < 4; k++) scal[k] = 0;
+}
+
 // scal[0] = max stars in node + neighbours over the level, scal[2] = list overflow flag
 __global__ void k_level_count(int off, int cnt, const int* __restrict__ scount, const int* __restrict__ nbr,
 	const int* __restrict__ nnbr, int* __restrict__ split, int* __restrict__ scal)
@@ -86,10 +112,13 @@ __global__ void k_scan(const int* __restrict__ flags, int n, int* __restrict__ r
 	if (threadIdx.x == 0) *total = carry;
 }
 
+// (also clears scal[0], the next level's maximum, which the host has read)
 __global__ void k_make_children(int off, int cnt, int new_off, const int* __restrict__ split, const int* __restrict__ rank,
-	double2* center, double* hl, int* parent, int* child, int* sstart, int* scount, int* nnbr, int* nsl, int* ndl, int* field)
+	double2* center, double* hl, int* parent, int* child, int* sstart, int* scount, int* nnbr, int* nsl, int* ndl, int* field,
+	int* scal)
 {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i == 0) scal[0] = 0;
 	if (i >= cnt || !split[i]) return;
 	int n = off + i;
 	int c0 = new_off + 4 * rank[i];
@@ -669,7 +698,17 @@ bool build_forest(ccf_ctx* ctx, ForestSpec& spec, int verbose)
 	int* scal = nullptr;
 	if (!CCF_HIP(ctx, ctx->dmalloc(&scal, 4 * sizeof(int)))) return false;
 
-	// root nodes
+	// root nodes: one field's by a kernel (no host copies to wait for),
+	// several fields' from host arrays
+	bool roots_set = false;
+	if (F == 1)
+	{
+		k_root_node<<<1, 1, 0, s>>>(spec.root_hl[0], spec.soff[0], spec.nstars[0], T.center, T.hl, T.parent, T.child, T.sstart,
+			T.scount, T.nnbr, T.nsl, T.ndl, T.field, scal);
+		if (!CCF_LAUNCH(ctx, "tree root node")) { ctx->dfree(scal); return false; }
+		roots_set = true;
+	}
+	if (!roots_set)
 	{
 		std::vector<double2> c(F, make_double2(0, 0));
 		std::vector<int> m1(F, -1), z(F, 0), fid(F);
@@ -695,13 +734,13 @@ bool build_forest(ccf_ctx* ctx, ForestSpec& spec, int verbose)
 	// scal[2], the list overflow flag, stays set once a level's (capped)
 	// lists overflow: it is read with the next level's counts (the loop
 	// ends at such a read-back), one read-back per level
-	bool ok = CCF_HIP(ctx, hipMemsetAsync(scal, 0, 4 * sizeof(int), s));
+	bool ok = roots_set || CCF_HIP(ctx, hipMemsetAsync(scal, 0, 4 * sizeof(int), s));
 	int h_scal[4] = {0, 0, 0, 0};
 	while (ok && !P.direct)
 	{
 		const int off = T.level_off[T.levels], cnt = T.level_cnt[T.levels];
-		ok = CCF_HIP(ctx, hipMemsetAsync(scal, 0, 2 * sizeof(int), s));
-		if (!ok) break;
+		// scal[0] was cleared at level 0's set-up or by the previous level's
+		// k_make_children; k_scan writes scal[1]
 		k_level_count<<<grid(cnt), BLOCK, 0, s>>>(off, cnt, T.scount, T.nbr, T.nnbr, T.split, scal);
 		k_scan<<<1, 1024, 0, s>>>(T.split, cnt, T.rank, scal + 1);
 		ok = CCF_LAUNCH(ctx, "tree level count") && CCF_HIP(ctx, hipMemcpyAsync(h_scal, scal, 4 * sizeof(int), hipMemcpyDeviceToHost, s)) &&
@@ -719,7 +758,7 @@ bool build_forest(ccf_ctx* ctx, ForestSpec& spec, int verbose)
 		if (verbose >= 3) std::cout << "Number of non-empty children: " << new_cnt << "\n";
 		if (!(ok = ensure_capacity(ctx, new_off + new_cnt))) break;
 		k_make_children<<<grid(cnt), BLOCK, 0, s>>>(off, cnt, new_off, T.split, T.rank, T.center, T.hl, T.parent, T.child,
-			T.sstart, T.scount, T.nnbr, T.nsl, T.ndl, T.field);
+			T.sstart, T.scount, T.nnbr, T.nsl, T.ndl, T.field, scal);
 		k_partition<<<cnt, BLOCK, 0, s>>>(off, T.split, T.child, T.center, T.sstart, T.scount, ctx->spos, ctx->smass,
 			ctx->spos_tmp, ctx->smass_tmp);
 		k_set_neighbors<<<grid(new_cnt), BLOCK, 0, s>>>(new_off, new_cnt, T.center, T.hl, T.parent, T.child, T.scount,
@@ -774,21 +813,22 @@ bool build_forest(ccf_ctx* ctx, ForestSpec& spec, int verbose)
 			k_lookup<<<grid((long long)n * n), BLOCK, 0, s>>>(T.lookup_level, 0, -H, 2 * H / n, T.nbox, T.lookup);
 		}
 		k_near_count<<<grid(nn), BLOCK, 0, s>>>(nn, T.child, T.nf, T.nnf, T.nf_cnt);
-		std::vector<int> cnt(nn), off(nn);
-		if (!CCF_LAUNCH(ctx, "node boxes") || !CCF_HIP(ctx, hipMemcpyAsync(cnt.data(), T.nf_cnt, sizeof(int) * nn, hipMemcpyDeviceToHost, s)) ||
+		// the leaves' runs in node order: an exclusive prefix of the counts
+		// (non-leaves count 0) on the device, the total read back to size
+		// the copies (T.rank and T.split are the level loop's scratch, of
+		// the tree's capacity)
+		k_scan<<<1, 1024, 0, s>>>(T.nf_cnt, nn, T.rank, T.split);
+		k_leaf_off<<<grid(nn), BLOCK, 0, s>>>(nn, T.child, T.rank, T.nf_off);
+		int tot = 0;
+		if (!CCF_LAUNCH(ctx, "node boxes") || !CCF_HIP(ctx, hipMemcpyAsync(&tot, T.split, sizeof(int), hipMemcpyDeviceToHost, s)) ||
 			!CCF_HIP(ctx, hipStreamSynchronize(s)))
 			return false;
-		std::vector<int> child(nn);
-		if (!CCF_HIP(ctx, hipMemcpy(child.data(), T.child, sizeof(int) * nn, hipMemcpyDeviceToHost))) return false;
-		long long tot = 0;
-		for (int n = 0; n < nn; n++)
+		if (tot < 0)
 		{
-			off[n] = child[n] < 0 ? (int)tot : -1;
-			tot += cnt[n];
-		}
-		if (!CCF_HIP(ctx, ctx->dmalloc(&T.nfs, sizeof(double4) * (size_t)std::max(tot, 1LL))) ||
-			!CCF_HIP(ctx, hipMemcpyAsync(T.nf_off, off.data(), sizeof(int) * nn, hipMemcpyHostToDevice, s)))
+			ctx->fail("Error. The near-field copies exceed 2^31 stars.\n");
 			return false;
+		}
+		if (!CCF_HIP(ctx, ctx->dmalloc(&T.nfs, sizeof(double4) * (size_t)std::max(tot, 1)))) return false;
 		k_near_fill<<<grid(nn), BLOCK, 0, s>>>(nn, T.nf_off, T.nf, T.nnf, ctx->spos, ctx->smass, T.nfs);
 		if (!CCF_LAUNCH(ctx, "near-field copies") || !CCF_HIP(ctx, hipStreamSynchronize(s))) return false;
 	}
