@@ -438,7 +438,17 @@ __global__ __launch_bounds__(BLOCK) void k_upward(int off, int cnt, int p, const
 
 // downward pass for one level >= 2: L2L from the parent (fmm.cuh:204-223),
 // M2L from the same-level list (fmm.cuh:281-322), P2L from the coarser
-// leaves (fmm.cuh:387-413); NS = 2 as in k_upward
+// leaves (fmm.cuh:387-413); NS = 2 as in k_upward. A node takes DOWN_LANES
+// lanes of a wave, lane l forming coefficient l (l <= p <= 31). The M2L sum
+// of a source, sum_{k=1..p} C(l+k-1, k-1) b_k (-dz)^-k, is formed from
+// u_k = b_k (-dz)^-k, which lane k computes once for every l and leaves in
+// LDS; lane l then adds its binomial-weighted u_k from k = p down to 1.
+constexpr int DOWN_LANES = 32;
+constexpr int DOWN_NODES = BLOCK / DOWN_LANES;
+
+// orders dividing an LDS row of a node's u_k (k <= MAX_EXPANSION_ORDER)
+static_assert(MAX_EXPANSION_ORDER < DOWN_LANES, "a node's coefficients must fit its lanes");
+
 template <int NS>
 __global__ __launch_bounds__(BLOCK) void k_downward(int off, int cnt, int p, const double2* __restrict__ center, const double* __restrict__ hl,
 	const int* __restrict__ parent, const int* __restrict__ sl, const int* __restrict__ nsl,
@@ -450,11 +460,12 @@ __global__ __launch_bounds__(BLOCK) void k_downward(int off, int cnt, int p, con
 	// the Horner loops below take their coefficients and binomials four at a
 	// time ahead of the dependent products, in the same order of operations
 	extern __shared__ double sB[];
+	__shared__ double2 su[DOWN_NODES][NS][DOWN_LANES];
 	const int nB = (2 * p + 1) * (2 * p + 2) / 2;
 	for (int q = threadIdx.x; q < nB; q += blockDim.x) sB[q] = B[q];
 	__syncthreads();
-	const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-	const int i = idx / (p + 1), l = idx % (p + 1);
+	const int slot = threadIdx.x / DOWN_LANES, l = threadIdx.x % DOWN_LANES;
+	const int i = blockIdx.x * DOWN_NODES + slot;
 	if (i >= cnt) return;
 	const int n = off + i;
 	const int P1 = p + 1;
@@ -505,55 +516,65 @@ __global__ __launch_bounds__(BLOCK) void k_downward(int off, int cnt, int p, con
 		const int m = sl[n * MAX_SAME_LEVEL + j];
 		const cd dz = (ld(center, m) - cn) / hl[m];
 		const cd imdz = (-dz).inv();
+		// lane k (1 <= k <= p): u_k of every mass set
+		{
+			cd pw;
+			if (l >= 1 && l <= p) pw = imdz.ipow(l);
+#pragma unroll
+			for (int q = 0; q < NS; q++)
+			{
+				const cd u = l >= 1 && l <= p ? ld(mps[q] + (size_t)m * P1, l) * pw : cd();
+				su[slot][q][l] = make_double2(u.re, u.im);
+			}
+		}
+		// the node's lanes are one wave's: its LDS writes above are seen by
+		// its reads below once the wave passes this point
+		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+		__builtin_amdgcn_wave_barrier();
+		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 		cd t[NS];
-		for (int k0 = p; k0 >= 1; k0 -= 4)
+		if (l <= p)
 		{
-			cd bv[NS][4];
-			double cv[4];
-#pragma unroll
-			for (int u = 0; u < 4; u++)
+			for (int k = p; k >= 1; k--)
 			{
-				const int k = max(k0 - u, 1);
+				const double c = l == 0 ? 1.0 : bin(sB, l + k - 1, k - 1);
 #pragma unroll
-				for (int q = 0; q < NS; q++) bv[q][u] = ld(mps[q] + (size_t)m * P1, k);
-				cv[u] = l == 0 ? 1.0 : bin(sB, l + k - 1, k - 1);
-			}
-#pragma unroll
-			for (int u = 0; u < 4; u++)
-				if (k0 - u >= 1)
+				for (int q = 0; q < NS; q++)
 				{
-#pragma unroll
-					for (int q = 0; q < NS; q++)
-					{
-						t[q] += l == 0 ? bv[q][u] : bv[q][u] * cv[u];
-						t[q] *= imdz;
-					}
+					const double2 u = su[slot][q][k];
+					t[q] += cd(u.x, u.y) * c;
 				}
-		}
-		if (l == 0)
-		{
-			const cd lg = (-dz).log();
-			const double lh = log(hl[m]);
-#pragma unroll
-			for (int q = 0; q < NS; q++)
-			{
-				const cd b0 = ld(mps[q] + (size_t)m * P1, 0);
-				t[q] += b0 * lg;
-				t[q] += b0 * lh;
 			}
-		}
-		else
-		{
-			const cd w = dz.inv().ipow(l);
-#pragma unroll
-			for (int q = 0; q < NS; q++)
+			if (l == 0)
 			{
-				t[q] -= ld(mps[q] + (size_t)m * P1, 0) / (double)l;
-				t[q] *= w;
-			}
-		}
+				const cd lg = (-dz).log();
+				const double lh = log(hl[m]);
 #pragma unroll
-		for (int q = 0; q < NS; q++) r[q] += t[q];
+				for (int q = 0; q < NS; q++)
+				{
+					const cd b0 = ld(mps[q] + (size_t)m * P1, 0);
+					t[q] += b0 * lg;
+					t[q] += b0 * lh;
+				}
+			}
+			else
+			{
+				const cd w = dz.inv().ipow(l);
+#pragma unroll
+				for (int q = 0; q < NS; q++)
+				{
+					t[q] -= ld(mps[q] + (size_t)m * P1, 0) / (double)l;
+					t[q] *= w;
+				}
+			}
+#pragma unroll
+			for (int q = 0; q < NS; q++) r[q] += t[q];
+		}
+		// every lane's reads of this source's u_k are done before the next
+		// source's writes
+		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+		__builtin_amdgcn_wave_barrier();
+		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 	}
 	for (int j = 0; j < ndl[n]; j++)
 	{
@@ -582,6 +603,7 @@ __global__ __launch_bounds__(BLOCK) void k_downward(int off, int cnt, int p, con
 			}
 		}
 	}
+	if (l > p) return; // (lanes beyond the order only formed u_k)
 #pragma unroll
 	for (int q = 0; q < NS; q++) st(locs[q], (size_t)n * P1 + l, r[q]);
 }
@@ -934,12 +956,12 @@ bool build_forest(ccf_ctx* ctx, ForestSpec& spec, int verbose)
 		}
 		for (int l = 2; l <= T.levels; l++)
 		{
-			const long long work = (long long)T.level_cnt[l] * P1;
+			const int blocks = grid(T.level_cnt[l], DOWN_NODES);
 			if (dual)
-				k_downward<2><<<grid(work), BLOCK, binom_lds(p), s>>>(T.level_off[l], T.level_cnt[l], p, T.center, T.hl, T.parent,
+				k_downward<2><<<blocks, BLOCK, binom_lds(p), s>>>(T.level_off[l], T.level_cnt[l], p, T.center, T.hl, T.parent,
 					T.sl, T.nsl, T.dl, T.ndl, T.sstart, T.scount, ctx->spos, ctx->smass, ctx->ones, T.binom, T.mp, T.mpu, T.loc, T.locu);
 			else
-				k_downward<1><<<grid(work), BLOCK, binom_lds(p), s>>>(T.level_off[l], T.level_cnt[l], p, T.center, T.hl, T.parent,
+				k_downward<1><<<blocks, BLOCK, binom_lds(p), s>>>(T.level_off[l], T.level_cnt[l], p, T.center, T.hl, T.parent,
 					T.sl, T.nsl, T.dl, T.ndl, T.sstart, T.scount, ctx->spos, ctx->smass, nullptr, T.binom, T.mp, nullptr, T.loc, nullptr);
 		}
 		if (!CCF_LAUNCH(ctx, "calculate_coeffs_kernels")) return false;
