@@ -726,6 +726,8 @@ __global__ __launch_bounds__(REGION_THREADS) void k_rup(RootView V, const double
 // k_rdown finds its own cell's; before, every region formed this chain of
 // ancestors itself.
 // (512 threads while the registers of an order allow two waves a SIMD)
+constexpr size_t RCOARSE_LDS = 48 * 1024;
+
 template <int P>
 constexpr int coarse_threads()
 {
@@ -733,16 +735,31 @@ constexpr int coarse_threads()
 }
 
 template <int P>
-__global__ __launch_bounds__(coarse_threads<P>()) void k_rcoarse(RootView V, unsigned long long* __restrict__ stats)
+__global__ __launch_bounds__(coarse_threads<P>()) void k_rcoarse(RootView V, unsigned long long* __restrict__ stats, int lds)
 {
 	constexpr int NT = coarse_threads<P>(), NC = P + 1;
 	__shared__ double binom[NC][NC];
+	// with lds set, the coarse cells' multipoles, local expansions and
+	// counts (the first coarse_cells(g) cells of the row, levels 0 .. rlev)
+	// live in LDS, launched with their size; level rlev's multipoles and
+	// counts come in from k_rup, and only level rlev's local expansions,
+	// which k_rdown reads, and the counts go back out
+	extern __shared__ double2 dyn[];
 	const int r = blockIdx.x;
 	if (!V.rmode[r]) return;
 	const RootGrid& g = V.g;
-	const int Q = g.rlev;
-	const size_t rowc = (size_t)r * g.ncells, rowq = (size_t)r * coarse_cells(g);
-	const int* __restrict__ cc = V.ccnt + rowc;
+	const int Q = g.rlev, CC = coarse_cells(g);
+	const size_t rowc = (size_t)r * g.ncells, rowq = (size_t)r * CC;
+	double2* const mpb = lds ? dyn : V.mp + rowc * NC;
+	double2* const lqb = lds ? dyn + (size_t)CC * NC : V.lcq + rowq * NC;
+	int* const ccb = lds ? reinterpret_cast<int*>(dyn + (size_t)2 * CC * NC) : V.ccnt + rowc;
+	const int* __restrict__ cc = ccb;
+	if (lds)
+	{
+		const int q0 = g.off[Q];
+		for (int i = threadIdx.x; i < (CC - q0) * NC; i += NT) mpb[(size_t)q0 * NC + i] = V.mp[(rowc + q0) * NC + i];
+		for (int i = threadIdx.x; i < CC - q0; i += NT) ccb[q0 + i] = V.ccnt[rowc + q0 + i];
+	}
 	// C(n, k) for n, k <= P as in k_rup
 	for (int i = threadIdx.x; i < NC * NC; i += NT)
 	{
@@ -763,14 +780,14 @@ __global__ __launch_bounds__(coarse_threads<P>()) void k_rcoarse(RootView V, uns
 			for (int qd = 0; qd < 4; qd++)
 			{
 				const int a = qd & 1, b = qd >> 1;
-				const size_t ch = rowc + g.off[l + 1] + (size_t)(2 * iy + b) * nxc + 2 * ix + a;
-				const int nch = V.ccnt[ch];
+				const size_t ch = g.off[l + 1] + (size_t)(2 * iy + b) * nxc + 2 * ix + a;
+				const int nch = ccb[ch];
 				tot += nch;
-				if (nch > 0) sum += m2m_term<P>(V.mp + ch * NC, n, cd(a - 0.5, b - 0.5), binom);
+				if (nch > 0) sum += m2m_term<P>(mpb + ch * NC, n, cd(a - 0.5, b - 0.5), binom);
 			}
-			const size_t cid = rowc + g.off[l] + c;
-			V.mp[cid * NC + n] = make_double2(sum.re, sum.im);
-			if (n == 0) V.ccnt[cid] = tot;
+			const size_t cid = g.off[l] + c;
+			mpb[cid * NC + n] = make_double2(sum.re, sum.im);
+			if (n == 0) ccb[cid] = tot;
 		}
 		__syncthreads();
 	}
@@ -793,7 +810,7 @@ __global__ __launch_bounds__(coarse_threads<P>()) void k_rcoarse(RootView V, uns
 			{
 				if (l >= 1 && q == 0)
 				{
-					const double2* lp = V.lcq + (rowq + g.off[l - 1] + (size_t)(iy >> 1) * (nxl >> 1) + (ix >> 1)) * NC;
+					const double2* lp = lqb + (g.off[l - 1] + (size_t)(iy >> 1) * (nxl >> 1) + (ix >> 1)) * NC;
 #pragma unroll
 					for (int n = 0; n < NC; n++) Lc[n] = cd(lp[n].x, lp[n].y);
 					l2l_shift<P>(Lc, cd((ix & 1) - 0.5, (iy & 1) - 0.5));
@@ -806,7 +823,7 @@ __global__ __launch_bounds__(coarse_threads<P>()) void k_rcoarse(RootView V, uns
 						if ((abs(jx - ix) >= 2 || abs(jy - iy) >= 2) && cc[k] > 0)
 						{
 							nm++;
-							m2l_add<P>(V.mp + (rowc + k) * NC, ix - jx, iy - jy, ih, Lc);
+							m2l_add<P>(mpb + (size_t)k * NC, ix - jx, iy - jy, ih, Lc);
 						}
 					}
 				}
@@ -821,7 +838,7 @@ __global__ __launch_bounds__(coarse_threads<P>()) void k_rcoarse(RootView V, uns
 						const int c = g.off[l] + jy * nxl + jx;
 						if (cc[c] <= 0) continue;
 						nm++;
-						m2l_add<P>(V.mp + (rowc + c) * NC, ix - jx, iy - jy, ih, Lc);
+						m2l_add<P>(mpb + (size_t)c * NC, ix - jx, iy - jy, ih, Lc);
 					}
 				}
 			}
@@ -835,10 +852,17 @@ __global__ __launch_bounds__(coarse_threads<P>()) void k_rcoarse(RootView V, uns
 			if (live)
 #pragma unroll
 				for (int m = 0; m < NC; m++)
-					if ((m & (tpt - 1)) == q) V.lcq[(rowq + g.off[l] + t) * NC + m] = make_double2(Lc[m].re, Lc[m].im);
+					if ((m & (tpt - 1)) == q) lqb[(g.off[l] + t) * NC + m] = make_double2(Lc[m].re, Lc[m].im);
 		}
 		// the level's expansions before the next level's L2L reads them
 		__syncthreads();
+	}
+	if (lds)
+	{
+		// (the last level's barrier is behind)
+		const int q0 = g.off[Q];
+		for (int i = threadIdx.x; i < (CC - q0) * NC; i += NT) V.lcq[(rowq + q0) * NC + i] = lqb[(size_t)q0 * NC + i];
+		for (int i = threadIdx.x; i < q0; i += NT) V.ccnt[rowc + i] = ccb[i];
 	}
 	const int which[1] = {C_RM2L};
 	unsigned long long v[1] = {nm};
@@ -1082,7 +1106,10 @@ void root_fmm_pass_p(const SweepBufs& B, const double2* roots, const RowMap& M, 
 	}
 	const int nreg = (g.nx << g.rlev) * (g.ny << g.rlev);
 	k_rup<P><<<dim3(nreg + RSUM_OUT_BLOCKS, nrows), REGION_THREADS, 0, s>>>(V, roots, M, stats);
-	k_rcoarse<P><<<nrows, coarse_threads<P>(), 0, s>>>(V, stats);
+	// the coarse levels in LDS when they fit in RCOARSE_LDS bytes
+	const size_t cl = (size_t)2 * coarse_cells(g) * (P + 1) * sizeof(double2) + (size_t)coarse_cells(g) * sizeof(int);
+	const int lds = cl <= RCOARSE_LDS ? 1 : 0;
+	k_rcoarse<P><<<nrows, coarse_threads<P>(), lds ? cl : 0, s>>>(V, stats, lds);
 	k_rdown<P><<<dim3(nreg, nrows), REGION_THREADS, 0, s>>>(V, roots, M, stats);
 }
 
