@@ -142,9 +142,14 @@ __global__ void k_make_children(int off, int cnt, int new_off, const int* __rest
 	}
 }
 
+// levels of at most this many nodes are partitioned by 1024-thread workgroups
+constexpr int PARTITION_WIDE_NODES = 64;
+
 // one workgroup per node of the level: stable 4-way partition of the stars
-// of every split node into its children (tree_node.cuh:392-501)
-__global__ __launch_bounds__(BLOCK) void k_partition(int off, const int* __restrict__ split, const int* __restrict__ child,
+// of every split node into its children (tree_node.cuh:392-501); a star's
+// place depends only on the order of the stars, not on BS
+template <int BS>
+__global__ __launch_bounds__(BS) void k_partition(int off, const int* __restrict__ split, const int* __restrict__ child,
 	const double2* __restrict__ center, int* sstart, int* scount,
 	double2* spos, double* smass, double2* tpos, double* tmass)
 {
@@ -165,17 +170,17 @@ __global__ __launch_bounds__(BLOCK) void k_partition(int off, const int* __restr
 	const double2 c = center[n];
 	__shared__ int qcnt[4];
 	__shared__ int run[4];
-	__shared__ int wtot[BLOCK / 64][4];
+	__shared__ int wtot[BS / 64][4];
 	if (threadIdx.x < 4) { qcnt[threadIdx.x] = 0; run[threadIdx.x] = 0; }
 	__syncthreads();
 	int lc[4] = {0, 0, 0, 0};
-	for (int s = threadIdx.x; s < count; s += BLOCK) lc[quadrant(spos[start + s], c)]++;
+	for (int s = threadIdx.x; s < count; s += BS) lc[quadrant(spos[start + s], c)]++;
 	for (int q = 0; q < 4; q++) atomicAdd(&qcnt[q], lc[q]);
 	__syncthreads();
 	const int base[4] = {0, qcnt[0], qcnt[0] + qcnt[1], qcnt[0] + qcnt[1] + qcnt[2]};
 	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 	const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-	for (int chunk = 0; chunk < count; chunk += BLOCK)
+	for (int chunk = 0; chunk < count; chunk += BS)
 	{
 		const int s = chunk + threadIdx.x;
 		const int q = s < count ? quadrant(spos[start + s], c) : -1;
@@ -198,12 +203,12 @@ __global__ __launch_bounds__(BLOCK) void k_partition(int off, const int* __restr
 		if (threadIdx.x < 4)
 		{
 			int t = 0;
-			for (int k = 0; k < BLOCK / 64; k++) t += wtot[k][threadIdx.x];
+			for (int k = 0; k < BS / 64; k++) t += wtot[k][threadIdx.x];
 			run[threadIdx.x] += t;
 		}
 		__syncthreads();
 	}
-	for (int s = threadIdx.x; s < count; s += BLOCK)
+	for (int s = threadIdx.x; s < count; s += BS)
 	{
 		spos[start + s] = tpos[start + s];
 		smass[start + s] = tmass[start + s];
@@ -781,8 +786,13 @@ bool build_forest(ccf_ctx* ctx, ForestSpec& spec, int verbose)
 		if (!(ok = ensure_capacity(ctx, new_off + new_cnt))) break;
 		k_make_children<<<grid(cnt), BLOCK, 0, s>>>(off, cnt, new_off, T.split, T.rank, T.center, T.hl, T.parent, T.child,
 			T.sstart, T.scount, T.nnbr, T.nsl, T.ndl, T.field, scal);
-		k_partition<<<cnt, BLOCK, 0, s>>>(off, T.split, T.child, T.center, T.sstart, T.scount, ctx->spos, ctx->smass,
-			ctx->spos_tmp, ctx->smass_tmp);
+		// (levels of few nodes, each with many stars: wider workgroups)
+		if (cnt <= PARTITION_WIDE_NODES)
+			k_partition<1024><<<cnt, 1024, 0, s>>>(off, T.split, T.child, T.center, T.sstart, T.scount, ctx->spos, ctx->smass,
+				ctx->spos_tmp, ctx->smass_tmp);
+		else
+			k_partition<BLOCK><<<cnt, BLOCK, 0, s>>>(off, T.split, T.child, T.center, T.sstart, T.scount, ctx->spos, ctx->smass,
+				ctx->spos_tmp, ctx->smass_tmp);
 		k_set_neighbors<<<grid(new_cnt), BLOCK, 0, s>>>(new_off, new_cnt, T.center, T.hl, T.parent, T.child, T.scount,
 			T.nbr, T.nnbr, T.sl, T.nsl, T.dl, T.ndl, scal);
 		ok = CCF_LAUNCH(ctx, "tree split level");
