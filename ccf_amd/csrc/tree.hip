@@ -394,12 +394,29 @@ __global__ __launch_bounds__(BLOCK) void k_upward(int off, int cnt, int p, const
 		}
 		else
 		{
-			for (int s = 0; s < ns; s++)
+			// the stars four at a time: their loads ahead of the terms, which
+			// are added in star order
+			for (int s4 = 0; s4 < ns; s4 += 4)
 			{
-				const cd t = (ld(spos, s0 + s) - c) / h;
-				const cd tl = t.ipow(l);
+				double2 zv[4];
+				double mv[NS][4];
 #pragma unroll
-				for (int q = 0; q < NS; q++) r[q] -= tl * ms[q][s0 + s];
+				for (int u = 0; u < 4; u++)
+				{
+					const int sv = s0 + min(s4 + u, ns - 1);
+					zv[u] = spos[sv];
+#pragma unroll
+					for (int q = 0; q < NS; q++) mv[q][u] = ms[q][sv];
+				}
+#pragma unroll
+				for (int u = 0; u < 4; u++)
+					if (s4 + u < ns)
+					{
+						const cd t = (cd(zv[u].x, zv[u].y) - c) / h;
+						const cd tl = t.ipow(l);
+#pragma unroll
+						for (int q = 0; q < NS; q++) r[q] -= tl * mv[q][u];
+					}
 			}
 #pragma unroll
 			for (int q = 0; q < NS; q++) r[q] /= (double)l;
@@ -420,20 +437,38 @@ __global__ __launch_bounds__(BLOCK) void k_upward(int off, int cnt, int p, const
 			const cd idz = dz.inv();
 			const cd dzl = dz.ipow(l);
 			const double dv = pow2_divisor(l);
+			// Horner from k = l down, every mass set's chain at once, the
+			// coefficients and binomials of four steps loaded ahead of them
+			cd t[NS];
+			for (int k0 = l; k0 >= 1; k0 -= 4)
+			{
+				cd av[NS][4];
+				double bv[4];
+#pragma unroll
+				for (int u = 0; u < 4; u++)
+				{
+					const int k = max(k0 - u, 1);
+#pragma unroll
+					for (int q = 0; q < NS; q++) av[q][u] = ld(out[q] + (size_t)ch * P1, k);
+					bv[u] = bin(B, l - 1, k - 1);
+				}
+#pragma unroll
+				for (int u = 0; u < 4; u++)
+					if (k0 - u >= 1)
+#pragma unroll
+						for (int q = 0; q < NS; q++)
+						{
+							t[q] += av[q][u] * bv[u];
+							t[q] *= idz;
+						}
+			}
 #pragma unroll
 			for (int q = 0; q < NS; q++)
 			{
-				const double2* a = out[q] + (size_t)ch * P1;
-				cd t;
-				for (int k = l; k >= 1; k--)
-				{
-					t += ld(a, k) * bin(B, l - 1, k - 1);
-					t *= idz;
-				}
-				t -= ld(a, 0) / (double)l;
-				t *= dzl;
-				t /= dv;
-				r[q] += t;
+				t[q] -= ld(out[q] + (size_t)ch * P1, 0) / (double)l;
+				t[q] *= dzl;
+				t[q] /= dv;
+				r[q] += t[q];
 			}
 		}
 	}
