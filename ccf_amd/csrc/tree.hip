@@ -488,6 +488,7 @@ constexpr int DOWN_NODES = BLOCK / DOWN_LANES;
 
 // orders dividing an LDS row of a node's u_k (k <= MAX_EXPANSION_ORDER)
 static_assert(MAX_EXPANSION_ORDER < DOWN_LANES, "a node's coefficients must fit its lanes");
+static_assert(MAX_SAME_LEVEL <= DOWN_LANES, "a node's same-level list must fit its lanes");
 
 template <int NS>
 __global__ __launch_bounds__(BLOCK) void k_downward(int off, int cnt, int p, const double2* __restrict__ center, const double* __restrict__ hl,
@@ -551,9 +552,12 @@ __global__ __launch_bounds__(BLOCK) void k_downward(int off, int cnt, int p, con
 			r[q] /= dv;
 		}
 	}
-	for (int j = 0; j < nsl[n]; j++)
+	// the node's same-level list, entry j in lane j of its DOWN_LANES
+	const int nsl_n = nsl[n];
+	const int my_src = l < nsl_n ? sl[n * MAX_SAME_LEVEL + l] : 0;
+	for (int j = 0; j < nsl_n; j++)
 	{
-		const int m = sl[n * MAX_SAME_LEVEL + j];
+		const int m = __shfl(my_src, j, DOWN_LANES);
 		const cd dz = (ld(center, m) - cn) / hl[m];
 		const cd imdz = (-dz).inv();
 		// lane k (1 <= k <= p): u_k of every mass set
@@ -620,26 +624,43 @@ __global__ __launch_bounds__(BLOCK) void k_downward(int off, int cnt, int p, con
 	{
 		const int m = dl[n * MAX_DIFF_LEVEL + j];
 		const int s0 = sstart[m], ns = scount[m];
-		for (int s = 0; s < ns; s++)
+		// the stars four at a time: their loads ahead of the terms, which
+		// are added in star order
+		for (int s4 = 0; s4 < ns; s4 += 4)
 		{
-			const cd dz = (ld(spos, s0 + s) - cn) / hn;
-			if (l == 0)
-			{
-				const cd lg = (-dz).log();
-				const double lh = log(hn);
+			double2 zv[4];
+			double mv[NS][4];
 #pragma unroll
-				for (int q = 0; q < NS; q++)
-				{
-					const double mq = ms[q][s0 + s];
-					r[q] += mq * lg;
-					r[q] += cd(mq * lh);
-				}
+			for (int u = 0; u < 4; u++)
+			{
+				const int sv = s0 + min(s4 + u, ns - 1);
+				zv[u] = spos[sv];
+#pragma unroll
+				for (int q = 0; q < NS; q++) mv[q][u] = ms[q][sv];
 			}
-			else
-			{
-				const cd w = dz.inv().ipow(l);
 #pragma unroll
-				for (int q = 0; q < NS; q++) r[q] -= (ms[q][s0 + s] / (double)l) * w;
+			for (int u = 0; u < 4; u++)
+			{
+				if (s4 + u >= ns) break;
+				const cd dz = (cd(zv[u].x, zv[u].y) - cn) / hn;
+				if (l == 0)
+				{
+					const cd lg = (-dz).log();
+					const double lh = log(hn);
+#pragma unroll
+					for (int q = 0; q < NS; q++)
+					{
+						const double mq = mv[q][u];
+						r[q] += mq * lg;
+						r[q] += cd(mq * lh);
+					}
+				}
+				else
+				{
+					const cd w = dz.inv().ipow(l);
+#pragma unroll
+					for (int q = 0; q < NS; q++) r[q] -= (mv[q][u] / (double)l) * w;
+				}
 			}
 		}
 	}
