@@ -346,7 +346,7 @@ __global__ void k_near_count(int nnodes, const int* __restrict__ child, const in
 // contiguous copy {x, y, m} of each leaf's near-field stars, in the order
 // of its ranges (own stars, then each neighbour's)
 __global__ void k_near_fill(int nnodes, const int* __restrict__ off, const int2* __restrict__ nf, const int* __restrict__ nnf,
-	const double2* __restrict__ spos, const double* __restrict__ smass, double4* nfs)
+	const double2* __restrict__ spos, const double* __restrict__ smass, double4* __restrict__ nfs)
 {
 	int n = blockIdx.x * blockDim.x + threadIdx.x;
 	if (n >= nnodes || off[n] < 0) return;
@@ -354,6 +354,7 @@ __global__ void k_near_fill(int nnodes, const int* __restrict__ off, const int2*
 	for (int k = 0; k < nnf[n]; k++)
 	{
 		const int2 rg = nf[n * MAX_NEAR_RANGES + k];
+#pragma unroll 4
 		for (int s = rg.x; s < rg.x + rg.y; s++) nfs[t++] = make_double4(spos[s].x, spos[s].y, smass[s], 0.0);
 	}
 }
