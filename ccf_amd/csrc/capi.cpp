@@ -700,6 +700,7 @@ void ccf_destroy(ccf_ctx* c)
 		if (e) (void)hipEventDestroy(e);
 	for (hipEvent_t e : c->read_event)
 		if (e) (void)hipEventDestroy(e);
+	if (c->host_stage) (void)hipHostFree(c->host_stage);
 	if (c->err_event) (void)hipEventDestroy(c->err_event);
 	if (c->err_stream) (void)hipStreamDestroy(c->err_stream);
 	if (c->group_event) (void)hipEventDestroy(c->group_event);

@@ -670,6 +670,12 @@ __global__ __launch_bounds__(BLOCK) void k_downward(int off, int cnt, int p, con
 	for (int q = 0; q < NS; q++) st(locs[q], (size_t)n * P1 + l, r[q]);
 }
 
+__global__ void k_fill_ones(double* __restrict__ p, int n)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) p[i] = 1.0;
+}
+
 __global__ void k_to_float(const double2* __restrict__ a, float2* b, size_t n)
 {
 	size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -919,7 +925,7 @@ bool build_forest(ccf_ctx* ctx, ForestSpec& spec, int verbose)
 		}
 		if (!CCF_HIP(ctx, ctx->dmalloc(&T.nfs, sizeof(double4) * (size_t)std::max(tot, 1)))) return false;
 		k_near_fill<<<grid(nn), BLOCK, 0, s>>>(nn, T.nf_off, T.nf, T.nnf, ctx->spos, ctx->smass, T.nfs);
-		if (!CCF_LAUNCH(ctx, "near-field copies") || !CCF_HIP(ctx, hipStreamSynchronize(s))) return false;
+		if (!CCF_LAUNCH(ctx, "near-field copies")) return false;
 	}
 
 	// depth of every field's tree
@@ -996,10 +1002,13 @@ bool build_forest(ccf_ctx* ctx, ForestSpec& spec, int verbose)
 	const bool dual = P.precision == 1 && T.order > 0;
 	if (dual)
 	{
-		std::vector<double> one(N, 1.0);
-		const bool ok2 = CCF_HIP(ctx, ctx->dmalloc(&ctx->ones, sizeof(double) * N)) &&
-			CCF_HIP(ctx, hipMemcpyAsync(ctx->ones, one.data(), sizeof(double) * N, hipMemcpyHostToDevice, s)) &&
-			CCF_HIP(ctx, ctx->dmalloc(&T.mpu, sizeof(double2) * (size_t)T.nnodes * P1)) &&
+		bool ok2 = CCF_HIP(ctx, ctx->dmalloc(&ctx->ones, sizeof(double) * N));
+		if (ok2)
+		{
+			k_fill_ones<<<grid(N), BLOCK, 0, s>>>(ctx->ones, N);
+			ok2 = CCF_LAUNCH(ctx, "unit masses");
+		}
+		ok2 = ok2 && CCF_HIP(ctx, ctx->dmalloc(&T.mpu, sizeof(double2) * (size_t)T.nnodes * P1)) &&
 			CCF_HIP(ctx, ctx->dmalloc(&T.locu, sizeof(double2) * (size_t)T.nnodes * P1)) &&
 			CCF_HIP(ctx, ctx->dmalloc(&T.locu32, sizeof(float2) * (size_t)T.nnodes * P1)) &&
 			CCF_HIP(ctx, hipMemsetAsync(T.mpu, 0, sizeof(double2) * (size_t)T.nnodes * P1, s)) &&
@@ -1008,9 +1017,15 @@ bool build_forest(ccf_ctx* ctx, ForestSpec& spec, int verbose)
 	}
 	if (T.order > 0)
 	{
+		// through pinned memory, so the copy does not hold the host (the
+		// table of a previous build was copied before that build's end)
 		std::vector<double> B = binomial_table(2 * p, P.binomial_exact != 0);
+		const size_t stage_max = (size_t)(2 * MAX_EXPANSION_ORDER + 1) * (2 * MAX_EXPANSION_ORDER + 2) / 2;
+		if (!ctx->host_stage && !CCF_HIP(ctx, hipHostMalloc((void**)&ctx->host_stage, sizeof(double) * stage_max, hipHostMallocDefault)))
+			return false;
+		std::copy(B.begin(), B.end(), ctx->host_stage);
 		if (!CCF_HIP(ctx, ctx->dmalloc(&T.binom, sizeof(double) * B.size()))) return false;
-		if (!CCF_HIP(ctx, hipMemcpyAsync(T.binom, B.data(), sizeof(double) * B.size(), hipMemcpyHostToDevice, s))) return false;
+		if (!CCF_HIP(ctx, hipMemcpyAsync(T.binom, ctx->host_stage, sizeof(double) * B.size(), hipMemcpyHostToDevice, s))) return false;
 		for (int l = T.levels; l >= 0; l--)
 		{
 			const long long work = (long long)T.level_cnt[l] * P1;
