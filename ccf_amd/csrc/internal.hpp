@@ -239,7 +239,7 @@ struct ccf_ctx
 	}
 	int* host_counts = nullptr;   // pinned: active roots per row, the two FMM row counts of a read-back, then per continuation group two slots of two for the mixed continuation
 	int tree_hint_stars = -1, tree_hint_fields = -1, tree_hint_nodes = 0; // the last tree build's node count
-	double* host_stage = nullptr; // pinned, until ccf_destroy: the tree build's binomial table on its way to the device
+	double* host_stage = nullptr; // pinned, until ccf_destroy: the tree build's level counts and binomial table on their way between host and device
 	hipStream_t err_stream = nullptr; // the error pass of find_ccs, beside the continuation (default priority)
 	hipEvent_t err_event = nullptr;   // a row set's roots are final (no timing)
 	hipEvent_t fmm_rows_event[2 * ccf::CONT_GROUPS] = {}; // mixed continuation: after the copy of a step's FMM row count, per group (no timing)

@@ -142,6 +142,10 @@ __global__ void k_make_children(int off, int cnt, int new_off, const int* __rest
 	}
 }
 
+// the pinned staging buffer of a tree build (ccf_ctx::host_stage): the
+// binomial table of the largest order, which also holds a level's counts
+constexpr size_t STAGE_DOUBLES = (size_t)(2 * MAX_EXPANSION_ORDER + 1) * (2 * MAX_EXPANSION_ORDER + 2) / 2;
+
 // levels of at most this many nodes are partitioned by 1024-thread workgroups
 constexpr int PARTITION_WIDE_NODES = 64;
 
@@ -826,6 +830,11 @@ bool build_forest(ccf_ctx* ctx, ForestSpec& spec, int verbose)
 	// ends at such a read-back), one read-back per level
 	bool ok = roots_set || CCF_HIP(ctx, hipMemsetAsync(scal, 0, 4 * sizeof(int), s));
 	int h_scal[4] = {0, 0, 0, 0};
+	// the level counts come back through the pinned staging buffer
+	// (STAGE_DOUBLES doubles, until ccf_destroy)
+	if (ok && !ctx->host_stage)
+		ok = CCF_HIP(ctx, hipHostMalloc((void**)&ctx->host_stage, sizeof(double) * STAGE_DOUBLES, hipHostMallocDefault));
+	int* const hs = ok ? reinterpret_cast<int*>(ctx->host_stage) : nullptr;
 	while (ok && !P.direct)
 	{
 		const int off = T.level_off[T.levels], cnt = T.level_cnt[T.levels];
@@ -833,9 +842,10 @@ bool build_forest(ccf_ctx* ctx, ForestSpec& spec, int verbose)
 		// k_make_children; k_scan writes scal[1]
 		k_level_count<<<grid(cnt), BLOCK, 0, s>>>(off, cnt, T.scount, T.nbr, T.nnbr, T.split, scal);
 		k_scan<<<1, 1024, 0, s>>>(T.split, cnt, T.rank, scal + 1);
-		ok = CCF_LAUNCH(ctx, "tree level count") && CCF_HIP(ctx, hipMemcpyAsync(h_scal, scal, 4 * sizeof(int), hipMemcpyDeviceToHost, s)) &&
+		ok = CCF_LAUNCH(ctx, "tree level count") && CCF_HIP(ctx, hipMemcpyAsync(hs, scal, 4 * sizeof(int), hipMemcpyDeviceToHost, s)) &&
 			CCF_HIP(ctx, hipStreamSynchronize(s));
 		if (!ok) break;
+		for (int k = 0; k < 4; k++) h_scal[k] = hs[k];
 		if (h_scal[2]) break;
 		if (verbose >= 3)
 		{
@@ -1020,9 +1030,6 @@ bool build_forest(ccf_ctx* ctx, ForestSpec& spec, int verbose)
 		// through pinned memory, so the copy does not hold the host (the
 		// table of a previous build was copied before that build's end)
 		std::vector<double> B = binomial_table(2 * p, P.binomial_exact != 0);
-		const size_t stage_max = (size_t)(2 * MAX_EXPANSION_ORDER + 1) * (2 * MAX_EXPANSION_ORDER + 2) / 2;
-		if (!ctx->host_stage && !CCF_HIP(ctx, hipHostMalloc((void**)&ctx->host_stage, sizeof(double) * stage_max, hipHostMallocDefault)))
-			return false;
 		std::copy(B.begin(), B.end(), ctx->host_stage);
 		if (!CCF_HIP(ctx, ctx->dmalloc(&T.binom, sizeof(double) * B.size()))) return false;
 		if (!CCF_HIP(ctx, hipMemcpyAsync(T.binom, ctx->host_stage, sizeof(double) * B.size(), hipMemcpyHostToDevice, s))) return false;
