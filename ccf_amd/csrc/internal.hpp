@@ -21,6 +21,10 @@ constexpr int MAX_NUM_STARS_DIRECT = 32; // tree_node.cuh:14
 constexpr int MAX_EXPANSION_ORDER = 31;  // tree_node.cuh:20
 constexpr int MAX_NEIGHBORS = 8;         // tree_node.cuh:23
 constexpr int MAX_SAME_LEVEL = 27;       // tree_node.cuh:24
+// the pinned staging buffer of a context (ccf_ctx::host_stage), in doubles:
+// the binomial table of the largest expansion order, which also holds a
+// tree level's counts or the run's folded statistics
+constexpr size_t STAGE_DOUBLES = (size_t)(2 * MAX_EXPANSION_ORDER + 1) * (2 * MAX_EXPANSION_ORDER + 2) / 2;
 constexpr int MAX_DIFF_LEVEL = 5;        // tree_node.cuh:25
 constexpr int MAX_NEAR_RANGES = MAX_NEIGHBORS + 1;
 constexpr int MAX_TAYLOR_SMOOTH = 101;   // ccf.cuh:79
@@ -239,7 +243,7 @@ struct ccf_ctx
 	}
 	int* host_counts = nullptr;   // pinned: active roots per row, the two FMM row counts of a read-back, then per continuation group two slots of two for the mixed continuation
 	int tree_hint_stars = -1, tree_hint_fields = -1, tree_hint_nodes = 0; // the last tree build's node count
-	double* host_stage = nullptr; // pinned, until ccf_destroy: the tree build's level counts and binomial table on their way between host and device
+	double* host_stage = nullptr; // pinned (STAGE_DOUBLES), until ccf_destroy: a tree build's level counts and binomial table, a run's folded statistics
 	hipStream_t err_stream = nullptr; // the error pass of find_ccs, beside the continuation (default priority)
 	hipEvent_t err_event = nullptr;   // a row set's roots are final (no timing)
 	hipEvent_t fmm_rows_event[2 * ccf::CONT_GROUPS] = {}; // mixed continuation: after the copy of a step's FMM row count, per group (no timing)

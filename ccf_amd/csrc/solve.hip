@@ -121,6 +121,23 @@ __device__ inline unsigned long long* stat_slot(unsigned long long* stats, int c
 }
 
 
+// the per-slot copies of every counter folded into out[k]: max for the
+// error bits and the residual, or for the NaN flag, sums otherwise
+__global__ void k_fold_stats(const unsigned long long* __restrict__ stats, unsigned long long* __restrict__ out)
+{
+	const int k = threadIdx.x;
+	if (k >= C_COUNT) return;
+	unsigned long long v = 0;
+	for (int q = 0; q < STAT_SLOTS; q++)
+	{
+		const unsigned long long x = stats[((size_t)k * STAT_SLOTS + q) * STAT_STRIDE];
+		if (k == C_ERRBITS || k == C_RESID) v = x > v ? x : v;
+		else if (k == C_NAN) v |= x;
+		else v += x;
+	}
+	out[k] = v;
+}
+
 // adds v[k] of every thread of the block to counter which[k]; every thread
 // of a BS-sized workgroup calls this
 template <int NC, int BS = BLOCK>
@@ -2022,7 +2039,7 @@ bool alloc_solver(ccf_ctx* ctx)
 		CCF_HIP(ctx, ctx->dmalloc(&ctx->ccs_init, pts * sizeof(double2))) && alloc_groups(ctx) &&
 		CCF_HIP(ctx, hipHostMalloc((void**)&ctx->host_counts, (ctx->max_rows + 6 * CONT_GROUPS) * sizeof(int), hipHostMallocDefault)) &&
 		CCF_HIP(ctx, ctx->dmalloc(&ctx->phase, (size_t)(np + nb) * sizeof(double2))) && launch_phase(ctx) &&
-		CCF_HIP(ctx, ctx->dmalloc(&ctx->counters, STAT_WORDS * sizeof(unsigned long long))) &&
+		CCF_HIP(ctx, ctx->dmalloc(&ctx->counters, (STAT_WORDS + C_COUNT) * sizeof(unsigned long long))) &&
 		CCF_HIP(ctx, hipMemsetAsync(ctx->counters, 0, STAT_WORDS * sizeof(unsigned long long), ctx->stream));
 }
 
@@ -2331,21 +2348,21 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 
 	unsigned long long c[C_COUNT] = {};
 	{
-		// fold the per-slot copies: max for the error bits, or for the NaN
-		// flag, sums otherwise
-		std::vector<unsigned long long> raw(STAT_WORDS);
-		if (!CCF_HIP(ctx, hipStreamSynchronize(es)) ||
-			!CCF_HIP(ctx, hipMemcpyAsync(raw.data(), ctx->counters, STAT_WORDS * sizeof(unsigned long long), hipMemcpyDeviceToHost, s)) ||
+		// the per-slot copies folded on the device (k_fold_stats, after the
+		// error pass) and the C_COUNT totals read back through the pinned
+		// staging buffer
+		if (!ctx->host_stage &&
+			!CCF_HIP(ctx, hipHostMalloc((void**)&ctx->host_stage, sizeof(double) * STAGE_DOUBLES, hipHostMallocDefault)))
+			return false;
+		unsigned long long* const hs = reinterpret_cast<unsigned long long*>(ctx->host_stage);
+		if (!CCF_HIP(ctx, hipEventRecord(ctx->err_event, es)) || !CCF_HIP(ctx, hipStreamWaitEvent(s, ctx->err_event, 0)))
+			return false;
+		k_fold_stats<<<1, 64, 0, s>>>(ctx->counters, ctx->counters + STAT_WORDS);
+		if (!CCF_LAUNCH(ctx, "statistics") ||
+			!CCF_HIP(ctx, hipMemcpyAsync(hs, ctx->counters + STAT_WORDS, C_COUNT * sizeof(unsigned long long), hipMemcpyDeviceToHost, s)) ||
 			!CCF_HIP(ctx, hipStreamSynchronize(s)))
 			return false;
-		for (int k = 0; k < C_COUNT; k++)
-			for (int q = 0; q < STAT_SLOTS; q++)
-			{
-				const unsigned long long v = raw[((size_t)k * STAT_SLOTS + q) * STAT_STRIDE];
-				if (k == C_ERRBITS || k == C_RESID) c[k] = std::max(c[k], v);
-				else if (k == C_NAN) c[k] |= v;
-				else c[k] += v;
-			}
+		for (int k = 0; k < C_COUNT; k++) c[k] = hs[k];
 	}
 	if (c[C_NAN])
 	{

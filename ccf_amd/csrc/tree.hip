@@ -142,10 +142,6 @@ __global__ void k_make_children(int off, int cnt, int new_off, const int* __rest
 	}
 }
 
-// the pinned staging buffer of a tree build (ccf_ctx::host_stage): the
-// binomial table of the largest order, which also holds a level's counts
-constexpr size_t STAGE_DOUBLES = (size_t)(2 * MAX_EXPANSION_ORDER + 1) * (2 * MAX_EXPANSION_ORDER + 2) / 2;
-
 // levels of at most this many nodes are partitioned by 1024-thread workgroups
 constexpr int PARTITION_WIDE_NODES = 64;
 
@@ -924,10 +920,11 @@ bool build_forest(ccf_ctx* ctx, ForestSpec& spec, int verbose)
 		// the tree's capacity)
 		k_scan<<<1, 1024, 0, s>>>(T.nf_cnt, nn, T.rank, T.split);
 		k_leaf_off<<<grid(nn), BLOCK, 0, s>>>(nn, T.child, T.rank, T.nf_off);
-		int tot = 0;
-		if (!CCF_LAUNCH(ctx, "node boxes") || !CCF_HIP(ctx, hipMemcpyAsync(&tot, T.split, sizeof(int), hipMemcpyDeviceToHost, s)) ||
+		int* const hst = reinterpret_cast<int*>(ctx->host_stage); // (pinned, allocated before the levels)
+		if (!CCF_LAUNCH(ctx, "node boxes") || !CCF_HIP(ctx, hipMemcpyAsync(hst, T.split, sizeof(int), hipMemcpyDeviceToHost, s)) ||
 			!CCF_HIP(ctx, hipStreamSynchronize(s)))
 			return false;
+		const int tot = hst[0];
 		if (tot < 0)
 		{
 			ctx->fail("Error. The near-field copies exceed 2^31 stars.\n");
