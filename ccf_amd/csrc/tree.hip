@@ -487,7 +487,7 @@ __global__ __launch_bounds__(BLOCK) void k_upward(int off, int cnt, int p, const
 constexpr int DOWN_LANES = 32;
 constexpr int DOWN_NODES = BLOCK / DOWN_LANES;
 
-// orders dividing an LDS row of a node's u_k (k <= MAX_EXPANSION_ORDER)
+// a node's coefficients, and its u_k (k <= MAX_EXPANSION_ORDER), fit its lanes
 static_assert(MAX_EXPANSION_ORDER < DOWN_LANES, "a node's coefficients must fit its lanes");
 static_assert(MAX_SAME_LEVEL <= DOWN_LANES, "a node's same-level list must fit its lanes");
 
