@@ -2346,6 +2346,12 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 	ctx->info.t_ccs = std::chrono::duration<double>(t1 - t0).count();
 	if (verbose >= 1) std::cout << "\nDone finding critical curve positions. Elapsed time: " << ctx->info.t_ccs << " seconds.\n";
 
+	// transpose to file order (ccf.cuh:1072-1080), beside the error pass's
+	// last launches (a failed error check below still writes no file)
+	if (!ctx->ccs && !CCF_HIP(ctx, ctx->dmalloc(&ctx->ccs, (size_t)total * sizeof(double2)))) return false;
+	k_transpose<<<dim3(grid(R, 32), grid(np + nb, 32)), BLOCK, 0, s>>>(ctx->ccs_init, np + nb, R, ctx->ccs);
+	if (!CCF_LAUNCH(ctx, "transpose_array_kernel")) return false;
+
 	unsigned long long c[C_COUNT] = {};
 	{
 		// the per-slot copies folded on the device (k_fold_stats, after the
@@ -2423,10 +2429,7 @@ bool find_ccs_dev(ccf_ctx* ctx, int verbose)
 			ef * (36.0 * (double)c[C_NEAR_EARLY] + early * 16.0 * p) + early * sm + (fmm_pole ? full * 8.0 * p : 0.0);
 	}
 
-	// transpose to file order (ccf.cuh:1072-1080)
-	if (!ctx->ccs && !CCF_HIP(ctx, ctx->dmalloc(&ctx->ccs, (size_t)total * sizeof(double2)))) return false;
-	k_transpose<<<dim3(grid(R, 32), grid(np + nb, 32)), BLOCK, 0, s>>>(ctx->ccs_init, np + nb, R, ctx->ccs);
-	if (!CCF_LAUNCH(ctx, "transpose_array_kernel")) return false;
+	// (the transpose to file order was queued after the continuation)
 	if (!CCF_HIP(ctx, hipStreamSynchronize(s))) return false;
 	if (ctx->P.timing)
 	{
